@@ -1,0 +1,234 @@
+#!/usr/bin/env python3
+"""Headline benchmark: seconds per variable-coefficient Poisson solve to
+1e-8 at 256^3 on one MI355X, mixed precision vs the FP64 drop-in.
+
+Workload (one "step" = one solve): 256^3 = 16.8M-cell uniform hex box in
+blockMesh order, fvm::laplacian(gamma, p) with log-normal gamma (sigma 1,
+correlation length 8 cells, seeded), x-min wall fixedValue 0, other
+patches zeroGradient, right-hand side U[-1,1] minus its mean (seeded),
+PCG + DIC, tolerance 1e-8 (normFactor-normalised L1 residual), relTol 0,
+psi0 = 0.  The timed solve is the mixed path (FP32 inner PCG+DIC, FP64
+residual, dots, normFactor and refinement); the FP64 drop-in solve of the
+same system is timed beside it.  Inputs (matrix, source) are resident in
+HBM when the timed region starts; the matrix is assembled on the device
+by the fvm::laplacian gather kernel before it.
+
+Prints one JSON line.  Only one GPU is used: --gpus above 1 exits with a
+message.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "openfoam-dev_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--n", type=int, default=256)
+    p.add_argument("--sigma", type=float, default=1.0)
+    p.add_argument("--corr", type=float, default=8.0)
+    p.add_argument("--tol", type=float, default=1e-8)
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    p.add_argument("--cpu-seconds", type=float, default=15.0,
+                   help="approximate CPU time budget of the baseline sample")
+    return p.parse_args()
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {a.gpus}); the multi-GPU "
+              "domain decomposition is a design, not part of this build.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import foamhip as fh
+    from foamhip.cases import BoxCase
+
+    torch.cuda.set_device(0)
+    fh._check(fh.lib().fh_set_device(0))
+    n = a.n
+    t0 = time.time()
+    case = BoxCase(n)
+    gamma = case.lognormal(11, a.sigma, a.corr)
+    rhs = case.rhs(1)
+    log(f"case {n}^3 generated in {time.time() - t0:.1f}s")
+
+    t0 = time.time()
+    mesh = fh.lduAddressing(case.N, case.lower, case.upper)
+    bnd = fh.Boundary(mesh, case.bFaceCells)
+    mat = fh.lduMatrix(mesh)
+    log(f"mesh setup {time.time() - t0:.1f}s: {mesh.info()}")
+
+    def dev(x):
+        return torch.from_numpy(np.ascontiguousarray(x)).to("cuda")
+
+    D = {k: dev(getattr(case, k)) for k in
+         ["weights", "magSf", "deltaCoeffs", "bMagSf", "bDeltaCoeffs", "bValue"]}
+    D["bType"] = dev(case.bType.astype(np.int32))
+    D["gamma"] = dev(gamma)
+    src = dev(rhs)
+    ic = torch.empty(case.nB, dtype=torch.float64, device="cuda")
+    bc = torch.empty_like(ic)
+    P = lambda t: ctypes.c_void_p(t.data_ptr())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fh._check(fh.lib().fh_assemble_laplacian(
+        mat.handle, bnd.handle, P(D["gamma"]), P(D["weights"]), P(D["magSf"]),
+        P(D["deltaCoeffs"]), P(D["bMagSf"]), P(D["bDeltaCoeffs"]), P(D["bType"]),
+        P(D["bValue"]), P(ic), P(bc), P(src), 1))
+    torch.cuda.synchronize()
+    assembly_ms = 1e3 * (time.perf_counter() - t0)
+
+    base = {"solver": "PCG", "preconditioner": "DIC", "tolerance": a.tol, "relTol": 0,
+            "maxIter": 20000}
+    mixed = fh.solver_New("p", mat, dict(base, precision="mixed"))
+    fp64 = fh.solver_New("p", mat, dict(base, precision="double"))
+    psi = torch.zeros(case.N, dtype=torch.float64, device="cuda")
+
+    def solve(s):
+        psi.zero_()
+        return s.solve_device(psi.data_ptr(), src.data_ptr())
+
+    for _ in range(a.warmup):
+        solve(mixed)
+    torch.cuda.synchronize()
+    perfs = []
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        perfs.append(solve(mixed))
+    torch.cuda.synchronize()
+    t_mixed = (time.perf_counter() - t0) / a.steps
+    x_mixed = psi.cpu().numpy()
+    pm = perfs[-1]
+    log(f"mixed: {pm.line()} refinements {pm.nRefinements}, {t_mixed:.3f} s/solve")
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    p64 = solve(fp64)
+    torch.cuda.synchronize()
+    t_fp64 = time.perf_counter() - t0
+    x_fp64 = psi.cpu().numpy()
+    err = float(np.linalg.norm(x_mixed - x_fp64) / np.linalg.norm(x_fp64))
+    log(f"fp64: {p64.line()}, {t_fp64:.3f} s/solve; mixed vs fp64 rel L2 {err:.3e}")
+
+    # roofline of the dominant kernels, timed live with HIP events on the
+    # matrix's stream (FP32: the precision of the timed inner iterations)
+    kernels = {}
+    for name, kid in [("spmv", 0), ("sweep_forward", 1), ("sweep_backward", 2),
+                      ("axpy_norm", 3)]:
+        ms, by = ctypes.c_double(), ctypes.c_double()
+        fh._check(fh.lib().fh_profile_kernel(mat.handle, kid, 1, 2, 20, ctypes.byref(ms),
+                                             ctypes.byref(by)))
+        kernels[name] = {"ms": ms.value, "bytes": by.value,
+                         "GBs": by.value / (ms.value * 1e-3) / 1e9}
+        ms64, by64 = ctypes.c_double(), ctypes.c_double()
+        fh._check(fh.lib().fh_profile_kernel(mat.handle, kid, 0, 2, 10, ctypes.byref(ms64),
+                                             ctypes.byref(by64)))
+        kernels[name + "_fp64"] = {"ms": ms64.value, "bytes": by64.value,
+                                   "GBs": by64.value / (ms64.value * 1e-3) / 1e9}
+        log(f"kernel {name}: fp32 {ms.value:.3f} ms {kernels[name]['GBs']:.0f} GB/s | "
+            f"fp64 {ms64.value:.3f} ms {kernels[name + '_fp64']['GBs']:.0f} GB/s")
+    # dominant = largest time share per inner iteration (2 sweeps + SpMV + update)
+    share = {"sweep_forward": kernels["sweep_forward"]["ms"],
+             "sweep_backward": kernels["sweep_backward"]["ms"],
+             "spmv": kernels["spmv"]["ms"], "axpy_norm": kernels["axpy_norm"]["ms"]}
+    dom = max(share, key=share.get)
+    kd = kernels[dom]
+
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline(case, gamma, rhs, a, p64.nIterations)
+
+    out = {
+        "metric": "s per Poisson solve to 1e-8 at 256^3 (mixed vs FP64), HBM GB/s; serve p99",
+        "value": t_mixed,
+        "unit": "s/solve",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * t_mixed,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 inner / fp64 outer",
+        "data": "synthetic (seeded log-normal diffusivity and right-hand side)",
+        "config": {
+            "workload": f"{n}^3 Poisson, fvm::laplacian with log-normal gamma "
+                        f"(sigma {a.sigma}, corr {a.corr} cells), PCG+DIC to {a.tol}, mixed "
+                        "precision vs FP64",
+            "cells": case.N, "faces": case.F,
+            "fp64_s_per_solve": t_fp64,
+            "speedup_mixed_vs_fp64": t_fp64 / t_mixed,
+            "mixed_iterations": pm.nIterations, "mixed_refinements": pm.nRefinements,
+            "fp64_iterations": p64.nIterations,
+            "mixed_final_residual": pm.finalResidual, "fp64_final_residual": p64.finalResidual,
+            "mixed_vs_fp64_rel_l2": err,
+            "assembly_ms": assembly_ms,
+            "kernels": kernels,
+        },
+        "roofline": {
+            "kernel": dom + " (fp32)",
+            "bound": "hbm",
+            "achieved": kd["GBs"],
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": kd["GBs"] / HBM_PEAK_GBS,
+            "traffic": None,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(case, gamma, rhs, a, fp64_iterations):
+    """The oracle's FP64 PCG+DIC (one core) on the same system, for a bounded
+    number of iterations, extrapolated to the FP64 iteration count."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    addr = O.Addr(case.N, case.lower, case.upper)
+    upper, diag, ic, bc = O.laplacian_assemble(addr, gamma, case.weights, case.magSf,
+                                               case.deltaCoeffs, case.bFaceCells, case.bMagSf,
+                                               case.bDeltaCoeffs, case.bType, case.bValue)
+    diag, b = O.add_boundary(case.bFaceCells, ic, bc, diag, rhs)
+    m = O.Matrix(addr, diag, upper)
+    ctl = {"solver": "PCG", "preconditioner": "DIC", "tolerance": a.tol, "relTol": 0}
+    # calibrate: 2 iterations, then as many as fit the budget
+    t0 = time.perf_counter()
+    m.solve(dict(ctl, maxIter=2), np.zeros(case.N), b)
+    t2 = time.perf_counter() - t0
+    k = int(max(3, min(200, a.cpu_seconds / max(t2 / 2, 1e-6))))
+    t0 = time.perf_counter()
+    _, perf = m.solve(dict(ctl, maxIter=k), np.zeros(case.N), b)
+    tk = time.perf_counter() - t0
+    # setup (normFactor, DIC factor) is in both timings; per-iteration cost
+    per_it = (tk - t2) / max(perf["nIterations"] - 2, 1)
+    setup = t2 - 2 * per_it
+    est = setup + per_it * fp64_iterations
+    log(f"cpu baseline: {perf['nIterations']} iterations in {tk:.1f}s, "
+        f"{per_it*1e3:.1f} ms/iteration -> {est:.1f} s/solve at {fp64_iterations} iterations")
+    return {"value": est, "unit": "s/solve", "cores": 1, "kind": "port",
+            "sample": f"oracle FP64 PCG+DIC on the same {case.nx}^3 system, "
+                      f"{perf['nIterations']} iterations timed ({tk:.1f}s), per-iteration cost "
+                      f"extrapolated to the FP64 GPU iteration count {fp64_iterations}"}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,274 @@
+/*
+ * foamhip.h -- C-ABI of the MI355X-native lduMatrix linear-solver library.
+ *
+ * The entry points replace, for one GPU, the run-time-selectable solver
+ * stack that fvMatrix<scalar>::solveSegregated reaches through
+ * lduMatrix::solver::New (src/OpenFOAM/matrices/lduMatrix/lduMatrix/
+ * lduMatrixSolver.C:40-130).  A thin OpenFOAM adapter (hipPCG, hipDIC, ...)
+ * maps its dictionary to fh_controls and its lduMatrix/lduAddressing arrays
+ * to the plain pointers below.
+ *
+ * Conventions
+ *   - every function returns FH_OK (0) or a non-zero FH_ERR_* status and never
+ *     throws across the ABI; fh_last_error() returns the message of the last
+ *     failure on the calling thread;
+ *   - "host" arrays are caller-owned host memory; "device" arrays are
+ *     caller-owned device memory on the handle's device;
+ *   - label = int32_t, scalar = double (WM_LABEL_SIZE=32, WM_DP);
+ *   - a handle is used by one host thread at a time; each fh_matrix owns one
+ *     HIP stream, so independent matrices run concurrently.
+ *   - coupled interfaces (processor/cyclic patches) are not part of this
+ *     ABI: the matrices handed over are single-domain.
+ */
+#ifndef FOAMHIP_H
+#define FOAMHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FH_OK               0
+#define FH_ERR_INVALID      1   /* bad argument / unknown name            */
+#define FH_ERR_HIP          2   /* HIP runtime failure                     */
+#define FH_ERR_NOMEM        3   /* device allocation failed               */
+#define FH_ERR_UNSUPPORTED  4   /* valid request this build does not cover */
+#define FH_ERR_NUMERIC      5   /* NaN/Inf in a residual or a setup factor */
+#define FH_ERR_SCHEDULE     6   /* a device-side bounded wait expired      */
+
+/* Selection ids.  Names accepted by the *_lookup functions are the
+   reference run-time-selection names (PCG.C:34-35, PBiCGStab.C:34-38,
+   DICPreconditioner.C:34-36, ...) with or without the "hip" prefix. */
+enum
+{
+    FH_SOLVER_PCG = 0,
+    FH_SOLVER_PBICGSTAB = 1,
+    FH_SOLVER_PBICG = 2,
+    FH_SOLVER_SMOOTH = 3,
+    FH_SOLVER_GAMG = 4,
+    FH_SOLVER_DIAGONAL = 5
+};
+
+enum
+{
+    FH_PRECON_NONE = 0,
+    FH_PRECON_DIAGONAL = 1,
+    FH_PRECON_DIC = 2,
+    FH_PRECON_FDIC = 3,
+    FH_PRECON_DILU = 4,
+    FH_PRECON_GAMG = 5
+};
+
+enum
+{
+    FH_SMOOTH_GS = 0,
+    FH_SMOOTH_SYMGS = 1,
+    FH_SMOOTH_DIC = 2,
+    FH_SMOOTH_DILU = 3,
+    FH_SMOOTH_DICGS = 4,
+    FH_SMOOTH_DILUGS = 5,
+    FH_SMOOTH_FDIC = 6,
+    FH_SMOOTH_NONE = 7
+};
+
+enum
+{
+    FH_PRECISION_DOUBLE = 0,  /* plain FP64 drop-in                        */
+    FH_PRECISION_MIXED = 1    /* FP32 inner iterations, FP64 outer residual,
+                                 dots, normFactor and refinement updates    */
+};
+
+/* Solver controls: the fvSolution entries read by
+   lduMatrix::solver::readControls (lduMatrixSolver.C:158-164),
+   smoothSolver::readControls (smoothSolver.C:70-74),
+   GAMGSolver::readControls (GAMGSolver.C:351-388) and the agglomeration
+   (GAMGAgglomeration.C:248-257, pairGAMGAgglomeration.C:44-52). */
+typedef struct fh_controls
+{
+    int32_t solver;
+    int32_t preconditioner;
+    int32_t smoother;
+    int32_t precision;
+    int32_t maxIter;                 /* 1000 */
+    int32_t minIter;                 /* 0    */
+    double  tolerance;               /* 1e-6 */
+    double  relTol;                  /* 0    */
+    int32_t nSweeps;                 /* smoothSolver: 1 */
+    int32_t nPreSweeps;              /* GAMG: 0 */
+    int32_t preSweepsLevelMultiplier;/* 1 */
+    int32_t maxPreSweeps;            /* 4 */
+    int32_t nPostSweeps;             /* 2 */
+    int32_t postSweepsLevelMultiplier;/* 1 */
+    int32_t maxPostSweeps;           /* 4 */
+    int32_t nFinestSweeps;           /* 2 */
+    int32_t scaleCorrection;         /* -1: matrix.symmetric() */
+    int32_t directSolveCoarsest;     /* 0 */
+    int32_t nCellsInCoarsestLevel;   /* 10 */
+    int32_t mergeLevels;             /* 1 */
+    int32_t nVcycles;                /* GAMG preconditioner: 2 */
+    /* mixed precision (no reference counterpart) */
+    double  innerRelTol;             /* inner FP32 solve: residual reduction
+                                        per refinement step, default 1e-4  */
+    int32_t maxRefinements;          /* default 50 */
+} fh_controls;
+
+/* SolverPerformance<scalar> (SolverPerformance.H:84-92) plus extras. */
+typedef struct fh_perf
+{
+    char    solverName[64];          /* e.g. "DICPCG" (PCG.C:73-77 naming) */
+    double  initialResidual;
+    double  finalResidual;
+    int32_t nIterations;             /* inner iterations in mixed mode      */
+    int32_t converged;
+    int32_t singular;
+    int32_t nRefinements;            /* mixed mode outer steps (0 in FP64)  */
+    double  normFactor;
+    double  solveMilliseconds;       /* device time of the solve           */
+} fh_perf;
+
+typedef struct fh_mesh fh_mesh;
+typedef struct fh_matrix fh_matrix;
+
+typedef struct fh_mesh_info
+{
+    int32_t nCells;
+    int32_t nFaces;
+    int32_t nLevels;                 /* depth of the lower-triangular DAG  */
+    int32_t nParts;                  /* pipelined sweep partitions (0=none)*/
+    int32_t boxNx, boxNy, boxNz;     /* detected hex-box shape, 0 if not   */
+    int64_t deviceBytes;             /* addressing bytes resident in HBM   */
+} fh_mesh_info;
+
+/* ---- library ---------------------------------------------------------- */
+int         fh_version(void);
+const char* fh_last_error(void);
+int         fh_set_device(int32_t device);
+void        fh_controls_default(fh_controls* c);
+int         fh_solver_lookup(const char* name, int32_t* id);
+int         fh_preconditioner_lookup(const char* name, int32_t* id);
+int         fh_smoother_lookup(const char* name, int32_t* id);
+/* pairGAMGAgglomeration::forward_ (pairGAMGAgglomeration.C:33) is a
+   process-global toggle in the reference; the library keeps the same
+   state.  These read and reset it (reset = true). */
+int         fh_gamg_direction(void);
+void        fh_gamg_reset_direction(void);
+
+/* ---- mesh: lduAddressing (lduAddressing.H:112-207) --------------------
+   lowerAddr = owner, upperAddr = neighbour of each internal face, faces in
+   upper-triangular order.  Builds the device row structure and the sweep
+   schedule once per mesh. */
+int fh_mesh_create(int32_t nCells, int32_t nFaces, const int32_t* lowerAddr,
+                   const int32_t* upperAddr, fh_mesh** mesh);
+int fh_mesh_info_get(const fh_mesh* mesh, fh_mesh_info* info);
+int fh_mesh_destroy(fh_mesh* mesh);
+
+/* ---- matrix: lduMatrix coefficients (lduMatrix.H:79-87) ----------------
+   upper == NULL: diagonal matrix; lower == NULL: symmetric (lower aliases
+   upper, lduMatrix.C:168-312).  Host arrays in the caller's face order. */
+int fh_matrix_create(fh_mesh* mesh, fh_matrix** matrix);
+int fh_matrix_set(fh_matrix* matrix, const double* diag, const double* upper,
+                  const double* lower);
+int fh_matrix_set_device(fh_matrix* matrix, const double* d_diag,
+                         const double* d_upper, const double* d_lower);
+int fh_matrix_destroy(fh_matrix* matrix);
+
+/* ---- solve: lduMatrix::solver::solve (lduMatrix.H:250-255) -------------
+   psi holds the initial guess on entry and the solution on return. */
+int fh_solve(fh_matrix* matrix, const fh_controls* controls, double* psi,
+             const double* source, fh_perf* perf);
+int fh_solve_device(fh_matrix* matrix, const fh_controls* controls,
+                    double* d_psi, const double* d_source, fh_perf* perf);
+
+/* ---- single operations (host arrays), the lduMatrix members -------------*/
+/* lduMatrixATmul.C:34-92 */
+int fh_amul(fh_matrix* m, const double* psi, double* Apsi);
+/* lduMatrixATmul.C:95-151 */
+int fh_tmul(fh_matrix* m, const double* psi, double* Tpsi);
+/* lduMatrixATmul.C:203-280 */
+int fh_residual(fh_matrix* m, const double* psi, const double* source,
+                double* rA);
+/* lduMatrixATmul.C:154-200 */
+int fh_sum_a(fh_matrix* m, double* sumA);
+/* lduMatrixSolver.C:174-197 */
+int fh_norm_factor(fh_matrix* m, const double* psi, const double* source,
+                   double* normFactor);
+/* DICPreconditioner.C:57-84 / DILUPreconditioner.C:57-85 /
+   diagonalPreconditioner.C:50-63: the reciprocal factor rD */
+int fh_reciprocal_d(fh_matrix* m, int32_t preconditioner, double* rD);
+/* lduMatrix::preconditioner::precondition / preconditionT
+   (lduMatrix.H:468-487) */
+int fh_precondition(fh_matrix* m, int32_t preconditioner, const double* rA,
+                    double* wA);
+int fh_precondition_t(fh_matrix* m, int32_t preconditioner, const double* rA,
+                      double* wA);
+/* lduMatrix::smoother::smooth (lduMatrix.H:399-405) */
+int fh_smooth(fh_matrix* m, int32_t smoother, double* psi,
+              const double* source, int32_t nSweeps);
+
+/* ---- measurement (no reference counterpart) -----------------------------
+   Times `reps` back-to-back launches of one kernel of the solve on the
+   matrix's stream with HIP events.  kernel: FH_KERNEL_*; precision:
+   FH_PRECISION_DOUBLE or FH_PRECISION_MIXED (FP32 values).  Returns the
+   average milliseconds per launch and the algorithmic HBM bytes one launch
+   moves (each array read or written once, indices int32). */
+enum
+{
+    FH_KERNEL_SPMV = 0,          /* y = A x                         */
+    FH_KERNEL_SWEEP_FORWARD = 1, /* DIC/DILU forward substitution    */
+    FH_KERNEL_SWEEP_BACKWARD = 2,/* DIC/DILU backward substitution   */
+    FH_KERNEL_AXPY_NORM = 3      /* psi += a p, r -= a w, sum |r|     */
+};
+int fh_profile_kernel(fh_matrix* matrix, int32_t kernel, int32_t precision,
+                      int32_t preconditioner, int32_t reps, double* msPerLaunch,
+                      double* bytesPerLaunch);
+
+/* ---- finite-volume assembly on the device ------------------------------
+   Boundary faces of all non-coupled patches, flattened in patch order
+   (fvBoundaryMesh order); bFaceCells = fvPatch::faceCells.  Device arrays
+   are in the caller's (natural) cell and face order. */
+typedef struct fh_boundary fh_boundary;
+int fh_boundary_create(fh_mesh* mesh, int32_t nBFaces, const int32_t* bFaceCells,
+                       fh_boundary** boundary);
+int fh_boundary_destroy(fh_boundary* boundary);
+
+/* fvm::laplacian(gamma, psi), Gauss linear uncorrected
+   (gaussLaplacianScheme.C:45-89, gaussLaplacianSchemes.C:43-56): writes the
+   matrix coefficients (symmetric), the patch internal/boundary coefficients
+   (bType 1 fixedValue with bValue, 0 zeroGradient).  addBoundaryDiag != 0
+   includes the internal coefficients in the diagonal and d_source != NULL
+   receives the boundary coefficients, as fvMatrix::solveSegregated does
+   before the solve (fvScalarMatrix.C:155-159). */
+int fh_assemble_laplacian(fh_matrix* matrix, const fh_boundary* boundary,
+                          const double* d_gamma, const double* d_weights,
+                          const double* d_magSf, const double* d_deltaCoeffs,
+                          const double* d_bMagSf, const double* d_bDeltaCoeffs,
+                          const int32_t* d_bType, const double* d_bValue,
+                          double* d_internalCoeffs, double* d_boundaryCoeffs,
+                          double* d_source, int32_t addBoundaryDiag);
+
+/* fvm::div(phi, psi) with upwind weights minus fvm::laplacian(D, psi)
+   (gaussConvectionScheme.C:119-170, upwind.H:116-128) */
+int fh_assemble_convection_diffusion(fh_matrix* matrix, const fh_boundary* boundary,
+                                     const double* d_phi, double D,
+                                     const double* d_magSf, const double* d_deltaCoeffs,
+                                     const double* d_bPhi, const double* d_bMagSf,
+                                     const double* d_bDeltaCoeffs, const int32_t* d_bType,
+                                     const double* d_bValue, double* d_internalCoeffs,
+                                     double* d_boundaryCoeffs, double* d_source,
+                                     int32_t addBoundaryDiag);
+
+/* fvc::grad(phi), Gauss linear (gaussGrad.C:30-75) with the patch-normal
+   boundary correction (gradScheme.C:123-148).  d_grad [nCells*3],
+   d_bGrad [nBFaces*3] (may be NULL). */
+int fh_grad_gauss_linear(fh_mesh* mesh, const fh_boundary* boundary, const double* d_phi,
+                         const double* d_weights, const double* d_Sf, const double* d_V,
+                         const double* d_bSf, const double* d_bMagSf, const double* d_bPhi,
+                         const double* d_bSnGrad, double* d_grad, double* d_bGrad);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

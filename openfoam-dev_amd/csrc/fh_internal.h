@@ -1,0 +1,142 @@
+// Internal declarations of the foamhip library (not part of the C-ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "foamhip.h"
+
+namespace fh
+{
+
+// ---------------------------------------------------------------------------
+// errors: internal code throws fh::Error, the C entry points catch it,
+// record the message for fh_last_error() and return the status.
+// ---------------------------------------------------------------------------
+struct Error
+{
+    int code;
+    std::string msg;
+};
+
+[[noreturn]] void fail(int code, const char* fmt, ...);
+void set_last_error(const std::string& msg);
+
+#define FH_HIP(call)                                                         \
+    do                                                                       \
+    {                                                                        \
+        hipError_t fh_e_ = (call);                                           \
+        if (fh_e_ != hipSuccess)                                             \
+        {                                                                    \
+            ::fh::fail(fh_e_ == hipErrorOutOfMemory ? FH_ERR_NOMEM           \
+                                                     : FH_ERR_HIP,           \
+                       "%s failed: %s (%s:%d)", #call,                       \
+                       hipGetErrorString(fh_e_), __FILE__, __LINE__);        \
+        }                                                                    \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// owning device buffer
+// ---------------------------------------------------------------------------
+template<class T>
+struct DBuf
+{
+    T* p = nullptr;
+    size_t n = 0;
+
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
+
+    void alloc(size_t count)
+    {
+        if (count == n && p) return;
+        release();
+        if (count == 0) return;
+        FH_HIP(hipMalloc(&p, count*sizeof(T)));
+        n = count;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void upload(const T* h, size_t count, hipStream_t s)
+    {
+        alloc(count);
+        if (count)
+            FH_HIP(hipMemcpyAsync(p, h, count*sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void upload(const std::vector<T>& v, hipStream_t s) { upload(v.data(), v.size(), s); }
+    size_t bytes() const { return n*sizeof(T); }
+};
+
+// ---------------------------------------------------------------------------
+// sweep schedule: rows renumbered by (part, level, original index); each
+// part is walked level by level by one workgroup, waiting on the progress
+// counters of the parts it depends on.
+// ---------------------------------------------------------------------------
+struct Schedule
+{
+    int32_t nParts = 1;
+    int32_t nLevels = 0;
+    // per part: [partLevelOff[p], partLevelOff[p+1]) indexes segStart; the
+    // part's first level is partMinLevel[p]; rows of level (minL + k) are
+    // [segStart[off+k], segStart[off+k+1])
+    std::vector<int32_t> partLevelOff;
+    std::vector<int32_t> partMinLevel;
+    std::vector<int32_t> segStart;
+    // parts whose rows are lower (up) / upper (down) neighbours of part p
+    std::vector<int32_t> upOff, upList, downOff, downList;
+    // ticket order: forward sweeps take parts in order[], backward in reverse
+    std::vector<int32_t> order;
+
+    DBuf<int32_t> d_partLevelOff, d_partMinLevel, d_segStart;
+    DBuf<int32_t> d_upOff, d_upList, d_downOff, d_downList, d_order;
+    int32_t maxSegRows = 0;
+};
+
+struct fh_mesh_impl;
+
+} // namespace fh
+
+// Public opaque types
+struct fh_mesh
+{
+    int device = 0;
+    int32_t nCells = 0;
+    int32_t nFaces = 0;
+    std::vector<int32_t> lower, upper;      // caller's face order
+    std::vector<int32_t> perm, iperm;       // perm[new] = old cell
+    std::vector<int32_t> levelOf;           // per old cell
+
+    // SELL-64 row structure in the new numbering
+    int32_t nSlices = 0;
+    int64_t nnzPad = 0;
+    std::vector<int32_t> sliceStart;        // [nSlices+1]
+    std::vector<int32_t> rowInfo;           // nLower | nEnt << 16
+    std::vector<int32_t> col;               // SELL layout, new numbering
+    std::vector<int32_t> face;              // (f << 1 | isUpper) or -1 pad
+
+    fh::DBuf<int32_t> d_sliceStart, d_rowInfo, d_col, d_face;
+    fh::DBuf<int32_t> d_perm;
+    fh::DBuf<int32_t> d_lower, d_upper;     // face-ordered addressing
+
+    fh::Schedule sched;
+    int32_t boxNx = 0, boxNy = 0, boxNz = 0;
+    hipStream_t setupStream = nullptr;
+};
+
+namespace fh
+{
+// mesh construction (fh_mesh.cpp)
+void build_mesh(fh_mesh* m, int32_t nCells, int32_t nFaces,
+                const int32_t* lower, const int32_t* upper);
+size_t mesh_device_bytes(const fh_mesh* m);
+} // namespace fh

@@ -1,0 +1,130 @@
+// Launch wrappers of the HIP kernels (fh_kernels.hip).
+#pragma once
+
+#include "fh_internal.h"
+
+namespace fh
+{
+
+// Device-resident scalar state of one iterative solve.  The reduction
+// finalize kernels update it, every iteration kernel reads `done` first,
+// so the host enqueues iterations in batches without a sync per iteration.
+struct DevState
+{
+    double normFactor;
+    double initialResidual;
+    double finalResidual;
+    double tolerance;
+    double relTol;
+    // PCG
+    double wArA, wArAold, wApA, alpha, beta;
+    // PBiCGStab
+    double rA0rA, rA0rAold, rA0AyA, omega, tAtA, tAsA, bicgAlpha, bicgBeta;
+    // scratch sums
+    double sum0, sum1;
+    int32_t maxIter, minIter;
+    int32_t nIterations;
+    int32_t converged;
+    int32_t singular;
+    int32_t done;
+    int32_t numericError;
+    int32_t scheduleError;
+    int32_t earlySeq;       // PBiCGStab early exit: launch sequence number
+};
+
+// Row structure in the new numbering (shared by every kernel)
+struct RowsView
+{
+    int32_t n;
+    const int32_t* rowInfo;     // nLower | nEnt << 16
+    const int32_t* sliceStart;  // SELL-64 slice offsets
+    const int32_t* col;
+    const int32_t* face;        // (f << 1 | isUpper), -1 pad
+};
+
+// Partitioned level sweep schedule (one workgroup per part)
+struct SchedView
+{
+    int32_t nParts;
+    int32_t nLevels;
+    const int32_t* partLevelOff;
+    const int32_t* partMinLevel;
+    const int32_t* segStart;
+    const int32_t* upOff;
+    const int32_t* upList;
+    const int32_t* downOff;
+    const int32_t* downList;
+    const int32_t* order;
+    int32_t maxDeps;
+};
+
+// Per-matrix synchronisation context of the pipelined sweeps
+struct SweepSync
+{
+    unsigned long long* ticket;    // [1]
+    unsigned long long* progress;  // [nParts]
+    unsigned long long epoch;      // launches made with this context so far
+};
+
+enum SweepKind
+{
+    SWEEP_PRECOND_FWD = 0,  // w = rD*x, w -= rD*lower*w(l)      (DIC/DILU)
+    SWEEP_PRECOND_BWD = 1,  // w -= rD*upper*w(u), faces reversed
+    SWEEP_RD_DIC = 2,       // rD -= upper*upper/rD(l)
+    SWEEP_RD_DILU = 3,      // rD -= upper*lower/rD(l)
+    SWEEP_GS_FWD = 4,       // Gauss-Seidel forward row update
+    SWEEP_GS_BWD = 5        // symmetric Gauss-Seidel backward row update
+};
+
+template<class T>
+struct SweepData
+{
+    const T* val;        // SELL coefficients
+    const T* rD;         // reciprocal factor (precond) / raw factor (RD)
+    const T* x;          // input: residual (precond) or source (GS)
+    T* w;                // output: w (precond), rD raw (RD), psi (GS)
+    const T* diag;       // GS / RD
+    const double* upperF;// RD_DILU: face-ordered upper
+};
+
+// ---- kernels: sizes in rows --------------------------------------------------
+template<class T>
+void launch_gather_values(const RowsView& R, int64_t nnzPad, const double* upperF,
+                          const double* lowerF, T* val, hipStream_t s);
+template<class Tin, class Tout>
+void launch_permute_in(int32_t n, const int32_t* perm, const Tin* xNat, Tout* xNew,
+                       hipStream_t s);
+template<class Tin, class Tout>
+void launch_permute_out(int32_t n, const int32_t* perm, const Tin* xNew, Tout* xNat,
+                        hipStream_t s);
+template<class Tin, class Tout>
+void launch_convert(int64_t n, const Tin* a, Tout* b, hipStream_t s);
+
+// y = A x (T = float or double)
+template<class T>
+void launch_amul(const RowsView& R, const T* diag, const T* val, const T* x, T* y,
+                 const DevState* st, hipStream_t s);
+// y = A^T x
+template<class T>
+void launch_tmul(const RowsView& R, const T* diag, const T* valT, const T* x, T* y,
+                 hipStream_t s);
+// r = b - A x
+template<class T>
+void launch_residual(const RowsView& R, const T* diag, const T* val, const T* x,
+                     const T* b, T* r, const DevState* st, hipStream_t s);
+void launch_sum_a(const RowsView& R, const double* diag, const double* val,
+                  double* sumA, hipStream_t s);
+
+// sweeps
+template<class T>
+void launch_sweep(int kind, const RowsView& R, const SchedView& S, SweepSync& sync,
+                  const SweepData<T>& d, DevState* st, hipStream_t s);
+template<class T>
+void launch_reciprocal(int32_t n, T* a, hipStream_t s);
+
+// reductions (deterministic two-pass).  partials must hold
+// reduction_partials(n) doubles per sum.
+int32_t reduction_blocks(int32_t n);
+size_t reduction_partials(int32_t n);
+
+} // namespace fh

@@ -1,0 +1,470 @@
+// HIP kernels of the foamhip lduMatrix solver library, written for gfx950
+// (CDNA4, 64-wide wavefronts).  Every kernel that reproduces a reference
+// loop keeps that loop's per-row accumulation order; the library is built
+// with -ffp-contract=off so no multiply-add is contracted into an FMA.
+//
+// Rows are in the library numbering (see fh_mesh.cpp); the off-diagonal
+// entries of row r are stored SELL-64: slice r/64, entry k of row r at
+// sliceStart[r/64] + 64*k + r%64, so one lane per row reads coalesced.
+#include "fh_kernels.h"
+
+#include <cfloat>
+
+namespace fh
+{
+
+#define FH_BLOCK 256
+
+__device__ __forceinline__ int32_t sell_base(const int32_t* sliceStart, int32_t r)
+{
+    return sliceStart[r >> 6] + (r & 63);
+}
+
+// --------------------------------------------------------------------------
+// deterministic block reduction: xor-butterfly within each wave, then the
+// wave sums in wave order.  Result valid in thread 0.
+// --------------------------------------------------------------------------
+template<int NT>
+__device__ __forceinline__ double block_sum(double v, double* smem)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) smem[wid] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+    {
+#pragma unroll
+        for (int i = 0; i < NT/64; i++) s += smem[i];
+    }
+    return s;
+}
+
+int32_t reduction_blocks(int32_t n) { return (n + FH_BLOCK - 1)/FH_BLOCK; }
+size_t reduction_partials(int32_t n) { return (size_t)reduction_blocks(n) + 1; }
+
+// --------------------------------------------------------------------------
+// coefficient gather: SELL values from the face-ordered upper/lower arrays
+// (lower entry of row u from face f is lower[f], upper entry is upper[f],
+// lduMatrixATmul.C:73-79)
+// --------------------------------------------------------------------------
+template<class T>
+__global__ void k_gather_values(int64_t nnzPad, const int32_t* __restrict__ face,
+                                const double* __restrict__ upperF,
+                                const double* __restrict__ lowerF, T* __restrict__ val)
+{
+    const int64_t e = (int64_t)blockIdx.x*FH_BLOCK + threadIdx.x;
+    if (e >= nnzPad) return;
+    const int32_t fc = face[e];
+    double v = 0.0;
+    if (fc >= 0)
+    {
+        const int32_t f = fc >> 1;
+        v = (fc & 1) ? upperF[f] : lowerF[f];
+    }
+    val[e] = (T)v;
+}
+
+template<class T>
+void launch_gather_values(const RowsView& R, int64_t nnzPad, const double* upperF,
+                          const double* lowerF, T* val, hipStream_t s)
+{
+    if (nnzPad == 0) return;
+    const int64_t nb = (nnzPad + FH_BLOCK - 1)/FH_BLOCK;
+    hipLaunchKernelGGL(k_gather_values<T>, dim3((unsigned)nb), dim3(FH_BLOCK), 0, s,
+                       nnzPad, R.face, upperF, lowerF, val);
+}
+
+template<class Tin, class Tout>
+__global__ void k_permute_in(int32_t n, const int32_t* __restrict__ perm,
+                             const Tin* __restrict__ xNat, Tout* __restrict__ xNew)
+{
+    const int32_t r = blockIdx.x*FH_BLOCK + threadIdx.x;
+    if (r < n) xNew[r] = (Tout)xNat[perm[r]];
+}
+
+template<class Tin, class Tout>
+__global__ void k_permute_out(int32_t n, const int32_t* __restrict__ perm,
+                              const Tin* __restrict__ xNew, Tout* __restrict__ xNat)
+{
+    const int32_t r = blockIdx.x*FH_BLOCK + threadIdx.x;
+    if (r < n) xNat[perm[r]] = (Tout)xNew[r];
+}
+
+template<class Tin, class Tout>
+void launch_permute_in(int32_t n, const int32_t* perm, const Tin* xNat, Tout* xNew,
+                       hipStream_t s)
+{
+    hipLaunchKernelGGL((k_permute_in<Tin, Tout>), dim3(reduction_blocks(n)), dim3(FH_BLOCK),
+                       0, s, n, perm, xNat, xNew);
+}
+
+template<class Tin, class Tout>
+void launch_permute_out(int32_t n, const int32_t* perm, const Tin* xNew, Tout* xNat,
+                        hipStream_t s)
+{
+    hipLaunchKernelGGL((k_permute_out<Tin, Tout>), dim3(reduction_blocks(n)), dim3(FH_BLOCK),
+                       0, s, n, perm, xNew, xNat);
+}
+
+template<class Tin, class Tout>
+__global__ void k_convert(int64_t n, const Tin* __restrict__ a, Tout* __restrict__ b)
+{
+    const int64_t i = (int64_t)blockIdx.x*FH_BLOCK + threadIdx.x;
+    if (i < n) b[i] = (Tout)a[i];
+}
+
+template<class Tin, class Tout>
+void launch_convert(int64_t n, const Tin* a, Tout* b, hipStream_t s)
+{
+    if (n == 0) return;
+    hipLaunchKernelGGL((k_convert<Tin, Tout>), dim3((unsigned)((n + FH_BLOCK - 1)/FH_BLOCK)),
+                       dim3(FH_BLOCK), 0, s, n, a, b);
+}
+
+// --------------------------------------------------------------------------
+// Amul / Tmul / residual / sumA
+// --------------------------------------------------------------------------
+
+// lduMatrixATmul.C:34-92: Apsi[c] = diag*psi, then += coefficient*psi(nbr)
+// over the row's faces in face order
+template<class T>
+__global__ void k_amul(RowsView R, const T* __restrict__ diag, const T* __restrict__ val,
+                       const T* __restrict__ x, T* __restrict__ y, const DevState* st)
+{
+    if (st && st->done) return;
+    const int32_t r = blockIdx.x*FH_BLOCK + threadIdx.x;
+    if (r >= R.n) return;
+    const int32_t nEnt = R.rowInfo[r] >> 16;
+    const int32_t base = sell_base(R.sliceStart, r);
+    T acc = diag[r]*x[r];
+    for (int32_t k = 0; k < nEnt; k++)
+    {
+        const int32_t e = base + (k << 6);
+        acc += val[e]*x[R.col[e]];
+    }
+    y[r] = acc;
+}
+
+template<class T>
+void launch_amul(const RowsView& R, const T* diag, const T* val, const T* x, T* y,
+                 const DevState* st, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_amul<T>, dim3(reduction_blocks(R.n)), dim3(FH_BLOCK), 0, s,
+                       R, diag, val, x, y, st);
+}
+
+// lduMatrixATmul.C:95-151: the same walk with upper/lower swapped, i.e. the
+// SELL values of the transposed roles (built by the caller)
+template<class T>
+void launch_tmul(const RowsView& R, const T* diag, const T* valT, const T* x, T* y,
+                 hipStream_t s)
+{
+    launch_amul<T>(R, diag, valT, x, y, nullptr, s);
+}
+
+// lduMatrixATmul.C:203-280: rA = source - diag*psi, then -= coeff*psi(nbr)
+template<class T>
+__global__ void k_residual(RowsView R, const T* __restrict__ diag, const T* __restrict__ val,
+                           const T* __restrict__ x, const T* __restrict__ b,
+                           T* __restrict__ res, const DevState* st)
+{
+    if (st && st->done) return;
+    const int32_t r = blockIdx.x*FH_BLOCK + threadIdx.x;
+    if (r >= R.n) return;
+    const int32_t nEnt = R.rowInfo[r] >> 16;
+    const int32_t base = sell_base(R.sliceStart, r);
+    T acc = b[r] - diag[r]*x[r];
+    for (int32_t k = 0; k < nEnt; k++)
+    {
+        const int32_t e = base + (k << 6);
+        acc -= val[e]*x[R.col[e]];
+    }
+    res[r] = acc;
+}
+
+template<class T>
+void launch_residual(const RowsView& R, const T* diag, const T* val, const T* x,
+                     const T* b, T* r, const DevState* st, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_residual<T>, dim3(reduction_blocks(R.n)), dim3(FH_BLOCK), 0, s,
+                       R, diag, val, x, b, r, st);
+}
+
+// lduMatrixATmul.C:154-200: sumA = diag + sum of the row's coefficients
+__global__ void k_sum_a(RowsView R, const double* __restrict__ diag,
+                        const double* __restrict__ val, double* __restrict__ sumA)
+{
+    const int32_t r = blockIdx.x*FH_BLOCK + threadIdx.x;
+    if (r >= R.n) return;
+    const int32_t nEnt = R.rowInfo[r] >> 16;
+    const int32_t base = sell_base(R.sliceStart, r);
+    double acc = diag[r];
+    for (int32_t k = 0; k < nEnt; k++) acc += val[base + (k << 6)];
+    sumA[r] = acc;
+}
+
+void launch_sum_a(const RowsView& R, const double* diag, const double* val,
+                  double* sumA, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_sum_a, dim3(reduction_blocks(R.n)), dim3(FH_BLOCK), 0, s,
+                       R, diag, val, sumA);
+}
+
+template<class T>
+__global__ void k_reciprocal(int32_t n, T* a)
+{
+    const int32_t i = blockIdx.x*FH_BLOCK + threadIdx.x;
+    if (i < n) a[i] = T(1)/a[i];
+}
+
+template<class T>
+void launch_reciprocal(int32_t n, T* a, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_reciprocal<T>, dim3(reduction_blocks(n)), dim3(FH_BLOCK), 0, s, n, a);
+}
+
+// --------------------------------------------------------------------------
+// Pipelined partitioned level sweep.
+//
+// One workgroup owns one part and walks its level segments in order
+// (forward) or reverse order (backward).  Before a level it waits until
+// every part it depends on has finished the levels its rows can read:
+// forward, all levels <= L-1 of the parts holding lower neighbours;
+// backward, all levels >= L+1 of the parts holding upper neighbours.
+// Parts are handed out through a ticket counter in a topological order, so a
+// waiting part only ever waits for parts that already hold a ticket, and
+// every wait is bounded (on expiry the solve reports FH_ERR_SCHEDULE).
+//
+// Progress values carry the launch epoch: v = epoch*(nLevels+2) + k with
+// k in [0, nLevels+1], so values of earlier launches never satisfy a wait.
+// --------------------------------------------------------------------------
+
+// one poll costs about a microsecond; a legitimate wait lasts at most the
+// sweep itself (milliseconds), so 2^20 polls (about a second) means a
+// broken schedule: the wait is abandoned and the solve reports it
+#define FH_SPIN_LIMIT (1u << 20)
+
+__device__ __forceinline__ unsigned long long ld_progress(const unsigned long long* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void st_progress(unsigned long long* p, unsigned long long v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template<class T, int KIND>
+__device__ __forceinline__ void sweep_row(const RowsView& R, const SweepData<T>& d, int32_t r)
+{
+    const int32_t info = R.rowInfo[r];
+    const int32_t nLo = info & 0xffff;
+    const int32_t nEnt = info >> 16;
+    const int32_t base = sell_base(R.sliceStart, r);
+    if (KIND == SWEEP_PRECOND_FWD)
+    {
+        // DICPreconditioner.C:108-117 / DILUPreconditioner.C:117-128
+        const T rDr = d.rD[r];
+        T w = rDr*d.x[r];
+        for (int32_t k = 0; k < nLo; k++)
+        {
+            const int32_t e = base + (k << 6);
+            w -= rDr*d.val[e]*d.w[R.col[e]];
+        }
+        d.w[r] = w;
+    }
+    else if (KIND == SWEEP_PRECOND_BWD)
+    {
+        // DICPreconditioner.C:119-122: faces in descending order
+        const T rDr = d.rD[r];
+        T w = d.w[r];
+        for (int32_t k = nEnt - 1; k >= nLo; k--)
+        {
+            const int32_t e = base + (k << 6);
+            w -= rDr*d.val[e]*d.w[R.col[e]];
+        }
+        d.w[r] = w;
+    }
+    else if (KIND == SWEEP_RD_DIC)
+    {
+        // DICPreconditioner.C:67-72 (raw factor; reciprocal taken afterwards)
+        T v = d.diag[r];
+        for (int32_t k = 0; k < nLo; k++)
+        {
+            const int32_t e = base + (k << 6);
+            const T u = d.val[e];
+            v -= u*u/d.w[R.col[e]];
+        }
+        d.w[r] = v;
+    }
+    else if (KIND == SWEEP_RD_DILU)
+    {
+        // DILUPreconditioner.C:67-72: upper[f]*lower[f]/rD[l]
+        T v = d.diag[r];
+        for (int32_t k = 0; k < nLo; k++)
+        {
+            const int32_t e = base + (k << 6);
+            const T up = (T)d.upperF[R.face[e] >> 1];
+            v -= up*d.val[e]/d.w[R.col[e]];
+        }
+        d.w[r] = v;
+    }
+    else if (KIND == SWEEP_GS_FWD || KIND == SWEEP_GS_BWD)
+    {
+        // GaussSeidelSmoother.C:150-176 / symGaussSeidelSmoother.C:151-205:
+        // bPrime accumulates the lower products in face order, then the
+        // upper products are subtracted in face order, then / diag
+        T t = d.x[r];
+        for (int32_t k = 0; k < nEnt; k++)
+        {
+            const int32_t e = base + (k << 6);
+            t -= d.val[e]*d.w[R.col[e]];
+        }
+        d.w[r] = t/d.diag[r];
+    }
+}
+
+template<class T, int KIND, bool BACKWARD>
+__global__ __launch_bounds__(FH_BLOCK)
+void k_sweep_parts(RowsView R, SchedView S, SweepSync sync, SweepData<T> d,
+                   DevState* st)
+{
+    __shared__ int32_t s_part;
+    __shared__ int32_t s_stop;
+    if (threadIdx.x == 0)
+    {
+        const unsigned long long t =
+            atomicAdd(sync.ticket, 1ull) - sync.epoch*(unsigned long long)S.nParts;
+        if (t >= (unsigned long long)S.nParts)
+        {
+            // ticket counter out of step with the launch count: never index
+            s_part = -1;
+            if (st) st->scheduleError = 1;
+        }
+        else
+        {
+            const int32_t ti = (int32_t)t;
+            s_part = BACKWARD ? S.order[S.nParts - 1 - ti] : S.order[ti];
+        }
+        s_stop = (st && st->done) ? 1 : 0;
+    }
+    __syncthreads();
+    const int32_t p = s_part;
+    if (p < 0) return;
+    const unsigned long long K = (unsigned long long)S.nLevels + 2;
+    const unsigned long long base = sync.epoch*K;
+    unsigned long long* progress = sync.progress;
+    const int32_t off = S.partLevelOff[p];
+    const int32_t nSeg = S.partLevelOff[p + 1] - off - 1;
+    const int32_t minL = S.partMinLevel[p];
+
+    if (s_stop)
+    {
+        // still publish completion so that nothing waits on this part
+        if (threadIdx.x == 0) st_progress(&progress[p], base + K - 1);
+        return;
+    }
+
+    const int32_t depBeg = BACKWARD ? S.downOff[p] : S.upOff[p];
+    const int32_t nDep = (BACKWARD ? S.downOff[p + 1] : S.upOff[p + 1]) - depBeg;
+    const int32_t* depList = BACKWARD ? S.downList : S.upList;
+    // thread t < nDep tracks dependency t; its last observed progress value
+    unsigned long long seen = 0;
+    int32_t myDep = -1;
+    if ((int32_t)threadIdx.x < nDep) myDep = depList[depBeg + threadIdx.x];
+    bool expired = false;
+    const bool multi = S.nParts > 1;
+    // levels below the first (above the last, backward) are trivially done
+    if (multi && threadIdx.x == 0)
+    {
+        const int32_t maxL = minL + nSeg - 1;
+        st_progress(&progress[p],
+                    base + (BACKWARD ? (unsigned long long)(S.nLevels - maxL)
+                                     : (unsigned long long)minL));
+    }
+
+    for (int32_t s = 0; s < nSeg; s++)
+    {
+        const int32_t si = BACKWARD ? nSeg - 1 - s : s;
+        const int32_t L = minL + si;
+        const int32_t r0 = S.segStart[off + si], r1 = S.segStart[off + si + 1];
+        if (r0 == r1) continue;
+        if (myDep >= 0)
+        {
+            // forward: need k >= L  (all levels <= L-1 done, k = level+1)
+            // backward: need k >= nLevels - L  (all levels >= L+1 done)
+            const unsigned long long need =
+                base + (BACKWARD ? (unsigned long long)(S.nLevels - L)
+                                 : (unsigned long long)L);
+            uint32_t spins = 0;
+            while (seen < need && !expired)
+            {
+                seen = ld_progress(&progress[myDep]);
+                if (seen >= need) break;
+                if (++spins > FH_SPIN_LIMIT) { expired = true; break; }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        if (nDep > 0)
+        {
+            __syncthreads();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        }
+        for (int32_t r = r0 + (int32_t)threadIdx.x; r < r1; r += FH_BLOCK)
+            sweep_row<T, KIND>(R, d, r);
+        if (multi) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        if (multi && threadIdx.x == 0)
+            st_progress(&progress[p],
+                        base + (BACKWARD ? (unsigned long long)(S.nLevels - L + 1)
+                                         : (unsigned long long)(L + 1)));
+    }
+    if (threadIdx.x == 0) st_progress(&progress[p], base + K - 1);
+    if (expired && st) st->scheduleError = 1;
+}
+
+template<class T, int KIND, bool BACKWARD>
+static void launch_sweep_kind(const RowsView& R, const SchedView& S, SweepSync& sync,
+                              const SweepData<T>& d, DevState* st, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_sweep_parts<T, KIND, BACKWARD>), dim3(S.nParts), dim3(FH_BLOCK),
+                       0, s, R, S, sync, d, st);
+    sync.epoch++;
+}
+
+template<class T>
+void launch_sweep(int kind, const RowsView& R, const SchedView& S, SweepSync& sync,
+                  const SweepData<T>& d, DevState* st, hipStream_t s)
+{
+    switch (kind)
+    {
+        case SWEEP_PRECOND_FWD: launch_sweep_kind<T, SWEEP_PRECOND_FWD, false>(R, S, sync, d, st, s); break;
+        case SWEEP_PRECOND_BWD: launch_sweep_kind<T, SWEEP_PRECOND_BWD, true>(R, S, sync, d, st, s); break;
+        case SWEEP_RD_DIC:      launch_sweep_kind<T, SWEEP_RD_DIC, false>(R, S, sync, d, st, s); break;
+        case SWEEP_RD_DILU:     launch_sweep_kind<T, SWEEP_RD_DILU, false>(R, S, sync, d, st, s); break;
+        case SWEEP_GS_FWD:      launch_sweep_kind<T, SWEEP_GS_FWD, false>(R, S, sync, d, st, s); break;
+        case SWEEP_GS_BWD:      launch_sweep_kind<T, SWEEP_GS_BWD, true>(R, S, sync, d, st, s); break;
+        default: fail(FH_ERR_INVALID, "unknown sweep kind %d", kind);
+    }
+}
+
+// explicit instantiations
+template void launch_gather_values<double>(const RowsView&, int64_t, const double*, const double*, double*, hipStream_t);
+template void launch_gather_values<float>(const RowsView&, int64_t, const double*, const double*, float*, hipStream_t);
+template void launch_permute_in<double, double>(int32_t, const int32_t*, const double*, double*, hipStream_t);
+template void launch_permute_out<double, double>(int32_t, const int32_t*, const double*, double*, hipStream_t);
+template void launch_convert<double, float>(int64_t, const double*, float*, hipStream_t);
+template void launch_convert<float, double>(int64_t, const float*, double*, hipStream_t);
+template void launch_amul<double>(const RowsView&, const double*, const double*, const double*, double*, const DevState*, hipStream_t);
+template void launch_amul<float>(const RowsView&, const float*, const float*, const float*, float*, const DevState*, hipStream_t);
+template void launch_tmul<double>(const RowsView&, const double*, const double*, const double*, double*, hipStream_t);
+template void launch_residual<double>(const RowsView&, const double*, const double*, const double*, const double*, double*, const DevState*, hipStream_t);
+template void launch_residual<float>(const RowsView&, const float*, const float*, const float*, const float*, float*, const DevState*, hipStream_t);
+template void launch_sweep<double>(int, const RowsView&, const SchedView&, SweepSync&, const SweepData<double>&, DevState*, hipStream_t);
+template void launch_sweep<float>(int, const RowsView&, const SchedView&, SweepSync&, const SweepData<float>&, DevState*, hipStream_t);
+template void launch_reciprocal<double>(int32_t, double*, hipStream_t);
+template void launch_reciprocal<float>(int32_t, float*, hipStream_t);
+
+} // namespace fh

@@ -1,0 +1,791 @@
+// Matrix handling and the device solver drivers: PCG (PCG.C:65-193),
+// PBiCGStab (PBiCGStab.C:68-254), PBiCG (PBiCG.C:65-222), smoothSolver
+// (smoothSolver.C:77-193) and diagonalSolver, plus the mixed-precision
+// iterative refinement around FP32 inner PCG / PBiCGStab solves.
+#include "fh_matrix.h"
+#include "fh_gamg.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace fh
+{
+
+void matrix_init(fh_matrix* M, fh_mesh* mesh)
+{
+    M->mesh = mesh;
+    FH_HIP(hipSetDevice(mesh->device));
+    FH_HIP(hipStreamCreateWithFlags(&M->stream, hipStreamNonBlocking));
+    FH_HIP(hipEventCreate(&M->evStart));
+    FH_HIP(hipEventCreate(&M->evStop));
+    M->d_state.alloc(1);
+    FH_HIP(hipHostMalloc((void**)&M->h_state, sizeof(DevState), hipHostMallocDefault));
+    std::memset(M->h_state, 0, sizeof(DevState));
+    M->d_partials.alloc(2*reduction_partials(mesh->nCells) + 64);
+    M->d_sync.alloc(1 + (size_t)mesh->sched.nParts);
+    FH_HIP(hipMemsetAsync(M->d_sync.p, 0, M->d_sync.bytes(), M->stream));
+    FH_HIP(hipStreamSynchronize(M->stream));
+    M->epoch = 0;
+}
+
+void matrix_free(fh_matrix* M)
+{
+    if (M->stream) (void)hipStreamSynchronize(M->stream);
+    if (M->h_state) (void)hipHostFree(M->h_state);
+    M->h_state = nullptr;
+    if (M->evStart) (void)hipEventDestroy(M->evStart);
+    if (M->evStop) (void)hipEventDestroy(M->evStop);
+    if (M->stream) (void)hipStreamDestroy(M->stream);
+    M->stream = nullptr;
+}
+
+void matrix_set(fh_matrix* M, const double* diag, const double* upper, const double* lower,
+                bool device)
+{
+    fh_mesh* mesh = M->mesh;
+    const int32_t N = mesh->nCells, F = mesh->nFaces;
+    if (!diag) fail(FH_ERR_INVALID, "diag must not be NULL");
+    if (!upper && lower) fail(FH_ERR_INVALID, "lower given without upper");
+    hipStream_t s = M->stream;
+    const hipMemcpyKind kind = device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+
+    M->diagonalOnly = (upper == nullptr);
+    M->symmetric = (lower == nullptr);
+    DBuf<double> diagNat;
+    diagNat.alloc(N);
+    FH_HIP(hipMemcpyAsync(diagNat.p, diag, sizeof(double)*N, kind, s));
+    M->d_diag.alloc(N);
+    launch_permute_in<double, double>(N, mesh->d_perm.p, diagNat.p, M->d_diag.p, s);
+    if (!M->diagonalOnly && F > 0)
+    {
+        M->d_upperF.alloc(F);
+        FH_HIP(hipMemcpyAsync(M->d_upperF.p, upper, sizeof(double)*F, kind, s));
+        if (!M->symmetric)
+        {
+            M->d_lowerF.alloc(F);
+            FH_HIP(hipMemcpyAsync(M->d_lowerF.p, lower, sizeof(double)*F, kind, s));
+        }
+        else
+        {
+            M->d_lowerF.release();
+        }
+    }
+    else
+    {
+        M->d_upperF.release();
+        M->d_lowerF.release();
+    }
+    matrix_coeffs_updated(M);
+}
+
+// after d_upperF/d_lowerF/d_diag changed: rebuild the SELL values and drop
+// every cached factor
+void matrix_coeffs_updated(fh_matrix* M)
+{
+    fh_mesh* mesh = M->mesh;
+    hipStream_t s = M->stream;
+    M->d_val.alloc(std::max<int64_t>(mesh->nnzPad, 1));
+    const double* lo = M->symmetric ? M->d_upperF.p : M->d_lowerF.p;
+    if (M->diagonalOnly || mesh->nFaces == 0)
+        FH_HIP(hipMemsetAsync(M->d_val.p, 0, M->d_val.bytes(), s));
+    else
+        launch_gather_values<double>(M->rows(), mesh->nnzPad, M->d_upperF.p, lo, M->d_val.p, s);
+    FH_HIP(hipStreamSynchronize(s));
+    M->hasCoeffs = true;
+    M->valTValid = M->sumAValid = M->f32Valid = false;
+    for (int i = 0; i < 3; i++) M->rDValid[i] = M->rD32Valid[i] = false;
+    gamg_invalidate(M);
+}
+
+void ensure_sumA(fh_matrix* M)
+{
+    if (M->sumAValid) return;
+    M->d_sumA.alloc(M->mesh->nCells);
+    launch_sum_a(M->rows(), M->d_diag.p, M->d_val.p, M->d_sumA.p, M->stream);
+    M->sumAValid = true;
+}
+
+void ensure_valT(fh_matrix* M)
+{
+    if (M->valTValid) return;
+    fh_mesh* mesh = M->mesh;
+    M->d_valT.alloc(std::max<int64_t>(mesh->nnzPad, 1));
+    if (M->diagonalOnly || mesh->nFaces == 0)
+        FH_HIP(hipMemsetAsync(M->d_valT.p, 0, M->d_valT.bytes(), M->stream));
+    else
+    {
+        const double* lo = M->symmetric ? M->d_upperF.p : M->d_lowerF.p;
+        // swapped roles: lower entries take upper[f], upper entries lower[f]
+        launch_gather_values<double>(M->rows(), mesh->nnzPad, lo, M->d_upperF.p,
+                                     M->d_valT.p, M->stream);
+    }
+    M->valTValid = true;
+}
+
+void ensure_f32(fh_matrix* M)
+{
+    if (M->f32Valid) return;
+    fh_mesh* mesh = M->mesh;
+    M->d_diag32.alloc(mesh->nCells);
+    M->d_val32.alloc(std::max<int64_t>(mesh->nnzPad, 1));
+    launch_convert<double, float>(mesh->nCells, M->d_diag.p, M->d_diag32.p, M->stream);
+    launch_convert<double, float>(std::max<int64_t>(mesh->nnzPad, 1), M->d_val.p,
+                                  M->d_val32.p, M->stream);
+    M->f32Valid = true;
+}
+
+static void sweep_d(fh_matrix* M, int kind, const SweepData<double>& d, DevState* st)
+{
+    SweepSync sy = M->sync();
+    launch_sweep<double>(kind, M->rows(), M->sched(), sy, d, st, M->stream);
+    M->epoch = sy.epoch;
+}
+
+static void sweep_f(fh_matrix* M, int kind, const SweepData<float>& d, DevState* st)
+{
+    SweepSync sy = M->sync();
+    launch_sweep<float>(kind, M->rows(), M->sched(), sy, d, st, M->stream);
+    M->epoch = sy.epoch;
+}
+
+const double* ensure_rD(fh_matrix* M, int which)
+{
+    DBuf<double>* bufs[3] = {&M->d_rD_DIC, &M->d_rD_DILU, &M->d_rD_diag};
+    DBuf<double>& b = *bufs[which];
+    if (M->rDValid[which]) return b.p;
+    const int32_t N = M->mesh->nCells;
+    b.alloc(N);
+    if (which == 2 || M->diagonalOnly || M->mesh->nFaces == 0)
+    {
+        // diagonalPreconditioner.C:50-63 (and the factor of a matrix
+        // without off-diagonals): 1/diag
+        FH_HIP(hipMemcpyAsync(b.p, M->d_diag.p, sizeof(double)*N, hipMemcpyDeviceToDevice,
+                              M->stream));
+    }
+    else
+    {
+        SweepData<double> d{};
+        d.val = M->d_val.p;
+        d.diag = M->d_diag.p;
+        d.w = b.p;
+        d.upperF = M->d_upperF.p;
+        sweep_d(M, which == 0 ? SWEEP_RD_DIC : SWEEP_RD_DILU, d, nullptr);
+    }
+    launch_reciprocal<double>(N, b.p, M->stream);
+    M->rDValid[which] = true;
+    return b.p;
+}
+
+const float* ensure_rD32(fh_matrix* M, int which)
+{
+    DBuf<float>* bufs[3] = {&M->d_rD32_DIC, &M->d_rD32_DILU, &M->d_rD32_diag};
+    DBuf<float>& b = *bufs[which];
+    if (M->rD32Valid[which]) return b.p;
+    const double* r64 = ensure_rD(M, which);
+    b.alloc(M->mesh->nCells);
+    launch_convert<double, float>(M->mesh->nCells, r64, b.p, M->stream);
+    M->rD32Valid[which] = true;
+    return b.p;
+}
+
+static int factor_kind(int precon)
+{
+    switch (precon)
+    {
+        case FH_PRECON_DIC:
+        case FH_PRECON_FDIC: return 0;
+        case FH_PRECON_DILU: return 1;
+        case FH_PRECON_DIAGONAL: return 2;
+        default: return -1;
+    }
+}
+
+template<class T> static const T* rD_for(fh_matrix* M, int which);
+template<> const double* rD_for<double>(fh_matrix* M, int which) { return ensure_rD(M, which); }
+template<> const float* rD_for<float>(fh_matrix* M, int which) { return ensure_rD32(M, which); }
+
+template<class T> static const T* val_for(fh_matrix* M, bool transpose);
+template<> const double* val_for<double>(fh_matrix* M, bool transpose)
+{
+    if (transpose) { ensure_valT(M); return M->d_valT.p; }
+    return M->d_val.p;
+}
+template<> const float* val_for<float>(fh_matrix* M, bool)
+{
+    ensure_f32(M);
+    return M->d_val32.p;
+}
+
+template<class T> static const T* diag_for(fh_matrix* M);
+template<> const double* diag_for<double>(fh_matrix* M) { return M->d_diag.p; }
+template<> const float* diag_for<float>(fh_matrix* M) { ensure_f32(M); return M->d_diag32.p; }
+
+template<class T>
+static void sweep_t(fh_matrix* M, int kind, const SweepData<T>& d, DevState* st);
+template<> void sweep_t<double>(fh_matrix* M, int kind, const SweepData<double>& d, DevState* st)
+{
+    sweep_d(M, kind, d, st);
+}
+template<> void sweep_t<float>(fh_matrix* M, int kind, const SweepData<float>& d, DevState* st)
+{
+    sweep_f(M, kind, d, st);
+}
+
+template<class T>
+void apply_precond(fh_matrix* M, int precon, const T* r, T* w, double* partials,
+                   DevState* st, bool transpose)
+{
+    const int32_t N = M->mesh->nCells;
+    if (M->diagonalOnly || M->mesh->nFaces == 0)
+    {
+        // lduMatrixPreconditioner.C:84-93: a diagonal matrix gets no
+        // preconditioner; DIC/DILU factors reduce to 1/diag otherwise
+        if (precon == FH_PRECON_DIC || precon == FH_PRECON_FDIC || precon == FH_PRECON_DILU)
+            precon = FH_PRECON_DIAGONAL;
+        if (M->diagonalOnly) precon = FH_PRECON_NONE;
+    }
+    if (precon == FH_PRECON_NONE || precon == FH_PRECON_DIAGONAL)
+    {
+        const T* rD = precon == FH_PRECON_DIAGONAL ? rD_for<T>(M, 2) : nullptr;
+        launch_precond_point<T>(N, precon, rD, r, w, partials ? partials : M->d_partials.p,
+                                st, M->stream);
+        return;
+    }
+    const int which = factor_kind(precon);
+    if (which < 0) fail(FH_ERR_UNSUPPORTED, "preconditioner %d has no sweep form", precon);
+    SweepData<T> d{};
+    d.val = val_for<T>(M, transpose);
+    d.rD = rD_for<T>(M, which);
+    d.x = r;
+    d.w = w;
+    sweep_t<T>(M, SWEEP_PRECOND_FWD, d, st);
+    sweep_t<T>(M, SWEEP_PRECOND_BWD, d, st);
+    if (partials) launch_dot<T, T>(N, w, r, partials, st, M->stream);
+}
+
+template void apply_precond<double>(fh_matrix*, int, const double*, double*, double*, DevState*, bool);
+template void apply_precond<float>(fh_matrix*, int, const float*, float*, double*, DevState*, bool);
+
+// ---------------------------------------------------------------------------
+// batched enqueue of device iterations with a poll of DevState.done
+// ---------------------------------------------------------------------------
+template<class F>
+static void run_batches(fh_matrix* M, int32_t maxIter, F&& enqueue)
+{
+    int32_t batch = 1, seq = 0;
+    const int64_t cap = 4*(int64_t)std::max(maxIter, 1) + 64;
+    for (;;)
+    {
+        for (int32_t i = 0; i < batch; i++) enqueue(seq++);
+        FH_HIP(hipMemcpyAsync(M->h_state, M->d_state.p, sizeof(DevState),
+                              hipMemcpyDeviceToHost, M->stream));
+        FH_HIP(hipStreamSynchronize(M->stream));
+        if (M->h_state->done) break;
+        if (seq > cap) fail(FH_ERR_SCHEDULE, "solver did not terminate after %d steps", seq);
+        batch = std::min(batch*2, 32);
+    }
+}
+
+static void state_upload(fh_matrix* M, const fh_controls* c, double tol, double relTol,
+                         int32_t maxIter, int32_t minIter)
+{
+    DevState* h = M->h_state;
+    std::memset(h, 0, sizeof(DevState));
+    h->tolerance = tol;
+    h->relTol = relTol;
+    h->maxIter = maxIter;
+    h->minIter = minIter;
+    h->wArA = 1e20;      // SolverPerformance::great_ (PCG.C:92)
+    h->earlySeq = -1;
+    FH_HIP(hipMemcpyAsync(M->d_state.p, h, sizeof(DevState), hipMemcpyHostToDevice, M->stream));
+    FH_HIP(hipStreamSynchronize(M->stream));
+    (void)c;
+}
+
+static void state_upload_outer(fh_matrix* M, const DevState& outer)
+{
+    *M->h_state = outer;
+    M->h_state->done = 0;
+    FH_HIP(hipMemcpyAsync(M->d_state.p, M->h_state, sizeof(DevState), hipMemcpyHostToDevice,
+                          M->stream));
+    FH_HIP(hipStreamSynchronize(M->stream));
+}
+
+// FP64 initialisation shared by the solvers: wA = A psi, normFactor,
+// rA = source - wA, initial residual (lduMatrixSolver.C:174-197)
+static void init_fp64(fh_matrix* M, const double* psi, const double* src, double* rA,
+                      double* tmp)
+{
+    const int32_t N = M->mesh->nCells;
+    DevState* st = M->d_state.p;
+    double* part = M->d_partials.p;
+    ensure_sumA(M);
+    launch_amul<double>(M->rows(), M->d_diag.p, M->d_val.p, psi, tmp, nullptr, M->stream);
+    launch_sum_abs<double>(N, psi, part, M->stream);
+    launch_finalize(N, part, FIN_XREF, st, 0, M->stream);
+    launch_init_residual(N, tmp, src, M->d_sumA.p, rA, part, st, M->stream);
+    launch_finalize(N, part, FIN_INIT, st, 0, M->stream);
+}
+
+template<class T>
+static void pcg_loop(fh_matrix* M, int precon, T* psi, T* r, T* p, T* w, int32_t maxIter)
+{
+    const int32_t N = M->mesh->nCells;
+    DevState* st = M->d_state.p;
+    double* part = M->d_partials.p;
+    const T* diag = diag_for<T>(M);
+    const T* val = val_for<T>(M, false);
+    if (precon != FH_PRECON_NONE && factor_kind(precon) >= 0)
+        (void)rD_for<T>(M, factor_kind(precon));
+    run_batches(M, maxIter, [&](int32_t)
+    {
+        apply_precond<T>(M, precon, r, w, part, st, false);
+        launch_finalize(N, part, FIN_PCG_WARA, st, 0, M->stream);
+        launch_pcg_p<T>(N, w, p, st, M->stream);
+        launch_spmv_dot<T>(M->rows(), diag, val, p, w, p, nullptr, part, st, M->stream);
+        launch_finalize(N, part, FIN_PCG_WAPA, st, 0, M->stream);
+        launch_pcg_update<T>(N, psi, r, p, w, part, st, M->stream);
+        launch_finalize(N, part, FIN_PCG_RES, st, 0, M->stream);
+    });
+}
+
+template<class T>
+static void bicgstab_loop(fh_matrix* M, int precon, T* psi, T* rA, T** V, int32_t maxIter)
+{
+    const int32_t N = M->mesh->nCells;
+    DevState* st = M->d_state.p;
+    double* part = M->d_partials.p;
+    const T* diag = diag_for<T>(M);
+    const T* val = val_for<T>(M, false);
+    T *pA = V[0], *yA = V[1], *AyA = V[2], *sA = V[3], *zA = V[4], *tA = V[5], *rA0 = V[6];
+    if (precon != FH_PRECON_NONE && factor_kind(precon) >= 0)
+        (void)rD_for<T>(M, factor_kind(precon));
+    FH_HIP(hipMemcpyAsync(rA0, rA, sizeof(T)*N, hipMemcpyDeviceToDevice, M->stream));
+    launch_dot<T, T>(N, rA0, rA, part, st, M->stream);
+    launch_finalize(N, part, FIN_BICG_START, st, 0, M->stream);
+    run_batches(M, maxIter, [&](int32_t seq)
+    {
+        launch_bicg_p<T>(N, rA, pA, AyA, st, M->stream);
+        apply_precond<T>(M, precon, pA, yA, nullptr, st, false);
+        launch_spmv_dot<T>(M->rows(), diag, val, yA, AyA, rA0, nullptr, part, st, M->stream);
+        launch_finalize(N, part, FIN_BICG_ALPHA, st, seq, M->stream);
+        launch_bicg_s<T>(N, rA, AyA, sA, part, st, M->stream);
+        launch_finalize(N, part, FIN_BICG_S, st, seq, M->stream);
+        launch_bicg_early<T>(N, psi, yA, st, seq, M->stream);
+        apply_precond<T>(M, precon, sA, zA, nullptr, st, false);
+        launch_spmv_dot<T>(M->rows(), diag, val, zA, tA, tA, sA, part, st, M->stream);
+        launch_finalize(N, part, FIN_BICG_OMEGA, st, seq, M->stream);
+        launch_bicg_update<T>(N, psi, yA, zA, rA, sA, tA, rA0, part, st, M->stream);
+        launch_finalize(N, part, FIN_BICG_RES, st, seq, M->stream);
+    });
+}
+
+static void fill_perf(fh_matrix* M, const fh_controls* c, fh_perf* perf, const char* name)
+{
+    const DevState* h = M->h_state;
+    std::memset(perf, 0, sizeof(*perf));
+    std::snprintf(perf->solverName, sizeof(perf->solverName), "%s", name);
+    perf->initialResidual = h->initialResidual;
+    perf->finalResidual = h->finalResidual;
+    perf->nIterations = h->nIterations;
+    perf->converged = h->converged;
+    perf->singular = h->singular;
+    perf->normFactor = h->normFactor;
+    (void)c;
+}
+
+static const char* precon_name(int p)
+{
+    switch (p)
+    {
+        case FH_PRECON_NONE: return "none";
+        case FH_PRECON_DIAGONAL: return "diagonal";
+        case FH_PRECON_DIC: return "DIC";
+        case FH_PRECON_FDIC: return "FDIC";
+        case FH_PRECON_DILU: return "DILU";
+        case FH_PRECON_GAMG: return "GAMG";
+        default: return "?";
+    }
+}
+
+// validation against the reference run-time tables (lduMatrixSolver.C:62-120,
+// lduMatrixPreconditioner.C:94-140)
+static void validate_selection(const fh_matrix* M, const fh_controls* c)
+{
+    if (M->diagonalOnly) return;
+    const bool sym = M->symmetric;
+    const int s = c->solver;
+    if (sym && s == FH_SOLVER_PBICG)
+        fail(FH_ERR_INVALID, "Unknown symmetric matrix solver PBiCG; valid: GAMG PBiCGStab PCG smoothSolver");
+    if (!sym && s == FH_SOLVER_PCG)
+        fail(FH_ERR_INVALID, "Unknown asymmetric matrix solver PCG; valid: GAMG PBiCG PBiCGStab smoothSolver");
+    if (s == FH_SOLVER_PCG || s == FH_SOLVER_PBICGSTAB || s == FH_SOLVER_PBICG)
+    {
+        const int p = c->preconditioner;
+        if (sym && p == FH_PRECON_DILU)
+            fail(FH_ERR_INVALID, "Unknown symmetric matrix preconditioner DILU; valid: DIC FDIC GAMG diagonal none");
+        if (!sym && (p == FH_PRECON_DIC || p == FH_PRECON_FDIC))
+            fail(FH_ERR_INVALID, "Unknown asymmetric matrix preconditioner %s; valid: DILU GAMG diagonal none",
+                 precon_name(p));
+    }
+    if (s == FH_SOLVER_SMOOTH || s == FH_SOLVER_GAMG)
+    {
+        const int m = c->smoother;
+        if (sym && (m == FH_SMOOTH_DILU || m == FH_SMOOTH_DILUGS))
+            fail(FH_ERR_INVALID, "Unknown symmetric matrix smoother; valid: DIC DICGaussSeidel FDIC GaussSeidel nonBlockingGaussSeidel symGaussSeidel");
+        if (!sym && (m == FH_SMOOTH_DIC || m == FH_SMOOTH_DICGS || m == FH_SMOOTH_FDIC))
+            fail(FH_ERR_INVALID, "Unknown asymmetric matrix smoother; valid: DILU DILUGaussSeidel GaussSeidel nonBlockingGaussSeidel symGaussSeidel");
+    }
+}
+
+// ---------------------------------------------------------------------------
+// smoothers (library numbering, FP64)
+// ---------------------------------------------------------------------------
+void smooth_device(fh_matrix* M, int smoother, double* psi, const double* src,
+                   int32_t nSweeps, DevState* st)
+{
+    const int32_t N = M->mesh->nCells;
+    if (nSweeps <= 0 || smoother == FH_SMOOTH_NONE) return;
+    auto gs = [&](bool sym)
+    {
+        SweepData<double> d{};
+        d.val = M->d_val.p;
+        d.diag = M->d_diag.p;
+        d.x = src;
+        d.w = psi;
+        for (int32_t s = 0; s < nSweeps; s++)
+        {
+            sweep_d(M, SWEEP_GS_FWD, d, st);
+            if (sym) sweep_d(M, SWEEP_GS_BWD, d, st);
+        }
+    };
+    auto ilu = [&](bool dilu)
+    {
+        // DICSmoother.C:67-116 / DILUSmoother.C:67-119
+        double* rA = M->v[9].p;
+        if (!rA || M->v[9].n < (size_t)N) { M->v[9].alloc(N); rA = M->v[9].p; }
+        double* wA = M->v[8].p;
+        if (!wA || M->v[8].n < (size_t)N) { M->v[8].alloc(N); wA = M->v[8].p; }
+        for (int32_t s = 0; s < nSweeps; s++)
+        {
+            launch_residual<double>(M->rows(), M->d_diag.p, M->d_val.p, psi, src, rA, st,
+                                    M->stream);
+            apply_precond<double>(M, dilu ? FH_PRECON_DILU : FH_PRECON_DIC, rA, wA, nullptr,
+                                  st, false);
+            launch_axpy_inplace(N, psi, wA, st, M->stream);
+        }
+    };
+    switch (smoother)
+    {
+        case FH_SMOOTH_GS: gs(false); break;
+        case FH_SMOOTH_SYMGS: gs(true); break;
+        case FH_SMOOTH_DIC:
+        case FH_SMOOTH_FDIC: ilu(false); break;
+        case FH_SMOOTH_DILU: ilu(true); break;
+        case FH_SMOOTH_DICGS: ilu(false); gs(false); break;
+        case FH_SMOOTH_DILUGS: ilu(true); gs(false); break;
+        default: fail(FH_ERR_INVALID, "unknown smoother %d", smoother);
+    }
+}
+
+// smoothSolver.C:77-193
+static void smooth_solver(fh_matrix* M, const fh_controls* c, double* psi, const double* src)
+{
+    const int32_t N = M->mesh->nCells;
+    DevState* st = M->d_state.p;
+    double* part = M->d_partials.p;
+    if (c->nSweeps < 0)
+    {
+        smooth_device(M, c->smoother, psi, src, -c->nSweeps, nullptr);
+        FH_HIP(hipStreamSynchronize(M->stream));
+        std::memset(M->h_state, 0, sizeof(DevState));
+        M->h_state->nIterations = -c->nSweeps;
+        return;
+    }
+    for (int i = 0; i < 2; i++) M->v[i].alloc(N);
+    init_fp64(M, psi, src, M->v[0].p, M->v[1].p);
+    run_batches(M, c->maxIter, [&](int32_t)
+    {
+        smooth_device(M, c->smoother, psi, src, c->nSweeps, st);
+        launch_residual<double>(M->rows(), M->d_diag.p, M->d_val.p, psi, src, M->v[0].p, st,
+                                M->stream);
+        launch_sum_abs_state<double>(N, M->v[0].p, part, st, M->stream);
+        launch_finalize_smooth(N, part, st, c->nSweeps, M->stream);
+    });
+}
+
+// ---------------------------------------------------------------------------
+// mixed precision: FP64 outer residual / normFactor / convergence test /
+// solution update around FP32 inner solves
+// ---------------------------------------------------------------------------
+static void mixed_solve(fh_matrix* M, const fh_controls* c, double* psi, const double* src,
+                        fh_perf* perf)
+{
+    const int32_t N = M->mesh->nCells;
+    DevState* st = M->d_state.p;
+    double* part = M->d_partials.p;
+    for (int i = 0; i < 2; i++) M->v[i].alloc(N);
+    double* rA = M->v[0].p;
+    double* tmp = M->v[1].p;
+    for (int i = 0; i < 9; i++) M->f[i].alloc(N);
+    ensure_f32(M);
+
+    state_upload(M, c, c->tolerance, c->relTol, c->maxIter, c->minIter);
+    init_fp64(M, psi, src, rA, tmp);
+    FH_HIP(hipMemcpyAsync(M->h_state, M->d_state.p, sizeof(DevState), hipMemcpyDeviceToHost,
+                          M->stream));
+    FH_HIP(hipStreamSynchronize(M->stream));
+    DevState outer = *M->h_state;
+    int32_t totalInner = 0, nRefine = 0;
+    const double innerRel = c->innerRelTol > 0 ? c->innerRelTol : 1e-4;
+    const int32_t maxRef = c->maxRefinements > 0 ? c->maxRefinements : 50;
+    bool singular = false;
+
+    while (!outer.done && !outer.numericError)
+    {
+        if (nRefine >= maxRef || totalInner >= c->maxIter) break;
+        // scaled FP32 right-hand side r / s, s = mean |r|
+        const double s = outer.finalResidual*outer.normFactor/N;
+        const double scale = (s > 0 && std::isfinite(s)) ? s : 1.0;
+        launch_scale_convert(N, rA, 1.0/scale, M->f[0].p, M->stream);     // r32
+        FH_HIP(hipMemsetAsync(M->f[1].p, 0, sizeof(float)*N, M->stream));  // d32 = 0
+        // inner controls: relative reduction only
+        const double target = std::max(innerRel, 0.3*c->tolerance/std::max(outer.finalResidual, 1e-300));
+        state_upload(M, c, 0.0, std::min(target, 0.5), c->maxIter - totalInner, 0);
+        launch_sum_abs<float>(N, M->f[0].p, part, M->stream);
+        launch_finalize(N, part, FIN_INNER_INIT, st, 0, M->stream);
+        float* r32 = M->f[0].p;
+        float* d32 = M->f[1].p;
+        if (c->solver == FH_SOLVER_PBICGSTAB)
+        {
+            float* V[7] = {M->f[2].p, M->f[3].p, M->f[4].p, M->f[5].p, M->f[6].p, M->f[7].p,
+                           M->f[8].p};
+            bicgstab_loop<float>(M, c->preconditioner, d32, r32, V, c->maxIter - totalInner);
+        }
+        else
+        {
+            pcg_loop<float>(M, c->preconditioner, d32, r32, M->f[2].p, M->f[3].p,
+                            c->maxIter - totalInner);
+        }
+        const DevState inner = *M->h_state;
+        totalInner += inner.nIterations;
+        singular = inner.singular;
+        // psi += scale*d; r = source - A psi; outer convergence test
+        launch_axpy_f32(N, psi, d32, scale, M->stream);
+        state_upload_outer(M, outer);
+        launch_residual<double>(M->rows(), M->d_diag.p, M->d_val.p, psi, src, rA, nullptr,
+                                M->stream);
+        launch_sum_abs<double>(N, rA, part, M->stream);
+        launch_finalize(N, part, FIN_OUTER_RES, st, 0, M->stream);
+        FH_HIP(hipMemcpyAsync(M->h_state, M->d_state.p, sizeof(DevState),
+                              hipMemcpyDeviceToHost, M->stream));
+        FH_HIP(hipStreamSynchronize(M->stream));
+        outer = *M->h_state;
+        nRefine++;
+        if (inner.nIterations == 0 && !outer.done) break;   // no progress possible
+    }
+    std::memset(perf, 0, sizeof(*perf));
+    std::snprintf(perf->solverName, sizeof(perf->solverName), "%s%s(mixed)",
+                  precon_name(c->preconditioner),
+                  c->solver == FH_SOLVER_PBICGSTAB ? "PBiCGStab" : "PCG");
+    perf->initialResidual = outer.initialResidual;
+    perf->finalResidual = outer.finalResidual;
+    perf->nIterations = totalInner;
+    perf->converged = outer.converged;
+    perf->singular = singular && !outer.converged;
+    perf->nRefinements = nRefine;
+    perf->normFactor = outer.normFactor;
+    if (outer.numericError) fail(FH_ERR_NUMERIC, "non-finite residual in mixed-precision solve");
+}
+
+// ---------------------------------------------------------------------------
+void solve_device(fh_matrix* M, const fh_controls* c, double* d_psiNat,
+                  const double* d_srcNat, fh_perf* perf)
+{
+    if (!M->hasCoeffs) fail(FH_ERR_INVALID, "matrix coefficients not set");
+    validate_selection(M, c);
+    const int32_t N = M->mesh->nCells;
+    const int32_t* perm = M->mesh->d_perm.p;
+    hipStream_t s = M->stream;
+    for (int i = 0; i < 2; i++) M->v[i].alloc(N);
+    M->v[2].alloc(N);
+    M->v[3].alloc(N);
+    double* psi = M->v[2].p;
+    double* src = M->v[3].p;
+    FH_HIP(hipEventRecord(M->evStart, s));
+    launch_permute_in<double, double>(N, perm, d_psiNat, psi, s);
+    launch_permute_in<double, double>(N, perm, d_srcNat, src, s);
+
+    if (M->diagonalOnly || c->solver == FH_SOLVER_DIAGONAL)
+    {
+        // diagonalSolver.C:62-81
+        launch_divide(N, src, M->d_diag.p, psi, s);
+        std::memset(perf, 0, sizeof(*perf));
+        std::snprintf(perf->solverName, sizeof(perf->solverName), "diagonal");
+        perf->converged = 1;
+    }
+    else if (c->precision == FH_PRECISION_MIXED
+             && (c->solver == FH_SOLVER_PCG || c->solver == FH_SOLVER_PBICGSTAB))
+    {
+        mixed_solve(M, c, psi, src, perf);
+    }
+    else if (c->precision == FH_PRECISION_MIXED && c->solver == FH_SOLVER_GAMG)
+    {
+        gamg_solve(M, c, psi, src, perf, true);
+    }
+    else
+    {
+        switch (c->solver)
+        {
+            case FH_SOLVER_PCG:
+            {
+                for (int i = 4; i < 7; i++) M->v[i].alloc(N);
+                double *rA = M->v[4].p, *pA = M->v[5].p, *wA = M->v[6].p;
+                state_upload(M, c, c->tolerance, c->relTol, c->maxIter, c->minIter);
+                init_fp64(M, psi, src, rA, wA);
+                if (c->preconditioner == FH_PRECON_GAMG)
+                    gamg_pcg(M, c, psi, rA, pA, wA);
+                else
+                    pcg_loop<double>(M, c->preconditioner, psi, rA, pA, wA, c->maxIter);
+                FH_HIP(hipMemcpyAsync(M->h_state, M->d_state.p, sizeof(DevState),
+                                      hipMemcpyDeviceToHost, s));
+                FH_HIP(hipStreamSynchronize(s));
+                char name[64];
+                std::snprintf(name, sizeof(name), "%sPCG", precon_name(c->preconditioner));
+                fill_perf(M, c, perf, name);
+                break;
+            }
+            case FH_SOLVER_PBICGSTAB:
+            {
+                for (int i = 4; i < 12 && i < 10; i++) M->v[i].alloc(N);
+                double* rA = M->v[4].p;
+                double* tmp = M->v[5].p;
+                state_upload(M, c, c->tolerance, c->relTol, c->maxIter, c->minIter);
+                init_fp64(M, psi, src, rA, tmp);
+                DBuf<double> extra[2];
+                extra[0].alloc(N);
+                extra[1].alloc(N);
+                double* V[7] = {M->v[5].p, M->v[6].p, M->v[7].p, M->v[8].p, M->v[9].p,
+                                extra[0].p, extra[1].p};
+                if (c->preconditioner == FH_PRECON_GAMG)
+                    fail(FH_ERR_UNSUPPORTED, "PBiCGStab with the GAMG preconditioner");
+                FH_HIP(hipMemcpyAsync(M->h_state, M->d_state.p, sizeof(DevState),
+                                      hipMemcpyDeviceToHost, s));
+                FH_HIP(hipStreamSynchronize(s));
+                if (!M->h_state->done)
+                    bicgstab_loop<double>(M, c->preconditioner, psi, rA, V, c->maxIter);
+                FH_HIP(hipMemcpyAsync(M->h_state, M->d_state.p, sizeof(DevState),
+                                      hipMemcpyDeviceToHost, s));
+                FH_HIP(hipStreamSynchronize(s));
+                char name[64];
+                std::snprintf(name, sizeof(name), "%sPBiCGStab", precon_name(c->preconditioner));
+                fill_perf(M, c, perf, name);
+                break;
+            }
+            case FH_SOLVER_PBICG:
+            {
+                pbicg_solve(M, c, psi, src, perf);
+                break;
+            }
+            case FH_SOLVER_SMOOTH:
+            {
+                state_upload(M, c, c->tolerance, c->relTol, c->maxIter, c->minIter);
+                smooth_solver(M, c, psi, src);
+                FH_HIP(hipStreamSynchronize(s));
+                fill_perf(M, c, perf, "smoothSolver");
+                break;
+            }
+            case FH_SOLVER_GAMG:
+                gamg_solve(M, c, psi, src, perf, false);
+                break;
+            default:
+                fail(FH_ERR_INVALID, "unknown solver id %d", c->solver);
+        }
+    }
+    launch_permute_out<double, double>(N, perm, psi, d_psiNat, s);
+    FH_HIP(hipEventRecord(M->evStop, s));
+    FH_HIP(hipStreamSynchronize(s));
+    float ms = 0.f;
+    FH_HIP(hipEventElapsedTime(&ms, M->evStart, M->evStop));
+    perf->solveMilliseconds = ms;
+    if (M->h_state->scheduleError)
+        fail(FH_ERR_SCHEDULE, "a sweep dependency wait expired");
+    if (M->h_state->numericError)
+        fail(FH_ERR_NUMERIC, "non-finite residual");
+}
+
+// ---------------------------------------------------------------------------
+// measurement: back-to-back launches of one kernel between HIP events
+// ---------------------------------------------------------------------------
+template<class T>
+static void profile_t(fh_matrix* M, int kernel, int precon, int reps, double* ms,
+                      double* bytes)
+{
+    const int32_t N = M->mesh->nCells;
+    const int64_t F = M->mesh->nFaces;
+    const double w = sizeof(T);
+    DBuf<T> x, y, z;
+    x.alloc(N); y.alloc(N); z.alloc(N);
+    FH_HIP(hipMemsetAsync(x.p, 0, x.bytes(), M->stream));
+    FH_HIP(hipMemsetAsync(y.p, 0, y.bytes(), M->stream));
+    FH_HIP(hipMemsetAsync(z.p, 0, z.bytes(), M->stream));
+    const T* diag = diag_for<T>(M);
+    const T* val = val_for<T>(M, false);
+    const int which = factor_kind(precon);
+    if ((kernel == FH_KERNEL_SWEEP_FORWARD || kernel == FH_KERNEL_SWEEP_BACKWARD)
+        && (which < 0 || which == 2))
+        fail(FH_ERR_INVALID, "sweep profiling needs DIC, FDIC or DILU");
+    const T* rD = which >= 0 ? rD_for<T>(M, which) : nullptr;
+    SweepData<T> d{};
+    d.val = val; d.rD = rD; d.x = x.p; d.w = y.p;
+    auto once = [&]()
+    {
+        switch (kernel)
+        {
+            case FH_KERNEL_SPMV:
+                launch_amul<T>(M->rows(), diag, val, x.p, y.p, nullptr, M->stream);
+                break;
+            case FH_KERNEL_SWEEP_FORWARD:
+                sweep_t<T>(M, SWEEP_PRECOND_FWD, d, nullptr);
+                break;
+            case FH_KERNEL_SWEEP_BACKWARD:
+                sweep_t<T>(M, SWEEP_PRECOND_BWD, d, nullptr);
+                break;
+            case FH_KERNEL_AXPY_NORM:
+                launch_pcg_update_nostate<T>(N, x.p, y.p, z.p, z.p, M->d_partials.p, M->stream);
+                break;
+            default:
+                fail(FH_ERR_INVALID, "unknown kernel id %d", kernel);
+        }
+    };
+    once();
+    FH_HIP(hipStreamSynchronize(M->stream));
+    FH_HIP(hipEventRecord(M->evStart, M->stream));
+    for (int i = 0; i < reps; i++) once();
+    FH_HIP(hipEventRecord(M->evStop, M->stream));
+    FH_HIP(hipStreamSynchronize(M->stream));
+    float t = 0.f;
+    FH_HIP(hipEventElapsedTime(&t, M->evStart, M->evStop));
+    *ms = (double)t/reps;
+    switch (kernel)
+    {
+        case FH_KERNEL_SPMV:          // diag, x, y, row info + (value, column) per off-diagonal
+            *bytes = N*(3*w + 4) + 2.0*F*(w + 4); break;
+        case FH_KERNEL_SWEEP_FORWARD: // input, rD, output, row info + one triangle
+        case FH_KERNEL_SWEEP_BACKWARD:
+            *bytes = N*(3*w + 4) + 1.0*F*(w + 4); break;
+        case FH_KERNEL_AXPY_NORM:     // psi, r read+write, p, w read
+            *bytes = 6.0*N*w; break;
+    }
+}
+
+void profile_kernel(fh_matrix* M, int kernel, int precision, int precon, int reps,
+                    double* ms, double* bytes)
+{
+    if (reps <= 0) fail(FH_ERR_INVALID, "reps must be positive");
+    if (precision == FH_PRECISION_MIXED) profile_t<float>(M, kernel, precon, reps, ms, bytes);
+    else profile_t<double>(M, kernel, precon, reps, ms, bytes);
+}
+
+} // namespace fh

@@ -1,0 +1,83 @@
+// Launchers of the fused solver kernels (fh_solver_kernels.hip).
+#pragma once
+
+#include "fh_kernels.h"
+
+namespace fh
+{
+
+enum FinalizeOp
+{
+    FIN_SUM = 0,
+    FIN_XREF = 1,
+    FIN_INIT = 2,
+    FIN_PCG_WARA = 3,
+    FIN_PCG_WAPA = 4,
+    FIN_PCG_RES = 5,
+    FIN_BICG_START = 6,
+    FIN_BICG_ALPHA = 7,
+    FIN_BICG_S = 8,
+    FIN_BICG_OMEGA = 9,
+    FIN_BICG_RES = 10,
+    FIN_INNER_INIT = 11,
+    FIN_OUTER_RES = 12,
+    FIN_SMOOTH_RES = 13
+};
+
+void launch_finalize(int32_t n, const double* partials, int32_t op, DevState* st,
+                     int32_t seq, hipStream_t s);
+template<class Ta, class Tb>
+void launch_dot(int32_t n, const Ta* a, const Tb* b, double* partials, const DevState* st,
+                hipStream_t s);
+template<class T>
+void launch_sum_abs(int32_t n, const T* a, double* partials, hipStream_t s);
+void launch_init_residual(int32_t n, const double* Apsi, const double* source,
+                          const double* sumA, double* rA, double* partials,
+                          const DevState* st, hipStream_t s);
+template<class T>
+void launch_precond_point(int32_t n, int32_t kind, const T* rD, const T* r, T* w,
+                          double* partials, const DevState* st, hipStream_t s);
+template<class T>
+void launch_pcg_p(int32_t n, const T* w, T* p, const DevState* st, hipStream_t s);
+template<class T>
+void launch_spmv_dot(const RowsView& R, const T* diag, const T* val, const T* x, T* y,
+                     const T* z, const T* v, double* partials, const DevState* st,
+                     hipStream_t s);
+template<class T>
+void launch_pcg_update(int32_t n, T* psi, T* r, const T* p, const T* w, double* partials,
+                       const DevState* st, hipStream_t s);
+template<class T>
+void launch_bicg_p(int32_t n, const T* r, T* p, const T* AyA, const DevState* st,
+                   hipStream_t s);
+template<class T>
+void launch_bicg_s(int32_t n, const T* r, const T* AyA, T* sA, double* partials,
+                   const DevState* st, hipStream_t s);
+template<class T>
+void launch_bicg_early(int32_t n, T* psi, const T* y, const DevState* st, int32_t seq,
+                       hipStream_t s);
+template<class T>
+void launch_bicg_update(int32_t n, T* psi, const T* y, const T* z, T* r, const T* sA,
+                        const T* tA, const T* r0, double* partials, const DevState* st,
+                        hipStream_t s);
+
+template<class T>
+void launch_pcg_update_nostate(int32_t n, T* psi, T* r, const T* p, const T* w,
+                               double* partials, hipStream_t s);
+// psi += w
+void launch_axpy_inplace(int32_t n, double* psi, const double* w, const DevState* st,
+                         hipStream_t s);
+// sum |x| (and sum x) with the done test
+template<class T>
+void launch_sum_abs_state(int32_t n, const T* a, double* partials, const DevState* st,
+                          hipStream_t s);
+void launch_finalize_smooth(int32_t n, const double* partials, DevState* st,
+                            int32_t nSweeps, hipStream_t s);
+// y32 = (float)(x*scale)
+void launch_scale_convert(int32_t n, const double* x, double scale, float* y, hipStream_t s);
+// psi += scale*(double)d
+void launch_axpy_f32(int32_t n, double* psi, const float* d, double scale, hipStream_t s);
+// psi = src/diag
+void launch_divide(int32_t n, const double* src, const double* diag, double* psi,
+                   hipStream_t s);
+
+} // namespace fh

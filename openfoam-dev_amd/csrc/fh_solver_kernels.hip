@@ -1,0 +1,636 @@
+// Fused vector kernels and the device-side scalar logic of the Krylov
+// solvers.  Each vector kernel writes one deterministic partial sum per
+// block; k_finalize adds the partials in a fixed order and applies the
+// solver's scalar update to DevState, so a solve needs no host round trip
+// per iteration.
+#include "fh_kernels.h"
+#include "fh_solver_kernels.h"
+
+#include <cfloat>
+#include <cmath>
+
+namespace fh
+{
+
+#define FH_BLOCK 256
+
+__device__ __forceinline__ int32_t sell_base2(const int32_t* sliceStart, int32_t r)
+{
+    return sliceStart[r >> 6] + (r & 63);
+}
+
+// deterministic two-value block sum; results valid in thread 0
+__device__ __forceinline__ void block_sum2(double& a, double& b, double* smem)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+    {
+        a += __shfl_xor(a, off, 64);
+        b += __shfl_xor(b, off, 64);
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0)
+    {
+        smem[wid] = a;
+        smem[FH_BLOCK/64 + wid] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        double sa = 0.0, sb = 0.0;
+#pragma unroll
+        for (int i = 0; i < FH_BLOCK/64; i++) { sa += smem[i]; sb += smem[FH_BLOCK/64 + i]; }
+        a = sa;
+        b = sb;
+    }
+}
+
+__device__ __forceinline__ void write_partials(double a, double b, double* partials,
+                                               int32_t nBlocks)
+{
+    __shared__ double smem[2*FH_BLOCK/64];
+    block_sum2(a, b, smem);
+    if (threadIdx.x == 0)
+    {
+        partials[blockIdx.x] = a;
+        partials[nBlocks + blockIdx.x] = b;
+    }
+}
+
+// ---- reductions over vectors ----------------------------------------------
+
+template<class Ta, class Tb>
+__global__ void k_dot(int32_t n, const Ta* __restrict__ a, const Tb* __restrict__ b,
+                      double* partials, const DevState* st)
+{
+    if (st && st->done) return;
+    const int32_t i = blockIdx.x*FH_BLOCK + threadIdx.x;
+    double v = 0.0;
+    if (i < n) v = (double)a[i]*(double)b[i];
+    write_partials(v, 0.0, partials, gridDim.x);
+}
+
+template<class T>
+__global__ void k_sum_abs(int32_t n, const T* __restrict__ a, double* partials)
+{
+    const int32_t i = blockIdx.x*FH_BLOCK + threadIdx.x;
+    double s = 0.0, m = 0.0;
+    if (i < n) { s = (double)a[i]; m = fabs((double)a[i]); }
+    write_partials(s, m, partials, gridDim.x);
+}
+
+// lduMatrixSolver.C:174-197 and PCG.C:95-111: rA = source - Apsi,
+// sum0 = sum |Apsi - xRef*sumA| + |source - xRef*sumA|, sum1 = sum |rA|
+__global__ void k_init_residual(int32_t n, const double* __restrict__ Apsi,
+                                const double* __restrict__ source,
+                                const double* __restrict__ sumA, double* __restrict__ rA,
+                                double* partials, const DevState* st)
+{
+    const int32_t i = blockIdx.x*FH_BLOCK + threadIdx.x;
+    double nf = 0.0, rm = 0.0;
+    if (i < n)
+    {
+        const double t = sumA[i]*st->sum0;
+        nf = fabs(Apsi[i] - t) + fabs(source[i] - t);
+        const double r = source[i] - Apsi[i];
+        rA[i] = r;
+        rm = fabs(r);
+    }
+    write_partials(nf, rm, partials, gridDim.x);
+}
+
+// ---- PCG (PCG.C:129-189) -------------------------------------------------
+
+// diagonal / none preconditioner fused with wArA = sum wA*rA
+template<class T>
+__global__ void k_precond_point(int32_t n, int32_t kind, const T* __restrict__ rD,
+                                const T* __restrict__ r, T* __restrict__ w,
+                                double* partials, const DevState* st)
+{
+    if (st && st->done) return;
+    const int32_t i = blockIdx.x*FH_BLOCK + threadIdx.x;
+    double v = 0.0;
+    if (i < n)
+    {
+        const T wi = (kind == FH_PRECON_DIAGONAL) ? rD[i]*r[i] : r[i];
+        w[i] = wi;
+        v = (double)wi*(double)r[i];
+    }
+    write_partials(v, 0.0, partials, gridDim.x);
+}
+
+// pA = wA (first iteration) or wA + beta*pA
+template<class T>
+__global__ void k_pcg_p(int32_t n, const T* __restrict__ w, T* __restrict__ p,
+                        const DevState* st)
+{
+    if (st->done) return;
+    const int32_t i = blockIdx.x*FH_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    if (st->nIterations == 0) p[i] = w[i];
+    else
+    {
+        const T beta = (T)st->beta;
+        p[i] = w[i] + beta*p[i];
+    }
+}
+
+// y = A x fused with sum0 = sum y*z, sum1 = sum y*v (v may be null)
+template<class T>
+__global__ void k_spmv_dot(RowsView R, const T* __restrict__ diag, const T* __restrict__ val,
+                           const T* __restrict__ x, T* __restrict__ y,
+                           const T* __restrict__ z, const T* __restrict__ v,
+                           double* partials, const DevState* st)
+{
+    if (st->done) return;
+    const int32_t r = blockIdx.x*FH_BLOCK + threadIdx.x;
+    double s0 = 0.0, s1 = 0.0;
+    if (r < R.n)
+    {
+        const int32_t nEnt = R.rowInfo[r] >> 16;
+        const int32_t base = sell_base2(R.sliceStart, r);
+        T acc = diag[r]*x[r];
+        for (int32_t k = 0; k < nEnt; k++)
+        {
+            const int32_t e = base + (k << 6);
+            acc += val[e]*x[R.col[e]];
+        }
+        y[r] = acc;
+        s0 = (double)acc*(double)z[r];
+        if (v) s1 = (double)acc*(double)v[r];
+    }
+    write_partials(s0, s1, partials, gridDim.x);
+}
+
+// psi += alpha*pA; rA -= alpha*wA; sum0 = sum |rA|
+template<class T>
+__global__ void k_pcg_update(int32_t n, T* __restrict__ psi, T* __restrict__ r,
+                             const T* __restrict__ p, const T* __restrict__ w,
+                             double* partials, const DevState* st)
+{
+    if (st->done) return;
+    const int32_t i = blockIdx.x*FH_BLOCK + threadIdx.x;
+    double s0 = 0.0;
+    if (i < n)
+    {
+        const T alpha = (T)st->alpha;
+        psi[i] += alpha*p[i];
+        const T ri = r[i] - alpha*w[i];
+        r[i] = ri;
+        s0 = fabs((double)ri);
+    }
+    write_partials(s0, 0.0, partials, gridDim.x);
+}
+
+// ---- PBiCGStab (PBiCGStab.C:149-248) --------------------------------------
+
+// pA = rA (first) or rA + beta*(pA - omega*AyA)
+template<class T>
+__global__ void k_bicg_p(int32_t n, const T* __restrict__ r, T* __restrict__ p,
+                         const T* __restrict__ AyA, const DevState* st)
+{
+    if (st->done) return;
+    const int32_t i = blockIdx.x*FH_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    if (st->nIterations == 0) p[i] = r[i];
+    else
+    {
+        const T beta = (T)st->bicgBeta, omega = (T)st->omega;
+        p[i] = r[i] + beta*(p[i] - omega*AyA[i]);
+    }
+}
+
+// sA = rA - alpha*AyA; sum0 = sum |sA|
+template<class T>
+__global__ void k_bicg_s(int32_t n, const T* __restrict__ r, const T* __restrict__ AyA,
+                         T* __restrict__ s, double* partials, const DevState* st)
+{
+    if (st->done) return;
+    const int32_t i = blockIdx.x*FH_BLOCK + threadIdx.x;
+    double s0 = 0.0;
+    if (i < n)
+    {
+        const T alpha = (T)st->bicgAlpha;
+        const T si = r[i] - alpha*AyA[i];
+        s[i] = si;
+        s0 = fabs((double)si);
+    }
+    write_partials(s0, 0.0, partials, gridDim.x);
+}
+
+// early exit of PBiCGStab.C:204-219: psi += alpha*yA, applied only by the
+// launch of the iteration that converged (seq match)
+template<class T>
+__global__ void k_bicg_early(int32_t n, T* __restrict__ psi, const T* __restrict__ y,
+                             const DevState* st, int32_t seq)
+{
+    if (st->earlySeq != seq) return;
+    const int32_t i = blockIdx.x*FH_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const T alpha = (T)st->bicgAlpha;
+    psi[i] += alpha*y[i];
+}
+
+// psi += alpha*yA + omega*zA; rA = sA - omega*tA; sum0 = |rA|, sum1 = rA0.rA
+template<class T>
+__global__ void k_bicg_update(int32_t n, T* __restrict__ psi, const T* __restrict__ y,
+                              const T* __restrict__ z, T* __restrict__ r,
+                              const T* __restrict__ s, const T* __restrict__ t,
+                              const T* __restrict__ r0, double* partials, const DevState* st)
+{
+    if (st->done) return;
+    const int32_t i = blockIdx.x*FH_BLOCK + threadIdx.x;
+    double s0 = 0.0, s1 = 0.0;
+    if (i < n)
+    {
+        const T alpha = (T)st->bicgAlpha, omega = (T)st->omega;
+        psi[i] += alpha*y[i] + omega*z[i];
+        const T ri = s[i] - omega*t[i];
+        r[i] = ri;
+        s0 = fabs((double)ri);
+        s1 = (double)r0[i]*(double)ri;
+    }
+    write_partials(s0, s1, partials, gridDim.x);
+}
+
+// ---- finalize: fixed-order sum of the block partials, then scalar logic ---
+
+__device__ bool dev_converged(DevState* st)
+{
+    // SolverPerformance.C:59-92
+    const bool c = st->finalResidual < st->tolerance
+                || (st->relTol > 1e-20 && st->finalResidual < st->relTol*st->initialResidual);
+    st->converged = c ? 1 : 0;
+    return c;
+}
+
+__device__ bool dev_singular(DevState* st, double v)
+{
+    st->singular = (v < DBL_MIN) ? 1 : 0;   // SolverPerformance.C:31-44
+    return st->singular;
+}
+
+__global__ void k_finalize(int32_t nBlocks, const double* partials, int32_t op,
+                           DevState* st, int32_t n, int32_t seq)
+{
+    if (op != FIN_SUM && op != FIN_XREF && op != FIN_INIT && op != FIN_INNER_INIT
+        && op != FIN_OUTER_RES && st->done) return;
+    __shared__ double smem[2*FH_BLOCK/64];
+    double a = 0.0, b = 0.0;
+    for (int32_t i = threadIdx.x; i < nBlocks; i += FH_BLOCK)
+    {
+        a += partials[i];
+        b += partials[nBlocks + i];
+    }
+    block_sum2(a, b, smem);
+    if (threadIdx.x != 0) return;
+    switch (op)
+    {
+        case FIN_SUM:
+            st->sum0 = a;
+            st->sum1 = b;
+            break;
+        case FIN_XREF:     // gAverage(psi) (FieldReductionFunctions.C:334-357)
+            st->sum0 = a/n;
+            break;
+        case FIN_INIT:     // normFactor, initial residual, first convergence test
+        {
+            st->normFactor = a + 1e-20;
+            st->initialResidual = b/st->normFactor;
+            st->finalResidual = st->initialResidual;
+            st->nIterations = 0;
+            st->singular = 0;
+            st->converged = 0;
+            st->earlySeq = -1;
+            if (!isfinite(st->initialResidual)) { st->numericError = 1; st->done = 1; break; }
+            if (st->minIter > 0) st->done = 0;
+            else st->done = dev_converged(st) ? 1 : 0;
+            break;
+        }
+        case FIN_PCG_WARA:
+            st->wArAold = st->wArA;
+            st->wArA = a;
+            if (st->nIterations > 0) st->beta = st->wArA/st->wArAold;
+            break;
+        case FIN_PCG_WAPA:
+            st->wApA = a;
+            if (dev_singular(st, fabs(a)/st->normFactor)) { st->done = 1; break; }
+            st->alpha = st->wArA/st->wApA;
+            break;
+        case FIN_PCG_RES:
+        {
+            st->finalResidual = a/st->normFactor;
+            if (!isfinite(st->finalResidual)) { st->numericError = 1; st->done = 1; break; }
+            st->nIterations++;
+            bool cont = false;
+            if (st->nIterations < st->maxIter) cont = !dev_converged(st);
+            cont = cont || st->nIterations < st->minIter;
+            if (!cont) st->done = 1;
+            break;
+        }
+        case FIN_BICG_START:
+        {
+            // first iteration: rA0rA = rA0 & rA
+            st->rA0rAold = st->rA0rA;
+            st->rA0rA = a;
+            if (dev_singular(st, fabs(st->rA0rA))) { st->done = 1; break; }
+            break;
+        }
+        case FIN_BICG_ALPHA:
+            st->rA0AyA = a;
+            st->bicgAlpha = st->rA0rA/st->rA0AyA;
+            break;
+        case FIN_BICG_S:
+        {
+            st->finalResidual = a/st->normFactor;
+            if (!isfinite(st->finalResidual)) { st->numericError = 1; st->done = 1; break; }
+            st->nIterations++;
+            if (st->nIterations >= st->minIter && dev_converged(st))
+            {
+                st->earlySeq = seq;
+                st->done = 1;
+            }
+            break;
+        }
+        case FIN_BICG_OMEGA:
+            st->tAtA = a;
+            st->tAsA = b;
+            st->omega = st->tAsA/st->tAtA;
+            break;
+        case FIN_BICG_RES:
+        {
+            st->finalResidual = a/st->normFactor;
+            if (!isfinite(st->finalResidual)) { st->numericError = 1; st->done = 1; break; }
+            bool cont = false;
+            if (st->nIterations < st->maxIter) cont = !dev_converged(st);
+            cont = cont || st->nIterations < st->minIter;
+            if (!cont) { st->done = 1; break; }
+            // start of the next iteration (PBiCGStab.C:149-185)
+            st->rA0rAold = st->rA0rA;
+            st->rA0rA = b;
+            if (dev_singular(st, fabs(st->rA0rA))) { st->done = 1; break; }
+            if (dev_singular(st, fabs(st->omega))) { st->done = 1; break; }
+            st->bicgBeta = (st->rA0rA/st->rA0rAold)*(st->bicgAlpha/st->omega);
+            break;
+        }
+        case FIN_INNER_INIT:
+        {
+            // inner FP32 solve of the mixed path: relative reduction of
+            // sum |r| only (tolerance 0), normFactor 1
+            st->normFactor = 1.0;
+            st->initialResidual = b;
+            st->finalResidual = b;
+            st->nIterations = 0;
+            st->singular = 0;
+            st->converged = 0;
+            st->earlySeq = -1;
+            st->done = (b > 0.0 && isfinite(b)) ? 0 : 1;
+            break;
+        }
+        case FIN_OUTER_RES:
+        {
+            st->finalResidual = b/st->normFactor;
+            if (!isfinite(st->finalResidual)) { st->numericError = 1; st->done = 1; break; }
+            st->done = dev_converged(st) ? 1 : 0;
+            break;
+        }
+        case FIN_SMOOTH_RES:
+        {
+            // smoothSolver.C:156-189: nIterations += nSweeps before the test
+            st->finalResidual = b/st->normFactor;
+            if (!isfinite(st->finalResidual)) { st->numericError = 1; st->done = 1; break; }
+            st->nIterations += seq;
+            bool cont = false;
+            if (st->nIterations < st->maxIter) cont = !dev_converged(st);
+            cont = cont || st->nIterations < st->minIter;
+            if (!cont) st->done = 1;
+            break;
+        }
+        default:
+            break;
+    }
+}
+
+__global__ void k_axpy_inplace(int32_t n, double* __restrict__ psi,
+                               const double* __restrict__ w, const DevState* st)
+{
+    if (st && st->done) return;
+    const int32_t i = blockIdx.x*FH_BLOCK + threadIdx.x;
+    if (i < n) psi[i] += w[i];
+}
+
+template<class T>
+__global__ void k_sum_abs_state(int32_t n, const T* __restrict__ a, double* partials,
+                                const DevState* st)
+{
+    if (st && st->done) return;
+    const int32_t i = blockIdx.x*FH_BLOCK + threadIdx.x;
+    double s = 0.0, m = 0.0;
+    if (i < n) { s = (double)a[i]; m = fabs((double)a[i]); }
+    write_partials(s, m, partials, gridDim.x);
+}
+
+__global__ void k_scale_convert(int32_t n, const double* __restrict__ x, double scale,
+                                float* __restrict__ y)
+{
+    const int32_t i = blockIdx.x*FH_BLOCK + threadIdx.x;
+    if (i < n) y[i] = (float)(x[i]*scale);
+}
+
+__global__ void k_axpy_f32(int32_t n, double* __restrict__ psi, const float* __restrict__ d,
+                           double scale)
+{
+    const int32_t i = blockIdx.x*FH_BLOCK + threadIdx.x;
+    if (i < n) psi[i] += scale*(double)d[i];
+}
+
+// diagonalSolver.C:70: psi = source/diag
+__global__ void k_divide(int32_t n, const double* __restrict__ src,
+                         const double* __restrict__ diag, double* __restrict__ psi)
+{
+    const int32_t i = blockIdx.x*FH_BLOCK + threadIdx.x;
+    if (i < n) psi[i] = src[i]/diag[i];
+}
+
+// ---- launchers ---------------------------------------------------------------
+
+void launch_axpy_inplace(int32_t n, double* psi, const double* w, const DevState* st,
+                         hipStream_t s)
+{
+    hipLaunchKernelGGL(k_axpy_inplace, dim3(reduction_blocks(n)), dim3(FH_BLOCK), 0, s,
+                       n, psi, w, st);
+}
+
+template<class T>
+void launch_sum_abs_state(int32_t n, const T* a, double* partials, const DevState* st,
+                          hipStream_t s)
+{
+    hipLaunchKernelGGL(k_sum_abs_state<T>, dim3(reduction_blocks(n)), dim3(FH_BLOCK), 0, s,
+                       n, a, partials, st);
+}
+template void launch_sum_abs_state<double>(int32_t, const double*, double*, const DevState*, hipStream_t);
+template void launch_sum_abs_state<float>(int32_t, const float*, double*, const DevState*, hipStream_t);
+
+void launch_finalize_smooth(int32_t n, const double* partials, DevState* st,
+                            int32_t nSweeps, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(FH_BLOCK), 0, s,
+                       reduction_blocks(n), partials, (int32_t)FIN_SMOOTH_RES, st, n, nSweeps);
+}
+
+void launch_scale_convert(int32_t n, const double* x, double scale, float* y, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_scale_convert, dim3(reduction_blocks(n)), dim3(FH_BLOCK), 0, s,
+                       n, x, scale, y);
+}
+
+void launch_axpy_f32(int32_t n, double* psi, const float* d, double scale, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_axpy_f32, dim3(reduction_blocks(n)), dim3(FH_BLOCK), 0, s,
+                       n, psi, d, scale);
+}
+
+void launch_divide(int32_t n, const double* src, const double* diag, double* psi,
+                   hipStream_t s)
+{
+    hipLaunchKernelGGL(k_divide, dim3(reduction_blocks(n)), dim3(FH_BLOCK), 0, s,
+                       n, src, diag, psi);
+}
+
+static inline dim3 grid_for(int32_t n) { return dim3(reduction_blocks(n)); }
+
+void launch_finalize(int32_t n, const double* partials, int32_t op, DevState* st,
+                     int32_t seq, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(FH_BLOCK), 0, s,
+                       reduction_blocks(n), partials, op, st, n, seq);
+}
+
+template<class Ta, class Tb>
+void launch_dot(int32_t n, const Ta* a, const Tb* b, double* partials, const DevState* st,
+                hipStream_t s)
+{
+    hipLaunchKernelGGL((k_dot<Ta, Tb>), grid_for(n), dim3(FH_BLOCK), 0, s, n, a, b, partials, st);
+}
+
+template<class T>
+void launch_sum_abs(int32_t n, const T* a, double* partials, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_sum_abs<T>, grid_for(n), dim3(FH_BLOCK), 0, s, n, a, partials);
+}
+
+void launch_init_residual(int32_t n, const double* Apsi, const double* source,
+                          const double* sumA, double* rA, double* partials,
+                          const DevState* st, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_init_residual, grid_for(n), dim3(FH_BLOCK), 0, s,
+                       n, Apsi, source, sumA, rA, partials, st);
+}
+
+template<class T>
+void launch_precond_point(int32_t n, int32_t kind, const T* rD, const T* r, T* w,
+                          double* partials, const DevState* st, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_precond_point<T>, grid_for(n), dim3(FH_BLOCK), 0, s,
+                       n, kind, rD, r, w, partials, st);
+}
+
+template<class T>
+void launch_pcg_p(int32_t n, const T* w, T* p, const DevState* st, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pcg_p<T>, grid_for(n), dim3(FH_BLOCK), 0, s, n, w, p, st);
+}
+
+template<class T>
+void launch_spmv_dot(const RowsView& R, const T* diag, const T* val, const T* x, T* y,
+                     const T* z, const T* v, double* partials, const DevState* st,
+                     hipStream_t s)
+{
+    hipLaunchKernelGGL(k_spmv_dot<T>, grid_for(R.n), dim3(FH_BLOCK), 0, s,
+                       R, diag, val, x, y, z, v, partials, st);
+}
+
+template<class T>
+void launch_pcg_update(int32_t n, T* psi, T* r, const T* p, const T* w, double* partials,
+                       const DevState* st, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pcg_update<T>, grid_for(n), dim3(FH_BLOCK), 0, s,
+                       n, psi, r, p, w, partials, st);
+}
+
+// the same update without a solver state (measurement)
+template<class T>
+__global__ void k_pcg_update_nostate(int32_t n, T* __restrict__ psi, T* __restrict__ r,
+                                     const T* __restrict__ p, const T* __restrict__ w,
+                                     double* partials)
+{
+    const int32_t i = blockIdx.x*FH_BLOCK + threadIdx.x;
+    double s0 = 0.0;
+    if (i < n)
+    {
+        const T alpha = T(1e-3);
+        psi[i] += alpha*p[i];
+        const T ri = r[i] - alpha*w[i];
+        r[i] = ri;
+        s0 = fabs((double)ri);
+    }
+    write_partials(s0, 0.0, partials, gridDim.x);
+}
+
+template<class T>
+void launch_pcg_update_nostate(int32_t n, T* psi, T* r, const T* p, const T* w,
+                               double* partials, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pcg_update_nostate<T>, grid_for(n), dim3(FH_BLOCK), 0, s,
+                       n, psi, r, p, w, partials);
+}
+template void launch_pcg_update_nostate<double>(int32_t, double*, double*, const double*, const double*, double*, hipStream_t);
+template void launch_pcg_update_nostate<float>(int32_t, float*, float*, const float*, const float*, double*, hipStream_t);
+
+template<class T>
+void launch_bicg_p(int32_t n, const T* r, T* p, const T* AyA, const DevState* st,
+                   hipStream_t s)
+{
+    hipLaunchKernelGGL(k_bicg_p<T>, grid_for(n), dim3(FH_BLOCK), 0, s, n, r, p, AyA, st);
+}
+
+template<class T>
+void launch_bicg_s(int32_t n, const T* r, const T* AyA, T* sA, double* partials,
+                   const DevState* st, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_bicg_s<T>, grid_for(n), dim3(FH_BLOCK), 0, s, n, r, AyA, sA,
+                       partials, st);
+}
+
+template<class T>
+void launch_bicg_early(int32_t n, T* psi, const T* y, const DevState* st, int32_t seq,
+                       hipStream_t s)
+{
+    hipLaunchKernelGGL(k_bicg_early<T>, grid_for(n), dim3(FH_BLOCK), 0, s, n, psi, y, st, seq);
+}
+
+template<class T>
+void launch_bicg_update(int32_t n, T* psi, const T* y, const T* z, T* r, const T* sA,
+                        const T* tA, const T* r0, double* partials, const DevState* st,
+                        hipStream_t s)
+{
+    hipLaunchKernelGGL(k_bicg_update<T>, grid_for(n), dim3(FH_BLOCK), 0, s,
+                       n, psi, y, z, r, sA, tA, r0, partials, st);
+}
+
+#define FH_INST(T)                                                                   \
+    template void launch_dot<T, T>(int32_t, const T*, const T*, double*, const DevState*, hipStream_t); \
+    template void launch_sum_abs<T>(int32_t, const T*, double*, hipStream_t);        \
+    template void launch_precond_point<T>(int32_t, int32_t, const T*, const T*, T*, double*, const DevState*, hipStream_t); \
+    template void launch_pcg_p<T>(int32_t, const T*, T*, const DevState*, hipStream_t); \
+    template void launch_spmv_dot<T>(const RowsView&, const T*, const T*, const T*, T*, const T*, const T*, double*, const DevState*, hipStream_t); \
+    template void launch_pcg_update<T>(int32_t, T*, T*, const T*, const T*, double*, const DevState*, hipStream_t); \
+    template void launch_bicg_p<T>(int32_t, const T*, T*, const T*, const DevState*, hipStream_t); \
+    template void launch_bicg_s<T>(int32_t, const T*, const T*, T*, double*, const DevState*, hipStream_t); \
+    template void launch_bicg_early<T>(int32_t, T*, const T*, const DevState*, int32_t, hipStream_t); \
+    template void launch_bicg_update<T>(int32_t, T*, const T*, const T*, T*, const T*, const T*, const T*, double*, const DevState*, hipStream_t);
+
+FH_INST(double)
+FH_INST(float)
+
+} // namespace fh

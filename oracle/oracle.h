@@ -1,0 +1,226 @@
+/*
+ * oracle.h -- CPU restatement of OpenFOAM-dev's lduMatrix linear-solver path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP
+ * implementation is compared against.  Only tests/, __graft_entry__.smoke()
+ * and the cpu_baseline leg of bench.py may load it; the product path never
+ * links or calls it.
+ *
+ * Every routine follows the loop order of the reference function it names
+ * (citations are relative to the OpenFOAM-dev source tree), so that results
+ * are bitwise reproducible in plain FP64 with no FMA contraction
+ * (compile with -O3 -ffp-contract=off, no -ffast-math).
+ *
+ * Scope: serial (single-rank) solves without coupled interfaces, i.e. the
+ * interfaceBouCoeffs / interfaces lists are empty.  That is the case for
+ * every synthetic configuration this build measures.
+ */
+#ifndef FOAMHIP_ORACLE_H
+#define FOAMHIP_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- LDU addressing (lduAddressing.H:112-207, lduAddressing.C:32-170) ---- */
+typedef struct
+{
+    int32_t nCells;
+    int32_t nFaces;
+    const int32_t* lower;   /* owner     (lowerAddr) [nFaces] */
+    const int32_t* upper;   /* neighbour (upperAddr) [nFaces] */
+    int32_t* losort;        /* [nFaces]   */
+    int32_t* ownerStart;    /* [nCells+1] */
+    int32_t* losortStart;   /* [nCells+1] */
+} or_addr;
+
+int  or_addr_init(or_addr* a, int32_t nCells, int32_t nFaces,
+                  const int32_t* lower, const int32_t* upper);
+void or_addr_free(or_addr* a);
+
+/* Matrix view: lower == upper (same pointer) for a symmetric matrix,
+   as lduMatrix::lower() const returns upper when lowerPtr_ is unset
+   (lduMatrix.C:168-312). */
+typedef struct
+{
+    const or_addr* addr;
+    const double* diag;
+    const double* upper;
+    const double* lower;
+} or_matrix;
+
+/* ---- reductions (FieldReductionFunctions.C) ---- */
+double or_sum(int32_t n, const double* f);
+double or_sumMag(int32_t n, const double* f);
+double or_sumSqr(int32_t n, const double* f);
+double or_sumProd(int32_t n, const double* a, const double* b);
+double or_average(int32_t n, const double* f);
+
+/* ---- matrix kernels (lduMatrixATmul.C, lduMatrixOperations.C) ---- */
+void or_amul(const or_matrix* m, const double* psi, double* Apsi);
+void or_tmul(const or_matrix* m, const double* psi, double* Tpsi);
+void or_sumA(const or_matrix* m, double* sumA);
+void or_residual(const or_matrix* m, const double* psi, const double* source,
+                 double* rA);
+double or_normFactor(const or_matrix* m, const double* psi,
+                     const double* source, const double* Apsi, double* tmp);
+void or_negSumDiag(const or_addr* a, const double* lower, const double* upper,
+                   double* diag);
+
+/* ---- selection ids (names as in the reference run-time tables) ---- */
+enum { OR_SOLVER_PCG = 0, OR_SOLVER_PBICGSTAB = 1, OR_SOLVER_PBICG = 2,
+       OR_SOLVER_SMOOTH = 3, OR_SOLVER_GAMG = 4, OR_SOLVER_DIAGONAL = 5 };
+enum { OR_PRECON_NONE = 0, OR_PRECON_DIAGONAL = 1, OR_PRECON_DIC = 2,
+       OR_PRECON_FDIC = 3, OR_PRECON_DILU = 4, OR_PRECON_GAMG = 5 };
+enum { OR_SMOOTH_GS = 0, OR_SMOOTH_SYMGS = 1, OR_SMOOTH_DIC = 2,
+       OR_SMOOTH_DILU = 3, OR_SMOOTH_DICGS = 4, OR_SMOOTH_DILUGS = 5,
+       OR_SMOOTH_FDIC = 6, OR_SMOOTH_NONE = 7 };
+
+/* ---- preconditioners ---- */
+/* rD setup: DICPreconditioner.C:57-84, DILUPreconditioner.C:57-85,
+   FDICPreconditioner.C:46-83, diagonalPreconditioner.C:50-63 */
+void or_calcReciprocalD_DIC(const or_matrix* m, double* rD);
+void or_calcReciprocalD_DILU(const or_matrix* m, double* rD);
+/* apply: w = M^-1 r */
+void or_precondition(const or_matrix* m, int precon, const double* rD,
+                     const double* r, double* w);
+void or_preconditionT(const or_matrix* m, int precon, const double* rD,
+                      const double* r, double* w);
+
+/* ---- smoothers (lduMatrix/smoothers) ---- */
+void or_smooth(const or_matrix* m, int smoother, const double* rD,
+               double* psi, const double* source, int32_t nSweeps);
+
+/* ---- solver controls / performance (lduMatrixSolver.C:158-164,
+        SolverPerformance.H:78-268) ---- */
+typedef struct
+{
+    int32_t solver;
+    int32_t preconditioner;
+    int32_t smoother;
+    int32_t maxIter;       /* default 1000 */
+    int32_t minIter;       /* default 0 */
+    double  tolerance;     /* default 1e-6 */
+    double  relTol;        /* default 0 */
+    int32_t nSweeps;       /* smoothSolver, default 1 */
+    /* GAMG (GAMGSolver.C:70-81, GAMGAgglomeration.C:248-257) */
+    int32_t nPreSweeps, preSweepsLevelMultiplier, maxPreSweeps;
+    int32_t nPostSweeps, postSweepsLevelMultiplier, maxPostSweeps;
+    int32_t nFinestSweeps;
+    int32_t scaleCorrection;      /* -1 = default (matrix.symmetric()) */
+    int32_t directSolveCoarsest;
+    int32_t nCellsInCoarsestLevel;
+    int32_t mergeLevels;
+    int32_t nVcycles;             /* GAMG preconditioner, default 2 */
+} or_controls;
+
+void or_controls_default(or_controls* c);
+
+typedef struct
+{
+    double  initialResidual;
+    double  finalResidual;
+    int32_t nIterations;
+    int32_t converged;
+    int32_t singular;
+    double  normFactor;
+} or_perf;
+
+/* Solve A psi = source in place.  history (optional, may be NULL) receives
+   finalResidual after every iteration, history[0] = initial residual;
+   it must hold maxIter+2 entries. */
+int or_solve(const or_matrix* m, const or_controls* c, double* psi,
+             const double* source, or_perf* perf, double* history);
+
+/* pairGAMGAgglomeration::forward_ is a process-global toggle
+   (pairGAMGAgglomeration.C:33); this resets it to true. */
+void or_gamg_reset_direction(void);
+int  or_gamg_get_direction(void);
+
+/* ---- GAMG building blocks exposed for the tests ---- */
+/* pairGAMGAgglomerate.C:123-301; returns nCoarse, fills restrictMap[nCells] */
+int32_t or_pair_agglomerate(const or_addr* a, const double* faceWeights,
+                            int32_t* restrictMap);
+
+/* Full hierarchy (pairGAMGAgglomerate.C:31-118 + GAMGAgglomerateLduAddressing.C)
+   nLevels = number of coarse levels.  Arrays owned by the struct. */
+typedef struct
+{
+    int32_t  nLevels;
+    int32_t* nCells;          /* [nLevels+1]: nCells[0] = finest */
+    int32_t* nFaces;          /* [nLevels+1] */
+    int32_t** lower;          /* [nLevels+1] addressing per level (level 0 borrowed) */
+    int32_t** upper;
+    int32_t** restrictMap;    /* [nLevels]: fine cell -> coarse cell */
+    int32_t** faceRestrict;   /* [nLevels]: fine face -> coarse face or -1-cell */
+    uint8_t** faceFlip;       /* [nLevels] */
+} or_hierarchy;
+
+int  or_gamg_agglomerate(const or_matrix* m, const or_controls* c,
+                         or_hierarchy* h);
+void or_hierarchy_free(or_hierarchy* h);
+
+/* GAMGSolverAgglomerateMatrix.C:33-193, one level */
+void or_agglomerate_matrix(int32_t nFineFaces, int32_t nCoarseCells,
+                           int32_t nCoarseFaces, const int32_t* restrictMap,
+                           int32_t nFineCells,
+                           const int32_t* faceRestrict, const uint8_t* faceFlip,
+                           const double* fineDiag, const double* fineUpper,
+                           const double* fineLower /* NULL if symmetric */,
+                           double* coarseDiag, double* coarseUpper,
+                           double* coarseLower /* NULL if symmetric */);
+
+/* scalarMatrices.C:77-180 and scalarMatricesTemplates.C:117-162 */
+int  or_LUDecompose(int32_t n, double* A, int32_t* pivot);
+void or_LUBacksubstitute(int32_t n, const double* LU, const int32_t* pivot,
+                         double* x);
+
+/* ---- finite-volume assembly (face loops feeding the solver) ---- */
+/* gaussLaplacianScheme.C:45-89 + gaussLaplacianSchemes.C:43-56,
+   surfaceInterpolationScheme.C:305-314 (linear interpolate),
+   fixedValueFvPatchField.C:108-119, zeroGradientFvPatchField.C:138-155.
+   Boundary faces are given flattened: bFaceCells, bMagSf, bDeltaCoeffs,
+   bGamma, bType (0 zeroGradient, 1 fixedValue), bValue.  Outputs upper[F],
+   diag[N] (negSumDiag), internalCoeffs[nB], boundaryCoeffs[nB]. */
+void or_laplacian_assemble(const or_addr* a, const double* gammaCell,
+                           const double* weights, const double* magSf,
+                           const double* nonOrthDeltaCoeffs,
+                           int32_t nB, const int32_t* bFaceCells,
+                           const double* bMagSf, const double* bDeltaCoeffs,
+                           const int32_t* bType, const double* bValue,
+                           double* upper, double* diag,
+                           double* internalCoeffs, double* boundaryCoeffs);
+
+/* gaussConvectionScheme.C:119-170 with upwind weights pos0(phi)
+   (upwind.H:116-128); outputs lower, upper, diag (negSumDiag) and patch
+   coefficients. */
+void or_upwind_assemble(const or_addr* a, const double* phi,
+                        int32_t nB, const int32_t* bType, const double* bPhi,
+                        const double* bValue,
+                        double* lower, double* upper, double* diag,
+                        double* internalCoeffs, double* boundaryCoeffs);
+
+/* fvMatrix.C:112-175: diag += internalCoeffs, source += boundaryCoeffs */
+void or_add_boundary(int32_t nB, const int32_t* bFaceCells,
+                     const double* internalCoeffs, const double* boundaryCoeffs,
+                     double* diag, double* source);
+
+/* gaussGrad.C:30-75 with linear interpolation and
+   gradScheme.C:123-148 boundary correction (the boundary gradient values
+   are written to bGrad[nB*3]).  bPhi = boundary values of the field,
+   bSnGrad = patch snGrad (0 for zeroGradient). Sf, bSf are [.*3]. */
+void or_grad_gauss_linear(const or_addr* a, const double* phi,
+                          const double* weights, const double* Sf,
+                          const double* V,
+                          int32_t nB, const int32_t* bFaceCells,
+                          const double* bSf, const double* bMagSf,
+                          const double* bPhi, const double* bSnGrad,
+                          double* grad, double* bGrad);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

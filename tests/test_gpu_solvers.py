@@ -1,0 +1,168 @@
+"""GPU solves against the FP64 oracle on the same matrix and right-hand
+side.  Bar (north star): solution relative L2 difference <= 1e-10,
+iteration count within +-2, the same residual definition (normFactor) --
+the only difference between the two is the order of the global sums.
+Mixed precision: error vs the FP64 oracle solution <= MIXED_TOL with the
+FP64 residual test met."""
+import numpy as np
+import pytest
+
+import foamhip as fh
+import oracle as O
+from foamhip.cases import BoxCase
+
+pytestmark = pytest.mark.gpu
+
+SOLUTION_TOL = 1e-10      # relative L2, FP64 path
+ITER_TOL = 2
+MIXED_TOL = 1e-8          # relative L2 vs the FP64 oracle, mixed path
+
+
+def laplacian_case(n, seed=1, sigma=None):
+    """Config-1 style: fvm::laplacian(gamma, p), x-min wall fixedValue 0."""
+    c = BoxCase(n)
+    gamma = np.ones(c.N) if sigma is None else c.lognormal(seed + 100, sigma, 4.0)
+    a = O.Addr(c.N, c.lower, c.upper)
+    upper, diag, ic, bc = O.laplacian_assemble(a, gamma, c.weights, c.magSf, c.deltaCoeffs,
+                                               c.bFaceCells, c.bMagSf, c.bDeltaCoeffs, c.bType,
+                                               c.bValue)
+    b = c.rhs(seed)
+    diag, b = O.add_boundary(c.bFaceCells, ic, bc, diag, b)
+    return c, a, diag, upper, b
+
+
+def convdiff_case(n, seed=2, D=1e-3):
+    c = BoxCase(n)
+    a = O.Addr(c.N, c.lower, c.upper)
+    phi, bPhi = c.fourier_flux(seed)
+    diag, upper, lower, ic, bc = O.convection_diffusion(a, c, phi, bPhi, D)
+    b = c.rhs(seed)
+    diag, b = O.add_boundary(c.bFaceCells, ic, bc, diag, b)
+    return c, a, diag, upper, lower, b
+
+
+def rel_l2(x, y):
+    return np.linalg.norm(x - y) / np.linalg.norm(y)
+
+
+def compare(perf, opsi, psi, operf, tol=SOLUTION_TOL):
+    assert perf.converged == operf["converged"]
+    assert abs(perf.nIterations - operf["nIterations"]) <= ITER_TOL, (perf, operf)
+    assert perf.normFactor == pytest.approx(operf["normFactor"], rel=1e-12)
+    assert perf.initialResidual == pytest.approx(operf["initialResidual"], rel=1e-12)
+    assert rel_l2(psi, opsi) <= tol, rel_l2(psi, opsi)
+
+
+@pytest.mark.parametrize("precon", ["DIC", "FDIC", "diagonal", "none"])
+def test_pcg_poisson(precon):
+    c, a, diag, upper, b = laplacian_case(24)
+    ctl = {"solver": "PCG", "preconditioner": precon, "tolerance": 1e-6, "relTol": 0}
+    opsi, operf = O.Matrix(a, diag, upper).solve(ctl, np.zeros(c.N), b)
+    mesh = fh.lduAddressing(c.N, c.lower, c.upper)
+    m = fh.lduMatrix(mesh, diag, upper)
+    psi = np.zeros(c.N)
+    perf = fh.solver_New("p", m, ctl).solve(psi, b)
+    compare(perf, opsi, psi, operf)
+    assert perf.solverName == precon + "PCG"
+
+
+def test_pcg_dic_lognormal_tight():
+    c, a, diag, upper, b = laplacian_case(32, sigma=1.0)
+    ctl = {"solver": "PCG", "preconditioner": "DIC", "tolerance": 1e-8, "relTol": 0}
+    opsi, operf = O.Matrix(a, diag, upper).solve(ctl, np.zeros(c.N), b)
+    mesh = fh.lduAddressing(c.N, c.lower, c.upper)
+    m = fh.lduMatrix(mesh, diag, upper)
+    psi = np.zeros(c.N)
+    perf = fh.solver_New("p", m, ctl).solve(psi, b)
+    compare(perf, opsi, psi, operf)
+    # the reported residual is the reference definition, recomputed
+    r = b - O.Matrix(a, diag, upper).Amul(psi)
+    assert np.abs(r).sum() / perf.normFactor == pytest.approx(perf.finalResidual, rel=1e-6)
+
+
+@pytest.mark.parametrize("precon", ["DILU", "diagonal"])
+def test_pbicgstab_convection_diffusion(precon):
+    c, a, diag, upper, lower, b = convdiff_case(20)
+    ctl = {"solver": "PBiCGStab", "preconditioner": precon, "tolerance": 1e-8, "relTol": 0}
+    opsi, operf = O.Matrix(a, diag, upper, lower).solve(ctl, np.zeros(c.N), b)
+    mesh = fh.lduAddressing(c.N, c.lower, c.upper)
+    m = fh.lduMatrix(mesh, diag, upper, lower)
+    psi = np.zeros(c.N)
+    perf = fh.solver_New("T", m, ctl).solve(psi, b)
+    compare(perf, opsi, psi, operf)
+
+
+@pytest.mark.parametrize("smoother,nSweeps", [("symGaussSeidel", 1), ("GaussSeidel", 2),
+                                              ("DIC", 1), ("DICGaussSeidel", 1)])
+def test_smooth_solver(smoother, nSweeps):
+    c, a, diag, upper, b = laplacian_case(12)
+    ctl = {"solver": "smoothSolver", "smoother": smoother, "nSweeps": nSweeps,
+           "tolerance": 1e-5, "relTol": 0, "maxIter": 3000}
+    opsi, operf = O.Matrix(a, diag, upper).solve(ctl, np.zeros(c.N), b)
+    mesh = fh.lduAddressing(c.N, c.lower, c.upper)
+    m = fh.lduMatrix(mesh, diag, upper)
+    psi = np.zeros(c.N)
+    perf = fh.solver_New("p", m, ctl).solve(psi, b)
+    compare(perf, opsi, psi, operf)
+
+
+def test_controls_semantics_on_device():
+    c, a, diag, upper, b = laplacian_case(10)
+    mesh = fh.lduAddressing(c.N, c.lower, c.upper)
+    m = fh.lduMatrix(mesh, diag, upper)
+    om = O.Matrix(a, diag, upper)
+    for ctl in [{"solver": "PCG", "preconditioner": "DIC", "tolerance": 1e-14, "maxIter": 3},
+                {"solver": "PCG", "preconditioner": "DIC", "tolerance": 10.0, "minIter": 2},
+                {"solver": "PCG", "preconditioner": "DIC", "tolerance": 10.0},
+                {"solver": "PCG", "preconditioner": "DIC", "tolerance": 0, "relTol": 1e-2},
+                {"solver": "PBiCGStab", "preconditioner": "diagonal", "tolerance": 10.0,
+                 "minIter": 1}]:
+        opsi, operf = om.solve(ctl, np.zeros(c.N), b)
+        psi = np.zeros(c.N)
+        perf = fh.solver_New("p", m, ctl).solve(psi, b)
+        assert perf.nIterations == operf["nIterations"], ctl
+        assert perf.converged == operf["converged"], ctl
+    # wrong table: DILU on a symmetric matrix, PCG on an asymmetric one
+    with pytest.raises(fh.FoamHipError, match="DILU"):
+        fh.solver_New("p", m, {"solver": "PCG", "preconditioner": "DILU"}).solve(
+            np.zeros(c.N), b)
+    m2 = fh.lduMatrix(mesh, diag, upper, upper * 1.01)
+    with pytest.raises(fh.FoamHipError, match="PCG"):
+        fh.solver_New("p", m2, {"solver": "PCG", "preconditioner": "DIC"}).solve(
+            np.zeros(c.N), b)
+
+
+def test_nan_source_is_an_error():
+    c, a, diag, upper, b = laplacian_case(8)
+    mesh = fh.lduAddressing(c.N, c.lower, c.upper)
+    m = fh.lduMatrix(mesh, diag, upper)
+    b = b.copy()
+    b[3] = np.nan
+    with pytest.raises(fh.FoamHipError) as e:
+        fh.solver_New("p", m, {"solver": "PCG", "preconditioner": "DIC"}).solve(
+            np.zeros(c.N), b)
+    assert e.value.code == 5
+
+
+@pytest.mark.parametrize("solver,precon", [("PCG", "DIC"), ("PBiCGStab", "DILU")])
+def test_mixed_precision_vs_fp64_oracle(solver, precon):
+    if solver == "PCG":
+        c, a, diag, upper, b = laplacian_case(32, sigma=1.0)
+        lower = None
+    else:
+        c, a, diag, upper, lower, b = convdiff_case(24)
+    ctl = {"solver": solver, "preconditioner": precon, "tolerance": 1e-8, "relTol": 0}
+    om = O.Matrix(a, diag, upper, lower)
+    opsi, operf = om.solve(ctl, np.zeros(c.N), b)
+    mesh = fh.lduAddressing(c.N, c.lower, c.upper)
+    m = fh.lduMatrix(mesh, diag, upper, lower)
+    psi = np.zeros(c.N)
+    perf = fh.solver_New("p", m, dict(ctl, precision="mixed")).solve(psi, b)
+    assert perf.converged and perf.finalResidual < 1e-8
+    # the FP64 residual test, recomputed on the host
+    r = b - om.Amul(psi)
+    assert np.abs(r).sum() / perf.normFactor < 1e-8
+    err = rel_l2(psi, opsi)
+    print(f"mixed {solver}: refinements {perf.nRefinements}, inner its {perf.nIterations}, "
+          f"fp64 its {operf['nIterations']}, error vs oracle {err:.3e}")
+    assert err <= MIXED_TOL

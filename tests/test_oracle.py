@@ -1,0 +1,439 @@
+"""Pins the oracle (oracle/oracle.c) against what the mathematics fixes:
+dense linear algebra on the same matrix, closed forms, invariants and
+hand-worked known answers.  The reference ships no golden vectors for this
+path, so these checks are what makes the oracle trustworthy; each is chosen
+so that a dropped term, a swapped upper/lower or a wrong index fails."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from meshes import asym_coeffs, box, dense, random_graph, spd_coeffs
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def mk(n, lo, up, rng, kind="spd"):
+    a = O.Addr(n, lo, up)
+    if kind == "spd":
+        d, u = spd_coeffs(n, lo, up, rng)
+        return a, O.Matrix(a, d, u), dense(n, lo, up, d, u)
+    d, u, l = asym_coeffs(n, lo, up, rng)
+    return a, O.Matrix(a, d, u, l), dense(n, lo, up, d, u, l)
+
+
+MESHES = [
+    ("box", lambda rng: box(4, 3, 5)),
+    ("rand", lambda rng: random_graph(40, 5, rng)),
+    ("rand_shuffled", lambda rng: random_graph(40, 5, rng, shuffle_within_owner=True)),
+]
+
+
+@pytest.mark.parametrize("name,gen", MESHES)
+def test_addressing_definitions(name, gen):
+    rng = np.random.default_rng(1)
+    n, lo, up = gen(rng)
+    a = O.Addr(n, lo, up)
+    # losort: faces grouped by neighbour, stable in face index
+    exp = np.argsort(up, kind="stable")
+    assert np.array_equal(a.losort(), exp)
+    own = a.ownerStart()
+    assert own[0] == 0 and own[-1] == len(lo)
+    for c in range(n):
+        assert np.all(lo[own[c]:own[c + 1]] == c)
+    ls = a.losortStart()
+    for c in range(n):
+        assert np.all(up[exp[ls[c]:ls[c + 1]]] == c)
+
+
+@pytest.mark.parametrize("name,gen", MESHES)
+@pytest.mark.parametrize("kind", ["spd", "asym"])
+def test_matrix_kernels_vs_dense(name, gen, kind):
+    rng = np.random.default_rng(2)
+    n, lo, up = gen(rng)
+    a, m, A = mk(n, lo, up, rng, kind)
+    x = rng.standard_normal(n)
+    b = rng.standard_normal(n)
+    np.testing.assert_allclose(m.Amul(x), A @ x, rtol=1e-13, atol=1e-13)
+    np.testing.assert_allclose(m.Tmul(x), A.T @ x, rtol=1e-13, atol=1e-13)
+    np.testing.assert_allclose(m.residual(x, b), b - A @ x, rtol=1e-13, atol=1e-13)
+    np.testing.assert_allclose(m.sumA(), A.sum(axis=1), rtol=1e-13, atol=1e-13)
+
+
+def test_norm_factor_closed_forms():
+    rng = np.random.default_rng(3)
+    n, lo, up = box(5, 4, 3)
+    a, m, A = mk(n, lo, up, rng)
+    b = rng.standard_normal(n)
+    # psi = 0: normFactor = sum|b| + 1e-20 (lduMatrixSolver.C:174-197)
+    assert m.normFactor(np.zeros(n), b) == pytest.approx(np.abs(b).sum() + 1e-20, rel=1e-14)
+    # psi = c: A psi = c*sumA, xRef = c -> sum|b - c*sumA| + 1e-20
+    c = 0.37
+    exp = np.abs(b - c * A.sum(axis=1)).sum() + 1e-20
+    assert m.normFactor(np.full(n, c), b) == pytest.approx(exp, rel=1e-12)
+
+
+def ic_factors(A, rD, kind):
+    """M = (D~ + L) D~^-1 (D~ + U) with D~ = 1/rD."""
+    Dt = np.diag(1.0 / rD)
+    L = np.tril(A, -1)
+    U = np.triu(A, 1)
+    return (Dt + L) @ np.linalg.inv(Dt) @ (Dt + U)
+
+
+@pytest.mark.parametrize("name,gen", MESHES)
+@pytest.mark.parametrize("precon", ["DIC", "FDIC", "DILU"])
+def test_incomplete_factor_preconditioners(name, gen, precon):
+    rng = np.random.default_rng(4)
+    n, lo, up = gen(rng)
+    a, m, A = mk(n, lo, up, rng, "asym" if precon == "DILU" else "spd")
+    rD = m.reciprocalD(precon)
+    M = ic_factors(A, rD, precon)
+    # defining property of the zero-fill factorisation: diag(M) = diag(A)
+    np.testing.assert_allclose(np.diag(M), np.diag(A), rtol=1e-12)
+    # and M has no entry outside the sparsity of A except the fill
+    r = rng.standard_normal(n)
+    w = m.precondition(precon, r)
+    np.testing.assert_allclose(M @ w, r, rtol=1e-10, atol=1e-10)
+    if precon == "DILU":
+        wT = m.precondition(precon, r, transpose=True)
+        np.testing.assert_allclose(M.T @ wT, r, rtol=1e-10, atol=1e-10)
+
+
+def test_dic_is_exact_cholesky_on_a_chain():
+    rng = np.random.default_rng(5)
+    n, lo, up = box(12, 1, 1)[0], np.arange(11, dtype=np.int32), np.arange(1, 12, dtype=np.int32)
+    a, m, A = mk(n, lo, up, rng)
+    r = rng.standard_normal(n)
+    np.testing.assert_allclose(m.precondition("DIC", r), np.linalg.solve(A, r), rtol=1e-12)
+    psi, perf = m.solve({"solver": "PCG", "preconditioner": "DIC", "tolerance": 1e-12},
+                        np.zeros(n), r)
+    assert perf["nIterations"] == 1 and perf["converged"]
+    # DILU is exact LU on an asymmetric chain: PBiCGStab converges in one step
+    a2, m2, A2 = mk(n, lo, up, rng, "asym")
+    psi, perf = m2.solve({"solver": "PBiCGStab", "preconditioner": "DILU",
+                          "tolerance": 1e-12}, np.zeros(n), r)
+    assert perf["nIterations"] == 1
+    np.testing.assert_allclose(psi, np.linalg.solve(A2, r), rtol=1e-10)
+
+
+def test_diagonal_preconditioner():
+    rng = np.random.default_rng(6)
+    n, lo, up = box(3, 3, 3)
+    a, m, A = mk(n, lo, up, rng)
+    r = rng.standard_normal(n)
+    np.testing.assert_allclose(m.precondition("diagonal", r), r / np.diag(A), rtol=1e-15)
+    np.testing.assert_array_equal(m.precondition("none", r), r)
+
+
+@pytest.mark.parametrize("name,gen", MESHES)
+@pytest.mark.parametrize("kind", ["spd", "asym"])
+def test_gauss_seidel_is_a_triangular_solve(name, gen, kind):
+    rng = np.random.default_rng(7)
+    n, lo, up = gen(rng)
+    a, m, A = mk(n, lo, up, rng, kind)
+    x0 = rng.standard_normal(n)
+    b = rng.standard_normal(n)
+    L = np.tril(A)          # D + L
+    U = np.triu(A, 1)
+    fwd = np.linalg.solve(L, b - U @ x0)
+    np.testing.assert_allclose(m.smooth("GaussSeidel", x0, b, 1), fwd, rtol=1e-11, atol=1e-12)
+    bwd = np.linalg.solve(np.triu(A), b - np.tril(A, -1) @ fwd)
+    np.testing.assert_allclose(m.smooth("symGaussSeidel", x0, b, 1), bwd, rtol=1e-11, atol=1e-12)
+    # two sweeps = the one-sweep map applied twice
+    two = m.smooth("GaussSeidel", m.smooth("GaussSeidel", x0, b, 1), b, 1)
+    np.testing.assert_array_equal(m.smooth("GaussSeidel", x0, b, 2), two)
+
+
+@pytest.mark.parametrize("smoother,kind", [("DIC", "spd"), ("DILU", "asym"), ("FDIC", "spd")])
+def test_ilu_smoothers(smoother, kind):
+    rng = np.random.default_rng(8)
+    n, lo, up = random_graph(30, 4, rng)
+    a, m, A = mk(n, lo, up, rng, kind)
+    x0 = rng.standard_normal(n)
+    b = rng.standard_normal(n)
+    M = ic_factors(A, m.reciprocalD("DILU" if smoother == "DILU" else "DIC"), smoother)
+    exp = x0 + np.linalg.solve(M, b - A @ x0)
+    np.testing.assert_allclose(m.smooth(smoother, x0, b, 1), exp, rtol=1e-10, atol=1e-11)
+
+
+SOLVES = [
+    ({"solver": "PCG", "preconditioner": "DIC"}, "spd"),
+    ({"solver": "PCG", "preconditioner": "FDIC"}, "spd"),
+    ({"solver": "PCG", "preconditioner": "diagonal"}, "spd"),
+    ({"solver": "PCG", "preconditioner": "none"}, "spd"),
+    ({"solver": "PBiCGStab", "preconditioner": "DILU"}, "asym"),
+    ({"solver": "PBiCGStab", "preconditioner": "diagonal"}, "asym"),
+    ({"solver": "PBiCG", "preconditioner": "DILU"}, "asym"),
+    ({"solver": "smoothSolver", "smoother": "symGaussSeidel", "nSweeps": 2}, "spd"),
+    ({"solver": "smoothSolver", "smoother": "GaussSeidel"}, "asym"),
+    ({"solver": "smoothSolver", "smoother": "DICGaussSeidel"}, "spd"),
+    ({"solver": "GAMG", "smoother": "GaussSeidel", "nCellsInCoarsestLevel": 4}, "spd"),
+    ({"solver": "GAMG", "smoother": "symGaussSeidel", "nCellsInCoarsestLevel": 8,
+      "directSolveCoarsest": 1, "mergeLevels": 2}, "spd"),
+    ({"solver": "GAMG", "smoother": "GaussSeidel", "nCellsInCoarsestLevel": 4}, "asym"),
+    ({"solver": "PCG", "preconditioner": "GAMG", "smoother": "DICGaussSeidel",
+      "nCellsInCoarsestLevel": 4}, "spd"),
+]
+
+
+@pytest.mark.parametrize("ctl,kind", SOLVES)
+def test_solvers_converge_to_dense_solution(ctl, kind):
+    rng = np.random.default_rng(9)
+    n, lo, up = box(6, 5, 4)
+    O.reset_direction()
+    a, m, A = mk(n, lo, up, rng, kind)
+    b = rng.standard_normal(n)
+    c = dict(ctl, tolerance=1e-10, relTol=0, maxIter=2000)
+    psi, perf = m.solve(c, np.zeros(n), b)
+    x = np.linalg.solve(A, b)
+    assert perf["converged"], perf
+    assert perf["finalResidual"] < 1e-10
+    # normalised L1 residual recomputed densely
+    assert np.abs(b - A @ psi).sum() / perf["normFactor"] == pytest.approx(
+        perf["finalResidual"], rel=1e-6, abs=1e-14)
+    np.testing.assert_allclose(psi, x, rtol=1e-6, atol=1e-8 * np.abs(x).max())
+
+
+def test_pcg_terminates_within_n_iterations_unpreconditioned():
+    rng = np.random.default_rng(10)
+    n, lo, up = box(2, 2, 2)
+    a, m, A = mk(n, lo, up, rng)
+    b = rng.standard_normal(n)
+    psi, perf = m.solve({"solver": "PCG", "preconditioner": "none", "tolerance": 1e-12},
+                        np.zeros(n), b)
+    assert perf["nIterations"] <= n + 1
+
+
+def test_solver_controls_semantics():
+    rng = np.random.default_rng(11)
+    n, lo, up = box(5, 5, 5)
+    a, m, A = mk(n, lo, up, rng)
+    b = rng.standard_normal(n)
+    # maxIter caps; converged stays false
+    psi, p = m.solve({"solver": "PCG", "preconditioner": "DIC", "tolerance": 1e-14,
+                      "maxIter": 3}, np.zeros(n), b)
+    assert p["nIterations"] == 3 and not p["converged"]
+    # minIter forces iterations even when converged initially
+    psi, p = m.solve({"solver": "PCG", "preconditioner": "DIC", "tolerance": 10.0,
+                      "minIter": 2}, np.zeros(n), b)
+    assert p["nIterations"] == 2
+    # already converged: no iteration
+    psi, p = m.solve({"solver": "PCG", "preconditioner": "DIC", "tolerance": 10.0},
+                     np.zeros(n), b)
+    assert p["nIterations"] == 0 and p["converged"] and p["initialResidual"] == pytest.approx(1.0)
+    # relTol
+    psi, p = m.solve({"solver": "PCG", "preconditioner": "DIC", "tolerance": 0,
+                      "relTol": 1e-2}, np.zeros(n), b)
+    assert p["finalResidual"] < 1e-2 * p["initialResidual"] and p["converged"]
+    # singular: zero right-hand side with zero initial guess -> initial
+    # residual 0 converges; a zero matrix row is singular
+    psi, p = m.solve({"solver": "PCG", "preconditioner": "none", "tolerance": 0,
+                      "minIter": 1}, np.zeros(n), np.zeros(n))
+    assert p["singular"]
+
+
+def load_kat():
+    with open(os.path.join(GOLDEN, "pair_agglomeration_kat.json")) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("case", range(3))
+def test_pair_agglomeration_hand_worked(case):
+    """Hand-worked known answers of the greedy pairing
+    (pairGAMGAgglomerate.C:123-301); see the fixture for the derivation."""
+    kat = load_kat()[case]
+    a = O.Addr(kat["nCells"], kat["lower"], kat["upper"])
+    O.reset_direction()
+    for step in kat["steps"]:
+        assert O.get_direction() == (1 if step["forward"] else 0)
+        n, cmap = O.pair_agglomerate(a, kat["weights"])
+        assert n == step["nCoarse"]
+        assert list(cmap) == step["map"]
+
+
+def test_pair_agglomeration_invariants():
+    rng = np.random.default_rng(12)
+    n, lo, up = random_graph(200, 5, rng)
+    a = O.Addr(n, lo, up)
+    w = rng.uniform(0.1, 1, len(lo))
+    for _ in range(2):
+        nc, cmap = O.pair_agglomerate(a, w)
+        assert cmap.min() == 0 and cmap.max() == nc - 1
+        assert len(np.unique(cmap)) == nc
+        assert nc < n
+        # clusters are connected through faces of the fine graph
+        for c in range(nc):
+            members = np.where(cmap == c)[0]
+            if len(members) == 1:
+                continue
+            inside = (cmap[lo] == c) & (cmap[up] == c)
+            adj = {m: set() for m in members}
+            for l, u in zip(lo[inside], up[inside]):
+                adj[l].add(u); adj[u].add(l)
+            seen, stack = {members[0]}, [members[0]]
+            while stack:
+                v = stack.pop()
+                for x in adj[v]:
+                    if x not in seen:
+                        seen.add(x); stack.append(x)
+            assert len(seen) == len(members)
+
+
+@pytest.mark.parametrize("kind", ["spd", "asym"])
+def test_galerkin_coarse_matrix(kind):
+    """agglomerateMatrix == P^T A P with piecewise-constant P."""
+    rng = np.random.default_rng(13)
+    n, lo, up = box(6, 6, 4)
+    a, m, A = mk(n, lo, up, rng, kind)
+    O.reset_direction()
+    levels = O.agglomerate(m, {"nCellsInCoarsestLevel": 4, "mergeLevels": 1})
+    assert len(levels) >= 2
+    lev = levels[0]
+    cd, cu, cl = O.agglomerate_matrix(lev, m.diag, m.upper, None if kind == "spd" else m.lower)
+    nc = lev["nCoarseCells"]
+    P = np.zeros((n, nc))
+    P[np.arange(n), lev["restrictMap"]] = 1
+    Ac = P.T @ A @ P
+    Cd = dense(nc, lev["lower"], lev["upper"], cd, cu, cl)
+    np.testing.assert_allclose(Cd, Ac, rtol=1e-12, atol=1e-12)
+    # coarse faces are upper-triangular: owner < neighbour, owners sorted
+    assert np.all(lev["lower"] < lev["upper"])
+    assert np.all(np.diff(lev["lower"]) >= 0)
+
+
+def test_merge_levels_composes_restriction():
+    rng = np.random.default_rng(14)
+    n, lo, up = box(8, 8, 4)
+    a, m, A = mk(n, lo, up, rng)
+    O.reset_direction()
+    one = O.agglomerate(m, {"nCellsInCoarsestLevel": 2, "mergeLevels": 1})
+    O.reset_direction()
+    two = O.agglomerate(m, {"nCellsInCoarsestLevel": 2, "mergeLevels": 2})
+    # level 0 of mergeLevels 2 = level 1 map composed with level 0 map
+    comp = one[1]["restrictMap"][one[0]["restrictMap"]]
+    np.testing.assert_array_equal(two[0]["restrictMap"], comp)
+    assert two[0]["nCoarseCells"] == one[1]["nCoarseCells"]
+
+
+def test_lu_matches_numpy():
+    rng = np.random.default_rng(15)
+    A = rng.standard_normal((30, 30))
+    A[0, 0] = 0.0     # forces a pivot
+    b = rng.standard_normal(30)
+    LU, piv = O.lu_decompose(A)
+    x = O.lu_backsubstitute(LU, piv, b)
+    np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=1e-10)
+
+
+def uniform_case(n, fixed=(0,)):
+    """Uniform unit-box geometry, as the case generator defines it."""
+    N, lo, up = box(n, n, n)
+    h = 1.0 / n
+    F = len(lo)
+    dirs = np.where(up - lo == 1, 0, np.where(up - lo == n, 1, 2))
+    bfc, bpatch = [], []
+    for p in range(6):
+        axis, hi = p // 2, p % 2
+        for k in range(n):
+            for j in range(n):
+                for i in range(n):
+                    idx = (i, j, k)
+                    if idx[axis] == (n - 1 if hi else 0):
+                        bfc.append(i + n * (j + n * k)); bpatch.append(p)
+    bfc, bpatch = np.array(bfc, np.int32), np.array(bpatch)
+    return dict(N=N, lo=lo, up=up, h=h, dirs=dirs, w=np.full(F, 0.5), magSf=np.full(F, h * h),
+                dc=np.full(F, 1 / h), bfc=bfc, bpatch=bpatch, bMagSf=np.full(len(bfc), h * h),
+                bdc=np.full(len(bfc), 2 / h),
+                bType=np.isin(bpatch, fixed).astype(np.int32), bValue=np.zeros(len(bfc)))
+
+
+def test_laplacian_closed_form_uniform_box():
+    c = uniform_case(6)
+    a = O.Addr(c["N"], c["lo"], c["up"])
+    gamma = np.ones(c["N"])
+    upper, diag, ic, bc = O.laplacian_assemble(a, gamma, c["w"], c["magSf"], c["dc"], c["bfc"],
+                                               c["bMagSf"], c["bdc"], c["bType"], c["bValue"])
+    h = c["h"]
+    np.testing.assert_allclose(upper, h, rtol=1e-15)
+    # interior rows sum to zero before the boundary terms
+    np.testing.assert_allclose(diag + O.Matrix(a, np.zeros(c["N"]), upper).sumA(), 0, atol=1e-15)
+    # fixedValue faces add -2 gamma h to the diagonal, zeroGradient 0
+    np.testing.assert_allclose(ic[c["bType"] == 1], -2 * h, rtol=1e-15)
+    assert np.all(ic[c["bType"] == 0] == 0)
+    d2, s2 = O.add_boundary(c["bfc"], ic, bc, diag, np.zeros(c["N"]))
+    A = dense(c["N"], c["lo"], c["up"], d2, upper)
+    # A*1 is non-zero only in cells touching the fixedValue wall
+    row = A @ np.ones(c["N"])
+    wall = np.zeros(c["N"], bool)
+    wall[c["bfc"][c["bType"] == 1]] = True
+    np.testing.assert_allclose(row[~wall], 0, atol=1e-14)
+    np.testing.assert_allclose(row[wall], -2 * h, rtol=1e-12)
+
+
+def test_laplacian_variable_gamma_vs_dense_fv():
+    """Two-point flux with linear face interpolation, built independently."""
+    rng = np.random.default_rng(16)
+    c = uniform_case(5)
+    a = O.Addr(c["N"], c["lo"], c["up"])
+    gamma = np.exp(rng.standard_normal(c["N"]))
+    upper, diag, ic, bc = O.laplacian_assemble(a, gamma, c["w"], c["magSf"], c["dc"], c["bfc"],
+                                               c["bMagSf"], c["bdc"], c["bType"], c["bValue"])
+    N = c["N"]
+    A = np.zeros((N, N))
+    for l, u in zip(c["lo"], c["up"]):
+        g = 0.5 * (gamma[l] + gamma[u]) * c["h"] ** 2 / c["h"]
+        A[l, u] += g; A[u, l] += g; A[l, l] -= g; A[u, u] -= g
+    for b, cell in enumerate(c["bfc"]):
+        if c["bType"][b]:
+            A[cell, cell] -= gamma[cell] * c["h"] ** 2 * 2 / c["h"]
+    d2, _ = O.add_boundary(c["bfc"], ic, bc, diag, np.zeros(N))
+    np.testing.assert_allclose(dense(N, c["lo"], c["up"], d2, upper), A, rtol=1e-12, atol=1e-15)
+
+
+def test_upwind_vs_dense_fv():
+    rng = np.random.default_rng(17)
+    c = uniform_case(4)
+    a = O.Addr(c["N"], c["lo"], c["up"])
+    phi = rng.standard_normal(len(c["lo"]))
+    bPhi = rng.standard_normal(len(c["bfc"]))
+    lo_, up_, dg, ic, bc = O.upwind_assemble(a, phi, c["bType"], bPhi, c["bValue"])
+    N = c["N"]
+    A = np.zeros((N, N))
+    for f, (l, u) in enumerate(zip(c["lo"], c["up"])):
+        if phi[f] >= 0:      # upwind value from the owner
+            A[l, l] += phi[f]; A[u, l] -= phi[f]
+        else:                # from the neighbour
+            A[l, u] += phi[f]; A[u, u] -= phi[f]
+    np.testing.assert_allclose(dense(N, c["lo"], c["up"], dg, up_, lo_), A, atol=1e-14)
+    zg = c["bType"] == 0
+    np.testing.assert_allclose(ic[zg], bPhi[zg])
+    np.testing.assert_allclose(bc[~zg], -bPhi[~zg] * c["bValue"][~zg])
+
+
+def test_gauss_grad_exact_for_linear_fields():
+    c = uniform_case(6)
+    a = O.Addr(c["N"], c["lo"], c["up"])
+    n, h = 6, c["h"]
+    ii = np.arange(c["N"])
+    C = np.stack([(ii % n + 0.5) * h, ((ii // n) % n + 0.5) * h, (ii // (n * n) + 0.5) * h], 1)
+    g = np.array([1.5, -2.0, 0.25])
+    phi = C @ g + 3.0
+    F = len(c["lo"])
+    Sf = np.zeros((F, 3))
+    Sf[np.arange(F), c["dirs"]] = h * h
+    bSf = np.zeros((len(c["bfc"]), 3))
+    axis = c["bpatch"] // 2
+    sign = np.where(c["bpatch"] % 2 == 1, 1.0, -1.0)
+    bSf[np.arange(len(c["bfc"])), axis] = sign * h * h
+    # boundary values of the linear field at the face centres (fixed value)
+    bCf = C[c["bfc"]].copy()
+    bCf[np.arange(len(c["bfc"])), axis] += sign * h / 2
+    bPhi = bCf @ g + 3.0
+    grad, bGrad = O.grad_gauss_linear(a, phi, c["w"], Sf.ravel(), np.full(c["N"], h ** 3),
+                                      c["bfc"], bSf.ravel(), c["bMagSf"], bPhi,
+                                      (sign * g[axis]))
+    np.testing.assert_allclose(grad.reshape(-1, 3), np.tile(g, (c["N"], 1)), rtol=1e-10)
+    np.testing.assert_allclose(bGrad.reshape(-1, 3), np.tile(g, (len(c["bfc"]), 1)), rtol=1e-10)
