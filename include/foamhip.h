@@ -58,7 +58,13 @@ enum
     FH_PRECON_DIC = 2,
     FH_PRECON_FDIC = 3,
     FH_PRECON_DILU = 4,
-    FH_PRECON_GAMG = 5
+    FH_PRECON_GAMG = 5,
+    /* no reference counterpart: DIC / DILU factorised on each 3-D tile of a
+       hex box separately (block-Jacobi over tiles), used by the FP32 inner
+       iterations of the mixed-precision path; on other meshes they fall back
+       to DIC / DILU */
+    FH_PRECON_TILE_DIC = 6,
+    FH_PRECON_TILE_DILU = 7
 };
 
 enum
@@ -112,6 +118,10 @@ typedef struct fh_controls
     double  innerRelTol;             /* inner FP32 solve: residual reduction
                                         per refinement step, default 1e-4  */
     int32_t maxRefinements;          /* default 50 */
+    int32_t innerPreconditioner;     /* mixed PCG: preconditioner of the FP32
+                                        iterations; -1 = tile-local variant of
+                                        `preconditioner` on hex boxes, else the
+                                        same preconditioner                 */
 } fh_controls;
 
 /* SolverPerformance<scalar> (SolverPerformance.H:84-92) plus extras. */
@@ -138,6 +148,7 @@ typedef struct fh_mesh_info
     int32_t nLevels;                 /* depth of the lower-triangular DAG  */
     int32_t nParts;                  /* pipelined sweep partitions (0=none)*/
     int32_t boxNx, boxNy, boxNz;     /* detected hex-box shape, 0 if not   */
+    int32_t tiled;                   /* 1: hex box swept in 3-D tiles      */
     int64_t deviceBytes;             /* addressing bytes resident in HBM   */
 } fh_mesh_info;
 

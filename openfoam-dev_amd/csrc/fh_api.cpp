@@ -89,7 +89,7 @@ static const NameId kSolvers[] = {
 static const NameId kPrecons[] = {
     {"none", FH_PRECON_NONE}, {"diagonal", FH_PRECON_DIAGONAL}, {"DIC", FH_PRECON_DIC},
     {"FDIC", FH_PRECON_FDIC}, {"DILU", FH_PRECON_DILU}, {"GAMG", FH_PRECON_GAMG},
-    {nullptr, 0}};
+    {"tileDIC", FH_PRECON_TILE_DIC}, {"tileDILU", FH_PRECON_TILE_DILU}, {nullptr, 0}};
 static const NameId kSmoothers[] = {
     {"GaussSeidel", FH_SMOOTH_GS}, {"nonBlockingGaussSeidel", FH_SMOOTH_GS},
     {"symGaussSeidel", FH_SMOOTH_SYMGS}, {"DIC", FH_SMOOTH_DIC}, {"DILU", FH_SMOOTH_DILU},
@@ -166,6 +166,7 @@ void fh_controls_default(fh_controls* c)
     c->nVcycles = 2;
     c->innerRelTol = 1e-4;
     c->maxRefinements = 50;
+    c->innerPreconditioner = -1;
 }
 
 int fh_solver_lookup(const char* name, int32_t* id)
@@ -218,6 +219,7 @@ int fh_mesh_info_get(const fh_mesh* m, fh_mesh_info* info)
         info->boxNx = m->boxNx;
         info->boxNy = m->boxNy;
         info->boxNz = m->boxNz;
+        info->tiled = m->tiled ? 1 : 0;
         info->deviceBytes = (int64_t)mesh_device_bytes(m);
     });
 }
@@ -396,6 +398,8 @@ int fh_reciprocal_d(fh_matrix* M, int32_t precon, double* rD)
         if (precon == FH_PRECON_DIC || precon == FH_PRECON_FDIC) which = 0;
         else if (precon == FH_PRECON_DILU) which = 1;
         else if (precon == FH_PRECON_DIAGONAL) which = 2;
+        else if (precon == FH_PRECON_TILE_DIC) which = 3;
+        else if (precon == FH_PRECON_TILE_DILU) which = 4;
         else fail(FH_ERR_INVALID, "preconditioner %d has no reciprocal factor", precon);
         const double* d = ensure_rD(M, which);
         DBuf<double> nat;

@@ -1,33 +1,972 @@
-// GAMG on the device (GAMGSolver*.C, GAMGAgglomeration*.C).
+// GAMG: agglomeration hierarchy, Galerkin coarse matrices and the V-cycle
+// (GAMGSolver.C:48-332, GAMGSolverSolve.C:31-552, GAMGSolverScale.C:31-76,
+// GAMGSolverAgglomerateMatrix.C:33-193, GAMGPreconditioner.C:87-148).
+//
+// The pairwise agglomeration (pairGAMGAgglomerate.C:31-301) is a greedy
+// sequential walk whose result depends on the cell order and on a
+// process-global direction toggle; it is computed on the host once per mesh
+// (the reference caches it on the mesh database, GAMGAgglomeration.C:357-399)
+// and everything that runs per solve -- coarse coefficients, restriction,
+// prolongation, scaling, smoothing and the coarsest solve -- runs on the
+// device.
 #include "fh_gamg.h"
+#include "fh_gamg_kernels.h"
+
+#include <rocblas/rocblas.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
 
 namespace fh
 {
 
-// pairGAMGAgglomeration::forward_ (pairGAMGAgglomeration.C:33): a
-// process-global direction toggle flipped after every pair agglomeration
+// pairGAMGAgglomeration::forward_ (pairGAMGAgglomeration.C:33)
 bool g_forward = true;
 
-void gamg_invalidate(fh_matrix*) {}
+// ---------------------------------------------------------------------------
+// host agglomeration
+// ---------------------------------------------------------------------------
 
-void gamg_solve(fh_matrix*, const fh_controls*, double*, const double*, fh_perf*, bool)
+// pairGAMGAgglomerate.C:123-301: greedy pairing over the cells (forward or
+// backward), pairing with the free neighbour of largest face weight, else
+// joining the neighbouring cluster across the heaviest face, else a
+// singleton; the direction flips after every call.
+static int32_t pair_agglomerate(int32_t nCells, const std::vector<int32_t>& lower,
+                                const std::vector<int32_t>& upper,
+                                const std::vector<double>& weights,
+                                std::vector<int32_t>& cmap)
 {
-    fail(FH_ERR_UNSUPPORTED, "GAMG is not available in this build yet");
+    const int32_t nF = (int32_t)lower.size();
+    std::vector<int32_t> off(nCells + 1, 0), cellFaces(2*(size_t)nF);
+    for (int32_t f = 0; f < nF; f++) { off[upper[f] + 1]++; off[lower[f] + 1]++; }
+    for (int32_t c = 0; c < nCells; c++) off[c + 1] += off[c];
+    {
+        std::vector<int32_t> fill(off.begin(), off.end() - 1);
+        // a cell's faces: those where it is the upper address first, then
+        // those where it is the lower address, each in face order
+        for (int32_t f = 0; f < nF; f++) cellFaces[fill[upper[f]]++] = f;
+        for (int32_t f = 0; f < nF; f++) cellFaces[fill[lower[f]]++] = f;
+    }
+    cmap.assign(nCells, -1);
+    int32_t nCoarse = 0;
+    const double great = 1.0/2.220446049250313e-16;
+    for (int32_t cf = 0; cf < nCells; cf++)
+    {
+        const int32_t c = g_forward ? cf : nCells - cf - 1;
+        if (cmap[c] >= 0) continue;
+        int32_t match = -1;
+        double best = -great;
+        for (int32_t q = off[c]; q < off[c + 1]; q++)
+        {
+            const int32_t f = cellFaces[q];
+            if (cmap[upper[f]] < 0 && cmap[lower[f]] < 0 && weights[f] > best)
+            {
+                match = f;
+                best = weights[f];
+            }
+        }
+        if (match >= 0)
+        {
+            cmap[upper[match]] = nCoarse;
+            cmap[lower[match]] = nCoarse;
+            nCoarse++;
+        }
+        else
+        {
+            int32_t cm = -1;
+            double cbest = -great;
+            for (int32_t q = off[c]; q < off[c + 1]; q++)
+            {
+                const int32_t f = cellFaces[q];
+                if (weights[f] > cbest) { cm = f; cbest = weights[f]; }
+            }
+            if (cm >= 0) cmap[c] = std::max(cmap[upper[cm]], cmap[lower[cm]]);
+        }
+    }
+    for (int32_t cf = 0; cf < nCells; cf++)
+    {
+        const int32_t c = g_forward ? cf : nCells - cf - 1;
+        if (cmap[c] < 0) cmap[c] = nCoarse++;
+    }
+    if (!g_forward)
+    {
+        const int32_t last = nCoarse - 1;
+        for (auto& v : cmap) v = last - v;
+    }
+    g_forward = !g_forward;
+    return nCoarse;
 }
 
-void gamg_pcg(fh_matrix*, const fh_controls*, double*, double*, double*, double*)
+// GAMGAgglomerateLduAddressing.C:32-239: coarse faces numbered by coarse
+// owner, within an owner in order of first appearance in the fine faces;
+// fine faces inside a coarse cell map to -1-cell.
+static void coarse_addressing(const std::vector<int32_t>& lower,
+                              const std::vector<int32_t>& upper,
+                              const std::vector<int32_t>& rmap, int32_t nCoarse,
+                              std::vector<int32_t>& cLower, std::vector<int32_t>& cUpper,
+                              std::vector<int32_t>& faceRestrict, std::vector<uint8_t>& flip)
 {
-    fail(FH_ERR_UNSUPPORTED, "the GAMG preconditioner is not available in this build yet");
+    const int32_t nF = (int32_t)lower.size();
+    faceRestrict.assign(nF, 0);
+    flip.assign(nF, 0);
+    // per coarse owner: list of (neighbour, provisional id) in discovery order
+    std::vector<std::vector<std::pair<int32_t, int32_t>>> own(nCoarse);
+    std::vector<int32_t> provOwner, provNei;
+    for (int32_t f = 0; f < nF; f++)
+    {
+        const int32_t ru = rmap[upper[f]], rl = rmap[lower[f]];
+        if (ru == rl)
+        {
+            faceRestrict[f] = -(ru + 1);
+            continue;
+        }
+        const int32_t cOwn = std::min(ru, rl), cNei = std::max(ru, rl);
+        int32_t id = -1;
+        for (const auto& pr : own[cOwn])
+            if (pr.first == cNei) { id = pr.second; break; }
+        if (id < 0)
+        {
+            id = (int32_t)provOwner.size();
+            own[cOwn].push_back({cNei, id});
+            provOwner.push_back(cOwn);
+            provNei.push_back(cNei);
+        }
+        faceRestrict[f] = id;
+    }
+    std::vector<int32_t> renum(provOwner.size());
+    cLower.clear();
+    cUpper.clear();
+    for (int32_t c = 0; c < nCoarse; c++)
+        for (const auto& pr : own[c])
+        {
+            renum[pr.second] = (int32_t)cLower.size();
+            cLower.push_back(c);
+            cUpper.push_back(pr.first);
+        }
+    for (int32_t f = 0; f < nF; f++)
+    {
+        if (faceRestrict[f] < 0) continue;
+        const int32_t cf = renum[faceRestrict[f]];
+        faceRestrict[f] = cf;
+        // flipped when the fine upper cell lands on the coarse owner
+        flip[f] = (cLower[cf] == rmap[upper[f]] && cUpper[cf] == rmap[lower[f]]) ? 1 : 0;
+    }
 }
 
-void pbicg_solve(fh_matrix*, const fh_controls*, double*, const double*, fh_perf*)
+struct HostLevel
 {
-    fail(FH_ERR_UNSUPPORTED, "PBiCG is not available in this build yet");
+    int32_t nFineCells, nCoarseCells;
+    std::vector<int32_t> restrictMap;      // fine cell -> coarse cell
+    std::vector<int32_t> faceRestrict;     // fine face -> coarse face / -1-cell
+    std::vector<uint8_t> faceFlip;
+    std::vector<int32_t> cLower, cUpper;   // coarse addressing
+};
+
+// pairGAMGAgglomerate.C:31-118 with combineLevels
+// (GAMGAgglomerateLduAddressing.C:493-611) for mergeLevels > 1 and
+// continueAgglomerating (GAMGAgglomeration.C:205-230)
+static std::vector<HostLevel> agglomerate(int32_t nCells, const std::vector<int32_t>& lower,
+                                          const std::vector<int32_t>& upper,
+                                          std::vector<double> weights, int32_t minCells,
+                                          int32_t mergeLevels)
+{
+    const int maxLevels = 50;
+    if (mergeLevels < 1) mergeLevels = 1;
+    std::vector<HostLevel> levels;
+    std::vector<int32_t> curLower = lower, curUpper = upper;
+    int32_t curN = nCells;
+    int nPairLevels = 0;
+    while ((int)levels.size() < maxLevels - 1)
+    {
+        std::vector<int32_t> rmap;
+        const int32_t nCoarse = pair_agglomerate(curN, curLower, curUpper, weights, rmap);
+        if (!(nCoarse >= minCells && nCoarse < curN)) break;
+        HostLevel L;
+        L.nFineCells = curN;
+        L.nCoarseCells = nCoarse;
+        L.restrictMap = rmap;
+        coarse_addressing(curLower, curUpper, rmap, nCoarse, L.cLower, L.cUpper,
+                          L.faceRestrict, L.faceFlip);
+        // restrictFaceField of the weights (GAMGAgglomerationTemplates.C:137-167)
+        std::vector<double> cw(L.cLower.size(), 0.0);
+        for (size_t f = 0; f < L.faceRestrict.size(); f++)
+            if (L.faceRestrict[f] >= 0) cw[L.faceRestrict[f]] += weights[f];
+        weights.swap(cw);
+        curLower = L.cLower;
+        curUpper = L.cUpper;
+        curN = nCoarse;
+        if (nPairLevels % mergeLevels)
+        {
+            // fold the new level into the previous one
+            HostLevel& P = levels.back();
+            for (size_t i = 0; i < P.faceRestrict.size(); i++)
+            {
+                if (P.faceRestrict[i] >= 0)
+                {
+                    const int32_t ff = P.faceRestrict[i];
+                    P.faceRestrict[i] = L.faceRestrict[ff];
+                    P.faceFlip[i] = L.faceFlip[ff];
+                }
+                else
+                {
+                    const int32_t fc = -P.faceRestrict[i] - 1;
+                    P.faceRestrict[i] = -L.restrictMap[fc] - 1;
+                    P.faceFlip[i] = L.faceFlip[fc];
+                }
+            }
+            for (auto& v : P.restrictMap) v = L.restrictMap[v];
+            P.nCoarseCells = L.nCoarseCells;
+            P.cLower = L.cLower;
+            P.cUpper = L.cUpper;
+        }
+        else
+        {
+            levels.push_back(std::move(L));
+        }
+        nPairLevels++;
+    }
+    return levels;
+}
+
+// ---------------------------------------------------------------------------
+// device agglomeration (per mesh)
+// ---------------------------------------------------------------------------
+
+static fh_mesh* make_mesh(int device, int32_t n, const std::vector<int32_t>& lo,
+                          const std::vector<int32_t>& up)
+{
+    fh_mesh* m = new fh_mesh();
+    m->device = device;
+    try
+    {
+        FH_HIP(hipStreamCreateWithFlags(&m->setupStream, hipStreamNonBlocking));
+        build_mesh(m, n, (int32_t)lo.size(), lo.data(), up.data());
+    }
+    catch (...)
+    {
+        if (m->setupStream) (void)hipStreamDestroy(m->setupStream);
+        delete m;
+        throw;
+    }
+    return m;
+}
+
+struct AggLevel
+{
+    fh_mesh* mesh = nullptr;            // the coarse mesh of this level
+    int32_t nFine = 0, nCoarse = 0, nCoarseFaces = 0;
+    DBuf<int32_t> rStart, rCells;       // coarse library row -> fine library rows
+    DBuf<int32_t> prolong;              // fine library row -> coarse library row
+    DBuf<int32_t> iStart, iFaces;       // coarse library row -> intra-cluster fine faces
+    DBuf<int32_t> fStart, fFaces;       // coarse face -> fine faces
+    DBuf<uint8_t> fFlip;
+
+    ~AggLevel()
+    {
+        if (mesh)
+        {
+            if (mesh->setupStream) (void)hipStreamDestroy(mesh->setupStream);
+            delete mesh;
+        }
+    }
+};
+
+struct GamgAgglomeration
+{
+    int32_t minCells = -1, mergeLevels = -1;
+    std::vector<std::unique_ptr<AggLevel>> levels;
+    // natural rows of the coarsest mesh for the single-workgroup solve
+    DBuf<int32_t> csRowStart, csEnt, csCol;
+};
+
+// CSR lists: bucket `key[i]` (natural coarse index, -1 skips) of the items
+// i (ascending), re-ordered by coarse library row
+static void bucket_by_lib(int32_t nKeys, const std::vector<int32_t>& key,
+                          const std::vector<int32_t>& value, const std::vector<int32_t>& perm,
+                          std::vector<int32_t>& start, std::vector<int32_t>& list,
+                          std::vector<int32_t>* itemOrder = nullptr)
+{
+    std::vector<int32_t> cnt(nKeys + 1, 0);
+    for (int32_t k : key) if (k >= 0) cnt[k + 1]++;
+    for (int32_t c = 0; c < nKeys; c++) cnt[c + 1] += cnt[c];
+    std::vector<int32_t> natList(cnt[nKeys]), natItem(cnt[nKeys]);
+    {
+        std::vector<int32_t> fill(cnt.begin(), cnt.end() - 1);
+        for (size_t i = 0; i < key.size(); i++)
+            if (key[i] >= 0)
+            {
+                natItem[fill[key[i]]] = (int32_t)i;
+                natList[fill[key[i]]++] = value[i];
+            }
+    }
+    start.assign(nKeys + 1, 0);
+    list.clear();
+    list.reserve(natList.size());
+    if (itemOrder) { itemOrder->clear(); itemOrder->reserve(natList.size()); }
+    for (int32_t r = 0; r < nKeys; r++)
+    {
+        const int32_t c = perm.empty() ? r : perm[r];
+        for (int32_t q = cnt[c]; q < cnt[c + 1]; q++)
+        {
+            list.push_back(natList[q]);
+            if (itemOrder) itemOrder->push_back(natItem[q]);
+        }
+        start[r + 1] = (int32_t)list.size();
+    }
+}
+
+static std::shared_ptr<GamgAgglomeration> build_agglomeration(fh_matrix* M,
+                                                              const fh_controls* c)
+{
+    fh_mesh* mesh = M->mesh;
+    const int32_t F = mesh->nFaces;
+    hipStream_t s = M->stream;
+    // algebraicPairGAMGAgglomeration.C:57-64: face weights mag(upper), or
+    // max(mag(upper), mag(lower)) for an asymmetric matrix
+    std::vector<double> up(F), lo;
+    if (F > 0)
+        FH_HIP(hipMemcpyAsync(up.data(), M->d_upperF.p, sizeof(double)*F,
+                              hipMemcpyDeviceToHost, s));
+    if (!M->symmetric && F > 0)
+    {
+        lo.resize(F);
+        FH_HIP(hipMemcpyAsync(lo.data(), M->d_lowerF.p, sizeof(double)*F,
+                              hipMemcpyDeviceToHost, s));
+    }
+    FH_HIP(hipStreamSynchronize(s));
+    std::vector<double> w(F);
+    for (int32_t f = 0; f < F; f++)
+    {
+        const double u = std::fabs(up[f]);
+        w[f] = lo.empty() ? u : std::max(u, std::fabs(lo[f]));
+    }
+    std::vector<HostLevel> host = agglomerate(mesh->nCells, mesh->lower, mesh->upper, w,
+                                              c->nCellsInCoarsestLevel, c->mergeLevels);
+    auto A = std::make_shared<GamgAgglomeration>();
+    A->minCells = c->nCellsInCoarsestLevel;
+    A->mergeLevels = c->mergeLevels;
+    const fh_mesh* fm = mesh;
+    for (const HostLevel& L : host)
+    {
+        auto D = std::make_unique<AggLevel>();
+        D->mesh = make_mesh(mesh->device, L.nCoarseCells, L.cLower, L.cUpper);
+        const fh_mesh* cm = D->mesh;
+        D->nFine = L.nFineCells;
+        D->nCoarse = L.nCoarseCells;
+        D->nCoarseFaces = (int32_t)L.cLower.size();
+        hipStream_t us = cm->setupStream;
+        std::vector<int32_t> start, list;
+        // restriction / Galerkin diagonal: fine cells of each coarse cell in
+        // ascending fine natural order, as fine library rows
+        std::vector<int32_t> fineLib(L.nFineCells);
+        for (int32_t i = 0; i < L.nFineCells; i++) fineLib[i] = fm->iperm[i];
+        bucket_by_lib(L.nCoarseCells, L.restrictMap, fineLib, cm->perm, start, list);
+        D->rStart.upload(start, us);
+        D->rCells.upload(list, us);
+        std::vector<int32_t> pro(L.nFineCells);
+        for (int32_t r = 0; r < L.nFineCells; r++)
+            pro[r] = cm->iperm[L.restrictMap[fm->perm[r]]];
+        D->prolong.upload(pro, us);
+        // intra-cluster fine faces per coarse library row
+        const int32_t nFF = (int32_t)L.faceRestrict.size();
+        std::vector<int32_t> key(nFF), ids(nFF);
+        for (int32_t f = 0; f < nFF; f++)
+        {
+            key[f] = L.faceRestrict[f] < 0 ? -L.faceRestrict[f] - 1 : -1;
+            ids[f] = f;
+        }
+        bucket_by_lib(L.nCoarseCells, key, ids, cm->perm, start, list);
+        D->iStart.upload(start, us);
+        D->iFaces.upload(list, us);
+        // fine faces per coarse face (coarse face order), with flips
+        for (int32_t f = 0; f < nFF; f++) key[f] = L.faceRestrict[f];
+        std::vector<int32_t> noPerm;
+        bucket_by_lib(D->nCoarseFaces, key, ids, noPerm, start, list);
+        std::vector<uint8_t> flip(list.size());
+        for (size_t q = 0; q < list.size(); q++) flip[q] = L.faceFlip[list[q]];
+        D->fStart.upload(start, us);
+        D->fFaces.upload(list, us);
+        D->fFlip.upload(flip, us);
+        FH_HIP(hipStreamSynchronize(us));
+        fm = cm;
+        A->levels.push_back(std::move(D));
+    }
+    if (!A->levels.empty())
+    {
+        // natural rows of the coarsest mesh: lower entries then upper entries,
+        // each in face order (the per-row order of lduMatrix::Amul)
+        const fh_mesh* cm = A->levels.back()->mesh;
+        const int32_t n = cm->nCells, nF = cm->nFaces;
+        std::vector<int32_t> rs(n + 1, 0), ent(2*(size_t)nF), col(2*(size_t)nF);
+        for (int32_t f = 0; f < nF; f++) { rs[cm->upper[f] + 1]++; rs[cm->lower[f] + 1]++; }
+        for (int32_t i = 0; i < n; i++) rs[i + 1] += rs[i];
+        std::vector<int32_t> fill(rs.begin(), rs.end() - 1);
+        for (int32_t f = 0; f < nF; f++)
+        {
+            const int32_t u = cm->upper[f];
+            ent[fill[u]] = f << 1;
+            col[fill[u]++] = cm->lower[f];
+        }
+        for (int32_t f = 0; f < nF; f++)
+        {
+            const int32_t l = cm->lower[f];
+            ent[fill[l]] = (f << 1) | 1;
+            col[fill[l]++] = cm->upper[f];
+        }
+        A->csRowStart.upload(rs, s);
+        A->csEnt.upload(ent, s);
+        A->csCol.upload(col, s);
+        FH_HIP(hipStreamSynchronize(s));
+    }
+    return A;
+}
+
+// ---------------------------------------------------------------------------
+// per-matrix hierarchy: coarse matrices, coarsest solve data, work vectors
+// ---------------------------------------------------------------------------
+
+template<class T>
+struct GamgWork
+{
+    std::vector<std::unique_ptr<DBuf<T>>> corr, src;   // per coarse level
+    DBuf<T> s1, s2;                                    // level scratch (largest level)
+    DBuf<T> Apsi, fCorr, fRes;                         // finest level
+    DBuf<T> e, r;                                      // mixed solver correction / residual
+    bool ready = false;
+};
+
+enum { COARSEST_ITERATIVE = 0, COARSEST_LU = 1, COARSEST_INVERSE = 2 };
+
+// LU back-substitution is one lane: it is used up to this size, the dense
+// explicit inverse (GEMMs) above it
+static const int32_t kLuMaxCells = 96;
+static const int32_t kInverseMaxCells = 4096;
+
+struct GamgHierarchy
+{
+    std::shared_ptr<GamgAgglomeration> agg;
+    std::vector<fh_matrix*> mats;        // coarse level matrices, on the parent's stream
+    bool coeffsValid = false;
+    fh_controls ctl{};
+    bool scale = false;
+    int coarseMode = -1;
+    bool coarseValid = false;
+    DBuf<double> csDiagN, csRD, csSumA, csWork;
+    DBuf<double> luLU, luWork;
+    DBuf<int32_t> luPivot;
+    DBuf<double> dA, dX, dT, dPart;
+    rocblas_handle blas = nullptr;
+    GamgWork<double> w64;
+    GamgWork<float> w32;
+
+    ~GamgHierarchy()
+    {
+        for (fh_matrix* A : mats)
+        {
+            matrix_free(A);
+            delete A;
+        }
+        if (blas) (void)rocblas_destroy_handle(blas);
+    }
+};
+
+template<class T> static GamgWork<T>& work_of(GamgHierarchy* H);
+template<> GamgWork<double>& work_of<double>(GamgHierarchy* H) { return H->w64; }
+template<> GamgWork<float>& work_of<float>(GamgHierarchy* H) { return H->w32; }
+
+template<class T>
+static GamgWork<T>& ensure_work(fh_matrix* M, GamgHierarchy* H)
+{
+    GamgWork<T>& W = work_of<T>(H);
+    if (W.ready) return W;
+    const int32_t N = M->mesh->nCells;
+    int32_t maxN = 1;
+    W.corr.clear();
+    W.src.clear();
+    for (fh_matrix* A : H->mats)
+    {
+        const int32_t n = A->mesh->nCells;
+        maxN = std::max(maxN, n);
+        W.corr.push_back(std::make_unique<DBuf<T>>());
+        W.src.push_back(std::make_unique<DBuf<T>>());
+        W.corr.back()->alloc(n);
+        W.src.back()->alloc(n);
+    }
+    W.s1.alloc(maxN);
+    W.s2.alloc(maxN);
+    W.Apsi.alloc(N);
+    W.fCorr.alloc(N);
+    W.fRes.alloc(N);
+    W.e.alloc(N);
+    W.r.alloc(N);
+    W.ready = true;
+    return W;
+}
+
+void gamg_invalidate(fh_matrix* M)
+{
+    GamgHierarchy* H = static_cast<GamgHierarchy*>(M->gamg);
+    if (!H) return;
+    H->coeffsValid = false;
+    H->coarseValid = false;
+}
+
+void gamg_release(fh_matrix* M)
+{
+    delete static_cast<GamgHierarchy*>(M->gamg);
+    M->gamg = nullptr;
+}
+
+bool gamg_forward() { return g_forward; }
+void gamg_set_forward(bool f) { g_forward = f; }
+
+// host Crout decomposition with implicit partial pivoting
+// (scalarMatrices.C:77-180), row-major, natural order
+static void crout_lu(int32_t n, std::vector<double>& A, std::vector<int32_t>& piv)
+{
+    std::vector<double> scale(n);
+    piv.assign(n, 0);
+    for (int32_t i = 0; i < n; i++)
+    {
+        double big = 0.0;
+        for (int32_t j = 0; j < n; j++) big = std::max(big, std::fabs(A[(size_t)i*n + j]));
+        if (big == 0.0) fail(FH_ERR_NUMERIC, "singular coarsest-level matrix (row %d)", i);
+        scale[i] = 1.0/big;
+    }
+    for (int32_t j = 0; j < n; j++)
+    {
+        for (int32_t i = 0; i < j; i++)
+        {
+            double sum = A[(size_t)i*n + j];
+            for (int32_t k = 0; k < i; k++) sum -= A[(size_t)i*n + k]*A[(size_t)k*n + j];
+            A[(size_t)i*n + j] = sum;
+        }
+        int32_t iMax = 0;
+        double best = 0.0;
+        for (int32_t i = j; i < n; i++)
+        {
+            double sum = A[(size_t)i*n + j];
+            for (int32_t k = 0; k < j; k++) sum -= A[(size_t)i*n + k]*A[(size_t)k*n + j];
+            A[(size_t)i*n + j] = sum;
+            const double t = scale[i]*std::fabs(sum);
+            if (t >= best) { best = t; iMax = i; }
+        }
+        piv[j] = iMax;
+        if (j != iMax)
+        {
+            for (int32_t k = 0; k < n; k++)
+                std::swap(A[(size_t)j*n + k], A[(size_t)iMax*n + k]);
+            scale[iMax] = scale[j];
+        }
+        if (A[(size_t)j*n + j] == 0.0) A[(size_t)j*n + j] = 2.220446049250313e-16;
+        if (j != n - 1)
+        {
+            const double r = 1.0/A[(size_t)j*n + j];
+            for (int32_t i = j + 1; i < n; i++) A[(size_t)i*n + j] *= r;
+        }
+    }
+}
+
+static void setup_coarsest(fh_matrix* M, GamgHierarchy* H, int mode)
+{
+    fh_matrix* C = H->mats.back();
+    const fh_mesh* cm = C->mesh;
+    const int32_t n = cm->nCells, nF = cm->nFaces;
+    hipStream_t s = M->stream;
+    H->coarseMode = mode;
+    if (mode == COARSEST_ITERATIVE)
+    {
+        H->csDiagN.alloc(n);
+        H->csRD.alloc(n);
+        H->csSumA.alloc(n);
+        H->csWork.alloc(13*(size_t)n);
+        CoarsestSys S{};
+        S.n = n;
+        S.nF = nF;
+        S.lower = cm->d_lower.p;
+        S.upper = cm->d_upper.p;
+        S.rowStart = H->agg->csRowStart.p;
+        S.ent = H->agg->csEnt.p;
+        S.colN = H->agg->csCol.p;
+        S.perm = cm->d_perm.p;
+        S.diagLib = C->d_diag.p;
+        S.upperF = C->d_upperF.p;
+        S.lowerF = C->symmetric ? C->d_upperF.p : C->d_lowerF.p;
+        S.diagN = H->csDiagN.p;
+        S.rD = H->csRD.p;
+        S.sumA = H->csSumA.p;
+        launch_coarsest_setup(S, s);
+    }
+    else
+    {
+        // dense natural copy (LUscalarMatrix.C:66-200 convert)
+        std::vector<double> diagLib(n), upper(nF), lower;
+        FH_HIP(hipMemcpyAsync(diagLib.data(), C->d_diag.p, sizeof(double)*n,
+                              hipMemcpyDeviceToHost, s));
+        if (nF > 0)
+            FH_HIP(hipMemcpyAsync(upper.data(), C->d_upperF.p, sizeof(double)*nF,
+                                  hipMemcpyDeviceToHost, s));
+        if (!C->symmetric && nF > 0)
+        {
+            lower.resize(nF);
+            FH_HIP(hipMemcpyAsync(lower.data(), C->d_lowerF.p, sizeof(double)*nF,
+                                  hipMemcpyDeviceToHost, s));
+        }
+        FH_HIP(hipStreamSynchronize(s));
+        const std::vector<double>& lo = C->symmetric ? upper : lower;
+        std::vector<double> A((size_t)n*n, 0.0);
+        for (int32_t r = 0; r < n; r++) A[(size_t)cm->perm[r]*n + cm->perm[r]] = diagLib[r];
+        for (int32_t f = 0; f < nF; f++)
+        {
+            const int32_t l = cm->lower[f], u = cm->upper[f];
+            A[(size_t)l*n + u] = upper[f];
+            A[(size_t)u*n + l] = lo[f];
+        }
+        if (mode == COARSEST_LU)
+        {
+            std::vector<int32_t> piv;
+            crout_lu(n, A, piv);
+            H->luLU.upload(A, s);
+            H->luPivot.upload(piv, s);
+            H->luWork.alloc(n);
+            FH_HIP(hipStreamSynchronize(s));
+        }
+        else
+        {
+            // explicit inverse by Newton-Schulz iteration X <- X (2I - A X)
+            // from X0 = A^T/(|A|_1 |A|_inf); the products are rocBLAS DGEMMs
+            // (FP64 MFMA).  Library row order, column-major.
+            double n1 = 0.0, ninf = 0.0;
+            for (int32_t j = 0; j < n; j++)
+            {
+                double cs = 0.0, rs = 0.0;
+                for (int32_t i = 0; i < n; i++)
+                {
+                    cs += std::fabs(A[(size_t)i*n + j]);
+                    rs += std::fabs(A[(size_t)j*n + i]);
+                }
+                n1 = std::max(n1, cs);
+                ninf = std::max(ninf, rs);
+            }
+            if (!(n1 > 0.0) || !(ninf > 0.0))
+                fail(FH_ERR_NUMERIC, "zero coarsest-level matrix");
+            const size_t nn = (size_t)n*n;
+            H->dA.alloc(nn);
+            H->dX.alloc(nn);
+            H->dT.alloc(nn);
+            DBuf<double> Xn;
+            Xn.alloc(nn);
+            H->dPart.alloc((nn + 255)/256 + 1);
+            launch_dense_fill(n, C->rows(), C->d_diag.p, C->d_val.p, H->dA.p, s);
+            launch_scaled_transpose(n, H->dA.p, 1.0/(n1*ninf), H->dX.p, s);
+            if (!H->blas)
+            {
+                if (rocblas_create_handle(&H->blas) != rocblas_status_success)
+                    fail(FH_ERR_HIP, "rocblas_create_handle failed");
+            }
+            if (rocblas_set_stream(H->blas, s) != rocblas_status_success)
+                fail(FH_ERR_HIP, "rocblas_set_stream failed");
+            const double one = 1.0, zero = 0.0;
+            std::vector<double> pm;
+            double prev = 1e300, err = 1e300;
+            for (int it = 0; it < 200; it++)
+            {
+                if (rocblas_dgemm(H->blas, rocblas_operation_none, rocblas_operation_none, n, n,
+                                  n, &one, H->dA.p, n, H->dX.p, n, &zero, H->dT.p, n)
+                    != rocblas_status_success)
+                    fail(FH_ERR_HIP, "rocblas_dgemm failed");
+                const int64_t nb = launch_ident_err(n, H->dT.p, H->dPart.p, s);
+                pm.resize(nb);
+                FH_HIP(hipMemcpyAsync(pm.data(), H->dPart.p, sizeof(double)*nb,
+                                      hipMemcpyDeviceToHost, s));
+                FH_HIP(hipStreamSynchronize(s));
+                err = 0.0;
+                for (double v : pm) err = std::max(err, v);
+                if (!std::isfinite(err)) fail(FH_ERR_NUMERIC, "coarsest inverse diverged");
+                if (err < 1e-13 || (err < 1e-8 && err > 0.5*prev)) break;
+                prev = err;
+                launch_two_minus(n, H->dT.p, s);
+                if (rocblas_dgemm(H->blas, rocblas_operation_none, rocblas_operation_none, n, n,
+                                  n, &one, H->dX.p, n, H->dT.p, n, &zero, Xn.p, n)
+                    != rocblas_status_success)
+                    fail(FH_ERR_HIP, "rocblas_dgemm failed");
+                std::swap(H->dX.p, Xn.p);
+            }
+            FH_HIP(hipStreamSynchronize(s));
+            if (!(err < 1e-6))
+                fail(FH_ERR_NUMERIC, "coarsest inverse did not converge (|I - AX| = %g)", err);
+        }
+    }
+    H->coarseValid = true;
+}
+
+void gamg_prepare(fh_matrix* M, const fh_controls* c)
+{
+    fh_mesh* mesh = M->mesh;
+    auto agg = std::static_pointer_cast<GamgAgglomeration>(mesh->gamgAgg);
+    if (!agg || agg->minCells != c->nCellsInCoarsestLevel || agg->mergeLevels != c->mergeLevels)
+    {
+        agg = build_agglomeration(M, c);
+        mesh->gamgAgg = agg;
+    }
+    GamgHierarchy* H = static_cast<GamgHierarchy*>(M->gamg);
+    if (H && H->agg != agg)
+    {
+        delete H;
+        H = nullptr;
+        M->gamg = nullptr;
+    }
+    if (!H)
+    {
+        H = new GamgHierarchy();
+        M->gamg = H;
+        H->agg = agg;
+        for (auto& L : agg->levels)
+        {
+            fh_matrix* A = new fh_matrix();
+            try { matrix_init(A, L->mesh, M->stream); }
+            catch (...) { matrix_free(A); delete A; throw; }
+            H->mats.push_back(A);
+        }
+    }
+    if (agg->levels.empty())
+        fail(FH_ERR_INVALID, "No coarse levels created, either matrix too small for GAMG or "
+             "nCellsInCoarsestLevel too large. Either choose another solver of reduce "
+             "nCellsInCoarsestLevel.");
+    H->ctl = *c;
+    H->scale = c->scaleCorrection < 0 ? M->symmetric : c->scaleCorrection != 0;
+    hipStream_t s = M->stream;
+    if (!H->coeffsValid)
+    {
+        // GAMGSolverAgglomerateMatrix.C:33-193 level by level
+        for (size_t l = 0; l < H->mats.size(); l++)
+        {
+            fh_matrix* fine = l == 0 ? M : H->mats[l - 1];
+            fh_matrix* C = H->mats[l];
+            const AggLevel& D = *agg->levels[l];
+            C->symmetric = M->symmetric;
+            C->diagonalOnly = false;
+            C->d_diag.alloc(D.nCoarse);
+            if (D.nCoarseFaces > 0)
+            {
+                C->d_upperF.alloc(D.nCoarseFaces);
+                if (!C->symmetric) C->d_lowerF.alloc(D.nCoarseFaces);
+                else C->d_lowerF.release();
+            }
+            GalerkinMaps G{D.rStart.p, D.rCells.p, D.iStart.p, D.iFaces.p,
+                           D.fStart.p, D.fFaces.p, D.fFlip.p};
+            launch_galerkin(D.nCoarse, D.nCoarseFaces, G, fine->d_diag.p, fine->d_upperF.p,
+                            fine->symmetric ? nullptr : fine->d_lowerF.p, C->d_diag.p,
+                            C->d_upperF.p, C->symmetric ? nullptr : C->d_lowerF.p, s);
+            matrix_coeffs_updated(C);
+        }
+        H->coeffsValid = true;
+        H->coarseValid = false;
+    }
+    const int32_t nC = H->mats.back()->mesh->nCells;
+    int mode = COARSEST_ITERATIVE;
+    if (c->directSolveCoarsest)
+    {
+        if (nC > kInverseMaxCells)
+            fail(FH_ERR_UNSUPPORTED, "directSolveCoarsest: coarsest level of %d cells exceeds "
+                 "the dense limit %d; raise the agglomeration or use the iterative "
+                 "coarsest solve", nC, kInverseMaxCells);
+        mode = nC <= kLuMaxCells ? COARSEST_LU : COARSEST_INVERSE;
+    }
+    if (!H->coarseValid || H->coarseMode != mode) setup_coarsest(M, H, mode);
+}
+
+// ---------------------------------------------------------------------------
+// V-cycle (GAMGSolverSolve.C:148-443) on library-numbered level vectors
+// ---------------------------------------------------------------------------
+
+template<class T>
+static void solve_coarsest(fh_matrix* M, GamgHierarchy* H, T* corr, const T* src,
+                           DevState* st)
+{
+    fh_matrix* C = H->mats.back();
+    const fh_mesh* cm = C->mesh;
+    const int32_t n = cm->nCells;
+    hipStream_t s = M->stream;
+    switch (H->coarseMode)
+    {
+        case COARSEST_LU:
+            launch_lu_backsub<T>(n, H->luLU.p, H->luPivot.p, cm->d_perm.p, H->luWork.p, src, corr,
+                                 st, s);
+            break;
+        case COARSEST_INVERSE:
+            launch_gemv<T>(n, H->dX.p, src, corr, st, s);
+            break;
+        default:
+        {
+            CoarsestSys S{};
+            S.n = n;
+            S.nF = cm->nFaces;
+            S.lower = cm->d_lower.p;
+            S.upper = cm->d_upper.p;
+            S.rowStart = H->agg->csRowStart.p;
+            S.ent = H->agg->csEnt.p;
+            S.colN = H->agg->csCol.p;
+            S.perm = cm->d_perm.p;
+            S.diagLib = C->d_diag.p;
+            S.upperF = C->d_upperF.p;
+            S.lowerF = C->symmetric ? C->d_upperF.p : C->d_lowerF.p;
+            S.diagN = H->csDiagN.p;
+            S.rD = H->csRD.p;
+            S.sumA = H->csSumA.p;
+            S.work = H->csWork.p;
+            // GAMGSolver.C:270-322: the parent's tolerance and relTol, the
+            // default iteration limits
+            S.tol = H->ctl.tolerance;
+            S.relTol = H->ctl.relTol;
+            S.maxIter = 1000;
+            S.minIter = 0;
+            S.sym = C->symmetric ? 1 : 0;
+            launch_coarsest_krylov<T>(S, src, corr, st, s);
+        }
+    }
+}
+
+template<class T>
+static void vcycle(fh_matrix* M, GamgHierarchy* H, T* psi, const T* source, T* Apsi,
+                   T* fCorr, const T* fRes, DevState* st)
+{
+    const fh_controls& c = H->ctl;
+    GamgWork<T>& W = work_of<T>(H);
+    const GamgAgglomeration& A = *H->agg;
+    const int coarsest = (int)H->mats.size() - 1;
+    hipStream_t s = M->stream;
+    const int32_t N = M->mesh->nCells;
+
+    launch_restrict<T>(A.levels[0]->nCoarse, A.levels[0]->rStart.p, A.levels[0]->rCells.p,
+                       fRes, W.src[0]->p, st, s);
+    for (int l = 0; l < coarsest; l++)
+    {
+        fh_matrix* L = H->mats[l];
+        const int32_t n = L->mesh->nCells;
+        if (c.nPreSweeps)
+        {
+            launch_zero_gamg<T>(n, W.corr[l]->p, st, s);
+            smooth_t<T>(L, c.smoother, W.corr[l]->p, W.src[l]->p,
+                        std::min(c.nPreSweeps + c.preSweepsLevelMultiplier*l, c.maxPreSweeps),
+                        st);
+            if (H->scale && l < coarsest - 1)
+                launch_gamg_scale<T>(L->rows(), diag_for<T>(L), val_for<T>(L, false),
+                                     W.corr[l]->p, W.s1.p, W.src[l]->p, L->d_partials.p, st, s);
+            launch_amul<T>(L->rows(), diag_for<T>(L), val_for<T>(L, false), W.corr[l]->p,
+                           W.s1.p, st, s);
+            launch_sub_gamg<T>(n, W.src[l]->p, W.s1.p, st, s);
+        }
+        const AggLevel& D = *A.levels[l + 1];
+        launch_restrict<T>(D.nCoarse, D.rStart.p, D.rCells.p, W.src[l]->p, W.src[l + 1]->p,
+                           st, s);
+    }
+
+    solve_coarsest<T>(M, H, W.corr[coarsest]->p, W.src[coarsest]->p, st);
+
+    for (int l = coarsest - 1; l >= 0; l--)
+    {
+        fh_matrix* L = H->mats[l];
+        const int32_t n = L->mesh->nCells;
+        if (c.nPreSweeps) launch_copy_gamg<T>(n, W.corr[l]->p, W.s2.p, st, s);
+        launch_prolong<T>(n, A.levels[l + 1]->prolong.p, W.corr[l + 1]->p, W.corr[l]->p, st, s);
+        if (H->scale && l < coarsest - 1)
+            launch_gamg_scale<T>(L->rows(), diag_for<T>(L), val_for<T>(L, false), W.corr[l]->p,
+                                 W.s1.p, W.src[l]->p, L->d_partials.p, st, s);
+        if (c.nPreSweeps) launch_axpy_gamg<T>(n, W.corr[l]->p, W.s2.p, st, s);
+        smooth_t<T>(L, c.smoother, W.corr[l]->p, W.src[l]->p,
+                    std::min(c.nPostSweeps + c.postSweepsLevelMultiplier*l, c.maxPostSweeps), st);
+    }
+
+    launch_prolong<T>(N, A.levels[0]->prolong.p, W.corr[0]->p, fCorr, st, s);
+    if (H->scale)
+        launch_gamg_scale<T>(M->rows(), diag_for<T>(M), val_for<T>(M, false), fCorr, Apsi, fRes,
+                             M->d_partials.p, st, s);
+    launch_axpy_gamg<T>(N, psi, fCorr, st, s);
+    smooth_t<T>(M, c.smoother, psi, source, c.nFinestSweeps, st);
+}
+
+// GAMGPreconditioner.C:87-148
+template<class T>
+void gamg_apply(fh_matrix* M, const T* r, T* w, DevState* st)
+{
+    GamgHierarchy* H = static_cast<GamgHierarchy*>(M->gamg);
+    if (!H || !H->coeffsValid || !H->coarseValid)
+        fail(FH_ERR_INVALID, "GAMG preconditioner applied before its hierarchy was prepared");
+    GamgWork<T>& W = ensure_work<T>(M, H);
+    const int32_t N = M->mesh->nCells;
+    hipStream_t s = M->stream;
+    launch_zero_gamg<T>(N, w, st, s);
+    launch_copy_gamg<T>(N, r, W.fRes.p, st, s);
+    const int32_t nV = std::max(H->ctl.nVcycles, 1);
+    for (int32_t cycle = 0; cycle < nV; cycle++)
+    {
+        vcycle<T>(M, H, w, r, W.Apsi.p, W.fCorr.p, W.fRes.p, st);
+        if (cycle < nV - 1)
+        {
+            launch_amul<T>(M->rows(), diag_for<T>(M), val_for<T>(M, false), w, W.Apsi.p, st, s);
+            launch_diff_gamg<T>(N, r, W.Apsi.p, W.fRes.p, st, s);
+        }
+    }
+}
+
+template void gamg_apply<double>(fh_matrix*, const double*, double*, DevState*);
+template void gamg_apply<float>(fh_matrix*, const float*, float*, DevState*);
+
+// GAMGSolverSolve.C:31-145.  The mixed form runs each V-cycle in FP32 on
+// the residual equation A e = r (e = 0 first, r scaled by its mean
+// magnitude), adds e to the FP64 solution and recomputes the FP64 residual
+// for the convergence test.
+void gamg_solve(fh_matrix* M, const fh_controls* c, double* psi, const double* src,
+                fh_perf* perf, bool mixed)
+{
+    gamg_prepare(M, c);
+    GamgHierarchy* H = static_cast<GamgHierarchy*>(M->gamg);
+    const int32_t N = M->mesh->nCells;
+    DevState* st = M->d_state.p;
+    double* part = M->d_partials.p;
+    hipStream_t s = M->stream;
+    GamgWork<double>& W = ensure_work<double>(M, H);
+    state_upload(M, c, c->tolerance, c->relTol, c->maxIter, c->minIter);
+    init_fp64(M, psi, src, W.fRes.p, W.Apsi.p);
+    if (!mixed)
+    {
+        run_batches(M, c->maxIter, [&](int32_t)
+        {
+            vcycle<double>(M, H, psi, src, W.Apsi.p, W.fCorr.p, W.fRes.p, st);
+            launch_amul<double>(M->rows(), M->d_diag.p, M->d_val.p, psi, W.Apsi.p, st, s);
+            launch_sub_sum_abs(N, W.Apsi.p, src, W.fRes.p, part, st, s);
+            launch_finalize_smooth(N, part, st, 1, s);
+        });
+    }
+    else
+    {
+        GamgWork<float>& X = ensure_work<float>(M, H);
+        run_batches(M, c->maxIter, [&](int32_t)
+        {
+            launch_to_f32_gamg(N, W.fRes.p, X.r.p, st, s);
+            launch_zero_gamg<float>(N, X.e.p, st, s);
+            vcycle<float>(M, H, X.e.p, X.r.p, X.Apsi.p, X.fCorr.p, X.r.p, st);
+            launch_add_f32_gamg(N, psi, X.e.p, st, s);
+            launch_amul<double>(M->rows(), M->d_diag.p, M->d_val.p, psi, W.Apsi.p, st, s);
+            launch_sub_sum_abs(N, W.Apsi.p, src, W.fRes.p, part, st, s);
+            launch_finalize_smooth(N, part, st, 1, s);
+        });
+    }
+    fill_perf(M, perf, mixed ? "GAMG(mixed)" : "GAMG");
 }
 
 } // namespace fh
 
-extern "C" {
-int fh_gamg_direction(void) { return fh::g_forward ? 1 : 0; }
-void fh_gamg_reset_direction(void) { fh::g_forward = true; }
+// pairGAMGAgglomeration::forward_ as seen by the C-ABI
+extern "C" int fh_gamg_direction(void)
+{
+    return fh::gamg_forward() ? 1 : 0;
+}
+
+extern "C" void fh_gamg_reset_direction(void)
+{
+    fh::gamg_set_forward(true);
 }

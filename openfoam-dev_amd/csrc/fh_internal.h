@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -130,7 +131,22 @@ struct fh_mesh
 
     fh::Schedule sched;
     int32_t boxNx = 0, boxNy = 0, boxNz = 0;
+
+    // hex box in natural numbering: tiled wavefront sweeps (16 x 16 x bz
+    // tiles, bz = 4, 8 or 16 by kind and precision)
+    bool tiled = false;
+    struct TileShape
+    {
+        int32_t bz = 0, nTI = 0, nTJ = 0, nTK = 0;
+        fh::DBuf<int32_t> d_order;
+    };
+    TileShape tiles[3];
+    int32_t maxTiles = 0;
     hipStream_t setupStream = nullptr;
+
+    // GAMG agglomeration cached with the mesh (built on the first GAMG
+    // solve, as GAMGAgglomeration is cached on the mesh database)
+    std::shared_ptr<void> gamgAgg;
 };
 
 namespace fh

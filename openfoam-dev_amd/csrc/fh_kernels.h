@@ -30,6 +30,12 @@ struct DevState
     int32_t numericError;
     int32_t scheduleError;
     int32_t earlySeq;       // PBiCGStab early exit: launch sequence number
+    // mixed precision reliable updates
+    int32_t replace;        // 1: recompute the FP64 residual this iteration
+    int32_t nReplace;       // FP64 residual replacements made
+    double resLast;         // true residual at the last replacement
+    double replaceRatio;    // replace when the FP32 residual fell by this
+    double gamgScale;       // GAMG correction scaling factor
 };
 
 // Row structure in the new numbering (shared by every kernel)
@@ -73,7 +79,9 @@ enum SweepKind
     SWEEP_RD_DIC = 2,       // rD -= upper*upper/rD(l)
     SWEEP_RD_DILU = 3,      // rD -= upper*lower/rD(l)
     SWEEP_GS_FWD = 4,       // Gauss-Seidel forward row update
-    SWEEP_GS_BWD = 5        // symmetric Gauss-Seidel backward row update
+    SWEEP_GS_BWD = 5,       // symmetric Gauss-Seidel backward row update
+    SWEEP_PRECOND_BWD_T = 6 // preconditionT backward: upper entries by
+                            // descending neighbour (losort order reversed)
 };
 
 template<class T>
@@ -85,6 +93,7 @@ struct SweepData
     T* w;                // output: w (precond), rD raw (RD), psi (GS)
     const T* diag;       // GS / RD
     const double* upperF;// RD_DILU: face-ordered upper
+    const int32_t* perm; // PRECOND_BWD_T: library row -> original cell
 };
 
 // ---- kernels: sizes in rows --------------------------------------------------

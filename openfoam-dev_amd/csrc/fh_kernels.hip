@@ -9,6 +9,7 @@
 #include "fh_kernels.h"
 
 #include <cfloat>
+#include <cstdint>
 
 namespace fh
 {
@@ -287,6 +288,27 @@ __device__ __forceinline__ void sweep_row(const RowsView& R, const SweepData<T>&
         }
         d.w[r] = w;
     }
+    else if (KIND == SWEEP_PRECOND_BWD_T)
+    {
+        // DILUPreconditioner.C:175-184: faces visited in reverse losort
+        // order, i.e. the row's upper neighbours by descending cell index
+        const T rDr = d.rD[r];
+        T w = d.w[r];
+        int32_t lastNat = INT32_MAX;
+        for (int32_t cnt = nLo; cnt < nEnt; cnt++)
+        {
+            int32_t best = -1, bestNat = -1;
+            for (int32_t k = nLo; k < nEnt; k++)
+            {
+                const int32_t e = base + (k << 6);
+                const int32_t nat = d.perm[R.col[e]];
+                if (nat < lastNat && nat > bestNat) { bestNat = nat; best = e; }
+            }
+            lastNat = bestNat;
+            w -= rDr*d.val[best]*d.w[R.col[best]];
+        }
+        d.w[r] = w;
+    }
     else if (KIND == SWEEP_RD_DIC)
     {
         // DICPreconditioner.C:67-72 (raw factor; reciprocal taken afterwards)
@@ -446,6 +468,7 @@ void launch_sweep(int kind, const RowsView& R, const SchedView& S, SweepSync& sy
         case SWEEP_RD_DILU:     launch_sweep_kind<T, SWEEP_RD_DILU, false>(R, S, sync, d, st, s); break;
         case SWEEP_GS_FWD:      launch_sweep_kind<T, SWEEP_GS_FWD, false>(R, S, sync, d, st, s); break;
         case SWEEP_GS_BWD:      launch_sweep_kind<T, SWEEP_GS_BWD, true>(R, S, sync, d, st, s); break;
+        case SWEEP_PRECOND_BWD_T: launch_sweep_kind<T, SWEEP_PRECOND_BWD_T, true>(R, S, sync, d, st, s); break;
         default: fail(FH_ERR_INVALID, "unknown sweep kind %d", kind);
     }
 }

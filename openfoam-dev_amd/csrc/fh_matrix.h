@@ -5,11 +5,14 @@
 
 #include "fh_kernels.h"
 #include "fh_solver_kernels.h"
+#include "fh_tile_sweep.h"
 
 struct fh_matrix
 {
     fh_mesh* mesh = nullptr;
     hipStream_t stream = nullptr;
+    bool ownsStream = true;                // false: a GAMG level sharing its parent's
+    void* gamg = nullptr;                  // GAMG level matrices (fh_gamg.cpp)
     hipEvent_t evStart = nullptr, evStop = nullptr;
 
     bool hasCoeffs = false;
@@ -30,14 +33,19 @@ struct fh_matrix
     bool f32Valid = false;
 
     // preconditioner / smoother factors (library numbering)
-    fh::DBuf<double> d_rD_DIC, d_rD_DILU, d_rD_diag;
-    bool rDValid[3] = {false, false, false};
-    fh::DBuf<float> d_rD32_DIC, d_rD32_DILU, d_rD32_diag;
-    bool rD32Valid[3] = {false, false, false};
+    // index: 0 DIC/FDIC, 1 DILU, 2 diagonal, 3 tile-local DIC, 4 tile-local DILU
+    fh::DBuf<double> d_rD[5];
+    bool rDValid[5] = {false, false, false, false, false};
+    fh::DBuf<float> d_rD32[5];
+    bool rD32Valid[5] = {false, false, false, false, false};
 
     // pipelined sweep synchronisation
     fh::DBuf<unsigned long long> d_sync;  // [0] ticket, [1..nParts] progress
     unsigned long long epoch = 0;
+    // tiled sweeps (hex boxes)
+    fh::DBuf<unsigned int> d_tileFlags;
+    fh::DBuf<unsigned long long> d_tileTicket;
+    unsigned long long tileTicketBase = 0, tileEpoch = 0;
 
     // device scalar state and reduction partials
     fh::DBuf<fh::DevState> d_state;
@@ -47,6 +55,8 @@ struct fh_matrix
     // work vectors (library numbering)
     fh::DBuf<double> v[10];
     fh::DBuf<float> f[10];
+    fh::DBuf<double> sw[2];               // smoother residual / correction
+    fh::DBuf<float> sw32[2];
     fh::DBuf<double> d_psiNat, d_srcNat;  // staging for host-array solves
 
     fh::RowsView rows() const
@@ -78,6 +88,29 @@ struct fh_matrix
         return v;
     }
 
+    fh::TileView tile_view(int bz) const
+    {
+        const fh_mesh::TileShape* T = nullptr;
+        for (const auto& t : mesh->tiles) if (t.bz == bz) T = &t;
+        if (!T) fh::fail(FH_ERR_INVALID, "no tile shape with depth %d", bz);
+        fh::TileView v;
+        v.nx = mesh->boxNx; v.ny = mesh->boxNy; v.nz = mesh->boxNz;
+        v.bx = 16; v.by = 16; v.bz = bz;
+        v.nTI = T->nTI; v.nTJ = T->nTJ; v.nTK = T->nTK;
+        v.order = T->d_order.p;
+        return v;
+    }
+
+    fh::TileSync tile_sync()
+    {
+        fh::TileSync s;
+        s.flags = d_tileFlags.p;
+        s.ticket = d_tileTicket.p;
+        s.ticketBase = tileTicketBase;
+        s.epoch = tileEpoch;
+        return s;
+    }
+
     fh::SweepSync sync()
     {
         fh::SweepSync s;
@@ -90,7 +123,7 @@ struct fh_matrix
 
 namespace fh
 {
-void matrix_init(fh_matrix* M, fh_mesh* mesh);
+void matrix_init(fh_matrix* M, fh_mesh* mesh, hipStream_t shared = nullptr);
 void matrix_free(fh_matrix* M);
 void matrix_set(fh_matrix* M, const double* diag, const double* upper, const double* lower,
                 bool device);
@@ -98,7 +131,8 @@ void matrix_coeffs_updated(fh_matrix* M);
 void ensure_sumA(fh_matrix* M);
 void ensure_valT(fh_matrix* M);
 void ensure_f32(fh_matrix* M);
-// factor of the DIC/FDIC (0), DILU (1) or diagonal (2) kind, FP64 or FP32
+// factor of the DIC/FDIC (0), DILU (1), diagonal (2), tile-local DIC (3)
+// or tile-local DILU (4) kind, FP64 or FP32
 const double* ensure_rD(fh_matrix* M, int which);
 const float* ensure_rD32(fh_matrix* M, int which);
 
@@ -114,4 +148,40 @@ void solve_device(fh_matrix* M, const fh_controls* c, double* d_psiNat,
                   const double* d_srcNat, fh_perf* perf);
 void smooth_device(fh_matrix* M, int smoother, double* psi, const double* src,
                    int32_t nSweeps, DevState* st);
+template<class T>
+void smooth_t(fh_matrix* M, int smoother, T* psi, const T* src, int32_t nSweeps,
+              DevState* st);
+// pieces shared with the GAMG driver
+void state_upload(fh_matrix* M, const fh_controls* c, double tol, double relTol,
+                  int32_t maxIter, int32_t minIter);
+void init_fp64(fh_matrix* M, const double* psi, const double* src, double* rA, double* tmp);
+template<class T> const T* diag_for(fh_matrix* M);
+template<class T> const T* val_for(fh_matrix* M, bool transpose);
+template<> const double* diag_for<double>(fh_matrix* M);
+template<> const float* diag_for<float>(fh_matrix* M);
+template<> const double* val_for<double>(fh_matrix* M, bool transpose);
+template<> const float* val_for<float>(fh_matrix* M, bool transpose);
+template<class T>
+void pcg_loop(fh_matrix* M, int precon, T* psi, T* r, T* p, T* w, int32_t maxIter);
+const char* precon_name(int p);
+void fill_perf(fh_matrix* M, fh_perf* perf, const char* name);
+
+// batched enqueue of device iterations with a poll of DevState.done: the
+// batch doubles up to 32 iterations between host polls
+template<class F>
+void run_batches(fh_matrix* M, int32_t maxIter, F&& enqueue)
+{
+    int32_t batch = 1, seq = 0;
+    const int64_t cap = 4*(int64_t)(maxIter > 1 ? maxIter : 1) + 64;
+    for (;;)
+    {
+        for (int32_t i = 0; i < batch; i++) enqueue(seq++);
+        FH_HIP(hipMemcpyAsync(M->h_state, M->d_state.p, sizeof(DevState),
+                              hipMemcpyDeviceToHost, M->stream));
+        FH_HIP(hipStreamSynchronize(M->stream));
+        if (M->h_state->done) break;
+        if (seq > cap) fail(FH_ERR_SCHEDULE, "solver did not terminate after %d steps", seq);
+        batch = batch*2 < 32 ? batch*2 : 32;
+    }
+}
 } // namespace fh

@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <cstdio>
 
 namespace fh
 {
@@ -127,7 +128,16 @@ void build_mesh(fh_mesh* m, int32_t nCells, int32_t nFaces,
     std::vector<int32_t> partOf(nCells, 0);
     int32_t nParts = 1;
     std::vector<int32_t> order(1, 0);
-    if (detect_box(m, nx, ny, nz))
+    const char* sweepEnv = std::getenv("FOAMHIP_SWEEP");
+    const bool levelSweeps = sweepEnv && std::strcmp(sweepEnv, "levels") == 0;
+    const bool box = detect_box(m, nx, ny, nz);
+    if (box && !levelSweeps)
+    {
+        // natural numbering; the sweeps walk 16 x 16 x bz tiles
+        m->boxNx = nx; m->boxNy = ny; m->boxNz = nz;
+        m->tiled = true;
+    }
+    else if (box)
     {
         m->boxNx = nx; m->boxNy = ny; m->boxNz = nz;
         const int T = column_size_for(nx, ny);
@@ -148,11 +158,14 @@ void build_mesh(fh_mesh* m, int32_t nCells, int32_t nFaces,
             }
     }
 
-    // renumber by (part, level, old)
+    // renumber by (part, level, old); tiled boxes keep the natural order
     std::vector<int32_t> idx(nCells);
     for (int32_t c = 0; c < nCells; c++) idx[c] = c;
-    counting_sort(m->levelOf, nLevels, idx);
-    counting_sort(partOf, nParts, idx);
+    if (!m->tiled)
+    {
+        counting_sort(m->levelOf, nLevels, idx);
+        counting_sort(partOf, nParts, idx);
+    }
     m->perm = idx;
     m->iperm.assign(nCells, 0);
     for (int32_t r = 0; r < nCells; r++) m->iperm[m->perm[r]] = r;
@@ -291,6 +304,31 @@ void build_mesh(fh_mesh* m, int32_t nCells, int32_t nFaces,
             for (int32_t q = S.upOff[p]; q < S.upOff[p + 1]; q++)
                 if (pos[S.upList[q]] >= pos[p])
                     fail(FH_ERR_INVALID, "internal: sweep order is not topological");
+    }
+
+    // tile shapes and their wavefront order (ascending TI+TJ+TK)
+    if (m->tiled)
+    {
+        const int bzs[3] = {4, 8, 16};
+        for (int t = 0; t < 3; t++)
+        {
+            fh_mesh::TileShape& T = m->tiles[t];
+            T.bz = bzs[t];
+            T.nTI = (nx + 15)/16;
+            T.nTJ = (ny + 15)/16;
+            T.nTK = (nz + T.bz - 1)/T.bz;
+            std::vector<int32_t> ord;
+            ord.reserve((size_t)T.nTI*T.nTJ*T.nTK);
+            for (int32_t d = 0; d <= T.nTI + T.nTJ + T.nTK - 3; d++)
+                for (int32_t K = 0; K < T.nTK; K++)
+                    for (int32_t J = 0; J < T.nTJ; J++)
+                    {
+                        const int32_t I = d - K - J;
+                        if (I >= 0 && I < T.nTI) ord.push_back(I + T.nTI*(J + T.nTJ*K));
+                    }
+            T.d_order.upload(ord, m->setupStream);
+            m->maxTiles = std::max<int32_t>(m->maxTiles, (int32_t)ord.size());
+        }
     }
 
     // upload

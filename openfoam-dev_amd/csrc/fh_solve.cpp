@@ -4,6 +4,7 @@
 // iterative refinement around FP32 inner PCG / PBiCGStab solves.
 #include "fh_matrix.h"
 #include "fh_gamg.h"
+#include "fh_gamg_kernels.h"
 
 #include <algorithm>
 #include <cmath>
@@ -12,11 +13,20 @@
 namespace fh
 {
 
-void matrix_init(fh_matrix* M, fh_mesh* mesh)
+void matrix_init(fh_matrix* M, fh_mesh* mesh, hipStream_t shared)
 {
     M->mesh = mesh;
     FH_HIP(hipSetDevice(mesh->device));
-    FH_HIP(hipStreamCreateWithFlags(&M->stream, hipStreamNonBlocking));
+    if (shared)
+    {
+        M->stream = shared;
+        M->ownsStream = false;
+    }
+    else
+    {
+        FH_HIP(hipStreamCreateWithFlags(&M->stream, hipStreamNonBlocking));
+        M->ownsStream = true;
+    }
     FH_HIP(hipEventCreate(&M->evStart));
     FH_HIP(hipEventCreate(&M->evStop));
     M->d_state.alloc(1);
@@ -25,6 +35,13 @@ void matrix_init(fh_matrix* M, fh_mesh* mesh)
     M->d_partials.alloc(2*reduction_partials(mesh->nCells) + 64);
     M->d_sync.alloc(1 + (size_t)mesh->sched.nParts);
     FH_HIP(hipMemsetAsync(M->d_sync.p, 0, M->d_sync.bytes(), M->stream));
+    if (mesh->tiled)
+    {
+        M->d_tileFlags.alloc(std::max(mesh->maxTiles, 1));
+        M->d_tileTicket.alloc(1);
+        FH_HIP(hipMemsetAsync(M->d_tileFlags.p, 0, M->d_tileFlags.bytes(), M->stream));
+        FH_HIP(hipMemsetAsync(M->d_tileTicket.p, 0, M->d_tileTicket.bytes(), M->stream));
+    }
     FH_HIP(hipStreamSynchronize(M->stream));
     M->epoch = 0;
 }
@@ -32,11 +49,12 @@ void matrix_init(fh_matrix* M, fh_mesh* mesh)
 void matrix_free(fh_matrix* M)
 {
     if (M->stream) (void)hipStreamSynchronize(M->stream);
+    gamg_release(M);
     if (M->h_state) (void)hipHostFree(M->h_state);
     M->h_state = nullptr;
     if (M->evStart) (void)hipEventDestroy(M->evStart);
     if (M->evStop) (void)hipEventDestroy(M->evStop);
-    if (M->stream) (void)hipStreamDestroy(M->stream);
+    if (M->stream && M->ownsStream) (void)hipStreamDestroy(M->stream);
     M->stream = nullptr;
 }
 
@@ -94,7 +112,7 @@ void matrix_coeffs_updated(fh_matrix* M)
     FH_HIP(hipStreamSynchronize(s));
     M->hasCoeffs = true;
     M->valTValid = M->sumAValid = M->f32Valid = false;
-    for (int i = 0; i < 3; i++) M->rDValid[i] = M->rD32Valid[i] = false;
+    for (int i = 0; i < 5; i++) M->rDValid[i] = M->rD32Valid[i] = false;
     gamg_invalidate(M);
 }
 
@@ -135,24 +153,41 @@ void ensure_f32(fh_matrix* M)
     M->f32Valid = true;
 }
 
-static void sweep_d(fh_matrix* M, int kind, const SweepData<double>& d, DevState* st)
+template<class T>
+static void sweep_any(fh_matrix* M, int kind, const SweepData<T>& d, DevState* st,
+                      bool local = false)
 {
+    if (M->mesh->tiled)
+    {
+        TileSync ts = M->tile_sync();
+        launch_tile_sweep<T>(kind, M->tile_view(tile_bz<T>(kind)), M->rows(), d, ts, st,
+                             M->stream, local);
+        M->tileTicketBase = ts.ticketBase;
+        M->tileEpoch = ts.epoch;
+        return;
+    }
     SweepSync sy = M->sync();
-    launch_sweep<double>(kind, M->rows(), M->sched(), sy, d, st, M->stream);
+    launch_sweep<T>(kind, M->rows(), M->sched(), sy, d, st, M->stream);
     M->epoch = sy.epoch;
 }
 
-static void sweep_f(fh_matrix* M, int kind, const SweepData<float>& d, DevState* st)
+static void sweep_d(fh_matrix* M, int kind, const SweepData<double>& d, DevState* st,
+                    bool local = false)
 {
-    SweepSync sy = M->sync();
-    launch_sweep<float>(kind, M->rows(), M->sched(), sy, d, st, M->stream);
-    M->epoch = sy.epoch;
+    sweep_any<double>(M, kind, d, st, local);
+}
+
+static void sweep_f(fh_matrix* M, int kind, const SweepData<float>& d, DevState* st,
+                    bool local = false)
+{
+    sweep_any<float>(M, kind, d, st, local);
 }
 
 const double* ensure_rD(fh_matrix* M, int which)
 {
-    DBuf<double>* bufs[3] = {&M->d_rD_DIC, &M->d_rD_DILU, &M->d_rD_diag};
-    DBuf<double>& b = *bufs[which];
+    // tile-local factors exist on tiled boxes only; elsewhere the global one
+    if (which >= 3 && !M->mesh->tiled) which -= 3;
+    DBuf<double>& b = M->d_rD[which];
     if (M->rDValid[which]) return b.p;
     const int32_t N = M->mesh->nCells;
     b.alloc(N);
@@ -170,7 +205,8 @@ const double* ensure_rD(fh_matrix* M, int which)
         d.diag = M->d_diag.p;
         d.w = b.p;
         d.upperF = M->d_upperF.p;
-        sweep_d(M, which == 0 ? SWEEP_RD_DIC : SWEEP_RD_DILU, d, nullptr);
+        const bool dilu = which == 1 || which == 4;
+        sweep_d(M, dilu ? SWEEP_RD_DILU : SWEEP_RD_DIC, d, nullptr, which >= 3);
     }
     launch_reciprocal<double>(N, b.p, M->stream);
     M->rDValid[which] = true;
@@ -179,12 +215,29 @@ const double* ensure_rD(fh_matrix* M, int which)
 
 const float* ensure_rD32(fh_matrix* M, int which)
 {
-    DBuf<float>* bufs[3] = {&M->d_rD32_DIC, &M->d_rD32_DILU, &M->d_rD32_diag};
-    DBuf<float>& b = *bufs[which];
+    if (which >= 3 && !M->mesh->tiled) which -= 3;
+    DBuf<float>& b = M->d_rD32[which];
     if (M->rD32Valid[which]) return b.p;
-    const double* r64 = ensure_rD(M, which);
-    b.alloc(M->mesh->nCells);
-    launch_convert<double, float>(M->mesh->nCells, r64, b.p, M->stream);
+    const int32_t N = M->mesh->nCells;
+    b.alloc(N);
+    if (which >= 3 && !M->diagonalOnly && M->mesh->nFaces > 0)
+    {
+        // tile-local factor over the tiles of the FP32 sweeps (deeper than
+        // the FP64 ones), so that factor and sweeps share their blocks
+        ensure_f32(M);
+        SweepData<float> d{};
+        d.val = M->d_val32.p;
+        d.diag = M->d_diag32.p;
+        d.w = b.p;
+        d.upperF = M->d_upperF.p;
+        sweep_f(M, which == 4 ? SWEEP_RD_DILU : SWEEP_RD_DIC, d, nullptr, true);
+        launch_reciprocal<float>(N, b.p, M->stream);
+    }
+    else
+    {
+        const double* r64 = ensure_rD(M, which);
+        launch_convert<double, float>(N, r64, b.p, M->stream);
+    }
     M->rD32Valid[which] = true;
     return b.p;
 }
@@ -197,6 +250,8 @@ static int factor_kind(int precon)
         case FH_PRECON_FDIC: return 0;
         case FH_PRECON_DILU: return 1;
         case FH_PRECON_DIAGONAL: return 2;
+        case FH_PRECON_TILE_DIC: return 3;
+        case FH_PRECON_TILE_DILU: return 4;
         default: return -1;
     }
 }
@@ -205,7 +260,6 @@ template<class T> static const T* rD_for(fh_matrix* M, int which);
 template<> const double* rD_for<double>(fh_matrix* M, int which) { return ensure_rD(M, which); }
 template<> const float* rD_for<float>(fh_matrix* M, int which) { return ensure_rD32(M, which); }
 
-template<class T> static const T* val_for(fh_matrix* M, bool transpose);
 template<> const double* val_for<double>(fh_matrix* M, bool transpose)
 {
     if (transpose) { ensure_valT(M); return M->d_valT.p; }
@@ -217,19 +271,21 @@ template<> const float* val_for<float>(fh_matrix* M, bool)
     return M->d_val32.p;
 }
 
-template<class T> static const T* diag_for(fh_matrix* M);
 template<> const double* diag_for<double>(fh_matrix* M) { return M->d_diag.p; }
 template<> const float* diag_for<float>(fh_matrix* M) { ensure_f32(M); return M->d_diag32.p; }
 
 template<class T>
-static void sweep_t(fh_matrix* M, int kind, const SweepData<T>& d, DevState* st);
-template<> void sweep_t<double>(fh_matrix* M, int kind, const SweepData<double>& d, DevState* st)
+static void sweep_t(fh_matrix* M, int kind, const SweepData<T>& d, DevState* st,
+                    bool local = false);
+template<> void sweep_t<double>(fh_matrix* M, int kind, const SweepData<double>& d,
+                                DevState* st, bool local)
 {
-    sweep_d(M, kind, d, st);
+    sweep_d(M, kind, d, st, local);
 }
-template<> void sweep_t<float>(fh_matrix* M, int kind, const SweepData<float>& d, DevState* st)
+template<> void sweep_t<float>(fh_matrix* M, int kind, const SweepData<float>& d,
+                               DevState* st, bool local)
 {
-    sweep_f(M, kind, d, st);
+    sweep_f(M, kind, d, st, local);
 }
 
 template<class T>
@@ -241,9 +297,18 @@ void apply_precond(fh_matrix* M, int precon, const T* r, T* w, double* partials,
     {
         // lduMatrixPreconditioner.C:84-93: a diagonal matrix gets no
         // preconditioner; DIC/DILU factors reduce to 1/diag otherwise
-        if (precon == FH_PRECON_DIC || precon == FH_PRECON_FDIC || precon == FH_PRECON_DILU)
+        if (precon == FH_PRECON_DIC || precon == FH_PRECON_FDIC || precon == FH_PRECON_DILU
+            || precon == FH_PRECON_TILE_DIC || precon == FH_PRECON_TILE_DILU)
             precon = FH_PRECON_DIAGONAL;
         if (M->diagonalOnly) precon = FH_PRECON_NONE;
+    }
+    if (precon == FH_PRECON_GAMG)
+    {
+        // GAMGPreconditioner.C:87-148; PBiCG's transposed apply uses the
+        // same V-cycles (GAMGPreconditioner.H:125-133)
+        gamg_apply<T>(M, r, w, st);
+        if (partials) launch_dot<T, T>(N, w, r, partials, st, M->stream);
+        return;
     }
     if (precon == FH_PRECON_NONE || precon == FH_PRECON_DIAGONAL)
     {
@@ -259,35 +324,18 @@ void apply_precond(fh_matrix* M, int precon, const T* r, T* w, double* partials,
     d.rD = rD_for<T>(M, which);
     d.x = r;
     d.w = w;
-    sweep_t<T>(M, SWEEP_PRECOND_FWD, d, st);
-    sweep_t<T>(M, SWEEP_PRECOND_BWD, d, st);
+    d.perm = M->mesh->d_perm.p;
+    const bool local = which >= 3 && M->mesh->tiled;
+    const bool dilu = which == 1 || which == 4;
+    sweep_t<T>(M, SWEEP_PRECOND_FWD, d, st, local);
+    sweep_t<T>(M, transpose && dilu ? SWEEP_PRECOND_BWD_T : SWEEP_PRECOND_BWD, d, st, local);
     if (partials) launch_dot<T, T>(N, w, r, partials, st, M->stream);
 }
 
 template void apply_precond<double>(fh_matrix*, int, const double*, double*, double*, DevState*, bool);
 template void apply_precond<float>(fh_matrix*, int, const float*, float*, double*, DevState*, bool);
 
-// ---------------------------------------------------------------------------
-// batched enqueue of device iterations with a poll of DevState.done
-// ---------------------------------------------------------------------------
-template<class F>
-static void run_batches(fh_matrix* M, int32_t maxIter, F&& enqueue)
-{
-    int32_t batch = 1, seq = 0;
-    const int64_t cap = 4*(int64_t)std::max(maxIter, 1) + 64;
-    for (;;)
-    {
-        for (int32_t i = 0; i < batch; i++) enqueue(seq++);
-        FH_HIP(hipMemcpyAsync(M->h_state, M->d_state.p, sizeof(DevState),
-                              hipMemcpyDeviceToHost, M->stream));
-        FH_HIP(hipStreamSynchronize(M->stream));
-        if (M->h_state->done) break;
-        if (seq > cap) fail(FH_ERR_SCHEDULE, "solver did not terminate after %d steps", seq);
-        batch = std::min(batch*2, 32);
-    }
-}
-
-static void state_upload(fh_matrix* M, const fh_controls* c, double tol, double relTol,
+void state_upload(fh_matrix* M, const fh_controls* c, double tol, double relTol,
                          int32_t maxIter, int32_t minIter)
 {
     DevState* h = M->h_state;
@@ -314,7 +362,7 @@ static void state_upload_outer(fh_matrix* M, const DevState& outer)
 
 // FP64 initialisation shared by the solvers: wA = A psi, normFactor,
 // rA = source - wA, initial residual (lduMatrixSolver.C:174-197)
-static void init_fp64(fh_matrix* M, const double* psi, const double* src, double* rA,
+void init_fp64(fh_matrix* M, const double* psi, const double* src, double* rA,
                       double* tmp)
 {
     const int32_t N = M->mesh->nCells;
@@ -329,7 +377,7 @@ static void init_fp64(fh_matrix* M, const double* psi, const double* src, double
 }
 
 template<class T>
-static void pcg_loop(fh_matrix* M, int precon, T* psi, T* r, T* p, T* w, int32_t maxIter)
+void pcg_loop(fh_matrix* M, int precon, T* psi, T* r, T* p, T* w, int32_t maxIter)
 {
     const int32_t N = M->mesh->nCells;
     DevState* st = M->d_state.p;
@@ -374,14 +422,14 @@ static void bicgstab_loop(fh_matrix* M, int precon, T* psi, T* rA, T** V, int32_
         launch_finalize(N, part, FIN_BICG_S, st, seq, M->stream);
         launch_bicg_early<T>(N, psi, yA, st, seq, M->stream);
         apply_precond<T>(M, precon, sA, zA, nullptr, st, false);
-        launch_spmv_dot<T>(M->rows(), diag, val, zA, tA, tA, sA, part, st, M->stream);
+        launch_spmv_dot<T>(M->rows(), diag, val, zA, tA, nullptr, sA, part, st, M->stream);
         launch_finalize(N, part, FIN_BICG_OMEGA, st, seq, M->stream);
         launch_bicg_update<T>(N, psi, yA, zA, rA, sA, tA, rA0, part, st, M->stream);
         launch_finalize(N, part, FIN_BICG_RES, st, seq, M->stream);
     });
 }
 
-static void fill_perf(fh_matrix* M, const fh_controls* c, fh_perf* perf, const char* name)
+void fill_perf(fh_matrix* M, fh_perf* perf, const char* name)
 {
     const DevState* h = M->h_state;
     std::memset(perf, 0, sizeof(*perf));
@@ -392,10 +440,9 @@ static void fill_perf(fh_matrix* M, const fh_controls* c, fh_perf* perf, const c
     perf->converged = h->converged;
     perf->singular = h->singular;
     perf->normFactor = h->normFactor;
-    (void)c;
 }
 
-static const char* precon_name(int p)
+const char* precon_name(int p)
 {
     switch (p)
     {
@@ -405,6 +452,8 @@ static const char* precon_name(int p)
         case FH_PRECON_FDIC: return "FDIC";
         case FH_PRECON_DILU: return "DILU";
         case FH_PRECON_GAMG: return "GAMG";
+        case FH_PRECON_TILE_DIC: return "tileDIC";
+        case FH_PRECON_TILE_DILU: return "tileDILU";
         default: return "?";
     }
 }
@@ -442,38 +491,45 @@ static void validate_selection(const fh_matrix* M, const fh_controls* c)
 // ---------------------------------------------------------------------------
 // smoothers (library numbering, FP64)
 // ---------------------------------------------------------------------------
-void smooth_device(fh_matrix* M, int smoother, double* psi, const double* src,
-                   int32_t nSweeps, DevState* st)
+template<class T> static DBuf<T>* smooth_work(fh_matrix* M);
+template<> DBuf<double>* smooth_work<double>(fh_matrix* M) { return M->sw; }
+template<> DBuf<float>* smooth_work<float>(fh_matrix* M) { return M->sw32; }
+
+// GaussSeidelSmoother.C:66-187, symGaussSeidelSmoother.C:66-216,
+// DICSmoother.C:67-116, DILUSmoother.C:67-119, DICGaussSeidelSmoother.C:79-89,
+// DILUGaussSeidelSmoother.C:79-89 (library numbering)
+template<class T>
+void smooth_t(fh_matrix* M, int smoother, T* psi, const T* src, int32_t nSweeps,
+              DevState* st)
 {
     const int32_t N = M->mesh->nCells;
     if (nSweeps <= 0 || smoother == FH_SMOOTH_NONE) return;
+    const T* diag = diag_for<T>(M);
+    const T* val = val_for<T>(M, false);
     auto gs = [&](bool sym)
     {
-        SweepData<double> d{};
-        d.val = M->d_val.p;
-        d.diag = M->d_diag.p;
+        SweepData<T> d{};
+        d.val = val;
+        d.diag = diag;
         d.x = src;
         d.w = psi;
         for (int32_t s = 0; s < nSweeps; s++)
         {
-            sweep_d(M, SWEEP_GS_FWD, d, st);
-            if (sym) sweep_d(M, SWEEP_GS_BWD, d, st);
+            sweep_t<T>(M, SWEEP_GS_FWD, d, st);
+            if (sym) sweep_t<T>(M, SWEEP_GS_BWD, d, st);
         }
     };
     auto ilu = [&](bool dilu)
     {
-        // DICSmoother.C:67-116 / DILUSmoother.C:67-119
-        double* rA = M->v[9].p;
-        if (!rA || M->v[9].n < (size_t)N) { M->v[9].alloc(N); rA = M->v[9].p; }
-        double* wA = M->v[8].p;
-        if (!wA || M->v[8].n < (size_t)N) { M->v[8].alloc(N); wA = M->v[8].p; }
+        DBuf<T>* wk = smooth_work<T>(M);
+        wk[0].alloc(N);
+        wk[1].alloc(N);
         for (int32_t s = 0; s < nSweeps; s++)
         {
-            launch_residual<double>(M->rows(), M->d_diag.p, M->d_val.p, psi, src, rA, st,
-                                    M->stream);
-            apply_precond<double>(M, dilu ? FH_PRECON_DILU : FH_PRECON_DIC, rA, wA, nullptr,
-                                  st, false);
-            launch_axpy_inplace(N, psi, wA, st, M->stream);
+            launch_residual<T>(M->rows(), diag, val, psi, src, wk[0].p, st, M->stream);
+            apply_precond<T>(M, dilu ? FH_PRECON_DILU : FH_PRECON_DIC, wk[0].p, wk[1].p,
+                             nullptr, st, false);
+            launch_axpy_gamg<T>(N, psi, wk[1].p, st, M->stream);
         }
     };
     switch (smoother)
@@ -487,6 +543,15 @@ void smooth_device(fh_matrix* M, int smoother, double* psi, const double* src,
         case FH_SMOOTH_DILUGS: ilu(true); gs(false); break;
         default: fail(FH_ERR_INVALID, "unknown smoother %d", smoother);
     }
+}
+
+template void smooth_t<double>(fh_matrix*, int, double*, const double*, int32_t, DevState*);
+template void smooth_t<float>(fh_matrix*, int, float*, const float*, int32_t, DevState*);
+
+void smooth_device(fh_matrix* M, int smoother, double* psi, const double* src,
+                   int32_t nSweeps, DevState* st)
+{
+    smooth_t<double>(M, smoother, psi, src, nSweeps, st);
 }
 
 // smoothSolver.C:77-193
@@ -516,8 +581,70 @@ static void smooth_solver(fh_matrix* M, const fh_controls* c, double* psi, const
 }
 
 // ---------------------------------------------------------------------------
+// mixed precision PCG with reliable updates: the Krylov recurrence (DIC
+// preconditioner, SpMV, search direction, recurrence residual) runs in
+// FP32; the solution is accumulated in FP64, the dot products are FP64
+// sums, and whenever the FP32 residual has fallen by a decade (or looks
+// converged) the true residual b - A x is recomputed in FP64, tested with
+// the reference criterion (normFactor-normalised L1) and replaces the FP32
+// one.  The recurrence is never restarted, so the iteration count stays
+// close to the FP64 one.
+// ---------------------------------------------------------------------------
+static void mixed_pcg(fh_matrix* M, const fh_controls* c, double* psi, const double* src,
+                      fh_perf* perf)
+{
+    const int32_t N = M->mesh->nCells;
+    DevState* st = M->d_state.p;
+    double* part = M->d_partials.p;
+    for (int i = 0; i < 2; i++) M->v[i].alloc(N);
+    double* rA = M->v[0].p;
+    for (int i = 0; i < 3; i++) M->f[i].alloc(N);
+    float *r32 = M->f[0].p, *p32 = M->f[1].p, *w32 = M->f[2].p;
+    ensure_f32(M);
+    int precon = c->preconditioner;
+    if (c->innerPreconditioner >= 0) precon = c->innerPreconditioner;
+    else if (M->mesh->tiled && (precon == FH_PRECON_DIC || precon == FH_PRECON_FDIC))
+        precon = FH_PRECON_TILE_DIC;
+    if (precon != FH_PRECON_NONE && factor_kind(precon) >= 0)
+        (void)rD_for<float>(M, factor_kind(precon));
+
+    state_upload(M, c, c->tolerance, c->relTol, c->maxIter, c->minIter);
+    init_fp64(M, psi, src, rA, M->v[1].p);
+    launch_convert<double, float>(N, rA, r32, M->stream);
+    // resLast = initial residual, replaceRatio: one decade per replacement
+    launch_mixed_setup(st, 0.1, M->stream);
+    const float* diag = M->d_diag32.p;
+    const float* val = M->d_val32.p;
+    run_batches(M, c->maxIter, [&](int32_t)
+    {
+        apply_precond<float>(M, precon, r32, w32, part, st, false);
+        launch_finalize(N, part, FIN_PCG_WARA, st, 0, M->stream);
+        launch_pcg_p<float>(N, w32, p32, st, M->stream);
+        launch_spmv_dot<float>(M->rows(), diag, val, p32, w32, p32, nullptr, part, st,
+                               M->stream);
+        launch_finalize(N, part, FIN_PCG_WAPA, st, 0, M->stream);
+        launch_mixed_update(N, psi, r32, p32, w32, part, st, M->stream);
+        launch_finalize(N, part, FIN_MIXED_RES, st, 0, M->stream);
+        launch_mixed_replace(M->rows(), M->d_diag.p, M->d_val.p, psi, src, r32, part, st,
+                             M->stream);
+        launch_finalize(N, part, FIN_MIXED_TRUE, st, 0, M->stream);
+    });
+    const DevState* h = M->h_state;
+    std::memset(perf, 0, sizeof(*perf));
+    std::snprintf(perf->solverName, sizeof(perf->solverName), "%sPCG(mixed)",
+                  precon_name(c->preconditioner));
+    perf->initialResidual = h->initialResidual;
+    perf->finalResidual = h->finalResidual;
+    perf->nIterations = h->nIterations;
+    perf->converged = h->converged;
+    perf->singular = h->singular;
+    perf->nRefinements = h->nReplace;
+    perf->normFactor = h->normFactor;
+}
+
+// ---------------------------------------------------------------------------
 // mixed precision: FP64 outer residual / normFactor / convergence test /
-// solution update around FP32 inner solves
+// solution update around restarted FP32 inner solves (PBiCGStab)
 // ---------------------------------------------------------------------------
 static void mixed_solve(fh_matrix* M, const fh_controls* c, double* psi, const double* src,
                         fh_perf* perf)
@@ -600,6 +727,51 @@ static void mixed_solve(fh_matrix* M, const fh_controls* c, double* psi, const d
 }
 
 // ---------------------------------------------------------------------------
+// PBiCG (PBiCG.C:65-222): the transposed system runs alongside with Tmul
+// and preconditionT
+// ---------------------------------------------------------------------------
+static void pbicg_solve(fh_matrix* M, const fh_controls* c, double* psi, const double* src,
+                        fh_perf* perf)
+{
+    const int32_t N = M->mesh->nCells;
+    DevState* st = M->d_state.p;
+    double* part = M->d_partials.p;
+    hipStream_t s = M->stream;
+    for (int i = 4; i < 10; i++) M->v[i].alloc(N);
+    double *rA = M->v[4].p, *wA = M->v[5].p, *pA = M->v[6].p;
+    double *rT = M->v[7].p, *wT = M->v[8].p, *pT = M->v[9].p;
+    const int precon = c->preconditioner;
+    state_upload(M, c, c->tolerance, c->relTol, c->maxIter, c->minIter);
+    init_fp64(M, psi, src, rA, wA);
+    ensure_valT(M);
+    const double* diag = M->d_diag.p;
+    const double* val = M->d_val.p;
+    const double* valT = M->d_valT.p;
+    if (precon != FH_PRECON_NONE && factor_kind(precon) >= 0) (void)ensure_rD(M, factor_kind(precon));
+    // rT = source - T psi
+    launch_amul<double>(M->rows(), diag, valT, psi, wT, st, s);
+    launch_diff_gamg<double>(N, src, wT, rT, st, s);
+    run_batches(M, c->maxIter, [&](int32_t)
+    {
+        apply_precond<double>(M, precon, rA, wA, nullptr, st, false);
+        apply_precond<double>(M, precon, rT, wT, nullptr, st, true);
+        launch_dot<double, double>(N, wA, rT, part, st, s);
+        launch_finalize(N, part, FIN_PCG_WARA, st, 0, s);
+        launch_pcg_p<double>(N, wA, pA, st, s);
+        launch_pcg_p<double>(N, wT, pT, st, s);
+        launch_amul<double>(M->rows(), diag, valT, pT, wT, st, s);
+        launch_spmv_dot<double>(M->rows(), diag, val, pA, wA, pT, nullptr, part, st, s);
+        launch_finalize(N, part, FIN_PCG_WAPA, st, 0, s);
+        launch_axpy_alpha<double>(N, rT, wT, st, s);
+        launch_pcg_update<double>(N, psi, rA, pA, wA, part, st, s);
+        launch_finalize(N, part, FIN_PCG_RES, st, 0, s);
+    });
+    char name[64];
+    std::snprintf(name, sizeof(name), "%sPBiCG", precon_name(precon));
+    fill_perf(M, perf, name);
+}
+
+// ---------------------------------------------------------------------------
 void solve_device(fh_matrix* M, const fh_controls* c, double* d_psiNat,
                   const double* d_srcNat, fh_perf* perf)
 {
@@ -625,9 +797,14 @@ void solve_device(fh_matrix* M, const fh_controls* c, double* d_psiNat,
         std::snprintf(perf->solverName, sizeof(perf->solverName), "diagonal");
         perf->converged = 1;
     }
-    else if (c->precision == FH_PRECISION_MIXED
-             && (c->solver == FH_SOLVER_PCG || c->solver == FH_SOLVER_PBICGSTAB))
+    else if (c->precision == FH_PRECISION_MIXED && c->solver == FH_SOLVER_PCG)
     {
+        if (c->preconditioner == FH_PRECON_GAMG) gamg_prepare(M, c);
+        mixed_pcg(M, c, psi, src, perf);
+    }
+    else if (c->precision == FH_PRECISION_MIXED && c->solver == FH_SOLVER_PBICGSTAB)
+    {
+        if (c->preconditioner == FH_PRECON_GAMG) gamg_prepare(M, c);
         mixed_solve(M, c, psi, src, perf);
     }
     else if (c->precision == FH_PRECISION_MIXED && c->solver == FH_SOLVER_GAMG)
@@ -636,6 +813,10 @@ void solve_device(fh_matrix* M, const fh_controls* c, double* d_psiNat,
     }
     else
     {
+        if (c->preconditioner == FH_PRECON_GAMG
+            && (c->solver == FH_SOLVER_PCG || c->solver == FH_SOLVER_PBICGSTAB
+                || c->solver == FH_SOLVER_PBICG))
+            gamg_prepare(M, c);
         switch (c->solver)
         {
             case FH_SOLVER_PCG:
@@ -644,16 +825,13 @@ void solve_device(fh_matrix* M, const fh_controls* c, double* d_psiNat,
                 double *rA = M->v[4].p, *pA = M->v[5].p, *wA = M->v[6].p;
                 state_upload(M, c, c->tolerance, c->relTol, c->maxIter, c->minIter);
                 init_fp64(M, psi, src, rA, wA);
-                if (c->preconditioner == FH_PRECON_GAMG)
-                    gamg_pcg(M, c, psi, rA, pA, wA);
-                else
-                    pcg_loop<double>(M, c->preconditioner, psi, rA, pA, wA, c->maxIter);
+                pcg_loop<double>(M, c->preconditioner, psi, rA, pA, wA, c->maxIter);
                 FH_HIP(hipMemcpyAsync(M->h_state, M->d_state.p, sizeof(DevState),
                                       hipMemcpyDeviceToHost, s));
                 FH_HIP(hipStreamSynchronize(s));
                 char name[64];
                 std::snprintf(name, sizeof(name), "%sPCG", precon_name(c->preconditioner));
-                fill_perf(M, c, perf, name);
+                fill_perf(M, perf, name);
                 break;
             }
             case FH_SOLVER_PBICGSTAB:
@@ -668,8 +846,6 @@ void solve_device(fh_matrix* M, const fh_controls* c, double* d_psiNat,
                 extra[1].alloc(N);
                 double* V[7] = {M->v[5].p, M->v[6].p, M->v[7].p, M->v[8].p, M->v[9].p,
                                 extra[0].p, extra[1].p};
-                if (c->preconditioner == FH_PRECON_GAMG)
-                    fail(FH_ERR_UNSUPPORTED, "PBiCGStab with the GAMG preconditioner");
                 FH_HIP(hipMemcpyAsync(M->h_state, M->d_state.p, sizeof(DevState),
                                       hipMemcpyDeviceToHost, s));
                 FH_HIP(hipStreamSynchronize(s));
@@ -680,7 +856,7 @@ void solve_device(fh_matrix* M, const fh_controls* c, double* d_psiNat,
                 FH_HIP(hipStreamSynchronize(s));
                 char name[64];
                 std::snprintf(name, sizeof(name), "%sPBiCGStab", precon_name(c->preconditioner));
-                fill_perf(M, c, perf, name);
+                fill_perf(M, perf, name);
                 break;
             }
             case FH_SOLVER_PBICG:
@@ -693,7 +869,7 @@ void solve_device(fh_matrix* M, const fh_controls* c, double* d_psiNat,
                 state_upload(M, c, c->tolerance, c->relTol, c->maxIter, c->minIter);
                 smooth_solver(M, c, psi, src);
                 FH_HIP(hipStreamSynchronize(s));
-                fill_perf(M, c, perf, "smoothSolver");
+                fill_perf(M, perf, "smoothSolver");
                 break;
             }
             case FH_SOLVER_GAMG:

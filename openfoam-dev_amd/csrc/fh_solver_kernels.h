@@ -21,7 +21,9 @@ enum FinalizeOp
     FIN_BICG_RES = 10,
     FIN_INNER_INIT = 11,
     FIN_OUTER_RES = 12,
-    FIN_SMOOTH_RES = 13
+    FIN_SMOOTH_RES = 13,
+    FIN_MIXED_RES = 14,
+    FIN_MIXED_TRUE = 15
 };
 
 void launch_finalize(int32_t n, const double* partials, int32_t op, DevState* st,
@@ -63,6 +65,15 @@ void launch_bicg_update(int32_t n, T* psi, const T* y, const T* z, T* r, const T
 template<class T>
 void launch_pcg_update_nostate(int32_t n, T* psi, T* r, const T* p, const T* w,
                                double* partials, hipStream_t s);
+// mixed PCG: x64 += alpha*p32, r32 -= alpha*w32, sum |r32|
+void launch_mixed_update(int32_t n, double* x, float* r, const float* p, const float* w,
+                         double* partials, const DevState* st, hipStream_t s);
+// mixed PCG residual replacement (only when st->replace): r = b - A x in
+// FP64, r32 = r, sum |r|
+void launch_mixed_replace(const RowsView& R, const double* diag, const double* val,
+                          const double* x, const double* b, float* r32, double* partials,
+                          const DevState* st, hipStream_t s);
+void launch_mixed_setup(DevState* st, double replaceRatio, hipStream_t s);
 // psi += w
 void launch_axpy_inplace(int32_t n, double* psi, const double* w, const DevState* st,
                          hipStream_t s);
@@ -76,6 +87,12 @@ void launch_finalize_smooth(int32_t n, const double* partials, DevState* st,
 void launch_scale_convert(int32_t n, const double* x, double scale, float* y, hipStream_t s);
 // psi += scale*(double)d
 void launch_axpy_f32(int32_t n, double* psi, const float* d, double scale, hipStream_t s);
+// y -= alpha*x with the state's alpha (PBiCG transpose residual)
+template<class T>
+void launch_axpy_alpha(int32_t n, T* y, const T* x, const DevState* st, hipStream_t s);
+// r = source - Apsi, sum |r| into the second partial (GAMGSolverSolve.C:120-128)
+void launch_sub_sum_abs(int32_t n, const double* Apsi, const double* source, double* r,
+                        double* partials, const DevState* st, hipStream_t s);
 // psi = src/diag
 void launch_divide(int32_t n, const double* src, const double* diag, double* psi,
                    hipStream_t s);

@@ -135,11 +135,12 @@ __global__ void k_pcg_p(int32_t n, const T* __restrict__ w, T* __restrict__ p,
     }
 }
 
-// y = A x fused with sum0 = sum y*z, sum1 = sum y*v (v may be null)
+// y = A x fused with sum0 = sum y*z, sum1 = sum y*v (v may be null; z null
+// means z = y, i.e. sum y*y)
 template<class T>
 __global__ void k_spmv_dot(RowsView R, const T* __restrict__ diag, const T* __restrict__ val,
                            const T* __restrict__ x, T* __restrict__ y,
-                           const T* __restrict__ z, const T* __restrict__ v,
+                           const T* z, const T* v,
                            double* partials, const DevState* st)
 {
     if (st->done) return;
@@ -156,7 +157,7 @@ __global__ void k_spmv_dot(RowsView R, const T* __restrict__ diag, const T* __re
             acc += val[e]*x[R.col[e]];
         }
         y[r] = acc;
-        s0 = (double)acc*(double)z[r];
+        s0 = (double)acc*(double)(z ? z[r] : acc);
         if (v) s1 = (double)acc*(double)v[r];
     }
     write_partials(s0, s1, partials, gridDim.x);
@@ -394,6 +395,34 @@ __global__ void k_finalize(int32_t nBlocks, const double* partials, int32_t op,
             st->done = dev_converged(st) ? 1 : 0;
             break;
         }
+        case FIN_MIXED_RES:
+        {
+            // FP32 recurrence residual; schedule an FP64 replacement when it
+            // fell by replaceRatio since the last one, when it looks
+            // converged, or at the iteration limit
+            const double est = a/st->normFactor;
+            if (!isfinite(est)) { st->numericError = 1; st->done = 1; break; }
+            st->nIterations++;
+            bool looks = est < st->tolerance
+                      || (st->relTol > 1e-20 && est < st->relTol*st->initialResidual);
+            st->replace = (looks || est < st->replaceRatio*st->resLast
+                           || st->nIterations >= st->maxIter) ? 1 : 0;
+            break;
+        }
+        case FIN_MIXED_TRUE:
+        {
+            if (!st->replace) break;
+            st->replace = 0;
+            st->nReplace++;
+            st->finalResidual = b/st->normFactor;
+            if (!isfinite(st->finalResidual)) { st->numericError = 1; st->done = 1; break; }
+            st->resLast = st->finalResidual;
+            bool cont = false;
+            if (st->nIterations < st->maxIter) cont = !dev_converged(st);
+            cont = cont || st->nIterations < st->minIter;
+            if (!cont) st->done = 1;
+            break;
+        }
         case FIN_SMOOTH_RES:
         {
             // smoothSolver.C:156-189: nIterations += nSweeps before the test
@@ -409,6 +438,76 @@ __global__ void k_finalize(int32_t nBlocks, const double* partials, int32_t op,
         default:
             break;
     }
+}
+
+__global__ void k_mixed_update(int32_t n, double* __restrict__ x, float* __restrict__ r,
+                               const float* __restrict__ p, const float* __restrict__ w,
+                               double* partials, const DevState* st)
+{
+    if (st->done) return;
+    const int32_t i = blockIdx.x*FH_BLOCK + threadIdx.x;
+    double s0 = 0.0;
+    if (i < n)
+    {
+        const double alpha = st->alpha;
+        x[i] += alpha*(double)p[i];
+        const float ri = r[i] - (float)alpha*w[i];
+        r[i] = ri;
+        s0 = fabs((double)ri);
+    }
+    write_partials(s0, 0.0, partials, gridDim.x);
+}
+
+__global__ void k_mixed_replace(RowsView R, const double* __restrict__ diag,
+                                const double* __restrict__ val, const double* __restrict__ x,
+                                const double* __restrict__ b, float* __restrict__ r32,
+                                double* partials, const DevState* st)
+{
+    if (st->done || !st->replace) return;
+    const int32_t r = blockIdx.x*FH_BLOCK + threadIdx.x;
+    double s1 = 0.0;
+    if (r < R.n)
+    {
+        const int32_t nEnt = R.rowInfo[r] >> 16;
+        const int32_t base = sell_base2(R.sliceStart, r);
+        double acc = b[r] - diag[r]*x[r];
+        for (int32_t k = 0; k < nEnt; k++)
+        {
+            const int32_t e = base + (k << 6);
+            acc -= val[e]*x[R.col[e]];
+        }
+        r32[r] = (float)acc;
+        s1 = fabs(acc);
+    }
+    write_partials(0.0, s1, partials, gridDim.x);
+}
+
+__global__ void k_mixed_setup(DevState* st, double ratio)
+{
+    st->resLast = st->initialResidual;
+    st->replaceRatio = ratio;
+    st->replace = 0;
+    st->nReplace = 0;
+}
+
+void launch_mixed_setup(DevState* st, double replaceRatio, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mixed_setup, dim3(1), dim3(1), 0, s, st, replaceRatio);
+}
+
+void launch_mixed_update(int32_t n, double* x, float* r, const float* p, const float* w,
+                         double* partials, const DevState* st, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mixed_update, dim3(reduction_blocks(n)), dim3(FH_BLOCK), 0, s,
+                       n, x, r, p, w, partials, st);
+}
+
+void launch_mixed_replace(const RowsView& R, const double* diag, const double* val,
+                          const double* x, const double* b, float* r32, double* partials,
+                          const DevState* st, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mixed_replace, dim3(reduction_blocks(R.n)), dim3(FH_BLOCK), 0, s,
+                       R, diag, val, x, b, r32, partials, st);
 }
 
 __global__ void k_axpy_inplace(int32_t n, double* __restrict__ psi,
@@ -428,6 +527,31 @@ __global__ void k_sum_abs_state(int32_t n, const T* __restrict__ a, double* part
     double s = 0.0, m = 0.0;
     if (i < n) { s = (double)a[i]; m = fabs((double)a[i]); }
     write_partials(s, m, partials, gridDim.x);
+}
+
+template<class T>
+__global__ void k_axpy_alpha(int32_t n, T* __restrict__ y, const T* __restrict__ x,
+                             const DevState* st)
+{
+    if (st->done) return;
+    const int32_t i = blockIdx.x*FH_BLOCK + threadIdx.x;
+    if (i < n) y[i] -= (T)st->alpha*x[i];
+}
+
+__global__ void k_sub_sum_abs(int32_t n, const double* __restrict__ Apsi,
+                              const double* __restrict__ source, double* __restrict__ r,
+                              double* partials, const DevState* st)
+{
+    if (st && st->done) return;
+    const int32_t i = blockIdx.x*FH_BLOCK + threadIdx.x;
+    double m = 0.0;
+    if (i < n)
+    {
+        const double v = source[i] - Apsi[i];
+        r[i] = v;
+        m = fabs(v);
+    }
+    write_partials(0.0, m, partials, gridDim.x);
 }
 
 __global__ void k_scale_convert(int32_t n, const double* __restrict__ x, double scale,
@@ -476,6 +600,22 @@ void launch_finalize_smooth(int32_t n, const double* partials, DevState* st,
 {
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(FH_BLOCK), 0, s,
                        reduction_blocks(n), partials, (int32_t)FIN_SMOOTH_RES, st, n, nSweeps);
+}
+
+template<class T>
+void launch_axpy_alpha(int32_t n, T* y, const T* x, const DevState* st, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_axpy_alpha<T>, dim3(reduction_blocks(n)), dim3(FH_BLOCK), 0, s,
+                       n, y, x, st);
+}
+template void launch_axpy_alpha<double>(int32_t, double*, const double*, const DevState*, hipStream_t);
+template void launch_axpy_alpha<float>(int32_t, float*, const float*, const DevState*, hipStream_t);
+
+void launch_sub_sum_abs(int32_t n, const double* Apsi, const double* source, double* r,
+                        double* partials, const DevState* st, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_sub_sum_abs, dim3(reduction_blocks(n)), dim3(FH_BLOCK), 0, s,
+                       n, Apsi, source, r, partials, st);
 }
 
 void launch_scale_convert(int32_t n, const double* x, double scale, float* y, hipStream_t s)

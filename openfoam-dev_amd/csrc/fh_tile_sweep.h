@@ -1,0 +1,43 @@
+// Tiled wavefront sweeps for hex-box meshes (fh_tile_sweep.hip).
+#pragma once
+
+#include "fh_kernels.h"
+
+namespace fh
+{
+
+// tile decomposition of an nx x ny x nz box into bx x by x bz tiles
+struct TileView
+{
+    int32_t nx, ny, nz;
+    int32_t bx, by, bz;
+    int32_t nTI, nTJ, nTK;
+    const int32_t* order;     // tiles in wavefront order (ascending TI+TJ+TK)
+};
+
+struct TileSync
+{
+    unsigned int* flags;               // per tile: launch epoch + 1 when done
+    unsigned long long* ticket;        // monotonic ticket counter
+    unsigned long long ticketBase;     // tickets handed out by earlier launches
+    unsigned long long epoch;          // launches made with this context
+};
+
+// tile depth per kind and precision (LDS budget: the tile's staged inputs)
+template<class T> struct TileBZ;
+template<> struct TileBZ<double> { static constexpr int pre = 8; static constexpr int gs = 4; };
+template<> struct TileBZ<float> { static constexpr int pre = 16; static constexpr int gs = 8; };
+
+template<class T>
+inline int tile_bz(int kind)
+{
+    return (kind == SWEEP_GS_FWD || kind == SWEEP_GS_BWD) ? TileBZ<T>::gs : TileBZ<T>::pre;
+}
+
+// local: block-Jacobi over tiles (each tile on its own, no inter-tile waits)
+template<class T>
+void launch_tile_sweep(int kind, const TileView& G, const RowsView& R,
+                       const SweepData<T>& d, TileSync& sync, DevState* st, hipStream_t s,
+                       bool local = false);
+
+} // namespace fh

@@ -25,7 +25,8 @@ __all__ = [
 ]
 
 FH_SOLVER = {"PCG": 0, "PBiCGStab": 1, "PBiCG": 2, "smoothSolver": 3, "GAMG": 4, "diagonal": 5}
-FH_PRECON = {"none": 0, "diagonal": 1, "DIC": 2, "FDIC": 3, "DILU": 4, "GAMG": 5}
+FH_PRECON = {"none": 0, "diagonal": 1, "DIC": 2, "FDIC": 3, "DILU": 4, "GAMG": 5,
+             "tileDIC": 6, "tileDILU": 7}
 FH_SMOOTH = {"GaussSeidel": 0, "symGaussSeidel": 1, "DIC": 2, "DILU": 3,
              "DICGaussSeidel": 4, "DILUGaussSeidel": 5, "FDIC": 6, "none": 7}
 
@@ -247,6 +248,9 @@ def controls_from_dict(d):
     c.preconditioner = _lookup(lib().fh_preconditioner_lookup, pre)
     if "smoother" in d:
         c.smoother = _lookup(lib().fh_smoother_lookup, d["smoother"])
+    if "innerPreconditioner" in d:
+        c.innerPreconditioner = _lookup(lib().fh_preconditioner_lookup,
+                                        d["innerPreconditioner"])
     ints = ["maxIter", "minIter", "nSweeps", "nPreSweeps", "preSweepsLevelMultiplier",
             "maxPreSweeps", "nPostSweeps", "postSweepsLevelMultiplier", "maxPostSweeps",
             "nFinestSweeps", "nCellsInCoarsestLevel", "mergeLevels", "nVcycles",

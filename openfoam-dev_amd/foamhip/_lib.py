@@ -40,7 +40,7 @@ class FhControls(ctypes.Structure):
         ("nFinestSweeps", _i32), ("scaleCorrection", _i32),
         ("directSolveCoarsest", _i32), ("nCellsInCoarsestLevel", _i32),
         ("mergeLevels", _i32), ("nVcycles", _i32), ("innerRelTol", _dbl),
-        ("maxRefinements", _i32),
+        ("maxRefinements", _i32), ("innerPreconditioner", _i32),
     ]
 
 
@@ -56,7 +56,8 @@ class FhPerf(ctypes.Structure):
 class FhMeshInfo(ctypes.Structure):
     _fields_ = [
         ("nCells", _i32), ("nFaces", _i32), ("nLevels", _i32), ("nParts", _i32),
-        ("boxNx", _i32), ("boxNy", _i32), ("boxNz", _i32), ("deviceBytes", _i64),
+        ("boxNx", _i32), ("boxNy", _i32), ("boxNz", _i32), ("tiled", _i32),
+        ("deviceBytes", _i64),
     ]
 
 

@@ -1,0 +1,213 @@
+"""GAMG (solver and preconditioner), PBiCG and the tile-local DIC on the
+device against the FP64 oracle.
+
+The device agglomerates once per mesh, the oracle on every solve; both use
+the process-global pairing direction, reset before each comparison, and each
+comparison uses a fresh mesh so both see the same weights.  Every sum except
+the global reductions is taken in the reference order, so the trajectories
+agree to rounding: solution relative L2 <= 1e-10, iterations within +-2."""
+import numpy as np
+import pytest
+
+import foamhip as fh
+import oracle as O
+from meshes import asym_coeffs, box, random_graph, spd_coeffs
+from test_gpu_solvers import ITER_TOL, SOLUTION_TOL, convdiff_case, laplacian_case, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def run_pair(c_N, lower, upper, diag, up, lo, b, ctl, tol=SOLUTION_TOL, iter_tol=ITER_TOL):
+    a = O.Addr(c_N, lower, upper)
+    om = O.Matrix(a, diag, up, lo)
+    O.reset_direction()
+    opsi, operf = om.solve(ctl, np.zeros(c_N), b)
+    mesh = fh.lduAddressing(c_N, lower, upper)
+    m = fh.lduMatrix(mesh, diag, up, lo)
+    fh.lib().fh_gamg_reset_direction()
+    psi = np.zeros(c_N)
+    perf = fh.solver_New("p", m, ctl).solve(psi, b)
+    print(ctl, "device", perf.nIterations, perf.finalResidual, "oracle", operf["nIterations"],
+          operf["finalResidual"], "diff", rel_l2(psi, opsi))
+    assert perf.converged == operf["converged"]
+    assert abs(perf.nIterations - operf["nIterations"]) <= iter_tol
+    assert perf.normFactor == pytest.approx(operf["normFactor"], rel=1e-12)
+    assert perf.initialResidual == pytest.approx(operf["initialResidual"], rel=1e-12)
+    assert rel_l2(psi, opsi) <= tol
+    return perf, operf, psi, opsi
+
+
+GAMG = {"solver": "GAMG", "tolerance": 1e-8, "relTol": 0}
+
+
+@pytest.mark.parametrize("smoother", ["symGaussSeidel", "GaussSeidel", "DIC", "FDIC",
+                                      "DICGaussSeidel"])
+def test_gamg_symmetric_smoothers(smoother):
+    c, a, diag, upper, b = laplacian_case(16, sigma=1.0)
+    ctl = dict(GAMG, smoother=smoother)
+    perf, *_ = run_pair(c.N, c.lower, c.upper, diag, upper, None, b, ctl)
+    assert perf.solverName == "GAMG"
+
+
+def test_gamg_fixed_cycles_trajectory():
+    """tolerance 0: exactly maxIter V-cycles on both sides."""
+    c, a, diag, upper, b = laplacian_case(20, sigma=1.0)
+    ctl = dict(GAMG, smoother="symGaussSeidel", tolerance=0, maxIter=6)
+    perf, operf, psi, opsi = run_pair(c.N, c.lower, c.upper, diag, upper, None, b, ctl,
+                                      iter_tol=0)
+    assert perf.nIterations == 6
+    assert perf.finalResidual == pytest.approx(operf["finalResidual"], rel=1e-9)
+
+
+@pytest.mark.parametrize("opts", [
+    {"nPreSweeps": 2},
+    {"scaleCorrection": False},
+    {"mergeLevels": 2},
+    {"nCellsInCoarsestLevel": 40, "nFinestSweeps": 1, "nPostSweeps": 1},
+    {"directSolveCoarsest": True},
+    {"directSolveCoarsest": True, "nCellsInCoarsestLevel": 300},
+])
+def test_gamg_controls(opts):
+    c, a, diag, upper, b = laplacian_case(16, sigma=0.5)
+    ctl = dict(GAMG, smoother="GaussSeidel", **opts)
+    run_pair(c.N, c.lower, c.upper, diag, upper, None, b, ctl)
+
+
+@pytest.mark.parametrize("smoother", ["GaussSeidel", "DILU", "DILUGaussSeidel",
+                                      "symGaussSeidel"])
+def test_gamg_asymmetric(smoother):
+    c, a, diag, upper, lower, b = convdiff_case(14)
+    ctl = dict(GAMG, smoother=smoother, maxIter=200)
+    run_pair(c.N, c.lower, c.upper, diag, upper, lower, b, ctl)
+
+
+def test_gamg_random_graph():
+    rng = np.random.default_rng(7)
+    n, lower, upper = random_graph(3000, 6, rng, shuffle_within_owner=True)
+    diag, up = spd_coeffs(n, lower, upper, rng)
+    b = rng.standard_normal(n)
+    ctl = dict(GAMG, smoother="symGaussSeidel", tolerance=1e-10)
+    run_pair(n, lower, upper, diag, up, None, b, ctl)
+
+
+def test_gamg_preconditioned_krylov():
+    c, a, diag, upper, b = laplacian_case(16, sigma=1.0)
+    ctl = {"solver": "PCG", "preconditioner": "GAMG", "smoother": "GaussSeidel",
+           "tolerance": 1e-9, "relTol": 0}
+    perf, *_ = run_pair(c.N, c.lower, c.upper, diag, upper, None, b, ctl)
+    assert perf.solverName == "GAMGPCG"
+    c, a, diag, upper, lower, b = convdiff_case(12)
+    ctl = {"solver": "PBiCGStab", "preconditioner": "GAMG", "smoother": "GaussSeidel",
+           "tolerance": 1e-9, "relTol": 0}
+    run_pair(c.N, c.lower, c.upper, diag, upper, lower, b, ctl)
+
+
+def test_gamg_mixed_precision():
+    """FP32 V-cycles on the residual equation inside FP64 residual updates:
+    the FP64 convergence test is met, the iteration count stays within two
+    of the FP64 count, and at a tight tolerance the solution matches the
+    FP64 oracle to 1e-8."""
+    c, a, diag, upper, b = laplacian_case(24, sigma=1.0)
+    om = O.Matrix(a, diag, upper)
+    ctl = dict(GAMG, smoother="GaussSeidel")
+    O.reset_direction()
+    opsi, operf = om.solve(ctl, np.zeros(c.N), b)
+    mesh = fh.lduAddressing(c.N, c.lower, c.upper)
+    m = fh.lduMatrix(mesh, diag, upper)
+    fh.lib().fh_gamg_reset_direction()
+    psi = np.zeros(c.N)
+    perf = fh.solver_New("p", m, dict(ctl, precision="mixed")).solve(psi, b)
+    assert perf.solverName == "GAMG(mixed)"
+    assert perf.converged
+    r = b - om.Amul(psi)
+    assert np.abs(r).sum() / perf.normFactor < 1e-8
+    assert abs(perf.nIterations - operf["nIterations"]) <= ITER_TOL
+    tight = dict(ctl, tolerance=1e-13, maxIter=200)
+    O.reset_direction()
+    xpsi, _ = om.solve(tight, np.zeros(c.N), b)
+    psi2 = np.zeros(c.N)
+    perf2 = fh.solver_New("p", m, dict(tight, precision="mixed")).solve(psi2, b)
+    assert perf2.converged
+    assert rel_l2(psi2, xpsi) <= 1e-8
+
+
+def test_gamg_no_coarse_levels_is_an_error():
+    n, lower, upper = box(2, 2, 2)
+    rng = np.random.default_rng(1)
+    diag, up = spd_coeffs(n, lower, upper, rng)
+    mesh = fh.lduAddressing(n, lower, upper)
+    m = fh.lduMatrix(mesh, diag, up)
+    with pytest.raises(fh.FoamHipError, match="No coarse levels"):
+        fh.solver_New("p", m, dict(GAMG, smoother="GaussSeidel")).solve(
+            np.zeros(n), np.ones(n))
+
+
+@pytest.mark.parametrize("precon", ["DILU", "diagonal", "none"])
+def test_pbicg(precon):
+    c, a, diag, upper, lower, b = convdiff_case(14)
+    ctl = {"solver": "PBiCG", "preconditioner": precon, "tolerance": 1e-9, "relTol": 0,
+           "maxIter": 2000}
+    perf, *_ = run_pair(c.N, c.lower, c.upper, diag, upper, lower, b, ctl)
+    assert perf.solverName == precon + "PBiCG"
+
+
+def test_pbicg_random_graph_shuffled():
+    rng = np.random.default_rng(11)
+    n, lower, upper = random_graph(1500, 5, rng, shuffle_within_owner=True)
+    diag, up, lo = asym_coeffs(n, lower, upper, rng)
+    b = rng.standard_normal(n)
+    ctl = {"solver": "PBiCG", "preconditioner": "DILU", "tolerance": 1e-10, "relTol": 0}
+    run_pair(n, lower, upper, diag, up, lo, b, ctl)
+
+
+def _tile_blocks(nx, ny, nz, lower, upper, bz):
+    def tile(c):
+        i = c % nx
+        j = (c // nx) % ny
+        k = c // (nx * ny)
+        return (i // 16, j // 16, k // bz)
+    return np.array([tile(l) != tile(u) for l, u in zip(lower, upper)])
+
+
+@pytest.mark.parametrize("kind", ["DIC", "DILU"])
+def test_tile_local_factor_is_block_jacobi(kind):
+    """tileDIC / tileDILU on a box = the reference DIC / DILU of the matrix
+    with every face between two 16 x 16 x 8 tiles removed (bitwise)."""
+    nx, ny, nz = 37, 20, 19
+    n, lower, upper = box(nx, ny, nz)
+    rng = np.random.default_rng(5)
+    if kind == "DIC":
+        diag, up = spd_coeffs(n, lower, upper, rng)
+        lo = None
+    else:
+        diag, up, lo = asym_coeffs(n, lower, upper, rng)
+    cut = _tile_blocks(nx, ny, nz, lower, upper, 8)
+    assert cut.any() and not cut.all()
+    up_b = np.where(cut, 0.0, up)
+    lo_b = None if lo is None else np.where(cut, 0.0, lo)
+    om = O.Matrix(O.Addr(n, lower, upper), diag, up_b, lo_b)
+    mesh = fh.lduAddressing(n, lower, upper)
+    assert mesh.info()["tiled"] == 1
+    m = fh.lduMatrix(mesh, diag, up, lo)
+    r = rng.standard_normal(n)
+    np.testing.assert_array_equal(m.reciprocalD("tile" + kind), om.reciprocalD(kind))
+    np.testing.assert_array_equal(m.precondition("tile" + kind, r), om.precondition(kind, r))
+
+
+def test_mixed_pcg_tile_dic_inner():
+    """Mixed PCG with the tile-local inner preconditioner meets the FP64
+    residual test and matches the FP64 oracle at a tight tolerance."""
+    c, a, diag, upper, b = laplacian_case(40, sigma=1.0)
+    om = O.Matrix(a, diag, upper)
+    mesh = fh.lduAddressing(c.N, c.lower, c.upper)
+    m = fh.lduMatrix(mesh, diag, upper)
+    for inner in ["tileDIC", "DIC"]:
+        ctl = {"solver": "PCG", "preconditioner": "DIC", "tolerance": 1e-12, "relTol": 0,
+               "maxIter": 3000, "precision": "mixed", "innerPreconditioner": inner}
+        psi = np.zeros(c.N)
+        perf = fh.solver_New("p", m, ctl).solve(psi, b)
+        assert perf.converged
+        r = b - om.Amul(psi)
+        assert np.abs(r).sum() / perf.normFactor < 1e-12
+        opsi, _ = om.solve(dict(ctl, precision="double", tolerance=1e-13), np.zeros(c.N), b)
+        assert rel_l2(psi, opsi) <= 1e-8

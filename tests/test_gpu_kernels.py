@@ -13,19 +13,30 @@ from meshes import asym_coeffs, box, random_graph, spd_coeffs
 pytestmark = pytest.mark.gpu
 
 
-def make(case, rng, kind, column=None):
-    n, lo, up = case
-    old = os.environ.get("FOAMHIP_COLUMN")
-    if column is not None:
-        os.environ["FOAMHIP_COLUMN"] = str(column)
-    try:
-        mesh = fh.lduAddressing(n, lo, up)
-    finally:
-        if column is not None:
-            if old is None:
-                del os.environ["FOAMHIP_COLUMN"]
+class env:
+    """Temporarily set environment variables read at mesh setup."""
+
+    def __init__(self, **kw):
+        self.kw = {k: str(v) for k, v in kw.items() if v is not None}
+
+    def __enter__(self):
+        self.old = {k: os.environ.get(k) for k in self.kw}
+        os.environ.update(self.kw)
+
+    def __exit__(self, *a):
+        for k, v in self.old.items():
+            if v is None:
+                del os.environ[k]
             else:
-                os.environ["FOAMHIP_COLUMN"] = old
+                os.environ[k] = v
+
+
+def make(case, rng, kind, column=None):
+    """column=None: default schedule (3-D tiles on boxes); column=T: the
+    level-partitioned sweep with T x T columns."""
+    n, lo, up = case
+    with env(FOAMHIP_COLUMN=column, FOAMHIP_SWEEP="levels" if column else None):
+        mesh = fh.lduAddressing(n, lo, up)
     a = O.Addr(n, lo, up)
     if kind == "spd":
         d, u = spd_coeffs(n, lo, up, rng)
@@ -36,6 +47,8 @@ def make(case, rng, kind, column=None):
 
 CASES = [
     ("box8", lambda r: box(8, 8, 8), None),
+    ("box_tiles", lambda r: box(37, 20, 19), None),
+    ("box_tiles_flat", lambda r: box(16, 48, 3), None),
     ("box_parts", lambda r: box(20, 18, 6), 4),
     ("box_parts_wide", lambda r: box(33, 17, 9), 8),
     ("rand", lambda r: random_graph(500, 6, r), None),
@@ -49,12 +62,13 @@ def test_box_schedule_detected():
     info = mesh.info()
     assert (info["boxNx"], info["boxNy"], info["boxNz"]) == (20, 18, 6)
     assert info["nLevels"] == 20 + 18 + 6 - 2
-    os.environ["FOAMHIP_COLUMN"] = "4"
-    try:
+    assert info["tiled"] == 1
+    with env(FOAMHIP_COLUMN=4, FOAMHIP_SWEEP="levels"):
         info = fh.lduAddressing(*box(20, 18, 6)).info()
-    finally:
-        del os.environ["FOAMHIP_COLUMN"]
-    assert info["nParts"] == 5 * 5
+    assert info["nParts"] == 5 * 5 and info["tiled"] == 0
+    # not a box: one part, level schedule
+    info = fh.lduAddressing(*random_graph(50, 4, np.random.default_rng(0))).info()
+    assert info["tiled"] == 0 and info["boxNx"] == 0
 
 
 @pytest.mark.parametrize("name,gen,col", CASES)
@@ -105,12 +119,19 @@ def test_smoothers_bitwise(name, gen, col):
                                               err_msg=f"{s} x{nS}")
 
 
-def test_pipelined_sweep_repeatable():
-    """The partitioned sweep is deterministic: same bits on every run."""
+@pytest.mark.parametrize("column", [None, 4])
+def test_pipelined_sweep_repeatable(column):
+    """The tiled and the partitioned sweeps are deterministic: the same bits
+    on every run, whatever the order tiles are scheduled in."""
     rng = np.random.default_rng(23)
-    mesh, m, om = make(box(40, 40, 10), rng, "spd", 4)
-    assert mesh.info()["nParts"] == 100
+    mesh, m, om = make(box(40, 40, 10), rng, "spd", column)
+    if column:
+        assert mesh.info()["nParts"] == 100
     r = rng.standard_normal(mesh.nCells)
     ref = om.precondition("DIC", r)
     for _ in range(5):
         np.testing.assert_array_equal(m.precondition("DIC", r), ref)
+    x0 = rng.standard_normal(mesh.nCells)
+    ref = om.smooth("symGaussSeidel", x0, r, 2)
+    for _ in range(3):
+        np.testing.assert_array_equal(m.smooth("symGaussSeidel", x0, r, 2), ref)
