@@ -152,6 +152,21 @@ void Foam::hipLduSolver::readControls(const dictionary& dict, fh_controls& c)
     c.mergeLevels = dict.lookupOrDefault<label>("mergeLevels", 1);
     c.nVcycles = dict.lookupOrDefault<label>("nVcycles", 2);
 
+    // the device agglomerates with algebraic pair weights only
+    // (algebraicPairGAMGAgglomeration.C:47-65); the reference default,
+    // faceAreaPair (GAMGAgglomeration.C:312), needs face areas and is
+    // replaced by algebraicPair when no agglomerator is named
+    if (dict.found("agglomerator"))
+    {
+        const word agg(dict.lookup("agglomerator"));
+        if (agg != "algebraicPair")
+        {
+            FatalIOErrorInFunction(dict)
+                << "Unsupported agglomerator " << agg << "; valid: algebraicPair"
+                << exit(FatalIOError);
+        }
+    }
+
     const word precision(dict.lookupOrDefault<word>("precision", "double"));
     if (precision == "mixed")
     {

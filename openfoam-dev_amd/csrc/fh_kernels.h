@@ -96,6 +96,18 @@ struct SweepData
     const int32_t* perm; // PRECOND_BWD_T: library row -> original cell
 };
 
+// Synchronisation context of the sync-free row sweep
+struct RowSync
+{
+    unsigned int* flags;               // per row: launch epoch + 1 when done
+    unsigned long long* ticket;        // monotonic chunk ticket counter
+    unsigned long long ticketBase;     // tickets handed out by earlier launches
+    unsigned long long epoch;          // launches made with this context
+};
+
+// persistent workgroups of one row-sweep launch
+constexpr int32_t kRowSweepBlocks = 512;
+
 // ---- kernels: sizes in rows --------------------------------------------------
 template<class T>
 void launch_gather_values(const RowsView& R, int64_t nnzPad, const double* upperF,
@@ -128,6 +140,9 @@ void launch_sum_a(const RowsView& R, const double* diag, const double* val,
 template<class T>
 void launch_sweep(int kind, const RowsView& R, const SchedView& S, SweepSync& sync,
                   const SweepData<T>& d, DevState* st, hipStream_t s);
+template<class T>
+void launch_sweep_rows(int kind, const RowsView& R, const SweepData<T>& d, RowSync& sync,
+                       DevState* st, hipStream_t s);
 template<class T>
 void launch_reciprocal(int32_t n, T* a, hipStream_t s);
 

@@ -257,7 +257,23 @@ __device__ __forceinline__ void st_progress(unsigned long long* p, unsigned long
     __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-template<class T, int KIND>
+// value loads / stores of the sweeps: plain inside one workgroup's fenced
+// levels, device-coherent in the sync-free row sweep
+template<bool COH, class T>
+__device__ __forceinline__ T sw_ld(const T* p)
+{
+    if constexpr (COH) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else return *p;
+}
+
+template<bool COH, class T>
+__device__ __forceinline__ void sw_st(T* p, T v)
+{
+    if constexpr (COH) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else *p = v;
+}
+
+template<class T, int KIND, bool COH = false>
 __device__ __forceinline__ void sweep_row(const RowsView& R, const SweepData<T>& d, int32_t r)
 {
     const int32_t info = R.rowInfo[r];
@@ -272,9 +288,9 @@ __device__ __forceinline__ void sweep_row(const RowsView& R, const SweepData<T>&
         for (int32_t k = 0; k < nLo; k++)
         {
             const int32_t e = base + (k << 6);
-            w -= rDr*d.val[e]*d.w[R.col[e]];
+            w -= rDr*d.val[e]*sw_ld<COH>(&d.w[R.col[e]]);
         }
-        d.w[r] = w;
+        sw_st<COH>(&d.w[r], w);
     }
     else if (KIND == SWEEP_PRECOND_BWD)
     {
@@ -284,9 +300,9 @@ __device__ __forceinline__ void sweep_row(const RowsView& R, const SweepData<T>&
         for (int32_t k = nEnt - 1; k >= nLo; k--)
         {
             const int32_t e = base + (k << 6);
-            w -= rDr*d.val[e]*d.w[R.col[e]];
+            w -= rDr*d.val[e]*sw_ld<COH>(&d.w[R.col[e]]);
         }
-        d.w[r] = w;
+        sw_st<COH>(&d.w[r], w);
     }
     else if (KIND == SWEEP_PRECOND_BWD_T)
     {
@@ -305,9 +321,9 @@ __device__ __forceinline__ void sweep_row(const RowsView& R, const SweepData<T>&
                 if (nat < lastNat && nat > bestNat) { bestNat = nat; best = e; }
             }
             lastNat = bestNat;
-            w -= rDr*d.val[best]*d.w[R.col[best]];
+            w -= rDr*d.val[best]*sw_ld<COH>(&d.w[R.col[best]]);
         }
-        d.w[r] = w;
+        sw_st<COH>(&d.w[r], w);
     }
     else if (KIND == SWEEP_RD_DIC)
     {
@@ -317,9 +333,9 @@ __device__ __forceinline__ void sweep_row(const RowsView& R, const SweepData<T>&
         {
             const int32_t e = base + (k << 6);
             const T u = d.val[e];
-            v -= u*u/d.w[R.col[e]];
+            v -= u*u/sw_ld<COH>(&d.w[R.col[e]]);
         }
-        d.w[r] = v;
+        sw_st<COH>(&d.w[r], v);
     }
     else if (KIND == SWEEP_RD_DILU)
     {
@@ -329,9 +345,9 @@ __device__ __forceinline__ void sweep_row(const RowsView& R, const SweepData<T>&
         {
             const int32_t e = base + (k << 6);
             const T up = (T)d.upperF[R.face[e] >> 1];
-            v -= up*d.val[e]/d.w[R.col[e]];
+            v -= up*d.val[e]/sw_ld<COH>(&d.w[R.col[e]]);
         }
-        d.w[r] = v;
+        sw_st<COH>(&d.w[r], v);
     }
     else if (KIND == SWEEP_GS_FWD || KIND == SWEEP_GS_BWD)
     {
@@ -342,9 +358,9 @@ __device__ __forceinline__ void sweep_row(const RowsView& R, const SweepData<T>&
         for (int32_t k = 0; k < nEnt; k++)
         {
             const int32_t e = base + (k << 6);
-            t -= d.val[e]*d.w[R.col[e]];
+            t -= d.val[e]*sw_ld<COH>(&d.w[R.col[e]]);
         }
-        d.w[r] = t/d.diag[r];
+        sw_st<COH>(&d.w[r], t/d.diag[r]);
     }
 }
 
@@ -446,6 +462,133 @@ void k_sweep_parts(RowsView R, SchedView S, SweepSync sync, SweepData<T> d,
     if (threadIdx.x == 0) st_progress(&progress[p], base + K - 1);
     if (expired && st) st->scheduleError = 1;
 }
+
+// --------------------------------------------------------------------------
+// Sync-free row sweep (meshes without a tile or part structure, e.g. GAMG
+// coarse levels).  The library numbering of such a mesh is sorted by
+// dependency level, so every row depends only on rows of lower (forward) or
+// higher (backward) library index.  Persistent workgroups take chunks of
+// 256 rows through a ticket counter in sweep order; a lane owns one row and
+// computes it as soon as the completion flags of the rows it reads carry
+// this launch's value (epoch + 1), publishing its own flag after its value
+// store has completed.  A workgroup only ever waits on chunks ticketed
+// before its own, whose holders are running, and the lowest pending row
+// always has its inputs complete, so the sweep cannot deadlock; every wait
+// is still bounded.  Row arithmetic is sweep_row's, so results are bitwise
+// those of the level sweep.
+// --------------------------------------------------------------------------
+template<int KIND> struct RowDeps
+{
+    // forward kinds read their lower entries, backward kinds their upper
+    static constexpr bool lower = KIND == SWEEP_PRECOND_FWD || KIND == SWEEP_RD_DIC
+                               || KIND == SWEEP_RD_DILU || KIND == SWEEP_GS_FWD;
+};
+
+template<class T, int KIND, bool BACKWARD>
+__global__ __launch_bounds__(FH_BLOCK)
+void k_sweep_rows(RowsView R, SweepData<T> d, unsigned int* flags, unsigned long long* ticket,
+                  unsigned long long ticketBase, unsigned int doneVal, int32_t nChunks,
+                  DevState* st)
+{
+    // a finished solve still takes its tickets, so that the counter stays
+    // in step with the host's ticketBase
+    const bool stop = st && st->done;
+    __shared__ int32_t s_chunk;
+    for (;;)
+    {
+        if (threadIdx.x == 0)
+        {
+            const unsigned long long t = atomicAdd(ticket, 1ull) - ticketBase;
+            s_chunk = t < (unsigned long long)nChunks ? (int32_t)t : -1;
+        }
+        __syncthreads();
+        const int32_t chunk = s_chunk;
+        __syncthreads();
+        if (chunk < 0) return;
+        const int32_t idx = chunk*FH_BLOCK + (int32_t)threadIdx.x;
+        const bool valid = idx < R.n;
+        const int32_t r = BACKWARD ? R.n - 1 - idx : idx;
+        int32_t k = 0, kEnd = 0;
+        if (valid)
+        {
+            const int32_t info = R.rowInfo[r];
+            const int32_t nLo = info & 0xffff, nEnt = info >> 16;
+            k = RowDeps<KIND>::lower ? 0 : nLo;
+            kEnd = RowDeps<KIND>::lower ? nLo : nEnt;
+        }
+        const int32_t base = valid ? sell_base(R.sliceStart, r) : 0;
+        bool pending = valid && !stop;
+        uint32_t spins = 0;
+        while (__ballot(pending))
+        {
+            bool fresh = false;
+            if (pending)
+            {
+                // dependencies seen complete stay complete: resume at k
+                while (k < kEnd
+                       && __hip_atomic_load(&flags[R.col[base + (k << 6)]], __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT) >= doneVal)
+                    k++;
+                if (k == kEnd)
+                {
+                    sweep_row<T, KIND, true>(R, d, r);
+                    pending = false;
+                    fresh = true;
+                }
+            }
+            // the row value is stored before its flag is published
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            if (fresh)
+                __hip_atomic_store(&flags[r], doneVal, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (pending)
+            {
+                if (++spins > FH_SPIN_LIMIT)
+                {
+                    if (st) st->scheduleError = 1;
+                    pending = false;
+                }
+                else
+                {
+                    __builtin_amdgcn_s_sleep(2);
+                }
+            }
+        }
+    }
+}
+
+template<class T, int KIND, bool BACKWARD>
+static void launch_rows_kind(const RowsView& R, const SweepData<T>& d, RowSync& sync,
+                             DevState* st, hipStream_t s)
+{
+    const int32_t nChunks = (R.n + FH_BLOCK - 1)/FH_BLOCK;
+    const int32_t grid = nChunks < kRowSweepBlocks ? nChunks : kRowSweepBlocks;
+    const unsigned int doneVal = (unsigned int)(sync.epoch + 1);
+    hipLaunchKernelGGL((k_sweep_rows<T, KIND, BACKWARD>), dim3(grid), dim3(FH_BLOCK), 0, s,
+                       R, d, sync.flags, sync.ticket, sync.ticketBase, doneVal, nChunks, st);
+    // every workgroup takes one ticket past the last chunk
+    sync.ticketBase += (unsigned long long)nChunks + (unsigned long long)grid;
+    sync.epoch++;
+}
+
+template<class T>
+void launch_sweep_rows(int kind, const RowsView& R, const SweepData<T>& d, RowSync& sync,
+                       DevState* st, hipStream_t s)
+{
+    switch (kind)
+    {
+        case SWEEP_PRECOND_FWD:   launch_rows_kind<T, SWEEP_PRECOND_FWD, false>(R, d, sync, st, s); break;
+        case SWEEP_PRECOND_BWD:   launch_rows_kind<T, SWEEP_PRECOND_BWD, true>(R, d, sync, st, s); break;
+        case SWEEP_PRECOND_BWD_T: launch_rows_kind<T, SWEEP_PRECOND_BWD_T, true>(R, d, sync, st, s); break;
+        case SWEEP_RD_DIC:        launch_rows_kind<T, SWEEP_RD_DIC, false>(R, d, sync, st, s); break;
+        case SWEEP_RD_DILU:       launch_rows_kind<T, SWEEP_RD_DILU, false>(R, d, sync, st, s); break;
+        case SWEEP_GS_FWD:        launch_rows_kind<T, SWEEP_GS_FWD, false>(R, d, sync, st, s); break;
+        case SWEEP_GS_BWD:        launch_rows_kind<T, SWEEP_GS_BWD, true>(R, d, sync, st, s); break;
+        default: fail(FH_ERR_INVALID, "unknown sweep kind %d", kind);
+    }
+}
+
+template void launch_sweep_rows<double>(int, const RowsView&, const SweepData<double>&, RowSync&, DevState*, hipStream_t);
+template void launch_sweep_rows<float>(int, const RowsView&, const SweepData<float>&, RowSync&, DevState*, hipStream_t);
 
 template<class T, int KIND, bool BACKWARD>
 static void launch_sweep_kind(const RowsView& R, const SchedView& S, SweepSync& sync,

@@ -42,6 +42,11 @@ struct fh_matrix
     // pipelined sweep synchronisation
     fh::DBuf<unsigned long long> d_sync;  // [0] ticket, [1..nParts] progress
     unsigned long long epoch = 0;
+    // sync-free row sweeps (meshes with one part)
+    fh::DBuf<unsigned int> d_rowFlags;
+    fh::DBuf<unsigned long long> d_rowTicket;
+    unsigned long long rowTicketBase = 0, rowEpoch = 0;
+    bool rowSweep = false;
     // tiled sweeps (hex boxes)
     fh::DBuf<unsigned int> d_tileFlags;
     fh::DBuf<unsigned long long> d_tileTicket;
@@ -108,6 +113,16 @@ struct fh_matrix
         s.ticket = d_tileTicket.p;
         s.ticketBase = tileTicketBase;
         s.epoch = tileEpoch;
+        return s;
+    }
+
+    fh::RowSync row_sync()
+    {
+        fh::RowSync s;
+        s.flags = d_rowFlags.p;
+        s.ticket = d_rowTicket.p;
+        s.ticketBase = rowTicketBase;
+        s.epoch = rowEpoch;
         return s;
     }
 

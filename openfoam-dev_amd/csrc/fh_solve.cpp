@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 namespace fh
@@ -35,6 +36,17 @@ void matrix_init(fh_matrix* M, fh_mesh* mesh, hipStream_t shared)
     M->d_partials.alloc(2*reduction_partials(mesh->nCells) + 64);
     M->d_sync.alloc(1 + (size_t)mesh->sched.nParts);
     FH_HIP(hipMemsetAsync(M->d_sync.p, 0, M->d_sync.bytes(), M->stream));
+    // one-part meshes sweep row by row unless the level sweep is forced
+    const char* sw = std::getenv("FOAMHIP_SWEEP");
+    M->rowSweep = !mesh->tiled && mesh->sched.nParts == 1
+               && !(sw && std::strcmp(sw, "parts") == 0);
+    if (M->rowSweep)
+    {
+        M->d_rowFlags.alloc(mesh->nCells);
+        M->d_rowTicket.alloc(1);
+        FH_HIP(hipMemsetAsync(M->d_rowFlags.p, 0, M->d_rowFlags.bytes(), M->stream));
+        FH_HIP(hipMemsetAsync(M->d_rowTicket.p, 0, M->d_rowTicket.bytes(), M->stream));
+    }
     if (mesh->tiled)
     {
         M->d_tileFlags.alloc(std::max(mesh->maxTiles, 1));
@@ -164,6 +176,14 @@ static void sweep_any(fh_matrix* M, int kind, const SweepData<T>& d, DevState* s
                              M->stream, local);
         M->tileTicketBase = ts.ticketBase;
         M->tileEpoch = ts.epoch;
+        return;
+    }
+    if (M->rowSweep)
+    {
+        RowSync rs = M->row_sync();
+        launch_sweep_rows<T>(kind, M->rows(), d, rs, st, M->stream);
+        M->rowTicketBase = rs.ticketBase;
+        M->rowEpoch = rs.epoch;
         return;
     }
     SweepSync sy = M->sync();

@@ -76,7 +76,9 @@ def test_gamg_controls(opts):
 @pytest.mark.parametrize("smoother", ["GaussSeidel", "DILU", "DILUGaussSeidel",
                                       "symGaussSeidel"])
 def test_gamg_asymmetric(smoother):
-    c, a, diag, upper, lower, b = convdiff_case(14)
+    # diffusivity 1e-2: the coarse Galerkin operators of the upwind
+    # operator at 1e-3 make GAMG diverge in the reference arithmetic too
+    c, a, diag, upper, lower, b = convdiff_case(14, D=1e-2)
     ctl = dict(GAMG, smoother=smoother, maxIter=200)
     run_pair(c.N, c.lower, c.upper, diag, upper, lower, b, ctl)
 
@@ -96,7 +98,7 @@ def test_gamg_preconditioned_krylov():
            "tolerance": 1e-9, "relTol": 0}
     perf, *_ = run_pair(c.N, c.lower, c.upper, diag, upper, None, b, ctl)
     assert perf.solverName == "GAMGPCG"
-    c, a, diag, upper, lower, b = convdiff_case(12)
+    c, a, diag, upper, lower, b = convdiff_case(12, D=1e-2)
     ctl = {"solver": "PBiCGStab", "preconditioner": "GAMG", "smoother": "GaussSeidel",
            "tolerance": 1e-9, "relTol": 0}
     run_pair(c.N, c.lower, c.upper, diag, upper, lower, b, ctl)
@@ -144,7 +146,10 @@ def test_gamg_no_coarse_levels_is_an_error():
 
 @pytest.mark.parametrize("precon", ["DILU", "diagonal", "none"])
 def test_pbicg(precon):
-    c, a, diag, upper, lower, b = convdiff_case(14)
+    # at diffusivity 1e-3 the reference's own PBiCG iteration count moves by
+    # up to 30 under a 1e-15 relative change of b (no preconditioner), so
+    # the +-2 bar is applied where the iteration is not chaotic
+    c, a, diag, upper, lower, b = convdiff_case(14, D=1e-2)
     ctl = {"solver": "PBiCG", "preconditioner": precon, "tolerance": 1e-9, "relTol": 0,
            "maxIter": 2000}
     perf, *_ = run_pair(c.N, c.lower, c.upper, diag, upper, lower, b, ctl)

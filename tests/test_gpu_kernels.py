@@ -32,17 +32,23 @@ class env:
 
 
 def make(case, rng, kind, column=None):
-    """column=None: default schedule (3-D tiles on boxes); column=T: the
-    level-partitioned sweep with T x T columns."""
+    """column=None: default schedule (3-D tiles on boxes, the sync-free row
+    sweep elsewhere); column=T: the level-partitioned sweep with T x T
+    columns; column="parts": the one-part level sweep instead of the row
+    sweep."""
     n, lo, up = case
-    with env(FOAMHIP_COLUMN=column, FOAMHIP_SWEEP="levels" if column else None):
+    if column == "parts":
+        e = env(FOAMHIP_SWEEP="parts")
+    else:
+        e = env(FOAMHIP_COLUMN=column, FOAMHIP_SWEEP="levels" if column else None)
+    with e:
         mesh = fh.lduAddressing(n, lo, up)
-    a = O.Addr(n, lo, up)
-    if kind == "spd":
-        d, u = spd_coeffs(n, lo, up, rng)
-        return mesh, fh.lduMatrix(mesh, d, u), O.Matrix(a, d, u)
-    d, u, l = asym_coeffs(n, lo, up, rng)
-    return mesh, fh.lduMatrix(mesh, d, u, l), O.Matrix(a, d, u, l)
+        a = O.Addr(n, lo, up)
+        if kind == "spd":
+            d, u = spd_coeffs(n, lo, up, rng)
+            return mesh, fh.lduMatrix(mesh, d, u), O.Matrix(a, d, u)
+        d, u, l = asym_coeffs(n, lo, up, rng)
+        return mesh, fh.lduMatrix(mesh, d, u, l), O.Matrix(a, d, u, l)
 
 
 CASES = [
@@ -53,6 +59,8 @@ CASES = [
     ("box_parts_wide", lambda r: box(33, 17, 9), 8),
     ("rand", lambda r: random_graph(500, 6, r), None),
     ("rand_shuffled", lambda r: random_graph(300, 5, r, shuffle_within_owner=True), None),
+    ("rand_parts", lambda r: random_graph(500, 6, r), "parts"),
+    ("rand_many_chunks", lambda r: random_graph(140000, 6, r, shuffle_within_owner=True), None),
     ("chain", lambda r: box(70, 1, 1), None),
 ]
 
@@ -131,6 +139,24 @@ def test_pipelined_sweep_repeatable(column):
     ref = om.precondition("DIC", r)
     for _ in range(5):
         np.testing.assert_array_equal(m.precondition("DIC", r), ref)
+    x0 = rng.standard_normal(mesh.nCells)
+    ref = om.smooth("symGaussSeidel", x0, r, 2)
+    for _ in range(3):
+        np.testing.assert_array_equal(m.smooth("symGaussSeidel", x0, r, 2), ref)
+
+
+def test_row_sweep_repeatable():
+    """The sync-free row sweep (persistent workgroups, per-row flags) gives
+    the same bits on every run and equals the one-part level sweep."""
+    rng = np.random.default_rng(24)
+    case = random_graph(200000, 6, rng)
+    mesh, m, om = make(case, rng, "asym")
+    _, mp, _ = make(case, np.random.default_rng(24), "asym", "parts")
+    assert mesh.info()["nParts"] == 1
+    r = rng.standard_normal(mesh.nCells)
+    ref = om.precondition("DILU", r)
+    for _ in range(4):
+        np.testing.assert_array_equal(m.precondition("DILU", r), ref)
     x0 = rng.standard_normal(mesh.nCells)
     ref = om.smooth("symGaussSeidel", x0, r, 2)
     for _ in range(3):
