@@ -404,6 +404,7 @@ int fh_reciprocal_d(fh_matrix* M, int32_t precon, double* rD)
         const double* d = ensure_rD(M, which);
         DBuf<double> nat;
         from_lib(M, d, nat, rD);
+        check_sweep_errors(M);
     });
 }
 
@@ -420,6 +421,7 @@ static int precondition_impl(fh_matrix* M, int32_t precon, const double* rA, dou
         to_lib(M, rA, nat, r.p);
         apply_precond<double>(M, precon, r.p, w.p, nullptr, nullptr, transpose);
         from_lib(M, w.p, nat, wA);
+        check_sweep_errors(M);
     });
 }
 
@@ -445,6 +447,7 @@ int fh_smooth(fh_matrix* M, int32_t smoother, double* psi, const double* source,
         to_lib(M, source, nat, b.p);
         smooth_device(M, smoother, x.p, b.p, nSweeps, nullptr);
         from_lib(M, x.p, nat, psi);
+        check_sweep_errors(M);
     });
 }
 

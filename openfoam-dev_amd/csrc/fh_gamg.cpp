@@ -519,6 +519,13 @@ void gamg_release(fh_matrix* M)
     M->gamg = nullptr;
 }
 
+void gamg_check_sweep_errors(fh_matrix* M)
+{
+    GamgHierarchy* H = static_cast<GamgHierarchy*>(M->gamg);
+    if (!H) return;
+    for (fh_matrix* A : H->mats) check_sweep_errors(A);
+}
+
 bool gamg_forward() { return g_forward; }
 void gamg_set_forward(bool f) { g_forward = f; }
 

@@ -19,6 +19,8 @@ void gamg_apply(fh_matrix* M, const T* r, T* w, DevState* st);
 // GAMG solver (FP64, or FP32 V-cycles inside FP64 residual updates)
 void gamg_solve(fh_matrix* M, const fh_controls* c, double* psi, const double* src,
                 fh_perf* perf, bool mixed);
+// FH_ERR_SCHEDULE when a sweep wait of a coarse level expired
+void gamg_check_sweep_errors(fh_matrix* M);
 // pairGAMGAgglomeration::forward_ access (process-global)
 bool gamg_forward();
 void gamg_set_forward(bool f);

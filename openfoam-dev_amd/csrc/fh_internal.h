@@ -144,6 +144,12 @@ struct fh_mesh
     int32_t maxTiles = 0;
     hipStream_t setupStream = nullptr;
 
+    // chunks of the sync-free sweep (one-part meshes whose levels are wide
+    // enough): rows of one level, at most 64 per chunk
+    bool chunkSweep = false;
+    int32_t nChunks = 0;
+    fh::DBuf<int32_t> d_chunkStart, d_fwdOff, d_fwdDeps, d_bwdOff, d_bwdDeps;
+
     // GAMG agglomeration cached with the mesh (built on the first GAMG
     // solve, as GAMGAgglomeration is cached on the mesh database)
     std::shared_ptr<void> gamgAgg;

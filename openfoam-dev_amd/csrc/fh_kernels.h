@@ -96,17 +96,31 @@ struct SweepData
     const int32_t* perm; // PRECOND_BWD_T: library row -> original cell
 };
 
-// Synchronisation context of the sync-free row sweep
+// Chunks of the sync-free sweep: rows of one level, at most 64 per chunk,
+// with the chunks each one reads in the forward / backward direction
+struct ChunkView
+{
+    int32_t nChunks;
+    const int32_t* start;      // [nChunks+1] library rows
+    const int32_t* fwdOff;     // [nChunks+1]
+    const int32_t* fwdDeps;    // chunks holding lower neighbours
+    const int32_t* bwdOff;
+    const int32_t* bwdDeps;    // chunks holding upper neighbours
+};
+
+// Synchronisation context of the chunk sweep
 struct RowSync
 {
-    unsigned int* flags;               // per row: launch epoch + 1 when done
+    unsigned int* flags;               // per chunk: launch epoch + 1 when done
     unsigned long long* ticket;        // monotonic chunk ticket counter
     unsigned long long ticketBase;     // tickets handed out by earlier launches
     unsigned long long epoch;          // launches made with this context
+    unsigned long long waitTicks;      // wall-clock bound of one wait
+    unsigned int* err;                 // set when a wait expired
 };
 
-// persistent workgroups of one row-sweep launch
-constexpr int32_t kRowSweepBlocks = 512;
+// persistent workgroups (4 wavefronts each) of one chunk-sweep launch
+constexpr int32_t kRowSweepBlocks = 256;
 
 // ---- kernels: sizes in rows --------------------------------------------------
 template<class T>
@@ -141,8 +155,8 @@ template<class T>
 void launch_sweep(int kind, const RowsView& R, const SchedView& S, SweepSync& sync,
                   const SweepData<T>& d, DevState* st, hipStream_t s);
 template<class T>
-void launch_sweep_rows(int kind, const RowsView& R, const SweepData<T>& d, RowSync& sync,
-                       DevState* st, hipStream_t s);
+void launch_sweep_rows(int kind, const RowsView& R, const SweepData<T>& d, const ChunkView& C,
+                       RowSync& sync, DevState* st, hipStream_t s);
 template<class T>
 void launch_reciprocal(int32_t n, T* a, hipStream_t s);
 

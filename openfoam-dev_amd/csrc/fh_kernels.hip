@@ -464,131 +464,113 @@ void k_sweep_parts(RowsView R, SchedView S, SweepSync sync, SweepData<T> d,
 }
 
 // --------------------------------------------------------------------------
-// Sync-free row sweep (meshes without a tile or part structure, e.g. GAMG
+// Sync-free chunk sweep (meshes without a tile or part structure, e.g. GAMG
 // coarse levels).  The library numbering of such a mesh is sorted by
-// dependency level, so every row depends only on rows of lower (forward) or
-// higher (backward) library index.  Persistent workgroups take chunks of
-// 256 rows through a ticket counter in sweep order; a lane owns one row and
-// computes it as soon as the completion flags of the rows it reads carry
-// this launch's value (epoch + 1), publishing its own flag after its value
-// store has completed.  A workgroup only ever waits on chunks ticketed
-// before its own, whose holders are running, and the lowest pending row
-// always has its inputs complete, so the sweep cannot deadlock; every wait
-// is still bounded.  Row arithmetic is sweep_row's, so results are bitwise
-// those of the level sweep.
+// dependency level; each level's rows are cut into chunks of at most 64,
+// so a chunk's rows are independent and every row it reads in the sweep
+// direction lies in an earlier chunk.  Persistent wavefronts take chunks
+// through a ticket counter in sweep order, wait until the completion flags
+// of the chunk's dependency chunks (precomputed per mesh) carry this
+// launch's value (epoch + 1), compute one row per lane with sweep_row's
+// arithmetic, and publish the chunk flag after the row stores completed.
+// A wave only waits on chunks ticketed before its own, whose holders are
+// running, so the sweep cannot deadlock; every wait is bounded in time
+// and an expired wait is reported (and the chunk still published, so that
+// the launch drains).
 // --------------------------------------------------------------------------
-template<int KIND> struct RowDeps
-{
-    // forward kinds read their lower entries, backward kinds their upper
-    static constexpr bool lower = KIND == SWEEP_PRECOND_FWD || KIND == SWEEP_RD_DIC
-                               || KIND == SWEEP_RD_DILU || KIND == SWEEP_GS_FWD;
-};
-
 template<class T, int KIND, bool BACKWARD>
 __global__ __launch_bounds__(FH_BLOCK)
-void k_sweep_rows(RowsView R, SweepData<T> d, unsigned int* flags, unsigned long long* ticket,
-                  unsigned long long ticketBase, unsigned int doneVal, int32_t nChunks,
-                  DevState* st)
+void k_sweep_chunks(RowsView R, SweepData<T> d, ChunkView C, RowSync sync,
+                    unsigned int doneVal, DevState* st)
 {
     // a finished solve still takes its tickets, so that the counter stays
     // in step with the host's ticketBase
     const bool stop = st && st->done;
-    __shared__ int32_t s_chunk;
+    const int lane = (int)(threadIdx.x & 63);
+    const int32_t* depOff = BACKWARD ? C.bwdOff : C.fwdOff;
+    const int32_t* deps = BACKWARD ? C.bwdDeps : C.fwdDeps;
     for (;;)
     {
-        if (threadIdx.x == 0)
+        unsigned long long t = 0;
+        if (lane == 0) t = atomicAdd(sync.ticket, 1ull) - sync.ticketBase;
+        t = __shfl(t, 0, 64);
+        if (t >= (unsigned long long)C.nChunks) return;
+        if (stop) continue;
+        const int32_t c = BACKWARD ? C.nChunks - 1 - (int32_t)t : (int32_t)t;
+        const int32_t q0 = depOff[c], q1 = depOff[c + 1];
+        bool expired = false;
+        if (q1 > q0)
         {
-            const unsigned long long t = atomicAdd(ticket, 1ull) - ticketBase;
-            s_chunk = t < (unsigned long long)nChunks ? (int32_t)t : -1;
-        }
-        __syncthreads();
-        const int32_t chunk = s_chunk;
-        __syncthreads();
-        if (chunk < 0) return;
-        const int32_t idx = chunk*FH_BLOCK + (int32_t)threadIdx.x;
-        const bool valid = idx < R.n;
-        const int32_t r = BACKWARD ? R.n - 1 - idx : idx;
-        int32_t k = 0, kEnd = 0;
-        if (valid)
-        {
-            const int32_t info = R.rowInfo[r];
-            const int32_t nLo = info & 0xffff, nEnt = info >> 16;
-            k = RowDeps<KIND>::lower ? 0 : nLo;
-            kEnd = RowDeps<KIND>::lower ? nLo : nEnt;
-        }
-        const int32_t base = valid ? sell_base(R.sliceStart, r) : 0;
-        bool pending = valid && !stop;
-        uint32_t spins = 0;
-        while (__ballot(pending))
-        {
-            bool fresh = false;
-            if (pending)
+            const unsigned long long t0 = wall_clock64();
+            for (;;)
             {
-                // dependencies seen complete stay complete: resume at k
-                while (k < kEnd
-                       && __hip_atomic_load(&flags[R.col[base + (k << 6)]], __ATOMIC_RELAXED,
-                                            __HIP_MEMORY_SCOPE_AGENT) >= doneVal)
-                    k++;
-                if (k == kEnd)
+                bool mine = true;
+                for (int32_t q = q0 + lane; q < q1; q += 64)
+                    if (__hip_atomic_load(&sync.flags[deps[q]], __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT) < doneVal)
+                    {
+                        mine = false;
+                        break;
+                    }
+                if (__all(mine)) break;
+                if ((unsigned long long)(wall_clock64() - t0) > sync.waitTicks)
                 {
-                    sweep_row<T, KIND, true>(R, d, r);
-                    pending = false;
-                    fresh = true;
+                    expired = true;
+                    break;
                 }
+                __builtin_amdgcn_s_sleep(1);
             }
-            // the row value is stored before its flag is published
-            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-            if (fresh)
-                __hip_atomic_store(&flags[r], doneVal, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (pending)
+        }
+        const int32_t r = C.start[c] + lane;
+        if (!expired && r < C.start[c + 1]) sweep_row<T, KIND, true>(R, d, r);
+        // the chunk's row stores complete before its flag is published
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        if (lane == 0)
+        {
+            if (expired)
             {
-                if (++spins > FH_SPIN_LIMIT)
-                {
-                    if (st) st->scheduleError = 1;
-                    pending = false;
-                }
-                else
-                {
-                    __builtin_amdgcn_s_sleep(2);
-                }
+                __hip_atomic_store(sync.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (st) st->scheduleError = 1;
             }
+            __hip_atomic_store(&sync.flags[c], doneVal, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
         }
     }
 }
 
 template<class T, int KIND, bool BACKWARD>
-static void launch_rows_kind(const RowsView& R, const SweepData<T>& d, RowSync& sync,
-                             DevState* st, hipStream_t s)
+static void launch_rows_kind(const RowsView& R, const SweepData<T>& d, const ChunkView& C,
+                             RowSync& sync, DevState* st, hipStream_t s)
 {
-    const int32_t nChunks = (R.n + FH_BLOCK - 1)/FH_BLOCK;
-    const int32_t grid = nChunks < kRowSweepBlocks ? nChunks : kRowSweepBlocks;
+    const int32_t wantBlocks = (C.nChunks + 3)/4;
+    const int32_t grid = wantBlocks < kRowSweepBlocks ? wantBlocks : kRowSweepBlocks;
     const unsigned int doneVal = (unsigned int)(sync.epoch + 1);
-    hipLaunchKernelGGL((k_sweep_rows<T, KIND, BACKWARD>), dim3(grid), dim3(FH_BLOCK), 0, s,
-                       R, d, sync.flags, sync.ticket, sync.ticketBase, doneVal, nChunks, st);
-    // every workgroup takes one ticket past the last chunk
-    sync.ticketBase += (unsigned long long)nChunks + (unsigned long long)grid;
+    hipLaunchKernelGGL((k_sweep_chunks<T, KIND, BACKWARD>), dim3(grid), dim3(FH_BLOCK), 0, s,
+                       R, d, C, sync, doneVal, st);
+    // every wavefront takes one ticket past the last chunk
+    sync.ticketBase += (unsigned long long)C.nChunks + 4ull*(unsigned long long)grid;
     sync.epoch++;
 }
 
 template<class T>
-void launch_sweep_rows(int kind, const RowsView& R, const SweepData<T>& d, RowSync& sync,
-                       DevState* st, hipStream_t s)
+void launch_sweep_rows(int kind, const RowsView& R, const SweepData<T>& d, const ChunkView& C,
+                       RowSync& sync, DevState* st, hipStream_t s)
 {
     switch (kind)
     {
-        case SWEEP_PRECOND_FWD:   launch_rows_kind<T, SWEEP_PRECOND_FWD, false>(R, d, sync, st, s); break;
-        case SWEEP_PRECOND_BWD:   launch_rows_kind<T, SWEEP_PRECOND_BWD, true>(R, d, sync, st, s); break;
-        case SWEEP_PRECOND_BWD_T: launch_rows_kind<T, SWEEP_PRECOND_BWD_T, true>(R, d, sync, st, s); break;
-        case SWEEP_RD_DIC:        launch_rows_kind<T, SWEEP_RD_DIC, false>(R, d, sync, st, s); break;
-        case SWEEP_RD_DILU:       launch_rows_kind<T, SWEEP_RD_DILU, false>(R, d, sync, st, s); break;
-        case SWEEP_GS_FWD:        launch_rows_kind<T, SWEEP_GS_FWD, false>(R, d, sync, st, s); break;
-        case SWEEP_GS_BWD:        launch_rows_kind<T, SWEEP_GS_BWD, true>(R, d, sync, st, s); break;
+        case SWEEP_PRECOND_FWD:   launch_rows_kind<T, SWEEP_PRECOND_FWD, false>(R, d, C, sync, st, s); break;
+        case SWEEP_PRECOND_BWD:   launch_rows_kind<T, SWEEP_PRECOND_BWD, true>(R, d, C, sync, st, s); break;
+        case SWEEP_PRECOND_BWD_T: launch_rows_kind<T, SWEEP_PRECOND_BWD_T, true>(R, d, C, sync, st, s); break;
+        case SWEEP_RD_DIC:        launch_rows_kind<T, SWEEP_RD_DIC, false>(R, d, C, sync, st, s); break;
+        case SWEEP_RD_DILU:       launch_rows_kind<T, SWEEP_RD_DILU, false>(R, d, C, sync, st, s); break;
+        case SWEEP_GS_FWD:        launch_rows_kind<T, SWEEP_GS_FWD, false>(R, d, C, sync, st, s); break;
+        case SWEEP_GS_BWD:        launch_rows_kind<T, SWEEP_GS_BWD, true>(R, d, C, sync, st, s); break;
         default: fail(FH_ERR_INVALID, "unknown sweep kind %d", kind);
     }
 }
 
-template void launch_sweep_rows<double>(int, const RowsView&, const SweepData<double>&, RowSync&, DevState*, hipStream_t);
-template void launch_sweep_rows<float>(int, const RowsView&, const SweepData<float>&, RowSync&, DevState*, hipStream_t);
+template void launch_sweep_rows<double>(int, const RowsView&, const SweepData<double>&, const ChunkView&, RowSync&, DevState*, hipStream_t);
+template void launch_sweep_rows<float>(int, const RowsView&, const SweepData<float>&, const ChunkView&, RowSync&, DevState*, hipStream_t);
 
 template<class T, int KIND, bool BACKWARD>
 static void launch_sweep_kind(const RowsView& R, const SchedView& S, SweepSync& sync,

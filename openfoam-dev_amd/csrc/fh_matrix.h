@@ -31,6 +31,10 @@ struct fh_matrix
     // FP32 copies for the mixed-precision inner solves
     fh::DBuf<float> d_diag32, d_val32;
     bool f32Valid = false;
+    // FP32 direction-packed coefficients of hex boxes (tile-local sweeps):
+    // lower K, J, I and upper I, J, K per cell
+    fh::DBuf<float> d_dir32[6];
+    bool dir32Valid = false;
 
     // preconditioner / smoother factors (library numbering)
     // index: 0 DIC/FDIC, 1 DILU, 2 diagonal, 3 tile-local DIC, 4 tile-local DILU
@@ -42,10 +46,11 @@ struct fh_matrix
     // pipelined sweep synchronisation
     fh::DBuf<unsigned long long> d_sync;  // [0] ticket, [1..nParts] progress
     unsigned long long epoch = 0;
-    // sync-free row sweeps (meshes with one part)
-    fh::DBuf<unsigned int> d_rowFlags;
+    // sync-free chunk sweeps (one-part meshes with wide levels)
+    fh::DBuf<unsigned int> d_rowFlags;     // per chunk
     fh::DBuf<unsigned long long> d_rowTicket;
-    unsigned long long rowTicketBase = 0, rowEpoch = 0;
+    fh::DBuf<unsigned int> d_rowErr;
+    unsigned long long rowTicketBase = 0, rowEpoch = 0, rowWaitTicks = 0;
     bool rowSweep = false;
     // tiled sweeps (hex boxes)
     fh::DBuf<unsigned int> d_tileFlags;
@@ -123,7 +128,21 @@ struct fh_matrix
         s.ticket = d_rowTicket.p;
         s.ticketBase = rowTicketBase;
         s.epoch = rowEpoch;
+        s.waitTicks = rowWaitTicks;
+        s.err = d_rowErr.p;
         return s;
+    }
+
+    fh::ChunkView chunks() const
+    {
+        fh::ChunkView c;
+        c.nChunks = mesh->nChunks;
+        c.start = mesh->d_chunkStart.p;
+        c.fwdOff = mesh->d_fwdOff.p;
+        c.fwdDeps = mesh->d_fwdDeps.p;
+        c.bwdOff = mesh->d_bwdOff.p;
+        c.bwdDeps = mesh->d_bwdDeps.p;
+        return c;
     }
 
     fh::SweepSync sync()
@@ -180,6 +199,8 @@ template<class T>
 void pcg_loop(fh_matrix* M, int precon, T* psi, T* r, T* p, T* w, int32_t maxIter);
 const char* precon_name(int p);
 void fill_perf(fh_matrix* M, fh_perf* perf, const char* name);
+// FH_ERR_SCHEDULE when a chunk-sweep wait expired since the last check
+void check_sweep_errors(fh_matrix* M);
 
 // batched enqueue of device iterations with a poll of DevState.done: the
 // batch doubles up to 32 iterations between host polls

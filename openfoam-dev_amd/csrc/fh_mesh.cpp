@@ -306,6 +306,61 @@ void build_mesh(fh_mesh* m, int32_t nCells, int32_t nFaces,
                     fail(FH_ERR_INVALID, "internal: sweep order is not topological");
     }
 
+    // chunks of the sync-free sweep: used when the levels hold 16 rows or
+    // more on average (narrower dependency graphs, e.g. chains, are walked
+    // faster by the one-workgroup level sweep)
+    if (!m->tiled && nParts == 1 && (int64_t)nLevels*16 <= (int64_t)nCells)
+    {
+        std::vector<int32_t> start, chunkOf(nCells);
+        for (size_t q = 0; q + 1 < S.segStart.size(); q++)
+            for (int32_t r = S.segStart[q]; r < S.segStart[q + 1]; r += 64)
+                start.push_back(r);
+        start.push_back(nCells);
+        const int32_t nCh = (int32_t)start.size() - 1;
+        for (int32_t c = 0; c < nCh; c++)
+            for (int32_t r = start[c]; r < start[c + 1]; r++) chunkOf[r] = c;
+        std::vector<int32_t> fOff(nCh + 1, 0), bOff(nCh + 1, 0), fDeps, bDeps, tmpF, tmpB;
+        for (int32_t c = 0; c < nCh; c++)
+        {
+            tmpF.clear();
+            tmpB.clear();
+            for (int32_t r = start[c]; r < start[c + 1]; r++)
+            {
+                const int32_t nLo = m->rowInfo[r] & 0xffff, nEnt = m->rowInfo[r] >> 16;
+                const int32_t base = m->sliceStart[r >> 6] + (r & 63);
+                for (int32_t k = 0; k < nEnt; k++)
+                {
+                    const int32_t oc = chunkOf[m->col[base + 64*k]];
+                    if (k < nLo)
+                    {
+                        if (oc >= c) fail(FH_ERR_INVALID, "internal: chunk order is not topological");
+                        tmpF.push_back(oc);
+                    }
+                    else
+                    {
+                        if (oc <= c) fail(FH_ERR_INVALID, "internal: chunk order is not topological");
+                        tmpB.push_back(oc);
+                    }
+                }
+            }
+            std::sort(tmpF.begin(), tmpF.end());
+            tmpF.erase(std::unique(tmpF.begin(), tmpF.end()), tmpF.end());
+            std::sort(tmpB.begin(), tmpB.end());
+            tmpB.erase(std::unique(tmpB.begin(), tmpB.end()), tmpB.end());
+            fDeps.insert(fDeps.end(), tmpF.begin(), tmpF.end());
+            bDeps.insert(bDeps.end(), tmpB.begin(), tmpB.end());
+            fOff[c + 1] = (int32_t)fDeps.size();
+            bOff[c + 1] = (int32_t)bDeps.size();
+        }
+        m->chunkSweep = true;
+        m->nChunks = nCh;
+        m->d_chunkStart.upload(start, m->setupStream);
+        m->d_fwdOff.upload(fOff, m->setupStream);
+        m->d_fwdDeps.upload(fDeps, m->setupStream);
+        m->d_bwdOff.upload(bOff, m->setupStream);
+        m->d_bwdDeps.upload(bDeps, m->setupStream);
+    }
+
     // tile shapes and their wavefront order (ascending TI+TJ+TK)
     if (m->tiled)
     {
@@ -361,7 +416,9 @@ size_t mesh_device_bytes(const fh_mesh* m)
          + m->d_face.bytes() + m->d_perm.bytes() + m->d_lower.bytes()
          + m->d_upper.bytes() + S.d_partLevelOff.bytes() + S.d_partMinLevel.bytes()
          + S.d_segStart.bytes() + S.d_upOff.bytes() + S.d_upList.bytes()
-         + S.d_downOff.bytes() + S.d_downList.bytes() + S.d_order.bytes();
+         + S.d_downOff.bytes() + S.d_downList.bytes() + S.d_order.bytes()
+         + m->d_chunkStart.bytes() + m->d_fwdOff.bytes() + m->d_fwdDeps.bytes()
+         + m->d_bwdOff.bytes() + m->d_bwdDeps.bytes();
 }
 
 } // namespace fh

@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 
 namespace fh
 {
@@ -38,14 +39,20 @@ void matrix_init(fh_matrix* M, fh_mesh* mesh, hipStream_t shared)
     FH_HIP(hipMemsetAsync(M->d_sync.p, 0, M->d_sync.bytes(), M->stream));
     // one-part meshes sweep row by row unless the level sweep is forced
     const char* sw = std::getenv("FOAMHIP_SWEEP");
-    M->rowSweep = !mesh->tiled && mesh->sched.nParts == 1
-               && !(sw && std::strcmp(sw, "parts") == 0);
+    M->rowSweep = mesh->chunkSweep && !(sw && std::strcmp(sw, "parts") == 0);
     if (M->rowSweep)
     {
-        M->d_rowFlags.alloc(mesh->nCells);
+        M->d_rowFlags.alloc(std::max(mesh->nChunks, 1));
         M->d_rowTicket.alloc(1);
+        M->d_rowErr.alloc(1);
         FH_HIP(hipMemsetAsync(M->d_rowFlags.p, 0, M->d_rowFlags.bytes(), M->stream));
         FH_HIP(hipMemsetAsync(M->d_rowTicket.p, 0, M->d_rowTicket.bytes(), M->stream));
+        FH_HIP(hipMemsetAsync(M->d_rowErr.p, 0, M->d_rowErr.bytes(), M->stream));
+        // a legitimate wait lasts at most one sweep; 4 s of the device's
+        // wall clock (kHz rate) marks a broken schedule
+        int khz = 0;
+        FH_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, mesh->device));
+        M->rowWaitTicks = 4000ull*(unsigned long long)(khz > 0 ? khz : 100000);
     }
     if (mesh->tiled)
     {
@@ -123,7 +130,7 @@ void matrix_coeffs_updated(fh_matrix* M)
         launch_gather_values<double>(M->rows(), mesh->nnzPad, M->d_upperF.p, lo, M->d_val.p, s);
     FH_HIP(hipStreamSynchronize(s));
     M->hasCoeffs = true;
-    M->valTValid = M->sumAValid = M->f32Valid = false;
+    M->valTValid = M->sumAValid = M->f32Valid = M->dir32Valid = false;
     for (int i = 0; i < 5; i++) M->rDValid[i] = M->rD32Valid[i] = false;
     gamg_invalidate(M);
 }
@@ -181,7 +188,7 @@ static void sweep_any(fh_matrix* M, int kind, const SweepData<T>& d, DevState* s
     if (M->rowSweep)
     {
         RowSync rs = M->row_sync();
-        launch_sweep_rows<T>(kind, M->rows(), d, rs, st, M->stream);
+        launch_sweep_rows<T>(kind, M->rows(), d, M->chunks(), rs, st, M->stream);
         M->rowTicketBase = rs.ticketBase;
         M->rowEpoch = rs.epoch;
         return;
@@ -276,6 +283,22 @@ static int factor_kind(int precon)
     }
 }
 
+static void ensure_dir32(fh_matrix* M)
+{
+    if (M->dir32Valid) return;
+    ensure_f32(M);
+    const int32_t N = M->mesh->nCells;
+    float* dst[6];
+    for (int d = 0; d < 6; d++)
+    {
+        M->d_dir32[d].alloc(N);
+        dst[d] = M->d_dir32[d].p;
+    }
+    launch_pack_dir_coefs(M->tile_view(TileBZ<float>::pre), M->rows(), M->d_val32.p, dst,
+                          M->stream);
+    M->dir32Valid = true;
+}
+
 template<class T> static const T* rD_for(fh_matrix* M, int which);
 template<> const double* rD_for<double>(fh_matrix* M, int which) { return ensure_rD(M, which); }
 template<> const float* rD_for<float>(fh_matrix* M, int which) { return ensure_rD32(M, which); }
@@ -347,6 +370,21 @@ void apply_precond(fh_matrix* M, int precon, const T* r, T* w, double* partials,
     d.perm = M->mesh->d_perm.p;
     const bool local = which >= 3 && M->mesh->tiled;
     const bool dilu = which == 1 || which == 4;
+    if constexpr (std::is_same<T, float>::value)
+    {
+        if (local)
+        {
+            // bandwidth path of the FP32 tile-local factor
+            ensure_dir32(M);
+            const TileView G = M->tile_view(TileBZ<float>::pre);
+            launch_tile_local_f32(false, G, r, d.rD, M->d_dir32[0].p, M->d_dir32[1].p,
+                                  M->d_dir32[2].p, w, st, M->stream);
+            launch_tile_local_f32(true, G, w, d.rD, M->d_dir32[5].p, M->d_dir32[4].p,
+                                  M->d_dir32[3].p, w, st, M->stream);
+            if (partials) launch_dot<T, T>(N, w, r, partials, st, M->stream);
+            return;
+        }
+    }
     sweep_t<T>(M, SWEEP_PRECOND_FWD, d, st, local);
     sweep_t<T>(M, transpose && dilu ? SWEEP_PRECOND_BWD_T : SWEEP_PRECOND_BWD, d, st, local);
     if (partials) launch_dot<T, T>(N, w, r, partials, st, M->stream);
@@ -460,6 +498,20 @@ void fill_perf(fh_matrix* M, fh_perf* perf, const char* name)
     perf->converged = h->converged;
     perf->singular = h->singular;
     perf->normFactor = h->normFactor;
+}
+
+void check_sweep_errors(fh_matrix* M)
+{
+    if (!M->rowSweep) return;
+    unsigned int e = 0;
+    FH_HIP(hipMemcpyAsync(&e, M->d_rowErr.p, sizeof(e), hipMemcpyDeviceToHost, M->stream));
+    FH_HIP(hipStreamSynchronize(M->stream));
+    if (e)
+    {
+        FH_HIP(hipMemsetAsync(M->d_rowErr.p, 0, sizeof(e), M->stream));
+        FH_HIP(hipStreamSynchronize(M->stream));
+        fail(FH_ERR_SCHEDULE, "a sweep dependency wait expired");
+    }
 }
 
 const char* precon_name(int p)
@@ -905,6 +957,8 @@ void solve_device(fh_matrix* M, const fh_controls* c, double* d_psiNat,
     float ms = 0.f;
     FH_HIP(hipEventElapsedTime(&ms, M->evStart, M->evStop));
     perf->solveMilliseconds = ms;
+    check_sweep_errors(M);
+    gamg_check_sweep_errors(M);
     if (M->h_state->scheduleError)
         fail(FH_ERR_SCHEDULE, "a sweep dependency wait expired");
     if (M->h_state->numericError)
@@ -931,8 +985,10 @@ static void profile_t(fh_matrix* M, int kernel, int precon, int reps, double* ms
     const int which = factor_kind(precon);
     if ((kernel == FH_KERNEL_SWEEP_FORWARD || kernel == FH_KERNEL_SWEEP_BACKWARD)
         && (which < 0 || which == 2))
-        fail(FH_ERR_INVALID, "sweep profiling needs DIC, FDIC or DILU");
+        fail(FH_ERR_INVALID, "sweep profiling needs DIC, FDIC, DILU, tileDIC or tileDILU");
     const T* rD = which >= 0 ? rD_for<T>(M, which) : nullptr;
+    const bool local = which >= 3 && M->mesh->tiled;
+    if (local && std::is_same<T, float>::value) ensure_dir32(M);
     SweepData<T> d{};
     d.val = val; d.rD = rD; d.x = x.p; d.w = y.p;
     auto once = [&]()
@@ -943,11 +999,29 @@ static void profile_t(fh_matrix* M, int kernel, int precon, int reps, double* ms
                 launch_amul<T>(M->rows(), diag, val, x.p, y.p, nullptr, M->stream);
                 break;
             case FH_KERNEL_SWEEP_FORWARD:
-                sweep_t<T>(M, SWEEP_PRECOND_FWD, d, nullptr);
-                break;
             case FH_KERNEL_SWEEP_BACKWARD:
-                sweep_t<T>(M, SWEEP_PRECOND_BWD, d, nullptr);
+            {
+                const bool bwd = kernel == FH_KERNEL_SWEEP_BACKWARD;
+                if constexpr (std::is_same<T, float>::value)
+                {
+                    if (local)
+                    {
+                        // the kernels apply_precond runs for FP32 tile-local factors
+                        const TileView G = M->tile_view(TileBZ<float>::pre);
+                        if (bwd)
+                            launch_tile_local_f32(true, G, y.p, rD, M->d_dir32[5].p,
+                                                  M->d_dir32[4].p, M->d_dir32[3].p, y.p,
+                                                  nullptr, M->stream);
+                        else
+                            launch_tile_local_f32(false, G, x.p, rD, M->d_dir32[0].p,
+                                                  M->d_dir32[1].p, M->d_dir32[2].p, y.p,
+                                                  nullptr, M->stream);
+                        break;
+                    }
+                }
+                sweep_t<T>(M, bwd ? SWEEP_PRECOND_BWD : SWEEP_PRECOND_FWD, d, nullptr, local);
                 break;
+            }
             case FH_KERNEL_AXPY_NORM:
                 launch_pcg_update_nostate<T>(N, x.p, y.p, z.p, z.p, M->d_partials.p, M->stream);
                 break;
@@ -970,7 +1044,10 @@ static void profile_t(fh_matrix* M, int kernel, int precon, int reps, double* ms
             *bytes = N*(3*w + 4) + 2.0*F*(w + 4); break;
         case FH_KERNEL_SWEEP_FORWARD: // input, rD, output, row info + one triangle
         case FH_KERNEL_SWEEP_BACKWARD:
-            *bytes = N*(3*w + 4) + 1.0*F*(w + 4); break;
+            // the tile-local FP32 kernels read packed coefficients: input,
+            // rD, output and one coefficient per face, no rows or columns
+            *bytes = (local && w == 4) ? N*3*w + 1.0*F*w : N*(3*w + 4) + 1.0*F*(w + 4);
+            break;
         case FH_KERNEL_AXPY_NORM:     // psi, r read+write, p, w read
             *bytes = 6.0*N*w; break;
     }

@@ -1,0 +1,189 @@
+// Tile-local (block-Jacobi over 16 x 16 x bz tiles) DIC / DILU application
+// in FP32 for the inner iterations of the mixed-precision solvers on hex
+// boxes.  Arithmetic as in the tiled sweep's LOCAL mode; the data path is
+// built for bandwidth: the coefficients of each direction are packed per
+// cell (no row indirection) and every lane reads and writes whole i-rows of
+// 16 contiguous cells.
+#include "fh_tile_sweep.h"
+
+namespace fh
+{
+
+#define TL_B 256
+
+// Per-cell coefficient gather: for each cell of the box, the coefficient of
+// its lower (K, J, I) and upper (I, J, K) neighbour entries from the SELL
+// values; 0 where the neighbour does not exist.
+__global__ void k_pack_dir_coefs(int32_t nx, int32_t ny, int32_t nz, RowsView R,
+                                 const float* __restrict__ val, float* __restrict__ lK,
+                                 float* __restrict__ lJ, float* __restrict__ lI,
+                                 float* __restrict__ uI, float* __restrict__ uJ,
+                                 float* __restrict__ uK)
+{
+    const int64_t c = (int64_t)blockIdx.x*TL_B + threadIdx.x;
+    const int64_t n = (int64_t)nx*ny*nz;
+    if (c >= n) return;
+    const int32_t gi = (int32_t)(c % nx), gj = (int32_t)((c/nx) % ny), gk = (int32_t)(c/((int64_t)nx*ny));
+    const int32_t base = R.sliceStart[c >> 6] + (int32_t)(c & 63);
+    int e = 0;
+    float vK = 0.f, vJ = 0.f, vI = 0.f;
+    if (gk > 0) vK = val[base + ((e++) << 6)];
+    if (gj > 0) vJ = val[base + ((e++) << 6)];
+    if (gi > 0) vI = val[base + ((e++) << 6)];
+    float wI = 0.f, wJ = 0.f, wK = 0.f;
+    if (gi < nx - 1) wI = val[base + ((e++) << 6)];
+    if (gj < ny - 1) wJ = val[base + ((e++) << 6)];
+    if (gk < nz - 1) wK = val[base + ((e++) << 6)];
+    lK[c] = vK; lJ[c] = vJ; lI[c] = vI;
+    uI[c] = wI; uJ[c] = wJ; uK[c] = wK;
+}
+
+void launch_pack_dir_coefs(const TileView& G, const RowsView& R, const float* val,
+                           float* const dst[6], hipStream_t s)
+{
+    const int64_t n = (int64_t)G.nx*G.ny*G.nz;
+    hipLaunchKernelGGL(k_pack_dir_coefs, dim3((unsigned)((n + TL_B - 1)/TL_B)), dim3(TL_B), 0,
+                       s, G.nx, G.ny, G.nz, R, val, dst[0], dst[1], dst[2], dst[3], dst[4],
+                       dst[5]);
+}
+
+// Row walk of a tile: a lane owns one i-row (j, k) of 16 cells and updates
+// them in order along i (the i-neighbour in a register), the rows of one
+// diagonal j + k run together, and a row reads the rows (j-1, k) and
+// (j, k-1) of the previous diagonal from LDS.  Every row is read and written
+// as 16 contiguous values.  One wavefront walks four tiles (16 lanes each),
+// so its barrier is the wave's own.  Forward: w = rD*x - rD*(cK*w(k-1) +
+// cJ*w(j-1) + cI*w(i-1)); backward (w in place): w -= rD*(cK*w(k+1) +
+// cJ*w(j+1) + cI*w(i+1)) -- the reference's per-row order either way.
+template<bool BWD, int BZ>
+__global__ __launch_bounds__(64)
+void k_tile_local_f32(TileView G, const float* a, const float* __restrict__ rD,
+                      const float* __restrict__ cK, const float* __restrict__ cJ,
+                      const float* __restrict__ cI, float* w, const DevState* st)
+{
+    if (st && st->done) return;
+    __shared__ float buf[4][2][BZ][16];          // per tile: diagonal rows by k slot
+    const int lane = threadIdx.x;
+    const int g = lane >> 4, l16 = lane & 15;
+    const int nTiles = G.nTI*G.nTJ*G.nTK;
+    const int tile = blockIdx.x*4 + g;
+    const bool tileOk = tile < nTiles;
+    const int t = tileOk ? tile : 0;
+    const int TI = t % G.nTI, TJ = (t/G.nTI) % G.nTJ, TK = t/(G.nTI*G.nTJ);
+    const int x0 = TI*16, y0 = TJ*16, z0 = TK*BZ;
+    const int ex = min(16, G.nx - x0), ey = min(16, G.ny - y0), ez = min(BZ, G.nz - z0);
+    const int64_t sy = G.nx, sz = (int64_t)G.nx*G.ny;
+    const int64_t base = x0 + sy*y0 + sz*z0;
+    const int nDiag = ey + ez - 1;
+    // whole 16-wide rows starting on 16-byte boundaries: vector access
+    const bool vec = (G.nx & 3) == 0 && ex == 16;
+    // the wave walks the longest diagonal count of its tiles
+    int nD = tileOk ? nDiag : 0;
+    nD = max(nD, __shfl_xor(nD, 16, 64));
+    nD = max(nD, __shfl_xor(nD, 32, 64));
+    for (int step = 0; step < nD; step++)
+    {
+        const int d = BWD ? nDiag - 1 - step : step;
+        // rows of diagonal d: k = kmin + l16, j = d - k
+        const int kmin = max(0, d - (ey - 1));
+        const int k = kmin + l16;
+        const int j = d - k;
+        const bool act = tileOk && step < nDiag && k < ez && j >= 0 && j < ey;
+        float out[16];
+        if (act)
+        {
+            const int64_t row = base + sy*j + sz*k;
+            // the row's inputs: 16 contiguous values per array (4-wide
+            // vector loads when every row starts 16-byte aligned)
+            float A[16], D[16], CK[16], CJ[16], CI[16];
+            if (vec)
+            {
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                {
+                    const float4 qa = reinterpret_cast<const float4*>(a + row)[v];
+                    const float4 qd = reinterpret_cast<const float4*>(rD + row)[v];
+                    const float4 qk = reinterpret_cast<const float4*>(cK + row)[v];
+                    const float4 qj = reinterpret_cast<const float4*>(cJ + row)[v];
+                    const float4 qi = reinterpret_cast<const float4*>(cI + row)[v];
+                    A[4*v] = qa.x; A[4*v + 1] = qa.y; A[4*v + 2] = qa.z; A[4*v + 3] = qa.w;
+                    D[4*v] = qd.x; D[4*v + 1] = qd.y; D[4*v + 2] = qd.z; D[4*v + 3] = qd.w;
+                    CK[4*v] = qk.x; CK[4*v + 1] = qk.y; CK[4*v + 2] = qk.z; CK[4*v + 3] = qk.w;
+                    CJ[4*v] = qj.x; CJ[4*v + 1] = qj.y; CJ[4*v + 2] = qj.z; CJ[4*v + 3] = qj.w;
+                    CI[4*v] = qi.x; CI[4*v + 1] = qi.y; CI[4*v + 2] = qi.z; CI[4*v + 3] = qi.w;
+                }
+            }
+            else
+            {
+#pragma unroll
+                for (int i = 0; i < 16; i++)
+                {
+                    const bool in = i < ex;
+                    A[i] = in ? a[row + i] : 0.f;
+                    D[i] = in ? rD[row + i] : 0.f;
+                    CK[i] = in ? cK[row + i] : 0.f;
+                    CJ[i] = in ? cJ[row + i] : 0.f;
+                    CI[i] = in ? cI[row + i] : 0.f;
+                }
+            }
+            const float* nb = buf[g][(step + 1) & 1][0];
+            const bool hasK = BWD ? k < ez - 1 : k > 0;
+            const bool hasJ = BWD ? j < ey - 1 : j > 0;
+            // slots of the neighbour rows on the previous diagonal: (j -/+ 1, k)
+            // keeps k, (j, k -/+ 1) moves one slot
+            const int sK = BWD ? k + 1 : k - 1;
+            float prev = 0.f;
+#pragma unroll
+            for (int q = 0; q < 16; q++)
+            {
+                const int i = BWD ? 15 - q : q;
+                out[i] = 0.f;
+                if (i < ex)
+                {
+                    const float Di = D[i];
+                    float v = BWD ? A[i] : Di*A[i];
+                    if (hasK) v -= Di*CK[i]*nb[sK*16 + i];
+                    if (hasJ) v -= Di*CJ[i]*nb[k*16 + i];
+                    if (BWD ? i < ex - 1 : i > 0) v -= Di*CI[i]*prev;
+                    prev = v;
+                    out[i] = v;
+                }
+            }
+            if (vec)
+            {
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    reinterpret_cast<float4*>(w + row)[v] =
+                        make_float4(out[4*v], out[4*v + 1], out[4*v + 2], out[4*v + 3]);
+            }
+            else
+            {
+#pragma unroll
+                for (int i = 0; i < 16; i++)
+                    if (i < ex) w[row + i] = out[i];
+            }
+            float* mine = buf[g][step & 1][k];
+#pragma unroll
+            for (int i = 0; i < 16; i++) mine[i] = out[i];
+        }
+        __syncthreads();
+    }
+}
+
+void launch_tile_local_f32(bool backward, const TileView& G, const float* a, const float* rD,
+                           const float* cK, const float* cJ, const float* cI, float* w,
+                           const DevState* st, hipStream_t s)
+{
+    if (G.bx != 16 || G.by != 16 || G.bz != TileBZ<float>::pre)
+        fail(FH_ERR_INVALID, "tile shape does not match the compiled local kernels");
+    const int nTiles = G.nTI*G.nTJ*G.nTK;
+    const dim3 grid((nTiles + 3)/4);
+    if (backward)
+        hipLaunchKernelGGL((k_tile_local_f32<true, TileBZ<float>::pre>), grid, dim3(64), 0, s,
+                           G, a, rD, cK, cJ, cI, w, st);
+    else
+        hipLaunchKernelGGL((k_tile_local_f32<false, TileBZ<float>::pre>), grid, dim3(64), 0, s,
+                           G, a, rD, cK, cJ, cI, w, st);
+}
+
+} // namespace fh

@@ -40,4 +40,14 @@ void launch_tile_sweep(int kind, const TileView& G, const RowsView& R,
                        const SweepData<T>& d, TileSync& sync, DevState* st, hipStream_t s,
                        bool local = false);
 
+// FP32 tile-local DIC / DILU on direction-packed coefficients (mixed inner
+// preconditioner): dst[0..5] = lower K, J, I and upper I, J, K per cell
+void launch_pack_dir_coefs(const TileView& G, const RowsView& R, const float* val,
+                           float* const dst[6], hipStream_t s);
+// forward: a = r, backward: a = w (in place); cK, cJ, cI the coefficients of
+// the pass (lower forward, upper backward)
+void launch_tile_local_f32(bool backward, const TileView& G, const float* a, const float* rD,
+                           const float* cK, const float* cJ, const float* cI, float* w,
+                           const DevState* st, hipStream_t s);
+
 } // namespace fh

@@ -216,3 +216,17 @@ def test_mixed_pcg_tile_dic_inner():
         assert np.abs(r).sum() / perf.normFactor < 1e-12
         opsi, _ = om.solve(dict(ctl, precision="double", tolerance=1e-13), np.zeros(c.N), b)
         assert rel_l2(psi, opsi) <= 1e-8
+        # iteration count of a working FP32 inner preconditioner: exact DIC
+        # follows the FP64 count; the FP32 tile-local factor (16-deep tiles)
+        # lies between global DIC and the FP64 tile-local factor (8-deep
+        # tiles, smaller blocks)
+        its = {}
+        for pre in ["DIC", "tileDIC"]:
+            p64 = fh.solver_New("p", m, dict(ctl, precision="double", preconditioner=pre)).solve(
+                np.zeros(c.N), b)
+            its[pre] = p64.nIterations
+        print(inner, "mixed its", perf.nIterations, "fp64 its", its)
+        if inner == "DIC":
+            assert abs(perf.nIterations - its["DIC"]) <= max(3, 0.02 * its["DIC"])
+        else:
+            assert its["DIC"] - 3 <= perf.nIterations <= its["tileDIC"] + 3
