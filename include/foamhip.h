@@ -278,6 +278,66 @@ int fh_grad_gauss_linear(fh_mesh* mesh, const fh_boundary* boundary, const doubl
                          const double* d_bSf, const double* d_bMagSf, const double* d_bPhi,
                          const double* d_bSnGrad, double* d_grad, double* d_bGrad);
 
+/* ------------------------------------------------------------------------
+   Serving: a request loop for many independent solves.  No counterpart in
+   the reference (which solves one system per call); it runs the fh_solve
+   path above on several HIP streams at once.  Requests are copied at
+   submission, admitted in arrival order as a solve slot and device memory
+   (an estimate of the request's footprint against memoryBudget) become
+   free, and solved on resident matrices kept per mesh, so meshes, factors
+   and GAMG hierarchies stay in HBM between requests.  A result is the same
+   bits whether the request ran alone or beside others.
+   ------------------------------------------------------------------------ */
+typedef struct fh_server fh_server;
+
+typedef struct fh_server_config
+{
+    int32_t maxConcurrent;      /* solves in flight, one stream each (default 8) */
+    int32_t borrowInputs;       /* 0: copy the request arrays at submission (default);
+                                   1: use them in place -- the caller keeps them valid
+                                   and unchanged until fh_server_wait returns the result */
+    int64_t memoryBudget;       /* device bytes of the resident matrices (default 32 GiB) */
+} fh_server_config;
+
+typedef struct fh_request
+{
+    int64_t id;                 /* caller's request id (unique among pending ones) */
+    fh_mesh* mesh;              /* caller-owned, outlives the request */
+    const double* diag;         /* host arrays (copied unless borrowInputs) */
+    const double* upper;
+    const double* lower;        /* NULL: symmetric */
+    const double* source;
+    const double* psi0;         /* NULL: zero initial guess */
+    fh_controls controls;
+} fh_request;
+
+typedef struct fh_result
+{
+    int64_t id;
+    int32_t status;             /* FH_OK or the solve's error status */
+    int32_t nCells;
+    fh_perf perf;
+    double submitTime;          /* server clock, seconds */
+    double startTime;
+    double endTime;
+    char message[160];          /* error text when status != FH_OK */
+} fh_result;
+
+void fh_server_config_default(fh_server_config* config);
+int  fh_server_create(const fh_server_config* config, fh_server** out);
+/* queues the request (copying its arrays unless borrowInputs); returns at once */
+int  fh_server_submit(fh_server* server, const fh_request* request);
+/* next completed result (completion order); *got = 0 after timeoutSeconds */
+int  fh_server_wait(fh_server* server, double timeoutSeconds, fh_result* result,
+                    int32_t* got);
+/* copy the solution of a completed request out and release it */
+int  fh_server_fetch(fh_server* server, int64_t id, double* psi);
+/* seconds since the server was created (the clock of fh_result) */
+int  fh_server_now(fh_server* server, double* seconds);
+/* requests submitted and not yet returned by fh_server_wait */
+int  fh_server_pending(fh_server* server, int64_t* count);
+int  fh_server_destroy(fh_server* server);
+
 #ifdef __cplusplus
 }
 #endif

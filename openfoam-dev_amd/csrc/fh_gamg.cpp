@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 
 namespace fh
 {
@@ -709,14 +710,23 @@ static void setup_coarsest(fh_matrix* M, GamgHierarchy* H, int mode)
     H->coarseValid = true;
 }
 
+// the agglomeration cache of a mesh and the process-global pairing
+// direction are shared by every matrix (and server thread) on the mesh
+static std::mutex g_aggMutex;
+
 void gamg_prepare(fh_matrix* M, const fh_controls* c)
 {
     fh_mesh* mesh = M->mesh;
-    auto agg = std::static_pointer_cast<GamgAgglomeration>(mesh->gamgAgg);
-    if (!agg || agg->minCells != c->nCellsInCoarsestLevel || agg->mergeLevels != c->mergeLevels)
+    std::shared_ptr<GamgAgglomeration> agg;
     {
-        agg = build_agglomeration(M, c);
-        mesh->gamgAgg = agg;
+        std::lock_guard<std::mutex> lk(g_aggMutex);
+        agg = std::static_pointer_cast<GamgAgglomeration>(mesh->gamgAgg);
+        if (!agg || agg->minCells != c->nCellsInCoarsestLevel
+            || agg->mergeLevels != c->mergeLevels)
+        {
+            agg = build_agglomeration(M, c);
+            mesh->gamgAgg = agg;
+        }
     }
     GamgHierarchy* H = static_cast<GamgHierarchy*>(M->gamg);
     if (H && H->agg != agg)

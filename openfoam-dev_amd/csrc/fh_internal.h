@@ -25,6 +25,8 @@ struct Error
 };
 
 [[noreturn]] void fail(int code, const char* fmt, ...);
+// runs fn(ctx), turning fh::Error and allocation failures into a status
+int guarded_call(void (*fn)(void*), void* ctx);
 void set_last_error(const std::string& msg);
 
 #define FH_HIP(call)                                                         \

@@ -123,7 +123,7 @@ void k_sweep_tile(TileView G, RowsView R, SweepData<T> dat, unsigned int* flags,
     const int TI = tile % G.nTI, TJ = (tile/G.nTI) % G.nTJ, TK = tile/(G.nTI*G.nTJ);
     const int x0 = TI*BX, y0 = TJ*BY, z0 = TK*BZ;
     const int ex = min(BX, G.nx - x0), ey = min(BY, G.ny - y0), ez = min(BZ, G.nz - z0);
-    const int64_t sx = 1, sy = G.nx, sz = (int64_t)G.nx*G.ny;
+    const int64_t sy = G.nx, sz = (int64_t)G.nx*G.ny;
     const int64_t base = x0 + sy*y0 + sz*z0;
 
     // ---- stage the tile's inputs (independent of upstream tiles) ----------

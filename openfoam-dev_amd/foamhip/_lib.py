@@ -61,6 +61,25 @@ class FhMeshInfo(ctypes.Structure):
     ]
 
 
+class FhServerConfig(ctypes.Structure):
+    _fields_ = [("maxConcurrent", _i32), ("borrowInputs", _i32), ("memoryBudget", _i64)]
+
+
+class FhRequest(ctypes.Structure):
+    _fields_ = [
+        ("id", _i64), ("mesh", _vp), ("diag", _vp), ("upper", _vp), ("lower", _vp),
+        ("source", _vp), ("psi0", _vp), ("controls", FhControls),
+    ]
+
+
+class FhResult(ctypes.Structure):
+    _fields_ = [
+        ("id", _i64), ("status", _i32), ("nCells", _i32), ("perf", FhPerf),
+        ("submitTime", _dbl), ("startTime", _dbl), ("endTime", _dbl),
+        ("message", ctypes.c_char * 160),
+    ]
+
+
 # every symbol declared in include/foamhip.h: name -> (restype, argtypes)
 FH_SYMBOLS = {
     "fh_version": (_i32, []),
@@ -97,6 +116,14 @@ FH_SYMBOLS = {
     "fh_assemble_laplacian": (_i32, [_vp, _vp] + [_vp] * 10 + [_vp, _i32]),
     "fh_assemble_convection_diffusion": (_i32, [_vp, _vp, _vp, _dbl] + [_vp] * 9 + [_vp, _i32]),
     "fh_grad_gauss_linear": (_i32, [_vp, _vp] + [_vp] * 10),
+    "fh_server_config_default": (None, [ctypes.POINTER(FhServerConfig)]),
+    "fh_server_create": (_i32, [ctypes.POINTER(FhServerConfig), ctypes.POINTER(_vp)]),
+    "fh_server_submit": (_i32, [_vp, ctypes.POINTER(FhRequest)]),
+    "fh_server_wait": (_i32, [_vp, _dbl, ctypes.POINTER(FhResult), ctypes.POINTER(_i32)]),
+    "fh_server_fetch": (_i32, [_vp, _i64, _vp]),
+    "fh_server_now": (_i32, [_vp, ctypes.POINTER(_dbl)]),
+    "fh_server_pending": (_i32, [_vp, ctypes.POINTER(_i64)]),
+    "fh_server_destroy": (_i32, [_vp]),
 }
 
 CASE_SYMBOLS = {

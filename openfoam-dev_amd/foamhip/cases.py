@@ -78,3 +78,19 @@ class BoxCase:
                                      _p(self.Sf), _p(self.bCf), _p(self.bSf), _p(phi),
                                      _p(bPhi))
         return phi, bPhi
+
+    def laplacian_system(self, gamma, rhs):
+        """Host assembly of fvm::laplacian(gamma, p) = rhs with the boundary
+        coefficients added (as fvMatrix::solveSegregated does for the call):
+        returns (diag, upper, source).  Workload generation for request
+        streams; the device assembly is fh_assemble_laplacian."""
+        gP, gN = gamma[self.lower], gamma[self.upper]
+        upper = self.deltaCoeffs * ((self.weights * (gP - gN) + gN) * self.magSf)
+        diag = -(np.bincount(self.lower, upper, self.N) + np.bincount(self.upper, upper, self.N))
+        pGamma = gamma[self.bFaceCells] * self.bMagSf
+        fixed = self.bType == 1
+        ic = np.where(fixed, -pGamma * self.bDeltaCoeffs, 0.0)
+        bc = np.where(fixed, -pGamma * (self.bDeltaCoeffs * self.bValue), 0.0)
+        diag = diag + np.bincount(self.bFaceCells, ic, self.N)
+        source = rhs + np.bincount(self.bFaceCells, bc, self.N)
+        return diag, upper, source

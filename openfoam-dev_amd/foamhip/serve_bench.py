@@ -1,0 +1,140 @@
+"""Serving benchmark: independent Poisson solve requests on one MI355X.
+
+Requests: hex boxes n^3 with n = round(exp(U[ln 16, ln 128])), each with a
+seeded log-normal diffusivity (sigma ~ U[0.5, 1.5], correlation length 8
+cells) and a seeded right-hand side, fvm::laplacian with the x-min wall
+fixed; mixed precision PCG (FP32 inner, FP64 residual) to 1e-8.  The
+request stream draws its systems from a pool of distinct seeded systems
+(``pool`` of them, reused round robin) so that generation stays off the
+clock.  First the saturated throughput is measured (all requests queued at
+once), then Poisson arrivals at fractions of it; latency is completion
+minus submission on the server clock.
+
+    python -m foamhip.serve_bench --requests 4096 --rates 0.5,0.8,1.0
+"""
+import argparse
+import json
+import math
+import sys
+import threading
+import time
+
+import numpy as np
+
+from . import lduAddressing
+from .cases import BoxCase
+from .serve import Server
+
+CONTROLS = {"solver": "PCG", "preconditioner": "DIC", "tolerance": 1e-8, "relTol": 0,
+            "precision": "mixed", "maxIter": 5000}
+
+
+def build_pool(n_systems, seed=2024, nmin=16, nmax=128):
+    rng = np.random.default_rng(seed)
+    meshes = {}
+    pool = []
+    for i in range(n_systems):
+        n = int(round(math.exp(rng.uniform(math.log(nmin), math.log(nmax)))))
+        sigma = float(rng.uniform(0.5, 1.5))
+        if n not in meshes:
+            c = BoxCase(n)
+            meshes[n] = (c, lduAddressing(c.N, c.lower, c.upper))
+        c, mesh = meshes[n]
+        gamma = c.lognormal(1000 + i, sigma, 8.0)
+        diag, upper, source = c.laplacian_system(gamma, c.rhs(5000 + i))
+        pool.append({"n": n, "sigma": sigma, "mesh": mesh, "diag": diag, "upper": upper,
+                     "source": source})
+    return pool
+
+
+def run_stream(srv, pool, n_requests, rate, rng):
+    """Submit n_requests (Poisson arrivals at `rate`/s, or all at once when
+    rate is None) and collect every result; returns the result records."""
+    results = []
+    done = threading.Event()
+
+    def collect():
+        while len(results) < n_requests:
+            r = srv.wait(1.0)
+            if r is None:
+                continue
+            if r["status"] != 0:
+                raise RuntimeError(f"request {r['id']} failed: {r['message']}")
+            srv.fetch(r["id"])
+            results.append(r)
+        done.set()
+
+    th = threading.Thread(target=collect, daemon=True)
+    th.start()
+    t0 = srv.now()
+    t_next = t0
+    window = 4 * srv.max_concurrent
+    for k in range(n_requests):
+        if rate is None:
+            # saturated: keep a bounded window of queued requests
+            while k - len(results) >= window:
+                time.sleep(0.0005)
+        else:
+            t_next += rng.exponential(1.0 / rate)
+            while True:
+                dt = t_next - srv.now()
+                if dt <= 0:
+                    break
+                time.sleep(min(dt, 0.01))
+        s = pool[k % len(pool)]
+        srv.submit(k, s["mesh"], s["diag"], s["upper"], s["source"], CONTROLS)
+    th.join()
+    return results, t0
+
+
+def summarize(results, t0):
+    lat = np.array([r["end"] - r["submit"] for r in results])
+    t_end = max(r["end"] for r in results)
+    return {"requests": len(results), "solves_per_s": len(results) / (t_end - t0),
+            "p50_ms": 1e3 * float(np.percentile(lat, 50)),
+            "p99_ms": 1e3 * float(np.percentile(lat, 99)),
+            "mean_ms": 1e3 * float(lat.mean()),
+            "mean_iterations": float(np.mean([r["nIterations"] for r in results]))}
+
+
+def benchmark(n_requests=4096, rates=(0.5, 0.8, 1.0), pool_size=48, max_concurrent=8,
+              seed=7, log=print):
+    pool = build_pool(pool_size)
+    cells = np.mean([p["n"] ** 3 for p in pool])
+    log(f"pool of {pool_size} systems, mean {cells:.0f} cells")
+    srv = Server(maxConcurrent=max_concurrent, borrowInputs=True)
+    rng = np.random.default_rng(seed)
+    # warm up: every pooled system once (resident matrices and kernels)
+    res, t0 = run_stream(srv, pool, len(pool), None, rng)
+    sat_res, t0 = run_stream(srv, pool, n_requests, None, rng)
+    sat = summarize(sat_res, t0)
+    log(f"saturated: {sat}")
+    out = {"saturated": sat, "rates": {}}
+    for f in rates:
+        lam = f * sat["solves_per_s"]
+        r, t0 = run_stream(srv, pool, n_requests, lam, rng)
+        s = summarize(r, t0)
+        s["arrival_rate_per_s"] = lam
+        out["rates"][str(f)] = s
+        log(f"rate {f} x saturated ({lam:.1f}/s): {s}")
+    srv.close()
+    out["pool"] = pool_size
+    out["mean_cells"] = float(cells)
+    out["max_concurrent"] = max_concurrent
+    return out
+
+
+def main():
+    p = argparse.ArgumentParser()
+    p.add_argument("--requests", type=int, default=4096)
+    p.add_argument("--rates", default="0.5,0.8,1.0")
+    p.add_argument("--pool", type=int, default=48)
+    p.add_argument("--concurrent", type=int, default=8)
+    a = p.parse_args()
+    out = benchmark(a.requests, tuple(float(x) for x in a.rates.split(",")), a.pool,
+                    a.concurrent, log=lambda m: print(m, file=sys.stderr, flush=True))
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

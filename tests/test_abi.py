@@ -103,3 +103,23 @@ def test_fourier_velocity_is_nearly_divergence_free():
     # second-order face-centre quadrature of an exactly solenoidal field
     scale = np.abs(phi).mean()
     assert np.abs(div).mean() < 0.05 * scale
+
+
+def test_server_lifecycle_without_solves():
+    """A server starts its workers, keeps a clock and shuts down with no
+    device work; bad requests are refused at submission."""
+    from foamhip.serve import Server
+    srv = Server(maxConcurrent=3)
+    t0 = srv.now()
+    assert t0 >= 0.0 and srv.pending() == 0
+    assert srv.wait(0.01) is None
+    import ctypes
+    r = _lib.FhRequest()
+    assert foamhip.lib().fh_server_submit(srv.handle, ctypes.byref(r)) == 1
+    assert srv.now() >= t0
+    srv.close()
+    cfg = _lib.FhServerConfig()
+    foamhip.lib().fh_server_config_default(ctypes.byref(cfg))
+    cfg.maxConcurrent = 0
+    h = ctypes.c_void_p()
+    assert foamhip.lib().fh_server_create(ctypes.byref(cfg), ctypes.byref(h)) == 1
