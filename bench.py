@@ -45,6 +45,11 @@ def parse():
     p.add_argument("--corr", type=float, default=8.0)
     p.add_argument("--tol", type=float, default=1e-8)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    p.add_argument("--serve-requests", type=int, default=256,
+                   help="requests of the serving sample (0 skips it)")
+    p.add_argument("--profile-kernels", action="store_true",
+                   help="only launch the profiled kernels (and a plain copy of known size "
+                        "for counter calibration), for rocprofv3 counter runs")
     p.add_argument("--cpu-seconds", type=float, default=15.0,
                    help="approximate CPU time budget of the baseline sample")
     return p.parse_args()
@@ -95,6 +100,10 @@ def main():
         P(D["bValue"]), P(ic), P(bc), P(src), 1))
     torch.cuda.synchronize()
     assembly_ms = 1e3 * (time.perf_counter() - t0)
+
+    if a.profile_kernels:
+        profile_only(fh, mat, torch, case.N)
+        return
 
     base = {"solver": "PCG", "preconditioner": "DIC", "tolerance": a.tol, "relTol": 0,
             "maxIter": 20000}
@@ -162,6 +171,21 @@ def main():
     dom = max(share, key=share.get)
     kd = kernels[dom]
 
+    serve = None
+    if a.serve_requests > 0:
+        # continuous-batching sample of the serving configuration: Poisson
+        # arrivals at 0.8x the saturated rate (the full study is
+        # python -m foamhip.serve_bench)
+        from foamhip import serve_bench
+        sb = serve_bench.benchmark(a.serve_requests, (0.8,), pool_size=24, log=log)
+        r = sb["rates"]["0.8"]
+        serve = {"requests": r["requests"], "arrival_rate_per_s": r["arrival_rate_per_s"],
+                 "solves_per_s": r["solves_per_s"], "p50_ms": r["p50_ms"], "p99_ms": r["p99_ms"],
+                 "saturated_solves_per_s": sb["saturated"]["solves_per_s"],
+                 "mean_cells": sb["mean_cells"], "max_concurrent": sb["max_concurrent"],
+                 "workload": "n^3 boxes, n log-uniform in [16, 128], log-normal gamma sigma "
+                             "U[0.5, 1.5], mixed PCG to 1e-8, pool of 24 seeded systems"}
+
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(case, gamma, rhs, a, p64.nIterations)
@@ -194,6 +218,7 @@ def main():
             "fp64_error_vs_tight_fp64": err_fp64,
             "mixed_inner_preconditioner": "tileDIC (16x16x16 block-Jacobi DIC, FP32)",
             "assembly_ms": assembly_ms,
+            "serve": serve,
             "kernels": kernels,
         },
         "roofline": {
@@ -208,6 +233,27 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def profile_only(fh, mat, torch, N):
+    """The kernels of the timed solve, 20 launches each, then a device copy
+    of 1 GiB (reads and writes exactly 1 GiB) as the counter reference."""
+    FH_TILE_DIC, FH_DIC = 6, 2
+    for kid in (0, 1, 2, 3):
+        ms, by = ctypes.c_double(), ctypes.c_double()
+        fh._check(fh.lib().fh_profile_kernel(mat.handle, kid, 1, FH_TILE_DIC, 20,
+                                             ctypes.byref(ms), ctypes.byref(by)))
+        log(f"kernel {kid} fp32: {ms.value:.4f} ms, {by.value:.0f} algorithmic bytes")
+        fh._check(fh.lib().fh_profile_kernel(mat.handle, kid, 0, FH_DIC, 5,
+                                             ctypes.byref(ms), ctypes.byref(by)))
+        log(f"kernel {kid} fp64: {ms.value:.4f} ms, {by.value:.0f} algorithmic bytes")
+    src = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    src.fill_(1.0)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log("copy reference: 1073741824 bytes read and written per launch")
 
 
 def cpu_baseline(case, gamma, rhs, a, fp64_iterations):

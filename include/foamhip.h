@@ -217,6 +217,10 @@ int fh_precondition_t(fh_matrix* m, int32_t preconditioner, const double* rA,
 /* lduMatrix::smoother::smooth (lduMatrix.H:399-405) */
 int fh_smooth(fh_matrix* m, int32_t smoother, double* psi,
               const double* source, int32_t nSweeps);
+/* The FP32 preconditioner application of the mixed-precision inner
+   iterations (host float arrays, natural order); no reference counterpart,
+   it is the single-precision form of fh_precondition. */
+int fh_precondition_f32(fh_matrix* m, int32_t preconditioner, const float* rA, float* wA);
 
 /* ---- measurement (no reference counterpart) -----------------------------
    Times `reps` back-to-back launches of one kernel of the solve on the

@@ -435,6 +435,26 @@ int fh_precondition_t(fh_matrix* M, int32_t precon, const double* rA, double* wA
     return precondition_impl(M, precon, rA, wA, true);
 }
 
+int fh_precondition_f32(fh_matrix* M, int32_t precon, const float* rA, float* wA)
+{
+    return guarded([&] {
+        check_matrix(M);
+        if (!rA || !wA) fail(FH_ERR_INVALID, "NULL argument");
+        if (precon == FH_PRECON_GAMG)
+            fail(FH_ERR_UNSUPPORTED, "use fh_solve for the GAMG preconditioner");
+        const int32_t N = M->mesh->nCells;
+        DBuf<float> nat, r, w;
+        nat.alloc(N); r.alloc(N); w.alloc(N);
+        FH_HIP(hipMemcpyAsync(nat.p, rA, sizeof(float)*N, hipMemcpyHostToDevice, M->stream));
+        launch_permute_in<float, float>(N, M->mesh->d_perm.p, nat.p, r.p, M->stream);
+        apply_precond<float>(M, precon, r.p, w.p, nullptr, nullptr, false);
+        launch_permute_out<float, float>(N, M->mesh->d_perm.p, w.p, nat.p, M->stream);
+        FH_HIP(hipMemcpyAsync(wA, nat.p, sizeof(float)*N, hipMemcpyDeviceToHost, M->stream));
+        FH_HIP(hipStreamSynchronize(M->stream));
+        check_sweep_errors(M);
+    });
+}
+
 int fh_smooth(fh_matrix* M, int32_t smoother, double* psi, const double* source,
               int32_t nSweeps)
 {

@@ -603,6 +603,8 @@ template void launch_gather_values<double>(const RowsView&, int64_t, const doubl
 template void launch_gather_values<float>(const RowsView&, int64_t, const double*, const double*, float*, hipStream_t);
 template void launch_permute_in<double, double>(int32_t, const int32_t*, const double*, double*, hipStream_t);
 template void launch_permute_out<double, double>(int32_t, const int32_t*, const double*, double*, hipStream_t);
+template void launch_permute_in<float, float>(int32_t, const int32_t*, const float*, float*, hipStream_t);
+template void launch_permute_out<float, float>(int32_t, const int32_t*, const float*, float*, hipStream_t);
 template void launch_convert<double, float>(int64_t, const double*, float*, hipStream_t);
 template void launch_convert<float, double>(int64_t, const float*, double*, hipStream_t);
 template void launch_amul<double>(const RowsView&, const double*, const double*, const double*, double*, const DevState*, hipStream_t);

@@ -34,7 +34,11 @@ void matrix_init(fh_matrix* M, fh_mesh* mesh, hipStream_t shared)
     M->d_state.alloc(1);
     FH_HIP(hipHostMalloc((void**)&M->h_state, sizeof(DevState), hipHostMallocDefault));
     std::memset(M->h_state, 0, sizeof(DevState));
-    M->d_partials.alloc(2*reduction_partials(mesh->nCells) + 64);
+    size_t nPart = 2*reduction_partials(mesh->nCells) + 64;
+    if (mesh->tiled)
+        nPart = std::max(nPart, 2*(size_t)box_spmv_blocks(mesh->boxNx, mesh->boxNy,
+                                                          mesh->boxNz) + 64);
+    M->d_partials.alloc(nPart);
     M->d_sync.alloc(1 + (size_t)mesh->sched.nParts);
     FH_HIP(hipMemsetAsync(M->d_sync.p, 0, M->d_sync.bytes(), M->stream));
     // one-part meshes sweep row by row unless the level sweep is forced
@@ -687,14 +691,33 @@ static void mixed_pcg(fh_matrix* M, const fh_controls* c, double* psi, const dou
     launch_mixed_setup(st, 0.1, M->stream);
     const float* diag = M->d_diag32.p;
     const float* val = M->d_val32.p;
+    // hex boxes: the FP32 product reads direction-packed coefficients
+    const bool box = M->mesh->tiled;
+    const float* dir[6] = {};
+    if (box)
+    {
+        ensure_dir32(M);
+        for (int d = 0; d < 6; d++) dir[d] = M->d_dir32[d].p;
+    }
+    const fh_mesh* mesh = M->mesh;
     run_batches(M, c->maxIter, [&](int32_t)
     {
         apply_precond<float>(M, precon, r32, w32, part, st, false);
         launch_finalize(N, part, FIN_PCG_WARA, st, 0, M->stream);
         launch_pcg_p<float>(N, w32, p32, st, M->stream);
-        launch_spmv_dot<float>(M->rows(), diag, val, p32, w32, p32, nullptr, part, st,
-                               M->stream);
-        launch_finalize(N, part, FIN_PCG_WAPA, st, 0, M->stream);
+        if (box)
+        {
+            const int32_t nb = launch_box_spmv_dot_f32(mesh->boxNx, mesh->boxNy, mesh->boxNz,
+                                                       diag, dir, M->symmetric, p32, w32, p32,
+                                                       part, st, M->stream);
+            launch_finalize_blocks(nb, N, part, FIN_PCG_WAPA, st, 0, M->stream);
+        }
+        else
+        {
+            launch_spmv_dot<float>(M->rows(), diag, val, p32, w32, p32, nullptr, part, st,
+                                   M->stream);
+            launch_finalize(N, part, FIN_PCG_WAPA, st, 0, M->stream);
+        }
         launch_mixed_update(N, psi, r32, p32, w32, part, st, M->stream);
         launch_finalize(N, part, FIN_MIXED_RES, st, 0, M->stream);
         launch_mixed_replace(M->rows(), M->d_diag.p, M->d_val.p, psi, src, r32, part, st,
@@ -988,7 +1011,12 @@ static void profile_t(fh_matrix* M, int kernel, int precon, int reps, double* ms
         fail(FH_ERR_INVALID, "sweep profiling needs DIC, FDIC, DILU, tileDIC or tileDILU");
     const T* rD = which >= 0 ? rD_for<T>(M, which) : nullptr;
     const bool local = which >= 3 && M->mesh->tiled;
-    if (local && std::is_same<T, float>::value) ensure_dir32(M);
+    if (std::is_same<T, float>::value && M->mesh->tiled) ensure_dir32(M);
+    // the box product checks a solver state: a zeroed one, never done
+    DBuf<DevState> pst;
+    pst.alloc(1);
+    FH_HIP(hipMemsetAsync(pst.p, 0, sizeof(DevState), M->stream));
+    const DevState* pstate = pst.p;
     SweepData<T> d{};
     d.val = val; d.rD = rD; d.x = x.p; d.w = y.p;
     auto once = [&]()
@@ -996,6 +1024,20 @@ static void profile_t(fh_matrix* M, int kernel, int precon, int reps, double* ms
         switch (kernel)
         {
             case FH_KERNEL_SPMV:
+                if constexpr (std::is_same<T, float>::value)
+                {
+                    if (M->mesh->tiled)
+                    {
+                        // the product of the mixed inner iterations on boxes
+                        const float* dir[6];
+                        for (int q = 0; q < 6; q++) dir[q] = M->d_dir32[q].p;
+                        launch_box_spmv_dot_f32(M->mesh->boxNx, M->mesh->boxNy,
+                                                M->mesh->boxNz, diag, dir, M->symmetric,
+                                                x.p, y.p, x.p, M->d_partials.p, pstate,
+                                                M->stream);
+                        break;
+                    }
+                }
                 launch_amul<T>(M->rows(), diag, val, x.p, y.p, nullptr, M->stream);
                 break;
             case FH_KERNEL_SWEEP_FORWARD:
@@ -1040,8 +1082,16 @@ static void profile_t(fh_matrix* M, int kernel, int precon, int reps, double* ms
     *ms = (double)t/reps;
     switch (kernel)
     {
-        case FH_KERNEL_SPMV:          // diag, x, y, row info + (value, column) per off-diagonal
-            *bytes = N*(3*w + 4) + 2.0*F*(w + 4); break;
+        case FH_KERNEL_SPMV:
+            // boxes in FP32: x, diag, y and one coefficient per face (the
+            // symmetric upper one read from the lower array of the
+            // neighbour); otherwise diag, x, y, row info and (value,
+            // column) per off-diagonal
+            if (w == 4 && M->mesh->tiled)
+                *bytes = N*3*w + (M->symmetric ? 1.0 : 2.0)*F*w;
+            else
+                *bytes = N*(3*w + 4) + 2.0*F*(w + 4);
+            break;
         case FH_KERNEL_SWEEP_FORWARD: // input, rD, output, row info + one triangle
         case FH_KERNEL_SWEEP_BACKWARD:
             // the tile-local FP32 kernels read packed coefficients: input,

@@ -45,6 +45,18 @@ template<class T>
 void launch_spmv_dot(const RowsView& R, const T* diag, const T* val, const T* x, T* y,
                      const T* z, const T* v, double* partials, const DevState* st,
                      hipStream_t s);
+// y = A x on a hex box from direction-packed FP32 coefficients dir[0..5] =
+// lower K, J, I, upper I, J, K (upper unused when symmetric), fused with
+// sum y*z (z null: y*y).  Returns the number of partials it wrote (at most
+// box_spmv_blocks), to be finalized with launch_finalize_blocks.
+int32_t box_spmv_blocks(int32_t nx, int32_t ny, int32_t nz);
+int32_t launch_box_spmv_dot_f32(int32_t nx, int32_t ny, int32_t nz, const float* diag,
+                                const float* const dir[6], bool symmetric, const float* x,
+                                float* y, const float* z, double* partials,
+                                const DevState* st, hipStream_t s);
+// finalize over an explicit number of partial blocks
+void launch_finalize_blocks(int32_t nBlocks, int32_t n, const double* partials, int32_t op,
+                            DevState* st, int32_t seq, hipStream_t s);
 template<class T>
 void launch_pcg_update(int32_t n, T* psi, T* r, const T* p, const T* w, double* partials,
                        const DevState* st, hipStream_t s);

@@ -163,6 +163,85 @@ __global__ void k_spmv_dot(RowsView R, const T* __restrict__ diag, const T* __re
     write_partials(s0, s1, partials, gridDim.x);
 }
 
+// The same product on a hex box from direction-packed coefficients (no
+// row indirection): diag, then the lower neighbours k, j, i, then the upper
+// neighbours i, j, k -- the row order of the general kernel, so the sums are
+// the same.  When uI is null the matrix is symmetric and the upper
+// coefficient of a face is read as the lower coefficient of its upper cell.
+// Blocks are 64 (i) x 4 (j) cells of one k-plane; consecutive blocks of one
+// XCD (hardware block id mod 8) walk a contiguous slab of the box, so the
+// neighbouring planes a block reads are in that XCD's L2.
+#define BOX_BX 64
+#define BOX_BY 4
+
+int32_t box_spmv_blocks(int32_t nx, int32_t ny, int32_t nz)
+{
+    const int64_t nb = (int64_t)((nx + BOX_BX - 1)/BOX_BX)*((ny + BOX_BY - 1)/BOX_BY)*nz;
+    return (int32_t)(8*((nb + 7)/8));
+}
+
+__global__ __launch_bounds__(BOX_BX*BOX_BY)
+void k_box_spmv_dot_f32(int32_t nx, int32_t ny, int32_t nz,
+                        const float* __restrict__ diag,
+                        const float* __restrict__ lK, const float* __restrict__ lJ,
+                        const float* __restrict__ lI, const float* __restrict__ uI,
+                        const float* __restrict__ uJ, const float* __restrict__ uK,
+                        const float* __restrict__ x, float* __restrict__ y,
+                        const float* z, double* partials, const DevState* st)
+{
+    if (st->done) return;
+    const int32_t gx = (nx + BOX_BX - 1)/BOX_BX, gy = (ny + BOX_BY - 1)/BOX_BY;
+    const int32_t per = gridDim.x/8;
+    // XCD-aware swizzle: hardware block b runs on XCD b % 8
+    const int32_t logical = (int32_t)(blockIdx.x % 8)*per + (int32_t)(blockIdx.x/8);
+    const int32_t nReal = gx*gy*nz;
+    double s0 = 0.0;
+    if (logical < nReal)
+    {
+        const int32_t bx = logical % gx, by = (logical/gx) % gy, bz = logical/(gx*gy);
+        const int32_t gi = bx*BOX_BX + (int32_t)(threadIdx.x % BOX_BX);
+        const int32_t gj = by*BOX_BY + (int32_t)(threadIdx.x/BOX_BX);
+        const int32_t gk = bz;
+        if (gi < nx && gj < ny)
+        {
+            const int64_t sxy = (int64_t)nx*ny;
+            const int64_t c = gi + (int64_t)nx*gj + sxy*gk;
+            float acc = diag[c]*x[c];
+            if (gk > 0) acc += lK[c]*x[c - sxy];
+            if (gj > 0) acc += lJ[c]*x[c - nx];
+            if (gi > 0) acc += lI[c]*x[c - 1];
+            if (uI)
+            {
+                if (gi < nx - 1) acc += uI[c]*x[c + 1];
+                if (gj < ny - 1) acc += uJ[c]*x[c + nx];
+                if (gk < nz - 1) acc += uK[c]*x[c + sxy];
+            }
+            else
+            {
+                if (gi < nx - 1) acc += lI[c + 1]*x[c + 1];
+                if (gj < ny - 1) acc += lJ[c + nx]*x[c + nx];
+                if (gk < nz - 1) acc += lK[c + sxy]*x[c + sxy];
+            }
+            y[c] = acc;
+            s0 = (double)acc*(double)(z ? z[c] : acc);
+        }
+    }
+    write_partials(s0, 0.0, partials, gridDim.x);
+}
+
+int32_t launch_box_spmv_dot_f32(int32_t nx, int32_t ny, int32_t nz, const float* diag,
+                                const float* const dir[6], bool symmetric, const float* x,
+                                float* y, const float* z, double* partials,
+                                const DevState* st, hipStream_t s)
+{
+    const int32_t nb = box_spmv_blocks(nx, ny, nz);
+    hipLaunchKernelGGL(k_box_spmv_dot_f32, dim3(nb), dim3(BOX_BX*BOX_BY), 0, s,
+                       nx, ny, nz, diag, dir[0], dir[1], dir[2], symmetric ? nullptr : dir[3],
+                       symmetric ? nullptr : dir[4], symmetric ? nullptr : dir[5], x, y, z,
+                       partials, st);
+    return nb;
+}
+
 // psi += alpha*pA; rA -= alpha*wA; sum0 = sum |rA|
 template<class T>
 __global__ void k_pcg_update(int32_t n, T* __restrict__ psi, T* __restrict__ r,
@@ -271,19 +350,57 @@ __device__ bool dev_singular(DevState* st, double v)
     return st->singular;
 }
 
-__global__ void k_finalize(int32_t nBlocks, const double* partials, int32_t op,
-                           DevState* st, int32_t n, int32_t seq)
+// The partials are summed by 1024 threads, each with four independent
+// accumulators over a fixed stride (loads in flight instead of one
+// dependent chain), then across the block in a fixed order.
+#define FIN_THREADS 1024
+__global__ __launch_bounds__(FIN_THREADS)
+void k_finalize(int32_t nBlocks, const double* partials, int32_t op,
+                DevState* st, int32_t n, int32_t seq)
 {
     if (op != FIN_SUM && op != FIN_XREF && op != FIN_INIT && op != FIN_INNER_INIT
         && op != FIN_OUTER_RES && st->done) return;
-    __shared__ double smem[2*FH_BLOCK/64];
-    double a = 0.0, b = 0.0;
-    for (int32_t i = threadIdx.x; i < nBlocks; i += FH_BLOCK)
+    __shared__ double smem[2*FIN_THREADS/64];
+    const double* pa = partials;
+    const double* pb = partials + nBlocks;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    double b0 = 0.0, b1 = 0.0, b2 = 0.0, b3 = 0.0;
+    int32_t i = threadIdx.x;
+    for (; i + 3*FIN_THREADS < nBlocks; i += 4*FIN_THREADS)
     {
-        a += partials[i];
-        b += partials[nBlocks + i];
+        a0 += pa[i]; a1 += pa[i + FIN_THREADS];
+        a2 += pa[i + 2*FIN_THREADS]; a3 += pa[i + 3*FIN_THREADS];
+        b0 += pb[i]; b1 += pb[i + FIN_THREADS];
+        b2 += pb[i + 2*FIN_THREADS]; b3 += pb[i + 3*FIN_THREADS];
     }
-    block_sum2(a, b, smem);
+    for (; i < nBlocks; i += FIN_THREADS)
+    {
+        a0 += pa[i];
+        b0 += pb[i];
+    }
+    double a = (a0 + a1) + (a2 + a3);
+    double b = (b0 + b1) + (b2 + b3);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+    {
+        a += __shfl_xor(a, off, 64);
+        b += __shfl_xor(b, off, 64);
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0)
+    {
+        smem[wid] = a;
+        smem[FIN_THREADS/64 + wid] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        double sa = 0.0, sb = 0.0;
+#pragma unroll
+        for (int w = 0; w < FIN_THREADS/64; w++) { sa += smem[w]; sb += smem[FIN_THREADS/64 + w]; }
+        a = sa;
+        b = sb;
+    }
     if (threadIdx.x != 0) return;
     switch (op)
     {
@@ -598,7 +715,7 @@ template void launch_sum_abs_state<float>(int32_t, const float*, double*, const 
 void launch_finalize_smooth(int32_t n, const double* partials, DevState* st,
                             int32_t nSweeps, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(FH_BLOCK), 0, s,
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(FIN_THREADS), 0, s,
                        reduction_blocks(n), partials, (int32_t)FIN_SMOOTH_RES, st, n, nSweeps);
 }
 
@@ -642,8 +759,15 @@ static inline dim3 grid_for(int32_t n) { return dim3(reduction_blocks(n)); }
 void launch_finalize(int32_t n, const double* partials, int32_t op, DevState* st,
                      int32_t seq, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(FH_BLOCK), 0, s,
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(FIN_THREADS), 0, s,
                        reduction_blocks(n), partials, op, st, n, seq);
+}
+
+void launch_finalize_blocks(int32_t nBlocks, int32_t n, const double* partials, int32_t op,
+                            DevState* st, int32_t seq, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(FIN_THREADS), 0, s, nBlocks, partials, op, st, n,
+                       seq);
 }
 
 template<class Ta, class Tb>

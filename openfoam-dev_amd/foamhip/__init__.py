@@ -178,6 +178,14 @@ class lduMatrix:
         _check(fn(self.handle, FH_PRECON[preconditioner], _ptr(rA), _ptr(out)))
         return out
 
+    def precondition_f32(self, preconditioner, rA):
+        """The FP32 application used by the mixed-precision inner iterations."""
+        r = np.ascontiguousarray(rA, dtype=np.float32)
+        out = np.empty(self.mesh.nCells, np.float32)
+        _check(lib().fh_precondition_f32(self.handle, FH_PRECON[preconditioner], _ptr(r),
+                                         _ptr(out)))
+        return out
+
     def smooth(self, smoother, psi, source, nSweeps):
         psi = _f64(psi).copy()
         source = _f64(source)

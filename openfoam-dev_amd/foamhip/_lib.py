@@ -109,6 +109,7 @@ FH_SYMBOLS = {
     "fh_precondition": (_i32, [_vp, _i32, _vp, _vp]),
     "fh_precondition_t": (_i32, [_vp, _i32, _vp, _vp]),
     "fh_smooth": (_i32, [_vp, _i32, _vp, _vp, _i32]),
+    "fh_precondition_f32": (_i32, [_vp, _i32, _vp, _vp]),
     "fh_profile_kernel": (_i32, [_vp, _i32, _i32, _i32, _i32, ctypes.POINTER(_dbl),
                                  ctypes.POINTER(_dbl)]),
     "fh_boundary_create": (_i32, [_vp, _i32, _vp, ctypes.POINTER(_vp)]),
