@@ -170,6 +170,7 @@ def main():
                                            "axpy_norm")}
     dom = max(share, key=share.get)
     kd = kernels[dom]
+    traffic = measured_traffic(dom, n)
 
     serve = None
     if a.serve_requests > 0:
@@ -228,11 +229,31 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": kd["GBs"] / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
         },
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+TRAFFIC_KEYS = {"spmv": "spmv_f32_box", "sweep_forward": "sweep_forward_f32_tile_local",
+                "sweep_backward": "sweep_backward_f32_tile_local", "axpy_norm": "axpy_norm_f32"}
+
+
+def measured_traffic(kernel, n):
+    """HBM bytes per launch of the kernel at 256^3 from the committed
+    rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated);
+    None for other sizes or when the summary does not list it."""
+    path = os.path.join(ROOT, "profiles", "traffic_256.md")
+    if n != 256 or not os.path.exists(path):
+        return None
+    key = TRAFFIC_KEYS.get(kernel)
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic" and parts[1] == key:
+                return float(parts[2])
+    return None
 
 
 def profile_only(fh, mat, torch, N):

@@ -170,11 +170,12 @@ void ensure_f32(fh_matrix* M);
 const double* ensure_rD(fh_matrix* M, int which);
 const float* ensure_rD32(fh_matrix* M, int which);
 
-// preconditioner apply w = M^-1 r on library-numbered vectors.  For the
-// FP64 PCG path a fused dot (w.r) can be requested (partials != null).
+// preconditioner apply w = M^-1 r on library-numbered vectors.  With
+// partials != null it also leaves sum w*r in the partials and returns their
+// block count (for launch_finalize_blocks).
 template<class T>
-void apply_precond(fh_matrix* M, int precon, const T* r, T* w, double* partials,
-                   DevState* st, bool transpose);
+int32_t apply_precond(fh_matrix* M, int precon, const T* r, T* w, double* partials,
+                      DevState* st, bool transpose);
 
 void profile_kernel(fh_matrix* M, int kernel, int precision, int precon, int reps,
                     double* ms, double* bytes);

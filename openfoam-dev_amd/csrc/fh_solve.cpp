@@ -336,8 +336,8 @@ template<> void sweep_t<float>(fh_matrix* M, int kind, const SweepData<float>& d
 }
 
 template<class T>
-void apply_precond(fh_matrix* M, int precon, const T* r, T* w, double* partials,
-                   DevState* st, bool transpose)
+int32_t apply_precond(fh_matrix* M, int precon, const T* r, T* w, double* partials,
+                      DevState* st, bool transpose)
 {
     const int32_t N = M->mesh->nCells;
     if (M->diagonalOnly || M->mesh->nFaces == 0)
@@ -355,14 +355,14 @@ void apply_precond(fh_matrix* M, int precon, const T* r, T* w, double* partials,
         // same V-cycles (GAMGPreconditioner.H:125-133)
         gamg_apply<T>(M, r, w, st);
         if (partials) launch_dot<T, T>(N, w, r, partials, st, M->stream);
-        return;
+        return reduction_blocks(N);
     }
     if (precon == FH_PRECON_NONE || precon == FH_PRECON_DIAGONAL)
     {
         const T* rD = precon == FH_PRECON_DIAGONAL ? rD_for<T>(M, 2) : nullptr;
         launch_precond_point<T>(N, precon, rD, r, w, partials ? partials : M->d_partials.p,
                                 st, M->stream);
-        return;
+        return reduction_blocks(N);
     }
     const int which = factor_kind(precon);
     if (which < 0) fail(FH_ERR_UNSUPPORTED, "preconditioner %d has no sweep form", precon);
@@ -383,19 +383,19 @@ void apply_precond(fh_matrix* M, int precon, const T* r, T* w, double* partials,
             const TileView G = M->tile_view(TileBZ<float>::pre);
             launch_tile_local_f32(false, G, r, d.rD, M->d_dir32[0].p, M->d_dir32[1].p,
                                   M->d_dir32[2].p, w, st, M->stream);
-            launch_tile_local_f32(true, G, w, d.rD, M->d_dir32[5].p, M->d_dir32[4].p,
-                                  M->d_dir32[3].p, w, st, M->stream);
-            if (partials) launch_dot<T, T>(N, w, r, partials, st, M->stream);
-            return;
+            // the backward pass also sums w*r (one partial per workgroup)
+            return launch_tile_local_f32(true, G, w, d.rD, M->d_dir32[5].p, M->d_dir32[4].p,
+                                         M->d_dir32[3].p, w, st, M->stream, r, partials);
         }
     }
     sweep_t<T>(M, SWEEP_PRECOND_FWD, d, st, local);
     sweep_t<T>(M, transpose && dilu ? SWEEP_PRECOND_BWD_T : SWEEP_PRECOND_BWD, d, st, local);
     if (partials) launch_dot<T, T>(N, w, r, partials, st, M->stream);
+    return reduction_blocks(N);
 }
 
-template void apply_precond<double>(fh_matrix*, int, const double*, double*, double*, DevState*, bool);
-template void apply_precond<float>(fh_matrix*, int, const float*, float*, double*, DevState*, bool);
+template int32_t apply_precond<double>(fh_matrix*, int, const double*, double*, double*, DevState*, bool);
+template int32_t apply_precond<float>(fh_matrix*, int, const float*, float*, double*, DevState*, bool);
 
 void state_upload(fh_matrix* M, const fh_controls* c, double tol, double relTol,
                          int32_t maxIter, int32_t minIter)
@@ -450,8 +450,8 @@ void pcg_loop(fh_matrix* M, int precon, T* psi, T* r, T* p, T* w, int32_t maxIte
         (void)rD_for<T>(M, factor_kind(precon));
     run_batches(M, maxIter, [&](int32_t)
     {
-        apply_precond<T>(M, precon, r, w, part, st, false);
-        launch_finalize(N, part, FIN_PCG_WARA, st, 0, M->stream);
+        const int32_t nb = apply_precond<T>(M, precon, r, w, part, st, false);
+        launch_finalize_blocks(nb, N, part, FIN_PCG_WARA, st, 0, M->stream);
         launch_pcg_p<T>(N, w, p, st, M->stream);
         launch_spmv_dot<T>(M->rows(), diag, val, p, w, p, nullptr, part, st, M->stream);
         launch_finalize(N, part, FIN_PCG_WAPA, st, 0, M->stream);
@@ -700,13 +700,31 @@ static void mixed_pcg(fh_matrix* M, const fh_controls* c, double* psi, const dou
         for (int d = 0; d < 6; d++) dir[d] = M->d_dir32[d].p;
     }
     const fh_mesh* mesh = M->mesh;
-    run_batches(M, c->maxIter, [&](int32_t)
+    // boxes with nx a multiple of 4: p = w + beta p fused into the product,
+    // ping-ponging between two p buffers
+    const bool fuseP = box && (mesh->boxNx & 3) == 0;
+    M->f[3].alloc(N);
+    M->f[4].alloc(N);
+    float* pBuf[2] = {p32, M->f[3].p};
+    // A p: in place of w when p is formed by its own kernel; a separate
+    // buffer when the product reads w to form p
+    float* ap = fuseP ? M->f[4].p : w32;
+    run_batches(M, c->maxIter, [&](int32_t seq)
     {
-        apply_precond<float>(M, precon, r32, w32, part, st, false);
-        launch_finalize(N, part, FIN_PCG_WARA, st, 0, M->stream);
-        launch_pcg_p<float>(N, w32, p32, st, M->stream);
-        if (box)
+        const int32_t nbw = apply_precond<float>(M, precon, r32, w32, part, st, false);
+        launch_finalize_blocks(nbw, N, part, FIN_PCG_WARA, st, 0, M->stream);
+        if (fuseP)
         {
+            p32 = pBuf[(seq + 1) & 1];
+            const int32_t nb = launch_box_pcg_spmv_f32(mesh->boxNx, mesh->boxNy, mesh->boxNz,
+                                                       diag, dir, M->symmetric, w32,
+                                                       pBuf[seq & 1], p32, ap, part, st,
+                                                       M->stream);
+            launch_finalize_blocks(nb, N, part, FIN_PCG_WAPA, st, 0, M->stream);
+        }
+        else if (box)
+        {
+            launch_pcg_p<float>(N, w32, p32, st, M->stream);
             const int32_t nb = launch_box_spmv_dot_f32(mesh->boxNx, mesh->boxNy, mesh->boxNz,
                                                        diag, dir, M->symmetric, p32, w32, p32,
                                                        part, st, M->stream);
@@ -714,12 +732,13 @@ static void mixed_pcg(fh_matrix* M, const fh_controls* c, double* psi, const dou
         }
         else
         {
+            launch_pcg_p<float>(N, w32, p32, st, M->stream);
             launch_spmv_dot<float>(M->rows(), diag, val, p32, w32, p32, nullptr, part, st,
                                    M->stream);
             launch_finalize(N, part, FIN_PCG_WAPA, st, 0, M->stream);
         }
-        launch_mixed_update(N, psi, r32, p32, w32, part, st, M->stream);
-        launch_finalize(N, part, FIN_MIXED_RES, st, 0, M->stream);
+        const int32_t nbu = launch_mixed_update(N, psi, r32, p32, ap, part, st, M->stream);
+        launch_finalize_blocks(nbu, N, part, FIN_MIXED_RES, st, 0, M->stream);
         launch_mixed_replace(M->rows(), M->d_diag.p, M->d_val.p, psi, src, r32, part, st,
                              M->stream);
         launch_finalize(N, part, FIN_MIXED_TRUE, st, 0, M->stream);

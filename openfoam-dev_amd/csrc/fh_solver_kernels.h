@@ -54,6 +54,13 @@ int32_t launch_box_spmv_dot_f32(int32_t nx, int32_t ny, int32_t nz, const float*
                                 const float* const dir[6], bool symmetric, const float* x,
                                 float* y, const float* z, double* partials,
                                 const DevState* st, hipStream_t s);
+// PCG's p = w + beta*pOld (first iteration: w) fused into the box product:
+// writes pNew and y = A pNew with sum y*pNew; nx must be a multiple of 4
+// (returns -1 otherwise, nothing launched)
+int32_t launch_box_pcg_spmv_f32(int32_t nx, int32_t ny, int32_t nz, const float* diag,
+                                const float* const dir[6], bool symmetric, const float* w,
+                                const float* pOld, float* pNew, float* y, double* partials,
+                                const DevState* st, hipStream_t s);
 // finalize over an explicit number of partial blocks
 void launch_finalize_blocks(int32_t nBlocks, int32_t n, const double* partials, int32_t op,
                             DevState* st, int32_t seq, hipStream_t s);
@@ -77,9 +84,10 @@ void launch_bicg_update(int32_t n, T* psi, const T* y, const T* z, T* r, const T
 template<class T>
 void launch_pcg_update_nostate(int32_t n, T* psi, T* r, const T* p, const T* w,
                                double* partials, hipStream_t s);
-// mixed PCG: x64 += alpha*p32, r32 -= alpha*w32, sum |r32|
-void launch_mixed_update(int32_t n, double* x, float* r, const float* p, const float* w,
-                         double* partials, const DevState* st, hipStream_t s);
+// mixed PCG: x64 += alpha*p32, r32 -= alpha*w32, sum |r32|; returns the
+// partial block count
+int32_t launch_mixed_update(int32_t n, double* x, float* r, const float* p, const float* w,
+                            double* partials, const DevState* st, hipStream_t s);
 // mixed PCG residual replacement (only when st->replace): r = b - A x in
 // FP64, r32 = r, sum |r|
 void launch_mixed_replace(const RowsView& R, const double* diag, const double* val,

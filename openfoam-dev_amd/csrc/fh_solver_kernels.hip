@@ -6,6 +6,8 @@
 #include "fh_kernels.h"
 #include "fh_solver_kernels.h"
 
+#include <algorithm>
+
 #include <cfloat>
 #include <cmath>
 
@@ -229,11 +231,162 @@ void k_box_spmv_dot_f32(int32_t nx, int32_t ny, int32_t nz,
     write_partials(s0, 0.0, partials, gridDim.x);
 }
 
+// Four cells along i per thread with 16-byte loads (nx a multiple of 4):
+// blocks of 64 threads x 4 rows = 256 x 4 cells of one k-plane, the same
+// XCD-aware block order, and the same per-cell sums as above.
+// FUSE_P: the product of p = w + beta*pOld (PCG.C:146-152, the pcg_p
+// expression) computed on the fly from w and pOld for every cell the
+// product reads; the new p of the block's cells is written to pNew (x is
+// pOld here and z is pNew's values, i.e. the dot is p.Ap).
+template<bool FUSE_P>
+__global__ __launch_bounds__(256)
+void k_box_spmv_dot_f32x4(int32_t nx, int32_t ny, int32_t nz,
+                          const float* __restrict__ diag,
+                          const float* __restrict__ lK, const float* __restrict__ lJ,
+                          const float* __restrict__ lI, const float* __restrict__ uI,
+                          const float* __restrict__ uJ, const float* __restrict__ uK,
+                          const float* __restrict__ x, float* __restrict__ y,
+                          const float* z, double* partials, const DevState* st,
+                          const float* __restrict__ wv, float* __restrict__ pNew)
+{
+    if (st->done) return;
+    const bool first = FUSE_P && st->nIterations == 0;
+    const float beta = FUSE_P ? (float)st->beta : 0.f;
+    const int32_t gx = (nx + 255)/256, gy = (ny + 3)/4;
+    const int32_t nReal = gx*gy*nz;
+    // XCD-aware order: the blocks of XCD (b mod 8) stride over one eighth
+    // of the tiles, contiguous in k
+    const int32_t slab = (nReal + 7)/8;
+    const int32_t stride = gridDim.x/8;
+    double s0 = 0.0;
+    for (int32_t local = (int32_t)(blockIdx.x/8); local < slab; local += stride)
+    {
+        const int32_t logical = (int32_t)(blockIdx.x % 8)*slab + local;
+        if (logical >= nReal) break;
+        const int32_t bx = logical % gx, by = (logical/gx) % gy, bz = logical/(gx*gy);
+        const int32_t i0 = bx*256 + 4*(int32_t)(threadIdx.x & 63);
+        const int32_t gj = by*4 + (int32_t)(threadIdx.x >> 6);
+        const int32_t gk = bz;
+        if (i0 < nx && gj < ny)
+        {
+            const int64_t sxy = (int64_t)nx*ny;
+            const int64_t c = i0 + (int64_t)nx*gj + sxy*gk;
+            typedef float4 f4;
+            auto ld = [](const float* p, int64_t i) { return *reinterpret_cast<const f4*>(p + i); };
+            // the product's input at cell i (4-wide / scalar)
+            auto px4 = [&](int64_t i) -> f4
+            {
+                if (!FUSE_P) return ld(x, i);
+                const f4 wq = ld(wv, i);
+                if (first) return wq;
+                const f4 pq = ld(x, i);
+                return make_float4(wq.x + beta*pq.x, wq.y + beta*pq.y, wq.z + beta*pq.z,
+                                   wq.w + beta*pq.w);
+            };
+            auto px1 = [&](int64_t i) -> float
+            {
+                if (!FUSE_P) return x[i];
+                return first ? wv[i] : wv[i] + beta*x[i];
+            };
+            const f4 X = px4(c), Dg = ld(diag, c);
+            const f4 LK = ld(lK, c), LJ = ld(lJ, c), LI = ld(lI, c);
+            const f4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+            const f4 XK = gk > 0 ? px4(c - sxy) : zero;
+            const f4 XJ = gj > 0 ? px4(c - nx) : zero;
+            const f4 XJp = gj < ny - 1 ? px4(c + nx) : zero;
+            const f4 XKp = gk < nz - 1 ? px4(c + sxy) : zero;
+            const float xm = i0 > 0 ? px1(c - 1) : 0.f;
+            const float xp = i0 + 4 < nx ? px1(c + 4) : 0.f;
+            if (FUSE_P) *reinterpret_cast<f4*>(pNew + c) = X;
+            f4 UI, UJ, UK;
+            if (uI)
+            {
+                UI = ld(uI, c); UJ = ld(uJ, c); UK = ld(uK, c);
+            }
+            else
+            {
+                // symmetric: the upper coefficient of a face is the lower
+                // one of its upper cell
+                const float li4 = i0 + 4 < nx ? lI[c + 4] : 0.f;
+                UI = make_float4(LI.y, LI.z, LI.w, li4);
+                UJ = gj < ny - 1 ? ld(lJ, c + nx) : zero;
+                UK = gk < nz - 1 ? ld(lK, c + sxy) : zero;
+            }
+            const float xs[6] = {xm, X.x, X.y, X.z, X.w, xp};
+            const float dg[4] = {Dg.x, Dg.y, Dg.z, Dg.w};
+            const float lk[4] = {LK.x, LK.y, LK.z, LK.w}, lj[4] = {LJ.x, LJ.y, LJ.z, LJ.w};
+            const float li[4] = {LI.x, LI.y, LI.z, LI.w};
+            const float ui[4] = {UI.x, UI.y, UI.z, UI.w}, uj[4] = {UJ.x, UJ.y, UJ.z, UJ.w};
+            const float uk[4] = {UK.x, UK.y, UK.z, UK.w};
+            const float xk[4] = {XK.x, XK.y, XK.z, XK.w}, xj[4] = {XJ.x, XJ.y, XJ.z, XJ.w};
+            const float xjp[4] = {XJp.x, XJp.y, XJp.z, XJp.w};
+            const float xkp[4] = {XKp.x, XKp.y, XKp.z, XKp.w};
+            float out[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+            {
+                const int32_t gi = i0 + q;
+                float acc = dg[q]*xs[q + 1];
+                if (gk > 0) acc += lk[q]*xk[q];
+                if (gj > 0) acc += lj[q]*xj[q];
+                if (gi > 0) acc += li[q]*xs[q];
+                if (gi < nx - 1) acc += ui[q]*xs[q + 2];
+                if (gj < ny - 1) acc += uj[q]*xjp[q];
+                if (gk < nz - 1) acc += uk[q]*xkp[q];
+                out[q] = acc;
+            }
+            *reinterpret_cast<f4*>(y + c) = make_float4(out[0], out[1], out[2], out[3]);
+            if (FUSE_P)
+            {
+                s0 += ((double)out[0]*(double)X.x + (double)out[1]*(double)X.y)
+                   + ((double)out[2]*(double)X.z + (double)out[3]*(double)X.w);
+            }
+            else if (z)
+            {
+                const f4 Z = ld(z, c);
+                s0 += ((double)out[0]*(double)Z.x + (double)out[1]*(double)Z.y)
+                   + ((double)out[2]*(double)Z.z + (double)out[3]*(double)Z.w);
+            }
+            else
+            {
+                s0 += ((double)out[0]*(double)out[0] + (double)out[1]*(double)out[1])
+                   + ((double)out[2]*(double)out[2] + (double)out[3]*(double)out[3]);
+            }
+        }
+    }
+    write_partials(s0, 0.0, partials, gridDim.x);
+}
+
+int32_t launch_box_pcg_spmv_f32(int32_t nx, int32_t ny, int32_t nz, const float* diag,
+                                const float* const dir[6], bool symmetric, const float* w,
+                                const float* pOld, float* pNew, float* y, double* partials,
+                                const DevState* st, hipStream_t s)
+{
+    if (nx & 3) return -1;
+    const int64_t nb4 = (int64_t)((nx + 255)/256)*((ny + 3)/4)*nz;
+    const int32_t nb = (int32_t)std::min<int64_t>(8*((nb4 + 7)/8), 2048);
+    hipLaunchKernelGGL(k_box_spmv_dot_f32x4<true>, dim3(nb), dim3(256), 0, s, nx, ny, nz, diag,
+                       dir[0], dir[1], dir[2], symmetric ? nullptr : dir[3],
+                       symmetric ? nullptr : dir[4], symmetric ? nullptr : dir[5], pOld, y,
+                       nullptr, partials, st, w, pNew);
+    return nb;
+}
+
 int32_t launch_box_spmv_dot_f32(int32_t nx, int32_t ny, int32_t nz, const float* diag,
                                 const float* const dir[6], bool symmetric, const float* x,
                                 float* y, const float* z, double* partials,
                                 const DevState* st, hipStream_t s)
 {
+    if ((nx & 3) == 0)
+    {
+        const int64_t nb4 = (int64_t)((nx + 255)/256)*((ny + 3)/4)*nz;
+        const int32_t nb = (int32_t)std::min<int64_t>(8*((nb4 + 7)/8), 2048);
+        hipLaunchKernelGGL(k_box_spmv_dot_f32x4<false>, dim3(nb), dim3(256), 0, s, nx, ny, nz,
+                           diag, dir[0], dir[1], dir[2], symmetric ? nullptr : dir[3],
+                           symmetric ? nullptr : dir[4], symmetric ? nullptr : dir[5], x, y, z,
+                           partials, st, nullptr, nullptr);
+        return nb;
+    }
     const int32_t nb = box_spmv_blocks(nx, ny, nz);
     hipLaunchKernelGGL(k_box_spmv_dot_f32, dim3(nb), dim3(BOX_BX*BOX_BY), 0, s,
                        nx, ny, nz, diag, dir[0], dir[1], dir[2], symmetric ? nullptr : dir[3],
@@ -360,6 +513,8 @@ void k_finalize(int32_t nBlocks, const double* partials, int32_t op,
 {
     if (op != FIN_SUM && op != FIN_XREF && op != FIN_INIT && op != FIN_INNER_INIT
         && op != FIN_OUTER_RES && st->done) return;
+    // no residual replacement this iteration: nothing to sum
+    if (op == FIN_MIXED_TRUE && !st->replace) return;
     __shared__ double smem[2*FIN_THREADS/64];
     const double* pa = partials;
     const double* pb = partials + nBlocks;
@@ -557,20 +712,23 @@ void k_finalize(int32_t nBlocks, const double* partials, int32_t op,
     }
 }
 
+// a fixed grid of at most MIX_BLOCKS blocks strides over the vector, so that
+// the finalize sums few partials (the order stays fixed)
+#define MIX_BLOCKS 1024
 __global__ void k_mixed_update(int32_t n, double* __restrict__ x, float* __restrict__ r,
                                const float* __restrict__ p, const float* __restrict__ w,
                                double* partials, const DevState* st)
 {
     if (st->done) return;
-    const int32_t i = blockIdx.x*FH_BLOCK + threadIdx.x;
+    const double alpha = st->alpha;
+    const float alphaF = (float)alpha;
     double s0 = 0.0;
-    if (i < n)
+    for (int32_t i = blockIdx.x*FH_BLOCK + threadIdx.x; i < n; i += gridDim.x*FH_BLOCK)
     {
-        const double alpha = st->alpha;
         x[i] += alpha*(double)p[i];
-        const float ri = r[i] - (float)alpha*w[i];
+        const float ri = r[i] - alphaF*w[i];
         r[i] = ri;
-        s0 = fabs((double)ri);
+        s0 += fabs((double)ri);
     }
     write_partials(s0, 0.0, partials, gridDim.x);
 }
@@ -612,11 +770,13 @@ void launch_mixed_setup(DevState* st, double replaceRatio, hipStream_t s)
     hipLaunchKernelGGL(k_mixed_setup, dim3(1), dim3(1), 0, s, st, replaceRatio);
 }
 
-void launch_mixed_update(int32_t n, double* x, float* r, const float* p, const float* w,
-                         double* partials, const DevState* st, hipStream_t s)
+int32_t launch_mixed_update(int32_t n, double* x, float* r, const float* p, const float* w,
+                            double* partials, const DevState* st, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_mixed_update, dim3(reduction_blocks(n)), dim3(FH_BLOCK), 0, s,
+    const int32_t nb = std::min(reduction_blocks(n), MIX_BLOCKS);
+    hipLaunchKernelGGL(k_mixed_update, dim3(nb), dim3(FH_BLOCK), 0, s,
                        n, x, r, p, w, partials, st);
+    return nb;
 }
 
 void launch_mixed_replace(const RowsView& R, const double* diag, const double* val,

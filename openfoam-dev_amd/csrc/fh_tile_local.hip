@@ -6,9 +6,6 @@
 // 16 contiguous cells.
 #include "fh_tile_sweep.h"
 
-#include <cstdlib>
-#include <cstring>
-
 namespace fh
 {
 
@@ -62,9 +59,11 @@ template<bool BWD, int BZ>
 __global__ __launch_bounds__(64)
 void k_tile_local_f32(TileView G, const float* a, const float* __restrict__ rD,
                       const float* __restrict__ cK, const float* __restrict__ cJ,
-                      const float* __restrict__ cI, float* w, const DevState* st)
+                      const float* __restrict__ cI, float* w, const DevState* st,
+                      const float* __restrict__ r, double* partials)
 {
     if (st && st->done) return;
+    double dotWR = 0.0;      // backward with r: this lane's share of sum w*r
     __shared__ float buf[4][2][BZ][16];          // per tile: diagonal rows by k slot
     const int lane = threadIdx.x;
     const int g = lane >> 4, l16 = lane & 15;
@@ -168,145 +167,61 @@ void k_tile_local_f32(TileView G, const float* a, const float* __restrict__ rD,
             float* mine = buf[g][step & 1][k];
 #pragma unroll
             for (int i = 0; i < 16; i++) mine[i] = out[i];
-        }
-        __syncthreads();
-    }
-}
-
-// The same factor application with the tile staged in LDS first: all
-// loads of a tile are in flight at once (one memory latency per tile instead
-// of one per diagonal), then the row walk runs on LDS only, writing each
-// result over its input (a cell is read by the lane that then writes it),
-// and the tile is written back with vector stores.  One tile per workgroup,
-// 80 KiB of LDS.
-template<bool BWD, int BZ>
-__global__ __launch_bounds__(256)
-void k_tile_local_staged_f32(TileView G, const float* a, const float* __restrict__ rD,
-                             const float* __restrict__ cK, const float* __restrict__ cJ,
-                             const float* __restrict__ cI, float* w, const DevState* st)
-{
-    if (st && st->done) return;
-    constexpr int NC = 16*16*BZ;
-    __shared__ float S[5][NC];            // a -> w in place, rD, cK, cJ, cI
-    const int tid = threadIdx.x;
-    const int tile = blockIdx.x;
-    const int TI = tile % G.nTI, TJ = (tile/G.nTI) % G.nTJ, TK = tile/(G.nTI*G.nTJ);
-    const int x0 = TI*16, y0 = TJ*16, z0 = TK*BZ;
-    const int ex = min(16, G.nx - x0), ey = min(16, G.ny - y0), ez = min(BZ, G.nz - z0);
-    const int64_t sy = G.nx, sz = (int64_t)G.nx*G.ny;
-    const int64_t base = x0 + sy*y0 + sz*z0;
-    const bool vec = (G.nx & 3) == 0 && ex == 16;
-    const float* src[5] = {a, rD, cK, cJ, cI};
-    // stage: quads of 4 cells along i
-    for (int q = tid; q < NC/4; q += 256)
-    {
-        const int i0 = (q & 3)*4, j = (q >> 2) & 15, k = q >> 6;
-        const bool rowIn = j < ey && k < ez;
-        const int64_t g = base + sy*j + sz*k + i0;
-#pragma unroll
-        for (int v = 0; v < 5; v++)
-        {
-            float4 t = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (rowIn)
+            if (BWD && partials)
             {
+                // the final w of the row against the preconditioned residual
+                float R[16];
                 if (vec)
-                    t = *reinterpret_cast<const float4*>(src[v] + g);
-                else
                 {
-                    if (i0 < ex) t.x = src[v][g];
-                    if (i0 + 1 < ex) t.y = src[v][g + 1];
-                    if (i0 + 2 < ex) t.z = src[v][g + 2];
-                    if (i0 + 3 < ex) t.w = src[v][g + 3];
-                }
-            }
-            *reinterpret_cast<float4*>(&S[v][4*q]) = t;
-        }
-    }
-    __syncthreads();
-    // row walk over the diagonals j + k by the first 16 lanes
-    const int nDiag = ey + ez - 1;
-    for (int step = 0; step < nDiag; step++)
-    {
-        if (tid < 16)
-        {
-            const int d = BWD ? nDiag - 1 - step : step;
-            const int kmin = max(0, d - (ey - 1));
-            const int k = kmin + tid;
-            const int j = d - k;
-            if (k < ez && j >= 0 && j < ey)
-            {
-                const int row = 16*(j + 16*k);
-                const bool hasK = BWD ? k < ez - 1 : k > 0;
-                const bool hasJ = BWD ? j < ey - 1 : j > 0;
-                const int oK = BWD ? 256 : -256, oJ = BWD ? 16 : -16;
-                float prev = 0.f;
 #pragma unroll
-                for (int q = 0; q < 16; q++)
-                {
-                    const int i = BWD ? 15 - q : q;
-                    if (i < ex)
+                    for (int v = 0; v < 4; v++)
                     {
-                        const int c = row + i;
-                        const float D = S[1][c];
-                        float v = BWD ? S[0][c] : D*S[0][c];
-                        if (hasK) v -= D*S[2][c]*S[0][c + oK];
-                        if (hasJ) v -= D*S[3][c]*S[0][c + oJ];
-                        if (BWD ? i < ex - 1 : i > 0) v -= D*S[4][c]*prev;
-                        prev = v;
-                        S[0][c] = v;
+                        const float4 q = reinterpret_cast<const float4*>(r + row)[v];
+                        R[4*v] = q.x; R[4*v + 1] = q.y; R[4*v + 2] = q.z; R[4*v + 3] = q.w;
                     }
                 }
+                else
+                {
+#pragma unroll
+                    for (int i = 0; i < 16; i++) R[i] = i < ex ? r[row + i] : 0.f;
+                }
+                double sr = 0.0;
+#pragma unroll
+                for (int i = 0; i < 16; i++) sr += (double)out[i]*(double)R[i];
+                dotWR += sr;
             }
         }
         __syncthreads();
     }
-    // write back
-    for (int q = tid; q < NC/4; q += 256)
+    if (BWD && partials)
     {
-        const int i0 = (q & 3)*4, j = (q >> 2) & 15, k = q >> 6;
-        if (j >= ey || k >= ez) continue;
-        const int64_t g = base + sy*j + sz*k + i0;
-        const float4 t = *reinterpret_cast<const float4*>(&S[0][4*q]);
-        if (vec)
-            *reinterpret_cast<float4*>(w + g) = t;
-        else
+        // one partial per wavefront (= workgroup), fixed butterfly order
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) dotWR += __shfl_xor(dotWR, off, 64);
+        if (lane == 0)
         {
-            if (i0 < ex) w[g] = t.x;
-            if (i0 + 1 < ex) w[g + 1] = t.y;
-            if (i0 + 2 < ex) w[g + 2] = t.z;
-            if (i0 + 3 < ex) w[g + 3] = t.w;
+            partials[blockIdx.x] = dotWR;
+            partials[gridDim.x + blockIdx.x] = 0.0;
         }
     }
 }
 
-void launch_tile_local_f32(bool backward, const TileView& G, const float* a, const float* rD,
-                           const float* cK, const float* cJ, const float* cI, float* w,
-                           const DevState* st, hipStream_t s)
+int32_t launch_tile_local_f32(bool backward, const TileView& G, const float* a, const float* rD,
+                              const float* cK, const float* cJ, const float* cI, float* w,
+                              const DevState* st, hipStream_t s, const float* r,
+                              double* partials)
 {
     if (G.bx != 16 || G.by != 16 || G.bz != TileBZ<float>::pre)
         fail(FH_ERR_INVALID, "tile shape does not match the compiled local kernels");
     const int nTiles = G.nTI*G.nTJ*G.nTK;
-    static const bool staged = [] {
-        const char* e = std::getenv("FOAMHIP_LOCAL");
-        return !(e && std::strcmp(e, "rows") == 0);
-    }();
-    if (staged)
-    {
-        if (backward)
-            hipLaunchKernelGGL((k_tile_local_staged_f32<true, TileBZ<float>::pre>), dim3(nTiles),
-                               dim3(256), 0, s, G, a, rD, cK, cJ, cI, w, st);
-        else
-            hipLaunchKernelGGL((k_tile_local_staged_f32<false, TileBZ<float>::pre>),
-                               dim3(nTiles), dim3(256), 0, s, G, a, rD, cK, cJ, cI, w, st);
-        return;
-    }
     const dim3 grid((nTiles + 3)/4);
     if (backward)
         hipLaunchKernelGGL((k_tile_local_f32<true, TileBZ<float>::pre>), grid, dim3(64), 0, s,
-                           G, a, rD, cK, cJ, cI, w, st);
+                           G, a, rD, cK, cJ, cI, w, st, r, partials);
     else
         hipLaunchKernelGGL((k_tile_local_f32<false, TileBZ<float>::pre>), grid, dim3(64), 0, s,
-                           G, a, rD, cK, cJ, cI, w, st);
+                           G, a, rD, cK, cJ, cI, w, st, nullptr, nullptr);
+    return (int32_t)grid.x;
 }
 
 } // namespace fh

@@ -45,9 +45,11 @@ void launch_tile_sweep(int kind, const TileView& G, const RowsView& R,
 void launch_pack_dir_coefs(const TileView& G, const RowsView& R, const float* val,
                            float* const dst[6], hipStream_t s);
 // forward: a = r, backward: a = w (in place); cK, cJ, cI the coefficients of
-// the pass (lower forward, upper backward)
-void launch_tile_local_f32(bool backward, const TileView& G, const float* a, const float* rD,
-                           const float* cK, const float* cJ, const float* cI, float* w,
-                           const DevState* st, hipStream_t s);
+// the pass (lower forward, upper backward).  Backward with partials: also
+// sum w*r into one partial per workgroup; returns the workgroup count.
+int32_t launch_tile_local_f32(bool backward, const TileView& G, const float* a, const float* rD,
+                              const float* cK, const float* cJ, const float* cI, float* w,
+                              const DevState* st, hipStream_t s, const float* r = nullptr,
+                              double* partials = nullptr);
 
 } // namespace fh
