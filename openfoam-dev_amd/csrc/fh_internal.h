@@ -139,10 +139,10 @@ struct fh_mesh
     bool tiled = false;
     struct TileShape
     {
-        int32_t bz = 0, nTI = 0, nTJ = 0, nTK = 0;
+        int32_t bx = 0, bz = 0, nTI = 0, nTJ = 0, nTK = 0;   // bx x bx x bz tiles
         fh::DBuf<int32_t> d_order;
     };
-    TileShape tiles[3];
+    TileShape tiles[4];
     int32_t maxTiles = 0;
     hipStream_t setupStream = nullptr;
 

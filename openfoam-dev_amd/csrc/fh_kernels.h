@@ -94,6 +94,15 @@ struct SweepData
     const T* diag;       // GS / RD
     const double* upperF;// RD_DILU: face-ordered upper
     const int32_t* perm; // PRECOND_BWD_T: library row -> original cell
+    // hex boxes, PRECOND kinds: the pass's coefficients packed per cell by
+    // direction (k, j, i), or null for the SELL values; dirShift: read the
+    // coefficient at the neighbour (k+1, j+1, i+1), i.e. the upper
+    // coefficient of a symmetric matrix from its lower-packed arrays
+    const T* dir[3];
+    int32_t dirShift;
+    // PRECOND_BWD on boxes: also sum w*dotR into one partial per tile
+    const T* dotR;
+    double* partials;
 };
 
 // Chunks of the sync-free sweep: rows of one level, at most 64 per chunk,

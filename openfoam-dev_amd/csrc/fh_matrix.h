@@ -35,6 +35,12 @@ struct fh_matrix
     // lower K, J, I and upper I, J, K per cell
     fh::DBuf<float> d_dir32[6];
     bool dir32Valid = false;
+    // FP64 direction-packed coefficients of hex boxes (FP64 box product and
+    // sweeps): lower K, J, I, and for asymmetric matrices upper I, J, K
+    fh::DBuf<double> d_dir64[6];
+    bool dir64Valid = false;
+    // tile depth of the global preconditioner sweeps on boxes
+    int32_t preBz64 = 0, preBz32 = 0;
 
     // preconditioner / smoother factors (library numbering)
     // index: 0 DIC/FDIC, 1 DILU, 2 diagonal, 3 tile-local DIC, 4 tile-local DILU
@@ -98,14 +104,14 @@ struct fh_matrix
         return v;
     }
 
-    fh::TileView tile_view(int bz) const
+    fh::TileView tile_view(int bz, int bx = 16) const
     {
         const fh_mesh::TileShape* T = nullptr;
-        for (const auto& t : mesh->tiles) if (t.bz == bz) T = &t;
-        if (!T) fh::fail(FH_ERR_INVALID, "no tile shape with depth %d", bz);
+        for (const auto& t : mesh->tiles) if (t.bz == bz && t.bx == bx) T = &t;
+        if (!T) fh::fail(FH_ERR_INVALID, "no tile shape %d x %d x %d", bx, bx, bz);
         fh::TileView v;
         v.nx = mesh->boxNx; v.ny = mesh->boxNy; v.nz = mesh->boxNz;
-        v.bx = 16; v.by = 16; v.bz = bz;
+        v.bx = bx; v.by = bx; v.bz = bz;
         v.nTI = T->nTI; v.nTJ = T->nTJ; v.nTK = T->nTK;
         v.order = T->d_order.p;
         return v;

@@ -364,13 +364,15 @@ void build_mesh(fh_mesh* m, int32_t nCells, int32_t nFaces,
     // tile shapes and their wavefront order (ascending TI+TJ+TK)
     if (m->tiled)
     {
-        const int bzs[3] = {4, 8, 16};
-        for (int t = 0; t < 3; t++)
+        // 16 x 16 x {4, 8, 16} (workgroup sweeps), 8 x 8 x 16 (one-wavefront sweeps)
+        const int bxs[4] = {16, 16, 16, 8}, bzs[4] = {4, 8, 16, 16};
+        for (int t = 0; t < 4; t++)
         {
             fh_mesh::TileShape& T = m->tiles[t];
+            T.bx = bxs[t];
             T.bz = bzs[t];
-            T.nTI = (nx + 15)/16;
-            T.nTJ = (ny + 15)/16;
+            T.nTI = (nx + T.bx - 1)/T.bx;
+            T.nTJ = (ny + T.bx - 1)/T.bx;
             T.nTK = (nz + T.bz - 1)/T.bz;
             std::vector<int32_t> ord;
             ord.reserve((size_t)T.nTI*T.nTJ*T.nTK);

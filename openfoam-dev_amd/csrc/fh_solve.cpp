@@ -36,8 +36,12 @@ void matrix_init(fh_matrix* M, fh_mesh* mesh, hipStream_t shared)
     std::memset(M->h_state, 0, sizeof(DevState));
     size_t nPart = 2*reduction_partials(mesh->nCells) + 64;
     if (mesh->tiled)
+    {
         nPart = std::max(nPart, 2*(size_t)box_spmv_blocks(mesh->boxNx, mesh->boxNy,
                                                           mesh->boxNz) + 64);
+        // one partial per tile of the fused sweep sums
+        nPart = std::max(nPart, 2*(size_t)mesh->maxTiles + 64);
+    }
     M->d_partials.alloc(nPart);
     M->d_sync.alloc(1 + (size_t)mesh->sched.nParts);
     FH_HIP(hipMemsetAsync(M->d_sync.p, 0, M->d_sync.bytes(), M->stream));
@@ -60,6 +64,15 @@ void matrix_init(fh_matrix* M, fh_mesh* mesh, hipStream_t shared)
     }
     if (mesh->tiled)
     {
+        // FOAMHIP_TILE_BZ: tile depth of the FP64 global preconditioner
+        // sweeps (4 or 8); FP32 ones run at the deeper default
+        M->preBz64 = TileBZ<double>::pre;
+        M->preBz32 = TileBZ<float>::pre;
+        if (const char* e = std::getenv("FOAMHIP_TILE_BZ"))
+        {
+            const int v = std::atoi(e);
+            if (v == TileBZ<double>::pre || v == TileBZ<double>::alt) M->preBz64 = v;
+        }
         M->d_tileFlags.alloc(std::max(mesh->maxTiles, 1));
         M->d_tileTicket.alloc(1);
         FH_HIP(hipMemsetAsync(M->d_tileFlags.p, 0, M->d_tileFlags.bytes(), M->stream));
@@ -134,7 +147,7 @@ void matrix_coeffs_updated(fh_matrix* M)
         launch_gather_values<double>(M->rows(), mesh->nnzPad, M->d_upperF.p, lo, M->d_val.p, s);
     FH_HIP(hipStreamSynchronize(s));
     M->hasCoeffs = true;
-    M->valTValid = M->sumAValid = M->f32Valid = M->dir32Valid = false;
+    M->valTValid = M->sumAValid = M->f32Valid = M->dir32Valid = M->dir64Valid = false;
     for (int i = 0; i < 5; i++) M->rDValid[i] = M->rD32Valid[i] = false;
     gamg_invalidate(M);
 }
@@ -176,6 +189,11 @@ void ensure_f32(fh_matrix* M)
     M->f32Valid = true;
 }
 
+// tile depth of the global preconditioner sweeps on a box
+template<class T> static int pre_bz(const fh_matrix* M);
+template<> int pre_bz<double>(const fh_matrix* M) { return M->preBz64; }
+template<> int pre_bz<float>(const fh_matrix* M) { return M->preBz32; }
+
 template<class T>
 static void sweep_any(fh_matrix* M, int kind, const SweepData<T>& d, DevState* st,
                       bool local = false)
@@ -183,8 +201,10 @@ static void sweep_any(fh_matrix* M, int kind, const SweepData<T>& d, DevState* s
     if (M->mesh->tiled)
     {
         TileSync ts = M->tile_sync();
-        launch_tile_sweep<T>(kind, M->tile_view(tile_bz<T>(kind)), M->rows(), d, ts, st,
-                             M->stream, local);
+        const bool pre = kind == SWEEP_PRECOND_FWD || kind == SWEEP_PRECOND_BWD
+                      || kind == SWEEP_PRECOND_BWD_T;
+        const int bz = pre && !local ? pre_bz<T>(M) : tile_bz<T>(kind);
+        launch_tile_sweep<T>(kind, M->tile_view(bz), M->rows(), d, ts, st, M->stream, local);
         M->tileTicketBase = ts.ticketBase;
         M->tileEpoch = ts.epoch;
         return;
@@ -298,9 +318,65 @@ static void ensure_dir32(fh_matrix* M)
         M->d_dir32[d].alloc(N);
         dst[d] = M->d_dir32[d].p;
     }
-    launch_pack_dir_coefs(M->tile_view(TileBZ<float>::pre), M->rows(), M->d_val32.p, dst,
-                          M->stream);
+    launch_pack_dir_coefs<float>(M->tile_view(TileBZ<float>::pre), M->rows(), M->d_val32.p, 6,
+                                 dst, M->stream);
     M->dir32Valid = true;
+}
+
+// FP64: the lower three of a symmetric matrix, all six otherwise
+static void ensure_dir64(fh_matrix* M)
+{
+    if (M->dir64Valid) return;
+    const int32_t N = M->mesh->nCells;
+    const int nDir = M->symmetric ? 3 : 6;
+    double* dst[6] = {};
+    for (int d = 0; d < nDir; d++)
+    {
+        M->d_dir64[d].alloc(N);
+        dst[d] = M->d_dir64[d].p;
+    }
+    for (int d = nDir; d < 6; d++) M->d_dir64[d].release();
+    launch_pack_dir_coefs<double>(M->tile_view(TileBZ<double>::pre), M->rows(), M->d_val.p, nDir,
+                                  dst, M->stream);
+    M->dir64Valid = true;
+}
+
+// the pass's packed coefficients (k, j, i) for the global box sweeps
+template<class T>
+static void set_sweep_dirs(fh_matrix* M, SweepData<T>& d, bool backward);
+template<>
+void set_sweep_dirs<double>(fh_matrix* M, SweepData<double>& d, bool backward)
+{
+    ensure_dir64(M);
+    if (!backward)
+    {
+        for (int q = 0; q < 3; q++) d.dir[q] = M->d_dir64[q].p;
+        d.dirShift = 0;
+    }
+    else if (M->symmetric)
+    {
+        for (int q = 0; q < 3; q++) d.dir[q] = M->d_dir64[q].p;
+        d.dirShift = 1;
+    }
+    else
+    {
+        d.dir[0] = M->d_dir64[5].p; d.dir[1] = M->d_dir64[4].p; d.dir[2] = M->d_dir64[3].p;
+        d.dirShift = 0;
+    }
+}
+template<>
+void set_sweep_dirs<float>(fh_matrix* M, SweepData<float>& d, bool backward)
+{
+    ensure_dir32(M);
+    if (!backward)
+    {
+        for (int q = 0; q < 3; q++) d.dir[q] = M->d_dir32[q].p;
+    }
+    else
+    {
+        d.dir[0] = M->d_dir32[5].p; d.dir[1] = M->d_dir32[4].p; d.dir[2] = M->d_dir32[3].p;
+    }
+    d.dirShift = 0;
 }
 
 template<class T> static const T* rD_for(fh_matrix* M, int which);
@@ -333,6 +409,39 @@ template<> void sweep_t<float>(fh_matrix* M, int kind, const SweepData<float>& d
                                DevState* st, bool local)
 {
     sweep_f(M, kind, d, st, local);
+}
+
+// One global DIC / DILU pass on a box from packed coefficients (partials:
+// the backward pass also sums w*x into one partial per tile); returns the
+// partial count.  One-wavefront 8 x 8 x 16 tiles unless the workgroup
+// kernel over 16 x 16 x bz tiles is asked for (FOAMHIP_BOX_SWEEP=wide).
+template<class T>
+static int32_t box_precond_pass(fh_matrix* M, bool backward, const SweepData<T>& d0,
+                                double* partials, DevState* st)
+{
+    SweepData<T> d = d0;
+    set_sweep_dirs<T>(M, d, backward);
+    if (backward && partials)
+    {
+        d.dotR = d.x;
+        d.partials = partials;
+    }
+    static const bool wide = [] {
+        const char* e = std::getenv("FOAMHIP_BOX_SWEEP");
+        return e && std::strcmp(e, "wide") == 0;
+    }();
+    if (!wide)
+    {
+        TileSync ts = M->tile_sync();
+        const int32_t n = launch_sweep_wave<T>(backward, M->tile_view(16, 8), d, ts, st,
+                                               M->stream);
+        M->tileTicketBase = ts.ticketBase;
+        M->tileEpoch = ts.epoch;
+        return n;
+    }
+    sweep_t<T>(M, backward ? SWEEP_PRECOND_BWD : SWEEP_PRECOND_FWD, d, st, false);
+    const TileView G = M->tile_view(pre_bz<T>(M));
+    return G.nTI*G.nTJ*G.nTK;
 }
 
 template<class T>
@@ -387,6 +496,13 @@ int32_t apply_precond(fh_matrix* M, int precon, const T* r, T* w, double* partia
             return launch_tile_local_f32(true, G, w, d.rD, M->d_dir32[5].p, M->d_dir32[4].p,
                                          M->d_dir32[3].p, w, st, M->stream, r, partials);
         }
+    }
+    if (M->mesh->tiled && !local && !transpose)
+    {
+        // global sweeps on a box: packed coefficients, and the backward pass
+        // sums w*r per tile
+        box_precond_pass<T>(M, false, d, nullptr, st);
+        return box_precond_pass<T>(M, true, d, partials, st);
     }
     sweep_t<T>(M, SWEEP_PRECOND_FWD, d, st, local);
     sweep_t<T>(M, transpose && dilu ? SWEEP_PRECOND_BWD_T : SWEEP_PRECOND_BWD, d, st, local);
@@ -448,6 +564,37 @@ void pcg_loop(fh_matrix* M, int precon, T* psi, T* r, T* p, T* w, int32_t maxIte
     const T* val = val_for<T>(M, false);
     if (precon != FH_PRECON_NONE && factor_kind(precon) >= 0)
         (void)rD_for<T>(M, factor_kind(precon));
+    if constexpr (std::is_same<T, double>::value)
+    {
+        const fh_mesh* mesh = M->mesh;
+        if (mesh->tiled && (mesh->boxNx & 3) == 0)
+        {
+            // boxes: p = w + beta*p fused into the direction-packed product
+            // (two p buffers in turn, A p in its own buffer); same values as
+            // the separate kernels below
+            ensure_dir64(M);
+            const double* dir[6];
+            for (int q = 0; q < 6; q++) dir[q] = M->d_dir64[q].p;
+            M->v[7].alloc(N);
+            M->v[8].alloc(N);
+            double* pBuf[2] = {p, M->v[7].p};
+            double* ap = M->v[8].p;
+            run_batches(M, maxIter, [&](int32_t seq)
+            {
+                const int32_t nbw = apply_precond<T>(M, precon, r, w, part, st, false);
+                launch_finalize_blocks(nbw, N, part, FIN_PCG_WARA, st, 0, M->stream);
+                double* pNew = pBuf[(seq + 1) & 1];
+                const int32_t nb = launch_box_pcg_spmv<double>(mesh->boxNx, mesh->boxNy,
+                                                               mesh->boxNz, diag, dir,
+                                                               M->symmetric, w, pBuf[seq & 1],
+                                                               pNew, ap, part, st, M->stream);
+                launch_finalize_blocks(nb, N, part, FIN_PCG_WAPA, st, 0, M->stream);
+                launch_pcg_update<T>(N, psi, r, pNew, ap, part, st, M->stream);
+                launch_finalize(N, part, FIN_PCG_RES, st, 0, M->stream);
+            });
+            return;
+        }
+    }
     run_batches(M, maxIter, [&](int32_t)
     {
         const int32_t nb = apply_precond<T>(M, precon, r, w, part, st, false);
@@ -716,7 +863,7 @@ static void mixed_pcg(fh_matrix* M, const fh_controls* c, double* psi, const dou
         if (fuseP)
         {
             p32 = pBuf[(seq + 1) & 1];
-            const int32_t nb = launch_box_pcg_spmv_f32(mesh->boxNx, mesh->boxNy, mesh->boxNz,
+            const int32_t nb = launch_box_pcg_spmv<float>(mesh->boxNx, mesh->boxNy, mesh->boxNz,
                                                        diag, dir, M->symmetric, w32,
                                                        pBuf[seq & 1], p32, ap, part, st,
                                                        M->stream);
@@ -1030,32 +1177,51 @@ static void profile_t(fh_matrix* M, int kernel, int precon, int reps, double* ms
         fail(FH_ERR_INVALID, "sweep profiling needs DIC, FDIC, DILU, tileDIC or tileDILU");
     const T* rD = which >= 0 ? rD_for<T>(M, which) : nullptr;
     const bool local = which >= 3 && M->mesh->tiled;
-    if (std::is_same<T, float>::value && M->mesh->tiled) ensure_dir32(M);
-    // the box product checks a solver state: a zeroed one, never done
+    const fh_mesh* mesh = M->mesh;
+    // boxes with nx a multiple of 4 run the fused PCG product and the
+    // packed-coefficient sweeps with their fused w.r sums (the kernels of
+    // the PCG solves); other meshes the general kernels
+    const bool box = mesh->tiled;
+    const bool boxX4 = box && (mesh->boxNx & 3) == 0;
+    const T* dir[6] = {};
+    if (box)
+    {
+        if constexpr (std::is_same<T, float>::value)
+        {
+            ensure_dir32(M);
+            for (int q = 0; q < 6; q++) dir[q] = M->d_dir32[q].p;
+        }
+        else
+        {
+            ensure_dir64(M);
+            for (int q = 0; q < 6; q++) dir[q] = M->d_dir64[q].p;
+        }
+    }
+    // the box product reads a solver state: a steady-state iteration
+    // (p = w + beta*pOld), never done
     DBuf<DevState> pst;
     pst.alloc(1);
-    FH_HIP(hipMemsetAsync(pst.p, 0, sizeof(DevState), M->stream));
+    DevState hs{};
+    hs.nIterations = 1;
+    hs.beta = 0.5;
+    FH_HIP(hipMemcpyAsync(pst.p, &hs, sizeof(DevState), hipMemcpyHostToDevice, M->stream));
     const DevState* pstate = pst.p;
+    DBuf<T> p2;
+    if (boxX4) { p2.alloc(N); FH_HIP(hipMemsetAsync(p2.p, 0, p2.bytes(), M->stream)); }
     SweepData<T> d{};
     d.val = val; d.rD = rD; d.x = x.p; d.w = y.p;
+    double* part = M->d_partials.p;
     auto once = [&]()
     {
         switch (kernel)
         {
             case FH_KERNEL_SPMV:
-                if constexpr (std::is_same<T, float>::value)
+                if (boxX4)
                 {
-                    if (M->mesh->tiled)
-                    {
-                        // the product of the mixed inner iterations on boxes
-                        const float* dir[6];
-                        for (int q = 0; q < 6; q++) dir[q] = M->d_dir32[q].p;
-                        launch_box_spmv_dot_f32(M->mesh->boxNx, M->mesh->boxNy,
-                                                M->mesh->boxNz, diag, dir, M->symmetric,
-                                                x.p, y.p, x.p, M->d_partials.p, pstate,
-                                                M->stream);
-                        break;
-                    }
+                    launch_box_pcg_spmv<T>(mesh->boxNx, mesh->boxNy, mesh->boxNz, diag, dir,
+                                           M->symmetric, x.p, z.p, p2.p, y.p, part, pstate,
+                                           M->stream);
+                    break;
                 }
                 launch_amul<T>(M->rows(), diag, val, x.p, y.p, nullptr, M->stream);
                 break;
@@ -1072,13 +1238,18 @@ static void profile_t(fh_matrix* M, int kernel, int precon, int reps, double* ms
                         if (bwd)
                             launch_tile_local_f32(true, G, y.p, rD, M->d_dir32[5].p,
                                                   M->d_dir32[4].p, M->d_dir32[3].p, y.p,
-                                                  nullptr, M->stream);
+                                                  nullptr, M->stream, x.p, part);
                         else
                             launch_tile_local_f32(false, G, x.p, rD, M->d_dir32[0].p,
                                                   M->d_dir32[1].p, M->d_dir32[2].p, y.p,
                                                   nullptr, M->stream);
                         break;
                     }
+                }
+                if (box && !local)
+                {
+                    box_precond_pass<T>(M, bwd, d, bwd ? part : nullptr, nullptr);
+                    break;
                 }
                 sweep_t<T>(M, bwd ? SWEEP_PRECOND_BWD : SWEEP_PRECOND_FWD, d, nullptr, local);
                 break;
@@ -1102,20 +1273,25 @@ static void profile_t(fh_matrix* M, int kernel, int precon, int reps, double* ms
     switch (kernel)
     {
         case FH_KERNEL_SPMV:
-            // boxes in FP32: x, diag, y and one coefficient per face (the
-            // symmetric upper one read from the lower array of the
-            // neighbour); otherwise diag, x, y, row info and (value,
-            // column) per off-diagonal
-            if (w == 4 && M->mesh->tiled)
-                *bytes = N*3*w + (M->symmetric ? 1.0 : 2.0)*F*w;
+            // boxes: the fused PCG product reads w, pOld, diag and one
+            // coefficient per face (the symmetric upper one from the lower
+            // array of the neighbour) and writes pNew and y; otherwise
+            // diag, x, y, row info and (value, column) per off-diagonal
+            if (boxX4)
+                *bytes = N*5*w + (M->symmetric ? 1.0 : 2.0)*F*w;
             else
                 *bytes = N*(3*w + 4) + 2.0*F*(w + 4);
             break;
-        case FH_KERNEL_SWEEP_FORWARD: // input, rD, output, row info + one triangle
+        case FH_KERNEL_SWEEP_FORWARD:
         case FH_KERNEL_SWEEP_BACKWARD:
-            // the tile-local FP32 kernels read packed coefficients: input,
-            // rD, output and one coefficient per face, no rows or columns
-            *bytes = (local && w == 4) ? N*3*w + 1.0*F*w : N*(3*w + 4) + 1.0*F*(w + 4);
+            // boxes (packed coefficients): input, rD, output and one
+            // coefficient per face, plus r for the fused w.r sum of the
+            // backward pass; otherwise input, rD, output, row info and
+            // (value, column) of one triangle
+            if (box)
+                *bytes = N*(kernel == FH_KERNEL_SWEEP_BACKWARD ? 4 : 3)*w + 1.0*F*w;
+            else
+                *bytes = N*(3*w + 4) + 1.0*F*(w + 4);
             break;
         case FH_KERNEL_AXPY_NORM:     // psi, r read+write, p, w read
             *bytes = 6.0*N*w; break;

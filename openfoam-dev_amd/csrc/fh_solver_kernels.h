@@ -54,13 +54,15 @@ int32_t launch_box_spmv_dot_f32(int32_t nx, int32_t ny, int32_t nz, const float*
                                 const float* const dir[6], bool symmetric, const float* x,
                                 float* y, const float* z, double* partials,
                                 const DevState* st, hipStream_t s);
-// PCG's p = w + beta*pOld (first iteration: w) fused into the box product:
-// writes pNew and y = A pNew with sum y*pNew; nx must be a multiple of 4
-// (returns -1 otherwise, nothing launched)
-int32_t launch_box_pcg_spmv_f32(int32_t nx, int32_t ny, int32_t nz, const float* diag,
-                                const float* const dir[6], bool symmetric, const float* w,
-                                const float* pOld, float* pNew, float* y, double* partials,
-                                const DevState* st, hipStream_t s);
+// PCG's p = w + beta*pOld (first iteration: w) fused into the box product
+// (FP32 or FP64 direction-packed coefficients): writes pNew and y = A pNew
+// with sum y*pNew; nx must be a multiple of 4 (returns -1 otherwise,
+// nothing launched)
+template<class T>
+int32_t launch_box_pcg_spmv(int32_t nx, int32_t ny, int32_t nz, const T* diag,
+                            const T* const dir[6], bool symmetric, const T* w,
+                            const T* pOld, T* pNew, T* y, double* partials,
+                            const DevState* st, hipStream_t s);
 // finalize over an explicit number of partial blocks
 void launch_finalize_blocks(int32_t nBlocks, int32_t n, const double* partials, int32_t op,
                             DevState* st, int32_t seq, hipStream_t s);

@@ -7,6 +7,8 @@
 #include "fh_solver_kernels.h"
 
 #include <algorithm>
+#include <cstdlib>
+#include <cstring>
 
 #include <cfloat>
 #include <cmath>
@@ -231,27 +233,58 @@ void k_box_spmv_dot_f32(int32_t nx, int32_t ny, int32_t nz,
     write_partials(s0, 0.0, partials, gridDim.x);
 }
 
-// Four cells along i per thread with 16-byte loads (nx a multiple of 4):
+// Four consecutive cells of one array: 16-byte (float) or two 16-byte
+// (double) accesses; the index is a multiple of four
+template<class T> struct Quad;
+template<> struct Quad<float>
+{
+    __device__ static void load(const float* p, int64_t i, float (&o)[4])
+    {
+        const float4 q = *reinterpret_cast<const float4*>(p + i);
+        o[0] = q.x; o[1] = q.y; o[2] = q.z; o[3] = q.w;
+    }
+    __device__ static void store(float* p, int64_t i, const float (&v)[4])
+    {
+        *reinterpret_cast<float4*>(p + i) = make_float4(v[0], v[1], v[2], v[3]);
+    }
+};
+template<> struct Quad<double>
+{
+    __device__ static void load(const double* p, int64_t i, double (&o)[4])
+    {
+        const double2 a = *reinterpret_cast<const double2*>(p + i);
+        const double2 b = *reinterpret_cast<const double2*>(p + i + 2);
+        o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y;
+    }
+    __device__ static void store(double* p, int64_t i, const double (&v)[4])
+    {
+        *reinterpret_cast<double2*>(p + i) = make_double2(v[0], v[1]);
+        *reinterpret_cast<double2*>(p + i + 2) = make_double2(v[2], v[3]);
+    }
+};
+
+// Four cells along i per thread with vector loads (nx a multiple of 4):
 // blocks of 64 threads x 4 rows = 256 x 4 cells of one k-plane, the same
 // XCD-aware block order, and the same per-cell sums as above.
 // FUSE_P: the product of p = w + beta*pOld (PCG.C:146-152, the pcg_p
 // expression) computed on the fly from w and pOld for every cell the
 // product reads; the new p of the block's cells is written to pNew (x is
 // pOld here and z is pNew's values, i.e. the dot is p.Ap).
-template<bool FUSE_P>
+template<class T, bool FUSE_P>
 __global__ __launch_bounds__(256)
-void k_box_spmv_dot_f32x4(int32_t nx, int32_t ny, int32_t nz,
-                          const float* __restrict__ diag,
-                          const float* __restrict__ lK, const float* __restrict__ lJ,
-                          const float* __restrict__ lI, const float* __restrict__ uI,
-                          const float* __restrict__ uJ, const float* __restrict__ uK,
-                          const float* __restrict__ x, float* __restrict__ y,
-                          const float* z, double* partials, const DevState* st,
-                          const float* __restrict__ wv, float* __restrict__ pNew)
+void k_box_spmv_dot_x4(int32_t nx, int32_t ny, int32_t nz,
+                       const T* __restrict__ diag,
+                       const T* __restrict__ lK, const T* __restrict__ lJ,
+                       const T* __restrict__ lI, const T* __restrict__ uI,
+                       const T* __restrict__ uJ, const T* __restrict__ uK,
+                       const T* __restrict__ x, T* __restrict__ y,
+                       const T* z, double* partials, const DevState* st,
+                       const T* __restrict__ wv, T* __restrict__ pNew)
 {
     if (st->done) return;
+    typedef Quad<T> Q;
     const bool first = FUSE_P && st->nIterations == 0;
-    const float beta = FUSE_P ? (float)st->beta : 0.f;
+    const T beta = FUSE_P ? (T)st->beta : T(0);
     const int32_t gx = (nx + 255)/256, gy = (ny + 3)/4;
     const int32_t nReal = gx*gy*nz;
     // XCD-aware order: the blocks of XCD (b mod 8) stride over one eighth
@@ -271,62 +304,56 @@ void k_box_spmv_dot_f32x4(int32_t nx, int32_t ny, int32_t nz,
         {
             const int64_t sxy = (int64_t)nx*ny;
             const int64_t c = i0 + (int64_t)nx*gj + sxy*gk;
-            typedef float4 f4;
-            auto ld = [](const float* p, int64_t i) { return *reinterpret_cast<const f4*>(p + i); };
-            // the product's input at cell i (4-wide / scalar)
-            auto px4 = [&](int64_t i) -> f4
+            // the product's input at cells i..i+3 / cell i
+            auto px4 = [&](int64_t i, T (&o)[4])
             {
-                if (!FUSE_P) return ld(x, i);
-                const f4 wq = ld(wv, i);
-                if (first) return wq;
-                const f4 pq = ld(x, i);
-                return make_float4(wq.x + beta*pq.x, wq.y + beta*pq.y, wq.z + beta*pq.z,
-                                   wq.w + beta*pq.w);
+                if (!FUSE_P) { Q::load(x, i, o); return; }
+                Q::load(wv, i, o);
+                if (first) return;
+                T pq[4];
+                Q::load(x, i, pq);
+#pragma unroll
+                for (int q = 0; q < 4; q++) o[q] = o[q] + beta*pq[q];
             };
-            auto px1 = [&](int64_t i) -> float
+            auto px1 = [&](int64_t i) -> T
             {
                 if (!FUSE_P) return x[i];
                 return first ? wv[i] : wv[i] + beta*x[i];
             };
-            const f4 X = px4(c), Dg = ld(diag, c);
-            const f4 LK = ld(lK, c), LJ = ld(lJ, c), LI = ld(lI, c);
-            const f4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
-            const f4 XK = gk > 0 ? px4(c - sxy) : zero;
-            const f4 XJ = gj > 0 ? px4(c - nx) : zero;
-            const f4 XJp = gj < ny - 1 ? px4(c + nx) : zero;
-            const f4 XKp = gk < nz - 1 ? px4(c + sxy) : zero;
-            const float xm = i0 > 0 ? px1(c - 1) : 0.f;
-            const float xp = i0 + 4 < nx ? px1(c + 4) : 0.f;
-            if (FUSE_P) *reinterpret_cast<f4*>(pNew + c) = X;
-            f4 UI, UJ, UK;
+            T X[4], dg[4], lk[4], lj[4], li[4], ui[4], uj[4], uk[4];
+            T xk[4] = {}, xj[4] = {}, xjp[4] = {}, xkp[4] = {};
+            px4(c, X);
+            Q::load(diag, c, dg);
+            Q::load(lK, c, lk); Q::load(lJ, c, lj); Q::load(lI, c, li);
+            if (gk > 0) px4(c - sxy, xk);
+            if (gj > 0) px4(c - nx, xj);
+            if (gj < ny - 1) px4(c + nx, xjp);
+            if (gk < nz - 1) px4(c + sxy, xkp);
+            const T xm = i0 > 0 ? px1(c - 1) : T(0);
+            const T xp = i0 + 4 < nx ? px1(c + 4) : T(0);
+            if (FUSE_P) Q::store(pNew, c, X);
             if (uI)
             {
-                UI = ld(uI, c); UJ = ld(uJ, c); UK = ld(uK, c);
+                Q::load(uI, c, ui); Q::load(uJ, c, uj); Q::load(uK, c, uk);
             }
             else
             {
                 // symmetric: the upper coefficient of a face is the lower
                 // one of its upper cell
-                const float li4 = i0 + 4 < nx ? lI[c + 4] : 0.f;
-                UI = make_float4(LI.y, LI.z, LI.w, li4);
-                UJ = gj < ny - 1 ? ld(lJ, c + nx) : zero;
-                UK = gk < nz - 1 ? ld(lK, c + sxy) : zero;
+                ui[0] = li[1]; ui[1] = li[2]; ui[2] = li[3];
+                ui[3] = i0 + 4 < nx ? lI[c + 4] : T(0);
+                if (gj < ny - 1) Q::load(lJ, c + nx, uj);
+                else { uj[0] = uj[1] = uj[2] = uj[3] = T(0); }
+                if (gk < nz - 1) Q::load(lK, c + sxy, uk);
+                else { uk[0] = uk[1] = uk[2] = uk[3] = T(0); }
             }
-            const float xs[6] = {xm, X.x, X.y, X.z, X.w, xp};
-            const float dg[4] = {Dg.x, Dg.y, Dg.z, Dg.w};
-            const float lk[4] = {LK.x, LK.y, LK.z, LK.w}, lj[4] = {LJ.x, LJ.y, LJ.z, LJ.w};
-            const float li[4] = {LI.x, LI.y, LI.z, LI.w};
-            const float ui[4] = {UI.x, UI.y, UI.z, UI.w}, uj[4] = {UJ.x, UJ.y, UJ.z, UJ.w};
-            const float uk[4] = {UK.x, UK.y, UK.z, UK.w};
-            const float xk[4] = {XK.x, XK.y, XK.z, XK.w}, xj[4] = {XJ.x, XJ.y, XJ.z, XJ.w};
-            const float xjp[4] = {XJp.x, XJp.y, XJp.z, XJp.w};
-            const float xkp[4] = {XKp.x, XKp.y, XKp.z, XKp.w};
-            float out[4];
+            const T xs[6] = {xm, X[0], X[1], X[2], X[3], xp};
+            T out[4];
 #pragma unroll
             for (int q = 0; q < 4; q++)
             {
                 const int32_t gi = i0 + q;
-                float acc = dg[q]*xs[q + 1];
+                T acc = dg[q]*xs[q + 1];
                 if (gk > 0) acc += lk[q]*xk[q];
                 if (gj > 0) acc += lj[q]*xj[q];
                 if (gi > 0) acc += li[q]*xs[q];
@@ -335,42 +362,200 @@ void k_box_spmv_dot_f32x4(int32_t nx, int32_t ny, int32_t nz,
                 if (gk < nz - 1) acc += uk[q]*xkp[q];
                 out[q] = acc;
             }
-            *reinterpret_cast<f4*>(y + c) = make_float4(out[0], out[1], out[2], out[3]);
-            if (FUSE_P)
+            Q::store(y, c, out);
+            T Z[4];
+            if (FUSE_P) { for (int q = 0; q < 4; q++) Z[q] = X[q]; }
+            else if (z) Q::load(z, c, Z);
+            else { for (int q = 0; q < 4; q++) Z[q] = out[q]; }
+            s0 += ((double)out[0]*(double)Z[0] + (double)out[1]*(double)Z[1])
+               + ((double)out[2]*(double)Z[2] + (double)out[3]*(double)Z[3]);
+        }
+    }
+    write_partials(s0, 0.0, partials, gridDim.x);
+}
+
+// The same fused product marching along k: a thread owns four cells along i
+// of one row (i0..i0+3, j) for a run of kc planes and keeps the product's
+// input of planes k-1, k, k+1 in registers (and, symmetric, the lower k
+// coefficient of plane k+1, which is the upper one of plane k), so every
+// value is read once from memory; only the j-1 / j+1 rows and the two i
+// neighbours come from the caches.  Blocks are 64 x 4 rows; an XCD
+// (hardware block id mod 8) takes a contiguous range of (row block, plane
+// run) pairs, so the rows a block shares with its neighbours sit in that
+// XCD's L2.  Same per-cell sums as k_box_spmv_dot_x4.
+template<class T, bool FUSE_P>
+__global__ __launch_bounds__(256)
+void k_box_spmv_march(int32_t nx, int32_t ny, int32_t nz, int32_t kc,
+                      const T* __restrict__ diag,
+                      const T* __restrict__ lK, const T* __restrict__ lJ,
+                      const T* __restrict__ lI, const T* __restrict__ uI,
+                      const T* __restrict__ uJ, const T* __restrict__ uK,
+                      const T* __restrict__ x, T* __restrict__ y,
+                      const T* z, double* partials, const DevState* st,
+                      const T* __restrict__ wv, T* __restrict__ pNew)
+{
+    if (st->done) return;
+    typedef Quad<T> Q;
+    const bool first = FUSE_P && st->nIterations == 0;
+    const T beta = FUSE_P ? (T)st->beta : T(0);
+    const int32_t nIB = (nx + 255)/256, nJB = (ny + 3)/4, nKB = (nz + kc - 1)/kc;
+    const int32_t nReal = nIB*nJB*nKB;
+    const int32_t slab = (nReal + 7)/8;
+    const int32_t stride = gridDim.x/8;
+    const int64_t sxy = (int64_t)nx*ny;
+    double s0 = 0.0;
+    for (int32_t local = (int32_t)(blockIdx.x/8); local < slab; local += stride)
+    {
+        const int32_t logical = (int32_t)(blockIdx.x % 8)*slab + local;
+        if (logical >= nReal) break;
+        const int32_t ib = logical % nIB, jb = (logical/nIB) % nJB, kb = logical/(nIB*nJB);
+        const int32_t i0 = ib*256 + 4*(int32_t)(threadIdx.x & 63);
+        const int32_t gj = jb*4 + (int32_t)(threadIdx.x >> 6);
+        if (i0 >= nx || gj >= ny) continue;
+        const int32_t k0 = kb*kc, k1 = min(nz, k0 + kc);
+        const int64_t rowc = i0 + (int64_t)nx*gj;
+        // the product's input at cells i..i+3 / cell i
+        auto px4 = [&](int64_t i, T (&o)[4])
+        {
+            if (!FUSE_P) { Q::load(x, i, o); return; }
+            Q::load(wv, i, o);
+            if (first) return;
+            T pq[4];
+            Q::load(x, i, pq);
+#pragma unroll
+            for (int q = 0; q < 4; q++) o[q] = o[q] + beta*pq[q];
+        };
+        auto px1 = [&](int64_t i) -> T
+        {
+            if (!FUSE_P) return x[i];
+            return first ? wv[i] : wv[i] + beta*x[i];
+        };
+        T xm[4] = {}, x0[4], xn[4] = {}, lkc[4], lkn[4] = {};
+        if (k0 > 0) px4(rowc + sxy*(k0 - 1), xm);
+        px4(rowc + sxy*k0, x0);
+        Q::load(lK, rowc + sxy*k0, lkc);
+        for (int32_t k = k0; k < k1; k++)
+        {
+            const int64_t c = rowc + sxy*k;
+            const bool up = k < nz - 1;
+            if (up)
             {
-                s0 += ((double)out[0]*(double)X.x + (double)out[1]*(double)X.y)
-                   + ((double)out[2]*(double)X.z + (double)out[3]*(double)X.w);
+                px4(c + sxy, xn);
+                if (!uI) Q::load(lK, c + sxy, lkn);
             }
-            else if (z)
+            T dg[4], lj[4], li[4], ui[4], uj[4] = {}, uk[4] = {};
+            T xj[4] = {}, xjp[4] = {};
+            Q::load(diag, c, dg);
+            Q::load(lJ, c, lj);
+            Q::load(lI, c, li);
+            if (gj > 0) px4(c - nx, xj);
+            if (gj < ny - 1) px4(c + nx, xjp);
+            const T xw = i0 > 0 ? px1(c - 1) : T(0);
+            const T xe = i0 + 4 < nx ? px1(c + 4) : T(0);
+            if (uI)
             {
-                const f4 Z = ld(z, c);
-                s0 += ((double)out[0]*(double)Z.x + (double)out[1]*(double)Z.y)
-                   + ((double)out[2]*(double)Z.z + (double)out[3]*(double)Z.w);
+                Q::load(lK, c, lkc);
+                Q::load(uI, c, ui); Q::load(uJ, c, uj); Q::load(uK, c, uk);
             }
             else
             {
-                s0 += ((double)out[0]*(double)out[0] + (double)out[1]*(double)out[1])
-                   + ((double)out[2]*(double)out[2] + (double)out[3]*(double)out[3]);
+                // symmetric: the upper coefficient of a face is the lower
+                // one of its upper cell
+                ui[0] = li[1]; ui[1] = li[2]; ui[2] = li[3];
+                ui[3] = i0 + 4 < nx ? lI[c + 4] : T(0);
+                if (gj < ny - 1) Q::load(lJ, c + nx, uj);
+#pragma unroll
+                for (int q = 0; q < 4; q++) uk[q] = lkn[q];
+            }
+            if (FUSE_P) Q::store(pNew, c, x0);
+            const T xs[6] = {xw, x0[0], x0[1], x0[2], x0[3], xe};
+            T out[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+            {
+                const int32_t gi = i0 + q;
+                T acc = dg[q]*xs[q + 1];
+                if (k > 0) acc += lkc[q]*xm[q];
+                if (gj > 0) acc += lj[q]*xj[q];
+                if (gi > 0) acc += li[q]*xs[q];
+                if (gi < nx - 1) acc += ui[q]*xs[q + 2];
+                if (gj < ny - 1) acc += uj[q]*xjp[q];
+                if (up) acc += uk[q]*xn[q];
+                out[q] = acc;
+            }
+            Q::store(y, c, out);
+            T Z[4];
+            if (FUSE_P) { for (int q = 0; q < 4; q++) Z[q] = x0[q]; }
+            else if (z) Q::load(z, c, Z);
+            else { for (int q = 0; q < 4; q++) Z[q] = out[q]; }
+            s0 += ((double)out[0]*(double)Z[0] + (double)out[1]*(double)Z[1])
+               + ((double)out[2]*(double)Z[2] + (double)out[3]*(double)Z[3]);
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+            {
+                xm[q] = x0[q];
+                x0[q] = xn[q];
+                lkc[q] = lkn[q];
+                xn[q] = T(0);
+                lkn[q] = T(0);
             }
         }
     }
     write_partials(s0, 0.0, partials, gridDim.x);
 }
 
-int32_t launch_box_pcg_spmv_f32(int32_t nx, int32_t ny, int32_t nz, const float* diag,
-                                const float* const dir[6], bool symmetric, const float* w,
-                                const float* pOld, float* pNew, float* y, double* partials,
-                                const DevState* st, hipStream_t s)
+// planes per marching run: about 1024 runs over the box, 2..32 planes each
+static int32_t box_march_planes(int32_t nx, int32_t ny, int32_t nz)
+{
+    const int64_t rows = (int64_t)((nx + 255)/256)*((ny + 3)/4);
+    const int64_t kc = (int64_t)nz*rows/1024;
+    return (int32_t)std::max<int64_t>(2, std::min<int64_t>(32, kc));
+}
+
+static int32_t box_x4_blocks(int32_t nx, int32_t ny, int32_t nz)
+{
+    const int64_t nb4 = (int64_t)((nx + 255)/256)*((ny + 3)/4)*nz;
+    return (int32_t)std::min<int64_t>(8*((nb4 + 7)/8), 2048);
+}
+
+template<class T>
+int32_t launch_box_pcg_spmv(int32_t nx, int32_t ny, int32_t nz, const T* diag,
+                            const T* const dir[6], bool symmetric, const T* w,
+                            const T* pOld, T* pNew, T* y, double* partials,
+                            const DevState* st, hipStream_t s)
 {
     if (nx & 3) return -1;
-    const int64_t nb4 = (int64_t)((nx + 255)/256)*((ny + 3)/4)*nz;
-    const int32_t nb = (int32_t)std::min<int64_t>(8*((nb4 + 7)/8), 2048);
-    hipLaunchKernelGGL(k_box_spmv_dot_f32x4<true>, dim3(nb), dim3(256), 0, s, nx, ny, nz, diag,
-                       dir[0], dir[1], dir[2], symmetric ? nullptr : dir[3],
+    static const bool plane = [] {
+        const char* e = std::getenv("FOAMHIP_BOX_SPMV");
+        return e && std::strcmp(e, "plane") == 0;
+    }();
+    if (plane)
+    {
+        const int32_t nb = box_x4_blocks(nx, ny, nz);
+        hipLaunchKernelGGL((k_box_spmv_dot_x4<T, true>), dim3(nb), dim3(256), 0, s, nx, ny, nz,
+                           diag, dir[0], dir[1], dir[2], symmetric ? nullptr : dir[3],
+                           symmetric ? nullptr : dir[4], symmetric ? nullptr : dir[5], pOld, y,
+                           nullptr, partials, st, w, pNew);
+        return nb;
+    }
+    const int32_t kc = box_march_planes(nx, ny, nz);
+    const int64_t runs = (int64_t)((nx + 255)/256)*((ny + 3)/4)*((nz + kc - 1)/kc);
+    const int32_t nb = (int32_t)std::min<int64_t>(8*((runs + 7)/8), 2048);
+    hipLaunchKernelGGL((k_box_spmv_march<T, true>), dim3(nb), dim3(256), 0, s, nx, ny, nz, kc,
+                       diag, dir[0], dir[1], dir[2], symmetric ? nullptr : dir[3],
                        symmetric ? nullptr : dir[4], symmetric ? nullptr : dir[5], pOld, y,
                        nullptr, partials, st, w, pNew);
     return nb;
 }
+
+template int32_t launch_box_pcg_spmv<float>(int32_t, int32_t, int32_t, const float*,
+                                            const float* const[6], bool, const float*,
+                                            const float*, float*, float*, double*,
+                                            const DevState*, hipStream_t);
+template int32_t launch_box_pcg_spmv<double>(int32_t, int32_t, int32_t, const double*,
+                                             const double* const[6], bool, const double*,
+                                             const double*, double*, double*, double*,
+                                             const DevState*, hipStream_t);
 
 int32_t launch_box_spmv_dot_f32(int32_t nx, int32_t ny, int32_t nz, const float* diag,
                                 const float* const dir[6], bool symmetric, const float* x,
@@ -379,9 +564,8 @@ int32_t launch_box_spmv_dot_f32(int32_t nx, int32_t ny, int32_t nz, const float*
 {
     if ((nx & 3) == 0)
     {
-        const int64_t nb4 = (int64_t)((nx + 255)/256)*((ny + 3)/4)*nz;
-        const int32_t nb = (int32_t)std::min<int64_t>(8*((nb4 + 7)/8), 2048);
-        hipLaunchKernelGGL(k_box_spmv_dot_f32x4<false>, dim3(nb), dim3(256), 0, s, nx, ny, nz,
+        const int32_t nb = box_x4_blocks(nx, ny, nz);
+        hipLaunchKernelGGL((k_box_spmv_dot_x4<float, false>), dim3(nb), dim3(256), 0, s, nx, ny, nz,
                            diag, dir[0], dir[1], dir[2], symmetric ? nullptr : dir[3],
                            symmetric ? nullptr : dir[4], symmetric ? nullptr : dir[5], x, y, z,
                            partials, st, nullptr, nullptr);

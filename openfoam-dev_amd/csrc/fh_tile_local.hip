@@ -13,12 +13,15 @@ namespace fh
 
 // Per-cell coefficient gather: for each cell of the box, the coefficient of
 // its lower (K, J, I) and upper (I, J, K) neighbour entries from the SELL
-// values; 0 where the neighbour does not exist.
+// values; 0 where the neighbour does not exist.  With three destinations only
+// the lower ones (a symmetric matrix: the upper coefficient of a cell is the
+// lower one of its upper neighbour).
+template<class T>
 __global__ void k_pack_dir_coefs(int32_t nx, int32_t ny, int32_t nz, RowsView R,
-                                 const float* __restrict__ val, float* __restrict__ lK,
-                                 float* __restrict__ lJ, float* __restrict__ lI,
-                                 float* __restrict__ uI, float* __restrict__ uJ,
-                                 float* __restrict__ uK)
+                                 const T* __restrict__ val, T* __restrict__ lK,
+                                 T* __restrict__ lJ, T* __restrict__ lI,
+                                 T* __restrict__ uI, T* __restrict__ uJ,
+                                 T* __restrict__ uK)
 {
     const int64_t c = (int64_t)blockIdx.x*TL_B + threadIdx.x;
     const int64_t n = (int64_t)nx*ny*nz;
@@ -26,26 +29,35 @@ __global__ void k_pack_dir_coefs(int32_t nx, int32_t ny, int32_t nz, RowsView R,
     const int32_t gi = (int32_t)(c % nx), gj = (int32_t)((c/nx) % ny), gk = (int32_t)(c/((int64_t)nx*ny));
     const int32_t base = R.sliceStart[c >> 6] + (int32_t)(c & 63);
     int e = 0;
-    float vK = 0.f, vJ = 0.f, vI = 0.f;
+    T vK = 0, vJ = 0, vI = 0;
     if (gk > 0) vK = val[base + ((e++) << 6)];
     if (gj > 0) vJ = val[base + ((e++) << 6)];
     if (gi > 0) vI = val[base + ((e++) << 6)];
-    float wI = 0.f, wJ = 0.f, wK = 0.f;
+    lK[c] = vK; lJ[c] = vJ; lI[c] = vI;
+    if (!uI) return;
+    T wI = 0, wJ = 0, wK = 0;
     if (gi < nx - 1) wI = val[base + ((e++) << 6)];
     if (gj < ny - 1) wJ = val[base + ((e++) << 6)];
     if (gk < nz - 1) wK = val[base + ((e++) << 6)];
-    lK[c] = vK; lJ[c] = vJ; lI[c] = vI;
     uI[c] = wI; uJ[c] = wJ; uK[c] = wK;
 }
 
-void launch_pack_dir_coefs(const TileView& G, const RowsView& R, const float* val,
-                           float* const dst[6], hipStream_t s)
+template<class T>
+void launch_pack_dir_coefs(const TileView& G, const RowsView& R, const T* val, int nDst,
+                           T* const dst[6], hipStream_t s)
 {
     const int64_t n = (int64_t)G.nx*G.ny*G.nz;
-    hipLaunchKernelGGL(k_pack_dir_coefs, dim3((unsigned)((n + TL_B - 1)/TL_B)), dim3(TL_B), 0,
-                       s, G.nx, G.ny, G.nz, R, val, dst[0], dst[1], dst[2], dst[3], dst[4],
-                       dst[5]);
+    T* up[3] = {nullptr, nullptr, nullptr};
+    if (nDst == 6) { up[0] = dst[3]; up[1] = dst[4]; up[2] = dst[5]; }
+    else if (nDst != 3) fail(FH_ERR_INVALID, "internal: %d direction arrays", nDst);
+    hipLaunchKernelGGL(k_pack_dir_coefs<T>, dim3((unsigned)((n + TL_B - 1)/TL_B)), dim3(TL_B), 0,
+                       s, G.nx, G.ny, G.nz, R, val, dst[0], dst[1], dst[2], up[0], up[1], up[2]);
 }
+
+template void launch_pack_dir_coefs<float>(const TileView&, const RowsView&, const float*, int,
+                                           float* const[6], hipStream_t);
+template void launch_pack_dir_coefs<double>(const TileView&, const RowsView&, const double*, int,
+                                            double* const[6], hipStream_t);
 
 // Row walk of a tile: a lane owns one i-row (j, k) of 16 cells and updates
 // them in order along i (the i-neighbour in a register), the rows of one

@@ -25,8 +25,9 @@ struct TileSync
 
 // tile depth per kind and precision (LDS budget: the tile's staged inputs)
 template<class T> struct TileBZ;
-template<> struct TileBZ<double> { static constexpr int pre = 8; static constexpr int gs = 4; };
-template<> struct TileBZ<float> { static constexpr int pre = 16; static constexpr int gs = 8; };
+// (alt: the shallower tiles of the global preconditioner sweeps)
+template<> struct TileBZ<double> { static constexpr int pre = 8; static constexpr int gs = 4; static constexpr int alt = 4; };
+template<> struct TileBZ<float> { static constexpr int pre = 16; static constexpr int gs = 8; static constexpr int alt = 8; };
 
 template<class T>
 inline int tile_bz(int kind)
@@ -40,10 +41,20 @@ void launch_tile_sweep(int kind, const TileView& G, const RowsView& R,
                        const SweepData<T>& d, TileSync& sync, DevState* st, hipStream_t s,
                        bool local = false);
 
+// one-wavefront global DIC / DILU pass over 8 x 8 x 16 tiles (packed
+// coefficients d.dir required); backward with d.dotR also sums w*dotR into
+// one partial per tile.  Returns the tile count.
+template<class T>
+int32_t launch_sweep_wave(bool backward, const TileView& G, const SweepData<T>& d,
+                          TileSync& sync, DevState* st, hipStream_t s);
+
+// direction-packed coefficients of a box: dst[0..5] = lower K, J, I and
+// upper I, J, K per cell (nDst 6), or the lower three only (nDst 3)
+template<class T>
+void launch_pack_dir_coefs(const TileView& G, const RowsView& R, const T* val, int nDst,
+                           T* const dst[6], hipStream_t s);
 // FP32 tile-local DIC / DILU on direction-packed coefficients (mixed inner
-// preconditioner): dst[0..5] = lower K, J, I and upper I, J, K per cell
-void launch_pack_dir_coefs(const TileView& G, const RowsView& R, const float* val,
-                           float* const dst[6], hipStream_t s);
+// preconditioner)
 // forward: a = r, backward: a = w (in place); cK, cJ, cI the coefficients of
 // the pass (lower forward, upper backward).  Backward with partials: also
 // sum w*r into one partial per workgroup; returns the workgroup count.

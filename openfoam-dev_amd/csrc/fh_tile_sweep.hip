@@ -61,8 +61,9 @@ template<class T, int BX, int BY, int BZ, int KIND>
 struct TileShared
 {
     static constexpr int NC = BX*BY*BZ;
-    T a[NC];                      // x (precond fwd), w (precond bwd), diag (rd), b (gs)
-    T d[NC];                      // rD (precond), 1/.. unused (rd), diag (gs)
+    // precond: a = rD*x (forward) or w (backward), c = rD*coefficient
+    T a[NC];                      // precond (above), diag (rd), b (gs)
+    T d[TileKind<KIND>::gs ? NC : 1];          // gs: diag
     T c[3][NC];                   // dependency-side coefficients, k, j, i
     T e[TileKind<KIND>::gs || KIND == SWEEP_RD_DILU ? 3 : 1]
        [TileKind<KIND>::gs || KIND == SWEEP_RD_DILU ? NC : 1];
@@ -127,56 +128,149 @@ void k_sweep_tile(TileView G, RowsView R, SweepData<T> dat, unsigned int* flags,
     const int64_t base = x0 + sy*y0 + sz*z0;
 
     // ---- stage the tile's inputs (independent of upstream tiles) ----------
-    for (int q = tid; q < BX*BY*BZ; q += BX*BY)
+    // Thread (il, jl) stages its own pencil, cells kl = 0..ez-1.  The loads
+    // are unconditional (a cell outside the tile or an absent neighbour
+    // reads index 0 and its value is replaced by 0), so every load of a
+    // phase is in flight before the first is used.  On a box the entry
+    // slots of a row follow from its position: the lower entries k-1, j-1,
+    // i-1 (those that exist) in face order, then the upper entries i+1,
+    // j+1, k+1.
+    const int il = tid % BX, jl = tid/BX;
+    const bool col = il < ex && jl < ey;
+    const int gi = x0 + il, gj = y0 + jl;
+    const int64_t cBase = base + il + sy*jl;
+    constexpr bool PRE = KIND == SWEEP_PRECOND_FWD || KIND == SWEEP_PRECOND_BWD;
+    const bool useDir = PRE && dat.dir[0] != nullptr;
+    const bool wantDot = KIND == SWEEP_PRECOND_BWD && dat.dotR != nullptr;
+    T rr[KIND == SWEEP_PRECOND_BWD ? BZ : 1];   // dotR of the pencil
     {
-        const int il = q % BX, jl = (q/BX) % BY, kl = q/(BX*BY);
-        if (il >= ex || jl >= ey || kl >= ez) continue;
-        const int gi = x0 + il, gj = y0 + jl, gk = z0 + kl;
-        const int64_t c = base + il + sy*jl + sz*kl;
-        const int32_t info = R.rowInfo[c];
-        const int32_t nLo = info & 0xffff;
-        // lower entries in face order: k-1, j-1, i-1 (those that exist)
-        int eK = -1, eJ = -1, eI = -1, n = 0;
-        if (gk > 0) eK = n++;
-        if (gj > 0) eJ = n++;
-        if (gi > 0) eI = n++;
-        // upper entries after the lower ones: i+1, j+1, k+1
-        int uI = -1, uJ = -1, uK = -1;
-        n = nLo;
-        if (gi < G.nx - 1) uI = n++;
-        if (gj < G.ny - 1) uJ = n++;
-        if (gk < G.nz - 1) uK = n++;
-        const int dK = K::bwd ? uK : eK, dJ = K::bwd ? uJ : eJ, dI = K::bwd ? uI : eI;
-        S.c[0][q] = dK >= 0 ? dat.val[sell_at(R.sliceStart, c, dK)] : T(0);
-        S.c[1][q] = dJ >= 0 ? dat.val[sell_at(R.sliceStart, c, dJ)] : T(0);
-        S.c[2][q] = dI >= 0 ? dat.val[sell_at(R.sliceStart, c, dI)] : T(0);
-        if constexpr (K::gs)
+        int32_t sb[BZ] = {};
+        if (!useDir)
         {
-            // other side: upper (forward) in i, j, k order / lower (backward)
-            // in k, j, i order
-            const int oA = K::bwd ? eK : uI, oB = K::bwd ? eJ : uJ, oC = K::bwd ? eI : uK;
-            S.e[0][q] = oA >= 0 ? dat.val[sell_at(R.sliceStart, c, oA)] : T(0);
-            S.e[1][q] = oB >= 0 ? dat.val[sell_at(R.sliceStart, c, oB)] : T(0);
-            S.e[2][q] = oC >= 0 ? dat.val[sell_at(R.sliceStart, c, oC)] : T(0);
-            S.o[q] = dat.w[c];
-            S.a[q] = dat.x[c];
-            S.d[q] = dat.diag[c];
-        }
-        else if constexpr (K::rd)
-        {
-            S.a[q] = dat.diag[c];
-            if constexpr (KIND == SWEEP_RD_DILU)
+#pragma unroll
+            for (int kk = 0; kk < BZ; kk++)
             {
-                // partner: upper[f] of the same faces (DILUPreconditioner.C:67-72)
-                S.e[0][q] = eK >= 0 ? (T)dat.upperF[R.face[sell_at(R.sliceStart, c, eK)] >> 1] : T(0);
-                S.e[1][q] = eJ >= 0 ? (T)dat.upperF[R.face[sell_at(R.sliceStart, c, eJ)] >> 1] : T(0);
-                S.e[2][q] = eI >= 0 ? (T)dat.upperF[R.face[sell_at(R.sliceStart, c, eI)] >> 1] : T(0);
+                const int64_t c = (col && kk < ez) ? cBase + sz*kk : 0;
+                sb[kk] = R.sliceStart[c >> 6] + (int32_t)(c & 63);
             }
         }
-        else
+        T ra[BZ], rd[BZ], rc[3][BZ];
+        T re[K::gs || KIND == SWEEP_RD_DILU ? 3 : 1][BZ];
+        T ro[K::gs ? BZ : 1];
+        int32_t fc[KIND == SWEEP_RD_DILU ? 3 : 1][BZ];
+#pragma unroll
+        for (int kk = 0; kk < BZ; kk++)
         {
-            S.a[q] = K::bwd ? dat.w[c] : dat.x[c];
-            S.d[q] = dat.rD[c];
+            const bool ok = col && kk < ez;
+            const int gk = z0 + kk;
+            const int64_t c = ok ? cBase + sz*kk : 0;
+            const int nLo = (gk > 0) + (gj > 0) + (gi > 0);
+            const int eK = gk > 0 ? 0 : -1;
+            const int eJ = gj > 0 ? (gk > 0) : -1;
+            const int eI = gi > 0 ? (gk > 0) + (gj > 0) : -1;
+            const int uI = gi < G.nx - 1 ? nLo : -1;
+            const int uJ = gj < G.ny - 1 ? nLo + (gi < G.nx - 1) : -1;
+            const int uK = gk < G.nz - 1 ? nLo + (gi < G.nx - 1) + (gj < G.ny - 1) : -1;
+            const int dK = K::bwd ? uK : eK, dJ = K::bwd ? uJ : eJ, dI = K::bwd ? uI : eI;
+            // SELL value of entry slot e of this row (0 when absent)
+            auto sell = [&](int e) -> T
+            {
+                const bool h = ok && e >= 0;
+                const T v = dat.val[h ? sb[kk] + (e << 6) : 0];
+                return h ? v : T(0);
+            };
+            {
+                // packed by direction (at the cell, or at the neighbour for
+                // dirShift) or the row's SELL entries: one pointer and index
+                // per coefficient, no branch
+                const bool sh = dat.dirShift != 0;
+                const int dd[3] = {dK, dJ, dI};
+                const int64_t off[3] = {sh ? sz : 0, sh ? sy : 0, sh ? 1 : 0};
+#pragma unroll
+                for (int e = 0; e < 3; e++)
+                {
+                    const bool h = ok && dd[e] >= 0;
+                    const T* src = useDir ? dat.dir[e] : dat.val;
+                    const int64_t ix = useDir ? c + off[e] : (int64_t)sb[kk] + (dd[e] << 6);
+                    const T v = src[h ? ix : 0];
+                    rc[e][kk] = h ? v : T(0);
+                }
+            }
+            if constexpr (K::gs)
+            {
+                // other side: upper (forward) in i, j, k order / lower
+                // (backward) in k, j, i order
+                re[0][kk] = sell(K::bwd ? eK : uI);
+                re[1][kk] = sell(K::bwd ? eJ : uJ);
+                re[2][kk] = sell(K::bwd ? eI : uK);
+                ro[kk] = dat.w[c];
+                ra[kk] = dat.x[c];
+                rd[kk] = dat.diag[c];
+            }
+            else if constexpr (K::rd)
+            {
+                ra[kk] = dat.diag[c];
+                if constexpr (KIND == SWEEP_RD_DILU)
+                {
+                    const int es[3] = {eK, eJ, eI};
+#pragma unroll
+                    for (int e = 0; e < 3; e++)
+                    {
+                        const bool h = ok && es[e] >= 0;
+                        const int32_t f = R.face[h ? sb[kk] + (es[e] << 6) : 0] >> 1;
+                        fc[e][kk] = h ? f : -1;
+                    }
+                }
+            }
+            else
+            {
+                ra[kk] = K::bwd ? dat.w[c] : dat.x[c];
+                rd[kk] = dat.rD[c];
+                if constexpr (KIND == SWEEP_PRECOND_BWD)
+                    rr[kk] = wantDot ? dat.dotR[c] : T(0);
+            }
+        }
+        if constexpr (KIND == SWEEP_RD_DILU)
+        {
+            // partner: upper[f] of the same faces (DILUPreconditioner.C:67-72)
+#pragma unroll
+            for (int kk = 0; kk < BZ; kk++)
+#pragma unroll
+                for (int e = 0; e < 3; e++)
+                {
+                    const T v = (T)dat.upperF[fc[e][kk] >= 0 ? fc[e][kk] : 0];
+                    re[e][kk] = fc[e][kk] >= 0 ? v : T(0);
+                }
+        }
+#pragma unroll
+        for (int kk = 0; kk < BZ; kk++)
+        {
+            const int q = tid + BX*BY*kk;
+            if constexpr (PRE)
+            {
+                // the products rD*x and rD*coefficient of the reference
+                // expressions wA = rD*rA and wA -= rD*coefficient*wA(nb),
+                // formed once here (the same values)
+                S.a[q] = K::bwd ? ra[kk] : rd[kk]*ra[kk];
+                S.c[0][q] = rd[kk]*rc[0][kk];
+                S.c[1][q] = rd[kk]*rc[1][kk];
+                S.c[2][q] = rd[kk]*rc[2][kk];
+            }
+            else
+            {
+                S.a[q] = ra[kk];
+                if constexpr (K::gs) S.d[q] = rd[kk];
+                S.c[0][q] = rc[0][kk];
+                S.c[1][q] = rc[1][kk];
+                S.c[2][q] = rc[2][kk];
+            }
+            if constexpr (K::gs || KIND == SWEEP_RD_DILU)
+            {
+                S.e[0][q] = re[0][kk];
+                S.e[1][q] = re[1][kk];
+                S.e[2][q] = re[2][kk];
+            }
+            if constexpr (K::gs) S.o[q] = ro[kk];
         }
     }
     if constexpr (K::gs)
@@ -259,104 +353,174 @@ void k_sweep_tile(TileView G, RowsView R, SweepData<T> dat, unsigned int* flags,
     __syncthreads();
 
     // ---- hyperplane walk ------------------------------------------------------
-    const int il = tid % BX, jl = tid/BX;
+    // Per step one batch of LDS reads (the neighbours of the previous step
+    // and the halos; clamped indices, values selected afterwards), then the
+    // row update in the reference order.  The preconditioner kinds read the
+    // next cell's staged inputs one step ahead.
     const int nSteps = ex + ey + ez - 2;
     T prev = T(0);           // result of this pencil's previous cell
-    const bool col = il < ex && jl < ey;
-    const int gi = x0 + il, gj = y0 + jl;
-    for (int s = 0; s < nSteps; s++)
+    // neighbour slots in the step buffer (own slot where there is none)
+    const int tJ = K::bwd ? (jl < BY - 1 ? tid + BX : tid) : (jl > 0 ? tid - BX : tid);
+    const int tI = K::bwd ? (il < BX - 1 ? tid + 1 : tid) : (il > 0 ? tid - 1 : tid);
+    const bool inJ = K::bwd ? jl < ey - 1 : jl > 0;
+    const bool inI = K::bwd ? il < ex - 1 : il > 0;
+    const bool hasJ = LOCAL ? inJ : (K::bwd ? gj < G.ny - 1 : gj > 0);
+    const bool hasI = LOCAL ? inI : (K::bwd ? gi < G.nx - 1 : gi > 0);
+    // the pencil's cell at step s (clamped into the tile)
+    auto cellAt = [&](int s) -> int
     {
-        // forward: s = il + jl + kl; backward walks from the far corner
-        const int kl = K::bwd ? (ex - 1 - il) + (ey - 1 - jl) + (ez - 1) - s
-                              : s - il - jl;
-        const bool act = col && kl >= 0 && kl < ez;
-        const T* prevStep = S.step[(s + 1) & 1];
-        T out = T(0);
-        if (act)
+        const int kl = K::bwd ? (ex - 1 - il) + (ey - 1 - jl) + (ez - 1) - s : s - il - jl;
+        return kl;
+    };
+    if constexpr (PRE)
+    {
+        int kc0 = cellAt(0);
+        kc0 = kc0 < 0 ? 0 : (kc0 >= ez ? ez - 1 : kc0);
+        int qn = il + BX*(jl + BY*kc0);
+        T nA = S.a[qn], nC0 = S.c[0][qn], nC1 = S.c[1][qn], nC2 = S.c[2][qn];
+        for (int s = 0; s < nSteps; s++)
         {
-            const int q = il + BX*(jl + BY*kl);
-            const int gk = z0 + kl;
+            const int kl = cellAt(s);
+            const bool act = col && kl >= 0 && kl < ez;
+            const int kc = kl < 0 ? 0 : (kl >= ez ? ez - 1 : kl);
+            const int q = il + BX*(jl + BY*kc);
+            const int gk = z0 + kc;
+            const bool inK = K::bwd ? kc < ez - 1 : kc > 0;
+            const bool hasK = LOCAL ? inK : (K::bwd ? gk < G.nz - 1 : gk > 0);
+            const T* prevStep = S.step[(s + 1) & 1];
+            const T A = nA, C0 = nC0, C1 = nC1, C2 = nC2;
+            const T sJ = prevStep[tJ], sI = prevStep[tI];
+            T hK = T(0), hJ = T(0), hI = T(0);
+            if (!LOCAL)
+            {
+                hK = S.haloK[il + BX*jl];
+                hJ = S.haloJ[il + BX*kc];
+                hI = S.haloI[jl + BY*kc];
+            }
+            {
+                // the next step's staged inputs
+                int kn = cellAt(s + 1);
+                kn = kn < 0 ? 0 : (kn >= ez ? ez - 1 : kn);
+                qn = il + BX*(jl + BY*kn);
+                nA = S.a[qn]; nC0 = S.c[0][qn]; nC1 = S.c[1][qn]; nC2 = S.c[2][qn];
+            }
             // dependency values in reference order k, j, i (forward: lower
             // neighbours ascending; backward: upper neighbours descending)
-            T vK = T(0), vJ = T(0), vI = T(0);
-            const bool inK = K::bwd ? kl < ez - 1 : kl > 0;
-            const bool inJ = K::bwd ? jl < ey - 1 : jl > 0;
-            const bool inI = K::bwd ? il < ex - 1 : il > 0;
-            const bool hasK = LOCAL ? inK : (K::bwd ? gk < G.nz - 1 : gk > 0);
-            const bool hasJ = LOCAL ? inJ : (K::bwd ? gj < G.ny - 1 : gj > 0);
-            const bool hasI = LOCAL ? inI : (K::bwd ? gi < G.nx - 1 : gi > 0);
-            if (hasK) vK = inK ? prev : S.haloK[il + BX*jl];
-            if (hasJ) vJ = inJ ? prevStep[tid + (K::bwd ? BX : -BX)] : S.haloJ[il + BX*kl];
-            if (hasI) vI = inI ? prevStep[tid + (K::bwd ? 1 : -1)] : S.haloI[jl + BY*kl];
-
-            if constexpr (KIND == SWEEP_PRECOND_FWD || KIND == SWEEP_PRECOND_BWD)
+            const T vK = inK ? prev : hK;
+            const T vJ = inJ ? sJ : hJ;
+            const T vI = inI ? sI : hI;
+            T w = A;
+            w = hasK ? w - C0*vK : w;
+            w = hasJ ? w - C1*vJ : w;
+            w = hasI ? w - C2*vI : w;
+            const T out = act ? w : T(0);
+            if (act)
             {
-                const T rDr = S.d[q];
-                T w = K::bwd ? S.a[q] : rDr*S.a[q];
-                if (hasK) w -= rDr*S.c[0][q]*vK;
-                if (hasJ) w -= rDr*S.c[1][q]*vJ;
-                if (hasI) w -= rDr*S.c[2][q]*vI;
-                out = w;
+                prev = out;
+                st_coh(&dat.w[base + il + sy*jl + sz*kl], out);
+                // the final value, for the fused w.r sum (own cell only)
+                if (KIND == SWEEP_PRECOND_BWD && wantDot) S.a[q] = out;
             }
-            else if constexpr (KIND == SWEEP_RD_DIC)
+            S.step[s & 1][tid] = out;
+            __syncthreads();
+        }
+    }
+    else
+    {
+    for (int s = 0; s < nSteps; s++)
+    {
+        const int kl = cellAt(s);
+        const bool act = col && kl >= 0 && kl < ez;
+        const int kc = kl < 0 ? 0 : (kl >= ez ? ez - 1 : kl);
+        const int q = il + BX*(jl + BY*kc);
+        const int gk = z0 + kc;
+        const T* prevStep = S.step[(s + 1) & 1];
+        const bool inK = K::bwd ? kc < ez - 1 : kc > 0;
+        const bool hasK = LOCAL ? inK : (K::bwd ? gk < G.nz - 1 : gk > 0);
+        // dependency values in reference order k, j, i (forward: lower
+        // neighbours ascending; backward: upper neighbours descending)
+        const T A = S.a[q];
+        const T C0 = S.c[0][q], C1 = S.c[1][q], C2 = S.c[2][q];
+        const T sJ = prevStep[tJ], sI = prevStep[tI];
+        T hK = T(0), hJ = T(0), hI = T(0);
+        if (!LOCAL)
+        {
+            hK = S.haloK[il + BX*jl];
+            hJ = S.haloJ[il + BX*kc];
+            hI = S.haloI[jl + BY*kc];
+        }
+        const T vK = inK ? prev : hK;
+        const T vJ = inJ ? sJ : hJ;
+        const T vI = inI ? sI : hI;
+        T out;
+        if constexpr (KIND == SWEEP_RD_DIC)
+        {
+            T v = A;
+            v = hasK ? v - C0*C0/vK : v;
+            v = hasJ ? v - C1*C1/vJ : v;
+            v = hasI ? v - C2*C2/vI : v;
+            out = v;
+        }
+        else if constexpr (KIND == SWEEP_RD_DILU)
+        {
+            const T E0 = S.e[0][q], E1 = S.e[1][q], E2 = S.e[2][q];
+            T v = A;
+            v = hasK ? v - E0*C0/vK : v;
+            v = hasJ ? v - E1*C1/vJ : v;
+            v = hasI ? v - E2*C2/vI : v;
+            out = v;
+        }
+        else // Gauss-Seidel
+        {
+            // old (forward: upper) / forward-pass (backward: lower) values
+            const bool oHasI = K::bwd ? gi > 0 : gi < G.nx - 1;
+            const bool oHasJ = K::bwd ? gj > 0 : gj < G.ny - 1;
+            const bool oHasK = K::bwd ? gk > 0 : gk < G.nz - 1;
+            const bool oInI = K::bwd ? il > 0 : il < ex - 1;
+            const bool oInJ = K::bwd ? jl > 0 : jl < ey - 1;
+            const bool oInK = K::bwd ? kc > 0 : kc < ez - 1;
+            const T E0 = S.e[0][q], E1 = S.e[1][q], E2 = S.e[2][q];
+            const T sOI = S.o[oInI ? q + (K::bwd ? -1 : 1) : q];
+            const T sOJ = S.o[oInJ ? q + (K::bwd ? -BX : BX) : q];
+            const T sOK = S.o[oInK ? q + (K::bwd ? -BX*BY : BX*BY) : q];
+            const T hOI = S.oI[jl + BY*kc], hOJ = S.oJ[il + BX*kc], hOK = S.oK[il + BX*jl];
+            const T oVI = oInI ? sOI : hOI;
+            const T oVJ = oInJ ? sOJ : hOJ;
+            const T oVK = oInK ? sOK : hOK;
+            T t = A;
+            if (!K::bwd)
             {
-                T v = S.a[q];
-                if (hasK) { const T u = S.c[0][q]; v -= u*u/vK; }
-                if (hasJ) { const T u = S.c[1][q]; v -= u*u/vJ; }
-                if (hasI) { const T u = S.c[2][q]; v -= u*u/vI; }
-                out = v;
+                // bPrime: lower products k, j, i (new values); then the
+                // upper products i, j, k (old values)
+                t = hasK ? t - C0*vK : t;
+                t = hasJ ? t - C1*vJ : t;
+                t = hasI ? t - C2*vI : t;
+                t = oHasI ? t - E0*oVI : t;
+                t = oHasJ ? t - E1*oVJ : t;
+                t = oHasK ? t - E2*oVK : t;
             }
-            else if constexpr (KIND == SWEEP_RD_DILU)
+            else
             {
-                T v = S.a[q];
-                if (hasK) v -= S.e[0][q]*S.c[0][q]/vK;
-                if (hasJ) v -= S.e[1][q]*S.c[1][q]/vJ;
-                if (hasI) v -= S.e[2][q]*S.c[2][q]/vI;
-                out = v;
+                // bPrime from the forward pass: lower products k, j, i
+                // (forward values); then upper products i, j, k (new)
+                t = oHasK ? t - E0*oVK : t;
+                t = oHasJ ? t - E1*oVJ : t;
+                t = oHasI ? t - E2*oVI : t;
+                t = hasI ? t - C2*vI : t;
+                t = hasJ ? t - C1*vJ : t;
+                t = hasK ? t - C0*vK : t;
             }
-            else // Gauss-Seidel
-            {
-                // old (forward: upper) / forward-pass (backward: lower) values
-                const bool oHasI = K::bwd ? gi > 0 : gi < G.nx - 1;
-                const bool oHasJ = K::bwd ? gj > 0 : gj < G.ny - 1;
-                const bool oHasK = K::bwd ? gk > 0 : gk < G.nz - 1;
-                const bool oInI = K::bwd ? il > 0 : il < ex - 1;
-                const bool oInJ = K::bwd ? jl > 0 : jl < ey - 1;
-                const bool oInK = K::bwd ? kl > 0 : kl < ez - 1;
-                T oVI = T(0), oVJ = T(0), oVK = T(0);
-                if (oHasI) oVI = oInI ? S.o[q + (K::bwd ? -1 : 1)] : S.oI[jl + BY*kl];
-                if (oHasJ) oVJ = oInJ ? S.o[q + (K::bwd ? -BX : BX)] : S.oJ[il + BX*kl];
-                if (oHasK) oVK = oInK ? S.o[q + (K::bwd ? -BX*BY : BX*BY)] : S.oK[il + BX*jl];
-                T t = S.a[q];
-                if (!K::bwd)
-                {
-                    // bPrime: lower products k, j, i (new values); then the
-                    // upper products i, j, k (old values)
-                    if (hasK) t -= S.c[0][q]*vK;
-                    if (hasJ) t -= S.c[1][q]*vJ;
-                    if (hasI) t -= S.c[2][q]*vI;
-                    if (oHasI) t -= S.e[0][q]*oVI;
-                    if (oHasJ) t -= S.e[1][q]*oVJ;
-                    if (oHasK) t -= S.e[2][q]*oVK;
-                }
-                else
-                {
-                    // bPrime from the forward pass: lower products k, j, i
-                    // (forward values); then upper products i, j, k (new)
-                    if (oHasK) t -= S.e[0][q]*oVK;
-                    if (oHasJ) t -= S.e[1][q]*oVJ;
-                    if (oHasI) t -= S.e[2][q]*oVI;
-                    if (hasI) t -= S.c[2][q]*vI;
-                    if (hasJ) t -= S.c[1][q]*vJ;
-                    if (hasK) t -= S.c[0][q]*vK;
-                }
-                out = t/S.d[q];
-            }
+            out = t/S.d[q];
+        }
+        if (!act) out = T(0);
+        if (act)
+        {
             prev = out;
             st_coh(&dat.w[base + il + sy*jl + sz*kl], out);
         }
         S.step[s & 1][tid] = out;
         __syncthreads();
+    }
     }
 
     // ---- publish completion -------------------------------------------------
@@ -365,6 +529,31 @@ void k_sweep_tile(TileView G, RowsView R, SweepData<T> dat, unsigned int* flags,
         wait_stores();
         __syncthreads();
         if (tid == 0) st_coh(&flags[tile], doneVal);
+    }
+
+    // ---- fused sum w*r of the tile (after publishing: off the chain) --------
+    if constexpr (KIND == SWEEP_PRECOND_BWD)
+    {
+        if (wantDot)
+        {
+            __shared__ double red[BX*BY/64];
+            double acc = 0.0;
+#pragma unroll
+            for (int kk = 0; kk < BZ; kk++)
+                if (col && kk < ez) acc += (double)S.a[tid + BX*BY*kk]*(double)rr[kk];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+            if ((tid & 63) == 0) red[tid >> 6] = acc;
+            __syncthreads();
+            if (tid == 0)
+            {
+                double t = 0.0;
+#pragma unroll
+                for (int w = 0; w < BX*BY/64; w++) t += red[w];
+                dat.partials[tile] = t;
+                dat.partials[nTiles + tile] = 0.0;
+            }
+        }
     }
 }
 
@@ -397,15 +586,27 @@ void launch_tile_sweep(int kind, const TileView& G, const RowsView& R,
                        const SweepData<T>& d, TileSync& sync, DevState* st, hipStream_t s,
                        bool local)
 {
-    if (G.bx != 16 || G.by != 16 || G.bz != tile_bz<T>(kind))
-        fail(FH_ERR_INVALID, "tile shape does not match the compiled sweep kernels");
     // on a box the faces of an owner are in ascending neighbour order, so
     // the reversed-losort walk of preconditionT is the reversed face walk
     if (kind == SWEEP_PRECOND_BWD_T) kind = SWEEP_PRECOND_BWD;
+    const bool pre = kind == SWEEP_PRECOND_FWD || kind == SWEEP_PRECOND_BWD;
+    // the global preconditioner sweeps also come in the shallower tiles
+    // (more tiles resident per CU); tile-local factors fix the shape
+    const bool alt = pre && !local && G.bz == TileBZ<T>::alt;
+    if (G.bx != 16 || G.by != 16 || !(G.bz == tile_bz<T>(kind) || alt))
+        fail(FH_ERR_INVALID, "tile shape does not match the compiled sweep kernels");
+    if (d.dotR && (kind != SWEEP_PRECOND_BWD || !d.partials))
+        fail(FH_ERR_INVALID, "internal: fused sum on a sweep that does not provide it");
     switch (kind)
     {
-        case SWEEP_PRECOND_FWD: launch_tile_kind<T, SWEEP_PRECOND_FWD, 16, 16, TileBZ<T>::pre>(G, R, d, sync, st, s, local); break;
-        case SWEEP_PRECOND_BWD: launch_tile_kind<T, SWEEP_PRECOND_BWD, 16, 16, TileBZ<T>::pre>(G, R, d, sync, st, s, local); break;
+        case SWEEP_PRECOND_FWD:
+            if (alt) launch_tile_kind<T, SWEEP_PRECOND_FWD, 16, 16, TileBZ<T>::alt>(G, R, d, sync, st, s, false);
+            else launch_tile_kind<T, SWEEP_PRECOND_FWD, 16, 16, TileBZ<T>::pre>(G, R, d, sync, st, s, local);
+            break;
+        case SWEEP_PRECOND_BWD:
+            if (alt) launch_tile_kind<T, SWEEP_PRECOND_BWD, 16, 16, TileBZ<T>::alt>(G, R, d, sync, st, s, false);
+            else launch_tile_kind<T, SWEEP_PRECOND_BWD, 16, 16, TileBZ<T>::pre>(G, R, d, sync, st, s, local);
+            break;
         case SWEEP_RD_DIC:      launch_tile_kind<T, SWEEP_RD_DIC, 16, 16, TileBZ<T>::pre>(G, R, d, sync, st, s, local); break;
         case SWEEP_RD_DILU:     launch_tile_kind<T, SWEEP_RD_DILU, 16, 16, TileBZ<T>::pre>(G, R, d, sync, st, s, local); break;
         case SWEEP_GS_FWD:      launch_tile_kind<T, SWEEP_GS_FWD, 16, 16, TileBZ<T>::gs>(G, R, d, sync, st, s, local); break;
