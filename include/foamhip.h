@@ -233,11 +233,21 @@ enum
     FH_KERNEL_SPMV = 0,          /* y = A x                         */
     FH_KERNEL_SWEEP_FORWARD = 1, /* DIC/DILU forward substitution    */
     FH_KERNEL_SWEEP_BACKWARD = 2,/* DIC/DILU backward substitution   */
-    FH_KERNEL_AXPY_NORM = 3      /* psi += a p, r -= a w, sum |r|     */
+    FH_KERNEL_AXPY_NORM = 3,     /* psi += a p, r -= a w, sum |r|     */
+    FH_KERNEL_PRECONDITION = 4   /* the whole preconditioner apply of the
+                                    PCG loop, with its fused w.r sum    */
 };
 int fh_profile_kernel(fh_matrix* matrix, int32_t kernel, int32_t precision,
                       int32_t preconditioner, int32_t reps, double* msPerLaunch,
                       double* bytesPerLaunch);
+/* Live timing inside the PCG loops (no reference counterpart): with bit
+   (1 << FH_KERNEL_*) set in kernelMask, every launch of that kernel in the
+   following solves is bracketed by HIP events on the matrix's stream and
+   the launches of the iterations that ran are summed (PRECONDITION, SPMV,
+   AXPY_NORM of the box PCG loops).  A call resets the sums; mask 0 stops. */
+int fh_kernel_timing(fh_matrix* matrix, uint32_t kernelMask);
+int fh_kernel_timing_get(fh_matrix* matrix, int32_t kernel, double* msTotal,
+                         int64_t* launches);
 
 /* ---- finite-volume assembly on the device ------------------------------
    Boundary faces of all non-coupled patches, flattened in patch order

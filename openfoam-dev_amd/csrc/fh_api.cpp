@@ -481,4 +481,31 @@ int fh_profile_kernel(fh_matrix* M, int32_t kernel, int32_t precision, int32_t p
     });
 }
 
+int fh_kernel_timing(fh_matrix* M, uint32_t kernelMask)
+{
+    return guarded([&] {
+        check_matrix(M);
+        if (kernelMask >> 5) fail(FH_ERR_INVALID, "kernel mask 0x%x has unknown bits", kernelMask);
+        auto& T = M->kt;
+        T.mask = kernelMask;
+        for (int k = 0; k < 5; k++)
+        {
+            T.used[k] = 0;
+            T.ms[k] = 0.0;
+            T.launches[k] = 0;
+        }
+    });
+}
+
+int fh_kernel_timing_get(fh_matrix* M, int32_t kernel, double* msTotal, int64_t* launches)
+{
+    return guarded([&] {
+        check_matrix(M);
+        if (!msTotal || !launches) fail(FH_ERR_INVALID, "NULL output");
+        if (kernel < 0 || kernel > 4) fail(FH_ERR_INVALID, "unknown kernel id %d", kernel);
+        *msTotal = M->kt.ms[kernel];
+        *launches = M->kt.launches[kernel];
+    });
+}
+
 } // extern "C"

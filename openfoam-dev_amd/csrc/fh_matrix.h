@@ -7,6 +7,9 @@
 #include "fh_solver_kernels.h"
 #include "fh_tile_sweep.h"
 
+#include <algorithm>
+#include <vector>
+
 struct fh_matrix
 {
     fh_mesh* mesh = nullptr;
@@ -62,6 +65,18 @@ struct fh_matrix
     fh::DBuf<unsigned int> d_tileFlags;
     fh::DBuf<unsigned long long> d_tileTicket;
     unsigned long long tileTicketBase = 0, tileEpoch = 0;
+
+    // live timing of chosen kernels inside the solver loops: a pair of HIP
+    // events around every launch of a kernel whose FH_KERNEL_* bit is set in
+    // mask; the pairs of the iterations that ran are summed after each batch
+    struct KernelTiming
+    {
+        uint32_t mask = 0;
+        std::vector<hipEvent_t> ev[5];
+        int used[5] = {0, 0, 0, 0, 0};
+        double ms[5] = {0, 0, 0, 0, 0};
+        int64_t launches[5] = {0, 0, 0, 0, 0};
+    } kt;
 
     // device scalar state and reduction partials
     fh::DBuf<fh::DevState> d_state;
@@ -211,6 +226,39 @@ void check_sweep_errors(fh_matrix* M);
 
 // batched enqueue of device iterations with a poll of DevState.done: the
 // batch doubles up to 32 iterations between host polls
+// event around a launch of kernel id k (FH_KERNEL_*) when it is timed
+inline void kt_mark(fh_matrix* M, int k)
+{
+    auto& T = M->kt;
+    if (!(T.mask & (1u << k))) return;
+    if (T.used[k] == (int)T.ev[k].size())
+    {
+        hipEvent_t e;
+        FH_HIP(hipEventCreate(&e));
+        T.ev[k].push_back(e);
+    }
+    FH_HIP(hipEventRecord(T.ev[k][T.used[k]++], M->stream));
+}
+
+// after a synchronised batch: add the pairs of the iterations that ran
+// (launches after convergence exit at once and are not counted)
+inline void kt_harvest(fh_matrix* M, int32_t itBefore, int32_t itAfter)
+{
+    auto& T = M->kt;
+    for (int k = 0; k < 5; k++)
+    {
+        const int pairs = std::min(T.used[k]/2, std::max(0, itAfter - itBefore));
+        for (int i = 0; i < pairs; i++)
+        {
+            float ms = 0.f;
+            FH_HIP(hipEventElapsedTime(&ms, T.ev[k][2*i], T.ev[k][2*i + 1]));
+            T.ms[k] += ms;
+            T.launches[k]++;
+        }
+        T.used[k] = 0;
+    }
+}
+
 template<class F>
 void run_batches(fh_matrix* M, int32_t maxIter, F&& enqueue)
 {
@@ -218,10 +266,12 @@ void run_batches(fh_matrix* M, int32_t maxIter, F&& enqueue)
     const int64_t cap = 4*(int64_t)(maxIter > 1 ? maxIter : 1) + 64;
     for (;;)
     {
+        const int32_t itBefore = M->h_state->nIterations;
         for (int32_t i = 0; i < batch; i++) enqueue(seq++);
         FH_HIP(hipMemcpyAsync(M->h_state, M->d_state.p, sizeof(DevState),
                               hipMemcpyDeviceToHost, M->stream));
         FH_HIP(hipStreamSynchronize(M->stream));
+        if (M->kt.mask) kt_harvest(M, itBefore, M->h_state->nIterations);
         if (M->h_state->done) break;
         if (seq > cap) fail(FH_ERR_SCHEDULE, "solver did not terminate after %d steps", seq);
         batch = batch*2 < 32 ? batch*2 : 32;

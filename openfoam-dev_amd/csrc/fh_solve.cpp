@@ -90,6 +90,8 @@ void matrix_free(fh_matrix* M)
     M->h_state = nullptr;
     if (M->evStart) (void)hipEventDestroy(M->evStart);
     if (M->evStop) (void)hipEventDestroy(M->evStop);
+    for (auto& v : M->kt.ev)
+        for (hipEvent_t e : v) (void)hipEventDestroy(e);
     if (M->stream && M->ownsStream) (void)hipStreamDestroy(M->stream);
     M->stream = nullptr;
 }
@@ -487,9 +489,17 @@ int32_t apply_precond(fh_matrix* M, int precon, const T* r, T* w, double* partia
     {
         if (local)
         {
-            // bandwidth path of the FP32 tile-local factor
+            // bandwidth path of the FP32 tile-local factor: both passes in
+            // one launch where the rows allow it
             ensure_dir32(M);
             const TileView G = M->tile_view(TileBZ<float>::pre);
+            {
+                const float* dir[6];
+                for (int q = 0; q < 6; q++) dir[q] = M->d_dir32[q].p;
+                const int32_t nb = launch_tile_local_apply_f32(G, r, d.rD, dir, M->symmetric, w,
+                                                               st, M->stream, partials);
+                if (nb >= 0) return nb;
+            }
             launch_tile_local_f32(false, G, r, d.rD, M->d_dir32[0].p, M->d_dir32[1].p,
                                   M->d_dir32[2].p, w, st, M->stream);
             // the backward pass also sums w*r (one partial per workgroup)
@@ -581,15 +591,21 @@ void pcg_loop(fh_matrix* M, int precon, T* psi, T* r, T* p, T* w, int32_t maxIte
             double* ap = M->v[8].p;
             run_batches(M, maxIter, [&](int32_t seq)
             {
+                kt_mark(M, FH_KERNEL_PRECONDITION);
                 const int32_t nbw = apply_precond<T>(M, precon, r, w, part, st, false);
+                kt_mark(M, FH_KERNEL_PRECONDITION);
                 launch_finalize_blocks(nbw, N, part, FIN_PCG_WARA, st, 0, M->stream);
                 double* pNew = pBuf[(seq + 1) & 1];
+                kt_mark(M, FH_KERNEL_SPMV);
                 const int32_t nb = launch_box_pcg_spmv<double>(mesh->boxNx, mesh->boxNy,
                                                                mesh->boxNz, diag, dir,
                                                                M->symmetric, w, pBuf[seq & 1],
                                                                pNew, ap, part, st, M->stream);
+                kt_mark(M, FH_KERNEL_SPMV);
                 launch_finalize_blocks(nb, N, part, FIN_PCG_WAPA, st, 0, M->stream);
+                kt_mark(M, FH_KERNEL_AXPY_NORM);
                 launch_pcg_update<T>(N, psi, r, pNew, ap, part, st, M->stream);
+                kt_mark(M, FH_KERNEL_AXPY_NORM);
                 launch_finalize(N, part, FIN_PCG_RES, st, 0, M->stream);
             });
             return;
@@ -858,15 +874,19 @@ static void mixed_pcg(fh_matrix* M, const fh_controls* c, double* psi, const dou
     float* ap = fuseP ? M->f[4].p : w32;
     run_batches(M, c->maxIter, [&](int32_t seq)
     {
+        kt_mark(M, FH_KERNEL_PRECONDITION);
         const int32_t nbw = apply_precond<float>(M, precon, r32, w32, part, st, false);
+        kt_mark(M, FH_KERNEL_PRECONDITION);
         launch_finalize_blocks(nbw, N, part, FIN_PCG_WARA, st, 0, M->stream);
         if (fuseP)
         {
             p32 = pBuf[(seq + 1) & 1];
+            kt_mark(M, FH_KERNEL_SPMV);
             const int32_t nb = launch_box_pcg_spmv<float>(mesh->boxNx, mesh->boxNy, mesh->boxNz,
                                                        diag, dir, M->symmetric, w32,
                                                        pBuf[seq & 1], p32, ap, part, st,
                                                        M->stream);
+            kt_mark(M, FH_KERNEL_SPMV);
             launch_finalize_blocks(nb, N, part, FIN_PCG_WAPA, st, 0, M->stream);
         }
         else if (box)
@@ -884,11 +904,13 @@ static void mixed_pcg(fh_matrix* M, const fh_controls* c, double* psi, const dou
                                    M->stream);
             launch_finalize(N, part, FIN_PCG_WAPA, st, 0, M->stream);
         }
+        kt_mark(M, FH_KERNEL_AXPY_NORM);
         const int32_t nbu = launch_mixed_update(N, psi, r32, p32, ap, part, st, M->stream);
+        kt_mark(M, FH_KERNEL_AXPY_NORM);
         launch_finalize_blocks(nbu, N, part, FIN_MIXED_RES, st, 0, M->stream);
-        launch_mixed_replace(M->rows(), M->d_diag.p, M->d_val.p, psi, src, r32, part, st,
-                             M->stream);
-        launch_finalize(N, part, FIN_MIXED_TRUE, st, 0, M->stream);
+        const int32_t nbr = launch_mixed_replace(M->rows(), M->d_diag.p, M->d_val.p, psi, src,
+                                                 r32, part, st, M->stream);
+        launch_finalize_blocks(nbr, N, part, FIN_MIXED_TRUE, st, 0, M->stream);
     });
     const DevState* h = M->h_state;
     std::memset(perf, 0, sizeof(*perf));
@@ -1172,8 +1194,8 @@ static void profile_t(fh_matrix* M, int kernel, int precon, int reps, double* ms
     const T* diag = diag_for<T>(M);
     const T* val = val_for<T>(M, false);
     const int which = factor_kind(precon);
-    if ((kernel == FH_KERNEL_SWEEP_FORWARD || kernel == FH_KERNEL_SWEEP_BACKWARD)
-        && (which < 0 || which == 2))
+    if ((kernel == FH_KERNEL_SWEEP_FORWARD || kernel == FH_KERNEL_SWEEP_BACKWARD
+         || kernel == FH_KERNEL_PRECONDITION) && (which < 0 || which == 2))
         fail(FH_ERR_INVALID, "sweep profiling needs DIC, FDIC, DILU, tileDIC or tileDILU");
     const T* rD = which >= 0 ? rD_for<T>(M, which) : nullptr;
     const bool local = which >= 3 && M->mesh->tiled;
@@ -1208,6 +1230,12 @@ static void profile_t(fh_matrix* M, int kernel, int precon, int reps, double* ms
     const DevState* pstate = pst.p;
     DBuf<T> p2;
     if (boxX4) { p2.alloc(N); FH_HIP(hipMemsetAsync(p2.p, 0, p2.bytes(), M->stream)); }
+    DBuf<double> psi64;
+    if (std::is_same<T, float>::value && kernel == FH_KERNEL_AXPY_NORM)
+    {
+        psi64.alloc(N);
+        FH_HIP(hipMemsetAsync(psi64.p, 0, psi64.bytes(), M->stream));
+    }
     SweepData<T> d{};
     d.val = val; d.rD = rD; d.x = x.p; d.w = y.p;
     double* part = M->d_partials.p;
@@ -1254,7 +1282,17 @@ static void profile_t(fh_matrix* M, int kernel, int precon, int reps, double* ms
                 sweep_t<T>(M, bwd ? SWEEP_PRECOND_BWD : SWEEP_PRECOND_FWD, d, nullptr, local);
                 break;
             }
+            case FH_KERNEL_PRECONDITION:
+                apply_precond<T>(M, precon, x.p, y.p, part, pst.p, false);
+                break;
             case FH_KERNEL_AXPY_NORM:
+                if constexpr (std::is_same<T, float>::value)
+                {
+                    // the update of the mixed inner iteration: FP64 solution,
+                    // FP32 recurrence residual
+                    launch_mixed_update(N, psi64.p, y.p, z.p, z.p, part, pstate, M->stream);
+                    break;
+                }
                 launch_pcg_update_nostate<T>(N, x.p, y.p, z.p, z.p, M->d_partials.p, M->stream);
                 break;
             default:
@@ -1293,8 +1331,22 @@ static void profile_t(fh_matrix* M, int kernel, int precon, int reps, double* ms
             else
                 *bytes = N*(3*w + 4) + 1.0*F*(w + 4);
             break;
-        case FH_KERNEL_AXPY_NORM:     // psi, r read+write, p, w read
-            *bytes = 6.0*N*w; break;
+        case FH_KERNEL_PRECONDITION:
+            if (box && local && w == 4 && (mesh->boxNx & 15) == 0)
+                // one launch: x, rD, one coefficient per face (each triangle
+                // from the lower arrays when symmetric), w written once
+                *bytes = N*3*w + (M->symmetric ? 1.0 : 2.0)*F*w;
+            else if (box && !local)
+                // two passes: x, rD, w, then w, rD, w, r; a coefficient
+                // per face in each
+                *bytes = N*7*w + 2.0*F*w;
+            else
+                *bytes = 2.0*(N*(3*w + 4) + 1.0*F*(w + 4)) + N*w;
+            break;
+        case FH_KERNEL_AXPY_NORM:
+            // FP32 (mixed): psi FP64 read+write, r read+write, p, w read;
+            // FP64: psi, r read+write, p, w read
+            *bytes = w == 4 ? N*(16.0 + 4*w) : 6.0*N*w; break;
     }
 }
 

@@ -91,10 +91,10 @@ void launch_pcg_update_nostate(int32_t n, T* psi, T* r, const T* p, const T* w,
 int32_t launch_mixed_update(int32_t n, double* x, float* r, const float* p, const float* w,
                             double* partials, const DevState* st, hipStream_t s);
 // mixed PCG residual replacement (only when st->replace): r = b - A x in
-// FP64, r32 = r, sum |r|
-void launch_mixed_replace(const RowsView& R, const double* diag, const double* val,
-                          const double* x, const double* b, float* r32, double* partials,
-                          const DevState* st, hipStream_t s);
+// FP64, r32 = r, sum |r|; returns the partial count
+int32_t launch_mixed_replace(const RowsView& R, const double* diag, const double* val,
+                             const double* x, const double* b, float* r32, double* partials,
+                             const DevState* st, hipStream_t s);
 void launch_mixed_setup(DevState* st, double replaceRatio, hipStream_t s);
 // psi += w
 void launch_axpy_inplace(int32_t n, double* psi, const double* w, const DevState* st,

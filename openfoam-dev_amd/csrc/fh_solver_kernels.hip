@@ -922,10 +922,11 @@ __global__ void k_mixed_replace(RowsView R, const double* __restrict__ diag,
                                 const double* __restrict__ b, float* __restrict__ r32,
                                 double* partials, const DevState* st)
 {
+    // launched every iteration, works only on replacement iterations: a
+    // strided grid of at most MIX_BLOCKS blocks keeps the idle launch short
     if (st->done || !st->replace) return;
-    const int32_t r = blockIdx.x*FH_BLOCK + threadIdx.x;
     double s1 = 0.0;
-    if (r < R.n)
+    for (int32_t r = blockIdx.x*FH_BLOCK + threadIdx.x; r < R.n; r += gridDim.x*FH_BLOCK)
     {
         const int32_t nEnt = R.rowInfo[r] >> 16;
         const int32_t base = sell_base2(R.sliceStart, r);
@@ -936,7 +937,7 @@ __global__ void k_mixed_replace(RowsView R, const double* __restrict__ diag,
             acc -= val[e]*x[R.col[e]];
         }
         r32[r] = (float)acc;
-        s1 = fabs(acc);
+        s1 += fabs(acc);
     }
     write_partials(0.0, s1, partials, gridDim.x);
 }
@@ -963,12 +964,14 @@ int32_t launch_mixed_update(int32_t n, double* x, float* r, const float* p, cons
     return nb;
 }
 
-void launch_mixed_replace(const RowsView& R, const double* diag, const double* val,
-                          const double* x, const double* b, float* r32, double* partials,
-                          const DevState* st, hipStream_t s)
+int32_t launch_mixed_replace(const RowsView& R, const double* diag, const double* val,
+                             const double* x, const double* b, float* r32, double* partials,
+                             const DevState* st, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_mixed_replace, dim3(reduction_blocks(R.n)), dim3(FH_BLOCK), 0, s,
+    const int32_t nb = std::min(reduction_blocks(R.n), MIX_BLOCKS);
+    hipLaunchKernelGGL(k_mixed_replace, dim3(nb), dim3(FH_BLOCK), 0, s,
                        R, diag, val, x, b, r32, partials, st);
+    return nb;
 }
 
 __global__ void k_axpy_inplace(int32_t n, double* __restrict__ psi,

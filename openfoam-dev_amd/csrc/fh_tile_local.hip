@@ -6,6 +6,7 @@
 // 16 contiguous cells.
 #include "fh_tile_sweep.h"
 
+
 namespace fh
 {
 
@@ -218,6 +219,214 @@ void k_tile_local_f32(TileView G, const float* a, const float* __restrict__ rD,
     }
 }
 
+// The whole tile-local apply in one kernel for boxes whose rows are whole
+// 16-wide, 16-byte aligned vectors (nx a multiple of 16): the forward walk,
+// then the backward walk of the same four tiles by the same wavefront, so
+// the backward pass reads the forward result and the tile's coefficients
+// while they are still in the caches.  The inputs of the lane's row on the
+// next diagonal are loaded while the current diagonal is computed (two
+// register sets in turn).  Coefficients of the backward pass: the upper
+// arrays, or (symmetric) the lower arrays of the rows (j, k+1), (j+1, k) and
+// of the row itself shifted by one cell.  Same per-row arithmetic as
+// k_tile_local_f32.
+struct TlRow
+{
+    float4 A[4], D[4], K[4], J[4], I[4];
+};
+
+__device__ __forceinline__ float q4(const float4 (&v)[4], int i)
+{
+    const float4 q = v[i >> 2];
+    const int e = i & 3;
+    return e == 0 ? q.x : (e == 1 ? q.y : (e == 2 ? q.z : q.w));
+}
+
+struct TlTile
+{
+    int ey, ez, nDiag;
+    int64_t base, sy, sz;
+    bool ok;
+};
+
+template<bool BWD>
+__device__ __forceinline__ bool tl_row(const TlTile& T, int l16, int step, int& j, int& k)
+{
+    const int d = BWD ? T.nDiag - 1 - step : step;
+    const int kmin = max(0, d - (T.ey - 1));
+    k = kmin + l16;
+    j = d - k;
+    return T.ok && step < T.nDiag && k < T.ez && j >= 0 && j < T.ey;
+}
+
+// coefficient arrays of a pass: K, J, I; with SHIFT (symmetric backward)
+// read at the rows (j, k+1), (j+1, k) and one cell further along i
+struct TlCoef
+{
+    const float *K, *J, *I;
+};
+
+template<bool BWD, bool SHIFT>
+__device__ __forceinline__ void tl_load(const TlTile& T, int l16, int step, const float* a,
+                                        const float* rD, const TlCoef& C, TlRow& o)
+{
+    int j, k;
+    const bool act = tl_row<BWD>(T, l16, step, j, k);
+    const int64_t row = act ? T.base + T.sy*j + T.sz*k : T.base;
+    // neighbour rows inside the tile (else the row itself: value unused)
+    const int64_t rowK = (SHIFT && act && k + 1 < T.ez) ? row + T.sz : row;
+    const int64_t rowJ = (SHIFT && act && j + 1 < T.ey) ? row + T.sy : row;
+#pragma unroll
+    for (int v = 0; v < 4; v++)
+    {
+        o.A[v] = reinterpret_cast<const float4*>(a + row)[v];
+        o.D[v] = reinterpret_cast<const float4*>(rD + row)[v];
+        o.K[v] = reinterpret_cast<const float4*>(C.K + rowK)[v];
+        o.J[v] = reinterpret_cast<const float4*>(C.J + rowJ)[v];
+        o.I[v] = reinterpret_cast<const float4*>(C.I + row)[v];
+    }
+}
+
+template<bool BWD, int BZ, bool SHIFT>
+__device__ __forceinline__ void tl_step(const TlTile& T, int l16, int step, const TlRow& c,
+                                        float (*bufg)[BZ][16], float* w, const float* r,
+                                        bool wantDot, double& dotWR)
+{
+    int j, k;
+    const bool act = tl_row<BWD>(T, l16, step, j, k);
+    if (act)
+    {
+        const int64_t row = T.base + T.sy*j + T.sz*k;
+        float4 R[4];
+        if (BWD && wantDot)
+        {
+#pragma unroll
+            for (int v = 0; v < 4; v++) R[v] = reinterpret_cast<const float4*>(r + row)[v];
+        }
+        const float* nb = bufg[(step + 1) & 1][0];
+        const bool hasK = BWD ? k < T.ez - 1 : k > 0;
+        const bool hasJ = BWD ? j < T.ey - 1 : j > 0;
+        const int sK = BWD ? k + 1 : k - 1;
+        float out[16];
+        float prev = 0.f;
+#pragma unroll
+        for (int q = 0; q < 16; q++)
+        {
+            const int i = BWD ? 15 - q : q;
+            const float Di = q4(c.D, i);
+            float v = BWD ? q4(c.A, i) : Di*q4(c.A, i);
+            if (hasK) v -= Di*q4(c.K, i)*nb[sK*16 + i];
+            if (hasJ) v -= Di*q4(c.J, i)*nb[k*16 + i];
+            // SHIFT: the upper i coefficient of cell i is the lower one of
+            // cell i+1 (cell 15 has no i+1 inside the tile)
+            if (BWD ? i < 15 : i > 0) v -= Di*q4(c.I, SHIFT ? min(i + 1, 15) : i)*prev;
+            prev = v;
+            out[i] = v;
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+            reinterpret_cast<float4*>(w + row)[v] =
+                make_float4(out[4*v], out[4*v + 1], out[4*v + 2], out[4*v + 3]);
+        float* mine = bufg[step & 1][k];
+#pragma unroll
+        for (int i = 0; i < 16; i++) mine[i] = out[i];
+        if (BWD && wantDot)
+        {
+            double sr = 0.0;
+#pragma unroll
+            for (int i = 0; i < 16; i++) sr += (double)out[i]*(double)q4(R, i);
+            dotWR += sr;
+        }
+    }
+    __syncthreads();
+}
+
+template<bool BWD, int BZ, bool SHIFT>
+__device__ __forceinline__ void tl_walk(const TlTile& T, int l16, int nD, const float* a,
+                                        const float* rD, const TlCoef& C,
+                                        float (*bufg)[BZ][16], float* w, const float* r,
+                                        bool wantDot, double& dotWR)
+{
+    TlRow r0, r1;
+    tl_load<BWD, SHIFT>(T, l16, 0, a, rD, C, r0);
+    for (int step = 0; step < nD; step += 2)
+    {
+        tl_load<BWD, SHIFT>(T, l16, step + 1, a, rD, C, r1);
+        tl_step<BWD, BZ, SHIFT>(T, l16, step, r0, bufg, w, r, wantDot, dotWR);
+        if (step + 1 >= nD) break;
+        tl_load<BWD, SHIFT>(T, l16, step + 2, a, rD, C, r0);
+        tl_step<BWD, BZ, SHIFT>(T, l16, step + 1, r1, bufg, w, r, wantDot, dotWR);
+    }
+}
+
+// PASSES: 1 forward, 2 backward, 3 forward then backward; TPW tiles per
+// wavefront (16 lanes each; the other lanes idle); SYM: the backward pass
+// reads the lower coefficient arrays (shifted)
+template<int PASSES, int BZ, int TPW, bool SYM>
+__global__ __launch_bounds__(64)
+void k_tile_local_vec(TileView G, const float* x, const float* __restrict__ rD,
+                      TlCoef lo, TlCoef up, float* w, const DevState* st,
+                      const float* __restrict__ r, double* partials)
+{
+    if (st && st->done) return;
+    __shared__ float buf[TPW][2][BZ][16];
+    const int lane = threadIdx.x;
+    const int g = min(lane >> 4, TPW - 1), l16 = lane & 15;
+    const int nTiles = G.nTI*G.nTJ*G.nTK;
+    const int tile = blockIdx.x*TPW + g;
+    TlTile T;
+    T.ok = (lane >> 4) < TPW && tile < nTiles;
+    const int t = T.ok ? tile : 0;
+    const int TI = t % G.nTI, TJ = (t/G.nTI) % G.nTJ, TK = t/(G.nTI*G.nTJ);
+    T.ey = min(16, G.ny - TJ*16);
+    T.ez = min(BZ, G.nz - TK*BZ);
+    T.nDiag = T.ey + T.ez - 1;
+    T.sy = G.nx;
+    T.sz = (int64_t)G.nx*G.ny;
+    T.base = TI*16 + T.sy*(TJ*16) + T.sz*(TK*BZ);
+    const bool wantDot = (PASSES & 2) && partials != nullptr;
+    double dotWR = 0.0;
+    // the wave walks the longest diagonal count of its tiles
+    int nD = T.ok ? T.nDiag : 0;
+    nD = max(nD, __shfl_xor(nD, 16, 64));
+    nD = max(nD, __shfl_xor(nD, 32, 64));
+    nD = __builtin_amdgcn_readfirstlane(nD);
+    float (*bufg)[BZ][16] = buf[g];
+    if (PASSES & 1) tl_walk<false, BZ, false>(T, l16, nD, x, rD, lo, bufg, w, r, false, dotWR);
+    // (the walk ends on a workgroup barrier: the forward rows are visible)
+    if (PASSES & 2) tl_walk<true, BZ, SYM>(T, l16, nD, w, rD, up, bufg, w, r, wantDot, dotWR);
+    if (wantDot)
+    {
+        // one partial per wavefront (= workgroup), fixed butterfly order
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) dotWR += __shfl_xor(dotWR, off, 64);
+        if (lane == 0)
+        {
+            partials[blockIdx.x] = dotWR;
+            partials[gridDim.x + blockIdx.x] = 0.0;
+        }
+    }
+}
+
+int32_t launch_tile_local_apply_f32(const TileView& G, const float* x, const float* rD,
+                                    const float* const dir[6], bool symmetric, float* w,
+                                    const DevState* st, hipStream_t s, double* partials)
+{
+    if (G.bx != 16 || G.by != 16 || G.bz != TileBZ<float>::pre || (G.nx & 15))
+        return -1;
+    const int nTiles = G.nTI*G.nTJ*G.nTK;
+    const TlCoef lo{dir[0], dir[1], dir[2]};
+    const TlCoef up = symmetric ? TlCoef{dir[0], dir[1], dir[2]}
+                                : TlCoef{dir[5], dir[4], dir[3]};
+    const dim3 grid((nTiles + 3)/4);
+    if (symmetric)
+        hipLaunchKernelGGL((k_tile_local_vec<3, TileBZ<float>::pre, 4, true>), grid, dim3(64), 0,
+                           s, G, x, rD, lo, up, w, st, x, partials);
+    else
+        hipLaunchKernelGGL((k_tile_local_vec<3, TileBZ<float>::pre, 4, false>), grid, dim3(64), 0,
+                           s, G, x, rD, lo, up, w, st, x, partials);
+    return (int32_t)grid.x;
+}
+
 int32_t launch_tile_local_f32(bool backward, const TileView& G, const float* a, const float* rD,
                               const float* cK, const float* cJ, const float* cI, float* w,
                               const DevState* st, hipStream_t s, const float* r,
@@ -227,6 +436,17 @@ int32_t launch_tile_local_f32(bool backward, const TileView& G, const float* a, 
         fail(FH_ERR_INVALID, "tile shape does not match the compiled local kernels");
     const int nTiles = G.nTI*G.nTJ*G.nTK;
     const dim3 grid((nTiles + 3)/4);
+    if ((G.nx & 15) == 0)
+    {
+        const TlCoef c{cK, cJ, cI};
+        if (backward)
+            hipLaunchKernelGGL((k_tile_local_vec<2, TileBZ<float>::pre, 4, false>), grid, dim3(64), 0, s,
+                               G, a, rD, c, c, w, st, r, partials);
+        else
+            hipLaunchKernelGGL((k_tile_local_vec<1, TileBZ<float>::pre, 4, false>), grid, dim3(64), 0, s,
+                               G, a, rD, c, c, w, st, nullptr, nullptr);
+        return (int32_t)grid.x;
+    }
     if (backward)
         hipLaunchKernelGGL((k_tile_local_f32<true, TileBZ<float>::pre>), grid, dim3(64), 0, s,
                            G, a, rD, cK, cJ, cI, w, st, r, partials);

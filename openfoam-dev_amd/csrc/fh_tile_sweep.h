@@ -58,6 +58,13 @@ void launch_pack_dir_coefs(const TileView& G, const RowsView& R, const T* val, i
 // forward: a = r, backward: a = w (in place); cK, cJ, cI the coefficients of
 // the pass (lower forward, upper backward).  Backward with partials: also
 // sum w*r into one partial per workgroup; returns the workgroup count.
+// both passes in one launch (nx a multiple of 16; returns -1 otherwise,
+// nothing launched): w = (tile-local DIC/DILU)^-1 x from the packed
+// coefficients (symmetric: the lower three only), with sum w*x into one
+// partial per workgroup when partials is given; returns the partial count
+int32_t launch_tile_local_apply_f32(const TileView& G, const float* x, const float* rD,
+                                    const float* const dir[6], bool symmetric, float* w,
+                                    const DevState* st, hipStream_t s, double* partials);
 int32_t launch_tile_local_f32(bool backward, const TileView& G, const float* a, const float* rD,
                               const float* cK, const float* cJ, const float* cI, float* w,
                               const DevState* st, hipStream_t s, const float* r = nullptr,

@@ -112,6 +112,8 @@ FH_SYMBOLS = {
     "fh_precondition_f32": (_i32, [_vp, _i32, _vp, _vp]),
     "fh_profile_kernel": (_i32, [_vp, _i32, _i32, _i32, _i32, ctypes.POINTER(_dbl),
                                  ctypes.POINTER(_dbl)]),
+    "fh_kernel_timing": (_i32, [_vp, ctypes.c_uint32]),
+    "fh_kernel_timing_get": (_i32, [_vp, _i32, ctypes.POINTER(_dbl), ctypes.POINTER(_i64)]),
     "fh_boundary_create": (_i32, [_vp, _i32, _vp, ctypes.POINTER(_vp)]),
     "fh_boundary_destroy": (_i32, [_vp]),
     "fh_assemble_laplacian": (_i32, [_vp, _vp] + [_vp] * 10 + [_vp, _i32]),
