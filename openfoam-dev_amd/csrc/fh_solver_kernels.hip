@@ -504,6 +504,201 @@ void k_box_spmv_march(int32_t nx, int32_t ny, int32_t nz, int32_t kc,
     write_partials(s0, 0.0, partials, gridDim.x);
 }
 
+// The fused PCG product over slabs of ROWS rows: a block of 64 x ROWS
+// threads owns a 256 (i) x ROWS (j) tile of every plane of a run of kc
+// planes and marches along k as k_box_spmv_march does (planes k-1, k, k+1 of
+// the product's input and the lower-k coefficient of plane k+1 in
+// registers), but the j-1 / j+1 rows and the j+1 row of the lower-j
+// coefficients come from the block's own LDS copy of the plane; only the two
+// rows just outside the slab are loaded from memory.  An XCD (block id mod
+// 8) takes a contiguous range of slabs, so those two rows are usually in its
+// L2.  Same per-cell sums as k_box_spmv_dot_x4.
+template<class T, int ROWS>
+__global__ __launch_bounds__(64*ROWS)
+void k_box_spmv_slab(int32_t nx, int32_t ny, int32_t nz, int32_t kc,
+                     const T* __restrict__ diag,
+                     const T* __restrict__ lK, const T* __restrict__ lJ,
+                     const T* __restrict__ lI, const T* __restrict__ uI,
+                     const T* __restrict__ uJ, const T* __restrict__ uK,
+                     const T* __restrict__ pOld, T* __restrict__ y,
+                     double* partials, const DevState* st,
+                     const T* __restrict__ wv, T* __restrict__ pNew)
+{
+    if (st->done) return;
+    typedef Quad<T> Q;
+    // plane buffers: rows -1 .. ROWS of the product's input and, symmetric,
+    // of the lower-j coefficients (row r at index r + 1)
+    __shared__ T sX[ROWS + 2][256];
+    __shared__ T sJ[ROWS + 2][256];
+    __shared__ double red[ROWS];
+    const bool first = st->nIterations == 0;
+    const T beta = (T)st->beta;
+    const int32_t nIB = (nx + 255)/256, nJB = (ny + ROWS - 1)/ROWS, nKB = (nz + kc - 1)/kc;
+    const int32_t nReal = nIB*nJB*nKB;
+    const int32_t slab = (nReal + 7)/8;
+    const int32_t stride = gridDim.x/8;
+    const int64_t sxy = (int64_t)nx*ny;
+    const int q = threadIdx.x & 63, r = threadIdx.x >> 6;
+    const bool sym = uI == nullptr;
+    double s0 = 0.0;
+    for (int32_t local = (int32_t)(blockIdx.x/8); local < slab; local += stride)
+    {
+        const int32_t logical = (int32_t)(blockIdx.x % 8)*slab + local;
+        if (logical >= nReal) break;        // uniform over the block
+        const int32_t ib = logical % nIB, jb = (logical/nIB) % nJB, kb = logical/(nIB*nJB);
+        const int32_t i0 = ib*256 + 4*q;
+        const int32_t gj = jb*ROWS + r;
+        const bool ok = i0 < nx && gj < ny;
+        const int32_t k0 = kb*kc, k1 = min(nz, k0 + kc);
+        // rows just outside the slab, loaded by the first / last wave
+        const int32_t hj = r == 0 ? jb*ROWS - 1 : jb*ROWS + ROWS;
+        const bool halo = (r == 0 || r == ROWS - 1) && i0 < nx && hj >= 0 && hj < ny;
+        const int hslot = r == 0 ? 0 : ROWS + 1;
+        const int64_t rowc = (int64_t)(ok ? i0 : 0) + (int64_t)nx*(ok ? gj : 0);
+        const int64_t rowh = (int64_t)(halo ? i0 : 0) + (int64_t)nx*(halo ? hj : 0);
+        auto px4 = [&](int64_t i, T (&o)[4])
+        {
+            Q::load(wv, i, o);
+            if (first) return;
+            T pq[4];
+            Q::load(pOld, i, pq);
+#pragma unroll
+            for (int e = 0; e < 4; e++) o[e] = o[e] + beta*pq[e];
+        };
+        auto px1 = [&](int64_t i) -> T { return first ? wv[i] : wv[i] + beta*pOld[i]; };
+        // every input of plane k+1 is loaded while plane k is computed
+        T xm[4] = {}, x0[4] = {}, xn[4] = {}, lkc[4] = {}, lkn[4] = {};
+        T dg[4] = {}, lj[4] = {}, li[4] = {}, hx[4] = {}, hl[4] = {};
+        T dgn[4] = {}, ljn[4] = {}, lin[4] = {}, hxn[4] = {}, hln[4] = {};
+        if (ok)
+        {
+            const int64_t c = rowc + sxy*k0;
+            if (k0 > 0) px4(c - sxy, xm);
+            px4(c, x0);
+            Q::load(lK, c, lkc);
+            Q::load(diag, c, dg);
+            Q::load(lJ, c, lj);
+            Q::load(lI, c, li);
+        }
+        if (halo)
+        {
+            px4(rowh + sxy*k0, hx);
+            if (sym) Q::load(lJ, rowh + sxy*k0, hl);
+        }
+        for (int32_t k = k0; k < k1; k++)
+        {
+            const int64_t c = rowc + sxy*k;
+            const bool up = k < nz - 1;
+            const bool more = k + 1 < k1;
+            if (ok && up)
+            {
+                px4(c + sxy, xn);
+                if (sym) Q::load(lK, c + sxy, lkn);
+                if (more)
+                {
+                    Q::load(diag, c + sxy, dgn);
+                    Q::load(lJ, c + sxy, ljn);
+                    Q::load(lI, c + sxy, lin);
+                }
+            }
+            if (halo && more)
+            {
+                px4(rowh + sxy*(k + 1), hxn);
+                if (sym) Q::load(lJ, rowh + sxy*(k + 1), hln);
+            }
+            // publish this plane's row (and the halo row) to the block
+#pragma unroll
+            for (int e = 0; e < 4; e++)
+            {
+                sX[r + 1][4*q + e] = x0[e];
+                sJ[r + 1][4*q + e] = lj[e];
+                if (halo) { sX[hslot][4*q + e] = hx[e]; sJ[hslot][4*q + e] = hl[e]; }
+            }
+            __syncthreads();
+            if (ok)
+            {
+                T xj[4], xjp[4], uj[4] = {}, ui[4], uk[4], lkk[4];
+#pragma unroll
+                for (int e = 0; e < 4; e++)
+                {
+                    xj[e] = gj > 0 ? sX[r][4*q + e] : T(0);
+                    xjp[e] = gj < ny - 1 ? sX[r + 2][4*q + e] : T(0);
+                }
+                // i neighbours: inside the 256-cell row from LDS, across
+                // i-blocks from memory
+                const T xw = i0 == 0 ? T(0) : (q > 0 ? sX[r + 1][4*q - 1] : px1(c - 1));
+                const T xe = i0 + 4 >= nx ? T(0) : (q < 63 ? sX[r + 1][4*q + 4] : px1(c + 4));
+                if (!sym)
+                {
+                    Q::load(lK, c, lkk);
+                    Q::load(uI, c, ui); Q::load(uJ, c, uj); Q::load(uK, c, uk);
+                }
+                else
+                {
+#pragma unroll
+                    for (int e = 0; e < 4; e++) lkk[e] = lkc[e];
+                    // symmetric: the upper coefficient of a face is the
+                    // lower one of its upper cell
+                    ui[0] = li[1]; ui[1] = li[2]; ui[2] = li[3];
+                    ui[3] = i0 + 4 < nx ? lI[c + 4] : T(0);
+#pragma unroll
+                    for (int e = 0; e < 4; e++)
+                    {
+                        uj[e] = gj < ny - 1 ? sJ[r + 2][4*q + e] : T(0);
+                        uk[e] = lkn[e];
+                    }
+                }
+                Q::store(pNew, c, x0);
+                const T xs[6] = {xw, x0[0], x0[1], x0[2], x0[3], xe};
+                T out[4];
+#pragma unroll
+                for (int e = 0; e < 4; e++)
+                {
+                    const int32_t gi = i0 + e;
+                    T acc = dg[e]*xs[e + 1];
+                    if (k > 0) acc += lkk[e]*xm[e];
+                    if (gj > 0) acc += lj[e]*xj[e];
+                    if (gi > 0) acc += li[e]*xs[e];
+                    if (gi < nx - 1) acc += ui[e]*xs[e + 2];
+                    if (gj < ny - 1) acc += uj[e]*xjp[e];
+                    if (up) acc += uk[e]*xn[e];
+                    out[e] = acc;
+                }
+                Q::store(y, c, out);
+                s0 += ((double)out[0]*(double)x0[0] + (double)out[1]*(double)x0[1])
+                   + ((double)out[2]*(double)x0[2] + (double)out[3]*(double)x0[3]);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int e = 0; e < 4; e++)
+            {
+                xm[e] = x0[e];
+                x0[e] = xn[e];
+                lkc[e] = lkn[e];
+                dg[e] = dgn[e];
+                lj[e] = ljn[e];
+                li[e] = lin[e];
+                hx[e] = hxn[e];
+                hl[e] = hln[e];
+                xn[e] = T(0);
+                lkn[e] = T(0);
+            }
+        }
+    }
+    // one partial per block: wave butterflies, then the waves in order
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s0 += __shfl_xor(s0, off, 64);
+    if (q == 0) red[r] = s0;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        double t = 0.0;
+        for (int w = 0; w < ROWS; w++) t += red[w];
+        partials[blockIdx.x] = t;
+        partials[gridDim.x + blockIdx.x] = 0.0;
+    }
+}
+
 // planes per marching run: about 1024 runs over the box, 2..32 planes each
 static int32_t box_march_planes(int32_t nx, int32_t ny, int32_t nz)
 {
@@ -536,6 +731,25 @@ int32_t launch_box_pcg_spmv(int32_t nx, int32_t ny, int32_t nz, const T* diag,
                            diag, dir[0], dir[1], dir[2], symmetric ? nullptr : dir[3],
                            symmetric ? nullptr : dir[4], symmetric ? nullptr : dir[5], pOld, y,
                            nullptr, partials, st, w, pNew);
+        return nb;
+    }
+    static const bool march = [] {
+        const char* e = std::getenv("FOAMHIP_BOX_SPMV");
+        return e && std::strcmp(e, "march") == 0;
+    }();
+    if (!march)
+    {
+        // slabs of 16 (FP32) / 8 (FP64) rows, about one block per CU
+        constexpr int ROWS = sizeof(T) == 4 ? 16 : 8;
+        const int64_t slabs = (int64_t)((nx + 255)/256)*((ny + ROWS - 1)/ROWS);
+        const int32_t kc = (int32_t)std::max<int64_t>(
+            4, std::min<int64_t>(64, (int64_t)nz*slabs/(sizeof(T) == 4 ? 256 : 512)));
+        const int64_t runs = slabs*((nz + kc - 1)/kc);
+        const int32_t nb = (int32_t)std::min<int64_t>(8*((runs + 7)/8), 2048);
+        hipLaunchKernelGGL((k_box_spmv_slab<T, ROWS>), dim3(nb), dim3(64*ROWS), 0, s, nx, ny, nz,
+                           kc, diag, dir[0], dir[1], dir[2], symmetric ? nullptr : dir[3],
+                           symmetric ? nullptr : dir[4], symmetric ? nullptr : dir[5], pOld, y,
+                           partials, st, w, pNew);
         return nb;
     }
     const int32_t kc = box_march_planes(nx, ny, nz);
