@@ -38,6 +38,10 @@ struct fh_matrix
     // lower K, J, I and upper I, J, K per cell
     fh::DBuf<float> d_dir32[6];
     bool dir32Valid = false;
+    // FP16 lower K, J, I coefficients of a symmetric box (tile-local inner
+    // preconditioner)
+    fh::DBuf<__half> d_dir16[3];
+    bool dir16Valid = false;
     // FP64 direction-packed coefficients of hex boxes (FP64 box product and
     // sweeps): lower K, J, I, and for asymmetric matrices upper I, J, K
     fh::DBuf<double> d_dir64[6];

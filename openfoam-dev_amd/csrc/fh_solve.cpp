@@ -150,6 +150,7 @@ void matrix_coeffs_updated(fh_matrix* M)
     FH_HIP(hipStreamSynchronize(s));
     M->hasCoeffs = true;
     M->valTValid = M->sumAValid = M->f32Valid = M->dir32Valid = M->dir64Valid = false;
+    M->dir16Valid = false;
     for (int i = 0; i < 5; i++) M->rDValid[i] = M->rD32Valid[i] = false;
     gamg_invalidate(M);
 }
@@ -266,6 +267,15 @@ const double* ensure_rD(fh_matrix* M, int which)
     return b.p;
 }
 
+// The FP32 tile-local DIC of a symmetric box whose rows are whole 16-wide
+// vectors stores its off-diagonal coefficients in FP16: the factor and both
+// passes use the coefficients rounded to FP16 precision (the incomplete
+// Cholesky factor of that matrix; diagonal, factor and arithmetic FP32)
+static bool tile_dic_half(const fh_matrix* M)
+{
+    return M->symmetric && M->mesh->tiled && (M->mesh->boxNx & 15) == 0;
+}
+
 const float* ensure_rD32(fh_matrix* M, int which)
 {
     if (which >= 3 && !M->mesh->tiled) which -= 3;
@@ -283,8 +293,17 @@ const float* ensure_rD32(fh_matrix* M, int which)
         d.diag = M->d_diag32.p;
         d.w = b.p;
         d.upperF = M->d_upperF.p;
+        DBuf<float> rounded;
+        if (which == 3 && tile_dic_half(M))
+        {
+            const int64_t nnz = std::max<int64_t>(M->mesh->nnzPad, 1);
+            rounded.alloc(nnz);
+            launch_round_half(nnz, M->d_val32.p, rounded.p, M->stream);
+            d.val = rounded.p;
+        }
         sweep_f(M, which == 4 ? SWEEP_RD_DILU : SWEEP_RD_DIC, d, nullptr, true);
         launch_reciprocal<float>(N, b.p, M->stream);
+        if (rounded.p) FH_HIP(hipStreamSynchronize(M->stream));
     }
     else
     {
@@ -496,8 +515,19 @@ int32_t apply_precond(fh_matrix* M, int precon, const T* r, T* w, double* partia
             {
                 const float* dir[6];
                 for (int q = 0; q < 6; q++) dir[q] = M->d_dir32[q].p;
+                const __half* h3[3] = {};
+                const bool half = which == 3 && tile_dic_half(M);
+                if (half && !M->dir16Valid)
+                {
+                    __half* dst[3];
+                    for (int q = 0; q < 3; q++) { M->d_dir16[q].alloc(N); dst[q] = M->d_dir16[q].p; }
+                    launch_pack_half(N, dir, dst, M->stream);
+                    M->dir16Valid = true;
+                }
+                for (int q = 0; q < 3; q++) h3[q] = M->d_dir16[q].p;
                 const int32_t nb = launch_tile_local_apply_f32(G, r, d.rD, dir, M->symmetric, w,
-                                                               st, M->stream, partials);
+                                                               st, M->stream, partials,
+                                                               half ? h3 : nullptr);
                 if (nb >= 0) return nb;
             }
             launch_tile_local_f32(false, G, r, d.rD, M->d_dir32[0].p, M->d_dir32[1].p,

@@ -6,6 +6,8 @@
 // 16 contiguous cells.
 #include "fh_tile_sweep.h"
 
+#include <hip/hip_fp16.h>
+
 
 namespace fh
 {
@@ -229,17 +231,41 @@ void k_tile_local_f32(TileView G, const float* a, const float* __restrict__ rD,
 // arrays, or (symmetric) the lower arrays of the rows (j, k+1), (j+1, k) and
 // of the row itself shifted by one cell.  Same per-row arithmetic as
 // k_tile_local_f32.
-struct TlRow
-{
-    float4 A[4], D[4], K[4], J[4], I[4];
-};
-
 __device__ __forceinline__ float q4(const float4 (&v)[4], int i)
 {
     const float4 q = v[i >> 2];
     const int e = i & 3;
     return e == 0 ? q.x : (e == 1 ? q.y : (e == 2 ? q.z : q.w));
 }
+
+// 16 coefficients of a row as stored: FP32 (four 16-byte words) or FP16
+// (two 16-byte words)
+template<class CT> struct CoefRow;
+template<> struct CoefRow<float>
+{
+    float4 v[4];
+    __device__ void load(const float* p) { for (int u = 0; u < 4; u++) v[u] = reinterpret_cast<const float4*>(p)[u]; }
+    __device__ float at(int i) const { return q4(v, i); }
+};
+template<> struct CoefRow<__half>
+{
+    uint4 v[2];
+    __device__ void load(const __half* p) { for (int u = 0; u < 2; u++) v[u] = reinterpret_cast<const uint4*>(p)[u]; }
+    __device__ float at(int i) const
+    {
+        const uint4 q = v[i >> 3];
+        const int e = (i >> 1) & 3;
+        const unsigned int wd = e == 0 ? q.x : (e == 1 ? q.y : (e == 2 ? q.z : q.w));
+        return __half2float(__ushort_as_half((unsigned short)(wd >> (16*(i & 1)))));
+    }
+};
+
+template<class CT>
+struct TlRow
+{
+    float4 A[4], D[4];
+    CoefRow<CT> K, J, I;
+};
 
 struct TlTile
 {
@@ -260,14 +286,15 @@ __device__ __forceinline__ bool tl_row(const TlTile& T, int l16, int step, int& 
 
 // coefficient arrays of a pass: K, J, I; with SHIFT (symmetric backward)
 // read at the rows (j, k+1), (j+1, k) and one cell further along i
+template<class CT>
 struct TlCoef
 {
-    const float *K, *J, *I;
+    const CT *K, *J, *I;
 };
 
-template<bool BWD, bool SHIFT>
+template<bool BWD, bool SHIFT, class CT>
 __device__ __forceinline__ void tl_load(const TlTile& T, int l16, int step, const float* a,
-                                        const float* rD, const TlCoef& C, TlRow& o)
+                                        const float* rD, const TlCoef<CT>& C, TlRow<CT>& o)
 {
     int j, k;
     const bool act = tl_row<BWD>(T, l16, step, j, k);
@@ -280,14 +307,14 @@ __device__ __forceinline__ void tl_load(const TlTile& T, int l16, int step, cons
     {
         o.A[v] = reinterpret_cast<const float4*>(a + row)[v];
         o.D[v] = reinterpret_cast<const float4*>(rD + row)[v];
-        o.K[v] = reinterpret_cast<const float4*>(C.K + rowK)[v];
-        o.J[v] = reinterpret_cast<const float4*>(C.J + rowJ)[v];
-        o.I[v] = reinterpret_cast<const float4*>(C.I + row)[v];
     }
+    o.K.load(C.K + rowK);
+    o.J.load(C.J + rowJ);
+    o.I.load(C.I + row);
 }
 
-template<bool BWD, int BZ, bool SHIFT>
-__device__ __forceinline__ void tl_step(const TlTile& T, int l16, int step, const TlRow& c,
+template<bool BWD, int BZ, bool SHIFT, class CT>
+__device__ __forceinline__ void tl_step(const TlTile& T, int l16, int step, const TlRow<CT>& c,
                                         float (*bufg)[BZ][16], float* w, const float* r,
                                         bool wantDot, double& dotWR)
 {
@@ -314,11 +341,11 @@ __device__ __forceinline__ void tl_step(const TlTile& T, int l16, int step, cons
             const int i = BWD ? 15 - q : q;
             const float Di = q4(c.D, i);
             float v = BWD ? q4(c.A, i) : Di*q4(c.A, i);
-            if (hasK) v -= Di*q4(c.K, i)*nb[sK*16 + i];
-            if (hasJ) v -= Di*q4(c.J, i)*nb[k*16 + i];
+            if (hasK) v -= Di*c.K.at(i)*nb[sK*16 + i];
+            if (hasJ) v -= Di*c.J.at(i)*nb[k*16 + i];
             // SHIFT: the upper i coefficient of cell i is the lower one of
             // cell i+1 (cell 15 has no i+1 inside the tile)
-            if (BWD ? i < 15 : i > 0) v -= Di*q4(c.I, SHIFT ? min(i + 1, 15) : i)*prev;
+            if (BWD ? i < 15 : i > 0) v -= Di*c.I.at(SHIFT ? min(i + 1, 15) : i)*prev;
             prev = v;
             out[i] = v;
         }
@@ -340,13 +367,13 @@ __device__ __forceinline__ void tl_step(const TlTile& T, int l16, int step, cons
     __syncthreads();
 }
 
-template<bool BWD, int BZ, bool SHIFT>
+template<bool BWD, int BZ, bool SHIFT, class CT>
 __device__ __forceinline__ void tl_walk(const TlTile& T, int l16, int nD, const float* a,
-                                        const float* rD, const TlCoef& C,
+                                        const float* rD, const TlCoef<CT>& C,
                                         float (*bufg)[BZ][16], float* w, const float* r,
                                         bool wantDot, double& dotWR)
 {
-    TlRow r0, r1;
+    TlRow<CT> r0, r1;
     tl_load<BWD, SHIFT>(T, l16, 0, a, rD, C, r0);
     for (int step = 0; step < nD; step += 2)
     {
@@ -361,10 +388,10 @@ __device__ __forceinline__ void tl_walk(const TlTile& T, int l16, int nD, const 
 // PASSES: 1 forward, 2 backward, 3 forward then backward; TPW tiles per
 // wavefront (16 lanes each; the other lanes idle); SYM: the backward pass
 // reads the lower coefficient arrays (shifted)
-template<int PASSES, int BZ, int TPW, bool SYM>
+template<int PASSES, int BZ, int TPW, bool SYM, class CT>
 __global__ __launch_bounds__(64)
 void k_tile_local_vec(TileView G, const float* x, const float* __restrict__ rD,
-                      TlCoef lo, TlCoef up, float* w, const DevState* st,
+                      TlCoef<CT> lo, TlCoef<CT> up, float* w, const DevState* st,
                       const float* __restrict__ r, double* partials)
 {
     if (st && st->done) return;
@@ -409,22 +436,61 @@ void k_tile_local_vec(TileView G, const float* x, const float* __restrict__ rD,
 
 int32_t launch_tile_local_apply_f32(const TileView& G, const float* x, const float* rD,
                                     const float* const dir[6], bool symmetric, float* w,
-                                    const DevState* st, hipStream_t s, double* partials)
+                                    const DevState* st, hipStream_t s, double* partials,
+                                    const __half* const half3[3])
 {
     if (G.bx != 16 || G.by != 16 || G.bz != TileBZ<float>::pre || (G.nx & 15))
         return -1;
     const int nTiles = G.nTI*G.nTJ*G.nTK;
-    const TlCoef lo{dir[0], dir[1], dir[2]};
-    const TlCoef up = symmetric ? TlCoef{dir[0], dir[1], dir[2]}
-                                : TlCoef{dir[5], dir[4], dir[3]};
     const dim3 grid((nTiles + 3)/4);
+    if (symmetric && half3)
+    {
+        const TlCoef<__half> h{half3[0], half3[1], half3[2]};
+        hipLaunchKernelGGL((k_tile_local_vec<3, TileBZ<float>::pre, 4, true, __half>), grid,
+                           dim3(64), 0, s, G, x, rD, h, h, w, st, x, partials);
+        return (int32_t)grid.x;
+    }
+    const TlCoef<float> lo{dir[0], dir[1], dir[2]};
+    const TlCoef<float> up = symmetric ? TlCoef<float>{dir[0], dir[1], dir[2]}
+                                       : TlCoef<float>{dir[5], dir[4], dir[3]};
     if (symmetric)
-        hipLaunchKernelGGL((k_tile_local_vec<3, TileBZ<float>::pre, 4, true>), grid, dim3(64), 0,
-                           s, G, x, rD, lo, up, w, st, x, partials);
+        hipLaunchKernelGGL((k_tile_local_vec<3, TileBZ<float>::pre, 4, true, float>), grid,
+                           dim3(64), 0, s, G, x, rD, lo, up, w, st, x, partials);
     else
-        hipLaunchKernelGGL((k_tile_local_vec<3, TileBZ<float>::pre, 4, false>), grid, dim3(64), 0,
-                           s, G, x, rD, lo, up, w, st, x, partials);
+        hipLaunchKernelGGL((k_tile_local_vec<3, TileBZ<float>::pre, 4, false, float>), grid,
+                           dim3(64), 0, s, G, x, rD, lo, up, w, st, x, partials);
     return (int32_t)grid.x;
+}
+
+// FP16 copies of the three lower coefficient arrays of a box
+__global__ void k_pack_half(int64_t n, const float* __restrict__ a, const float* __restrict__ b,
+                            const float* __restrict__ c, __half* __restrict__ ha,
+                            __half* __restrict__ hb, __half* __restrict__ hc)
+{
+    const int64_t i = (int64_t)blockIdx.x*TL_B + threadIdx.x;
+    if (i >= n) return;
+    ha[i] = __float2half_rn(a[i]);
+    hb[i] = __float2half_rn(b[i]);
+    hc[i] = __float2half_rn(c[i]);
+}
+
+// values rounded to FP16 precision, kept in FP32
+__global__ void k_round_half(int64_t n, const float* __restrict__ a, float* __restrict__ b)
+{
+    const int64_t i = (int64_t)blockIdx.x*TL_B + threadIdx.x;
+    if (i < n) b[i] = __half2float(__float2half_rn(a[i]));
+}
+
+void launch_round_half(int64_t n, const float* a, float* b, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_round_half, dim3((unsigned)((n + TL_B - 1)/TL_B)), dim3(TL_B), 0, s, n,
+                       a, b);
+}
+
+void launch_pack_half(int64_t n, const float* const src[3], __half* const dst[3], hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pack_half, dim3((unsigned)((n + TL_B - 1)/TL_B)), dim3(TL_B), 0, s, n,
+                       src[0], src[1], src[2], dst[0], dst[1], dst[2]);
 }
 
 int32_t launch_tile_local_f32(bool backward, const TileView& G, const float* a, const float* rD,
@@ -438,13 +504,13 @@ int32_t launch_tile_local_f32(bool backward, const TileView& G, const float* a, 
     const dim3 grid((nTiles + 3)/4);
     if ((G.nx & 15) == 0)
     {
-        const TlCoef c{cK, cJ, cI};
+        const TlCoef<float> c{cK, cJ, cI};
         if (backward)
-            hipLaunchKernelGGL((k_tile_local_vec<2, TileBZ<float>::pre, 4, false>), grid, dim3(64), 0, s,
-                               G, a, rD, c, c, w, st, r, partials);
+            hipLaunchKernelGGL((k_tile_local_vec<2, TileBZ<float>::pre, 4, false, float>), grid,
+                               dim3(64), 0, s, G, a, rD, c, c, w, st, r, partials);
         else
-            hipLaunchKernelGGL((k_tile_local_vec<1, TileBZ<float>::pre, 4, false>), grid, dim3(64), 0, s,
-                               G, a, rD, c, c, w, st, nullptr, nullptr);
+            hipLaunchKernelGGL((k_tile_local_vec<1, TileBZ<float>::pre, 4, false, float>), grid,
+                               dim3(64), 0, s, G, a, rD, c, c, w, st, nullptr, nullptr);
         return (int32_t)grid.x;
     }
     if (backward)

@@ -3,6 +3,8 @@
 
 #include "fh_kernels.h"
 
+#include <hip/hip_fp16.h>
+
 namespace fh
 {
 
@@ -64,7 +66,12 @@ void launch_pack_dir_coefs(const TileView& G, const RowsView& R, const T* val, i
 // partial per workgroup when partials is given; returns the partial count
 int32_t launch_tile_local_apply_f32(const TileView& G, const float* x, const float* rD,
                                     const float* const dir[6], bool symmetric, float* w,
-                                    const DevState* st, hipStream_t s, double* partials);
+                                    const DevState* st, hipStream_t s, double* partials,
+                                    const __half* const half3[3] = nullptr);
+// FP16 copies of three FP32 arrays of n values
+void launch_pack_half(int64_t n, const float* const src[3], __half* const dst[3], hipStream_t s);
+// b = a rounded to FP16 precision (kept in FP32)
+void launch_round_half(int64_t n, const float* a, float* b, hipStream_t s);
 int32_t launch_tile_local_f32(bool backward, const TileView& G, const float* a, const float* rD,
                               const float* cK, const float* cJ, const float* cI, float* w,
                               const DevState* st, hipStream_t s, const float* r = nullptr,

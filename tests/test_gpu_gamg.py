@@ -248,13 +248,42 @@ def test_mixed_pcg_tile_dic_inner():
         assert abs(p8.nIterations - exact[inner]) <= max(3, 0.05 * exact[inner])
 
 
+def test_mixed_pcg_tile_dic_half():
+    """Mixed PCG on a box whose tile-local inner preconditioner stores its
+    coefficients in FP16 (nx a multiple of 16): the FP64 residual test holds,
+    the solution matches the FP64 oracle at a tight tolerance, and at the
+    working tolerance it takes about the iterations of PCG with that
+    preconditioner (the FP16-rounded, 16^3 block-Jacobi DIC) in exact
+    arithmetic."""
+    c, a, diag, upper, b = laplacian_case(48, sigma=1.0)
+    om = O.Matrix(a, diag, upper)
+    mesh = fh.lduAddressing(c.N, c.lower, c.upper)
+    m = fh.lduMatrix(mesh, diag, upper)
+    cut = _tile_blocks(48, 48, 48, c.lower, c.upper, 16)
+    up16 = upper.astype(np.float32).astype(np.float16).astype(np.float64)
+    om_t = O.Matrix(O.Addr(c.N, c.lower, c.upper), diag, np.where(cut, 0.0, up16))
+    exact = _numpy_pcg(om, b, lambda r: om_t.precondition("DIC", r), 1e-8)
+    ctl = {"solver": "PCG", "preconditioner": "DIC", "tolerance": 1e-12, "relTol": 0,
+           "maxIter": 3000, "precision": "mixed"}
+    psi = np.zeros(c.N)
+    perf = fh.solver_New("p", m, ctl).solve(psi, b)
+    assert perf.converged and perf.finalResidual < 1e-12
+    opsi, _ = om.solve(dict(ctl, precision="double", tolerance=1e-13), np.zeros(c.N), b)
+    assert rel_l2(psi, opsi) <= 1e-8
+    p8 = fh.solver_New("p", m, dict(ctl, tolerance=1e-8)).solve(np.zeros(c.N), b)
+    print("fp16 tileDIC mixed its at 1e-8", p8.nIterations, "exact arithmetic", exact)
+    assert abs(p8.nIterations - exact) <= max(3, 0.05 * exact)
+
+
 @pytest.mark.parametrize("kind", ["DIC", "DILU"])
 @pytest.mark.parametrize("shape", [(37, 20, 19), (40, 40, 40), (16, 32, 48)])
 def test_fp32_preconditioners(kind, shape):
     """The FP32 inner preconditioners against the FP64 oracle: the exact
     factor, and the tile-local one = the oracle factor of the matrix with the
-    faces between the FP32 tiles (16 x 16 x 16) removed.  FP32 bar: relative
-    L2 difference 1e-5."""
+    faces between the FP32 tiles (16 x 16 x 16) removed -- for a symmetric
+    box with nx a multiple of 16 of that matrix with its off-diagonal
+    coefficients rounded to FP16 (the storage of that path).  FP32 bar:
+    relative L2 difference 1e-5."""
     nx, ny, nz = shape
     n, lower, upper = box(nx, ny, nz)
     rng = np.random.default_rng(9)
@@ -271,7 +300,10 @@ def test_fp32_preconditioners(kind, shape):
     ref = om.precondition(kind, r)
     assert rel_l2(w, ref) <= 1e-5, rel_l2(w, ref)
     cut = _tile_blocks(nx, ny, nz, lower, upper, 16)
-    om_t = O.Matrix(O.Addr(n, lower, upper), diag, np.where(cut, 0.0, up),
+    up_t = up
+    if kind == "DIC" and nx % 16 == 0:
+        up_t = up.astype(np.float32).astype(np.float16).astype(np.float64)
+    om_t = O.Matrix(O.Addr(n, lower, upper), diag, np.where(cut, 0.0, up_t),
                     None if lo is None else np.where(cut, 0.0, lo))
     w = m.precondition_f32("tile" + kind, r).astype(np.float64)
     ref = om_t.precondition(kind, r)
