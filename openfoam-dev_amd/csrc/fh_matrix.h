@@ -86,6 +86,9 @@ struct fh_matrix
     fh::DBuf<fh::DevState> d_state;
     fh::DevState* h_state = nullptr;      // pinned
     fh::DBuf<double> d_partials;
+    // arrival counter of the kernels that finalize themselves (zero
+    // between launches)
+    fh::DBuf<unsigned int> d_finCount;
 
     // work vectors (library numbering)
     fh::DBuf<double> v[10];
@@ -197,10 +200,12 @@ const float* ensure_rD32(fh_matrix* M, int which);
 
 // preconditioner apply w = M^-1 r on library-numbered vectors.  With
 // partials != null it also leaves sum w*r in the partials and returns their
-// block count (for launch_finalize_blocks).
+// block count (for launch_finalize_blocks).  Paths whose last kernel can
+// finalize itself apply F there and set *fused.
 template<class T>
 int32_t apply_precond(fh_matrix* M, int precon, const T* r, T* w, double* partials,
-                      DevState* st, bool transpose);
+                      DevState* st, bool transpose, const fh::FusedFinalize& F = {},
+                      bool* fused = nullptr);
 
 void profile_kernel(fh_matrix* M, int kernel, int precision, int precon, int reps,
                     double* ms, double* bytes);

@@ -42,7 +42,11 @@ void matrix_init(fh_matrix* M, fh_mesh* mesh, hipStream_t shared)
         // one partial per tile of the fused sweep sums
         nPart = std::max(nPart, 2*(size_t)mesh->maxTiles + 64);
     }
-    M->d_partials.alloc(nPart);
+    // (nPart counts a and b partials; room for the group sums of the
+    // self-finalizing kernels)
+    M->d_partials.alloc(fin_partials(nPart/2));
+    M->d_finCount.alloc(fin_counters(nPart/2));
+    FH_HIP(hipMemsetAsync(M->d_finCount.p, 0, M->d_finCount.bytes(), M->stream));
     M->d_sync.alloc(1 + (size_t)mesh->sched.nParts);
     FH_HIP(hipMemsetAsync(M->d_sync.p, 0, M->d_sync.bytes(), M->stream));
     // one-part meshes sweep row by row unless the level sweep is forced
@@ -438,7 +442,8 @@ template<> void sweep_t<float>(fh_matrix* M, int kind, const SweepData<float>& d
 // kernel over 16 x 16 x bz tiles is asked for (FOAMHIP_BOX_SWEEP=wide).
 template<class T>
 static int32_t box_precond_pass(fh_matrix* M, bool backward, const SweepData<T>& d0,
-                                double* partials, DevState* st)
+                                double* partials, DevState* st,
+                                const FusedFinalize& F = FusedFinalize(), bool* fused = nullptr)
 {
     SweepData<T> d = d0;
     set_sweep_dirs<T>(M, d, backward);
@@ -454,8 +459,10 @@ static int32_t box_precond_pass(fh_matrix* M, bool backward, const SweepData<T>&
     if (!wide)
     {
         TileSync ts = M->tile_sync();
+        const bool fuse = backward && partials && F.counter;
         const int32_t n = launch_sweep_wave<T>(backward, M->tile_view(16, 8), d, ts, st,
-                                               M->stream);
+                                               M->stream, fuse ? F : FusedFinalize());
+        if (fused) *fused = fuse;
         M->tileTicketBase = ts.ticketBase;
         M->tileEpoch = ts.epoch;
         return n;
@@ -467,9 +474,10 @@ static int32_t box_precond_pass(fh_matrix* M, bool backward, const SweepData<T>&
 
 template<class T>
 int32_t apply_precond(fh_matrix* M, int precon, const T* r, T* w, double* partials,
-                      DevState* st, bool transpose)
+                      DevState* st, bool transpose, const FusedFinalize& F, bool* fused)
 {
     const int32_t N = M->mesh->nCells;
+    if (fused) *fused = false;
     if (M->diagonalOnly || M->mesh->nFaces == 0)
     {
         // lduMatrixPreconditioner.C:84-93: a diagonal matrix gets no
@@ -525,10 +533,16 @@ int32_t apply_precond(fh_matrix* M, int precon, const T* r, T* w, double* partia
                     M->dir16Valid = true;
                 }
                 for (int q = 0; q < 3; q++) h3[q] = M->d_dir16[q].p;
+                const bool fuse = partials && F.counter;
                 const int32_t nb = launch_tile_local_apply_f32(G, r, d.rD, dir, M->symmetric, w,
                                                                st, M->stream, partials,
-                                                               half ? h3 : nullptr);
-                if (nb >= 0) return nb;
+                                                               half ? h3 : nullptr,
+                                                               fuse ? F : FusedFinalize());
+                if (nb >= 0)
+                {
+                    if (fused) *fused = fuse;
+                    return nb;
+                }
             }
             launch_tile_local_f32(false, G, r, d.rD, M->d_dir32[0].p, M->d_dir32[1].p,
                                   M->d_dir32[2].p, w, st, M->stream);
@@ -542,7 +556,7 @@ int32_t apply_precond(fh_matrix* M, int precon, const T* r, T* w, double* partia
         // global sweeps on a box: packed coefficients, and the backward pass
         // sums w*r per tile
         box_precond_pass<T>(M, false, d, nullptr, st);
-        return box_precond_pass<T>(M, true, d, partials, st);
+        return box_precond_pass<T>(M, true, d, partials, st, F, fused);
     }
     sweep_t<T>(M, SWEEP_PRECOND_FWD, d, st, local);
     sweep_t<T>(M, transpose && dilu ? SWEEP_PRECOND_BWD_T : SWEEP_PRECOND_BWD, d, st, local);
@@ -550,8 +564,10 @@ int32_t apply_precond(fh_matrix* M, int precon, const T* r, T* w, double* partia
     return reduction_blocks(N);
 }
 
-template int32_t apply_precond<double>(fh_matrix*, int, const double*, double*, double*, DevState*, bool);
-template int32_t apply_precond<float>(fh_matrix*, int, const float*, float*, double*, DevState*, bool);
+template int32_t apply_precond<double>(fh_matrix*, int, const double*, double*, double*, DevState*,
+                                       bool, const FusedFinalize&, bool*);
+template int32_t apply_precond<float>(fh_matrix*, int, const float*, float*, double*, DevState*,
+                                      bool, const FusedFinalize&, bool*);
 
 void state_upload(fh_matrix* M, const fh_controls* c, double tol, double relTol,
                          int32_t maxIter, int32_t minIter)
@@ -594,6 +610,36 @@ void init_fp64(fh_matrix* M, const double* psi, const double* src, double* rA,
     launch_finalize(N, part, FIN_INIT, st, 0, M->stream);
 }
 
+// the finalize of op applied by the last block of the reduction kernel
+// itself (FOAMHIP_FUSED_FINALIZE=0: separate finalize launches)
+static FusedFinalize fused_fin(fh_matrix* M, int32_t op)
+{
+    static const bool off = [] {
+        const char* e = std::getenv("FOAMHIP_FUSED_FINALIZE");
+        return e && std::strcmp(e, "0") == 0;
+    }();
+    FusedFinalize F;
+    if (off) return F;
+    F.counter = M->d_finCount.p;
+    F.st = M->d_state.p;
+    F.op = op;
+    F.n = M->mesh->nCells;
+    return F;
+}
+
+// preconditioner apply with sum w*r, finalized by op (inside the apply's
+// last kernel where its path allows)
+template<class T>
+static void precond_finalize(fh_matrix* M, int precon, const T* r, T* w, int32_t op)
+{
+    bool fused = false;
+    const int32_t nb = apply_precond<T>(M, precon, r, w, M->d_partials.p, M->d_state.p, false,
+                                        fused_fin(M, op), &fused);
+    if (!fused)
+        launch_finalize_blocks(nb, M->mesh->nCells, M->d_partials.p, op, M->d_state.p, 0,
+                               M->stream);
+}
+
 template<class T>
 void pcg_loop(fh_matrix* M, int precon, T* psi, T* r, T* p, T* w, int32_t maxIter)
 {
@@ -622,21 +668,24 @@ void pcg_loop(fh_matrix* M, int precon, T* psi, T* r, T* p, T* w, int32_t maxIte
             run_batches(M, maxIter, [&](int32_t seq)
             {
                 kt_mark(M, FH_KERNEL_PRECONDITION);
-                const int32_t nbw = apply_precond<T>(M, precon, r, w, part, st, false);
+                precond_finalize<T>(M, precon, r, w, FIN_PCG_WARA);
                 kt_mark(M, FH_KERNEL_PRECONDITION);
-                launch_finalize_blocks(nbw, N, part, FIN_PCG_WARA, st, 0, M->stream);
                 double* pNew = pBuf[(seq + 1) & 1];
+                const FusedFinalize fa = fused_fin(M, FIN_PCG_WAPA);
                 kt_mark(M, FH_KERNEL_SPMV);
                 const int32_t nb = launch_box_pcg_spmv<double>(mesh->boxNx, mesh->boxNy,
                                                                mesh->boxNz, diag, dir,
                                                                M->symmetric, w, pBuf[seq & 1],
-                                                               pNew, ap, part, st, M->stream);
+                                                               pNew, ap, part, st, M->stream,
+                                                               fa);
                 kt_mark(M, FH_KERNEL_SPMV);
-                launch_finalize_blocks(nb, N, part, FIN_PCG_WAPA, st, 0, M->stream);
+                if (!fa.counter)
+                    launch_finalize_blocks(nb, N, part, FIN_PCG_WAPA, st, 0, M->stream);
+                const FusedFinalize fr = fused_fin(M, FIN_PCG_RES);
                 kt_mark(M, FH_KERNEL_AXPY_NORM);
-                launch_pcg_update<T>(N, psi, r, pNew, ap, part, st, M->stream);
+                launch_pcg_update<T>(N, psi, r, pNew, ap, part, st, M->stream, fr);
                 kt_mark(M, FH_KERNEL_AXPY_NORM);
-                launch_finalize(N, part, FIN_PCG_RES, st, 0, M->stream);
+                if (!fr.counter) launch_finalize(N, part, FIN_PCG_RES, st, 0, M->stream);
             });
             return;
         }
@@ -905,19 +954,20 @@ static void mixed_pcg(fh_matrix* M, const fh_controls* c, double* psi, const dou
     run_batches(M, c->maxIter, [&](int32_t seq)
     {
         kt_mark(M, FH_KERNEL_PRECONDITION);
-        const int32_t nbw = apply_precond<float>(M, precon, r32, w32, part, st, false);
+        precond_finalize<float>(M, precon, r32, w32, FIN_PCG_WARA);
         kt_mark(M, FH_KERNEL_PRECONDITION);
-        launch_finalize_blocks(nbw, N, part, FIN_PCG_WARA, st, 0, M->stream);
         if (fuseP)
         {
             p32 = pBuf[(seq + 1) & 1];
+            const FusedFinalize fa = fused_fin(M, FIN_PCG_WAPA);
             kt_mark(M, FH_KERNEL_SPMV);
             const int32_t nb = launch_box_pcg_spmv<float>(mesh->boxNx, mesh->boxNy, mesh->boxNz,
                                                        diag, dir, M->symmetric, w32,
                                                        pBuf[seq & 1], p32, ap, part, st,
-                                                       M->stream);
+                                                       M->stream, fa);
             kt_mark(M, FH_KERNEL_SPMV);
-            launch_finalize_blocks(nb, N, part, FIN_PCG_WAPA, st, 0, M->stream);
+            if (!fa.counter)
+                launch_finalize_blocks(nb, N, part, FIN_PCG_WAPA, st, 0, M->stream);
         }
         else if (box)
         {
@@ -934,13 +984,15 @@ static void mixed_pcg(fh_matrix* M, const fh_controls* c, double* psi, const dou
                                    M->stream);
             launch_finalize(N, part, FIN_PCG_WAPA, st, 0, M->stream);
         }
+        const FusedFinalize fu = fused_fin(M, FIN_MIXED_RES);
         kt_mark(M, FH_KERNEL_AXPY_NORM);
-        const int32_t nbu = launch_mixed_update(N, psi, r32, p32, ap, part, st, M->stream);
+        const int32_t nbu = launch_mixed_update(N, psi, r32, p32, ap, part, st, M->stream, fu);
         kt_mark(M, FH_KERNEL_AXPY_NORM);
-        launch_finalize_blocks(nbu, N, part, FIN_MIXED_RES, st, 0, M->stream);
+        if (!fu.counter) launch_finalize_blocks(nbu, N, part, FIN_MIXED_RES, st, 0, M->stream);
+        const FusedFinalize ft = fused_fin(M, FIN_MIXED_TRUE);
         const int32_t nbr = launch_mixed_replace(M->rows(), M->d_diag.p, M->d_val.p, psi, src,
-                                                 r32, part, st, M->stream);
-        launch_finalize_blocks(nbr, N, part, FIN_MIXED_TRUE, st, 0, M->stream);
+                                                 r32, part, st, M->stream, ft);
+        if (!ft.counter) launch_finalize_blocks(nbr, N, part, FIN_MIXED_TRUE, st, 0, M->stream);
     });
     const DevState* h = M->h_state;
     std::memset(perf, 0, sizeof(*perf));

@@ -26,6 +26,20 @@ enum FinalizeOp
     FIN_MIXED_TRUE = 15
 };
 
+// A reduction kernel that finalizes itself: the last of its blocks sums the
+// partials and applies op to st.  counter: fin_counters(nb) zeroed device
+// counters that the kernel leaves zeroed, and the partials buffer holds
+// fin_partials(nb) values; null counter: the kernel only writes its
+// partials and a separate finalize follows.
+inline size_t fin_counters(size_t nb) { return 1 + (nb + 63)/64; }
+inline size_t fin_partials(size_t nb) { return 2*nb + 2*((nb + 63)/64); }
+struct FusedFinalize
+{
+    unsigned int* counter = nullptr;
+    DevState* st = nullptr;
+    int32_t op = 0;
+    int32_t n = 0;
+};
 void launch_finalize(int32_t n, const double* partials, int32_t op, DevState* st,
                      int32_t seq, hipStream_t s);
 template<class Ta, class Tb>
@@ -62,13 +76,15 @@ template<class T>
 int32_t launch_box_pcg_spmv(int32_t nx, int32_t ny, int32_t nz, const T* diag,
                             const T* const dir[6], bool symmetric, const T* w,
                             const T* pOld, T* pNew, T* y, double* partials,
-                            const DevState* st, hipStream_t s);
+                            const DevState* st, hipStream_t s,
+                            const FusedFinalize& F = FusedFinalize());
 // finalize over an explicit number of partial blocks
 void launch_finalize_blocks(int32_t nBlocks, int32_t n, const double* partials, int32_t op,
                             DevState* st, int32_t seq, hipStream_t s);
 template<class T>
 void launch_pcg_update(int32_t n, T* psi, T* r, const T* p, const T* w, double* partials,
-                       const DevState* st, hipStream_t s);
+                       const DevState* st, hipStream_t s,
+                       const FusedFinalize& F = FusedFinalize());
 template<class T>
 void launch_bicg_p(int32_t n, const T* r, T* p, const T* AyA, const DevState* st,
                    hipStream_t s);
@@ -89,12 +105,14 @@ void launch_pcg_update_nostate(int32_t n, T* psi, T* r, const T* p, const T* w,
 // mixed PCG: x64 += alpha*p32, r32 -= alpha*w32, sum |r32|; returns the
 // partial block count
 int32_t launch_mixed_update(int32_t n, double* x, float* r, const float* p, const float* w,
-                            double* partials, const DevState* st, hipStream_t s);
+                            double* partials, const DevState* st, hipStream_t s,
+                            const FusedFinalize& F = FusedFinalize());
 // mixed PCG residual replacement (only when st->replace): r = b - A x in
 // FP64, r32 = r, sum |r|; returns the partial count
 int32_t launch_mixed_replace(const RowsView& R, const double* diag, const double* val,
                              const double* x, const double* b, float* r32, double* partials,
-                             const DevState* st, hipStream_t s);
+                             const DevState* st, hipStream_t s,
+                             const FusedFinalize& F = FusedFinalize());
 void launch_mixed_setup(DevState* st, double replaceRatio, hipStream_t s);
 // psi += w
 void launch_axpy_inplace(int32_t n, double* psi, const double* w, const DevState* st,

@@ -5,6 +5,7 @@
 // per iteration.
 #include "fh_kernels.h"
 #include "fh_solver_kernels.h"
+#include "fh_finalize_dev.h"
 
 #include <algorithm>
 #include <cstdlib>
@@ -17,6 +18,8 @@ namespace fh
 {
 
 #define FH_BLOCK 256
+// grid of the strided reduction kernels (few partials to sum)
+#define MIX_BLOCKS 1024
 
 __device__ __forceinline__ int32_t sell_base2(const int32_t* sliceStart, int32_t r)
 {
@@ -59,6 +62,15 @@ __device__ __forceinline__ void write_partials(double a, double b, double* parti
         partials[blockIdx.x] = a;
         partials[nBlocks + blockIdx.x] = b;
     }
+}
+
+// block sum of (a, b) over an FH_BLOCK block, then partials_arrive
+__device__ __forceinline__ void write_partials_fin(double a, double b, double* partials,
+                                                   const FusedFinalize& F)
+{
+    __shared__ double smem[2*FH_BLOCK/64];
+    block_sum2(a, b, smem);
+    partials_arrive(a, b, partials, gridDim.x, blockIdx.x, F);
 }
 
 // ---- reductions over vectors ----------------------------------------------
@@ -522,7 +534,7 @@ void k_box_spmv_slab(int32_t nx, int32_t ny, int32_t nz, int32_t kc,
                      const T* __restrict__ uJ, const T* __restrict__ uK,
                      const T* __restrict__ pOld, T* __restrict__ y,
                      double* partials, const DevState* st,
-                     const T* __restrict__ wv, T* __restrict__ pNew)
+                     const T* __restrict__ wv, T* __restrict__ pNew, FusedFinalize F)
 {
     if (st->done) return;
     typedef Quad<T> Q;
@@ -690,13 +702,10 @@ void k_box_spmv_slab(int32_t nx, int32_t ny, int32_t nz, int32_t kc,
     for (int off = 32; off > 0; off >>= 1) s0 += __shfl_xor(s0, off, 64);
     if (q == 0) red[r] = s0;
     __syncthreads();
+    double t = 0.0;
     if (threadIdx.x == 0)
-    {
-        double t = 0.0;
         for (int w = 0; w < ROWS; w++) t += red[w];
-        partials[blockIdx.x] = t;
-        partials[gridDim.x + blockIdx.x] = 0.0;
-    }
+    partials_arrive(t, 0.0, partials, gridDim.x, blockIdx.x, F);
 }
 
 // planes per marching run: about 1024 runs over the box, 2..32 planes each
@@ -717,7 +726,7 @@ template<class T>
 int32_t launch_box_pcg_spmv(int32_t nx, int32_t ny, int32_t nz, const T* diag,
                             const T* const dir[6], bool symmetric, const T* w,
                             const T* pOld, T* pNew, T* y, double* partials,
-                            const DevState* st, hipStream_t s)
+                            const DevState* st, hipStream_t s, const FusedFinalize& F)
 {
     if (nx & 3) return -1;
     static const bool plane = [] {
@@ -749,7 +758,7 @@ int32_t launch_box_pcg_spmv(int32_t nx, int32_t ny, int32_t nz, const T* diag,
         hipLaunchKernelGGL((k_box_spmv_slab<T, ROWS>), dim3(nb), dim3(64*ROWS), 0, s, nx, ny, nz,
                            kc, diag, dir[0], dir[1], dir[2], symmetric ? nullptr : dir[3],
                            symmetric ? nullptr : dir[4], symmetric ? nullptr : dir[5], pOld, y,
-                           partials, st, w, pNew);
+                           partials, st, w, pNew, F);
         return nb;
     }
     const int32_t kc = box_march_planes(nx, ny, nz);
@@ -765,11 +774,13 @@ int32_t launch_box_pcg_spmv(int32_t nx, int32_t ny, int32_t nz, const T* diag,
 template int32_t launch_box_pcg_spmv<float>(int32_t, int32_t, int32_t, const float*,
                                             const float* const[6], bool, const float*,
                                             const float*, float*, float*, double*,
-                                            const DevState*, hipStream_t);
+                                            const DevState*, hipStream_t,
+                                            const FusedFinalize&);
 template int32_t launch_box_pcg_spmv<double>(int32_t, int32_t, int32_t, const double*,
                                              const double* const[6], bool, const double*,
                                              const double*, double*, double*, double*,
-                                             const DevState*, hipStream_t);
+                                             const DevState*, hipStream_t,
+                                             const FusedFinalize&);
 
 int32_t launch_box_spmv_dot_f32(int32_t nx, int32_t ny, int32_t nz, const float* diag,
                                 const float* const dir[6], bool symmetric, const float* x,
@@ -797,20 +808,21 @@ int32_t launch_box_spmv_dot_f32(int32_t nx, int32_t ny, int32_t nz, const float*
 template<class T>
 __global__ void k_pcg_update(int32_t n, T* __restrict__ psi, T* __restrict__ r,
                              const T* __restrict__ p, const T* __restrict__ w,
-                             double* partials, const DevState* st)
+                             double* partials, const DevState* st, FusedFinalize F)
 {
     if (st->done) return;
-    const int32_t i = blockIdx.x*FH_BLOCK + threadIdx.x;
+    const T alpha = (T)st->alpha;
     double s0 = 0.0;
-    if (i < n)
+    // one value per thread on a full grid; strided on the short grid of
+    // the self-finalizing launch
+    for (int32_t i = blockIdx.x*FH_BLOCK + threadIdx.x; i < n; i += gridDim.x*FH_BLOCK)
     {
-        const T alpha = (T)st->alpha;
         psi[i] += alpha*p[i];
         const T ri = r[i] - alpha*w[i];
         r[i] = ri;
-        s0 = fabs((double)ri);
+        s0 += fabs((double)ri);
     }
-    write_partials(s0, 0.0, partials, gridDim.x);
+    write_partials_fin(s0, 0.0, partials, F);
 }
 
 // ---- PBiCGStab (PBiCGStab.C:149-248) --------------------------------------
@@ -886,21 +898,6 @@ __global__ void k_bicg_update(int32_t n, T* __restrict__ psi, const T* __restric
 
 // ---- finalize: fixed-order sum of the block partials, then scalar logic ---
 
-__device__ bool dev_converged(DevState* st)
-{
-    // SolverPerformance.C:59-92
-    const bool c = st->finalResidual < st->tolerance
-                || (st->relTol > 1e-20 && st->finalResidual < st->relTol*st->initialResidual);
-    st->converged = c ? 1 : 0;
-    return c;
-}
-
-__device__ bool dev_singular(DevState* st, double v)
-{
-    st->singular = (v < DBL_MIN) ? 1 : 0;   // SolverPerformance.C:31-44
-    return st->singular;
-}
-
 // The partials are summed by 1024 threads, each with four independent
 // accumulators over a fixed stride (loads in flight instead of one
 // dependent chain), then across the block in a fixed order.
@@ -955,167 +952,15 @@ void k_finalize(int32_t nBlocks, const double* partials, int32_t op,
         b = sb;
     }
     if (threadIdx.x != 0) return;
-    switch (op)
-    {
-        case FIN_SUM:
-            st->sum0 = a;
-            st->sum1 = b;
-            break;
-        case FIN_XREF:     // gAverage(psi) (FieldReductionFunctions.C:334-357)
-            st->sum0 = a/n;
-            break;
-        case FIN_INIT:     // normFactor, initial residual, first convergence test
-        {
-            st->normFactor = a + 1e-20;
-            st->initialResidual = b/st->normFactor;
-            st->finalResidual = st->initialResidual;
-            st->nIterations = 0;
-            st->singular = 0;
-            st->converged = 0;
-            st->earlySeq = -1;
-            if (!isfinite(st->initialResidual)) { st->numericError = 1; st->done = 1; break; }
-            if (st->minIter > 0) st->done = 0;
-            else st->done = dev_converged(st) ? 1 : 0;
-            break;
-        }
-        case FIN_PCG_WARA:
-            st->wArAold = st->wArA;
-            st->wArA = a;
-            if (st->nIterations > 0) st->beta = st->wArA/st->wArAold;
-            break;
-        case FIN_PCG_WAPA:
-            st->wApA = a;
-            if (dev_singular(st, fabs(a)/st->normFactor)) { st->done = 1; break; }
-            st->alpha = st->wArA/st->wApA;
-            break;
-        case FIN_PCG_RES:
-        {
-            st->finalResidual = a/st->normFactor;
-            if (!isfinite(st->finalResidual)) { st->numericError = 1; st->done = 1; break; }
-            st->nIterations++;
-            bool cont = false;
-            if (st->nIterations < st->maxIter) cont = !dev_converged(st);
-            cont = cont || st->nIterations < st->minIter;
-            if (!cont) st->done = 1;
-            break;
-        }
-        case FIN_BICG_START:
-        {
-            // first iteration: rA0rA = rA0 & rA
-            st->rA0rAold = st->rA0rA;
-            st->rA0rA = a;
-            if (dev_singular(st, fabs(st->rA0rA))) { st->done = 1; break; }
-            break;
-        }
-        case FIN_BICG_ALPHA:
-            st->rA0AyA = a;
-            st->bicgAlpha = st->rA0rA/st->rA0AyA;
-            break;
-        case FIN_BICG_S:
-        {
-            st->finalResidual = a/st->normFactor;
-            if (!isfinite(st->finalResidual)) { st->numericError = 1; st->done = 1; break; }
-            st->nIterations++;
-            if (st->nIterations >= st->minIter && dev_converged(st))
-            {
-                st->earlySeq = seq;
-                st->done = 1;
-            }
-            break;
-        }
-        case FIN_BICG_OMEGA:
-            st->tAtA = a;
-            st->tAsA = b;
-            st->omega = st->tAsA/st->tAtA;
-            break;
-        case FIN_BICG_RES:
-        {
-            st->finalResidual = a/st->normFactor;
-            if (!isfinite(st->finalResidual)) { st->numericError = 1; st->done = 1; break; }
-            bool cont = false;
-            if (st->nIterations < st->maxIter) cont = !dev_converged(st);
-            cont = cont || st->nIterations < st->minIter;
-            if (!cont) { st->done = 1; break; }
-            // start of the next iteration (PBiCGStab.C:149-185)
-            st->rA0rAold = st->rA0rA;
-            st->rA0rA = b;
-            if (dev_singular(st, fabs(st->rA0rA))) { st->done = 1; break; }
-            if (dev_singular(st, fabs(st->omega))) { st->done = 1; break; }
-            st->bicgBeta = (st->rA0rA/st->rA0rAold)*(st->bicgAlpha/st->omega);
-            break;
-        }
-        case FIN_INNER_INIT:
-        {
-            // inner FP32 solve of the mixed path: relative reduction of
-            // sum |r| only (tolerance 0), normFactor 1
-            st->normFactor = 1.0;
-            st->initialResidual = b;
-            st->finalResidual = b;
-            st->nIterations = 0;
-            st->singular = 0;
-            st->converged = 0;
-            st->earlySeq = -1;
-            st->done = (b > 0.0 && isfinite(b)) ? 0 : 1;
-            break;
-        }
-        case FIN_OUTER_RES:
-        {
-            st->finalResidual = b/st->normFactor;
-            if (!isfinite(st->finalResidual)) { st->numericError = 1; st->done = 1; break; }
-            st->done = dev_converged(st) ? 1 : 0;
-            break;
-        }
-        case FIN_MIXED_RES:
-        {
-            // FP32 recurrence residual; schedule an FP64 replacement when it
-            // fell by replaceRatio since the last one, when it looks
-            // converged, or at the iteration limit
-            const double est = a/st->normFactor;
-            if (!isfinite(est)) { st->numericError = 1; st->done = 1; break; }
-            st->nIterations++;
-            bool looks = est < st->tolerance
-                      || (st->relTol > 1e-20 && est < st->relTol*st->initialResidual);
-            st->replace = (looks || est < st->replaceRatio*st->resLast
-                           || st->nIterations >= st->maxIter) ? 1 : 0;
-            break;
-        }
-        case FIN_MIXED_TRUE:
-        {
-            if (!st->replace) break;
-            st->replace = 0;
-            st->nReplace++;
-            st->finalResidual = b/st->normFactor;
-            if (!isfinite(st->finalResidual)) { st->numericError = 1; st->done = 1; break; }
-            st->resLast = st->finalResidual;
-            bool cont = false;
-            if (st->nIterations < st->maxIter) cont = !dev_converged(st);
-            cont = cont || st->nIterations < st->minIter;
-            if (!cont) st->done = 1;
-            break;
-        }
-        case FIN_SMOOTH_RES:
-        {
-            // smoothSolver.C:156-189: nIterations += nSweeps before the test
-            st->finalResidual = b/st->normFactor;
-            if (!isfinite(st->finalResidual)) { st->numericError = 1; st->done = 1; break; }
-            st->nIterations += seq;
-            bool cont = false;
-            if (st->nIterations < st->maxIter) cont = !dev_converged(st);
-            cont = cont || st->nIterations < st->minIter;
-            if (!cont) st->done = 1;
-            break;
-        }
-        default:
-            break;
-    }
+    finalize_apply(a, b, op, st, n, seq);
 }
+
 
 // a fixed grid of at most MIX_BLOCKS blocks strides over the vector, so that
 // the finalize sums few partials (the order stays fixed)
-#define MIX_BLOCKS 1024
 __global__ void k_mixed_update(int32_t n, double* __restrict__ x, float* __restrict__ r,
                                const float* __restrict__ p, const float* __restrict__ w,
-                               double* partials, const DevState* st)
+                               double* partials, const DevState* st, FusedFinalize F)
 {
     if (st->done) return;
     const double alpha = st->alpha;
@@ -1128,13 +973,13 @@ __global__ void k_mixed_update(int32_t n, double* __restrict__ x, float* __restr
         r[i] = ri;
         s0 += fabs((double)ri);
     }
-    write_partials(s0, 0.0, partials, gridDim.x);
+    write_partials_fin(s0, 0.0, partials, F);
 }
 
 __global__ void k_mixed_replace(RowsView R, const double* __restrict__ diag,
                                 const double* __restrict__ val, const double* __restrict__ x,
                                 const double* __restrict__ b, float* __restrict__ r32,
-                                double* partials, const DevState* st)
+                                double* partials, const DevState* st, FusedFinalize F)
 {
     // launched every iteration, works only on replacement iterations: a
     // strided grid of at most MIX_BLOCKS blocks keeps the idle launch short
@@ -1153,7 +998,7 @@ __global__ void k_mixed_replace(RowsView R, const double* __restrict__ diag,
         r32[r] = (float)acc;
         s1 += fabs(acc);
     }
-    write_partials(0.0, s1, partials, gridDim.x);
+    write_partials_fin(0.0, s1, partials, F);
 }
 
 __global__ void k_mixed_setup(DevState* st, double ratio)
@@ -1170,21 +1015,22 @@ void launch_mixed_setup(DevState* st, double replaceRatio, hipStream_t s)
 }
 
 int32_t launch_mixed_update(int32_t n, double* x, float* r, const float* p, const float* w,
-                            double* partials, const DevState* st, hipStream_t s)
+                            double* partials, const DevState* st, hipStream_t s,
+                            const FusedFinalize& F)
 {
     const int32_t nb = std::min(reduction_blocks(n), MIX_BLOCKS);
     hipLaunchKernelGGL(k_mixed_update, dim3(nb), dim3(FH_BLOCK), 0, s,
-                       n, x, r, p, w, partials, st);
+                       n, x, r, p, w, partials, st, F);
     return nb;
 }
 
 int32_t launch_mixed_replace(const RowsView& R, const double* diag, const double* val,
                              const double* x, const double* b, float* r32, double* partials,
-                             const DevState* st, hipStream_t s)
+                             const DevState* st, hipStream_t s, const FusedFinalize& F)
 {
     const int32_t nb = std::min(reduction_blocks(R.n), MIX_BLOCKS);
     hipLaunchKernelGGL(k_mixed_replace, dim3(nb), dim3(FH_BLOCK), 0, s,
-                       R, diag, val, x, b, r32, partials, st);
+                       R, diag, val, x, b, r32, partials, st, F);
     return nb;
 }
 
@@ -1377,10 +1223,11 @@ void launch_spmv_dot(const RowsView& R, const T* diag, const T* val, const T* x,
 
 template<class T>
 void launch_pcg_update(int32_t n, T* psi, T* r, const T* p, const T* w, double* partials,
-                       const DevState* st, hipStream_t s)
+                       const DevState* st, hipStream_t s, const FusedFinalize& F)
 {
-    hipLaunchKernelGGL(k_pcg_update<T>, grid_for(n), dim3(FH_BLOCK), 0, s,
-                       n, psi, r, p, w, partials, st);
+    const dim3 grid = F.counter ? dim3(std::min(reduction_blocks(n), MIX_BLOCKS)) : grid_for(n);
+    hipLaunchKernelGGL(k_pcg_update<T>, grid, dim3(FH_BLOCK), 0, s,
+                       n, psi, r, p, w, partials, st, F);
 }
 
 // the same update without a solver state (measurement)
@@ -1449,7 +1296,7 @@ void launch_bicg_update(int32_t n, T* psi, const T* y, const T* z, T* r, const T
     template void launch_precond_point<T>(int32_t, int32_t, const T*, const T*, T*, double*, const DevState*, hipStream_t); \
     template void launch_pcg_p<T>(int32_t, const T*, T*, const DevState*, hipStream_t); \
     template void launch_spmv_dot<T>(const RowsView&, const T*, const T*, const T*, T*, const T*, const T*, double*, const DevState*, hipStream_t); \
-    template void launch_pcg_update<T>(int32_t, T*, T*, const T*, const T*, double*, const DevState*, hipStream_t); \
+    template void launch_pcg_update<T>(int32_t, T*, T*, const T*, const T*, double*, const DevState*, hipStream_t, const FusedFinalize&); \
     template void launch_bicg_p<T>(int32_t, const T*, T*, const T*, const DevState*, hipStream_t); \
     template void launch_bicg_s<T>(int32_t, const T*, const T*, T*, double*, const DevState*, hipStream_t); \
     template void launch_bicg_early<T>(int32_t, T*, const T*, const DevState*, int32_t, hipStream_t); \

@@ -5,8 +5,11 @@
 // cell (no row indirection) and every lane reads and writes whole i-rows of
 // 16 contiguous cells.
 #include "fh_tile_sweep.h"
+#include "fh_finalize_dev.h"
 
 #include <hip/hip_fp16.h>
+
+#include <cstdlib>
 
 
 namespace fh
@@ -329,10 +332,19 @@ __device__ __forceinline__ void tl_step(const TlTile& T, int l16, int step, cons
 #pragma unroll
             for (int v = 0; v < 4; v++) R[v] = reinterpret_cast<const float4*>(r + row)[v];
         }
-        const float* nb = bufg[(step + 1) & 1][0];
         const bool hasK = BWD ? k < T.ez - 1 : k > 0;
         const bool hasJ = BWD ? j < T.ey - 1 : j > 0;
-        const int sK = BWD ? k + 1 : k - 1;
+        // the neighbour rows of the previous diagonal, read whole and
+        // unconditionally (a missing neighbour's term is selected away:
+        // v - 0 == v, the same value as skipping it)
+        const int sK = hasK ? (BWD ? k + 1 : k - 1) : k;
+        float4 NK[4], NJ[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+        {
+            NK[u] = reinterpret_cast<const float4*>(bufg[(step + 1) & 1][sK])[u];
+            NJ[u] = reinterpret_cast<const float4*>(bufg[(step + 1) & 1][k])[u];
+        }
         float out[16];
         float prev = 0.f;
 #pragma unroll
@@ -341,8 +353,10 @@ __device__ __forceinline__ void tl_step(const TlTile& T, int l16, int step, cons
             const int i = BWD ? 15 - q : q;
             const float Di = q4(c.D, i);
             float v = BWD ? q4(c.A, i) : Di*q4(c.A, i);
-            if (hasK) v -= Di*c.K.at(i)*nb[sK*16 + i];
-            if (hasJ) v -= Di*c.J.at(i)*nb[k*16 + i];
+            const float tK = Di*c.K.at(i)*q4(NK, i);
+            const float tJ = Di*c.J.at(i)*q4(NJ, i);
+            v -= hasK ? tK : 0.f;
+            v -= hasJ ? tJ : 0.f;
             // SHIFT: the upper i coefficient of cell i is the lower one of
             // cell i+1 (cell 15 has no i+1 inside the tile)
             if (BWD ? i < 15 : i > 0) v -= Di*c.I.at(SHIFT ? min(i + 1, 15) : i)*prev;
@@ -367,7 +381,7 @@ __device__ __forceinline__ void tl_step(const TlTile& T, int l16, int step, cons
     __syncthreads();
 }
 
-template<bool BWD, int BZ, bool SHIFT, class CT>
+template<bool BWD, int BZ, bool SHIFT, int NSET, class CT>
 __device__ __forceinline__ void tl_walk(const TlTile& T, int l16, int nD, const float* a,
                                         const float* rD, const TlCoef<CT>& C,
                                         float (*bufg)[BZ][16], float* w, const float* r,
@@ -375,27 +389,47 @@ __device__ __forceinline__ void tl_walk(const TlTile& T, int l16, int nD, const 
 {
     TlRow<CT> r0, r1;
     tl_load<BWD, SHIFT>(T, l16, 0, a, rD, C, r0);
-    for (int step = 0; step < nD; step += 2)
+    if constexpr (NSET == 2)
     {
-        tl_load<BWD, SHIFT>(T, l16, step + 1, a, rD, C, r1);
-        tl_step<BWD, BZ, SHIFT>(T, l16, step, r0, bufg, w, r, wantDot, dotWR);
-        if (step + 1 >= nD) break;
-        tl_load<BWD, SHIFT>(T, l16, step + 2, a, rD, C, r0);
-        tl_step<BWD, BZ, SHIFT>(T, l16, step + 1, r1, bufg, w, r, wantDot, dotWR);
+        for (int step = 0; step < nD; step += 2)
+        {
+            tl_load<BWD, SHIFT>(T, l16, step + 1, a, rD, C, r1);
+            tl_step<BWD, BZ, SHIFT>(T, l16, step, r0, bufg, w, r, wantDot, dotWR);
+            if (step + 1 >= nD) break;
+            tl_load<BWD, SHIFT>(T, l16, step + 2, a, rD, C, r0);
+            tl_step<BWD, BZ, SHIFT>(T, l16, step + 1, r1, bufg, w, r, wantDot, dotWR);
+        }
+    }
+    else
+    {
+        // three sets: the rows of the next two diagonals in flight
+        TlRow<CT> r2;
+        tl_load<BWD, SHIFT>(T, l16, 1, a, rD, C, r1);
+        for (int step = 0; step < nD; step += 3)
+        {
+            tl_load<BWD, SHIFT>(T, l16, step + 2, a, rD, C, r2);
+            tl_step<BWD, BZ, SHIFT>(T, l16, step, r0, bufg, w, r, wantDot, dotWR);
+            if (step + 1 >= nD) break;
+            tl_load<BWD, SHIFT>(T, l16, step + 3, a, rD, C, r0);
+            tl_step<BWD, BZ, SHIFT>(T, l16, step + 1, r1, bufg, w, r, wantDot, dotWR);
+            if (step + 2 >= nD) break;
+            tl_load<BWD, SHIFT>(T, l16, step + 4, a, rD, C, r1);
+            tl_step<BWD, BZ, SHIFT>(T, l16, step + 2, r2, bufg, w, r, wantDot, dotWR);
+        }
     }
 }
 
 // PASSES: 1 forward, 2 backward, 3 forward then backward; TPW tiles per
 // wavefront (16 lanes each; the other lanes idle); SYM: the backward pass
 // reads the lower coefficient arrays (shifted)
-template<int PASSES, int BZ, int TPW, bool SYM, class CT>
+template<int PASSES, int BZ, int TPW, bool SYM, class CT, int NSET = 2>
 __global__ __launch_bounds__(64)
 void k_tile_local_vec(TileView G, const float* x, const float* __restrict__ rD,
                       TlCoef<CT> lo, TlCoef<CT> up, float* w, const DevState* st,
-                      const float* __restrict__ r, double* partials)
+                      const float* __restrict__ r, double* partials, FusedFinalize F)
 {
     if (st && st->done) return;
-    __shared__ float buf[TPW][2][BZ][16];
+    __shared__ __align__(16) float buf[TPW][2][BZ][16];
     const int lane = threadIdx.x;
     const int g = min(lane >> 4, TPW - 1), l16 = lane & 15;
     const int nTiles = G.nTI*G.nTJ*G.nTK;
@@ -418,26 +452,22 @@ void k_tile_local_vec(TileView G, const float* x, const float* __restrict__ rD,
     nD = max(nD, __shfl_xor(nD, 32, 64));
     nD = __builtin_amdgcn_readfirstlane(nD);
     float (*bufg)[BZ][16] = buf[g];
-    if (PASSES & 1) tl_walk<false, BZ, false>(T, l16, nD, x, rD, lo, bufg, w, r, false, dotWR);
+    if (PASSES & 1) tl_walk<false, BZ, false, NSET>(T, l16, nD, x, rD, lo, bufg, w, r, false, dotWR);
     // (the walk ends on a workgroup barrier: the forward rows are visible)
-    if (PASSES & 2) tl_walk<true, BZ, SYM>(T, l16, nD, w, rD, up, bufg, w, r, wantDot, dotWR);
+    if (PASSES & 2) tl_walk<true, BZ, SYM, NSET>(T, l16, nD, w, rD, up, bufg, w, r, wantDot, dotWR);
     if (wantDot)
     {
         // one partial per wavefront (= workgroup), fixed butterfly order
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) dotWR += __shfl_xor(dotWR, off, 64);
-        if (lane == 0)
-        {
-            partials[blockIdx.x] = dotWR;
-            partials[gridDim.x + blockIdx.x] = 0.0;
-        }
+        partials_arrive(dotWR, 0.0, partials, gridDim.x, blockIdx.x, F);
     }
 }
 
 int32_t launch_tile_local_apply_f32(const TileView& G, const float* x, const float* rD,
                                     const float* const dir[6], bool symmetric, float* w,
                                     const DevState* st, hipStream_t s, double* partials,
-                                    const __half* const half3[3])
+                                    const __half* const half3[3], const FusedFinalize& F)
 {
     if (G.bx != 16 || G.by != 16 || G.bz != TileBZ<float>::pre || (G.nx & 15))
         return -1;
@@ -445,9 +475,31 @@ int32_t launch_tile_local_apply_f32(const TileView& G, const float* x, const flo
     const dim3 grid((nTiles + 3)/4);
     if (symmetric && half3)
     {
+        // FOAMHIP_TL_SHAPE: tiles per wavefront and register sets in flight
+        // (42: 4 tiles, 2 sets; 43, 22, 23)
+        static const int shape = [] {
+            const char* e = std::getenv("FOAMHIP_TL_SHAPE");
+            return e ? std::atoi(e) : 42;
+        }();
         const TlCoef<__half> h{half3[0], half3[1], half3[2]};
-        hipLaunchKernelGGL((k_tile_local_vec<3, TileBZ<float>::pre, 4, true, __half>), grid,
-                           dim3(64), 0, s, G, x, rD, h, h, w, st, x, partials);
+        constexpr int BZ = TileBZ<float>::pre;
+        if (shape == 43)
+            hipLaunchKernelGGL((k_tile_local_vec<3, BZ, 4, true, __half, 3>), grid,
+                               dim3(64), 0, s, G, x, rD, h, h, w, st, x, partials, F);
+        else if (shape == 22 || shape == 23)
+        {
+            const dim3 g2((nTiles + 1)/2);
+            if (shape == 22)
+                hipLaunchKernelGGL((k_tile_local_vec<3, BZ, 2, true, __half, 2>), g2,
+                                   dim3(64), 0, s, G, x, rD, h, h, w, st, x, partials, F);
+            else
+                hipLaunchKernelGGL((k_tile_local_vec<3, BZ, 2, true, __half, 3>), g2,
+                                   dim3(64), 0, s, G, x, rD, h, h, w, st, x, partials, F);
+            return (int32_t)g2.x;
+        }
+        else
+            hipLaunchKernelGGL((k_tile_local_vec<3, BZ, 4, true, __half>), grid,
+                               dim3(64), 0, s, G, x, rD, h, h, w, st, x, partials, F);
         return (int32_t)grid.x;
     }
     const TlCoef<float> lo{dir[0], dir[1], dir[2]};
@@ -455,10 +507,10 @@ int32_t launch_tile_local_apply_f32(const TileView& G, const float* x, const flo
                                        : TlCoef<float>{dir[5], dir[4], dir[3]};
     if (symmetric)
         hipLaunchKernelGGL((k_tile_local_vec<3, TileBZ<float>::pre, 4, true, float>), grid,
-                           dim3(64), 0, s, G, x, rD, lo, up, w, st, x, partials);
+                           dim3(64), 0, s, G, x, rD, lo, up, w, st, x, partials, F);
     else
         hipLaunchKernelGGL((k_tile_local_vec<3, TileBZ<float>::pre, 4, false, float>), grid,
-                           dim3(64), 0, s, G, x, rD, lo, up, w, st, x, partials);
+                           dim3(64), 0, s, G, x, rD, lo, up, w, st, x, partials, F);
     return (int32_t)grid.x;
 }
 
@@ -507,10 +559,12 @@ int32_t launch_tile_local_f32(bool backward, const TileView& G, const float* a, 
         const TlCoef<float> c{cK, cJ, cI};
         if (backward)
             hipLaunchKernelGGL((k_tile_local_vec<2, TileBZ<float>::pre, 4, false, float>), grid,
-                               dim3(64), 0, s, G, a, rD, c, c, w, st, r, partials);
+                               dim3(64), 0, s, G, a, rD, c, c, w, st, r, partials,
+                               FusedFinalize());
         else
             hipLaunchKernelGGL((k_tile_local_vec<1, TileBZ<float>::pre, 4, false, float>), grid,
-                               dim3(64), 0, s, G, a, rD, c, c, w, st, nullptr, nullptr);
+                               dim3(64), 0, s, G, a, rD, c, c, w, st, nullptr, nullptr,
+                               FusedFinalize());
         return (int32_t)grid.x;
     }
     if (backward)

@@ -2,6 +2,7 @@
 #pragma once
 
 #include "fh_kernels.h"
+#include "fh_solver_kernels.h"
 
 #include <hip/hip_fp16.h>
 
@@ -48,7 +49,8 @@ void launch_tile_sweep(int kind, const TileView& G, const RowsView& R,
 // one partial per tile.  Returns the tile count.
 template<class T>
 int32_t launch_sweep_wave(bool backward, const TileView& G, const SweepData<T>& d,
-                          TileSync& sync, DevState* st, hipStream_t s);
+                          TileSync& sync, DevState* st, hipStream_t s,
+                          const FusedFinalize& F = FusedFinalize());
 
 // direction-packed coefficients of a box: dst[0..5] = lower K, J, I and
 // upper I, J, K per cell (nDst 6), or the lower three only (nDst 3)
@@ -67,7 +69,8 @@ void launch_pack_dir_coefs(const TileView& G, const RowsView& R, const T* val, i
 int32_t launch_tile_local_apply_f32(const TileView& G, const float* x, const float* rD,
                                     const float* const dir[6], bool symmetric, float* w,
                                     const DevState* st, hipStream_t s, double* partials,
-                                    const __half* const half3[3] = nullptr);
+                                    const __half* const half3[3] = nullptr,
+                                    const FusedFinalize& F = FusedFinalize());
 // FP16 copies of three FP32 arrays of n values
 void launch_pack_half(int64_t n, const float* const src[3], __half* const dst[3], hipStream_t s);
 // b = a rounded to FP16 precision (kept in FP32)

@@ -19,6 +19,7 @@
 // reads their boundary values with device-coherent loads and publishes its
 // own completion after its stores have completed -- as in fh_tile_sweep.hip.
 #include "fh_tile_sweep.h"
+#include "fh_finalize_dev.h"
 
 namespace fh
 {
@@ -72,7 +73,7 @@ template<class T, bool BWD>
 __global__ __launch_bounds__(64)
 void k_sweep_wave(TileView G, SweepData<T> dat, unsigned int* flags,
                   unsigned long long* ticket, unsigned long long ticketBase,
-                  unsigned long long epoch, DevState* st)
+                  unsigned long long epoch, DevState* st, FusedFinalize F)
 {
     constexpr int BX = WAVE_BX, BY = WAVE_BX, BZ = WAVE_BZ, NC = BX*BY*BZ;
     // staged inputs, premultiplied: a = rD*x (forward) or w (backward),
@@ -299,18 +300,14 @@ void k_sweep_wave(TileView G, SweepData<T> dat, unsigned int* flags,
                 if (col && kk < ez) acc += (double)sA[lane + BX*BY*kk]*(double)rr[kk];
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-            if (lane == 0)
-            {
-                dat.partials[tile] = acc;
-                dat.partials[nTiles + tile] = 0.0;
-            }
+            partials_arrive(acc, 0.0, dat.partials, nTiles, tile, F);
         }
     }
 }
 
 template<class T>
 int32_t launch_sweep_wave(bool backward, const TileView& G, const SweepData<T>& d,
-                          TileSync& sync, DevState* st, hipStream_t s)
+                          TileSync& sync, DevState* st, hipStream_t s, const FusedFinalize& F)
 {
     if (G.bx != WAVE_BX || G.by != WAVE_BX || G.bz != WAVE_BZ)
         fail(FH_ERR_INVALID, "tile shape does not match the one-wavefront sweep kernels");
@@ -321,18 +318,20 @@ int32_t launch_sweep_wave(bool backward, const TileView& G, const SweepData<T>& 
     const int nTiles = G.nTI*G.nTJ*G.nTK;
     if (backward)
         hipLaunchKernelGGL((k_sweep_wave<T, true>), dim3(nTiles), dim3(64), 0, s, G, d,
-                           sync.flags, sync.ticket, sync.ticketBase, sync.epoch, st);
+                           sync.flags, sync.ticket, sync.ticketBase, sync.epoch, st, F);
     else
         hipLaunchKernelGGL((k_sweep_wave<T, false>), dim3(nTiles), dim3(64), 0, s, G, d,
-                           sync.flags, sync.ticket, sync.ticketBase, sync.epoch, st);
+                           sync.flags, sync.ticket, sync.ticketBase, sync.epoch, st, F);
     sync.ticketBase += (unsigned long long)nTiles;
     sync.epoch++;
     return nTiles;
 }
 
 template int32_t launch_sweep_wave<double>(bool, const TileView&, const SweepData<double>&,
-                                           TileSync&, DevState*, hipStream_t);
+                                           TileSync&, DevState*, hipStream_t,
+                                           const FusedFinalize&);
 template int32_t launch_sweep_wave<float>(bool, const TileView&, const SweepData<float>&,
-                                          TileSync&, DevState*, hipStream_t);
+                                          TileSync&, DevState*, hipStream_t,
+                                           const FusedFinalize&);
 
 } // namespace fh
