@@ -191,6 +191,8 @@ def main():
         live[k]["bytes"] = kernels[k]["bytes"]
         live[k]["GBs"] = kernels[k]["bytes"] / (live[k]["ms"] * 1e-3) / 1e9
     traffic = measured_traffic(dom, n)
+    copy_gbs = measured_copy(torch)
+    log(f"device copy of 1 GiB: {copy_gbs:.0f} GB/s (read + write)")
 
     serve = None
     if a.serve_requests > 0:
@@ -237,7 +239,8 @@ def main():
             "mixed_vs_fp64_rel_l2": err,
             "mixed_error_vs_tight_fp64": err_mixed,
             "fp64_error_vs_tight_fp64": err_fp64,
-            "mixed_inner_preconditioner": "tileDIC (16x16x16 block-Jacobi DIC, FP32)",
+            "mixed_inner_preconditioner": "tileDIC (16x16x16 block-Jacobi DIC, FP32 factor, "
+                                          "FP16 coefficients)",
             "assembly_ms": assembly_ms,
             "serve": serve,
             "kernels_live": live,
@@ -252,10 +255,35 @@ def main():
             "unit": "GB/s",
             "frac": kd["GBs"] / HBM_PEAK_GBS,
             "traffic": traffic,
+            "copy_GBs": copy_gbs,
+            "frac_of_copy": kd["GBs"] / copy_gbs,
+            "spmv_GBs": live["spmv"]["GBs"],
+            "spmv_frac": live["spmv"]["GBs"] / HBM_PEAK_GBS,
+            "spmv_frac_of_copy": live["spmv"]["GBs"] / copy_gbs,
         },
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def measured_copy(torch):
+    """Bandwidth of a plain device copy of 1 GiB (bytes read + written per
+    second, best of 10 launches timed with events): what a streaming kernel
+    reaches on this device, beside the published peak."""
+    src = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    src.fill_(1.0)
+    dst.copy_(src)
+    best = float("inf")
+    for _ in range(10):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        dst.copy_(src)
+        e1.record()
+        e1.synchronize()
+        best = min(best, e0.elapsed_time(e1))
+    del src, dst
+    return 2.0 * (1 << 30) / (best * 1e-3) / 1e9
 
 
 TRAFFIC_KEYS = {"spmv": "spmv_f32_box", "precondition": "precondition_f32_tile_local",

@@ -9,8 +9,6 @@
 
 #include <hip/hip_fp16.h>
 
-#include <cstdlib>
-
 
 namespace fh
 {
@@ -381,7 +379,7 @@ __device__ __forceinline__ void tl_step(const TlTile& T, int l16, int step, cons
     __syncthreads();
 }
 
-template<bool BWD, int BZ, bool SHIFT, int NSET, class CT>
+template<bool BWD, int BZ, bool SHIFT, class CT>
 __device__ __forceinline__ void tl_walk(const TlTile& T, int l16, int nD, const float* a,
                                         const float* rD, const TlCoef<CT>& C,
                                         float (*bufg)[BZ][16], float* w, const float* r,
@@ -389,40 +387,20 @@ __device__ __forceinline__ void tl_walk(const TlTile& T, int l16, int nD, const 
 {
     TlRow<CT> r0, r1;
     tl_load<BWD, SHIFT>(T, l16, 0, a, rD, C, r0);
-    if constexpr (NSET == 2)
+    for (int step = 0; step < nD; step += 2)
     {
-        for (int step = 0; step < nD; step += 2)
-        {
-            tl_load<BWD, SHIFT>(T, l16, step + 1, a, rD, C, r1);
-            tl_step<BWD, BZ, SHIFT>(T, l16, step, r0, bufg, w, r, wantDot, dotWR);
-            if (step + 1 >= nD) break;
-            tl_load<BWD, SHIFT>(T, l16, step + 2, a, rD, C, r0);
-            tl_step<BWD, BZ, SHIFT>(T, l16, step + 1, r1, bufg, w, r, wantDot, dotWR);
-        }
-    }
-    else
-    {
-        // three sets: the rows of the next two diagonals in flight
-        TlRow<CT> r2;
-        tl_load<BWD, SHIFT>(T, l16, 1, a, rD, C, r1);
-        for (int step = 0; step < nD; step += 3)
-        {
-            tl_load<BWD, SHIFT>(T, l16, step + 2, a, rD, C, r2);
-            tl_step<BWD, BZ, SHIFT>(T, l16, step, r0, bufg, w, r, wantDot, dotWR);
-            if (step + 1 >= nD) break;
-            tl_load<BWD, SHIFT>(T, l16, step + 3, a, rD, C, r0);
-            tl_step<BWD, BZ, SHIFT>(T, l16, step + 1, r1, bufg, w, r, wantDot, dotWR);
-            if (step + 2 >= nD) break;
-            tl_load<BWD, SHIFT>(T, l16, step + 4, a, rD, C, r1);
-            tl_step<BWD, BZ, SHIFT>(T, l16, step + 2, r2, bufg, w, r, wantDot, dotWR);
-        }
+        tl_load<BWD, SHIFT>(T, l16, step + 1, a, rD, C, r1);
+        tl_step<BWD, BZ, SHIFT>(T, l16, step, r0, bufg, w, r, wantDot, dotWR);
+        if (step + 1 >= nD) break;
+        tl_load<BWD, SHIFT>(T, l16, step + 2, a, rD, C, r0);
+        tl_step<BWD, BZ, SHIFT>(T, l16, step + 1, r1, bufg, w, r, wantDot, dotWR);
     }
 }
 
 // PASSES: 1 forward, 2 backward, 3 forward then backward; TPW tiles per
 // wavefront (16 lanes each; the other lanes idle); SYM: the backward pass
 // reads the lower coefficient arrays (shifted)
-template<int PASSES, int BZ, int TPW, bool SYM, class CT, int NSET = 2>
+template<int PASSES, int BZ, int TPW, bool SYM, class CT>
 __global__ __launch_bounds__(64)
 void k_tile_local_vec(TileView G, const float* x, const float* __restrict__ rD,
                       TlCoef<CT> lo, TlCoef<CT> up, float* w, const DevState* st,
@@ -452,9 +430,9 @@ void k_tile_local_vec(TileView G, const float* x, const float* __restrict__ rD,
     nD = max(nD, __shfl_xor(nD, 32, 64));
     nD = __builtin_amdgcn_readfirstlane(nD);
     float (*bufg)[BZ][16] = buf[g];
-    if (PASSES & 1) tl_walk<false, BZ, false, NSET>(T, l16, nD, x, rD, lo, bufg, w, r, false, dotWR);
+    if (PASSES & 1) tl_walk<false, BZ, false>(T, l16, nD, x, rD, lo, bufg, w, r, false, dotWR);
     // (the walk ends on a workgroup barrier: the forward rows are visible)
-    if (PASSES & 2) tl_walk<true, BZ, SYM, NSET>(T, l16, nD, w, rD, up, bufg, w, r, wantDot, dotWR);
+    if (PASSES & 2) tl_walk<true, BZ, SYM>(T, l16, nD, w, rD, up, bufg, w, r, wantDot, dotWR);
     if (wantDot)
     {
         // one partial per wavefront (= workgroup), fixed butterfly order
@@ -475,31 +453,9 @@ int32_t launch_tile_local_apply_f32(const TileView& G, const float* x, const flo
     const dim3 grid((nTiles + 3)/4);
     if (symmetric && half3)
     {
-        // FOAMHIP_TL_SHAPE: tiles per wavefront and register sets in flight
-        // (42: 4 tiles, 2 sets; 43, 22, 23)
-        static const int shape = [] {
-            const char* e = std::getenv("FOAMHIP_TL_SHAPE");
-            return e ? std::atoi(e) : 42;
-        }();
         const TlCoef<__half> h{half3[0], half3[1], half3[2]};
-        constexpr int BZ = TileBZ<float>::pre;
-        if (shape == 43)
-            hipLaunchKernelGGL((k_tile_local_vec<3, BZ, 4, true, __half, 3>), grid,
-                               dim3(64), 0, s, G, x, rD, h, h, w, st, x, partials, F);
-        else if (shape == 22 || shape == 23)
-        {
-            const dim3 g2((nTiles + 1)/2);
-            if (shape == 22)
-                hipLaunchKernelGGL((k_tile_local_vec<3, BZ, 2, true, __half, 2>), g2,
-                                   dim3(64), 0, s, G, x, rD, h, h, w, st, x, partials, F);
-            else
-                hipLaunchKernelGGL((k_tile_local_vec<3, BZ, 2, true, __half, 3>), g2,
-                                   dim3(64), 0, s, G, x, rD, h, h, w, st, x, partials, F);
-            return (int32_t)g2.x;
-        }
-        else
-            hipLaunchKernelGGL((k_tile_local_vec<3, BZ, 4, true, __half>), grid,
-                               dim3(64), 0, s, G, x, rD, h, h, w, st, x, partials, F);
+        hipLaunchKernelGGL((k_tile_local_vec<3, TileBZ<float>::pre, 4, true, __half>), grid,
+                           dim3(64), 0, s, G, x, rD, h, h, w, st, x, partials, F);
         return (int32_t)grid.x;
     }
     const TlCoef<float> lo{dir[0], dir[1], dir[2]};
