@@ -1415,9 +1415,13 @@ static void profile_t(fh_matrix* M, int kernel, int precon, int reps, double* ms
             break;
         case FH_KERNEL_PRECONDITION:
             if (box && local && w == 4 && (mesh->boxNx & 15) == 0)
+            {
                 // one launch: x, rD, one coefficient per face (each triangle
-                // from the lower arrays when symmetric), w written once
-                *bytes = N*3*w + (M->symmetric ? 1.0 : 2.0)*F*w;
+                // from the lower arrays when symmetric; FP16 for the
+                // symmetric tile-local DIC), w written once
+                const double cw = which == 3 && tile_dic_half(M) ? 2.0 : (double)w;
+                *bytes = N*3*w + (M->symmetric ? 1.0 : 2.0)*F*cw;
+            }
             else if (box && !local)
                 // two passes: x, rD, w, then w, rD, w, r; a coefficient
                 // per face in each

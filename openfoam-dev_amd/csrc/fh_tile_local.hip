@@ -265,6 +265,7 @@ template<class CT>
 struct TlRow
 {
     float4 A[4], D[4];
+    float4 R[4];          // backward pass with the w.r sum: the row of r
     CoefRow<CT> K, J, I;
 };
 
@@ -293,9 +294,10 @@ struct TlCoef
     const CT *K, *J, *I;
 };
 
-template<bool BWD, bool SHIFT, class CT>
+template<bool BWD, bool SHIFT, bool DOT, class CT>
 __device__ __forceinline__ void tl_load(const TlTile& T, int l16, int step, const float* a,
-                                        const float* rD, const TlCoef<CT>& C, TlRow<CT>& o)
+                                        const float* rD, const TlCoef<CT>& C, TlRow<CT>& o,
+                                        const float* r)
 {
     int j, k;
     const bool act = tl_row<BWD>(T, l16, step, j, k);
@@ -312,102 +314,117 @@ __device__ __forceinline__ void tl_load(const TlTile& T, int l16, int step, cons
     o.K.load(C.K + rowK);
     o.J.load(C.J + rowJ);
     o.I.load(C.I + row);
+    if constexpr (BWD && DOT)
+    {
+#pragma unroll
+        for (int v = 0; v < 4; v++) o.R[v] = reinterpret_cast<const float4*>(r + row)[v];
+    }
 }
 
-template<bool BWD, int BZ, bool SHIFT, class CT>
+template<bool BWD, int BZ, bool SHIFT, bool DOT, class CT>
 __device__ __forceinline__ void tl_step(const TlTile& T, int l16, int step, const TlRow<CT>& c,
-                                        float (*bufg)[BZ][16], float* w, const float* r,
-                                        bool wantDot, double& dotWR)
+                                        float (*bufg)[BZ + 1][16], float* w, double& dotWR)
 {
     int j, k;
     const bool act = tl_row<BWD>(T, l16, step, j, k);
+    // every lane computes (an idle lane on its clamped row, results
+    // discarded), so the step has no branch around its loads: the inputs
+    // loaded ahead are waited for on one path only
+    const int kc = act ? k : 0;
+    const bool hasK = act && (BWD ? k < T.ez - 1 : k > 0);
+    const bool hasJ = act && (BWD ? j < T.ey - 1 : j > 0);
+    // the neighbour rows of the previous diagonal, read whole and
+    // unconditionally (a missing neighbour's term is selected away:
+    // v - 0 == v, the same value as skipping it)
+    const int sK = hasK ? (BWD ? k + 1 : k - 1) : kc;
+    float4 NK[4], NJ[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+    {
+        NK[u] = reinterpret_cast<const float4*>(bufg[(step + 1) & 1][sK])[u];
+        NJ[u] = reinterpret_cast<const float4*>(bufg[(step + 1) & 1][kc])[u];
+    }
+    float out[16];
+    float prev = 0.f;
+#pragma unroll
+    for (int q = 0; q < 16; q++)
+    {
+        const int i = BWD ? 15 - q : q;
+        const float Di = q4(c.D, i);
+        float v = BWD ? q4(c.A, i) : Di*q4(c.A, i);
+        const float tK = Di*c.K.at(i)*q4(NK, i);
+        const float tJ = Di*c.J.at(i)*q4(NJ, i);
+        v -= hasK ? tK : 0.f;
+        v -= hasJ ? tJ : 0.f;
+        // SHIFT: the upper i coefficient of cell i is the lower one of
+        // cell i+1 (cell 15 has no i+1 inside the tile)
+        if (BWD ? i < 15 : i > 0) v -= Di*c.I.at(SHIFT ? min(i + 1, 15) : i)*prev;
+        prev = v;
+        out[i] = v;
+    }
+    if constexpr (BWD && DOT)
+    {
+        // four independent partial sums of the row (fixed order)
+        double sr[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int i = 0; i < 16; i++) sr[i & 3] += (double)out[i]*(double)q4(c.R, i);
+        const double t = (sr[0] + sr[1]) + (sr[2] + sr[3]);
+        dotWR += act ? t : 0.0;
+    }
+    // the LDS row is written by every lane (an idle lane into the spare
+    // row BZ), which keeps the computation above out of the branch
+    float* mine = bufg[step & 1][act ? k : BZ];
+#pragma unroll
+    for (int i = 0; i < 16; i++) mine[i] = out[i];
     if (act)
     {
         const int64_t row = T.base + T.sy*j + T.sz*k;
-        float4 R[4];
-        if (BWD && wantDot)
-        {
-#pragma unroll
-            for (int v = 0; v < 4; v++) R[v] = reinterpret_cast<const float4*>(r + row)[v];
-        }
-        const bool hasK = BWD ? k < T.ez - 1 : k > 0;
-        const bool hasJ = BWD ? j < T.ey - 1 : j > 0;
-        // the neighbour rows of the previous diagonal, read whole and
-        // unconditionally (a missing neighbour's term is selected away:
-        // v - 0 == v, the same value as skipping it)
-        const int sK = hasK ? (BWD ? k + 1 : k - 1) : k;
-        float4 NK[4], NJ[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++)
-        {
-            NK[u] = reinterpret_cast<const float4*>(bufg[(step + 1) & 1][sK])[u];
-            NJ[u] = reinterpret_cast<const float4*>(bufg[(step + 1) & 1][k])[u];
-        }
-        float out[16];
-        float prev = 0.f;
-#pragma unroll
-        for (int q = 0; q < 16; q++)
-        {
-            const int i = BWD ? 15 - q : q;
-            const float Di = q4(c.D, i);
-            float v = BWD ? q4(c.A, i) : Di*q4(c.A, i);
-            const float tK = Di*c.K.at(i)*q4(NK, i);
-            const float tJ = Di*c.J.at(i)*q4(NJ, i);
-            v -= hasK ? tK : 0.f;
-            v -= hasJ ? tJ : 0.f;
-            // SHIFT: the upper i coefficient of cell i is the lower one of
-            // cell i+1 (cell 15 has no i+1 inside the tile)
-            if (BWD ? i < 15 : i > 0) v -= Di*c.I.at(SHIFT ? min(i + 1, 15) : i)*prev;
-            prev = v;
-            out[i] = v;
-        }
 #pragma unroll
         for (int v = 0; v < 4; v++)
             reinterpret_cast<float4*>(w + row)[v] =
                 make_float4(out[4*v], out[4*v + 1], out[4*v + 2], out[4*v + 3]);
-        float* mine = bufg[step & 1][k];
-#pragma unroll
-        for (int i = 0; i < 16; i++) mine[i] = out[i];
-        if (BWD && wantDot)
-        {
-            double sr = 0.0;
-#pragma unroll
-            for (int i = 0; i < 16; i++) sr += (double)out[i]*(double)q4(R, i);
-            dotWR += sr;
-        }
     }
     __syncthreads();
 }
 
-template<bool BWD, int BZ, bool SHIFT, class CT>
+template<bool BWD, int BZ, bool SHIFT, bool DOT, class CT>
 __device__ __forceinline__ void tl_walk(const TlTile& T, int l16, int nD, const float* a,
                                         const float* rD, const TlCoef<CT>& C,
-                                        float (*bufg)[BZ][16], float* w, const float* r,
-                                        bool wantDot, double& dotWR)
+                                        float (*bufg)[BZ + 1][16], float* w, const float* r,
+                                        double& dotWR)
 {
     TlRow<CT> r0, r1;
-    tl_load<BWD, SHIFT>(T, l16, 0, a, rD, C, r0);
-    for (int step = 0; step < nD; step += 2)
+    tl_load<BWD, SHIFT, DOT>(T, l16, 0, a, rD, C, r0, r);
+    // the scheduling barriers keep each set's loads issued ahead of the
+    // step that precedes their use (the compiler would otherwise sink them
+    // next to their uses).  Pairs of diagonals in the loop, an odd last one
+    // after it: no exit from the middle of the body, whose path would leave
+    // the prefetched loads outstanding at the loop head.
+    int step = 0;
+    for (; step + 1 < nD; step += 2)
     {
-        tl_load<BWD, SHIFT>(T, l16, step + 1, a, rD, C, r1);
-        tl_step<BWD, BZ, SHIFT>(T, l16, step, r0, bufg, w, r, wantDot, dotWR);
-        if (step + 1 >= nD) break;
-        tl_load<BWD, SHIFT>(T, l16, step + 2, a, rD, C, r0);
-        tl_step<BWD, BZ, SHIFT>(T, l16, step + 1, r1, bufg, w, r, wantDot, dotWR);
+        tl_load<BWD, SHIFT, DOT>(T, l16, step + 1, a, rD, C, r1, r);
+        __builtin_amdgcn_sched_barrier(0);
+        tl_step<BWD, BZ, SHIFT, DOT>(T, l16, step, r0, bufg, w, dotWR);
+        tl_load<BWD, SHIFT, DOT>(T, l16, step + 2, a, rD, C, r0, r);
+        __builtin_amdgcn_sched_barrier(0);
+        tl_step<BWD, BZ, SHIFT, DOT>(T, l16, step + 1, r1, bufg, w, dotWR);
     }
+    if (step < nD) tl_step<BWD, BZ, SHIFT, DOT>(T, l16, step, r0, bufg, w, dotWR);
 }
 
 // PASSES: 1 forward, 2 backward, 3 forward then backward; TPW tiles per
 // wavefront (16 lanes each; the other lanes idle); SYM: the backward pass
-// reads the lower coefficient arrays (shifted)
-template<int PASSES, int BZ, int TPW, bool SYM, class CT>
+// reads the lower coefficient arrays (shifted); DOT: the backward pass sums
+// w*r into one partial per workgroup
+template<int PASSES, int BZ, int TPW, bool SYM, class CT, bool DOT>
 __global__ __launch_bounds__(64)
 void k_tile_local_vec(TileView G, const float* x, const float* __restrict__ rD,
                       TlCoef<CT> lo, TlCoef<CT> up, float* w, const DevState* st,
                       const float* __restrict__ r, double* partials, FusedFinalize F)
 {
     if (st && st->done) return;
-    __shared__ __align__(16) float buf[TPW][2][BZ][16];
+    __shared__ __align__(16) float buf[TPW][2][BZ + 1][16];
     const int lane = threadIdx.x;
     const int g = min(lane >> 4, TPW - 1), l16 = lane & 15;
     const int nTiles = G.nTI*G.nTJ*G.nTK;
@@ -422,24 +439,41 @@ void k_tile_local_vec(TileView G, const float* x, const float* __restrict__ rD,
     T.sy = G.nx;
     T.sz = (int64_t)G.nx*G.ny;
     T.base = TI*16 + T.sy*(TJ*16) + T.sz*(TK*BZ);
-    const bool wantDot = (PASSES & 2) && partials != nullptr;
+    constexpr bool wantDot = (PASSES & 2) && DOT;
     double dotWR = 0.0;
     // the wave walks the longest diagonal count of its tiles
     int nD = T.ok ? T.nDiag : 0;
     nD = max(nD, __shfl_xor(nD, 16, 64));
     nD = max(nD, __shfl_xor(nD, 32, 64));
     nD = __builtin_amdgcn_readfirstlane(nD);
-    float (*bufg)[BZ][16] = buf[g];
-    if (PASSES & 1) tl_walk<false, BZ, false>(T, l16, nD, x, rD, lo, bufg, w, r, false, dotWR);
+    float (*bufg)[BZ + 1][16] = buf[g];
+    if constexpr ((PASSES & 1) != 0)
+        tl_walk<false, BZ, false, false>(T, l16, nD, x, rD, lo, bufg, w, r, dotWR);
     // (the walk ends on a workgroup barrier: the forward rows are visible)
-    if (PASSES & 2) tl_walk<true, BZ, SYM>(T, l16, nD, w, rD, up, bufg, w, r, wantDot, dotWR);
-    if (wantDot)
+    if constexpr ((PASSES & 2) != 0)
+        tl_walk<true, BZ, SYM, wantDot>(T, l16, nD, w, rD, up, bufg, w, r, dotWR);
+    if constexpr (wantDot)
     {
         // one partial per wavefront (= workgroup), fixed butterfly order
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) dotWR += __shfl_xor(dotWR, off, 64);
         partials_arrive(dotWR, 0.0, partials, gridDim.x, blockIdx.x, F);
     }
+}
+
+// launch with the w*r sum compiled in when partials are given
+template<int PASSES, int BZ, bool SYM, class CT>
+static void tlv_launch(dim3 grid, hipStream_t s, const TileView& G, const float* x,
+                       const float* rD, const TlCoef<CT>& lo, const TlCoef<CT>& up, float* w,
+                       const DevState* st, const float* r, double* partials,
+                       const FusedFinalize& F)
+{
+    if (partials)
+        hipLaunchKernelGGL((k_tile_local_vec<PASSES, BZ, 4, SYM, CT, true>), grid, dim3(64), 0,
+                           s, G, x, rD, lo, up, w, st, r, partials, F);
+    else
+        hipLaunchKernelGGL((k_tile_local_vec<PASSES, BZ, 4, SYM, CT, false>), grid, dim3(64), 0,
+                           s, G, x, rD, lo, up, w, st, r, partials, F);
 }
 
 int32_t launch_tile_local_apply_f32(const TileView& G, const float* x, const float* rD,
@@ -451,22 +485,20 @@ int32_t launch_tile_local_apply_f32(const TileView& G, const float* x, const flo
         return -1;
     const int nTiles = G.nTI*G.nTJ*G.nTK;
     const dim3 grid((nTiles + 3)/4);
+    constexpr int BZ = TileBZ<float>::pre;
     if (symmetric && half3)
     {
         const TlCoef<__half> h{half3[0], half3[1], half3[2]};
-        hipLaunchKernelGGL((k_tile_local_vec<3, TileBZ<float>::pre, 4, true, __half>), grid,
-                           dim3(64), 0, s, G, x, rD, h, h, w, st, x, partials, F);
+        tlv_launch<3, BZ, true>(grid, s, G, x, rD, h, h, w, st, x, partials, F);
         return (int32_t)grid.x;
     }
     const TlCoef<float> lo{dir[0], dir[1], dir[2]};
     const TlCoef<float> up = symmetric ? TlCoef<float>{dir[0], dir[1], dir[2]}
                                        : TlCoef<float>{dir[5], dir[4], dir[3]};
     if (symmetric)
-        hipLaunchKernelGGL((k_tile_local_vec<3, TileBZ<float>::pre, 4, true, float>), grid,
-                           dim3(64), 0, s, G, x, rD, lo, up, w, st, x, partials, F);
+        tlv_launch<3, BZ, true>(grid, s, G, x, rD, lo, up, w, st, x, partials, F);
     else
-        hipLaunchKernelGGL((k_tile_local_vec<3, TileBZ<float>::pre, 4, false, float>), grid,
-                           dim3(64), 0, s, G, x, rD, lo, up, w, st, x, partials, F);
+        tlv_launch<3, BZ, false>(grid, s, G, x, rD, lo, up, w, st, x, partials, F);
     return (int32_t)grid.x;
 }
 
@@ -514,12 +546,11 @@ int32_t launch_tile_local_f32(bool backward, const TileView& G, const float* a, 
     {
         const TlCoef<float> c{cK, cJ, cI};
         if (backward)
-            hipLaunchKernelGGL((k_tile_local_vec<2, TileBZ<float>::pre, 4, false, float>), grid,
-                               dim3(64), 0, s, G, a, rD, c, c, w, st, r, partials,
-                               FusedFinalize());
+            tlv_launch<2, TileBZ<float>::pre, false>(grid, s, G, a, rD, c, c, w, st, r, partials,
+                                                     FusedFinalize());
         else
-            hipLaunchKernelGGL((k_tile_local_vec<1, TileBZ<float>::pre, 4, false, float>), grid,
-                               dim3(64), 0, s, G, a, rD, c, c, w, st, nullptr, nullptr,
+            hipLaunchKernelGGL((k_tile_local_vec<1, TileBZ<float>::pre, 4, false, float, false>),
+                               grid, dim3(64), 0, s, G, a, rD, c, c, w, st, nullptr, nullptr,
                                FusedFinalize());
         return (int32_t)grid.x;
     }
