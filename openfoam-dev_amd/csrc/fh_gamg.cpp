@@ -15,6 +15,7 @@
 #include <rocblas/rocblas.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -244,6 +245,13 @@ static fh_mesh* make_mesh(int device, int32_t n, const std::vector<int32_t>& lo,
     {
         FH_HIP(hipStreamCreateWithFlags(&m->setupStream, hipStreamNonBlocking));
         build_mesh(m, n, (int32_t)lo.size(), lo.data(), up.data());
+        // FOAMHIP_GAMG_VERBOSE: the shape of each coarse level and how its
+        // sweeps run
+        static const bool verbose = std::getenv("FOAMHIP_GAMG_VERBOSE") != nullptr;
+        if (verbose)
+            std::fprintf(stderr, "foamhip GAMG level: %d cells, %d faces, %d dependency levels, "
+                                 "%s\n", n, (int)lo.size(), m->sched.nLevels,
+                         m->chunkSweep ? "chunk sweep" : "one-workgroup level sweep");
     }
     catch (...)
     {

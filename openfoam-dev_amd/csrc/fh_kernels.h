@@ -160,6 +160,11 @@ void launch_sum_a(const RowsView& R, const double* diag, const double* val,
                   double* sumA, hipStream_t s);
 
 // sweeps
+// One-workgroup pipelined level sweep of a one-part mesh (Gauss-Seidel and
+// the DIC/DILU passes; fh_level_sweep.hip); false: kind or mesh not covered
+template<class T>
+bool launch_sweep_level_wg(int kind, const RowsView& R, const SchedView& S,
+                           const SweepData<T>& d, DevState* st, hipStream_t s);
 template<class T>
 void launch_sweep(int kind, const RowsView& R, const SchedView& S, SweepSync& sync,
                   const SweepData<T>& d, DevState* st, hipStream_t s);

@@ -216,6 +216,16 @@ static void sweep_any(fh_matrix* M, int kind, const SweepData<T>& d, DevState* s
         M->tileEpoch = ts.epoch;
         return;
     }
+    // one-part meshes (GAMG coarse levels): one workgroup walks the levels
+    // with the next level's rows loaded ahead, unless FOAMHIP_SWEEP asks
+    // for the chunk or part sweeps
+    static const bool multiWg = [] {
+        const char* e = std::getenv("FOAMHIP_SWEEP");
+        return e && (std::strcmp(e, "chunks") == 0 || std::strcmp(e, "parts") == 0);
+    }();
+    if (!multiWg && M->mesh->sched.nParts == 1
+        && launch_sweep_level_wg<T>(kind, M->rows(), M->sched(), d, st, M->stream))
+        return;
     if (M->rowSweep)
     {
         RowSync rs = M->row_sync();
