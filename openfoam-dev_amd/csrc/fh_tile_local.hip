@@ -346,6 +346,9 @@ __device__ __forceinline__ void tl_step(const TlTile& T, int l16, int step, cons
     }
     float out[16];
     float prev = 0.f;
+    // forward pass with the sum: t = r - sum coefficient*w(neighbour), the
+    // row's w = rD*t; sum w*t (see k_tile_local_vec)
+    double st[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int q = 0; q < 16; q++)
     {
@@ -358,9 +361,23 @@ __device__ __forceinline__ void tl_step(const TlTile& T, int l16, int step, cons
         v -= hasJ ? tJ : 0.f;
         // SHIFT: the upper i coefficient of cell i is the lower one of
         // cell i+1 (cell 15 has no i+1 inside the tile)
-        if (BWD ? i < 15 : i > 0) v -= Di*c.I.at(SHIFT ? min(i + 1, 15) : i)*prev;
+        const bool hasI = BWD ? i < 15 : i > 0;
+        if (hasI) v -= Di*c.I.at(SHIFT ? min(i + 1, 15) : i)*prev;
+        if constexpr (!BWD && DOT)
+        {
+            float t = q4(c.A, i);
+            t -= hasK ? c.K.at(i)*q4(NK, i) : 0.f;
+            t -= hasJ ? c.J.at(i)*q4(NJ, i) : 0.f;
+            if (hasI) t -= c.I.at(i)*prev;
+            st[i & 3] += (double)v*(double)t;
+        }
         prev = v;
         out[i] = v;
+    }
+    if constexpr (!BWD && DOT)
+    {
+        const double t = (st[0] + st[1]) + (st[2] + st[3]);
+        dotWR += act ? t : 0.0;
     }
     if constexpr (BWD && DOT)
     {
@@ -447,11 +464,18 @@ void k_tile_local_vec(TileView G, const float* x, const float* __restrict__ rD,
     nD = max(nD, __shfl_xor(nD, 32, 64));
     nD = __builtin_amdgcn_readfirstlane(nD);
     float (*bufg)[BZ + 1][16] = buf[g];
+    // Symmetric, both passes: the sum w*r is taken in the forward pass as
+    // sum w'*t, where the forward pass gives w' = rD*t with t = r - L w'
+    // (L: the lower coefficients): with M = (D + L) D^-1 (D + L^T) and
+    // D = 1/rD, r^T M^-1 r = w'^T D w' = sum w'*t.  The backward pass then
+    // does not read r.  Otherwise the backward pass sums w*r.
+    constexpr bool dotFwd = wantDot && SYM && PASSES == 3;
+    constexpr bool dotBwd = wantDot && !dotFwd;
     if constexpr ((PASSES & 1) != 0)
-        tl_walk<false, BZ, false, false>(T, l16, nD, x, rD, lo, bufg, w, r, dotWR);
+        tl_walk<false, BZ, false, dotFwd>(T, l16, nD, x, rD, lo, bufg, w, r, dotWR);
     // (the walk ends on a workgroup barrier: the forward rows are visible)
     if constexpr ((PASSES & 2) != 0)
-        tl_walk<true, BZ, SYM, wantDot>(T, l16, nD, w, rD, up, bufg, w, r, dotWR);
+        tl_walk<true, BZ, SYM, dotBwd>(T, l16, nD, w, rD, up, bufg, w, r, dotWR);
     if constexpr (wantDot)
     {
         // one partial per wavefront (= workgroup), fixed butterfly order
