@@ -29,12 +29,12 @@ namespace fh
 namespace
 {
 
-constexpr int LVL_THREADS = 1024;
+constexpr int LVL_THREADS = 512;
 
 template<class T> struct LvlCfg;
-// values of the previous level kept in LDS (two buffers: 2 x 48 KB) and the
-// entries a thread prefetches for its row
-template<> struct LvlCfg<double> { static constexpr int CAP = 6144; static constexpr int MAXE = 12; };
+// values of a level kept in LDS (two buffers by level parity: 2 x 48 KB) and
+// the entries a thread holds in registers for each of its pipelined rows
+template<> struct LvlCfg<double> { static constexpr int CAP = 6144; static constexpr int MAXE = 10; };
 template<> struct LvlCfg<float> { static constexpr int CAP = 12288; static constexpr int MAXE = 16; };
 
 __device__ __forceinline__ int32_t lvl_base(const int32_t* sliceStart, int32_t r)
@@ -50,13 +50,14 @@ template<> struct LvlKind<SWEEP_PRECOND_BWD> { static constexpr bool bwd = true;
 
 // the row a thread holds for a level: its scalars and, per used entry in
 // the order the update takes them (position p), the coefficient and where
-// the value comes from: loaded ahead (slot -1, value in v), the LDS copy of
-// the previous level (slot >= 0), or the vector at compute time (slot <= -2:
-// column -slot - 2, a previous-level row past the LDS capacity)
+// the value comes from: loaded ahead (slot -1, value in v) or the LDS copy
+// of the previous level (slot >= 0); a row that reads a previous-level row
+// past the LDS capacity is computed from memory instead (deferred)
 template<class T, int MAXE>
 struct LvlRow
 {
     int32_t r, cnt, base, nLo, nEnt;
+    bool deferred;        // reads a previous-level row past the LDS capacity
     T a, dg;
     T val[MAXE];
     T v[MAXE];
@@ -90,50 +91,6 @@ __device__ __forceinline__ T lvl_value(int32_t c, int32_t pLo, int32_t pHi, cons
     return w[c];
 }
 
-// row r of a level whose previous level (in sweep order) is rows
-// [pLo, pHi), still being computed: load everything else
-template<class T, int KIND, int MAXE, int CAP>
-__device__ __forceinline__ void lvl_fetch(const RowsView& R, const SweepData<T>& d, int32_t r,
-                                          int32_t pLo, int32_t pHi, LvlRow<T, MAXE>& o)
-{
-    const int32_t info = R.rowInfo[r];
-    o.r = r;
-    o.nLo = info & 0xffff;
-    o.nEnt = info >> 16;
-    o.cnt = lvl_count<KIND>(o.nLo, o.nEnt);
-    o.base = lvl_base(R.sliceStart, r);
-    if constexpr (KIND == SWEEP_GS_FWD || KIND == SWEEP_GS_BWD)
-    {
-        o.a = d.x[r];
-        o.dg = d.diag[r];
-    }
-    else if constexpr (KIND == SWEEP_PRECOND_FWD)
-    {
-        o.a = d.x[r];
-        o.dg = d.rD[r];
-    }
-    else
-    {
-        o.a = d.w[r];          // the forward pass's value of the row
-        o.dg = d.rD[r];
-    }
-#pragma unroll
-    for (int p = 0; p < MAXE; p++)
-    {
-        o.slot[p] = -1;
-        o.val[p] = T(0);
-        o.v[p] = T(0);
-        if (p < o.cnt)
-        {
-            const int32_t e = o.base + (lvl_entry<KIND>(p, o.nEnt) << 6);
-            const int32_t c = R.col[e];
-            o.val[p] = d.val[e];
-            if (c >= pLo && c < pHi) o.slot[p] = c - pLo < CAP ? c - pLo : -2 - c;
-            else o.v[p] = d.w[c];
-        }
-    }
-}
-
 // the row update of sweep_row on a held row; returns the new value
 template<class T, int KIND, int MAXE, int CAP>
 __device__ __forceinline__ T lvl_compute(const RowsView& R, const SweepData<T>& d,
@@ -153,7 +110,7 @@ __device__ __forceinline__ T lvl_compute(const RowsView& R, const SweepData<T>& 
         if (p < o.cnt)
         {
             const int32_t sl = o.slot[p];
-            const T value = sl >= 0 ? lds[sl] : (sl == -1 ? o.v[p] : d.w[-2 - sl]);
+            const T value = sl >= 0 ? lds[sl] : o.v[p];
             if constexpr (GS) t -= o.val[p]*value;
             else t -= rDr*o.val[p]*value;
         }
@@ -193,65 +150,191 @@ __device__ __forceinline__ T lvl_direct(const RowsView& R, const SweepData<T>& d
     else return t;
 }
 
+// stage 0 (three levels ahead): the row's shape
+struct LvlS0
+{
+    int32_t r, info, base;
+};
+
+// stage 1 (two levels ahead): scalars, columns and coefficients in the
+// update's order
+template<class T, int MAXE>
+struct LvlS1
+{
+    int32_t r, info, base;
+    T a, dg;
+    int32_t col[MAXE];
+    T val[MAXE];
+};
+
+template<class T, int KIND>
+__device__ __forceinline__ void lvl_s0(const RowsView& R, int32_t r, bool ok, LvlS0& o)
+{
+    o.r = ok ? r : -1;
+    const int32_t rr = ok ? r : 0;
+    o.info = ok ? R.rowInfo[rr] : 0;
+    o.base = lvl_base(R.sliceStart, rr);
+}
+
+template<class T, int KIND, int MAXE>
+__device__ __forceinline__ void lvl_s1(const RowsView& R, const SweepData<T>& d, const LvlS0& i,
+                                       LvlS1<T, MAXE>& o)
+{
+    o.r = i.r;
+    o.info = i.info;
+    o.base = i.base;
+    const int32_t rr = i.r >= 0 ? i.r : 0;
+    if constexpr (KIND == SWEEP_GS_FWD || KIND == SWEEP_GS_BWD)
+    {
+        o.a = d.x[rr];
+        o.dg = d.diag[rr];
+    }
+    else if constexpr (KIND == SWEEP_PRECOND_FWD)
+    {
+        o.a = d.x[rr];
+        o.dg = d.rD[rr];
+    }
+    else
+    {
+        o.a = d.w[rr];         // the forward pass's value of the row
+        o.dg = d.rD[rr];
+    }
+    const int32_t nLo = i.info & 0xffff, nEnt = i.info >> 16;
+    const int32_t cnt = lvl_count<KIND>(nLo, nEnt);
+    // every load is issued (an unused entry reads entry 0: the kernel runs
+    // on meshes with faces only),
+    // so no branch separates a load from its use a level later
+#pragma unroll
+    for (int p = 0; p < MAXE; p++)
+    {
+        const bool use = p < cnt;
+        const int32_t e = use ? i.base + (lvl_entry<KIND>(p, nEnt) << 6) : 0;
+        const int32_t c = R.col[e];
+        const T v = d.val[e];
+        o.col[p] = use ? c : -1;
+        o.val[p] = use ? v : T(0);
+    }
+}
+
+// stage 2 (one level ahead, at the start of the level before): the values,
+// or the LDS slot of a value of that level, still being computed
+// ([cLo, cHi)); values of the level finished last ([pLo, pHi)) from its LDS
+// copy, older ones from the vector
+template<class T, int KIND, int MAXE, int CAP>
+__device__ __forceinline__ void lvl_s2(const SweepData<T>& d, const LvlS1<T, MAXE>& i,
+                                       int32_t cLo, int32_t cHi, int32_t pLo, int32_t pHi,
+                                       const T* ldsP, LvlRow<T, MAXE>& o)
+{
+    o.r = i.r;
+    o.nLo = i.info & 0xffff;
+    o.nEnt = i.info >> 16;
+    o.cnt = i.r >= 0 ? lvl_count<KIND>(o.nLo, o.nEnt) : 0;
+    o.base = i.base;
+    o.a = i.a;
+    o.dg = i.dg;
+    o.deferred = false;
+#pragma unroll
+    for (int p = 0; p < MAXE; p++)
+    {
+        const int32_t c = i.col[p];
+        const bool use = p < o.cnt;
+        const bool inCur = use && c >= cLo && c < cHi;
+        const bool inPrev = use && c >= pLo && c < pHi && c - pLo < CAP;
+        const bool inVec = use && !inCur && !inPrev;
+        // the vector read is issued unconditionally (row 0 when unused)
+        T fromLds = T(0);
+        if (inPrev) fromLds = ldsP[c - pLo];
+        const T fromVec = d.w[inVec ? c : 0];
+        o.val[p] = i.val[p];
+        o.slot[p] = inCur && c - cLo < CAP ? c - cLo : -1;
+        o.deferred = o.deferred || (inCur && c - cLo >= CAP);
+        o.v[p] = inPrev ? fromLds : fromVec;
+    }
+}
+
+// rows [lo, hi) of the level taken s-th in sweep order (empty outside)
+struct LvlRange
+{
+    const int32_t* seg;
+    int32_t nSeg;
+    bool bwd;
+    __device__ int32_t lo(int32_t s) const
+    {
+        return s < 0 || s >= nSeg ? 0 : seg[bwd ? nSeg - 1 - s : s];
+    }
+    __device__ int32_t hi(int32_t s) const
+    {
+        return s < 0 || s >= nSeg ? 0 : seg[(bwd ? nSeg - 1 - s : s) + 1];
+    }
+};
+
+// one level: values of level s+1 (into `next`), columns of s+2 (s1),
+// shape of s+3 (s0), then the rows of level s (from `cur`) and a barrier
+template<class T, int KIND, int MAXE, int CAP>
+__device__ __forceinline__ void lvl_level(const RowsView& R, const SweepData<T>& d,
+                                          const LvlRange& G, int32_t s, T (*lv)[CAP],
+                                          LvlRow<T, MAXE>& cur, LvlRow<T, MAXE>& next,
+                                          LvlS1<T, MAXE>& s1, LvlS0& s0)
+{
+    const int32_t tid = (int32_t)threadIdx.x, nt = (int32_t)blockDim.x;
+    const int32_t r0 = G.lo(s), r1 = G.hi(s);
+    const int32_t pLo = G.lo(s - 1), pHi = G.hi(s - 1);
+    const T* ldsPrev = lv[(s + 1) & 1];
+    T* ldsMine = lv[s & 1];
+    lvl_s2<T, KIND, MAXE, CAP>(d, s1, r0, r1, pLo, pHi, ldsPrev, next);
+    lvl_s1<T, KIND, MAXE>(R, d, s0, s1);
+    {
+        const int32_t r = G.lo(s + 3) + tid;
+        lvl_s0<T, KIND>(R, r, r < G.hi(s + 3), s0);
+    }
+    if (cur.r >= 0)
+    {
+        const T v = cur.deferred
+                  ? lvl_direct<T, KIND, CAP>(R, d, cur.r, pLo, pHi, ldsPrev)
+                  : lvl_compute<T, KIND, MAXE, CAP>(R, d, cur, pLo, pHi, ldsPrev);
+        d.w[cur.r] = v;
+        if (cur.r - r0 < CAP) ldsMine[cur.r - r0] = v;
+    }
+    for (int32_t r = r0 + tid + nt; r < r1; r += nt)
+    {
+        const T v = lvl_direct<T, KIND, CAP>(R, d, r, pLo, pHi, ldsPrev);
+        d.w[r] = v;
+        if (r - r0 < CAP) ldsMine[r - r0] = v;
+    }
+    __syncthreads();
+}
+
 template<class T, int KIND>
 __global__ __launch_bounds__(LVL_THREADS)
 void k_sweep_level_wg(RowsView R, SchedView S, SweepData<T> d, DevState* st)
 {
     constexpr int CAP = LvlCfg<T>::CAP, MAXE = LvlCfg<T>::MAXE;
-    constexpr bool BWD = LvlKind<KIND>::bwd;
     __shared__ T lv[2][CAP];
     if (st && st->done) return;
-    const int32_t nSeg = S.partLevelOff[1] - S.partLevelOff[0] - 1;
-    const int32_t* seg = S.segStart + S.partLevelOff[0];
+    LvlRange G;
+    G.nSeg = S.partLevelOff[1] - S.partLevelOff[0] - 1;
+    G.seg = S.segStart + S.partLevelOff[0];
+    G.bwd = LvlKind<KIND>::bwd;
     const int32_t tid = (int32_t)threadIdx.x;
-    const int32_t nt = (int32_t)blockDim.x;
-    // level s in sweep order is library level L(s); its previous level in
-    // sweep order supplies the LDS values
-    auto levelOf = [&](int32_t s) { return BWD ? nSeg - 1 - s : s; };
-    LvlRow<T, MAXE> cur;
-    cur.cnt = 0;
+    // pipeline: A (level s+1, values), s1 (level s+2, columns and
+    // coefficients), s0 (level s+3, shape); the rows of level s are
+    // computed from B
+    LvlRow<T, MAXE> A, B;
+    LvlS1<T, MAXE> s1;
+    LvlS0 s0;
+    lvl_s0<T, KIND>(R, G.lo(0) + tid, G.lo(0) + tid < G.hi(0), s0);
+    lvl_s1<T, KIND, MAXE>(R, d, s0, s1);
+    lvl_s2<T, KIND, MAXE, CAP>(d, s1, 0, 0, 0, 0, lv[0], B);
+    lvl_s0<T, KIND>(R, G.lo(1) + tid, G.lo(1) + tid < G.hi(1), s0);
+    lvl_s1<T, KIND, MAXE>(R, d, s0, s1);
+    lvl_s0<T, KIND>(R, G.lo(2) + tid, G.lo(2) + tid < G.hi(2), s0);
+    int32_t s = 0;
+    for (; s + 1 < G.nSeg; s += 2)
     {
-        const int32_t L = levelOf(0);
-        const int32_t r = seg[L] + tid;
-        if (r < seg[L + 1]) lvl_fetch<T, KIND, MAXE, CAP>(R, d, r, 0, 0, cur);
+        lvl_level<T, KIND, MAXE, CAP>(R, d, G, s, lv, B, A, s1, s0);
+        lvl_level<T, KIND, MAXE, CAP>(R, d, G, s + 1, lv, A, B, s1, s0);
     }
-    for (int32_t s = 0; s < nSeg; s++)
-    {
-        const int32_t L = levelOf(s);
-        const int32_t r0 = seg[L], r1 = seg[L + 1];
-        // previous level in sweep order (none for the first)
-        int32_t pLo = 0, pHi = 0;
-        if (s > 0)
-        {
-            const int32_t P = levelOf(s - 1);
-            pLo = seg[P];
-            pHi = seg[P + 1];
-        }
-        const T* ldsPrev = lv[(s + 1) & 1];
-        T* ldsMine = lv[s & 1];
-        if (r0 + tid < r1)
-        {
-            const T v = lvl_compute<T, KIND, MAXE, CAP>(R, d, cur, pLo, pHi, ldsPrev);
-            d.w[cur.r] = v;
-            if (cur.r - r0 < CAP) ldsMine[cur.r - r0] = v;
-        }
-        for (int32_t r = r0 + tid + nt; r < r1; r += nt)
-        {
-            const T v = lvl_direct<T, KIND, CAP>(R, d, r, pLo, pHi, ldsPrev);
-            d.w[r] = v;
-            if (r - r0 < CAP) ldsMine[r - r0] = v;
-        }
-        // the next level's row, loaded while the workgroup finishes this one
-        // (its reads of this level are taken from LDS after the barrier)
-        if (s + 1 < nSeg)
-        {
-            const int32_t N = levelOf(s + 1);
-            const int32_t r = seg[N] + tid;
-            cur.cnt = 0;
-            if (r < seg[N + 1]) lvl_fetch<T, KIND, MAXE, CAP>(R, d, r, r0, r1, cur);
-        }
-        __syncthreads();
-    }
+    if (s < G.nSeg) lvl_level<T, KIND, MAXE, CAP>(R, d, G, s, lv, B, A, s1, s0);
 }
 
 template<class T, int KIND>

@@ -223,7 +223,7 @@ static void sweep_any(fh_matrix* M, int kind, const SweepData<T>& d, DevState* s
         const char* e = std::getenv("FOAMHIP_SWEEP");
         return e && (std::strcmp(e, "chunks") == 0 || std::strcmp(e, "parts") == 0);
     }();
-    if (!multiWg && M->mesh->sched.nParts == 1
+    if (!multiWg && M->mesh->sched.nParts == 1 && M->mesh->nFaces > 0
         && launch_sweep_level_wg<T>(kind, M->rows(), M->sched(), d, st, M->stream))
         return;
     if (M->rowSweep)
