@@ -146,6 +146,7 @@ __device__ inline void finalize_apply(double a, double b, int32_t op, DevState* 
             // fell by replaceRatio since the last one, when it looks
             // converged, or at the iteration limit
             const double est = a/st->normFactor;
+            st->foldPending = 0;     // this iteration's update folded it
             if (!isfinite(est)) { st->numericError = 1; st->done = 1; break; }
             st->nIterations++;
             bool looks = est < st->tolerance
@@ -159,6 +160,7 @@ __device__ inline void finalize_apply(double a, double b, int32_t op, DevState* 
             if (!st->replace) break;
             st->replace = 0;
             st->nReplace++;
+            st->foldPending = 1;     // the next update folds the FP32 correction
             st->finalResidual = b/st->normFactor;
             if (!isfinite(st->finalResidual)) { st->numericError = 1; st->done = 1; break; }
             st->resLast = st->finalResidual;

@@ -33,6 +33,7 @@ struct DevState
     // mixed precision reliable updates
     int32_t replace;        // 1: recompute the FP64 residual this iteration
     int32_t nReplace;       // FP64 residual replacements made
+    int32_t foldPending;    // 1: the next update folds the FP32 correction into x
     double resLast;         // true residual at the last replacement
     double replaceRatio;    // replace when the FP32 residual fell by this
     double gamgScale;       // GAMG correction scaling factor

@@ -958,6 +958,10 @@ static void mixed_pcg(fh_matrix* M, const fh_controls* c, double* psi, const dou
     M->f[3].alloc(N);
     M->f[4].alloc(N);
     float* pBuf[2] = {p32, M->f[3].p};
+    // the FP32 solution correction since the last fold into psi
+    M->f[5].alloc(N);
+    float* xc = M->f[5].p;
+    FH_HIP(hipMemsetAsync(xc, 0, (size_t)N*sizeof(float), M->stream));
     // A p: in place of w when p is formed by its own kernel; a separate
     // buffer when the product reads w to form p
     float* ap = fuseP ? M->f[4].p : w32;
@@ -996,14 +1000,17 @@ static void mixed_pcg(fh_matrix* M, const fh_controls* c, double* psi, const dou
         }
         const FusedFinalize fu = fused_fin(M, FIN_MIXED_RES);
         kt_mark(M, FH_KERNEL_AXPY_NORM);
-        const int32_t nbu = launch_mixed_update(N, psi, r32, p32, ap, part, st, M->stream, fu);
+        const int32_t nbu = launch_mixed_update(N, psi, xc, r32, p32, ap, part, st, M->stream,
+                                                fu);
         kt_mark(M, FH_KERNEL_AXPY_NORM);
         if (!fu.counter) launch_finalize_blocks(nbu, N, part, FIN_MIXED_RES, st, 0, M->stream);
         const FusedFinalize ft = fused_fin(M, FIN_MIXED_TRUE);
-        const int32_t nbr = launch_mixed_replace(M->rows(), M->d_diag.p, M->d_val.p, psi, src,
-                                                 r32, part, st, M->stream, ft);
+        const int32_t nbr = launch_mixed_replace(M->rows(), M->d_diag.p, M->d_val.p, psi, xc,
+                                                 src, r32, part, st, M->stream, ft);
         if (!ft.counter) launch_finalize_blocks(nbr, N, part, FIN_MIXED_TRUE, st, 0, M->stream);
     });
+    // the correction of the last iterations
+    launch_axpy_f32(N, psi, xc, 1.0, M->stream);
     const DevState* h = M->h_state;
     std::memset(perf, 0, sizeof(*perf));
     std::snprintf(perf->solverName, sizeof(perf->solverName), "%sPCG(mixed)",
@@ -1382,7 +1389,8 @@ static void profile_t(fh_matrix* M, int kernel, int precon, int reps, double* ms
                 {
                     // the update of the mixed inner iteration: FP64 solution,
                     // FP32 recurrence residual
-                    launch_mixed_update(N, psi64.p, y.p, z.p, z.p, part, pstate, M->stream);
+                    launch_mixed_update(N, psi64.p, x.p, y.p, z.p, z.p, part, pstate,
+                                        M->stream);
                     break;
                 }
                 launch_pcg_update_nostate<T>(N, x.p, y.p, z.p, z.p, M->d_partials.p, M->stream);
@@ -1442,7 +1450,9 @@ static void profile_t(fh_matrix* M, int kernel, int precon, int reps, double* ms
         case FH_KERNEL_AXPY_NORM:
             // FP32 (mixed): psi FP64 read+write, r read+write, p, w read;
             // FP64: psi, r read+write, p, w read
-            *bytes = w == 4 ? N*(16.0 + 4*w) : 6.0*N*w; break;
+            // FP32 (the mixed update between folds): correction and
+            // recurrence residual read and written, p and A p read
+            *bytes = 6.0*N*w; break;
     }
 }
 

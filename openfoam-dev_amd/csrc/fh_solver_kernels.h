@@ -104,14 +104,14 @@ void launch_pcg_update_nostate(int32_t n, T* psi, T* r, const T* p, const T* w,
                                double* partials, hipStream_t s);
 // mixed PCG: x64 += alpha*p32, r32 -= alpha*w32, sum |r32|; returns the
 // partial block count
-int32_t launch_mixed_update(int32_t n, double* x, float* r, const float* p, const float* w,
-                            double* partials, const DevState* st, hipStream_t s,
-                            const FusedFinalize& F = FusedFinalize());
-// mixed PCG residual replacement (only when st->replace): r = b - A x in
-// FP64, r32 = r, sum |r|; returns the partial count
+int32_t launch_mixed_update(int32_t n, double* x, float* xc, float* r, const float* p,
+                            const float* w, double* partials, const DevState* st,
+                            hipStream_t s, const FusedFinalize& F = FusedFinalize());
+// mixed PCG residual replacement (only when st->replace): r = b - A (x + xc)
+// in FP64, r32 = r, sum |r|; returns the partial count
 int32_t launch_mixed_replace(const RowsView& R, const double* diag, const double* val,
-                             const double* x, const double* b, float* r32, double* partials,
-                             const DevState* st, hipStream_t s,
+                             const double* x, const float* xc, const double* b, float* r32,
+                             double* partials, const DevState* st, hipStream_t s,
                              const FusedFinalize& F = FusedFinalize());
 void launch_mixed_setup(DevState* st, double replaceRatio, hipStream_t s);
 // psi += w

@@ -813,9 +813,31 @@ __global__ void k_pcg_update(int32_t n, T* __restrict__ psi, T* __restrict__ r,
     if (st->done) return;
     const T alpha = (T)st->alpha;
     double s0 = 0.0;
-    // one value per thread on a full grid; strided on the short grid of
-    // the self-finalizing launch
-    for (int32_t i = blockIdx.x*FH_BLOCK + threadIdx.x; i < n; i += gridDim.x*FH_BLOCK)
+    // one value per thread on a full grid; strided, four values per thread
+    // in flight, on the short grid of the self-finalizing launch
+    const int32_t stride = gridDim.x*FH_BLOCK;
+    int32_t i = blockIdx.x*FH_BLOCK + threadIdx.x;
+    for (; i + 3*stride < n; i += 4*stride)
+    {
+        T x4[4], r4[4], p4[4], w4[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+        {
+            x4[u] = psi[i + u*stride];
+            r4[u] = r[i + u*stride];
+            p4[u] = p[i + u*stride];
+            w4[u] = w[i + u*stride];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+        {
+            psi[i + u*stride] = x4[u] + alpha*p4[u];
+            const T ri = r4[u] - alpha*w4[u];
+            r[i + u*stride] = ri;
+            s0 += fabs((double)ri);
+        }
+    }
+    for (; i < n; i += stride)
     {
         psi[i] += alpha*p[i];
         const T ri = r[i] - alpha*w[i];
@@ -958,26 +980,70 @@ void k_finalize(int32_t nBlocks, const double* partials, int32_t op,
 
 // a fixed grid of at most MIX_BLOCKS blocks strides over the vector, so that
 // the finalize sums few partials (the order stays fixed)
-__global__ void k_mixed_update(int32_t n, double* __restrict__ x, float* __restrict__ r,
-                               const float* __restrict__ p, const float* __restrict__ w,
-                               double* partials, const DevState* st, FusedFinalize F)
+// The solution update alpha*p accumulates in FP32 (xc) between residual
+// replacements and is folded into the FP64 solution x on the iteration after
+// each replacement (x += xc + alpha*p in FP64, xc = 0): the FP64 solution is
+// read and written a few times per solve instead of every iteration.
+__global__ void k_mixed_update(int32_t n, double* __restrict__ x, float* __restrict__ xc,
+                               float* __restrict__ r, const float* __restrict__ p,
+                               const float* __restrict__ w, double* partials,
+                               const DevState* st, FusedFinalize F)
 {
     if (st->done) return;
     const double alpha = st->alpha;
     const float alphaF = (float)alpha;
+    const bool fold = st->foldPending != 0;
     double s0 = 0.0;
-    for (int32_t i = blockIdx.x*FH_BLOCK + threadIdx.x; i < n; i += gridDim.x*FH_BLOCK)
+    const int32_t stride = gridDim.x*FH_BLOCK;
+    int32_t i = blockIdx.x*FH_BLOCK + threadIdx.x;
+    if (fold)
     {
-        x[i] += alpha*(double)p[i];
-        const float ri = r[i] - alphaF*w[i];
-        r[i] = ri;
-        s0 += fabs((double)ri);
+        for (; i < n; i += stride)
+        {
+            x[i] += (double)xc[i] + alpha*(double)p[i];
+            xc[i] = 0.f;
+            const float ri = r[i] - alphaF*w[i];
+            r[i] = ri;
+            s0 += fabs((double)ri);
+        }
+    }
+    else
+    {
+        // four values per thread in flight
+        for (; i + 3*stride < n; i += 4*stride)
+        {
+            float c[4], rr[4], pp[4], ww[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+            {
+                c[u] = xc[i + u*stride];
+                rr[u] = r[i + u*stride];
+                pp[u] = p[i + u*stride];
+                ww[u] = w[i + u*stride];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+            {
+                xc[i + u*stride] = c[u] + alphaF*pp[u];
+                const float ri = rr[u] - alphaF*ww[u];
+                r[i + u*stride] = ri;
+                s0 += fabs((double)ri);
+            }
+        }
+        for (; i < n; i += stride)
+        {
+            xc[i] += alphaF*p[i];
+            const float ri = r[i] - alphaF*w[i];
+            r[i] = ri;
+            s0 += fabs((double)ri);
+        }
     }
     write_partials_fin(s0, 0.0, partials, F);
 }
 
 __global__ void k_mixed_replace(RowsView R, const double* __restrict__ diag,
                                 const double* __restrict__ val, const double* __restrict__ x,
+                                const float* __restrict__ xc,
                                 const double* __restrict__ b, float* __restrict__ r32,
                                 double* partials, const DevState* st, FusedFinalize F)
 {
@@ -989,11 +1055,13 @@ __global__ void k_mixed_replace(RowsView R, const double* __restrict__ diag,
     {
         const int32_t nEnt = R.rowInfo[r] >> 16;
         const int32_t base = sell_base2(R.sliceStart, r);
-        double acc = b[r] - diag[r]*x[r];
+        // the current solution: x plus the FP32 correction not yet folded
+        double acc = b[r] - diag[r]*(x[r] + (double)xc[r]);
         for (int32_t k = 0; k < nEnt; k++)
         {
             const int32_t e = base + (k << 6);
-            acc -= val[e]*x[R.col[e]];
+            const int32_t c = R.col[e];
+            acc -= val[e]*(x[c] + (double)xc[c]);
         }
         r32[r] = (float)acc;
         s1 += fabs(acc);
@@ -1007,6 +1075,7 @@ __global__ void k_mixed_setup(DevState* st, double ratio)
     st->replaceRatio = ratio;
     st->replace = 0;
     st->nReplace = 0;
+    st->foldPending = 0;
 }
 
 void launch_mixed_setup(DevState* st, double replaceRatio, hipStream_t s)
@@ -1014,23 +1083,24 @@ void launch_mixed_setup(DevState* st, double replaceRatio, hipStream_t s)
     hipLaunchKernelGGL(k_mixed_setup, dim3(1), dim3(1), 0, s, st, replaceRatio);
 }
 
-int32_t launch_mixed_update(int32_t n, double* x, float* r, const float* p, const float* w,
-                            double* partials, const DevState* st, hipStream_t s,
-                            const FusedFinalize& F)
+int32_t launch_mixed_update(int32_t n, double* x, float* xc, float* r, const float* p,
+                            const float* w, double* partials, const DevState* st,
+                            hipStream_t s, const FusedFinalize& F)
 {
     const int32_t nb = std::min(reduction_blocks(n), MIX_BLOCKS);
     hipLaunchKernelGGL(k_mixed_update, dim3(nb), dim3(FH_BLOCK), 0, s,
-                       n, x, r, p, w, partials, st, F);
+                       n, x, xc, r, p, w, partials, st, F);
     return nb;
 }
 
 int32_t launch_mixed_replace(const RowsView& R, const double* diag, const double* val,
-                             const double* x, const double* b, float* r32, double* partials,
-                             const DevState* st, hipStream_t s, const FusedFinalize& F)
+                             const double* x, const float* xc, const double* b, float* r32,
+                             double* partials, const DevState* st, hipStream_t s,
+                             const FusedFinalize& F)
 {
     const int32_t nb = std::min(reduction_blocks(R.n), MIX_BLOCKS);
     hipLaunchKernelGGL(k_mixed_replace, dim3(nb), dim3(FH_BLOCK), 0, s,
-                       R, diag, val, x, b, r32, partials, st, F);
+                       R, diag, val, x, xc, b, r32, partials, st, F);
     return nb;
 }
 
