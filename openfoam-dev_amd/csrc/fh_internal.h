@@ -155,6 +155,13 @@ struct fh_mesh
     // GAMG agglomeration cached with the mesh (built on the first GAMG
     // solve, as GAMGAgglomeration is cached on the mesh database)
     std::shared_ptr<void> gamgAgg;
+
+    // multi-colour row order (greedy over the caller's cell order; red-black
+    // on a box), built on first use by the FP32 Gauss-Seidel of the mixed
+    // GAMG V-cycles
+    bool coloured = false;
+    std::vector<int32_t> colourStart;       // [nColours + 1]
+    fh::DBuf<int32_t> d_colourRows;         // library rows grouped by colour
 };
 
 namespace fh

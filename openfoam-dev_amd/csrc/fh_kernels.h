@@ -161,6 +161,10 @@ void launch_sum_a(const RowsView& R, const double* diag, const double* val,
                   double* sumA, hipStream_t s);
 
 // sweeps
+// One colour of a multi-colour Gauss-Seidel sweep over the listed rows
+template<class T>
+void launch_sweep_colour(const RowsView& R, const SweepData<T>& d, const int32_t* rows,
+                         int32_t n, const DevState* st, hipStream_t s);
 // One-workgroup pipelined level sweep of a one-part mesh (Gauss-Seidel and
 // the DIC/DILU passes; fh_level_sweep.hip); false: kind or mesh not covered
 template<class T>

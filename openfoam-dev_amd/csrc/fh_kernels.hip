@@ -364,6 +364,34 @@ __device__ __forceinline__ void sweep_row(const RowsView& R, const SweepData<T>&
     }
 }
 
+// One colour of a multi-colour Gauss-Seidel sweep: no row of a colour reads
+// another row of the same colour, so the rows update in parallel with the
+// row arithmetic of the ordered sweep (neighbours of earlier colours new,
+// of later colours old)
+template<class T>
+__global__ __launch_bounds__(FH_BLOCK)
+void k_sweep_colour(RowsView R, SweepData<T> d, const int32_t* __restrict__ rows, int32_t n,
+                    const DevState* st)
+{
+    if (st && st->done) return;
+    const int32_t i = blockIdx.x*FH_BLOCK + threadIdx.x;
+    if (i < n) sweep_row<T, SWEEP_GS_FWD>(R, d, rows[i]);
+}
+
+template<class T>
+void launch_sweep_colour(const RowsView& R, const SweepData<T>& d, const int32_t* rows,
+                         int32_t n, const DevState* st, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_sweep_colour<T>, dim3((n + FH_BLOCK - 1)/FH_BLOCK), dim3(FH_BLOCK), 0,
+                       s, R, d, rows, n, st);
+}
+
+template void launch_sweep_colour<double>(const RowsView&, const SweepData<double>&,
+                                          const int32_t*, int32_t, const DevState*, hipStream_t);
+template void launch_sweep_colour<float>(const RowsView&, const SweepData<float>&,
+                                         const int32_t*, int32_t, const DevState*, hipStream_t);
+
 template<class T, int KIND, bool BACKWARD>
 __global__ __launch_bounds__(FH_BLOCK)
 void k_sweep_parts(RowsView R, SchedView S, SweepSync sync, SweepData<T> d,

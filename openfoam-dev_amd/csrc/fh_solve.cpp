@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <type_traits>
 
 namespace fh
@@ -823,6 +824,47 @@ template<class T> static DBuf<T>* smooth_work(fh_matrix* M);
 template<> DBuf<double>* smooth_work<double>(fh_matrix* M) { return M->sw; }
 template<> DBuf<float>* smooth_work<float>(fh_matrix* M) { return M->sw32; }
 
+// Greedy colouring in the caller's cell order: a cell takes the smallest
+// colour none of its lower-numbered neighbours has (on a box in natural
+// order: red-black); rows are grouped by colour, ascending within a colour.
+static std::mutex g_colourMutex;
+static void ensure_colours(fh_mesh* m)
+{
+    std::lock_guard<std::mutex> lk(g_colourMutex);
+    if (m->coloured) return;
+    const int32_t n = m->nCells;
+    std::vector<int32_t> start(n + 1, 0), nb((size_t)m->nFaces);
+    for (int32_t f = 0; f < m->nFaces; f++) start[m->upper[f] + 1]++;
+    for (int32_t c = 0; c < n; c++) start[c + 1] += start[c];
+    {
+        std::vector<int32_t> fill(start.begin(), start.end() - 1);
+        for (int32_t f = 0; f < m->nFaces; f++) nb[fill[m->upper[f]]++] = m->lower[f];
+    }
+    std::vector<int32_t> colour(n, 0), seen(16, -1);
+    int32_t nCol = 1;
+    for (int32_t c = 0; c < n; c++)
+    {
+        for (int32_t q = start[c]; q < start[c + 1]; q++)
+        {
+            const int32_t k = colour[nb[q]];
+            if (k >= (int32_t)seen.size()) seen.resize(k + 1, -1);
+            seen[k] = c;
+        }
+        int32_t k = 0;
+        while (k < (int32_t)seen.size() && seen[k] == c) k++;
+        colour[c] = k;
+        nCol = std::max(nCol, k + 1);
+    }
+    m->colourStart.assign(nCol + 1, 0);
+    for (int32_t r = 0; r < n; r++) m->colourStart[colour[m->perm[r]] + 1]++;
+    for (int32_t k = 0; k < nCol; k++) m->colourStart[k + 1] += m->colourStart[k];
+    std::vector<int32_t> rows(n), fill(m->colourStart.begin(), m->colourStart.end() - 1);
+    for (int32_t r = 0; r < n; r++) rows[fill[colour[m->perm[r]]]++] = r;
+    m->d_colourRows.upload(rows, m->setupStream);
+    FH_HIP(hipStreamSynchronize(m->setupStream));
+    m->coloured = true;
+}
+
 // GaussSeidelSmoother.C:66-187, symGaussSeidelSmoother.C:66-216,
 // DICSmoother.C:67-116, DILUSmoother.C:67-119, DICGaussSeidelSmoother.C:79-89,
 // DILUGaussSeidelSmoother.C:79-89 (library numbering)
@@ -841,6 +883,27 @@ void smooth_t(fh_matrix* M, int smoother, T* psi, const T* src, int32_t nSweeps,
         d.diag = diag;
         d.x = src;
         d.w = psi;
+        if constexpr (std::is_same<T, float>::value)
+        {
+            // FP32 (the V-cycles of the mixed GAMG): multi-colour
+            // Gauss-Seidel, colours ascending (and descending for the
+            // symmetric smoother); every colour is one parallel pass
+            ensure_colours(M->mesh);
+            const std::vector<int32_t>& cs = M->mesh->colourStart;
+            const int32_t nCol = (int32_t)cs.size() - 1;
+            const int32_t* rows = M->mesh->d_colourRows.p;
+            for (int32_t s = 0; s < nSweeps; s++)
+            {
+                for (int32_t k = 0; k < nCol; k++)
+                    launch_sweep_colour<T>(M->rows(), d, rows + cs[k], cs[k + 1] - cs[k], st,
+                                           M->stream);
+                if (sym)
+                    for (int32_t k = nCol - 1; k >= 0; k--)
+                        launch_sweep_colour<T>(M->rows(), d, rows + cs[k], cs[k + 1] - cs[k],
+                                               st, M->stream);
+            }
+            return;
+        }
         for (int32_t s = 0; s < nSweeps; s++)
         {
             sweep_t<T>(M, SWEEP_GS_FWD, d, st);
