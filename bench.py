@@ -52,6 +52,8 @@ def parse():
                         "for counter calibration), for rocprofv3 counter runs")
     p.add_argument("--cpu-seconds", type=float, default=15.0,
                    help="approximate CPU time budget of the baseline sample")
+    p.add_argument("--gamg-n", type=int, default=128,
+                   help="box size of the GAMG FP64 / mixed comparison (0: skip)")
     return p.parse_args()
 
 
@@ -194,6 +196,8 @@ def main():
     copy_gbs = measured_copy(torch)
     log(f"device copy of 1 GiB: {copy_gbs:.0f} GB/s (read + write)")
 
+    gamg = gamg_comparison(fh, torch, a.gamg_n) if a.gamg_n > 0 else None
+
     serve = None
     if a.serve_requests > 0:
         # continuous-batching sample of the serving configuration: Poisson
@@ -243,6 +247,7 @@ def main():
                                           "FP16 coefficients)",
             "assembly_ms": assembly_ms,
             "serve": serve,
+            "gamg": gamg,
             "kernels_live": live,
             "kernels": kernels,
         },
@@ -264,6 +269,56 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def gamg_comparison(fh, torch, n):
+    """GAMG (symGaussSeidel, algebraic pair agglomeration, defaults otherwise)
+    to 1e-8 on an n^3 box with the same log-normal diffusivity model, FP64
+    (reference-ordered smoother) against mixed (FP32 V-cycles with
+    multi-colour Gauss-Seidel inside FP64 residual updates); the first solve
+    of each builds the hierarchy and is not timed."""
+    from foamhip.cases import BoxCase
+    case = BoxCase(n)
+    gamma = case.lognormal(11, 1.0, 8.0)
+    rhs = case.rhs(1)
+    mesh = fh.lduAddressing(case.N, case.lower, case.upper)
+    bnd = fh.Boundary(mesh, case.bFaceCells)
+    mat = fh.lduMatrix(mesh)
+    dev = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to("cuda")
+    D = {k: dev(getattr(case, k)) for k in
+         ["weights", "magSf", "deltaCoeffs", "bMagSf", "bDeltaCoeffs", "bValue"]}
+    D["bType"] = dev(case.bType.astype(np.int32))
+    g = dev(gamma)
+    src = dev(rhs)
+    ic = torch.empty(case.nB, dtype=torch.float64, device="cuda")
+    bc = torch.empty_like(ic)
+    P = lambda t: ctypes.c_void_p(t.data_ptr())
+    fh._check(fh.lib().fh_assemble_laplacian(
+        mat.handle, bnd.handle, P(g), P(D["weights"]), P(D["magSf"]), P(D["deltaCoeffs"]),
+        P(D["bMagSf"]), P(D["bDeltaCoeffs"]), P(D["bType"]), P(D["bValue"]), P(ic), P(bc),
+        P(src), 1))
+    psi = torch.zeros(case.N, dtype=torch.float64, device="cuda")
+    out = {"workload": f"{n}^3 Poisson box, log-normal gamma, GAMG symGaussSeidel to 1e-8"}
+    xs = {}
+    for prec in ("double", "mixed"):
+        s = fh.solver_New("p", mat, {"solver": "GAMG", "smoother": "symGaussSeidel",
+                                     "tolerance": 1e-8, "relTol": 0, "maxIter": 200,
+                                     "precision": prec})
+        for rep in range(2):
+            psi.zero_()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            pf = s.solve_device(psi.data_ptr(), src.data_ptr())
+            torch.cuda.synchronize()
+            t = time.perf_counter() - t0
+        xs[prec] = psi.cpu().numpy()
+        out[prec] = {"s_per_solve": t, "cycles": pf.nIterations,
+                     "final_residual": pf.finalResidual}
+        log(f"gamg {n}^3 {prec}: {pf.line()}, {t:.3f} s/solve")
+    out["speedup_mixed_vs_fp64"] = out["double"]["s_per_solve"] / out["mixed"]["s_per_solve"]
+    out["mixed_vs_fp64_rel_l2"] = float(np.linalg.norm(xs["mixed"] - xs["double"])
+                                        / np.linalg.norm(xs["double"]))
+    return out
 
 
 def measured_copy(torch):

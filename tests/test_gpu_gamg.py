@@ -106,9 +106,10 @@ def test_gamg_preconditioned_krylov():
 
 def test_gamg_mixed_precision():
     """FP32 V-cycles on the residual equation inside FP64 residual updates:
-    the FP64 convergence test is met, the iteration count stays within two
-    of the FP64 count, and at a tight tolerance the solution matches the
-    FP64 oracle to 1e-8."""
+    the FP64 convergence test is met, the cycle count stays close to the FP64
+    count (the FP32 cycles smooth with multi-colour Gauss-Seidel, not the
+    ordered sweep), and at a tight tolerance the solution matches the FP64
+    oracle to 1e-8."""
     c, a, diag, upper, b = laplacian_case(24, sigma=1.0)
     om = O.Matrix(a, diag, upper)
     ctl = dict(GAMG, smoother="GaussSeidel")
@@ -123,13 +124,43 @@ def test_gamg_mixed_precision():
     assert perf.converged
     r = b - om.Amul(psi)
     assert np.abs(r).sum() / perf.normFactor < 1e-8
-    assert abs(perf.nIterations - operf["nIterations"]) <= ITER_TOL
+    assert perf.nIterations <= 1.25*operf["nIterations"] + ITER_TOL
     tight = dict(ctl, tolerance=1e-13, maxIter=200)
     O.reset_direction()
     xpsi, _ = om.solve(tight, np.zeros(c.N), b)
     psi2 = np.zeros(c.N)
     perf2 = fh.solver_New("p", m, dict(tight, precision="mixed")).solve(psi2, b)
     assert perf2.converged
+    assert rel_l2(psi2, xpsi) <= 1e-8
+
+
+@pytest.mark.parametrize("smoother", ["GaussSeidel", "symGaussSeidel"])
+def test_gamg_mixed_precision_unstructured(smoother):
+    """Mixed GAMG on a random graph (irregular coarse levels, many colours in
+    the FP32 colour sweeps): the FP64 convergence test holds for the true
+    residual, and at a tight tolerance the solution matches the FP64
+    oracle."""
+    rng = np.random.default_rng(7)
+    n = 3000
+    _, lower, upper = random_graph(n, 6, rng)
+    diag, up = spd_coeffs(n, lower, upper, rng)
+    b = rng.uniform(-1, 1, n)
+    a = O.Addr(n, lower, upper)
+    om = O.Matrix(a, diag, up)
+    mesh = fh.lduAddressing(n, lower, upper)
+    m = fh.lduMatrix(mesh, diag, up)
+    ctl = dict(GAMG, smoother=smoother, maxIter=200)
+    fh.lib().fh_gamg_reset_direction()
+    psi = np.zeros(n)
+    perf = fh.solver_New("p", m, dict(ctl, precision="mixed")).solve(psi, b)
+    assert perf.solverName == "GAMG(mixed)" and perf.converged
+    assert np.abs(b - om.Amul(psi)).sum() / perf.normFactor < 1e-8
+    tight = dict(ctl, tolerance=1e-13)
+    O.reset_direction()
+    xpsi, _ = om.solve(tight, np.zeros(n), b)
+    fh.lib().fh_gamg_reset_direction()
+    psi2 = np.zeros(n)
+    assert fh.solver_New("p", m, dict(tight, precision="mixed")).solve(psi2, b).converged
     assert rel_l2(psi2, xpsi) <= 1e-8
 
 
