@@ -162,6 +162,8 @@ struct fh_mesh
     bool coloured = false;
     std::vector<int32_t> colourStart;       // [nColours + 1]
     fh::DBuf<int32_t> d_colourRows;         // library rows grouped by colour
+    fh::DBuf<int32_t> d_colourStart;        // device copy of colourStart
+    int32_t maxColourRows = 0;              // rows of the largest colour
 };
 
 namespace fh

@@ -165,6 +165,13 @@ void launch_sum_a(const RowsView& R, const double* diag, const double* val,
 template<class T>
 void launch_sweep_colour(const RowsView& R, const SweepData<T>& d, const int32_t* rows,
                          int32_t n, const DevState* st, hipStream_t s);
+// nSweeps multi-colour sweeps of a small mesh in one workgroup (colours
+// ascending, then descending when sym; cstart: device copy of the colour
+// offsets)
+template<class T>
+void launch_sweep_colours_wg(const RowsView& R, const SweepData<T>& d, const int32_t* rows,
+                             const int32_t* cstart, int32_t nCol, int32_t nSweeps, bool sym,
+                             const DevState* st, hipStream_t s);
 // One-workgroup pipelined level sweep of a one-part mesh (Gauss-Seidel and
 // the DIC/DILU passes; fh_level_sweep.hip); false: kind or mesh not covered
 template<class T>

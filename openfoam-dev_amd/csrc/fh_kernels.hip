@@ -387,6 +387,43 @@ void launch_sweep_colour(const RowsView& R, const SweepData<T>& d, const int32_t
                        s, R, d, rows, n, st);
 }
 
+// All colours of a small mesh in one workgroup: nSweeps sweeps over the
+// colours ascending (then descending when sym), a barrier between colours
+template<class T>
+__global__ __launch_bounds__(1024)
+void k_sweep_colours_wg(RowsView R, SweepData<T> d, const int32_t* __restrict__ rows,
+                        const int32_t* __restrict__ cstart, int32_t nCol, int32_t nSweeps,
+                        int32_t sym, const DevState* st)
+{
+    if (st && st->done) return;
+    const int32_t nPass = sym ? 2*nCol : nCol;
+    for (int32_t s = 0; s < nSweeps; s++)
+        for (int32_t q = 0; q < nPass; q++)
+        {
+            const int32_t k = q < nCol ? q : 2*nCol - 1 - q;
+            const int32_t r0 = cstart[k], r1 = cstart[k + 1];
+            for (int32_t i = r0 + (int32_t)threadIdx.x; i < r1; i += (int32_t)blockDim.x)
+                sweep_row<T, SWEEP_GS_FWD>(R, d, rows[i]);
+            __syncthreads();
+        }
+}
+
+template<class T>
+void launch_sweep_colours_wg(const RowsView& R, const SweepData<T>& d, const int32_t* rows,
+                             const int32_t* cstart, int32_t nCol, int32_t nSweeps, bool sym,
+                             const DevState* st, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_sweep_colours_wg<T>, dim3(1), dim3(1024), 0, s, R, d, rows, cstart,
+                       nCol, nSweeps, (int32_t)sym, st);
+}
+
+template void launch_sweep_colours_wg<double>(const RowsView&, const SweepData<double>&,
+                                              const int32_t*, const int32_t*, int32_t, int32_t,
+                                              bool, const DevState*, hipStream_t);
+template void launch_sweep_colours_wg<float>(const RowsView&, const SweepData<float>&,
+                                             const int32_t*, const int32_t*, int32_t, int32_t,
+                                             bool, const DevState*, hipStream_t);
+
 template void launch_sweep_colour<double>(const RowsView&, const SweepData<double>&,
                                           const int32_t*, int32_t, const DevState*, hipStream_t);
 template void launch_sweep_colour<float>(const RowsView&, const SweepData<float>&,

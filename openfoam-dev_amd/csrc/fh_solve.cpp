@@ -860,7 +860,11 @@ static void ensure_colours(fh_mesh* m)
     for (int32_t k = 0; k < nCol; k++) m->colourStart[k + 1] += m->colourStart[k];
     std::vector<int32_t> rows(n), fill(m->colourStart.begin(), m->colourStart.end() - 1);
     for (int32_t r = 0; r < n; r++) rows[fill[colour[m->perm[r]]]++] = r;
+    m->maxColourRows = 0;
+    for (int32_t k = 0; k < nCol; k++)
+        m->maxColourRows = std::max(m->maxColourRows, m->colourStart[k + 1] - m->colourStart[k]);
     m->d_colourRows.upload(rows, m->setupStream);
+    m->d_colourStart.upload(m->colourStart, m->setupStream);
     FH_HIP(hipStreamSynchronize(m->setupStream));
     m->coloured = true;
 }
@@ -892,6 +896,14 @@ void smooth_t(fh_matrix* M, int smoother, T* psi, const T* src, int32_t nSweeps,
             const std::vector<int32_t>& cs = M->mesh->colourStart;
             const int32_t nCol = (int32_t)cs.size() - 1;
             const int32_t* rows = M->mesh->d_colourRows.p;
+            // small meshes (at most four rows per thread in every colour):
+            // all sweeps in one workgroup launch
+            if (M->mesh->maxColourRows <= 4096)
+            {
+                launch_sweep_colours_wg<T>(M->rows(), d, rows, M->mesh->d_colourStart.p, nCol,
+                                           nSweeps, sym, st, M->stream);
+                return;
+            }
             for (int32_t s = 0; s < nSweeps; s++)
             {
                 for (int32_t k = 0; k < nCol; k++)
