@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
@@ -863,6 +864,10 @@ static void ensure_colours(fh_mesh* m)
     m->maxColourRows = 0;
     for (int32_t k = 0; k < nCol; k++)
         m->maxColourRows = std::max(m->maxColourRows, m->colourStart[k + 1] - m->colourStart[k]);
+    static const bool verbose = std::getenv("FOAMHIP_GAMG_VERBOSE") != nullptr;
+    if (verbose)
+        std::fprintf(stderr, "foamhip colours: %d cells, %d colours, largest %d rows\n", n, nCol,
+                     m->maxColourRows);
     m->d_colourRows.upload(rows, m->setupStream);
     m->d_colourStart.upload(m->colourStart, m->setupStream);
     FH_HIP(hipStreamSynchronize(m->setupStream));
