@@ -195,6 +195,12 @@ def main():
     traffic = measured_traffic(dom, n)
     copy_gbs = measured_copy(torch)
     log(f"device copy of 1 GiB: {copy_gbs:.0f} GB/s (read + write)")
+    stream = {}
+    for name, pat in (("read", 0), ("copy", 1), ("3r1w", 2)):
+        v = ctypes.c_double()
+        fh._check(fh.lib().fh_profile_stream(pat, 1 << 30, ctypes.byref(v)))
+        stream[name] = v.value / 1e9
+    log("stream kernels, GB/s: " + ", ".join(f"{k} {v:.0f}" for k, v in stream.items()))
 
     gamg = gamg_comparison(fh, torch, a.gamg_n) if a.gamg_n > 0 else None
 
@@ -265,6 +271,8 @@ def main():
             "spmv_GBs": live["spmv"]["GBs"],
             "spmv_frac": live["spmv"]["GBs"] / HBM_PEAK_GBS,
             "spmv_frac_of_copy": live["spmv"]["GBs"] / copy_gbs,
+            "stream_GBs": stream,
+            "spmv_frac_of_stream_3r1w": live["spmv"]["GBs"] / stream["3r1w"],
         },
         "cpu_baseline": cpu,
     }

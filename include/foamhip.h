@@ -248,6 +248,19 @@ int fh_profile_kernel(fh_matrix* matrix, int32_t kernel, int32_t precision,
 int fh_kernel_timing(fh_matrix* matrix, uint32_t kernelMask);
 int fh_kernel_timing_get(fh_matrix* matrix, int32_t kernel, double* msTotal,
                          int64_t* launches);
+/* Streaming bandwidth of the current device (no reference counterpart; the
+   achievable figure beside the published peak): a grid-stride kernel over
+   bytes-sized arrays with 16-byte accesses, best of five launches, in
+   bytes moved per second.  pattern: FH_STREAM_READ (one array read),
+   FH_STREAM_COPY (read one, write one), FH_STREAM_3R1W (read three, write
+   one: the mix of the box product). */
+enum
+{
+    FH_STREAM_READ = 0,
+    FH_STREAM_COPY = 1,
+    FH_STREAM_3R1W = 2
+};
+int fh_profile_stream(int32_t pattern, int64_t bytes, double* bytesPerSecond);
 
 /* ---- finite-volume assembly on the device ------------------------------
    Boundary faces of all non-coupled patches, flattened in patch order

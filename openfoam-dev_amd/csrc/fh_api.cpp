@@ -508,4 +508,44 @@ int fh_kernel_timing_get(fh_matrix* M, int32_t kernel, double* msTotal, int64_t*
     });
 }
 
+int fh_profile_stream(int32_t pattern, int64_t bytes, double* bytesPerSecond)
+{
+    return guarded([&] {
+        if (!bytesPerSecond) fail(FH_ERR_INVALID, "NULL output");
+        if (pattern < 0 || pattern > 2) fail(FH_ERR_INVALID, "unknown stream pattern %d", pattern);
+        if (bytes < (1 << 20) || (bytes & 15)) fail(FH_ERR_INVALID, "bytes must be >= 1 MiB and a multiple of 16");
+        const int nBuf = pattern == 0 ? 1 : (pattern == 1 ? 2 : 4);
+        fh::DBuf<float4> buf[4];
+        float4* p[4] = {};
+        for (int k = 0; k < nBuf; k++)
+        {
+            buf[k].alloc((size_t)bytes/16);
+            FH_HIP(hipMemset(buf[k].p, 0, (size_t)bytes));
+            p[k] = buf[k].p;
+        }
+        fh::DBuf<float> out;
+        out.alloc(1);
+        hipEvent_t e0, e1;
+        FH_HIP(hipEventCreate(&e0));
+        FH_HIP(hipEventCreate(&e1));
+        float best = 1e30f;
+        const int64_t n4 = bytes/16;
+        fh::launch_stream_probe(pattern, n4, 2048, p, out.p, nullptr);
+        for (int rep = 0; rep < 5; rep++)
+        {
+            FH_HIP(hipEventRecord(e0, nullptr));
+            fh::launch_stream_probe(pattern, n4, 2048, p, out.p, nullptr);
+            FH_HIP(hipEventRecord(e1, nullptr));
+            FH_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            FH_HIP(hipEventElapsedTime(&ms, e0, e1));
+            best = std::min(best, ms);
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        const double moved = (double)bytes*(pattern == 0 ? 1 : (pattern == 1 ? 2 : 4));
+        *bytesPerSecond = moved/(best*1e-3);
+    });
+}
+
 } // extern "C"

@@ -161,6 +161,10 @@ void launch_sum_a(const RowsView& R, const double* diag, const double* val,
                   double* sumA, hipStream_t s);
 
 // sweeps
+// streaming probes: 0 read buf[0]; 1 copy buf[0] -> buf[1]; 2 buf[0] + buf[2]
+// + buf[3] -> buf[1] (n4 16-byte elements each)
+void launch_stream_probe(int pattern, int64_t n4, int grid, float4* const buf[4], float* out,
+                         hipStream_t s);
 // One colour of a multi-colour Gauss-Seidel sweep over the listed rows
 template<class T>
 void launch_sweep_colour(const RowsView& R, const SweepData<T>& d, const int32_t* rows,

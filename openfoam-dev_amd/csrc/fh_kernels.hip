@@ -663,6 +663,50 @@ void launch_sweep(int kind, const RowsView& R, const SchedView& S, SweepSync& sy
     }
 }
 
+// ---- streaming probes (achievable bandwidth) -----------------------------
+__global__ void k_stream_read(const float4* __restrict__ a, int64_t n4, float* out)
+{
+    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int64_t i = blockIdx.x*(int64_t)blockDim.x + threadIdx.x; i < n4;
+         i += (int64_t)gridDim.x*blockDim.x)
+    {
+        const float4 v = a[i];
+        s.x += v.x; s.y += v.y; s.z += v.z; s.w += v.w;
+    }
+    // never true for the zero-filled input: keeps the loads alive
+    if (s.x + s.y + s.z + s.w == 1.0f) out[0] = s.x;
+}
+
+__global__ void k_stream_copy(const float4* __restrict__ a, float4* __restrict__ b, int64_t n4)
+{
+    for (int64_t i = blockIdx.x*(int64_t)blockDim.x + threadIdx.x; i < n4;
+         i += (int64_t)gridDim.x*blockDim.x)
+        b[i] = a[i];
+}
+
+__global__ void k_stream_3r1w(const float4* __restrict__ a, const float4* __restrict__ c,
+                              const float4* __restrict__ d, float4* __restrict__ b, int64_t n4)
+{
+    for (int64_t i = blockIdx.x*(int64_t)blockDim.x + threadIdx.x; i < n4;
+         i += (int64_t)gridDim.x*blockDim.x)
+    {
+        const float4 x = a[i], y = c[i], z = d[i];
+        b[i] = make_float4(x.x + y.x + z.x, x.y + y.y + z.y, x.z + y.z + z.z, x.w + y.w + z.w);
+    }
+}
+
+void launch_stream_probe(int pattern, int64_t n4, int grid, float4* const buf[4], float* out,
+                         hipStream_t s)
+{
+    if (pattern == 0)
+        hipLaunchKernelGGL(k_stream_read, dim3(grid), dim3(FH_BLOCK), 0, s, buf[0], n4, out);
+    else if (pattern == 1)
+        hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(FH_BLOCK), 0, s, buf[0], buf[1], n4);
+    else
+        hipLaunchKernelGGL(k_stream_3r1w, dim3(grid), dim3(FH_BLOCK), 0, s, buf[0], buf[2],
+                           buf[3], buf[1], n4);
+}
+
 // explicit instantiations
 template void launch_gather_values<double>(const RowsView&, int64_t, const double*, const double*, double*, hipStream_t);
 template void launch_gather_values<float>(const RowsView&, int64_t, const double*, const double*, float*, hipStream_t);

@@ -114,6 +114,7 @@ FH_SYMBOLS = {
                                  ctypes.POINTER(_dbl)]),
     "fh_kernel_timing": (_i32, [_vp, ctypes.c_uint32]),
     "fh_kernel_timing_get": (_i32, [_vp, _i32, ctypes.POINTER(_dbl), ctypes.POINTER(_i64)]),
+    "fh_profile_stream": (_i32, [_i32, _i64, ctypes.POINTER(_dbl)]),
     "fh_boundary_create": (_i32, [_vp, _i32, _vp, ctypes.POINTER(_vp)]),
     "fh_boundary_destroy": (_i32, [_vp]),
     "fh_assemble_laplacian": (_i32, [_vp, _vp] + [_vp] * 10 + [_vp, _i32]),
