@@ -117,15 +117,37 @@ def main():
         psi.zero_()
         return s.solve_device(psi.data_ptr(), src.data_ptr())
 
+    def assemble():
+        fh._check(fh.lib().fh_assemble_laplacian(
+            mat.handle, bnd.handle, P(D["gamma"]), P(D["weights"]), P(D["magSf"]),
+            P(D["deltaCoeffs"]), P(D["bMagSf"]), P(D["bDeltaCoeffs"]), P(D["bType"]),
+            P(D["bValue"]), P(ic), P(bc), P(src), 1))
+        torch.cuda.synchronize()
+
+    def cold(s):
+        """One solve right after the coefficients were set again (as every
+        fvMatrix::solve call sets them): the solver's whole setup -- FP32
+        copies, factors, packed coefficients, the coarse matrix and its
+        Cholesky inverse -- is inside the timed call; the assembly is not."""
+        assemble()
+        t = time.perf_counter()
+        pf = solve(s)
+        torch.cuda.synchronize()
+        return pf, time.perf_counter() - t
+
     for _ in range(a.warmup):
-        solve(mixed)
-    torch.cuda.synchronize()
-    perfs = []
-    t0 = time.perf_counter()
+        cold(mixed)
+    perfs, times = [], []
     for _ in range(a.steps):
-        perfs.append(solve(mixed))
+        pf, t = cold(mixed)
+        perfs.append(pf)
+        times.append(t)
+    t_mixed = sum(times) / a.steps
+    # the same solve with every setup cached (coefficients unchanged)
+    t0 = time.perf_counter()
+    solve(mixed)
     torch.cuda.synchronize()
-    t_mixed = (time.perf_counter() - t0) / a.steps
+    t_mixed_warm = time.perf_counter() - t0
     # live kernel timing: one more solve of the same system with HIP events
     # on the matrix's stream around every launch of the preconditioner, the
     # product and the update of the inner iterations (kept out of the timed
@@ -143,14 +165,11 @@ def main():
     fh._check(fh.lib().fh_kernel_timing(mat.handle, 0))
     x_mixed = psi.cpu().numpy()
     pm = perfs[-1]
-    log(f"mixed: {pm.line()} refinements {pm.nRefinements}, {t_mixed:.3f} s/solve; live kernel ms "
+    log(f"mixed: {pm.line()} refinements {pm.nRefinements}, {t_mixed:.4f} s/solve "
+        f"(setup cached: {t_mixed_warm:.4f} s); live kernel ms "
         + ", ".join(f"{k} {v['ms']:.4f} ({v['launches']})" for k, v in live.items()))
 
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    p64 = solve(fp64)
-    torch.cuda.synchronize()
-    t_fp64 = time.perf_counter() - t0
+    p64, t_fp64 = cold(fp64)
     x_fp64 = psi.cpu().numpy()
     # accuracy: both solutions against a tight FP64 solve of the same system
     tight = fh.solver_New("p", mat, dict(base, precision="double", tolerance=1e-13))
@@ -241,6 +260,10 @@ def main():
                         f"(sigma {a.sigma}, corr {a.corr} cells), PCG+DIC to {a.tol}, mixed "
                         "precision vs FP64",
             "cells": case.N, "faces": case.F,
+            "timing": "every timed solve follows a fresh coefficient set, so the solver "
+                      "setup (factors, packed coefficients, coarse matrix and its Cholesky "
+                      "inverse) is inside it",
+            "mixed_s_per_solve_setup_cached": t_mixed_warm,
             "fp64_s_per_solve": t_fp64,
             "speedup_mixed_vs_fp64": t_fp64 / t_mixed,
             "mixed_iterations": pm.nIterations, "mixed_refinements": pm.nRefinements,
@@ -250,7 +273,8 @@ def main():
             "mixed_error_vs_tight_fp64": err_mixed,
             "fp64_error_vs_tight_fp64": err_fp64,
             "mixed_inner_preconditioner": "tileDIC (16x16x16 block-Jacobi DIC, FP32 factor, "
-                                          "FP16 coefficients)",
+                                          "FP16 coefficients) plus the coarse correction over "
+                                          "the tiles (dense FP32 inverse of P^T A P)",
             "assembly_ms": assembly_ms,
             "serve": serve,
             "gamg": gamg,

@@ -1,0 +1,29 @@
+// Coarse correction of the tile-local preconditioner (fh_coarse.hip): the
+// coarse matrix over 16^3 tiles, its dense FP32 inverse and its apply.
+#pragma once
+
+#include "fh_kernels.h"
+#include "fh_solver_kernels.h"
+
+namespace fh
+{
+
+// per tile t: ctile[4t] = diagonal sum + twice the inner couplings,
+// ctile[4t+1..3] = couplings across the tile's lower x, y, z faces
+void launch_coarse_tiles(int32_t nx, int32_t ny, int32_t nz, int32_t nTI, int32_t nTJ,
+                         int32_t nT, const double* diag, const double* const dir[3],
+                         double* ctile, hipStream_t s);
+// dense (column-major) sign*Ac
+void launch_coarse_dense(int32_t nT, int32_t nTI, int32_t nTJ, const double* ctile, double sign,
+                         double* dense, hipStream_t s);
+// full FP32 sign*inv from the lower triangle of the factored inverse, rows of
+// ld floats
+void launch_coarse_inv32(int32_t nT, int32_t ld, const double* dense, double sign, float* inv,
+                         hipStream_t s);
+// z = inv rc, and sum z*rc plus the nF fine partials of sum w*r, finalized
+// by the last block with F (FIN_PCG_WARA); returns the block count
+int32_t launch_coarse_apply(int32_t nT, int32_t ld, const float* inv, const double* rc,
+                            float* z, const double* fine, int32_t nF, double* partials,
+                            const FusedFinalize& F, const DevState* st, hipStream_t s);
+
+} // namespace fh

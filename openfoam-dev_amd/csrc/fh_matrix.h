@@ -48,6 +48,15 @@ struct fh_matrix
     bool dir64Valid = false;
     // tile depth of the global preconditioner sweeps on boxes
     int32_t preBz64 = 0, preBz32 = 0;
+    // coarse correction of the tile-local preconditioner (mixed PCG on
+    // symmetric boxes): the coarse matrix over 16^3 tiles as a dense FP32
+    // inverse (rows of cLd floats); state 0 not built, 1 ready, -1 not used
+    int coarseState = 0;
+    int32_t cN = 0, cLd = 0;
+    fh::DBuf<float> d_cinv, d_cz;
+    fh::DBuf<double> d_cdense, d_ctile, d_crc;
+    fh::DBuf<int32_t> d_cinfo;
+    void* cblas = nullptr;                 // rocblas handle of the factorization
 
     // preconditioner / smoother factors (library numbering)
     // index: 0 DIC/FDIC, 1 DILU, 2 diagonal, 3 tile-local DIC, 4 tile-local DILU
@@ -205,7 +214,7 @@ const float* ensure_rD32(fh_matrix* M, int which);
 template<class T>
 int32_t apply_precond(fh_matrix* M, int precon, const T* r, T* w, double* partials,
                       DevState* st, bool transpose, const fh::FusedFinalize& F = {},
-                      bool* fused = nullptr);
+                      bool* fused = nullptr, double* tileSum = nullptr);
 
 void profile_kernel(fh_matrix* M, int kernel, int precision, int precon, int reps,
                     double* ms, double* bytes);

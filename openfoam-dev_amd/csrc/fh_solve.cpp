@@ -5,6 +5,9 @@
 #include "fh_matrix.h"
 #include "fh_gamg.h"
 #include "fh_gamg_kernels.h"
+#include "fh_coarse.h"
+
+#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <cmath>
@@ -92,6 +95,8 @@ void matrix_free(fh_matrix* M)
 {
     if (M->stream) (void)hipStreamSynchronize(M->stream);
     gamg_release(M);
+    if (M->cblas) (void)rocblas_destroy_handle(static_cast<rocblas_handle>(M->cblas));
+    M->cblas = nullptr;
     if (M->h_state) (void)hipHostFree(M->h_state);
     M->h_state = nullptr;
     if (M->evStart) (void)hipEventDestroy(M->evStart);
@@ -157,6 +162,7 @@ void matrix_coeffs_updated(fh_matrix* M)
     M->hasCoeffs = true;
     M->valTValid = M->sumAValid = M->f32Valid = M->dir32Valid = M->dir64Valid = false;
     M->dir16Valid = false;
+    M->coarseState = 0;
     for (int i = 0; i < 5; i++) M->rDValid[i] = M->rD32Valid[i] = false;
     gamg_invalidate(M);
 }
@@ -486,7 +492,8 @@ static int32_t box_precond_pass(fh_matrix* M, bool backward, const SweepData<T>&
 
 template<class T>
 int32_t apply_precond(fh_matrix* M, int precon, const T* r, T* w, double* partials,
-                      DevState* st, bool transpose, const FusedFinalize& F, bool* fused)
+                      DevState* st, bool transpose, const FusedFinalize& F, bool* fused,
+                      double* tileSum)
 {
     const int32_t N = M->mesh->nCells;
     if (fused) *fused = false;
@@ -549,7 +556,8 @@ int32_t apply_precond(fh_matrix* M, int precon, const T* r, T* w, double* partia
                 const int32_t nb = launch_tile_local_apply_f32(G, r, d.rD, dir, M->symmetric, w,
                                                                st, M->stream, partials,
                                                                half ? h3 : nullptr,
-                                                               fuse ? F : FusedFinalize());
+                                                               fuse ? F : FusedFinalize(),
+                                                               tileSum);
                 if (nb >= 0)
                 {
                     if (fused) *fused = fuse;
@@ -577,9 +585,9 @@ int32_t apply_precond(fh_matrix* M, int precon, const T* r, T* w, double* partia
 }
 
 template int32_t apply_precond<double>(fh_matrix*, int, const double*, double*, double*, DevState*,
-                                       bool, const FusedFinalize&, bool*);
+                                       bool, const FusedFinalize&, bool*, double*);
 template int32_t apply_precond<float>(fh_matrix*, int, const float*, float*, double*, DevState*,
-                                      bool, const FusedFinalize&, bool*);
+                                      bool, const FusedFinalize&, bool*, double*);
 
 void state_upload(fh_matrix* M, const fh_controls* c, double tol, double relTol,
                          int32_t maxIter, int32_t minIter)
@@ -998,6 +1006,74 @@ static void smooth_solver(fh_matrix* M, const fh_controls* c, double* psi, const
 // one.  The recurrence is never restarted, so the iteration count stays
 // close to the FP64 one.
 // ---------------------------------------------------------------------------
+// The coarse correction of the tile-local inner preconditioner: the coarse
+// matrix over the 16^3 tiles of a symmetric box (at most 4096 tiles, else
+// the inner preconditioner stays one-level), factored (Cholesky of -Ac,
+// rocSOLVER, FP64) and inverted once per coefficient update.
+// FOAMHIP_TILE_COARSE=0 turns it off.
+static bool ensure_coarse(fh_matrix* M)
+{
+    static const bool off = [] {
+        const char* e = std::getenv("FOAMHIP_TILE_COARSE");
+        return e && std::strcmp(e, "0") == 0;
+    }();
+    if (M->coarseState != 0) return M->coarseState > 0;
+    M->coarseState = -1;
+    const fh_mesh* mesh = M->mesh;
+    if (off || !mesh->tiled || !M->symmetric || (mesh->boxNx & 15)) return false;
+    const int32_t nTI = (mesh->boxNx + 15)/16, nTJ = (mesh->boxNy + 15)/16;
+    const int32_t nTK = (mesh->boxNz + 15)/16;
+    const int64_t nT = (int64_t)nTI*nTJ*nTK;
+    if (nT < 2 || nT > 4096) return false;
+    const int32_t n = (int32_t)nT, ld = (n + 3) & ~3;
+    // room for the fine partials and the coarse kernel's partials
+    const int64_t nF = (nT + 3)/4, nC = (nT + 3)/4;
+    if (2*nF + (int64_t)fin_partials(nC) > (int64_t)M->d_partials.n
+        || (int64_t)fin_counters(nC) > (int64_t)M->d_finCount.n)
+        return false;
+    hipStream_t s = M->stream;
+    if (!M->cblas)
+    {
+        rocblas_handle h = nullptr;
+        if (rocblas_create_handle(&h) != rocblas_status_success)
+            fail(FH_ERR_HIP, "rocblas_create_handle failed");
+        M->cblas = h;
+    }
+    rocblas_handle h = static_cast<rocblas_handle>(M->cblas);
+    if (rocblas_set_stream(h, s) != rocblas_status_success)
+        fail(FH_ERR_HIP, "rocblas_set_stream failed");
+    ensure_dir64(M);
+    const double* dir[3] = {M->d_dir64[0].p, M->d_dir64[1].p, M->d_dir64[2].p};
+    M->d_ctile.alloc(4*(size_t)n);
+    M->d_cdense.alloc((size_t)n*n);
+    M->d_cinv.alloc((size_t)n*ld);
+    M->d_crc.alloc((size_t)n);
+    M->d_cz.alloc((size_t)n);
+    M->d_cinfo.alloc(2);
+    launch_coarse_tiles(mesh->boxNx, mesh->boxNy, mesh->boxNz, nTI, nTJ, n, M->d_diag.p, dir,
+                        M->d_ctile.p, s);
+    // the Laplacian is negative definite: factor -Ac (sign from a diagonal)
+    double d0 = 0.0;
+    FH_HIP(hipMemcpyAsync(&d0, M->d_ctile.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    FH_HIP(hipStreamSynchronize(s));
+    const double sign = d0 < 0.0 ? -1.0 : 1.0;
+    launch_coarse_dense(n, nTI, nTJ, M->d_ctile.p, sign, M->d_cdense.p, s);
+    if (rocsolver_dpotrf(h, rocblas_fill_lower, n, M->d_cdense.p, n, M->d_cinfo.p)
+            != rocblas_status_success
+        || rocsolver_dpotri(h, rocblas_fill_lower, n, M->d_cdense.p, n, M->d_cinfo.p + 1)
+            != rocblas_status_success)
+        fail(FH_ERR_HIP, "rocSOLVER Cholesky of the coarse matrix failed to launch");
+    int32_t info[2] = {0, 0};
+    FH_HIP(hipMemcpyAsync(info, M->d_cinfo.p, sizeof(info), hipMemcpyDeviceToHost, s));
+    FH_HIP(hipStreamSynchronize(s));
+    if (info[0] != 0 || info[1] != 0) return false;      // not definite: one-level
+    launch_coarse_inv32(n, ld, M->d_cdense.p, sign, M->d_cinv.p, s);
+    M->cN = n;
+    M->cLd = ld;
+    M->coarseState = 1;
+    return true;
+}
+
 static void mixed_pcg(fh_matrix* M, const fh_controls* c, double* psi, const double* src,
                       fh_perf* perf)
 {
@@ -1045,10 +1121,29 @@ static void mixed_pcg(fh_matrix* M, const fh_controls* c, double* psi, const dou
     // A p: in place of w when p is formed by its own kernel; a separate
     // buffer when the product reads w to form p
     float* ap = fuseP ? M->f[4].p : w32;
+    // two-level inner preconditioner on boxes (tile-local DIC plus the tile
+    // coarse correction, carried into p by the product)
+    const bool coarse = precon == FH_PRECON_TILE_DIC && fuseP && ensure_coarse(M);
     run_batches(M, c->maxIter, [&](int32_t seq)
     {
         kt_mark(M, FH_KERNEL_PRECONDITION);
-        precond_finalize<float>(M, precon, r32, w32, FIN_PCG_WARA);
+        if (coarse)
+        {
+            // two-level: the tile-local apply leaves its w.r partials and the
+            // tile sums of r; the coarse solve adds P Ac^-1 P^T r (through
+            // p in the product) and finalizes the whole w.r
+            const int32_t nbw = apply_precond<float>(M, precon, r32, w32, part, st, false,
+                                                     FusedFinalize(), nullptr, M->d_crc.p);
+            const FusedFinalize fc = fused_fin(M, FIN_PCG_WARA);
+            double* partC = part + 2*(size_t)nbw;
+            const int32_t nbc = launch_coarse_apply(M->cN, M->cLd, M->d_cinv.p, M->d_crc.p,
+                                                    M->d_cz.p, part, nbw, partC, fc, st,
+                                                    M->stream);
+            if (!fc.counter)
+                launch_finalize_blocks(nbc, N, partC, FIN_PCG_WARA, st, 0, M->stream);
+        }
+        else
+            precond_finalize<float>(M, precon, r32, w32, FIN_PCG_WARA);
         kt_mark(M, FH_KERNEL_PRECONDITION);
         if (fuseP)
         {
@@ -1058,7 +1153,8 @@ static void mixed_pcg(fh_matrix* M, const fh_controls* c, double* psi, const dou
             const int32_t nb = launch_box_pcg_spmv<float>(mesh->boxNx, mesh->boxNy, mesh->boxNz,
                                                        diag, dir, M->symmetric, w32,
                                                        pBuf[seq & 1], p32, ap, part, st,
-                                                       M->stream, fa);
+                                                       M->stream, fa,
+                                                       coarse ? M->d_cz.p : nullptr);
             kt_mark(M, FH_KERNEL_SPMV);
             if (!fa.counter)
                 launch_finalize_blocks(nb, N, part, FIN_PCG_WAPA, st, 0, M->stream);

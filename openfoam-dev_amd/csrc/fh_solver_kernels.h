@@ -77,7 +77,7 @@ int32_t launch_box_pcg_spmv(int32_t nx, int32_t ny, int32_t nz, const T* diag,
                             const T* const dir[6], bool symmetric, const T* w,
                             const T* pOld, T* pNew, T* y, double* partials,
                             const DevState* st, hipStream_t s,
-                            const FusedFinalize& F = FusedFinalize());
+                            const FusedFinalize& F = FusedFinalize(), const T* zc = nullptr);
 // finalize over an explicit number of partial blocks
 void launch_finalize_blocks(int32_t nBlocks, int32_t n, const double* partials, int32_t op,
                             DevState* st, int32_t seq, hipStream_t s);

@@ -534,7 +534,8 @@ void k_box_spmv_slab(int32_t nx, int32_t ny, int32_t nz, int32_t kc,
                      const T* __restrict__ uJ, const T* __restrict__ uK,
                      const T* __restrict__ pOld, T* __restrict__ y,
                      double* partials, const DevState* st,
-                     const T* __restrict__ wv, T* __restrict__ pNew, FusedFinalize F)
+                     const T* __restrict__ wv, T* __restrict__ pNew, FusedFinalize F,
+                     const T* __restrict__ zc, int32_t nTI, int32_t nTJ)
 {
     if (st->done) return;
     typedef Quad<T> Q;
@@ -568,16 +569,32 @@ void k_box_spmv_slab(int32_t nx, int32_t ny, int32_t nz, int32_t kc,
         const int hslot = r == 0 ? 0 : ROWS + 1;
         const int64_t rowc = (int64_t)(ok ? i0 : 0) + (int64_t)nx*(ok ? gj : 0);
         const int64_t rowh = (int64_t)(halo ? i0 : 0) + (int64_t)nx*(halo ? hj : 0);
-        auto px4 = [&](int64_t i, T (&o)[4])
+        // zc: a coarse correction per 16^3 tile added to w (two-level
+        // preconditioner): p = (w + zc) + beta*pOld
+        auto zt = [&](int32_t xx, int32_t yy, int32_t kk) -> T
+        {
+            return zc ? zc[(xx >> 4) + nTI*((yy >> 4) + nTJ*(kk >> 4))] : T(0);
+        };
+        auto px4 = [&](int64_t i, int32_t yy, int32_t kk, T (&o)[4])
         {
             Q::load(wv, i, o);
+            if (zc)
+            {
+                const T zv = zt(i0, yy, kk);
+#pragma unroll
+                for (int e = 0; e < 4; e++) o[e] = o[e] + zv;
+            }
             if (first) return;
             T pq[4];
             Q::load(pOld, i, pq);
 #pragma unroll
             for (int e = 0; e < 4; e++) o[e] = o[e] + beta*pq[e];
         };
-        auto px1 = [&](int64_t i) -> T { return first ? wv[i] : wv[i] + beta*pOld[i]; };
+        auto px1 = [&](int64_t i, int32_t xx, int32_t kk) -> T
+        {
+            const T wz = zc ? wv[i] + zt(xx, gj, kk) : wv[i];
+            return first ? wz : wz + beta*pOld[i];
+        };
         // every input of plane k+1 is loaded while plane k is computed
         T xm[4] = {}, x0[4] = {}, xn[4] = {}, lkc[4] = {}, lkn[4] = {};
         T dg[4] = {}, lj[4] = {}, li[4] = {}, hx[4] = {}, hl[4] = {};
@@ -585,8 +602,8 @@ void k_box_spmv_slab(int32_t nx, int32_t ny, int32_t nz, int32_t kc,
         if (ok)
         {
             const int64_t c = rowc + sxy*k0;
-            if (k0 > 0) px4(c - sxy, xm);
-            px4(c, x0);
+            if (k0 > 0) px4(c - sxy, gj, k0 - 1, xm);
+            px4(c, gj, k0, x0);
             Q::load(lK, c, lkc);
             Q::load(diag, c, dg);
             Q::load(lJ, c, lj);
@@ -594,7 +611,7 @@ void k_box_spmv_slab(int32_t nx, int32_t ny, int32_t nz, int32_t kc,
         }
         if (halo)
         {
-            px4(rowh + sxy*k0, hx);
+            px4(rowh + sxy*k0, hj, k0, hx);
             if (sym) Q::load(lJ, rowh + sxy*k0, hl);
         }
         for (int32_t k = k0; k < k1; k++)
@@ -604,7 +621,7 @@ void k_box_spmv_slab(int32_t nx, int32_t ny, int32_t nz, int32_t kc,
             const bool more = k + 1 < k1;
             if (ok && up)
             {
-                px4(c + sxy, xn);
+                px4(c + sxy, gj, k + 1, xn);
                 if (sym) Q::load(lK, c + sxy, lkn);
                 if (more)
                 {
@@ -615,7 +632,7 @@ void k_box_spmv_slab(int32_t nx, int32_t ny, int32_t nz, int32_t kc,
             }
             if (halo && more)
             {
-                px4(rowh + sxy*(k + 1), hxn);
+                px4(rowh + sxy*(k + 1), hj, k + 1, hxn);
                 if (sym) Q::load(lJ, rowh + sxy*(k + 1), hln);
             }
             // publish this plane's row (and the halo row) to the block
@@ -638,8 +655,10 @@ void k_box_spmv_slab(int32_t nx, int32_t ny, int32_t nz, int32_t kc,
                 }
                 // i neighbours: inside the 256-cell row from LDS, across
                 // i-blocks from memory
-                const T xw = i0 == 0 ? T(0) : (q > 0 ? sX[r + 1][4*q - 1] : px1(c - 1));
-                const T xe = i0 + 4 >= nx ? T(0) : (q < 63 ? sX[r + 1][4*q + 4] : px1(c + 4));
+                const T xw = i0 == 0 ? T(0)
+                           : (q > 0 ? sX[r + 1][4*q - 1] : px1(c - 1, i0 - 1, k));
+                const T xe = i0 + 4 >= nx ? T(0)
+                           : (q < 63 ? sX[r + 1][4*q + 4] : px1(c + 4, i0 + 4, k));
                 if (!sym)
                 {
                     Q::load(lK, c, lkk);
@@ -726,7 +745,8 @@ template<class T>
 int32_t launch_box_pcg_spmv(int32_t nx, int32_t ny, int32_t nz, const T* diag,
                             const T* const dir[6], bool symmetric, const T* w,
                             const T* pOld, T* pNew, T* y, double* partials,
-                            const DevState* st, hipStream_t s, const FusedFinalize& F)
+                            const DevState* st, hipStream_t s, const FusedFinalize& F,
+                            const T* zc)
 {
     if (nx & 3) return -1;
     static const bool plane = [] {
@@ -758,7 +778,7 @@ int32_t launch_box_pcg_spmv(int32_t nx, int32_t ny, int32_t nz, const T* diag,
         hipLaunchKernelGGL((k_box_spmv_slab<T, ROWS>), dim3(nb), dim3(64*ROWS), 0, s, nx, ny, nz,
                            kc, diag, dir[0], dir[1], dir[2], symmetric ? nullptr : dir[3],
                            symmetric ? nullptr : dir[4], symmetric ? nullptr : dir[5], pOld, y,
-                           partials, st, w, pNew, F);
+                           partials, st, w, pNew, F, zc, (nx + 15)/16, (ny + 15)/16);
         return nb;
     }
     const int32_t kc = box_march_planes(nx, ny, nz);
@@ -775,12 +795,12 @@ template int32_t launch_box_pcg_spmv<float>(int32_t, int32_t, int32_t, const flo
                                             const float* const[6], bool, const float*,
                                             const float*, float*, float*, double*,
                                             const DevState*, hipStream_t,
-                                            const FusedFinalize&);
+                                            const FusedFinalize&, const float*);
 template int32_t launch_box_pcg_spmv<double>(int32_t, int32_t, int32_t, const double*,
                                              const double* const[6], bool, const double*,
                                              const double*, double*, double*, double*,
                                              const DevState*, hipStream_t,
-                                             const FusedFinalize&);
+                                             const FusedFinalize&, const double*);
 
 int32_t launch_box_spmv_dot_f32(int32_t nx, int32_t ny, int32_t nz, const float* diag,
                                 const float* const dir[6], bool symmetric, const float* x,

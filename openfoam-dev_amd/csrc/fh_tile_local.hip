@@ -323,7 +323,8 @@ __device__ __forceinline__ void tl_load(const TlTile& T, int l16, int step, cons
 
 template<bool BWD, int BZ, bool SHIFT, bool DOT, class CT>
 __device__ __forceinline__ void tl_step(const TlTile& T, int l16, int step, const TlRow<CT>& c,
-                                        float (*bufg)[BZ + 1][16], float* w, double& dotWR)
+                                        float (*bufg)[BZ + 1][16], float* w, double& dotWR,
+                                        double& rsum)
 {
     int j, k;
     const bool act = tl_row<BWD>(T, l16, step, j, k);
@@ -378,6 +379,12 @@ __device__ __forceinline__ void tl_step(const TlTile& T, int l16, int step, cons
     {
         const double t = (st[0] + st[1]) + (st[2] + st[3]);
         dotWR += act ? t : 0.0;
+        // the row's sum of r (the tile sums of the coarse correction)
+        double sr[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int i = 0; i < 16; i++) sr[i & 3] += (double)q4(c.A, i);
+        const double u = (sr[0] + sr[1]) + (sr[2] + sr[3]);
+        rsum += act ? u : 0.0;
     }
     if constexpr (BWD && DOT)
     {
@@ -408,7 +415,7 @@ template<bool BWD, int BZ, bool SHIFT, bool DOT, class CT>
 __device__ __forceinline__ void tl_walk(const TlTile& T, int l16, int nD, const float* a,
                                         const float* rD, const TlCoef<CT>& C,
                                         float (*bufg)[BZ + 1][16], float* w, const float* r,
-                                        double& dotWR)
+                                        double& dotWR, double& rsum)
 {
     TlRow<CT> r0, r1;
     tl_load<BWD, SHIFT, DOT>(T, l16, 0, a, rD, C, r0, r);
@@ -422,12 +429,12 @@ __device__ __forceinline__ void tl_walk(const TlTile& T, int l16, int nD, const 
     {
         tl_load<BWD, SHIFT, DOT>(T, l16, step + 1, a, rD, C, r1, r);
         __builtin_amdgcn_sched_barrier(0);
-        tl_step<BWD, BZ, SHIFT, DOT>(T, l16, step, r0, bufg, w, dotWR);
+        tl_step<BWD, BZ, SHIFT, DOT>(T, l16, step, r0, bufg, w, dotWR, rsum);
         tl_load<BWD, SHIFT, DOT>(T, l16, step + 2, a, rD, C, r0, r);
         __builtin_amdgcn_sched_barrier(0);
-        tl_step<BWD, BZ, SHIFT, DOT>(T, l16, step + 1, r1, bufg, w, dotWR);
+        tl_step<BWD, BZ, SHIFT, DOT>(T, l16, step + 1, r1, bufg, w, dotWR, rsum);
     }
-    if (step < nD) tl_step<BWD, BZ, SHIFT, DOT>(T, l16, step, r0, bufg, w, dotWR);
+    if (step < nD) tl_step<BWD, BZ, SHIFT, DOT>(T, l16, step, r0, bufg, w, dotWR, rsum);
 }
 
 // PASSES: 1 forward, 2 backward, 3 forward then backward; TPW tiles per
@@ -438,7 +445,8 @@ template<int PASSES, int BZ, int TPW, bool SYM, class CT, bool DOT>
 __global__ __launch_bounds__(64)
 void k_tile_local_vec(TileView G, const float* x, const float* __restrict__ rD,
                       TlCoef<CT> lo, TlCoef<CT> up, float* w, const DevState* st,
-                      const float* __restrict__ r, double* partials, FusedFinalize F)
+                      const float* __restrict__ r, double* partials, FusedFinalize F,
+                      double* tileSum)
 {
     if (st && st->done) return;
     __shared__ __align__(16) float buf[TPW][2][BZ + 1][16];
@@ -471,11 +479,23 @@ void k_tile_local_vec(TileView G, const float* x, const float* __restrict__ rD,
     // does not read r.  Otherwise the backward pass sums w*r.
     constexpr bool dotFwd = wantDot && SYM && PASSES == 3;
     constexpr bool dotBwd = wantDot && !dotFwd;
+    double rsum = 0.0;
     if constexpr ((PASSES & 1) != 0)
-        tl_walk<false, BZ, false, dotFwd>(T, l16, nD, x, rD, lo, bufg, w, r, dotWR);
+        tl_walk<false, BZ, false, dotFwd>(T, l16, nD, x, rD, lo, bufg, w, r, dotWR, rsum);
     // (the walk ends on a workgroup barrier: the forward rows are visible)
     if constexpr ((PASSES & 2) != 0)
-        tl_walk<true, BZ, SYM, dotBwd>(T, l16, nD, w, rD, up, bufg, w, r, dotWR);
+        tl_walk<true, BZ, SYM, dotBwd>(T, l16, nD, w, rD, up, bufg, w, r, dotWR, rsum);
+    if constexpr (dotFwd)
+    {
+        // tileSum: the sum of r over each tile, the 16 lanes of its group
+        // in fixed butterfly order
+        if (tileSum)
+        {
+#pragma unroll
+            for (int off = 8; off > 0; off >>= 1) rsum += __shfl_xor(rsum, off, 64);
+            if (l16 == 0 && T.ok) tileSum[tile] = rsum;
+        }
+    }
     if constexpr (wantDot)
     {
         // one partial per wavefront (= workgroup), fixed butterfly order
@@ -490,20 +510,21 @@ template<int PASSES, int BZ, bool SYM, class CT>
 static void tlv_launch(dim3 grid, hipStream_t s, const TileView& G, const float* x,
                        const float* rD, const TlCoef<CT>& lo, const TlCoef<CT>& up, float* w,
                        const DevState* st, const float* r, double* partials,
-                       const FusedFinalize& F)
+                       const FusedFinalize& F, double* tileSum = nullptr)
 {
     if (partials)
         hipLaunchKernelGGL((k_tile_local_vec<PASSES, BZ, 4, SYM, CT, true>), grid, dim3(64), 0,
-                           s, G, x, rD, lo, up, w, st, r, partials, F);
+                           s, G, x, rD, lo, up, w, st, r, partials, F, tileSum);
     else
         hipLaunchKernelGGL((k_tile_local_vec<PASSES, BZ, 4, SYM, CT, false>), grid, dim3(64), 0,
-                           s, G, x, rD, lo, up, w, st, r, partials, F);
+                           s, G, x, rD, lo, up, w, st, r, partials, F, tileSum);
 }
 
 int32_t launch_tile_local_apply_f32(const TileView& G, const float* x, const float* rD,
                                     const float* const dir[6], bool symmetric, float* w,
                                     const DevState* st, hipStream_t s, double* partials,
-                                    const __half* const half3[3], const FusedFinalize& F)
+                                    const __half* const half3[3], const FusedFinalize& F,
+                                    double* tileSum)
 {
     if (G.bx != 16 || G.by != 16 || G.bz != TileBZ<float>::pre || (G.nx & 15))
         return -1;
@@ -513,14 +534,14 @@ int32_t launch_tile_local_apply_f32(const TileView& G, const float* x, const flo
     if (symmetric && half3)
     {
         const TlCoef<__half> h{half3[0], half3[1], half3[2]};
-        tlv_launch<3, BZ, true>(grid, s, G, x, rD, h, h, w, st, x, partials, F);
+        tlv_launch<3, BZ, true>(grid, s, G, x, rD, h, h, w, st, x, partials, F, tileSum);
         return (int32_t)grid.x;
     }
     const TlCoef<float> lo{dir[0], dir[1], dir[2]};
     const TlCoef<float> up = symmetric ? TlCoef<float>{dir[0], dir[1], dir[2]}
                                        : TlCoef<float>{dir[5], dir[4], dir[3]};
     if (symmetric)
-        tlv_launch<3, BZ, true>(grid, s, G, x, rD, lo, up, w, st, x, partials, F);
+        tlv_launch<3, BZ, true>(grid, s, G, x, rD, lo, up, w, st, x, partials, F, tileSum);
     else
         tlv_launch<3, BZ, false>(grid, s, G, x, rD, lo, up, w, st, x, partials, F);
     return (int32_t)grid.x;
@@ -575,7 +596,7 @@ int32_t launch_tile_local_f32(bool backward, const TileView& G, const float* a, 
         else
             hipLaunchKernelGGL((k_tile_local_vec<1, TileBZ<float>::pre, 4, false, float, false>),
                                grid, dim3(64), 0, s, G, a, rD, c, c, w, st, nullptr, nullptr,
-                               FusedFinalize());
+                               FusedFinalize(), nullptr);
         return (int32_t)grid.x;
     }
     if (backward)

@@ -70,7 +70,8 @@ int32_t launch_tile_local_apply_f32(const TileView& G, const float* x, const flo
                                     const float* const dir[6], bool symmetric, float* w,
                                     const DevState* st, hipStream_t s, double* partials,
                                     const __half* const half3[3] = nullptr,
-                                    const FusedFinalize& F = FusedFinalize());
+                                    const FusedFinalize& F = FusedFinalize(),
+                                    double* tileSum = nullptr);
 // FP16 copies of three FP32 arrays of n values
 void launch_pack_half(int64_t n, const float* const src[3], __half* const dst[3], hipStream_t s);
 // b = a rounded to FP16 precision (kept in FP32)

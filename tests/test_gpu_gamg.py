@@ -279,13 +279,26 @@ def test_mixed_pcg_tile_dic_inner():
         assert abs(p8.nIterations - exact[inner]) <= max(3, 0.05 * exact[inner])
 
 
+def _tile_coarse(om, n, nx, ny, nz):
+    """The coarse correction over 16^3 tiles in numpy: P (cell -> tile
+    indicator) and the inverse of Ac = P^T A P, from the FP64 matrix."""
+    c = np.arange(n)
+    nTI, nTJ = (nx + 15) // 16, (ny + 15) // 16
+    tile = (c % nx) // 16 + nTI * (((c // nx) % ny) // 16 + nTJ * ((c // (nx * ny)) // 16))
+    nT = tile.max() + 1
+    AP = np.stack([om.Amul((tile == t).astype(float)) for t in range(nT)], axis=1)
+    Ac = np.stack([AP[tile == t].sum(axis=0) for t in range(nT)])
+    return tile, nT, np.linalg.inv(Ac)
+
+
 def test_mixed_pcg_tile_dic_half():
     """Mixed PCG on a box whose tile-local inner preconditioner stores its
-    coefficients in FP16 (nx a multiple of 16): the FP64 residual test holds,
-    the solution matches the FP64 oracle at a tight tolerance, and at the
-    working tolerance it takes about the iterations of PCG with that
-    preconditioner (the FP16-rounded, 16^3 block-Jacobi DIC) in exact
-    arithmetic."""
+    coefficients in FP16 (nx a multiple of 16) and carries the coarse
+    correction over the tiles: the FP64 residual test holds, the solution
+    matches the FP64 oracle at a tight tolerance, and at the working
+    tolerance it takes about the iterations of PCG with that preconditioner
+    (the FP16-rounded 16^3 block-Jacobi DIC plus P Ac^-1 P^T) in exact
+    arithmetic -- half the one-level count here."""
     c, a, diag, upper, b = laplacian_case(48, sigma=1.0)
     om = O.Matrix(a, diag, upper)
     mesh = fh.lduAddressing(c.N, c.lower, c.upper)
@@ -293,7 +306,15 @@ def test_mixed_pcg_tile_dic_half():
     cut = _tile_blocks(48, 48, 48, c.lower, c.upper, 16)
     up16 = upper.astype(np.float32).astype(np.float16).astype(np.float64)
     om_t = O.Matrix(O.Addr(c.N, c.lower, c.upper), diag, np.where(cut, 0.0, up16))
-    exact = _numpy_pcg(om, b, lambda r: om_t.precondition("DIC", r), 1e-8)
+    tile, nT, Aci = _tile_coarse(om, c.N, 48, 48, 48)
+
+    def two_level(r):
+        rc = np.bincount(tile, r, nT)
+        return om_t.precondition("DIC", r) + (Aci @ rc)[tile]
+
+    exact = _numpy_pcg(om, b, two_level, 1e-8)
+    one_level = _numpy_pcg(om, b, lambda r: om_t.precondition("DIC", r), 1e-8)
+    assert exact < 0.75 * one_level
     ctl = {"solver": "PCG", "preconditioner": "DIC", "tolerance": 1e-12, "relTol": 0,
            "maxIter": 3000, "precision": "mixed"}
     psi = np.zeros(c.N)
