@@ -13,13 +13,14 @@ namespace fh
 void launch_coarse_tiles(int32_t nx, int32_t ny, int32_t nz, int32_t nTI, int32_t nTJ,
                          int32_t nT, const double* diag, const double* const dir[3],
                          double* ctile, hipStream_t s);
-// dense (column-major) sign*Ac
-void launch_coarse_dense(int32_t nT, int32_t nTI, int32_t nTJ, const double* ctile, double sign,
+// dense (column-major) s*Ac, s the sign of the first diagonal entry
+// (positive definite for a definite matrix of either sign)
+void launch_coarse_dense(int32_t nT, int32_t nTI, int32_t nTJ, const double* ctile,
                          double* dense, hipStream_t s);
-// full FP32 sign*inv from the lower triangle of the factored inverse, rows of
-// ld floats
-void launch_coarse_inv32(int32_t nT, int32_t ld, const double* dense, double sign, float* inv,
-                         hipStream_t s);
+// full FP32 inverse of Ac from the lower triangle of the factored inverse of
+// s*Ac, rows of ld floats
+void launch_coarse_inv32(int32_t nT, int32_t ld, const double* dense, const double* ctile,
+                         float* inv, hipStream_t s);
 // z = inv rc, and sum z*rc plus the nF fine partials of sum w*r, finalized
 // by the last block with F (FIN_PCG_WARA); returns the block count
 int32_t launch_coarse_apply(int32_t nT, int32_t ld, const float* inv, const double* rc,

@@ -66,13 +66,20 @@ void k_coarse_tiles(int32_t nx, int32_t ny, int32_t nz, int32_t nTI, int32_t nTJ
     }
 }
 
-// the dense coarse matrix, negated (the Laplacian is negative definite;
-// the Cholesky factor is taken of -Ac), column-major
+// the sign that makes the coarse matrix positive definite: that of its
+// first diagonal entry (the Laplacian is negative definite)
+__device__ __forceinline__ double coarse_sign(const double* ctile)
+{
+    return ctile[0] < 0.0 ? -1.0 : 1.0;
+}
+
+// the dense coarse matrix times coarse_sign, column-major
 __global__ void k_coarse_dense(int32_t nT, int32_t nTI, int32_t nTJ, const double* __restrict__ ctile,
-                               double sign, double* __restrict__ dense)
+                               double* __restrict__ dense)
 {
     const int64_t idx = blockIdx.x*(int64_t)blockDim.x + threadIdx.x;
     if (idx >= (int64_t)nT*nT) return;
+    const double sign = coarse_sign(ctile);
     const int32_t i = (int32_t)(idx % nT), j = (int32_t)(idx/nT);
     double v = 0.0;
     const int32_t lo = i < j ? i : j, hi = i < j ? j : i, d = hi - lo;
@@ -84,13 +91,14 @@ __global__ void k_coarse_dense(int32_t nT, int32_t nTI, int32_t nTJ, const doubl
     dense[idx] = sign*v;
 }
 
-// the FP32 inverse from the lower triangle of inv(sign*Ac): full symmetric,
-// rows of ld >= nT floats (ld a multiple of 4, zero padding)
+// the FP32 inverse of Ac from the lower triangle of inv(sign*Ac): full
+// symmetric, rows of ld >= nT floats (ld a multiple of 4, zero padding)
 __global__ void k_coarse_inv32(int32_t nT, int32_t ld, const double* __restrict__ dense,
-                               double sign, float* __restrict__ inv)
+                               const double* __restrict__ ctile, float* __restrict__ inv)
 {
     const int64_t idx = blockIdx.x*(int64_t)blockDim.x + threadIdx.x;
     if (idx >= (int64_t)nT*ld) return;
+    const double sign = coarse_sign(ctile);
     const int32_t j = (int32_t)(idx % ld), i = (int32_t)(idx/ld);
     float v = 0.f;
     if (j < nT)
@@ -106,20 +114,20 @@ void launch_coarse_tiles(int32_t nx, int32_t ny, int32_t nz, int32_t nTI, int32_
                        dir[0], dir[1], dir[2], ctile);
 }
 
-void launch_coarse_dense(int32_t nT, int32_t nTI, int32_t nTJ, const double* ctile, double sign,
+void launch_coarse_dense(int32_t nT, int32_t nTI, int32_t nTJ, const double* ctile,
                          double* dense, hipStream_t s)
 {
     const int64_t n = (int64_t)nT*nT;
     hipLaunchKernelGGL(k_coarse_dense, dim3((unsigned)((n + CO_B - 1)/CO_B)), dim3(CO_B), 0, s,
-                       nT, nTI, nTJ, ctile, sign, dense);
+                       nT, nTI, nTJ, ctile, dense);
 }
 
-void launch_coarse_inv32(int32_t nT, int32_t ld, const double* dense, double sign, float* inv,
-                         hipStream_t s)
+void launch_coarse_inv32(int32_t nT, int32_t ld, const double* dense, const double* ctile,
+                         float* inv, hipStream_t s)
 {
     const int64_t n = (int64_t)nT*ld;
     hipLaunchKernelGGL(k_coarse_inv32, dim3((unsigned)((n + CO_B - 1)/CO_B)), dim3(CO_B), 0, s,
-                       nT, ld, dense, sign, inv);
+                       nT, ld, dense, ctile, inv);
 }
 
 // z = Ainv rc (one wavefront per row, four rows per block) and sum z*rc;

@@ -26,9 +26,6 @@
 namespace fh
 {
 
-namespace
-{
-
 constexpr int LVL_THREADS = 512;
 
 template<class T> struct LvlCfg;
@@ -338,14 +335,12 @@ void k_sweep_level_wg(RowsView R, SchedView S, SweepData<T> d, DevState* st)
 }
 
 template<class T, int KIND>
-void launch_kind(const RowsView& R, const SchedView& S, const SweepData<T>& d, DevState* st,
-                 hipStream_t s)
+static void launch_kind(const RowsView& R, const SchedView& S, const SweepData<T>& d,
+                        DevState* st, hipStream_t s)
 {
     hipLaunchKernelGGL((k_sweep_level_wg<T, KIND>), dim3(1), dim3(LVL_THREADS), 0, s, R, S, d,
                        st);
 }
-
-} // namespace
 
 template<class T>
 bool launch_sweep_level_wg(int kind, const RowsView& R, const SchedView& S,
