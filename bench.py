@@ -201,6 +201,14 @@ def main():
                                    "GBs": by64.value / (ms64.value * 1e-3) / 1e9}
         log(f"kernel {name}: fp32 {ms.value:.4f} ms {kernels[name]['GBs']:.0f} GB/s | "
             f"fp64 {ms64.value:.4f} ms {kernels[name + '_fp64']['GBs']:.0f} GB/s")
+    # lduMatrix::H of a scalar field (FP64, the momentum predictor's face
+    # loop around the solve)
+    ms64, by64 = ctypes.c_double(), ctypes.c_double()
+    fh._check(fh.lib().fh_profile_kernel(mat.handle, 5, 0, FH_DIC, 20,
+                                         ctypes.byref(ms64), ctypes.byref(by64)))
+    kernels["H_fp64"] = {"ms": ms64.value, "bytes": by64.value,
+                         "GBs": by64.value / (ms64.value * 1e-3) / 1e9}
+    log(f"kernel H: fp64 {ms64.value:.4f} ms {kernels['H_fp64']['GBs']:.0f} GB/s")
     # dominant kernel of the timed (mixed) solve: largest live time per
     # inner iteration (one launch of each per iteration; on a box the FP32
     # tile-local preconditioner is one launch, both passes); algorithmic

@@ -202,6 +202,23 @@ int fh_residual(fh_matrix* m, const double* psi, const double* source,
                 double* rA);
 /* lduMatrixATmul.C:154-200 */
 int fh_sum_a(fh_matrix* m, double* sumA);
+/* The off-diagonal face loops around the solve (pressure-velocity
+   coupling).  A field of a Type with nCmpt components (1..9: scalar,
+   vector, symmTensor, tensor ...) holds nCells (or nFaces) x nCmpt values,
+   component fastest: the memory layout of Field<Type>.  A diagonal-only
+   matrix gives H = 0 and H1 = 0 and leaves sumOff unchanged. */
+/* lduMatrixTemplates.C:33-64: Hpsi = -(off-diagonal part of A) psi */
+int fh_h(fh_matrix* m, int32_t nCmpt, const double* psi, double* Hpsi);
+/* lduMatrixATmul.C:298-324: H1 = -(sum of the row's off-diagonal
+   coefficients) */
+int fh_h1(fh_matrix* m, double* H1);
+/* lduMatrixTemplates.C:77-111: faceHpsi[f] = upper[f] psi[u[f]]
+   - lower[f] psi[l[f]], natural face order; FH_ERR_INVALID for a
+   diagonal-only matrix (a fatal error in the reference) */
+int fh_face_h(fh_matrix* m, int32_t nCmpt, const double* psi, double* faceHpsi);
+/* lduMatrixOperations.C:77-97: sumOff += the row's sum of |off-diagonal
+   coefficient| (sumOff is read and written) */
+int fh_sum_mag_off_diag(fh_matrix* m, double* sumOff);
 /* lduMatrixSolver.C:174-197 */
 int fh_norm_factor(fh_matrix* m, const double* psi, const double* source,
                    double* normFactor);
@@ -234,8 +251,9 @@ enum
     FH_KERNEL_SWEEP_FORWARD = 1, /* DIC/DILU forward substitution    */
     FH_KERNEL_SWEEP_BACKWARD = 2,/* DIC/DILU backward substitution   */
     FH_KERNEL_AXPY_NORM = 3,     /* psi += a p, r -= a w, sum |r|     */
-    FH_KERNEL_PRECONDITION = 4   /* the whole preconditioner apply of the
+    FH_KERNEL_PRECONDITION = 4,  /* the whole preconditioner apply of the
                                     PCG loop, with its fused w.r sum    */
+    FH_KERNEL_H = 5              /* H x of a scalar field (fh_h), FP64   */
 };
 int fh_profile_kernel(fh_matrix* matrix, int32_t kernel, int32_t precision,
                       int32_t preconditioner, int32_t reps, double* msPerLaunch,

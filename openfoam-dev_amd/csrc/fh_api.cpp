@@ -3,7 +3,9 @@
 // records the message for fh_last_error().
 #include "fh_matrix.h"
 #include "fh_gamg.h"
+#include "fh_hops.h"
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstring>
 #include <new>
@@ -361,6 +363,105 @@ int fh_sum_a(fh_matrix* M, double* sumA)
         ensure_sumA(M);
         DBuf<double> nat;
         from_lib(M, M->d_sumA.p, nat, sumA);
+    });
+}
+
+// a field of nCmpt components per cell (component fastest) into and out of
+// the library numbering
+static void to_lib_cmpt(fh_matrix* M, int32_t nC, const double* h, DBuf<double>& nat,
+                        double* lib)
+{
+    const int32_t N = M->mesh->nCells;
+    nat.alloc((size_t)N*nC);
+    FH_HIP(hipMemcpyAsync(nat.p, h, sizeof(double)*N*nC, hipMemcpyHostToDevice, M->stream));
+    launch_permute_cmpt_in(N, nC, M->mesh->d_perm.p, nat.p, lib, M->stream);
+}
+
+static void from_lib_cmpt(fh_matrix* M, int32_t nC, const double* lib, DBuf<double>& nat,
+                          double* h)
+{
+    const int32_t N = M->mesh->nCells;
+    nat.alloc((size_t)N*nC);
+    launch_permute_cmpt_out(N, nC, M->mesh->d_perm.p, lib, nat.p, M->stream);
+    FH_HIP(hipMemcpyAsync(h, nat.p, sizeof(double)*N*nC, hipMemcpyDeviceToHost, M->stream));
+    FH_HIP(hipStreamSynchronize(M->stream));
+}
+
+int fh_h(fh_matrix* M, int32_t nCmpt, const double* psi, double* Hpsi)
+{
+    return guarded([&] {
+        check_matrix(M);
+        if (!psi || !Hpsi) fail(FH_ERR_INVALID, "NULL argument");
+        if (nCmpt < 1 || nCmpt > 9) fail(FH_ERR_INVALID, "nCmpt must be in 1..9, got %d", nCmpt);
+        const int32_t N = M->mesh->nCells;
+        if (M->diagonalOnly || M->mesh->nFaces == 0)
+        {
+            std::fill(Hpsi, Hpsi + (int64_t)N*nCmpt, 0.0);
+            return;
+        }
+        DBuf<double> nat, x, y;
+        x.alloc((size_t)N*nCmpt); y.alloc((size_t)N*nCmpt);
+        to_lib_cmpt(M, nCmpt, psi, nat, x.p);
+        launch_h(M->rows(), M->d_val.p, nCmpt, x.p, y.p, M->stream);
+        from_lib_cmpt(M, nCmpt, y.p, nat, Hpsi);
+    });
+}
+
+int fh_h1(fh_matrix* M, double* H1)
+{
+    return guarded([&] {
+        check_matrix(M);
+        if (!H1) fail(FH_ERR_INVALID, "NULL argument");
+        const int32_t N = M->mesh->nCells;
+        if (M->diagonalOnly || M->mesh->nFaces == 0)
+        {
+            std::fill(H1, H1 + N, 0.0);
+            return;
+        }
+        DBuf<double> nat, y;
+        y.alloc(N);
+        launch_h1(M->rows(), M->d_val.p, y.p, M->stream);
+        from_lib(M, y.p, nat, H1);
+    });
+}
+
+int fh_face_h(fh_matrix* M, int32_t nCmpt, const double* psi, double* faceHpsi)
+{
+    return guarded([&] {
+        check_matrix(M);
+        if (!psi || !faceHpsi) fail(FH_ERR_INVALID, "NULL argument");
+        if (nCmpt < 1 || nCmpt > 9) fail(FH_ERR_INVALID, "nCmpt must be in 1..9, got %d", nCmpt);
+        if (M->diagonalOnly)
+            fail(FH_ERR_INVALID, "Cannot calculate faceH the matrix does not have any "
+                                 "off-diagonal coefficients.");
+        const fh_mesh* mesh = M->mesh;
+        const int32_t N = mesh->nCells, F = mesh->nFaces;
+        if (F == 0) return;
+        DBuf<double> x, y;
+        x.alloc((size_t)N*nCmpt); y.alloc((size_t)F*nCmpt);
+        FH_HIP(hipMemcpyAsync(x.p, psi, sizeof(double)*N*nCmpt, hipMemcpyHostToDevice,
+                              M->stream));
+        const double* lower = M->symmetric ? M->d_upperF.p : M->d_lowerF.p;
+        launch_face_h(F, mesh->d_lower.p, mesh->d_upper.p, M->d_upperF.p, lower, nCmpt, x.p,
+                      y.p, M->stream);
+        FH_HIP(hipMemcpyAsync(faceHpsi, y.p, sizeof(double)*F*nCmpt, hipMemcpyDeviceToHost,
+                              M->stream));
+        FH_HIP(hipStreamSynchronize(M->stream));
+    });
+}
+
+int fh_sum_mag_off_diag(fh_matrix* M, double* sumOff)
+{
+    return guarded([&] {
+        check_matrix(M);
+        if (!sumOff) fail(FH_ERR_INVALID, "NULL argument");
+        const int32_t N = M->mesh->nCells;
+        if (M->diagonalOnly || M->mesh->nFaces == 0) return;
+        DBuf<double> nat, y;
+        y.alloc(N);
+        to_lib(M, sumOff, nat, y.p);
+        launch_sum_mag_off_diag(M->rows(), M->d_val.p, y.p, M->stream);
+        from_lib(M, y.p, nat, sumOff);
     });
 }
 

@@ -1,0 +1,167 @@
+// lduMatrix H, H1, faceH and sumMagOffDiag: the off-diagonal face loops of
+// the pressure-velocity coupling (the H operator of the momentum matrix,
+// its coefficient sum H1, the face flux of H, and the off-diagonal
+// magnitude sum that fvMatrix::relax starts from).
+//
+// The reference walks the faces once and scatters into both cells.  Here
+// each library row is one thread over its SELL-64 entries: the lower-
+// neighbour entries by ascending face, then the upper-neighbour entries by
+// ascending face, which is the order in which the face loop reaches the
+// cell, so every sum is the reference's to the bit.  The row's coefficient
+// of a neighbour is the one the face loop uses for that cell (the lower
+// coefficient for the neighbour cell of a face, the upper one for its
+// owner), i.e. the SELL values of the product A x.
+#include "fh_hops.h"
+
+namespace fh
+{
+
+#define HO_B 256
+
+// entry k of row r at sliceStart[r/64] + 64*k + r%64
+__device__ __forceinline__ int32_t ho_base(const int32_t* sliceStart, int32_t r)
+{
+    return sliceStart[r >> 6] + (r & 63);
+}
+
+// lduMatrixTemplates.C:33-64 for NC components (NC = 0: nC at run time,
+// one component per launch, component c0)
+template<int NC>
+__global__ void k_h(RowsView R, const double* __restrict__ val, int32_t nC, int32_t c0,
+                    const double* __restrict__ x, double* __restrict__ y)
+{
+    const int32_t r = blockIdx.x*HO_B + threadIdx.x;
+    if (r >= R.n) return;
+    const int32_t nEnt = R.rowInfo[r] >> 16;
+    const int32_t base = ho_base(R.sliceStart, r);
+    if constexpr (NC > 0)
+    {
+        double acc[NC];
+#pragma unroll
+        for (int c = 0; c < NC; c++) acc[c] = 0.0;
+        for (int32_t k = 0; k < nEnt; k++)
+        {
+            const int32_t e = base + (k << 6);
+            const double a = val[e];
+            const int64_t j = (int64_t)R.col[e]*NC;
+#pragma unroll
+            for (int c = 0; c < NC; c++) acc[c] -= a*x[j + c];
+        }
+#pragma unroll
+        for (int c = 0; c < NC; c++) y[(int64_t)r*NC + c] = acc[c];
+    }
+    else
+    {
+        double acc = 0.0;
+        for (int32_t k = 0; k < nEnt; k++)
+        {
+            const int32_t e = base + (k << 6);
+            acc -= val[e]*x[(int64_t)R.col[e]*nC + c0];
+        }
+        y[(int64_t)r*nC + c0] = acc;
+    }
+}
+
+// lduMatrixATmul.C:298-324
+__global__ void k_h1(RowsView R, const double* __restrict__ val, double* __restrict__ y)
+{
+    const int32_t r = blockIdx.x*HO_B + threadIdx.x;
+    if (r >= R.n) return;
+    const int32_t nEnt = R.rowInfo[r] >> 16;
+    const int32_t base = ho_base(R.sliceStart, r);
+    double acc = 0.0;
+    for (int32_t k = 0; k < nEnt; k++) acc -= val[base + (k << 6)];
+    y[r] = acc;
+}
+
+// lduMatrixOperations.C:77-97 (added to y)
+__global__ void k_sum_mag_off_diag(RowsView R, const double* __restrict__ val,
+                                   double* __restrict__ y)
+{
+    const int32_t r = blockIdx.x*HO_B + threadIdx.x;
+    if (r >= R.n) return;
+    const int32_t nEnt = R.rowInfo[r] >> 16;
+    const int32_t base = ho_base(R.sliceStart, r);
+    double acc = y[r];
+    for (int32_t k = 0; k < nEnt; k++) acc += fabs(val[base + (k << 6)]);
+    y[r] = acc;
+}
+
+// lduMatrixTemplates.C:77-111: one thread per face, natural numbering
+__global__ void k_face_h(int32_t nFaces, const int32_t* __restrict__ l,
+                         const int32_t* __restrict__ u, const double* __restrict__ upper,
+                         const double* __restrict__ lower, int32_t nC,
+                         const double* __restrict__ x, double* __restrict__ y)
+{
+    const int32_t f = blockIdx.x*HO_B + threadIdx.x;
+    if (f >= nFaces) return;
+    const double U = upper[f], L = lower[f];
+    const int64_t jl = (int64_t)l[f]*nC, ju = (int64_t)u[f]*nC;
+    for (int32_t c = 0; c < nC; c++) y[(int64_t)f*nC + c] = U*x[ju + c] - L*x[jl + c];
+}
+
+__global__ void k_permute_cmpt(int32_t n, int32_t nC, const int32_t* __restrict__ perm,
+                               const double* __restrict__ in, double* __restrict__ out,
+                               bool toLib)
+{
+    const int64_t i = blockIdx.x*(int64_t)HO_B + threadIdx.x;
+    if (i >= (int64_t)n*nC) return;
+    const int32_t r = (int32_t)(i/nC), c = (int32_t)(i % nC);
+    const int64_t nat = (int64_t)perm[r]*nC + c;
+    if (toLib) out[i] = in[nat];
+    else out[nat] = in[i];
+}
+
+static dim3 grid_of(int64_t n) { return dim3((unsigned)((n + HO_B - 1)/HO_B)); }
+
+void launch_h(const RowsView& R, const double* val, int32_t nC, const double* x, double* y,
+              hipStream_t s)
+{
+    if (R.n == 0) return;
+    if (nC == 1)
+        hipLaunchKernelGGL(k_h<1>, grid_of(R.n), dim3(HO_B), 0, s, R, val, 1, 0, x, y);
+    else if (nC == 3)
+        hipLaunchKernelGGL(k_h<3>, grid_of(R.n), dim3(HO_B), 0, s, R, val, 3, 0, x, y);
+    else
+        for (int32_t c = 0; c < nC; c++)
+            hipLaunchKernelGGL(k_h<0>, grid_of(R.n), dim3(HO_B), 0, s, R, val, nC, c, x,
+                               y);
+}
+
+void launch_h1(const RowsView& R, const double* val, double* y, hipStream_t s)
+{
+    if (R.n == 0) return;
+    hipLaunchKernelGGL(k_h1, grid_of(R.n), dim3(HO_B), 0, s, R, val, y);
+}
+
+void launch_sum_mag_off_diag(const RowsView& R, const double* val, double* y, hipStream_t s)
+{
+    if (R.n == 0) return;
+    hipLaunchKernelGGL(k_sum_mag_off_diag, grid_of(R.n), dim3(HO_B), 0, s, R, val, y);
+}
+
+void launch_face_h(int32_t nFaces, const int32_t* l, const int32_t* u, const double* upper,
+                   const double* lower, int32_t nC, const double* x, double* y, hipStream_t s)
+{
+    if (nFaces == 0) return;
+    hipLaunchKernelGGL(k_face_h, grid_of(nFaces), dim3(HO_B), 0, s, nFaces, l, u, upper,
+                       lower, nC, x, y);
+}
+
+void launch_permute_cmpt_in(int32_t n, int32_t nC, const int32_t* perm, const double* xNat,
+                            double* xLib, hipStream_t s)
+{
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_permute_cmpt, grid_of((int64_t)n*nC), dim3(HO_B), 0, s, n, nC,
+                       perm, xNat, xLib, true);
+}
+
+void launch_permute_cmpt_out(int32_t n, int32_t nC, const int32_t* perm, const double* xLib,
+                             double* xNat, hipStream_t s)
+{
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_permute_cmpt, grid_of((int64_t)n*nC), dim3(HO_B), 0, s, n, nC,
+                       perm, xLib, xNat, false);
+}
+
+} // namespace fh

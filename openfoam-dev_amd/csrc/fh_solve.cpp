@@ -3,6 +3,7 @@
 // (smoothSolver.C:77-193) and diagonalSolver, plus the mixed-precision
 // iterative refinement around FP32 inner PCG / PBiCGStab solves.
 #include "fh_matrix.h"
+#include "fh_hops.h"
 #include "fh_gamg.h"
 #include "fh_gamg_kernels.h"
 #include "fh_coarse.h"
@@ -1572,6 +1573,12 @@ static void profile_t(fh_matrix* M, int kernel, int precon, int reps, double* ms
                 }
                 launch_pcg_update_nostate<T>(N, x.p, y.p, z.p, z.p, M->d_partials.p, M->stream);
                 break;
+            case FH_KERNEL_H:
+                if constexpr (std::is_same<T, float>::value)
+                    fail(FH_ERR_INVALID, "H is profiled in FP64 only");
+                else
+                    launch_h(M->rows(), M->d_val.p, 1, x.p, y.p, M->stream);
+                break;
             default:
                 fail(FH_ERR_INVALID, "unknown kernel id %d", kernel);
         }
@@ -1630,6 +1637,9 @@ static void profile_t(fh_matrix* M, int kernel, int precon, int reps, double* ms
             // FP32 (the mixed update between folds): correction and
             // recurrence residual read and written, p and A p read
             *bytes = 6.0*N*w; break;
+        case FH_KERNEL_H:
+            // x, y and row info per cell; (value, column) per off-diagonal
+            *bytes = N*(2*w + 4) + 2.0*F*(w + 4); break;
     }
 }
 

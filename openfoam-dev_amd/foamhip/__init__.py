@@ -160,6 +160,39 @@ class lduMatrix:
         _check(lib().fh_sum_a(self.handle, _ptr(out)))
         return out
 
+    def H(self, psi):
+        """lduMatrix::H (lduMatrixTemplates.C:33-64): minus the off-diagonal
+        part of the matrix times psi; psi of shape (nCells,) or
+        (nCells, nCmpt)."""
+        psi = _f64(psi)
+        nc = 1 if psi.ndim == 1 else psi.shape[1]
+        out = np.empty_like(psi)
+        _check(lib().fh_h(self.handle, nc, _ptr(psi), _ptr(out)))
+        return out
+
+    def H1(self):
+        """lduMatrix::H1 (lduMatrixATmul.C:298-324)."""
+        out = np.empty(self.mesh.nCells)
+        _check(lib().fh_h1(self.handle, _ptr(out)))
+        return out
+
+    def faceH(self, psi):
+        """lduMatrix::faceH (lduMatrixTemplates.C:77-111): one value (or
+        nCmpt values) per face."""
+        psi = _f64(psi)
+        nc = 1 if psi.ndim == 1 else psi.shape[1]
+        nF = self.mesh.nFaces
+        out = np.empty((nF,) if psi.ndim == 1 else (nF, nc))
+        _check(lib().fh_face_h(self.handle, nc, _ptr(psi), _ptr(out)))
+        return out
+
+    def sumMagOffDiag(self, sumOff):
+        """lduMatrix::sumMagOffDiag (lduMatrixOperations.C:77-97): returns
+        sumOff plus each row's sum of off-diagonal magnitudes."""
+        out = _f64(sumOff).copy()
+        _check(lib().fh_sum_mag_off_diag(self.handle, _ptr(out)))
+        return out
+
     def normFactor(self, psi, source):
         psi, source = _f64(psi), _f64(source)
         v = ctypes.c_double()

@@ -151,6 +151,65 @@ void or_amul(const or_matrix* m, const double* psi, double* Apsi)
     }
 }
 
+/* lduMatrixTemplates.C:33-64: H starts at zero; each face takes the
+   lower coefficient times the owner's psi from the neighbour's H and the
+   upper coefficient times the neighbour's psi from the owner's H, in face
+   order, every component alike */
+void or_H(const or_matrix* m, int32_t nCmpt, const double* psi, double* Hpsi)
+{
+    const or_addr* a = m->addr;
+    for (int64_t i = 0; i < (int64_t)a->nCells*nCmpt; i++) Hpsi[i] = 0.0;
+    if (!m->upper) return;
+    for (int32_t f = 0; f < a->nFaces; f++)
+    {
+        const int64_t l = (int64_t)a->lower[f]*nCmpt, u = (int64_t)a->upper[f]*nCmpt;
+        for (int32_t c = 0; c < nCmpt; c++)
+        {
+            Hpsi[u + c] -= m->lower[f]*psi[l + c];
+            Hpsi[l + c] -= m->upper[f]*psi[u + c];
+        }
+    }
+}
+
+/* lduMatrixATmul.C:298-324 */
+void or_H1(const or_matrix* m, double* H1)
+{
+    const or_addr* a = m->addr;
+    for (int32_t c = 0; c < a->nCells; c++) H1[c] = 0.0;
+    if (!m->upper) return;
+    for (int32_t f = 0; f < a->nFaces; f++)
+    {
+        H1[a->upper[f]] -= m->lower[f];
+        H1[a->lower[f]] -= m->upper[f];
+    }
+}
+
+/* lduMatrixTemplates.C:77-111 */
+int or_faceH(const or_matrix* m, int32_t nCmpt, const double* psi, double* faceHpsi)
+{
+    const or_addr* a = m->addr;
+    if (!m->upper) return -1;
+    for (int32_t f = 0; f < a->nFaces; f++)
+    {
+        const int64_t l = (int64_t)a->lower[f]*nCmpt, u = (int64_t)a->upper[f]*nCmpt;
+        for (int32_t c = 0; c < nCmpt; c++)
+            faceHpsi[(int64_t)f*nCmpt + c] = m->upper[f]*psi[u + c] - m->lower[f]*psi[l + c];
+    }
+    return 0;
+}
+
+/* lduMatrixOperations.C:77-97 */
+void or_sumMagOffDiag(const or_matrix* m, double* sumOff)
+{
+    const or_addr* a = m->addr;
+    if (!m->upper) return;
+    for (int32_t f = 0; f < a->nFaces; f++)
+    {
+        sumOff[a->upper[f]] += fabs(m->lower[f]);
+        sumOff[a->lower[f]] += fabs(m->upper[f]);
+    }
+}
+
 /* lduMatrixATmul.C:95-151 */
 void or_tmul(const or_matrix* m, const double* psi, double* Tpsi)
 {

@@ -69,6 +69,20 @@ double or_normFactor(const or_matrix* m, const double* psi,
 void or_negSumDiag(const or_addr* a, const double* lower, const double* upper,
                    double* diag);
 
+/* ---- the off-diagonal face loops of the pressure-velocity coupling ----
+   Fields of a Type with nCmpt components are stored component fastest
+   (the memory layout of Field<Type>); m->upper == NULL is a diagonal-only
+   matrix. */
+/* lduMatrixTemplates.C:33-64 */
+void or_H(const or_matrix* m, int32_t nCmpt, const double* psi, double* Hpsi);
+/* lduMatrixATmul.C:298-324 */
+void or_H1(const or_matrix* m, double* H1);
+/* lduMatrixTemplates.C:77-111; returns -1 (the reference's fatal error)
+   when the matrix has no off-diagonal coefficients, else 0 */
+int or_faceH(const or_matrix* m, int32_t nCmpt, const double* psi, double* faceHpsi);
+/* lduMatrixOperations.C:77-97: sumOff is added to */
+void or_sumMagOffDiag(const or_matrix* m, double* sumOff);
+
 /* ---- selection ids (names as in the reference run-time tables) ---- */
 enum { OR_SOLVER_PCG = 0, OR_SOLVER_PBICGSTAB = 1, OR_SOLVER_PBICG = 2,
        OR_SOLVER_SMOOTH = 3, OR_SOLVER_GAMG = 4, OR_SOLVER_DIAGONAL = 5 };

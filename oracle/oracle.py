@@ -79,6 +79,11 @@ def lib():
         L.or_normFactor.argtypes = [M, _vp, _vp, _vp, _vp]
         L.or_normFactor.restype = _dbl
         L.or_negSumDiag.argtypes = [ctypes.POINTER(OrAddr), _vp, _vp, _vp]
+        L.or_H.argtypes = [M, _i32, _vp, _vp]
+        L.or_H1.argtypes = [M, _vp]
+        L.or_faceH.argtypes = [M, _i32, _vp, _vp]
+        L.or_faceH.restype = ctypes.c_int
+        L.or_sumMagOffDiag.argtypes = [M, _vp]
         L.or_calcReciprocalD_DIC.argtypes = [M, _vp]
         L.or_calcReciprocalD_DILU.argtypes = [M, _vp]
         L.or_precondition.argtypes = [M, ctypes.c_int, _vp, _vp, _vp]
@@ -171,6 +176,31 @@ class Matrix:
     def residual(self, psi, source):
         psi, source = _f(psi), _f(source)
         o = self._out(); lib().or_residual(self.ref, _p(psi), _p(source), _p(o)); return o
+
+    def H(self, psi):
+        """lduMatrix::H; psi of shape (nCells,) or (nCells, nCmpt)."""
+        psi = _f(psi)
+        nc = 1 if psi.ndim == 1 else psi.shape[1]
+        o = np.empty_like(psi)
+        lib().or_H(self.ref, nc, _p(psi), _p(o))
+        return o
+
+    def H1(self):
+        o = self._out(); lib().or_H1(self.ref, _p(o)); return o
+
+    def faceH(self, psi):
+        """lduMatrix::faceH: one value (or nCmpt) per face."""
+        psi = _f(psi)
+        nc = 1 if psi.ndim == 1 else psi.shape[1]
+        shape = (self.addr.nFaces,) if psi.ndim == 1 else (self.addr.nFaces, nc)
+        o = np.empty(shape)
+        if lib().or_faceH(self.ref, nc, _p(psi), _p(o)) != 0:
+            raise RuntimeError("Cannot calculate faceH: the matrix does not have any "
+                               "off-diagonal coefficients.")
+        return o
+
+    def sumMagOffDiag(self, sumOff):
+        o = _f(sumOff).copy(); lib().or_sumMagOffDiag(self.ref, _p(o)); return o
 
     def normFactor(self, psi, source):
         psi, source = _f(psi), _f(source)

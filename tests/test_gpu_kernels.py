@@ -111,6 +111,44 @@ def test_matrix_kernels_bitwise(name, gen, col, kind):
 
 
 @pytest.mark.parametrize("name,gen,col", CASES)
+@pytest.mark.parametrize("kind", ["spd", "asym"])
+def test_h_operators_bitwise(name, gen, col, kind):
+    """lduMatrix::H (scalar, vector and six-component fields), H1, faceH and
+    sumMagOffDiag against the oracle's face loops."""
+    rng = np.random.default_rng(21)
+    mesh, m, om = make(gen(rng), rng, kind, col)
+    n = mesh.nCells
+    x = rng.standard_normal(n)
+    np.testing.assert_array_equal(m.H(x), om.H(x))
+    for nc in (3, 6):
+        X = rng.standard_normal((n, nc))
+        np.testing.assert_array_equal(m.H(X), om.H(X))
+        np.testing.assert_array_equal(m.faceH(X), om.faceH(X))
+    np.testing.assert_array_equal(m.H1(), om.H1())
+    np.testing.assert_array_equal(m.faceH(x), om.faceH(x))
+    s0 = rng.standard_normal(n)
+    np.testing.assert_array_equal(m.sumMagOffDiag(s0), om.sumMagOffDiag(s0))
+
+
+def test_h_operators_edge_cases():
+    rng = np.random.default_rng(22)
+    n, lo, up = box(5, 4, 3)
+    mesh = fh.lduAddressing(n, lo, up)
+    x = rng.standard_normal(n)
+    # diagonal-only matrix: H = 0, H1 = 0, sumOff unchanged, faceH refused
+    d = fh.lduMatrix(mesh, rng.uniform(1, 2, n))
+    assert not d.H(x).any() and not d.H1().any()
+    s0 = rng.standard_normal(n)
+    np.testing.assert_array_equal(d.sumMagOffDiag(s0), s0)
+    with pytest.raises(fh.FoamHipError, match="faceH"):
+        d.faceH(x)
+    # nCmpt out of range
+    m = fh.lduMatrix(mesh, *spd_coeffs(n, lo, up, rng))
+    with pytest.raises(fh.FoamHipError):
+        m.H(rng.standard_normal((n, 10)))
+
+
+@pytest.mark.parametrize("name,gen,col", CASES)
 def test_preconditioners_bitwise(name, gen, col):
     rng = np.random.default_rng(21)
     for kind, precons in [("spd", ["DIC", "FDIC", "diagonal", "none"]),

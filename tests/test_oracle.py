@@ -540,3 +540,77 @@ def test_gauss_grad_exact_for_linear_fields():
                                       (sign * g[axis]))
     np.testing.assert_allclose(grad.reshape(-1, 3), np.tile(g, (c["N"], 1)), rtol=1e-10)
     np.testing.assert_allclose(bGrad.reshape(-1, 3), np.tile(g, (len(c["bfc"]), 1)), rtol=1e-10)
+
+
+# ---- lduMatrix H, H1, faceH, sumMagOffDiag -------------------------------
+
+def test_h_operators_hand_worked():
+    """Three cells in a line, faces (0,1) and (1,2), upper != lower so a
+    swapped role or sign changes the answer; values worked by hand from
+    the face loops of lduMatrixTemplates.C:33-111, lduMatrixATmul.C:298-324
+    and lduMatrixOperations.C:77-97."""
+    lo = np.array([0, 1], np.int32)
+    up = np.array([1, 2], np.int32)
+    a = O.Addr(3, lo, up)
+    m = O.Matrix(a, [1.0, 1.0, 1.0], [2.0, 3.0], [5.0, 7.0])
+    psi = np.array([1.0, 10.0, 100.0])
+    # H[0] = -2*10; H[1] = -5*1 - 3*100; H[2] = -7*10
+    assert m.H(psi).tolist() == [-20.0, -305.0, -70.0]
+    assert m.H1().tolist() == [-2.0, -8.0, -7.0]
+    # faceH = upper*psi(neighbour) - lower*psi(owner)
+    assert m.faceH(psi).tolist() == [2.0 * 10 - 5.0 * 1, 3.0 * 100 - 7.0 * 10]
+    mm = O.Matrix(a, [1.0, 1.0, 1.0], [-2.0, 3.0], [5.0, -7.0])
+    assert mm.sumMagOffDiag([0.5, 0.0, 0.0]).tolist() == [2.5, 8.0, 7.0]
+    # a vector field: component by component
+    v = np.stack([psi, 2 * psi, -psi], axis=1)
+    np.testing.assert_array_equal(m.H(v), np.stack([m.H(psi), m.H(2 * psi), m.H(-psi)], 1))
+    np.testing.assert_array_equal(m.faceH(v)[:, 2], -m.faceH(psi))
+    # diagonal-only: H = 0, H1 = 0, faceH is an error
+    d = O.Matrix(a, [1.0, 2.0, 3.0])
+    assert d.H(psi).tolist() == [0.0, 0.0, 0.0] and d.H1().tolist() == [0.0, 0.0, 0.0]
+    with pytest.raises(RuntimeError):
+        d.faceH(psi)
+
+
+@pytest.mark.parametrize("name,gen", MESHES)
+@pytest.mark.parametrize("kind", ["spd", "asym"])
+def test_h_operators_vs_dense(name, gen, kind):
+    rng = np.random.default_rng(5)
+    n, lo, up = gen(rng)
+    a, m, A = mk(n, lo, up, rng, kind)
+    off = A - np.diag(np.diag(A))
+    x = rng.standard_normal(n)
+    np.testing.assert_allclose(m.H(x), -off @ x, rtol=1e-13, atol=1e-13)
+    np.testing.assert_allclose(m.H1(), -off.sum(axis=1), rtol=1e-13, atol=1e-13)
+    s0 = rng.standard_normal(n)
+    np.testing.assert_allclose(m.sumMagOffDiag(s0), s0 + np.abs(off).sum(axis=1),
+                               rtol=1e-13, atol=1e-13)
+    # faceH from the dense coefficients: A[l, u] is the face's upper one
+    np.testing.assert_allclose(m.faceH(x), A[lo, up] * x[up] - A[up, lo] * x[lo],
+                               rtol=1e-13, atol=1e-13)
+    # H of a unit field is H1, to the bit (coefficient * 1 is exact)
+    np.testing.assert_array_equal(m.H(np.ones(n)), m.H1())
+    # a three-component field is three scalar fields, to the bit
+    X = rng.standard_normal((n, 3))
+    np.testing.assert_array_equal(m.H(X), np.stack([m.H(X[:, c]) for c in range(3)], 1))
+
+
+@pytest.mark.parametrize("name,gen", MESHES)
+def test_h_operators_identities(name, gen):
+    """Negative off-diagonals (a Laplacian): sumMagOffDiag from zero is H1
+    to the bit.  Symmetric: the discrete divergence of faceH (out of the
+    owner, into the neighbour) is psi*H1 - H(psi)."""
+    rng = np.random.default_rng(6)
+    n, lo, up = gen(rng)
+    a = O.Addr(n, lo, up)
+    d, u = spd_coeffs(n, lo, up, rng)
+    assert np.all(u <= 0)
+    m = O.Matrix(a, d, u)
+    # 0 + |a| + |b| and (0 - a) - b round alike for a, b <= 0
+    np.testing.assert_array_equal(m.sumMagOffDiag(np.zeros(n)), m.H1())
+    x = rng.standard_normal(n)
+    fH = m.faceH(x)
+    div = np.zeros(n)
+    np.add.at(div, lo, fH)
+    np.add.at(div, up, -fH)
+    np.testing.assert_allclose(div, x * m.H1() - m.H(x), rtol=1e-12, atol=1e-12)
