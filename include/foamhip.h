@@ -183,6 +183,11 @@ int fh_matrix_set(fh_matrix* matrix, const double* diag, const double* upper,
                   const double* lower);
 int fh_matrix_set_device(fh_matrix* matrix, const double* d_diag,
                          const double* d_upper, const double* d_lower);
+/* lduMatrix::diag(), upper(), lower() (lduMatrix.C:168-312): copies of the
+   coefficients, natural order; any output may be NULL.  lower of a
+   symmetric matrix is its upper (lower() returns upper when lowerPtr_ is
+   unset); the off-diagonals of a diagonal-only matrix are zeros. */
+int fh_matrix_get(fh_matrix* matrix, double* diag, double* upper, double* lower);
 int fh_matrix_destroy(fh_matrix* matrix);
 
 /* ---- solve: lduMatrix::solver::solve (lduMatrix.H:250-255) -------------
@@ -314,6 +319,20 @@ int fh_assemble_convection_diffusion(fh_matrix* matrix, const fh_boundary* bound
                                      const double* d_bValue, double* d_internalCoeffs,
                                      double* d_boundaryCoeffs, double* d_source,
                                      int32_t addBoundaryDiag);
+
+/* fvMatrix<Type>::relax(alpha) (fvMatrix.C:594-740) for a matrix whose
+   patches are all non-coupled (those of fh_boundary_create): with D0 the
+   diagonal, D = D0 plus each boundary face's largest internal-coefficient
+   magnitude (patch order), D = max(|D|, sumMagOffDiag)/alpha, less each
+   face's smallest internal coefficient; source += (D - D0) psi.  The
+   matrix's diagonal becomes D (cached factors are dropped).  Device arrays,
+   natural order, component fastest: d_internalCoeffs nBFaces x nCmpt,
+   d_psi and d_source nCells x nCmpt (d_source updated in place); d_diag
+   (may be NULL) receives D.  alpha <= 0 does nothing, as in the
+   reference. */
+int fh_relax(fh_matrix* matrix, const fh_boundary* boundary, double alpha, int32_t nCmpt,
+             const double* d_internalCoeffs, const double* d_psi, double* d_source,
+             double* d_diag);
 
 /* fvc::grad(phi), Gauss linear (gaussGrad.C:30-75) with the patch-normal
    boundary correction (gradScheme.C:123-148).  d_grad [nCells*3],

@@ -264,6 +264,27 @@ int fh_matrix_set_device(fh_matrix* M, const double* d_diag, const double* d_upp
     });
 }
 
+int fh_matrix_get(fh_matrix* M, double* diag, double* upper, double* lower)
+{
+    return guarded([&] {
+        check_matrix(M);
+        const int32_t F = M->mesh->nFaces;
+        if (diag)
+        {
+            DBuf<double> nat;
+            from_lib(M, M->d_diag.p, nat, diag);
+        }
+        auto face = [&](double* h, const double* d) {
+            if (!h) return;
+            if (M->diagonalOnly || F == 0) std::fill(h, h + F, 0.0);
+            else FH_HIP(hipMemcpyAsync(h, d, sizeof(double)*F, hipMemcpyDeviceToHost, M->stream));
+        };
+        face(upper, M->d_upperF.p);
+        face(lower, M->symmetric ? M->d_upperF.p : M->d_lowerF.p);
+        FH_HIP(hipStreamSynchronize(M->stream));
+    });
+}
+
 int fh_matrix_destroy(fh_matrix* M)
 {
     return guarded([&] {

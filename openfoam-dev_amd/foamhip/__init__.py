@@ -160,6 +160,14 @@ class lduMatrix:
         _check(lib().fh_sum_a(self.handle, _ptr(out)))
         return out
 
+    def coeffs(self):
+        """(diag, upper, lower) read back from the device, natural order
+        (lower is upper for a symmetric matrix)."""
+        n, f = self.mesh.nCells, self.mesh.nFaces
+        d, u, lo = np.empty(n), np.empty(f), np.empty(f)
+        _check(lib().fh_matrix_get(self.handle, _ptr(d), _ptr(u), _ptr(lo)))
+        return d, u, lo
+
     def H(self, psi):
         """lduMatrix::H (lduMatrixTemplates.C:33-64): minus the off-diagonal
         part of the matrix times psi; psi of shape (nCells,) or

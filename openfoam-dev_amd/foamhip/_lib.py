@@ -104,6 +104,7 @@ FH_SYMBOLS = {
     "fh_tmul": (_i32, [_vp, _vp, _vp]),
     "fh_residual": (_i32, [_vp, _vp, _vp, _vp]),
     "fh_sum_a": (_i32, [_vp, _vp]),
+    "fh_matrix_get": (_i32, [_vp, _vp, _vp, _vp]),
     "fh_h": (_i32, [_vp, _i32, _vp, _vp]),
     "fh_h1": (_i32, [_vp, _vp]),
     "fh_face_h": (_i32, [_vp, _i32, _vp, _vp]),
@@ -124,6 +125,7 @@ FH_SYMBOLS = {
     "fh_assemble_laplacian": (_i32, [_vp, _vp] + [_vp] * 10 + [_vp, _i32]),
     "fh_assemble_convection_diffusion": (_i32, [_vp, _vp, _vp, _dbl] + [_vp] * 9 + [_vp, _i32]),
     "fh_grad_gauss_linear": (_i32, [_vp, _vp] + [_vp] * 10),
+    "fh_relax": (_i32, [_vp, _vp, _dbl, _i32, _vp, _vp, _vp, _vp]),
     "fh_server_config_default": (None, [ctypes.POINTER(FhServerConfig)]),
     "fh_server_create": (_i32, [ctypes.POINTER(FhServerConfig), ctypes.POINTER(_vp)]),
     "fh_server_submit": (_i32, [_vp, ctypes.POINTER(FhRequest)]),
