@@ -1,6 +1,7 @@
 // C-ABI of the device assembly kernels (fh_assembly.hip).
 #include "fh_assembly.h"
 #include "fh_matrix.h"
+#include "fh_hops.h"
 
 #include <vector>
 
@@ -142,6 +143,23 @@ int fh_assemble_convection_diffusion(fh_matrix* M, const fh_boundary* b, const d
         launch_convection_diffusion(M->rows(), mesh->d_perm.p, mesh->nFaces, b->view(), in,
                                     M->d_upperF.p, M->d_lowerF.p, work.p, M->d_diag.p, d_ic,
                                     d_bc, d_source, addBoundaryDiag, M->stream);
+        matrix_coeffs_updated(M);
+    });
+}
+
+int fh_relax(fh_matrix* M, const fh_boundary* b, double alpha, int32_t nCmpt,
+             const double* d_ic, const double* d_psi, double* d_source, double* d_diag)
+{
+    return run([&] {
+        if (!M) fail(FH_ERR_INVALID, "NULL matrix handle");
+        check_boundary(b, M->mesh);
+        if (!M->hasCoeffs) fail(FH_ERR_INVALID, "matrix has no coefficients");
+        if (nCmpt < 1 || nCmpt > 9) fail(FH_ERR_INVALID, "nCmpt must be in 1..9, got %d", nCmpt);
+        if (alpha <= 0) return;
+        if (!d_psi || !d_source || (b->nB && !d_ic)) fail(FH_ERR_INVALID, "NULL argument");
+        launch_relax(M->rows(), M->mesh->d_perm.p, M->d_val.p, b->view(), d_ic, nCmpt, alpha,
+                     d_psi, M->d_diag.p, d_source, d_diag, M->stream);
+        // the diagonal changed: drop the cached factors
         matrix_coeffs_updated(M);
     });
 }

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include "fh_kernels.h"
+#include "fh_assembly.h"
 
 namespace fh
 {
@@ -18,6 +19,13 @@ void launch_sum_mag_off_diag(const RowsView& R, const double* val, double* y, hi
 // y[f] = upper[f]*x[u[f]] - lower[f]*x[l[f]] per face (natural numbering)
 void launch_face_h(int32_t nFaces, const int32_t* l, const int32_t* u, const double* upper,
                    const double* lower, int32_t nC, const double* x, double* y, hipStream_t s);
+// fvMatrix<Type>::relax(alpha) with non-coupled patches: diag (library
+// numbering) replaced, source (natural, nC components) updated; diagOut
+// (natural, may be null) receives the relaxed diagonal
+void launch_relax(const RowsView& R, const int32_t* perm, const double* val,
+                  const BoundaryView& B, const double* ic, int32_t nC, double alpha,
+                  const double* psi, double* diag, double* source, double* diagOut,
+                  hipStream_t s);
 // component-wise permutations between the natural and library numbering
 void launch_permute_cmpt_in(int32_t n, int32_t nC, const int32_t* perm, const double* xNat,
                             double* xLib, hipStream_t s);

@@ -100,6 +100,70 @@ __global__ void k_face_h(int32_t nFaces, const int32_t* __restrict__ l,
     for (int32_t c = 0; c < nC; c++) y[(int64_t)f*nC + c] = U*x[ju + c] - L*x[jl + c];
 }
 
+// fvMatrix.C:594-740, fvMatrix<Type>::relax(alpha), every patch
+// non-coupled: per row the off-diagonal magnitude sum (sumMagOffDiag from
+// zero, row order = face order), the cell's boundary faces in patch order
+// (the largest internal-coefficient magnitude added, then after the
+// dominance and relaxation the smallest coefficient taken off), and the
+// source update S += (D - D0) psi.  max and min select as the reference's
+// (s1 > s2 ? s1 : s2, s1 < s2 ? s1 : s2).
+__global__ void k_relax(RowsView R, const int32_t* __restrict__ perm,
+                        const double* __restrict__ val, BoundaryView B,
+                        const double* __restrict__ ic, int32_t nC, double alpha,
+                        const double* __restrict__ psi, double* __restrict__ diag,
+                        double* __restrict__ source, double* __restrict__ diagOut)
+{
+    const int32_t r = blockIdx.x*HO_B + threadIdx.x;
+    if (r >= R.n) return;
+    const int32_t nEnt = R.rowInfo[r] >> 16;
+    const int32_t base = ho_base(R.sliceStart, r);
+    double sumOff = 0.0;
+    for (int32_t k = 0; k < nEnt; k++) sumOff += fabs(val[base + (k << 6)]);
+    const int32_t c = perm[r];
+    const double D0 = diag[r];
+    double D = D0;
+    const int32_t q0 = B.nB > 0 ? B.cellStart[c] : 0, q1 = B.nB > 0 ? B.cellStart[c + 1] : 0;
+    for (int32_t q = q0; q < q1; q++)
+    {
+        const double* v = ic + (int64_t)B.cellFaces[q]*nC;
+        double mx = fabs(v[0]);
+        for (int32_t k = 1; k < nC; k++)
+        {
+            const double a = fabs(v[k]);
+            mx = mx > a ? mx : a;
+        }
+        D += mx;
+    }
+    const double aD = fabs(D);
+    D = aD > sumOff ? aD : sumOff;
+    D /= alpha;
+    for (int32_t q = q0; q < q1; q++)
+    {
+        const double* v = ic + (int64_t)B.cellFaces[q]*nC;
+        double mn = v[0];
+        for (int32_t k = 1; k < nC; k++) mn = mn < v[k] ? mn : v[k];
+        D -= mn;
+    }
+    diag[r] = D;
+    if (diagOut) diagOut[c] = D;
+    const double dD = D - D0;
+    for (int32_t k = 0; k < nC; k++)
+    {
+        const int64_t i = (int64_t)c*nC + k;
+        source[i] += dD*psi[i];
+    }
+}
+
+void launch_relax(const RowsView& R, const int32_t* perm, const double* val,
+                  const BoundaryView& B, const double* ic, int32_t nC, double alpha,
+                  const double* psi, double* diag, double* source, double* diagOut,
+                  hipStream_t s)
+{
+    if (R.n == 0) return;
+    hipLaunchKernelGGL(k_relax, dim3((unsigned)((R.n + HO_B - 1)/HO_B)), dim3(HO_B), 0, s, R,
+                       perm, val, B, ic, nC, alpha, psi, diag, source, diagOut);
+}
+
 __global__ void k_permute_cmpt(int32_t n, int32_t nC, const int32_t* __restrict__ perm,
                                const double* __restrict__ in, double* __restrict__ out,
                                bool toLib)

@@ -210,6 +210,48 @@ void or_sumMagOffDiag(const or_matrix* m, double* sumOff)
     }
 }
 
+/* fvMatrix.C:594-740 (non-coupled patches; Foam::max and Foam::min select
+   s1 > s2 ? s1 : s2 and s1 < s2 ? s1 : s2; cmptMax/cmptMin fold from the
+   first component) */
+void or_relax(const or_matrix* m, int32_t nB, const int32_t* bFaceCells, const double* ic,
+              int32_t nCmpt, double alpha, const double* psi, double* diag, double* source)
+{
+    const int32_t N = m->addr->nCells;
+    if (alpha <= 0) return;
+    double* D0 = (double*)malloc(sizeof(double)*(N > 0 ? N : 1));
+    double* sumOff = (double*)calloc(N > 0 ? N : 1, sizeof(double));
+    for (int32_t c = 0; c < N; c++) D0[c] = diag[c];
+    or_sumMagOffDiag(m, sumOff);
+    for (int32_t b = 0; b < nB; b++)
+    {
+        const double* v = ic + (int64_t)b*nCmpt;
+        double mx = fabs(v[0]);
+        for (int32_t k = 1; k < nCmpt; k++) { const double a = fabs(v[k]); mx = mx > a ? mx : a; }
+        diag[bFaceCells[b]] += mx;
+    }
+    for (int32_t c = 0; c < N; c++)
+    {
+        const double a = fabs(diag[c]);
+        diag[c] = a > sumOff[c] ? a : sumOff[c];
+    }
+    for (int32_t c = 0; c < N; c++) diag[c] /= alpha;
+    for (int32_t b = 0; b < nB; b++)
+    {
+        const double* v = ic + (int64_t)b*nCmpt;
+        double mn = v[0];
+        for (int32_t k = 1; k < nCmpt; k++) mn = mn < v[k] ? mn : v[k];
+        diag[bFaceCells[b]] -= mn;
+    }
+    for (int32_t c = 0; c < N; c++)
+    {
+        const double dD = diag[c] - D0[c];
+        for (int32_t k = 0; k < nCmpt; k++)
+            source[(int64_t)c*nCmpt + k] += dD*psi[(int64_t)c*nCmpt + k];
+    }
+    free(D0);
+    free(sumOff);
+}
+
 /* lduMatrixATmul.C:95-151 */
 void or_tmul(const or_matrix* m, const double* psi, double* Tpsi)
 {

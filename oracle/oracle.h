@@ -82,6 +82,12 @@ void or_H1(const or_matrix* m, double* H1);
 int or_faceH(const or_matrix* m, int32_t nCmpt, const double* psi, double* faceHpsi);
 /* lduMatrixOperations.C:77-97: sumOff is added to */
 void or_sumMagOffDiag(const or_matrix* m, double* sumOff);
+/* fvMatrix.C:594-740, fvMatrix<Type>::relax(alpha) with every patch
+   non-coupled: bFaceCells and ic (nB x nCmpt) in patch order; diag
+   (m's off-diagonals are used, diag is the in/out diagonal) and source
+   (nCells x nCmpt) are updated in place */
+void or_relax(const or_matrix* m, int32_t nB, const int32_t* bFaceCells, const double* ic,
+              int32_t nCmpt, double alpha, const double* psi, double* diag, double* source);
 
 /* ---- selection ids (names as in the reference run-time tables) ---- */
 enum { OR_SOLVER_PCG = 0, OR_SOLVER_PBICGSTAB = 1, OR_SOLVER_PBICG = 2,

@@ -84,6 +84,7 @@ def lib():
         L.or_faceH.argtypes = [M, _i32, _vp, _vp]
         L.or_faceH.restype = ctypes.c_int
         L.or_sumMagOffDiag.argtypes = [M, _vp]
+        L.or_relax.argtypes = [M, _i32, _vp, _vp, _i32, _dbl, _vp, _vp, _vp]
         L.or_calcReciprocalD_DIC.argtypes = [M, _vp]
         L.or_calcReciprocalD_DILU.argtypes = [M, _vp]
         L.or_precondition.argtypes = [M, ctypes.c_int, _vp, _vp, _vp]
@@ -201,6 +202,17 @@ class Matrix:
 
     def sumMagOffDiag(self, sumOff):
         o = _f(sumOff).copy(); lib().or_sumMagOffDiag(self.ref, _p(o)); return o
+
+    def relax(self, bFaceCells, ic, alpha, psi, source):
+        """fvMatrix::relax(alpha), non-coupled patches: returns the relaxed
+        (diag, source); ic of shape (nB,) or (nB, nCmpt), psi and source
+        (nCells,) or (nCells, nCmpt)."""
+        ic, psi = _f(ic), _f(psi)
+        nc = 1 if psi.ndim == 1 else psi.shape[1]
+        d = self.diag.copy(); s = _f(source).copy()
+        bfc = np.ascontiguousarray(bFaceCells, np.int32)
+        lib().or_relax(self.ref, len(bfc), _p(bfc), _p(ic), nc, alpha, _p(psi), _p(d), _p(s))
+        return d, s
 
     def normFactor(self, psi, source):
         psi, source = _f(psi), _f(source)

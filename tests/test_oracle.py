@@ -614,3 +614,58 @@ def test_h_operators_identities(name, gen):
     np.add.at(div, lo, fH)
     np.add.at(div, up, -fH)
     np.testing.assert_allclose(div, x * m.H1() - m.H(x), rtol=1e-12, atol=1e-12)
+
+
+# ---- fvMatrix::relax --------------------------------------------------------
+
+def test_relax_hand_worked():
+    """fvMatrix.C:594-740 on three cells in a line, one boundary face at each
+    end; values worked by hand (all exact in binary)."""
+    lo = np.array([0, 1], np.int32)
+    up = np.array([1, 2], np.int32)
+    a = O.Addr(3, lo, up)
+    m = O.Matrix(a, [1.5, 2.0, 3.5], [-1.0, -2.0])
+    bfc = np.array([0, 2], np.int32)
+    psi = np.array([1.0, 2.0, 3.0])
+    # sumOff = [1, 3, 2]; D + max|ic| = [5.5, 2, 4]; max(|D|, sumOff) = [5.5, 3, 4];
+    # /0.5 = [11, 6, 8]; - min ic = [15, 6, 7.5]; S += (D - D0) psi
+    d, s = m.relax(bfc, [-4.0, 0.5], 0.5, psi, [10.0, 20.0, 30.0])
+    assert d.tolist() == [15.0, 6.0, 7.5]
+    assert s.tolist() == [23.5, 28.0, 42.0]
+    # three components: the largest magnitude in, the smallest value out
+    ic3 = np.array([[-4.0, 1.0, 2.0], [0.5, -0.25, 0.75]])
+    P3 = np.array([[1.0, 0.0, -1.0], [2.0, 1.0, 0.0], [3.0, 0.0, 1.0]])
+    d, s = m.relax(bfc, ic3, 0.5, P3, np.zeros((3, 3)))
+    assert d.tolist() == [15.0, 6.0, 8.75]
+    assert s.tolist() == [[13.5, 0.0, -13.5], [8.0, 4.0, 0.0], [15.75, 0.0, 5.25]]
+    # alpha <= 0: nothing changes
+    d, s = m.relax(bfc, [-4.0, 0.5], 0.0, psi, [10.0, 20.0, 30.0])
+    assert d.tolist() == [1.5, 2.0, 3.5] and s.tolist() == [10.0, 20.0, 30.0]
+
+
+@pytest.mark.parametrize("name,gen", MESHES)
+@pytest.mark.parametrize("kind", ["spd", "asym"])
+def test_relax_invariants(name, gen, kind):
+    rng = np.random.default_rng(8)
+    n, lo, up = gen(rng)
+    a, m, A = mk(n, lo, up, rng, kind)
+    nB = 2 * n
+    bfc = rng.integers(0, n, nB).astype(np.int32)
+    ic = rng.standard_normal((nB, 3))
+    psi = rng.standard_normal((n, 3))
+    S = rng.standard_normal((n, 3))
+    d1, s1 = m.relax(bfc, ic, 0.7, psi, S)
+    # the relaxed equation has the same residual at psi: A' psi - S' = A psi - S
+    A1 = A - np.diag(np.diag(A)) + np.diag(d1)
+    np.testing.assert_allclose(A1 @ psi - s1, A @ psi - S, rtol=1e-11, atol=1e-11)
+    # (D + sum of the faces' smallest coefficients) * alpha is the dominant
+    # diagonal, the same for every alpha
+    mn = np.zeros(n)
+    np.add.at(mn, bfc, ic.min(axis=1))
+    d2, _ = m.relax(bfc, ic, 0.35, psi, S)
+    np.testing.assert_allclose((d1 + mn) * 0.7, (d2 + mn) * 0.35, rtol=1e-13, atol=1e-13)
+    dom = (d1 + mn) * 0.7
+    off = np.abs(A - np.diag(np.diag(A))).sum(axis=1)
+    mx = np.zeros(n)
+    np.add.at(mx, bfc, np.abs(ic).max(axis=1))
+    np.testing.assert_allclose(dom, np.maximum(np.abs(np.diag(A) + mx), off), rtol=1e-13)
