@@ -122,7 +122,20 @@ typedef struct fh_controls
                                         iterations; -1 = tile-local variant of
                                         `preconditioner` on hex boxes, else the
                                         same preconditioner                 */
+    int32_t agglomerator;            /* GAMG: FH_AGGLOM_*, default
+                                        algebraicPair (faceAreaPair, the
+                                        reference's default, needs the mesh's
+                                        face areas: fh_mesh_set_face_areas)  */
 } fh_controls;
+
+/* GAMGAgglomeration types (GAMGAgglomeration.C:291-340) */
+enum
+{
+    FH_AGGLOM_ALGEBRAIC_PAIR = 0,   /* algebraicPairGAMGAgglomeration.C:57-64:
+                                       face weights from the coefficients  */
+    FH_AGGLOM_FACE_AREA_PAIR = 1    /* faceAreaPairGAMGAgglomeration.C:55-80:
+                                       face weights from the face areas   */
+};
 
 /* SolverPerformance<scalar> (SolverPerformance.H:84-92) plus extras. */
 typedef struct fh_perf
@@ -173,6 +186,12 @@ void        fh_gamg_reset_direction(void);
 int fh_mesh_create(int32_t nCells, int32_t nFaces, const int32_t* lowerAddr,
                    const int32_t* upperAddr, fh_mesh** mesh);
 int fh_mesh_info_get(const fh_mesh* mesh, fh_mesh_info* info);
+/* The face area vectors Sf [nFaces x 3] and magnitudes magSf [nFaces] of
+   the internal faces (fvMesh::Sf(), magSf()), host arrays: the weights of
+   the faceAreaPair agglomeration, |(Sf/sqrt(magSf)) * (1, 1.01, 1.02)|
+   (faceAreaPairGAMGAgglomeration.C:55-80).  A change drops the mesh's
+   cached agglomeration. */
+int fh_mesh_set_face_areas(fh_mesh* mesh, const double* Sf, const double* magSf);
 int fh_mesh_destroy(fh_mesh* mesh);
 
 /* ---- matrix: lduMatrix coefficients (lduMatrix.H:79-87) ----------------
@@ -333,6 +352,19 @@ int fh_assemble_convection_diffusion(fh_matrix* matrix, const fh_boundary* bound
 int fh_relax(fh_matrix* matrix, const fh_boundary* boundary, double alpha, int32_t nCmpt,
              const double* d_internalCoeffs, const double* d_psi, double* d_source,
              double* d_diag);
+
+/* fvMatrix<Type>::flux() (fvMatrix.C:953-1028) for a matrix whose patches
+   are all non-coupled: internal faces upper psi(neighbour) - lower
+   psi(owner) (faceH, per component), boundary faces internalCoeffs *
+   psi(faceCell) - boundaryCoeffs (component by component), each plus the
+   face flux correction when d_corr / d_bCorr are given (NULL: none).
+   Device arrays, natural order, component fastest: d_psi nCells x nCmpt,
+   d_internalCoeffs / d_boundaryCoeffs / d_bCorr / d_bFlux nBFaces x nCmpt,
+   d_corr / d_flux nFaces x nCmpt. */
+int fh_flux(fh_matrix* matrix, const fh_boundary* boundary, int32_t nCmpt,
+            const double* d_psi, const double* d_internalCoeffs,
+            const double* d_boundaryCoeffs, const double* d_corr, const double* d_bCorr,
+            double* d_flux, double* d_bFlux);
 
 /* fvc::grad(phi), Gauss linear (gaussGrad.C:30-75) with the patch-normal
    boundary correction (gradScheme.C:123-148).  d_grad [nCells*3],

@@ -1,7 +1,9 @@
 #include "hipLduSolver.H"
 #include "addToRunTimeSelectionTable.H"
+#include "fvMesh.H"
 
 #include <map>
+#include <set>
 
 namespace Foam
 {
@@ -152,19 +154,22 @@ void Foam::hipLduSolver::readControls(const dictionary& dict, fh_controls& c)
     c.mergeLevels = dict.lookupOrDefault<label>("mergeLevels", 1);
     c.nVcycles = dict.lookupOrDefault<label>("nVcycles", 2);
 
-    // the device agglomerates with algebraic pair weights only
-    // (algebraicPairGAMGAgglomeration.C:47-65); the reference default,
-    // faceAreaPair (GAMGAgglomeration.C:312), needs face areas and is
-    // replaced by algebraicPair when no agglomerator is named
-    if (dict.found("agglomerator"))
+    // GAMGAgglomeration::New (GAMGAgglomeration.C:291-340): faceAreaPair
+    // by default, its face areas handed to the device mesh in solve()
+    const word agg(dict.lookupOrDefault<word>("agglomerator", "faceAreaPair"));
+    if (agg == "faceAreaPair")
     {
-        const word agg(dict.lookup("agglomerator"));
-        if (agg != "algebraicPair")
-        {
-            FatalIOErrorInFunction(dict)
-                << "Unsupported agglomerator " << agg << "; valid: algebraicPair"
-                << exit(FatalIOError);
-        }
+        c.agglomerator = FH_AGGLOM_FACE_AREA_PAIR;
+    }
+    else if (agg == "algebraicPair")
+    {
+        c.agglomerator = FH_AGGLOM_ALGEBRAIC_PAIR;
+    }
+    else
+    {
+        FatalIOErrorInFunction(dict)
+            << "Unsupported agglomerator " << agg
+            << "; valid: algebraicPair faceAreaPair" << exit(FatalIOError);
     }
 
     const word precision(dict.lookupOrDefault<word>("precision", "double"));
@@ -207,6 +212,34 @@ Foam::solverPerformance Foam::hipLduSolver::solve
 
     fh_mesh* mesh = deviceMesh(matrix_.lduAddr());
     fh_matrix* m = deviceMatrix(mesh);
+
+    // faceAreaPair weights from fvMesh::Sf() and magSf()
+    // (faceAreaPairGAMGAgglomeration.C:55-80), once per device mesh
+    static std::set<fh_mesh*> withFaceAreas;
+    const bool gamg =
+        solverId_ == FH_SOLVER_GAMG || c.preconditioner == FH_PRECON_GAMG;
+    if
+    (
+        gamg
+     && c.agglomerator == FH_AGGLOM_FACE_AREA_PAIR
+     && !withFaceAreas.count(mesh)
+    )
+    {
+        const fvMesh& fvm = refCast<const fvMesh>(matrix_.mesh());
+        const vectorField& Sf = fvm.Sf().primitiveField();
+        const scalarField& magSf = fvm.magSf().primitiveField();
+        check
+        (
+            fh_mesh_set_face_areas
+            (
+                mesh,
+                reinterpret_cast<const scalar*>(Sf.cdata()),
+                magSf.cdata()
+            ),
+            "fh_mesh_set_face_areas"
+        );
+        withFaceAreas.insert(mesh);
+    }
 
     const scalar* upper = matrix_.hasUpper() ? matrix_.upper().begin() : nullptr;
     const scalar* lower =

@@ -6,6 +6,7 @@
 #include "fh_hops.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstring>
 #include <new>
@@ -169,6 +170,7 @@ void fh_controls_default(fh_controls* c)
     c->innerRelTol = 1e-4;
     c->maxRefinements = 50;
     c->innerPreconditioner = -1;
+    c->agglomerator = FH_AGGLOM_ALGEBRAIC_PAIR;
 }
 
 int fh_solver_lookup(const char* name, int32_t* id)
@@ -207,6 +209,27 @@ int fh_mesh_create(int32_t nCells, int32_t nFaces, const int32_t* lowerAddr,
             throw;
         }
         *out = m;
+    });
+}
+
+int fh_mesh_set_face_areas(fh_mesh* mesh, const double* Sf, const double* magSf)
+{
+    return guarded([&] {
+        if (!mesh) fail(FH_ERR_INVALID, "NULL mesh handle");
+        const int32_t F = mesh->nFaces;
+        if (F > 0 && (!Sf || !magSf)) fail(FH_ERR_INVALID, "NULL face areas");
+        // mag(cmptMultiply(Sf/sqrt(magSf), (1, 1.01, 1.02))), the reference's
+        // operation order: each component divided by sqrt(magSf), scaled,
+        // then sqrt((x*x + y*y) + z*z)
+        std::vector<double> w(F);
+        for (int32_t f = 0; f < F; f++)
+        {
+            const double s = std::sqrt(magSf[f]);
+            const double x = (Sf[3*f]/s)*1.0, y = (Sf[3*f + 1]/s)*1.01,
+                         z = (Sf[3*f + 2]/s)*1.02;
+            w[f] = std::sqrt(x*x + y*y + z*z);
+        }
+        gamg_set_face_weights(mesh, std::move(w));
     });
 }
 

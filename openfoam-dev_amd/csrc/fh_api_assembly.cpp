@@ -164,6 +164,30 @@ int fh_relax(fh_matrix* M, const fh_boundary* b, double alpha, int32_t nCmpt,
     });
 }
 
+int fh_flux(fh_matrix* M, const fh_boundary* b, int32_t nCmpt, const double* d_psi,
+            const double* d_ic, const double* d_bc, const double* d_corr, const double* d_bCorr,
+            double* d_flux, double* d_bFlux)
+{
+    return run([&] {
+        if (!M) fail(FH_ERR_INVALID, "NULL matrix handle");
+        check_boundary(b, M->mesh);
+        if (!M->hasCoeffs) fail(FH_ERR_INVALID, "matrix has no coefficients");
+        if (nCmpt < 1 || nCmpt > 9) fail(FH_ERR_INVALID, "nCmpt must be in 1..9, got %d", nCmpt);
+        if (M->diagonalOnly)
+            fail(FH_ERR_INVALID, "Cannot calculate faceH the matrix does not have any "
+                                 "off-diagonal coefficients.");
+        fh_mesh* mesh = M->mesh;
+        if (!d_psi || (mesh->nFaces && !d_flux)
+            || (b->nB && (!d_ic || !d_bc || !d_bFlux)))
+            fail(FH_ERR_INVALID, "NULL argument");
+        const double* lower = M->symmetric ? M->d_upperF.p : M->d_lowerF.p;
+        launch_flux(mesh->nFaces, mesh->d_lower.p, mesh->d_upper.p, M->d_upperF.p, lower,
+                    b->view(), nCmpt, d_psi, d_ic, d_bc, d_corr, d_bCorr, d_flux, d_bFlux,
+                    M->stream);
+        FH_HIP(hipStreamSynchronize(M->stream));
+    });
+}
+
 int fh_grad_gauss_linear(fh_mesh* mesh, const fh_boundary* b, const double* d_phi,
                          const double* d_weights, const double* d_Sf, const double* d_V,
                          const double* d_bSf, const double* d_bMagSf, const double* d_bPhi,

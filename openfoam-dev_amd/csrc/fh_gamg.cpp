@@ -284,7 +284,7 @@ struct AggLevel
 
 struct GamgAgglomeration
 {
-    int32_t minCells = -1, mergeLevels = -1;
+    int32_t minCells = -1, mergeLevels = -1, kind = -1;
     std::vector<std::unique_ptr<AggLevel>> levels;
     // natural rows of the coarsest mesh for the single-workgroup solve
     DBuf<int32_t> csRowStart, csEnt, csCol;
@@ -333,12 +333,17 @@ static std::shared_ptr<GamgAgglomeration> build_agglomeration(fh_matrix* M,
     const int32_t F = mesh->nFaces;
     hipStream_t s = M->stream;
     // algebraicPairGAMGAgglomeration.C:57-64: face weights mag(upper), or
-    // max(mag(upper), mag(lower)) for an asymmetric matrix
-    std::vector<double> up(F), lo;
-    if (F > 0)
+    // max(mag(upper), mag(lower)) for an asymmetric matrix;
+    // faceAreaPairGAMGAgglomeration.C:55-80: the mesh's face-area weights
+    const bool byArea = c->agglomerator == FH_AGGLOM_FACE_AREA_PAIR;
+    if (byArea && (int32_t)mesh->faceAreaWeights.size() != F)
+        fail(FH_ERR_INVALID, "agglomerator faceAreaPair needs the mesh's face areas "
+                             "(fh_mesh_set_face_areas)");
+    std::vector<double> up(byArea ? 0 : F), lo;
+    if (!byArea && F > 0)
         FH_HIP(hipMemcpyAsync(up.data(), M->d_upperF.p, sizeof(double)*F,
                               hipMemcpyDeviceToHost, s));
-    if (!M->symmetric && F > 0)
+    if (!byArea && !M->symmetric && F > 0)
     {
         lo.resize(F);
         FH_HIP(hipMemcpyAsync(lo.data(), M->d_lowerF.p, sizeof(double)*F,
@@ -346,16 +351,20 @@ static std::shared_ptr<GamgAgglomeration> build_agglomeration(fh_matrix* M,
     }
     FH_HIP(hipStreamSynchronize(s));
     std::vector<double> w(F);
-    for (int32_t f = 0; f < F; f++)
-    {
-        const double u = std::fabs(up[f]);
-        w[f] = lo.empty() ? u : std::max(u, std::fabs(lo[f]));
-    }
+    if (byArea)
+        w = mesh->faceAreaWeights;
+    else
+        for (int32_t f = 0; f < F; f++)
+        {
+            const double u = std::fabs(up[f]);
+            w[f] = lo.empty() ? u : std::max(u, std::fabs(lo[f]));
+        }
     std::vector<HostLevel> host = agglomerate(mesh->nCells, mesh->lower, mesh->upper, w,
                                               c->nCellsInCoarsestLevel, c->mergeLevels);
     auto A = std::make_shared<GamgAgglomeration>();
     A->minCells = c->nCellsInCoarsestLevel;
     A->mergeLevels = c->mergeLevels;
+    A->kind = c->agglomerator;
     const fh_mesh* fm = mesh;
     for (const HostLevel& L : host)
     {
@@ -722,6 +731,13 @@ static void setup_coarsest(fh_matrix* M, GamgHierarchy* H, int mode)
 // direction are shared by every matrix (and server thread) on the mesh
 static std::mutex g_aggMutex;
 
+void gamg_set_face_weights(fh_mesh* mesh, std::vector<double>&& w)
+{
+    std::lock_guard<std::mutex> lk(g_aggMutex);
+    mesh->faceAreaWeights = std::move(w);
+    mesh->gamgAgg.reset();
+}
+
 void gamg_prepare(fh_matrix* M, const fh_controls* c)
 {
     fh_mesh* mesh = M->mesh;
@@ -730,7 +746,7 @@ void gamg_prepare(fh_matrix* M, const fh_controls* c)
         std::lock_guard<std::mutex> lk(g_aggMutex);
         agg = std::static_pointer_cast<GamgAgglomeration>(mesh->gamgAgg);
         if (!agg || agg->minCells != c->nCellsInCoarsestLevel
-            || agg->mergeLevels != c->mergeLevels)
+            || agg->mergeLevels != c->mergeLevels || agg->kind != c->agglomerator)
         {
             agg = build_agglomeration(M, c);
             mesh->gamgAgg = agg;

@@ -4,6 +4,8 @@
 
 #include "fh_matrix.h"
 
+#include <vector>
+
 namespace fh
 {
 // coefficients changed: coarse matrices must be re-agglomerated
@@ -13,6 +15,9 @@ void gamg_release(fh_matrix* M);
 // build (or reuse) the hierarchy for these controls and bring the coarse
 // coefficients up to date
 void gamg_prepare(fh_matrix* M, const fh_controls* c);
+// the faceAreaPair weights of a mesh's internal faces (drops its cached
+// agglomeration)
+void gamg_set_face_weights(fh_mesh* mesh, std::vector<double>&& w);
 // GAMG preconditioner: w = nVcycles V-cycles on A w = r from w = 0
 template<class T>
 void gamg_apply(fh_matrix* M, const T* r, T* w, DevState* st);

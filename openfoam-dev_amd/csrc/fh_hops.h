@@ -26,6 +26,12 @@ void launch_relax(const RowsView& R, const int32_t* perm, const double* val,
                   const BoundaryView& B, const double* ic, int32_t nC, double alpha,
                   const double* psi, double* diag, double* source, double* diagOut,
                   hipStream_t s);
+// fvMatrix<Type>::flux() with non-coupled patches: y (nFaces x nC) and
+// by (nB x nC); corr, bCorr: the face flux correction or null
+void launch_flux(int32_t nFaces, const int32_t* l, const int32_t* u, const double* upper,
+                 const double* lower, const BoundaryView& B, int32_t nC, const double* x,
+                 const double* ic, const double* bc, const double* corr, const double* bCorr,
+                 double* y, double* by, hipStream_t s);
 // component-wise permutations between the natural and library numbering
 void launch_permute_cmpt_in(int32_t n, int32_t nC, const int32_t* perm, const double* xNat,
                             double* xLib, hipStream_t s);

@@ -24,6 +24,8 @@ __device__ __forceinline__ int32_t ho_base(const int32_t* sliceStart, int32_t r)
     return sliceStart[r >> 6] + (r & 63);
 }
 
+static dim3 grid_of(int64_t n) { return dim3((unsigned)((n + HO_B - 1)/HO_B)); }
+
 // lduMatrixTemplates.C:33-64 for NC components (NC = 0: nC at run time,
 // one component per launch, component c0)
 template<int NC>
@@ -164,6 +166,57 @@ void launch_relax(const RowsView& R, const int32_t* perm, const double* val,
                        perm, val, B, ic, nC, alpha, psi, diag, source, diagOut);
 }
 
+// fvMatrix.C:953-1028, fvMatrix<Type>::flux() with non-coupled patches:
+// internal faces faceH (+ the face flux correction), boundary faces
+// cmptMultiply(internalCoeffs, patchInternalField) - boundaryCoeffs
+// (+ its boundary part); one thread per face
+__global__ void k_flux_faces(int32_t nFaces, const int32_t* __restrict__ l,
+                             const int32_t* __restrict__ u, const double* __restrict__ upper,
+                             const double* __restrict__ lower, int32_t nC,
+                             const double* __restrict__ x, const double* __restrict__ corr,
+                             double* __restrict__ y)
+{
+    const int32_t f = blockIdx.x*HO_B + threadIdx.x;
+    if (f >= nFaces) return;
+    const double U = upper[f], L = lower[f];
+    const int64_t jl = (int64_t)l[f]*nC, ju = (int64_t)u[f]*nC;
+    for (int32_t c = 0; c < nC; c++)
+    {
+        const int64_t i = (int64_t)f*nC + c;
+        const double h = U*x[ju + c] - L*x[jl + c];
+        y[i] = corr ? h + corr[i] : h;
+    }
+}
+
+__global__ void k_flux_bfaces(int32_t nB, const int32_t* __restrict__ faceCells, int32_t nC,
+                              const double* __restrict__ x, const double* __restrict__ ic,
+                              const double* __restrict__ bc, const double* __restrict__ corr,
+                              double* __restrict__ y)
+{
+    const int32_t b = blockIdx.x*HO_B + threadIdx.x;
+    if (b >= nB) return;
+    const int64_t j = (int64_t)faceCells[b]*nC;
+    for (int32_t c = 0; c < nC; c++)
+    {
+        const int64_t i = (int64_t)b*nC + c;
+        const double v = ic[i]*x[j + c] - bc[i];
+        y[i] = corr ? v + corr[i] : v;
+    }
+}
+
+void launch_flux(int32_t nFaces, const int32_t* l, const int32_t* u, const double* upper,
+                 const double* lower, const BoundaryView& B, int32_t nC, const double* x,
+                 const double* ic, const double* bc, const double* corr, const double* bCorr,
+                 double* y, double* by, hipStream_t s)
+{
+    if (nFaces > 0)
+        hipLaunchKernelGGL(k_flux_faces, grid_of(nFaces), dim3(HO_B), 0, s, nFaces, l, u, upper,
+                           lower, nC, x, corr, y);
+    if (B.nB > 0)
+        hipLaunchKernelGGL(k_flux_bfaces, grid_of(B.nB), dim3(HO_B), 0, s, B.nB, B.faceCells,
+                           nC, x, ic, bc, bCorr, by);
+}
+
 __global__ void k_permute_cmpt(int32_t n, int32_t nC, const int32_t* __restrict__ perm,
                                const double* __restrict__ in, double* __restrict__ out,
                                bool toLib)
@@ -175,8 +228,6 @@ __global__ void k_permute_cmpt(int32_t n, int32_t nC, const int32_t* __restrict_
     if (toLib) out[i] = in[nat];
     else out[nat] = in[i];
 }
-
-static dim3 grid_of(int64_t n) { return dim3((unsigned)((n + HO_B - 1)/HO_B)); }
 
 void launch_h(const RowsView& R, const double* val, int32_t nC, const double* x, double* y,
               hipStream_t s)

@@ -155,6 +155,9 @@ struct fh_mesh
     // GAMG agglomeration cached with the mesh (built on the first GAMG
     // solve, as GAMGAgglomeration is cached on the mesh database)
     std::shared_ptr<void> gamgAgg;
+    // faceAreaPair agglomeration weights, one per internal face (empty
+    // until the face areas are given)
+    std::vector<double> faceAreaWeights;
 
     // multi-colour row order (greedy over the caller's cell order; red-black
     // on a box), built on first use by the FP32 Gauss-Seidel of the mixed

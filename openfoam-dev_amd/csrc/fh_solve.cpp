@@ -801,6 +801,9 @@ const char* precon_name(int p)
 // lduMatrixPreconditioner.C:94-140)
 static void validate_selection(const fh_matrix* M, const fh_controls* c)
 {
+    if (c->agglomerator != FH_AGGLOM_ALGEBRAIC_PAIR && c->agglomerator != FH_AGGLOM_FACE_AREA_PAIR)
+        fail(FH_ERR_INVALID, "Unknown GAMGAgglomeration type %d; valid: algebraicPair faceAreaPair",
+             c->agglomerator);
     if (M->diagonalOnly) return;
     const bool sym = M->symmetric;
     const int s = c->solver;

@@ -72,6 +72,13 @@ class lduAddressing:
                                     _ptr(self.upper), ctypes.byref(h)))
         self.handle = h
 
+    def set_face_areas(self, Sf, magSf):
+        """fvMesh::Sf() (nFaces x 3) and magSf() of the internal faces: the
+        weights of the faceAreaPair agglomeration."""
+        self._Sf = _f64(Sf).reshape(-1)
+        self._magSf = _f64(magSf)
+        _check(lib().fh_mesh_set_face_areas(self.handle, _ptr(self._Sf), _ptr(self._magSf)))
+
     def info(self):
         inf = FhMeshInfo()
         _check(lib().fh_mesh_info_get(self.handle, ctypes.byref(inf)))
@@ -318,8 +325,10 @@ def controls_from_dict(d):
             if isinstance(v, str):
                 v = v.lower() in ("yes", "on", "true", "1")
             setattr(c, k, int(bool(v)))
-    if "agglomerator" in d and d["agglomerator"] not in ("algebraicPair", "faceAreaPair"):
-        raise FoamHipError(1, f"Unknown GAMGAgglomeration type {d['agglomerator']}")
+    if "agglomerator" in d:
+        if d["agglomerator"] not in ("algebraicPair", "faceAreaPair"):
+            raise FoamHipError(1, f"Unknown GAMGAgglomeration type {d['agglomerator']}")
+        c.agglomerator = 1 if d["agglomerator"] == "faceAreaPair" else 0
     prec = d.get("precision", "double")
     if prec not in ("double", "mixed"):
         raise FoamHipError(1, f"Unknown precision {prec}; valid: double mixed")

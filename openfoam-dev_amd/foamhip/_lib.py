@@ -40,7 +40,7 @@ class FhControls(ctypes.Structure):
         ("nFinestSweeps", _i32), ("scaleCorrection", _i32),
         ("directSolveCoarsest", _i32), ("nCellsInCoarsestLevel", _i32),
         ("mergeLevels", _i32), ("nVcycles", _i32), ("innerRelTol", _dbl),
-        ("maxRefinements", _i32), ("innerPreconditioner", _i32),
+        ("maxRefinements", _i32), ("innerPreconditioner", _i32), ("agglomerator", _i32),
     ]
 
 
@@ -105,6 +105,7 @@ FH_SYMBOLS = {
     "fh_residual": (_i32, [_vp, _vp, _vp, _vp]),
     "fh_sum_a": (_i32, [_vp, _vp]),
     "fh_matrix_get": (_i32, [_vp, _vp, _vp, _vp]),
+    "fh_mesh_set_face_areas": (_i32, [_vp, _vp, _vp]),
     "fh_h": (_i32, [_vp, _i32, _vp, _vp]),
     "fh_h1": (_i32, [_vp, _vp]),
     "fh_face_h": (_i32, [_vp, _i32, _vp, _vp]),
@@ -126,6 +127,7 @@ FH_SYMBOLS = {
     "fh_assemble_convection_diffusion": (_i32, [_vp, _vp, _vp, _dbl] + [_vp] * 9 + [_vp, _i32]),
     "fh_grad_gauss_linear": (_i32, [_vp, _vp] + [_vp] * 10),
     "fh_relax": (_i32, [_vp, _vp, _dbl, _i32, _vp, _vp, _vp, _vp]),
+    "fh_flux": (_i32, [_vp, _vp, _i32] + [_vp] * 7),
     "fh_server_config_default": (None, [ctypes.POINTER(FhServerConfig)]),
     "fh_server_create": (_i32, [ctypes.POINTER(FhServerConfig), ctypes.POINTER(_vp)]),
     "fh_server_submit": (_i32, [_vp, ctypes.POINTER(FhRequest)]),

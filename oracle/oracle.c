@@ -210,6 +210,26 @@ void or_sumMagOffDiag(const or_matrix* m, double* sumOff)
     }
 }
 
+/* fvMatrix.C:953-1028 (non-coupled patches: NeighbourContrib is the
+   boundary coefficients themselves) */
+int or_flux(const or_matrix* m, int32_t nCmpt, const double* psi, int32_t nB,
+            const int32_t* bFaceCells, const double* ic, const double* bc, const double* corr,
+            const double* bCorr, double* flux, double* bFlux)
+{
+    if (or_faceH(m, nCmpt, psi, flux) != 0) return -1;
+    if (corr)
+        for (int64_t i = 0; i < (int64_t)m->addr->nFaces*nCmpt; i++) flux[i] += corr[i];
+    for (int32_t b = 0; b < nB; b++)
+        for (int32_t k = 0; k < nCmpt; k++)
+        {
+            const int64_t i = (int64_t)b*nCmpt + k;
+            const double internalContrib = ic[i]*psi[(int64_t)bFaceCells[b]*nCmpt + k];
+            bFlux[i] = internalContrib - bc[i];
+            if (bCorr) bFlux[i] += bCorr[i];
+        }
+    return 0;
+}
+
 /* fvMatrix.C:594-740 (non-coupled patches; Foam::max and Foam::min select
    s1 > s2 ? s1 : s2 and s1 < s2 ? s1 : s2; cmptMax/cmptMin fold from the
    first component) */
@@ -1143,6 +1163,18 @@ static void agglomerate_ldu_addressing(int32_t nFineFaces,
    combineLevels GAMGAgglomerateLduAddressing.C:493-611),
    continueAgglomerating GAMGAgglomeration.C:205-230,
    algebraicPair weights algebraicPairGAMGAgglomerationion.C:57-64 */
+/* faceAreaPairGAMGAgglomeration.C:55-80: vector / scalar divides each
+   component; mag = sqrt of the squares summed x, y, z */
+void or_face_area_weights(int32_t nFaces, const double* Sf, const double* magSf, double* w)
+{
+    for (int32_t f = 0; f < nFaces; f++)
+    {
+        const double s = sqrt(magSf[f]);
+        const double x = (Sf[3*f]/s)*1.0, y = (Sf[3*f + 1]/s)*1.01, z = (Sf[3*f + 2]/s)*1.02;
+        w[f] = sqrt(x*x + y*y + z*z);
+    }
+}
+
 int or_gamg_agglomerate(const or_matrix* m, const or_controls* c,
                         or_hierarchy* h)
 {
@@ -1164,10 +1196,12 @@ int or_gamg_agglomerate(const or_matrix* m, const or_controls* c,
     h->lower[0] = (int32_t*)a0->lower;   /* borrowed */
     h->upper[0] = (int32_t*)a0->upper;
 
-    /* level-0 weights */
+    /* level-0 weights: the face areas' (faceAreaPair) or the coefficients'
+       (algebraicPairGAMGAgglomeration.C:57-64) */
     double* weights = dalloc(a0->nFaces);
     for (int32_t f = 0; f < a0->nFaces; f++)
     {
+        if (c->faceWeights) { weights[f] = c->faceWeights[f]; continue; }
         const double u = fabs(m->upper[f]);
         const double l = fabs(m->lower[f]);
         weights[f] = (m->lower == m->upper) ? u : (u > l ? u : l);

@@ -82,6 +82,13 @@ void or_H1(const or_matrix* m, double* H1);
 int or_faceH(const or_matrix* m, int32_t nCmpt, const double* psi, double* faceHpsi);
 /* lduMatrixOperations.C:77-97: sumOff is added to */
 void or_sumMagOffDiag(const or_matrix* m, double* sumOff);
+/* fvMatrix.C:953-1028, fvMatrix<Type>::flux() with every patch
+   non-coupled: flux (nFaces x nCmpt) = faceH (+ corr), bFlux (nB x nCmpt)
+   = ic * psi(faceCell) - bc (+ bCorr); corr, bCorr may be NULL; returns
+   -1 for a diagonal-only matrix */
+int or_flux(const or_matrix* m, int32_t nCmpt, const double* psi, int32_t nB,
+            const int32_t* bFaceCells, const double* ic, const double* bc, const double* corr,
+            const double* bCorr, double* flux, double* bFlux);
 /* fvMatrix.C:594-740, fvMatrix<Type>::relax(alpha) with every patch
    non-coupled: bFaceCells and ic (nB x nCmpt) in patch order; diag
    (m's off-diagonals are used, diag is the in/out diagonal) and source
@@ -134,7 +141,14 @@ typedef struct
     int32_t nCellsInCoarsestLevel;
     int32_t mergeLevels;
     int32_t nVcycles;             /* GAMG preconditioner, default 2 */
+    /* faceAreaPair agglomeration: the face weights of or_face_area_weights
+       (NULL: algebraicPair, weights from the coefficients) */
+    const double* faceWeights;
 } or_controls;
+
+/* faceAreaPairGAMGAgglomeration.C:55-80: w = mag(cmptMultiply(Sf/sqrt(magSf),
+   (1, 1.01, 1.02))) per internal face; Sf [nFaces x 3] */
+void or_face_area_weights(int32_t nFaces, const double* Sf, const double* magSf, double* w);
 
 void or_controls_default(or_controls* c);
 

@@ -35,7 +35,7 @@ class OrControls(ctypes.Structure):
                 ("postSweepsLevelMultiplier", _i32), ("maxPostSweeps", _i32),
                 ("nFinestSweeps", _i32), ("scaleCorrection", _i32),
                 ("directSolveCoarsest", _i32), ("nCellsInCoarsestLevel", _i32),
-                ("mergeLevels", _i32), ("nVcycles", _i32)]
+                ("mergeLevels", _i32), ("nVcycles", _i32), ("faceWeights", _vp)]
 
 
 class OrPerf(ctypes.Structure):
@@ -85,6 +85,9 @@ def lib():
         L.or_faceH.restype = ctypes.c_int
         L.or_sumMagOffDiag.argtypes = [M, _vp]
         L.or_relax.argtypes = [M, _i32, _vp, _vp, _i32, _dbl, _vp, _vp, _vp]
+        L.or_face_area_weights.argtypes = [_i32, _vp, _vp, _vp]
+        L.or_flux.argtypes = [M, _i32, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+        L.or_flux.restype = ctypes.c_int
         L.or_calcReciprocalD_DIC.argtypes = [M, _vp]
         L.or_calcReciprocalD_DILU.argtypes = [M, _vp]
         L.or_precondition.argtypes = [M, ctypes.c_int, _vp, _vp, _vp]
@@ -203,6 +206,22 @@ class Matrix:
     def sumMagOffDiag(self, sumOff):
         o = _f(sumOff).copy(); lib().or_sumMagOffDiag(self.ref, _p(o)); return o
 
+    def flux(self, bFaceCells, ic, bc, psi, corr=None, bCorr=None):
+        """fvMatrix::flux(), non-coupled patches: (internal, boundary) face
+        fluxes; arrays (n,) or (n, nCmpt)."""
+        psi, ic, bc = _f(psi), _f(ic), _f(bc)
+        nc = 1 if psi.ndim == 1 else psi.shape[1]
+        nF, nB = self.addr.nFaces, len(bFaceCells)
+        fl = np.empty((nF,) if nc == 1 else (nF, nc))
+        bf = np.empty((nB,) if nc == 1 else (nB, nc))
+        bfc = np.ascontiguousarray(bFaceCells, np.int32)
+        corr = None if corr is None else _f(corr)
+        bCorr = None if bCorr is None else _f(bCorr)
+        if lib().or_flux(self.ref, nc, _p(psi), nB, _p(bfc), _p(ic), _p(bc), _p(corr),
+                         _p(bCorr), _p(fl), _p(bf)) != 0:
+            raise RuntimeError("Cannot calculate faceH")
+        return fl, bf
+
     def relax(self, bFaceCells, ic, alpha, psi, source):
         """fvMatrix::relax(alpha), non-coupled patches: returns the relaxed
         (diag, source); ic of shape (nB,) or (nB, nCmpt), psi and source
@@ -270,6 +289,8 @@ class Matrix:
         for k in ["scaleCorrection", "directSolveCoarsest"]:
             if k in controls:
                 setattr(c, k, int(bool(controls[k])))
+        fw = _face_weights(controls)
+        c.faceWeights = _p(fw)
         psi = _f(psi).copy(); source = _f(source)
         perf = OrPerf()
         hist = np.zeros(c.maxIter + 2) if history else None
@@ -300,11 +321,32 @@ def pair_agglomerate(addr, weights):
     return int(n), m
 
 
+def face_area_weights(Sf, magSf):
+    """faceAreaPair weights of the internal faces; Sf of shape (nFaces, 3)
+    or flat."""
+    Sf, magSf = _f(Sf), _f(magSf)
+    w = np.empty(magSf.size)
+    lib().or_face_area_weights(int(magSf.size), _p(Sf), _p(magSf), _p(w))
+    return w
+
+
+def _face_weights(controls):
+    """The faceAreaPair weights named by the controls ("agglomerator":
+    "faceAreaPair" with "Sf" and "magSf"), or None (algebraicPair)."""
+    if controls.get("agglomerator", "algebraicPair") != "faceAreaPair":
+        return None
+    return face_area_weights(controls["Sf"], controls["magSf"])
+
+
 def agglomerate(matrix, controls):
     """Full pair-agglomeration hierarchy; returns a list of level dicts."""
     c = OrControls()
     lib().or_controls_default(ctypes.byref(c))
+    fw = _face_weights(controls)
+    c.faceWeights = _p(fw)
     for k, v in controls.items():
+        if k in ("agglomerator", "Sf", "magSf"):
+            continue
         setattr(c, k, int(v))
     h = OrHierarchy()
     lib().or_gamg_agglomerate(matrix.ref, ctypes.byref(c), ctypes.byref(h))

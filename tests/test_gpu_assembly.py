@@ -170,3 +170,39 @@ def test_relax_bitwise(shape, kind, ncmpt):
     fh._check(fh.lib().fh_relax(m.handle, bnd.handle, ctypes.c_double(0.0), ncmpt, P(d_ic),
                                 P(d_psi), P(d_src), P(d_diag)))
     np.testing.assert_array_equal(host(d_src), before)
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("ncmpt,with_corr", [(1, False), (1, True), (3, True)])
+def test_flux_bitwise(shape, ncmpt, with_corr):
+    """fvMatrix::flux (fvMatrix.C:953-1028) of an assembled asymmetric
+    matrix: internal and boundary face fluxes."""
+    c = BoxCase(*shape, fixed_patches=(0, 2))
+    rng = np.random.default_rng(41 + sum(shape) + ncmpt)
+    c.bValue = rng.standard_normal(c.nB)
+    a = O.Addr(c.N, c.lower, c.upper)
+    phi, bPhi = c.fourier_flux(9)
+    diag, upper, lower, ic, bc = O.convection_diffusion(a, c, phi, bPhi, 0.02)
+    ic, bc = ic[:c.nB], bc[:c.nB]
+    if ncmpt > 1:
+        ic = np.stack([ic, 0.5 * ic, -ic], 1)
+        bc = np.stack([bc, -bc, 2 * bc], 1)
+    om = O.Matrix(a, diag, upper[:c.F], lower[:c.F])
+    mesh = fh.lduAddressing(c.N, c.lower, c.upper)
+    bnd = fh.Boundary(mesh, c.bFaceCells)
+    m = fh.lduMatrix(mesh, diag, upper[:c.F], lower[:c.F])
+    shp = lambda n: (n,) if ncmpt == 1 else (n, ncmpt)
+    psi = rng.standard_normal(shp(c.N))
+    corr = rng.standard_normal(shp(c.F)) if with_corr else None
+    bCorr = rng.standard_normal(shp(c.nB)) if with_corr else None
+    fl, bf = om.flux(c.bFaceCells, ic, bc, psi, corr, bCorr)
+    d_psi, d_ic, d_bc = dev(psi), dev(ic), dev(bc)
+    d_corr = dev(corr) if with_corr else None
+    d_bCorr = dev(bCorr) if with_corr else None
+    d_fl = torch.empty(c.F * ncmpt, dtype=torch.float64, device="cuda")
+    d_bf = torch.empty(c.nB * ncmpt, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    fh._check(fh.lib().fh_flux(m.handle, bnd.handle, ncmpt, P(d_psi), P(d_ic), P(d_bc),
+                               P(d_corr), P(d_bCorr), P(d_fl), P(d_bf)))
+    np.testing.assert_array_equal(host(d_fl), fl.reshape(-1))
+    np.testing.assert_array_equal(host(d_bf), bf.reshape(-1))

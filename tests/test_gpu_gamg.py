@@ -59,6 +59,31 @@ def test_gamg_fixed_cycles_trajectory():
     assert perf.finalResidual == pytest.approx(operf["finalResidual"], rel=1e-9)
 
 
+@pytest.mark.parametrize("smoother", ["symGaussSeidel", "DICGaussSeidel"])
+def test_gamg_face_area_pair(smoother):
+    """faceAreaPair (faceAreaPairGAMGAgglomeration.C:55-80): the hierarchy
+    from the face areas, trajectories against the oracle's."""
+    c, a, diag, upper, b = laplacian_case(16, sigma=1.0)
+    ctl = dict(GAMG, smoother=smoother, agglomerator="faceAreaPair")
+    om = O.Matrix(a, diag, upper)
+    O.reset_direction()
+    opsi, operf = om.solve(dict(ctl, Sf=c.Sf, magSf=c.magSf), np.zeros(c.N), b)
+    mesh = fh.lduAddressing(c.N, c.lower, c.upper)
+    mesh.set_face_areas(c.Sf, c.magSf)
+    m = fh.lduMatrix(mesh, diag, upper)
+    fh.lib().fh_gamg_reset_direction()
+    psi = np.zeros(c.N)
+    perf = fh.solver_New("p", m, ctl).solve(psi, b)
+    print("faceAreaPair", smoother, perf.nIterations, operf["nIterations"], rel_l2(psi, opsi))
+    assert perf.converged and operf["converged"]
+    assert abs(perf.nIterations - operf["nIterations"]) <= ITER_TOL
+    assert rel_l2(psi, opsi) <= SOLUTION_TOL
+    # without the face areas the agglomerator is refused
+    m2 = fh.lduMatrix(fh.lduAddressing(c.N, c.lower, c.upper), diag, upper)
+    with pytest.raises(fh.FoamHipError, match="face areas"):
+        fh.solver_New("p", m2, ctl).solve(np.zeros(c.N), b)
+
+
 @pytest.mark.parametrize("opts", [
     {"nPreSweeps": 2},
     {"scaleCorrection": False},

@@ -669,3 +669,69 @@ def test_relax_invariants(name, gen, kind):
     mx = np.zeros(n)
     np.add.at(mx, bfc, np.abs(ic).max(axis=1))
     np.testing.assert_allclose(dom, np.maximum(np.abs(np.diag(A) + mx), off), rtol=1e-13)
+
+
+# ---- faceAreaPair agglomeration --------------------------------------------
+
+def _box_face_areas(nx, ny, nz, lo, up, area=1.0):
+    """Axis-aligned face area vectors of a box mesh: the direction from the
+    index step between owner and neighbour."""
+    step = up - lo
+    Sf = np.zeros((len(lo), 3))
+    Sf[step == 1, 0] = area
+    Sf[step == nx, 1] = area
+    Sf[step == nx * ny, 2] = area
+    return Sf, np.full(len(lo), area)
+
+
+def test_face_area_weights_axis_aligned():
+    """faceAreaPairGAMGAgglomeration.C:55-80: an x, y or z face of area A
+    weighs sqrt(A) times 1, 1.01 or 1.02."""
+    A = 2.25
+    Sf = np.array([[A, 0, 0], [0, A, 0], [0, 0, A], [0, 0, -A]])
+    w = O.face_area_weights(Sf, np.full(4, A))
+    np.testing.assert_allclose(w, np.sqrt(A) * np.array([1.0, 1.01, 1.02, 1.02]), rtol=1e-15)
+
+
+def test_face_area_pair_agglomerates_along_z():
+    """On a uniform box the z faces weigh most, so the first level pairs
+    every cell with its z neighbour (nz even); algebraicPair with uniform
+    coefficients does not."""
+    nx, ny, nz = 4, 3, 6
+    n, lo, up = box(nx, ny, nz)
+    a = O.Addr(n, lo, up)
+    Sf, magSf = _box_face_areas(nx, ny, nz, lo, up)
+    m = O.Matrix(a, np.full(n, -6.0), np.ones(len(lo)))
+    O.reset_direction()
+    lev = O.agglomerate(m, {"nCellsInCoarsestLevel": 2, "agglomerator": "faceAreaPair",
+                            "Sf": Sf, "magSf": magSf})[0]
+    rmap = lev["restrictMap"]
+    assert lev["nCoarseCells"] == n // 2
+    for cc in range(n // 2):
+        cells = np.flatnonzero(rmap == cc)
+        assert len(cells) == 2 and cells[1] - cells[0] == nx * ny
+    O.reset_direction()
+    lev_alg = O.agglomerate(m, {"nCellsInCoarsestLevel": 2})[0]
+    assert not np.array_equal(lev_alg["restrictMap"], rmap)
+
+
+def test_flux_hand_worked():
+    """fvMatrix.C:953-1028 on three cells in a line with a boundary face at
+    each end: internal faces upper psi(N) - lower psi(P), boundary faces
+    ic psi(cell) - bc, plus the correction."""
+    lo = np.array([0, 1], np.int32)
+    up = np.array([1, 2], np.int32)
+    m = O.Matrix(O.Addr(3, lo, up), [1.0, 1.0, 1.0], [2.0, 3.0], [5.0, 7.0])
+    psi = np.array([1.0, 10.0, 100.0])
+    fl, bf = m.flux([0, 2], [-4.0, 0.5], [8.0, -1.0], psi)
+    assert fl.tolist() == [15.0, 230.0]
+    assert bf.tolist() == [-4.0 * 1 - 8.0, 0.5 * 100 + 1.0]
+    fl, bf = m.flux([0, 2], [-4.0, 0.5], [8.0, -1.0], psi, [0.5, -0.25], [1.0, 2.0])
+    assert fl.tolist() == [15.5, 229.75] and bf.tolist() == [-11.0, 53.0]
+    # components: cmptMultiply on the boundary
+    P3 = np.stack([psi, -psi, 2 * psi], 1)
+    ic3 = np.array([[1.0, 2.0, 3.0], [0.5, 0.25, -1.0]])
+    bc3 = np.zeros((2, 3))
+    fl, bf = m.flux([0, 2], ic3, bc3, P3)
+    assert bf.tolist() == [[1.0, -2.0, 6.0], [50.0, -25.0, -200.0]]
+    np.testing.assert_array_equal(fl[:, 1], -m.faceH(psi))
