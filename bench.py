@@ -413,6 +413,10 @@ def profile_only(fh, mat, torch, N):
         fh._check(fh.lib().fh_profile_kernel(mat.handle, kid, 0, FH_DIC, 5,
                                              ctypes.byref(ms), ctypes.byref(by)))
         log(f"kernel {kid} fp64: {ms.value:.4f} ms, {by.value:.0f} algorithmic bytes")
+    # lduMatrix::H (FP64)
+    fh._check(fh.lib().fh_profile_kernel(mat.handle, 5, 0, FH_DIC, 5,
+                                         ctypes.byref(ms), ctypes.byref(by)))
+    log(f"kernel 5 fp64: {ms.value:.4f} ms, {by.value:.0f} algorithmic bytes")
     src = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
     dst = torch.empty_like(src)
     src.fill_(1.0)
