@@ -1103,10 +1103,17 @@ static void mixed_pcg(fh_matrix* M, const fh_controls* c, double* psi, const dou
     // hex boxes: the FP32 product reads direction-packed coefficients
     const bool box = M->mesh->tiled;
     const float* dir[6] = {};
+    // and the FP64 residual replacements the packed FP64 coefficients
+    const double* dir64[6] = {};
     if (box)
     {
         ensure_dir32(M);
-        for (int d = 0; d < 6; d++) dir[d] = M->d_dir32[d].p;
+        ensure_dir64(M);
+        for (int d = 0; d < 6; d++)
+        {
+            dir[d] = M->d_dir32[d].p;
+            dir64[d] = M->d_dir64[d].p;
+        }
     }
     const fh_mesh* mesh = M->mesh;
     // boxes with nx a multiple of 4: p = w + beta p fused into the product,
@@ -1186,8 +1193,12 @@ static void mixed_pcg(fh_matrix* M, const fh_controls* c, double* psi, const dou
         kt_mark(M, FH_KERNEL_AXPY_NORM);
         if (!fu.counter) launch_finalize_blocks(nbu, N, part, FIN_MIXED_RES, st, 0, M->stream);
         const FusedFinalize ft = fused_fin(M, FIN_MIXED_TRUE);
-        const int32_t nbr = launch_mixed_replace(M->rows(), M->d_diag.p, M->d_val.p, psi, xc,
-                                                 src, r32, part, st, M->stream, ft);
+        const int32_t nbr =
+            box ? launch_mixed_replace_box(mesh->boxNx, mesh->boxNy, mesh->boxNz, M->d_diag.p,
+                                           dir64, M->symmetric, psi, xc, src, r32, part, st,
+                                           M->stream, ft)
+                : launch_mixed_replace(M->rows(), M->d_diag.p, M->d_val.p, psi, xc, src, r32,
+                                       part, st, M->stream, ft);
         if (!ft.counter) launch_finalize_blocks(nbr, N, part, FIN_MIXED_TRUE, st, 0, M->stream);
     });
     // the correction of the last iterations

@@ -109,6 +109,12 @@ int32_t launch_mixed_update(int32_t n, double* x, float* xc, float* r, const flo
                             hipStream_t s, const FusedFinalize& F = FusedFinalize());
 // mixed PCG residual replacement (only when st->replace): r = b - A (x + xc)
 // in FP64, r32 = r, sum |r|; returns the partial count
+// the same on a box from the packed FP64 coefficients (bitwise equal)
+int32_t launch_mixed_replace_box(int32_t nx, int32_t ny, int32_t nz, const double* diag,
+                                 const double* const dir[6], bool symmetric, const double* x,
+                                 const float* xc, const double* b, float* r32,
+                                 double* partials, const DevState* st, hipStream_t s,
+                                 const FusedFinalize& F);
 int32_t launch_mixed_replace(const RowsView& R, const double* diag, const double* val,
                              const double* x, const float* xc, const double* b, float* r32,
                              double* partials, const DevState* st, hipStream_t s,

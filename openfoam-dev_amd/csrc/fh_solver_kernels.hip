@@ -1089,6 +1089,54 @@ __global__ void k_mixed_replace(RowsView R, const double* __restrict__ diag,
     write_partials_fin(0.0, s1, partials, F);
 }
 
+// k_mixed_replace on a box from the packed FP64 coefficients (lK, lJ, lI
+// and, asymmetric, uI, uJ, uK; symmetric: the upper coefficient of a cell is
+// the lower one of its upper neighbour).  The row order of the SELL rows
+// (diag, -k, -j, -i, +i, +j, +k), the same cells per thread and the same
+// partials: the result and its sum are that kernel's to the bit, without
+// the column indices and the padded SELL values.
+__global__ void k_mixed_replace_box(int32_t nx, int32_t ny, int32_t nz,
+                                    const double* __restrict__ diag,
+                                    const double* __restrict__ lK, const double* __restrict__ lJ,
+                                    const double* __restrict__ lI, const double* __restrict__ uI,
+                                    const double* __restrict__ uJ, const double* __restrict__ uK,
+                                    const double* __restrict__ x, const float* __restrict__ xc,
+                                    const double* __restrict__ b, float* __restrict__ r32,
+                                    double* partials, const DevState* st, FusedFinalize F)
+{
+    if (st->done || !st->replace) return;
+    const int32_t sy = nx, sz = nx*ny, n = sz*nz;
+    double s1 = 0.0;
+    for (int32_t c = blockIdx.x*FH_BLOCK + threadIdx.x; c < n; c += gridDim.x*FH_BLOCK)
+    {
+        const int32_t gi = c % nx, gj = (c/nx) % ny, gk = c/sz;
+        double acc = b[c] - diag[c]*(x[c] + (double)xc[c]);
+        if (gk > 0) acc -= lK[c]*(x[c - sz] + (double)xc[c - sz]);
+        if (gj > 0) acc -= lJ[c]*(x[c - sy] + (double)xc[c - sy]);
+        if (gi > 0) acc -= lI[c]*(x[c - 1] + (double)xc[c - 1]);
+        if (gi < nx - 1) acc -= (uI ? uI[c] : lI[c + 1])*(x[c + 1] + (double)xc[c + 1]);
+        if (gj < ny - 1) acc -= (uJ ? uJ[c] : lJ[c + sy])*(x[c + sy] + (double)xc[c + sy]);
+        if (gk < nz - 1) acc -= (uK ? uK[c] : lK[c + sz])*(x[c + sz] + (double)xc[c + sz]);
+        r32[c] = (float)acc;
+        s1 += fabs(acc);
+    }
+    write_partials_fin(0.0, s1, partials, F);
+}
+
+int32_t launch_mixed_replace_box(int32_t nx, int32_t ny, int32_t nz, const double* diag,
+                                 const double* const dir[6], bool symmetric, const double* x,
+                                 const float* xc, const double* b, float* r32,
+                                 double* partials, const DevState* st, hipStream_t s,
+                                 const FusedFinalize& F)
+{
+    const int32_t nb = std::min(reduction_blocks(nx*ny*nz), MIX_BLOCKS);
+    hipLaunchKernelGGL(k_mixed_replace_box, dim3(nb), dim3(FH_BLOCK), 0, s, nx, ny, nz, diag,
+                       dir[0], dir[1], dir[2], symmetric ? nullptr : dir[3],
+                       symmetric ? nullptr : dir[4], symmetric ? nullptr : dir[5], x, xc, b,
+                       r32, partials, st, F);
+    return nb;
+}
+
 __global__ void k_mixed_setup(DevState* st, double ratio)
 {
     st->resLast = st->initialResidual;
