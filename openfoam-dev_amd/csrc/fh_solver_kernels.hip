@@ -1109,7 +1109,7 @@ __global__ void k_mixed_replace_box(int32_t nx, int32_t ny, int32_t nz,
     double s1 = 0.0;
     for (int32_t c = blockIdx.x*FH_BLOCK + threadIdx.x; c < n; c += gridDim.x*FH_BLOCK)
     {
-        const int32_t gi = c % nx, gj = (c/nx) % ny, gk = c/sz;
+        const int32_t q = c/nx, gk = q/ny, gi = c - q*nx, gj = q - gk*ny;
         double acc = b[c] - diag[c]*(x[c] + (double)xc[c]);
         if (gk > 0) acc -= lK[c]*(x[c - sz] + (double)xc[c - sz]);
         if (gj > 0) acc -= lJ[c]*(x[c - sy] + (double)xc[c - sy]);
