@@ -243,6 +243,10 @@ int fh_face_h(fh_matrix* m, int32_t nCmpt, const double* psi, double* faceHpsi);
 /* lduMatrixOperations.C:77-97: sumOff += the row's sum of |off-diagonal
    coefficient| (sumOff is read and written) */
 int fh_sum_mag_off_diag(fh_matrix* m, double* sumOff);
+/* fh_h and fh_face_h on device arrays (natural order, component fastest),
+   for callers whose fields stay in HBM */
+int fh_h_device(fh_matrix* m, int32_t nCmpt, const double* d_psi, double* d_Hpsi);
+int fh_face_h_device(fh_matrix* m, int32_t nCmpt, const double* d_psi, double* d_faceHpsi);
 /* lduMatrixSolver.C:174-197 */
 int fh_norm_factor(fh_matrix* m, const double* psi, const double* source,
                    double* normFactor);

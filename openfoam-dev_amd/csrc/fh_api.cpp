@@ -509,6 +509,46 @@ int fh_sum_mag_off_diag(fh_matrix* M, double* sumOff)
     });
 }
 
+int fh_h_device(fh_matrix* M, int32_t nCmpt, const double* d_psi, double* d_Hpsi)
+{
+    return guarded([&] {
+        check_matrix(M);
+        if (!d_psi || !d_Hpsi) fail(FH_ERR_INVALID, "NULL argument");
+        if (nCmpt < 1 || nCmpt > 9) fail(FH_ERR_INVALID, "nCmpt must be in 1..9, got %d", nCmpt);
+        const int32_t N = M->mesh->nCells;
+        if (N == 0) return;
+        if (M->diagonalOnly || M->mesh->nFaces == 0)
+        {
+            FH_HIP(hipMemsetAsync(d_Hpsi, 0, sizeof(double)*N*nCmpt, M->stream));
+            FH_HIP(hipStreamSynchronize(M->stream));
+            return;
+        }
+        DBuf<double> x, y;
+        x.alloc((size_t)N*nCmpt); y.alloc((size_t)N*nCmpt);
+        launch_permute_cmpt_in(N, nCmpt, M->mesh->d_perm.p, d_psi, x.p, M->stream);
+        launch_h(M->rows(), M->d_val.p, nCmpt, x.p, y.p, M->stream);
+        launch_permute_cmpt_out(N, nCmpt, M->mesh->d_perm.p, y.p, d_Hpsi, M->stream);
+        FH_HIP(hipStreamSynchronize(M->stream));
+    });
+}
+
+int fh_face_h_device(fh_matrix* M, int32_t nCmpt, const double* d_psi, double* d_faceHpsi)
+{
+    return guarded([&] {
+        check_matrix(M);
+        if (!d_psi || !d_faceHpsi) fail(FH_ERR_INVALID, "NULL argument");
+        if (nCmpt < 1 || nCmpt > 9) fail(FH_ERR_INVALID, "nCmpt must be in 1..9, got %d", nCmpt);
+        if (M->diagonalOnly)
+            fail(FH_ERR_INVALID, "Cannot calculate faceH the matrix does not have any "
+                                 "off-diagonal coefficients.");
+        const fh_mesh* mesh = M->mesh;
+        const double* lower = M->symmetric ? M->d_upperF.p : M->d_lowerF.p;
+        launch_face_h(mesh->nFaces, mesh->d_lower.p, mesh->d_upper.p, M->d_upperF.p, lower,
+                      nCmpt, d_psi, d_faceHpsi, M->stream);
+        FH_HIP(hipStreamSynchronize(M->stream));
+    });
+}
+
 int fh_norm_factor(fh_matrix* M, const double* psi, const double* source, double* normFactor)
 {
     return guarded([&] {

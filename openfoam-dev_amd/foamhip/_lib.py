@@ -110,6 +110,8 @@ FH_SYMBOLS = {
     "fh_h1": (_i32, [_vp, _vp]),
     "fh_face_h": (_i32, [_vp, _i32, _vp, _vp]),
     "fh_sum_mag_off_diag": (_i32, [_vp, _vp]),
+    "fh_h_device": (_i32, [_vp, _i32, _vp, _vp]),
+    "fh_face_h_device": (_i32, [_vp, _i32, _vp, _vp]),
     "fh_norm_factor": (_i32, [_vp, _vp, _vp, ctypes.POINTER(_dbl)]),
     "fh_reciprocal_d": (_i32, [_vp, _i32, _vp]),
     "fh_precondition": (_i32, [_vp, _i32, _vp, _vp]),

@@ -206,3 +206,27 @@ def test_flux_bitwise(shape, ncmpt, with_corr):
                                P(d_corr), P(d_bCorr), P(d_fl), P(d_bf)))
     np.testing.assert_array_equal(host(d_fl), fl.reshape(-1))
     np.testing.assert_array_equal(host(d_bf), bf.reshape(-1))
+
+
+@pytest.mark.parametrize("case", ["box", "rand_shuffled"])
+def test_h_device_matches_host(case):
+    """fh_h_device / fh_face_h_device on device arrays give the host entry
+    points' results (which match the oracle bitwise), also where the library
+    numbering differs from the caller's."""
+    from meshes import asym_coeffs, box, random_graph
+    rng = np.random.default_rng(51)
+    n, lo, up = box(13, 9, 7) if case == "box" else random_graph(3000, 6, rng,
+                                                                 shuffle_within_owner=True)
+    d, u, l = asym_coeffs(n, lo, up, rng)
+    mesh = fh.lduAddressing(n, lo, up)
+    m = fh.lduMatrix(mesh, d, u, l)
+    for nc in (1, 3):
+        X = rng.standard_normal((n, nc) if nc > 1 else n)
+        d_x = dev(X)
+        d_h = torch.empty(n * nc, dtype=torch.float64, device="cuda")
+        d_f = torch.empty(len(lo) * nc, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        fh._check(fh.lib().fh_h_device(m.handle, nc, P(d_x), P(d_h)))
+        fh._check(fh.lib().fh_face_h_device(m.handle, nc, P(d_x), P(d_f)))
+        np.testing.assert_array_equal(host(d_h), m.H(X).reshape(-1))
+        np.testing.assert_array_equal(host(d_f), m.faceH(X).reshape(-1))
