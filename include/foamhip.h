@@ -126,6 +126,14 @@ typedef struct fh_controls
                                         algebraicPair (faceAreaPair, the
                                         reference's default, needs the mesh's
                                         face areas: fh_mesh_set_face_areas)  */
+    /* GAMG preconditioner: the controls of the preconditioner
+       sub-dictionary (preconditioner { preconditioner GAMG; smoother ...; }),
+       which lduMatrix::preconditioner::New hands to GAMGPreconditioner as
+       its whole dictionary (lduMatrixPreconditioner.C:72-82,
+       GAMGPreconditioner.C:44-63): smoother, sweeps, agglomeration,
+       nVcycles and the tolerance / relTol of its coarsest-level solve.
+       NULL: the GAMG entries of these controls. */
+    const struct fh_controls* preconditionerControls;
 } fh_controls;
 
 /* GAMGAgglomeration types (GAMGAgglomeration.C:291-340) */

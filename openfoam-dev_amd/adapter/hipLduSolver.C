@@ -1,9 +1,9 @@
 #include "hipLduSolver.H"
 #include "addToRunTimeSelectionTable.H"
 #include "fvMesh.H"
+#include "surfaceFields.H"
 
 #include <map>
-#include <set>
 
 namespace Foam
 {
@@ -70,41 +70,116 @@ void Foam::hipLduSolver::check(int status, const char* what)
 }
 
 
+namespace
+{
+    // FNV-1a over a block of memory: identifies the contents of addressing
+    // and face-area arrays, so a new mesh at a reused address, or moved
+    // points, is not mistaken for the cached one
+    uint64_t contentHash(const void* data, size_t bytes, uint64_t h = 1469598103934665603ull)
+    {
+        const unsigned char* p = static_cast<const unsigned char*>(data);
+        for (size_t i = 0; i < bytes; i++)
+        {
+            h = (h ^ p[i])*1099511628211ull;
+        }
+        return h;
+    }
+
+    struct cachedMesh
+    {
+        Foam::label nCells;
+        Foam::label nFaces;
+        uint64_t hash;
+        fh_mesh* mesh;
+    };
+
+    std::map<fh_mesh*, fh_matrix*>& matrixCache()
+    {
+        static std::map<fh_mesh*, fh_matrix*> matrices;
+        return matrices;
+    }
+}
+
+
 fh_mesh* Foam::hipLduSolver::deviceMesh(const lduAddressing& addr)
 {
-    // keyed by the addressing object and its size; a topology change
-    // produces a new lduAddressing
-    static std::map<std::pair<const lduAddressing*, label>, fh_mesh*> meshes;
-    const auto key = std::make_pair(&addr, addr.size());
-    auto it = meshes.find(key);
-    if (it != meshes.end()) return it->second;
-    fh_mesh* m = nullptr;
-    check
+    // keyed by the addressing object; a hit is checked against the cell
+    // and face counts and the contents of lowerAddr/upperAddr, and a stale
+    // entry (topology change, or another addressing allocated at the same
+    // address) is destroyed with its matrix and rebuilt
+    static std::map<const lduAddressing*, cachedMesh> meshes;
+    const labelUList& l = addr.lowerAddr();
+    const labelUList& u = addr.upperAddr();
+    const uint64_t h = contentHash
     (
-        fh_mesh_create
-        (
-            addr.size(),
-            addr.lowerAddr().size(),
-            addr.lowerAddr().begin(),
-            addr.upperAddr().begin(),
-            &m
-        ),
-        "fh_mesh_create"
+        u.begin(),
+        u.size()*sizeof(label),
+        contentHash(l.begin(), l.size()*sizeof(label))
     );
-    meshes[key] = m;
+    auto it = meshes.find(&addr);
+    if (it != meshes.end())
+    {
+        const cachedMesh& cm = it->second;
+        if (cm.nCells == addr.size() && cm.nFaces == l.size() && cm.hash == h)
+        {
+            return cm.mesh;
+        }
+        auto mit = matrixCache().find(cm.mesh);
+        if (mit != matrixCache().end())
+        {
+            fh_matrix_destroy(mit->second);
+            matrixCache().erase(mit);
+        }
+        fh_mesh_destroy(cm.mesh);
+        meshes.erase(it);
+    }
+    fh_mesh* m = nullptr;
+    check(fh_mesh_create(addr.size(), l.size(), l.begin(), u.begin(), &m), "fh_mesh_create");
+    meshes[&addr] = cachedMesh{addr.size(), l.size(), h, m};
     return m;
 }
 
 
 fh_matrix* Foam::hipLduSolver::deviceMatrix(fh_mesh* mesh)
 {
-    static std::map<fh_mesh*, fh_matrix*> matrices;
+    std::map<fh_mesh*, fh_matrix*>& matrices = matrixCache();
     auto it = matrices.find(mesh);
     if (it != matrices.end()) return it->second;
     fh_matrix* m = nullptr;
     check(fh_matrix_create(mesh, &m), "fh_matrix_create");
     matrices[mesh] = m;
     return m;
+}
+
+
+void Foam::hipLduSolver::readControls
+(
+    const dictionary& dict,
+    fh_controls& c,
+    fh_controls& gamgPrecond
+)
+{
+    readControls(dict, c);
+
+    // a GAMG preconditioner is a GAMG solver built from the preconditioner
+    // sub-dictionary alone (lduMatrixPreconditioner.C:72-82,
+    // GAMGPreconditioner.C:44-63); the word form hands it dictionary::null,
+    // which has no smoother entry
+    if (c.preconditioner == FH_PRECON_GAMG)
+    {
+        const entry& e = dict.lookupEntry("preconditioner", false, false);
+        if (!e.isDict())
+        {
+            FatalIOErrorInFunction(dict)
+                << "preconditioner GAMG reads its controls from a preconditioner"
+                << " sub-dictionary: keyword smoother is undefined"
+                << exit(FatalIOError);
+        }
+        readControls(e.dict(), gamgPrecond);
+        gamgPrecond.solver = FH_SOLVER_GAMG;
+        gamgPrecond.preconditioner = FH_PRECON_NONE;
+        c.preconditionerControls = &gamgPrecond;
+    }
 }
 
 
@@ -206,39 +281,48 @@ Foam::solverPerformance Foam::hipLduSolver::solve
         }
     }
 
-    fh_controls c;
-    readControls(controlDict_, c);
+    fh_controls c, gamgPrecond;
+    readControls(controlDict_, c, gamgPrecond);
     c.solver = solverId_;
 
     fh_mesh* mesh = deviceMesh(matrix_.lduAddr());
     fh_matrix* m = deviceMatrix(mesh);
 
     // faceAreaPair weights from fvMesh::Sf() and magSf()
-    // (faceAreaPairGAMGAgglomeration.C:55-80), once per device mesh
-    static std::set<fh_mesh*> withFaceAreas;
+    // (faceAreaPairGAMGAgglomeration.C:55-80); handed over again when the
+    // face areas change (mesh motion), which drops the device mesh's cached
+    // agglomeration as movePoints drops the reference's GeometricMeshObject
+    static std::map<fh_mesh*, uint64_t> faceAreaHash;
+    const fh_controls& gc =
+        c.preconditionerControls ? *c.preconditionerControls : c;
     const bool gamg =
         solverId_ == FH_SOLVER_GAMG || c.preconditioner == FH_PRECON_GAMG;
-    if
-    (
-        gamg
-     && c.agglomerator == FH_AGGLOM_FACE_AREA_PAIR
-     && !withFaceAreas.count(mesh)
-    )
+    if (gamg && gc.agglomerator == FH_AGGLOM_FACE_AREA_PAIR)
     {
         const fvMesh& fvm = refCast<const fvMesh>(matrix_.mesh());
         const vectorField& Sf = fvm.Sf().primitiveField();
         const scalarField& magSf = fvm.magSf().primitiveField();
-        check
+        const uint64_t h = contentHash
         (
-            fh_mesh_set_face_areas
-            (
-                mesh,
-                reinterpret_cast<const scalar*>(Sf.cdata()),
-                magSf.cdata()
-            ),
-            "fh_mesh_set_face_areas"
+            magSf.cdata(),
+            magSf.size()*sizeof(scalar),
+            contentHash(Sf.cdata(), Sf.size()*sizeof(vector))
         );
-        withFaceAreas.insert(mesh);
+        auto it = faceAreaHash.find(mesh);
+        if (it == faceAreaHash.end() || it->second != h)
+        {
+            check
+            (
+                fh_mesh_set_face_areas
+                (
+                    mesh,
+                    reinterpret_cast<const scalar*>(Sf.cdata()),
+                    magSf.cdata()
+                ),
+                "fh_mesh_set_face_areas"
+            );
+            faceAreaHash[mesh] = h;
+        }
     }
 
     const scalar* upper = matrix_.hasUpper() ? matrix_.upper().begin() : nullptr;

@@ -70,6 +70,15 @@ public:
     //- fvSolution dictionary -> fh_controls
     static void readControls(const dictionary& dict, fh_controls& c);
 
+    //- The same, and for a GAMG preconditioner its sub-dictionary's
+    //  controls into gamgPrecond (pointed to by c.preconditionerControls)
+    static void readControls
+    (
+        const dictionary& dict,
+        fh_controls& c,
+        fh_controls& gamgPrecond
+    );
+
     //- Turn a foamhip status into FatalError
     static void check(int status, const char* what);
 };

@@ -171,6 +171,7 @@ void fh_controls_default(fh_controls* c)
     c->maxRefinements = 50;
     c->innerPreconditioner = -1;
     c->agglomerator = FH_AGGLOM_ALGEBRAIC_PAIR;
+    c->preconditionerControls = nullptr;
 }
 
 int fh_solver_lookup(const char* name, int32_t* id)

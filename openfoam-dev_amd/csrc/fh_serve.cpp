@@ -36,6 +36,7 @@ struct Pending
     const double *pDiag = nullptr, *pUpper = nullptr, *pLower = nullptr, *pSource = nullptr;
     bool symmetric = true;
     fh_controls controls{};
+    fh_controls precondControls{};     // owned copy of controls.preconditionerControls
     double submitTime = 0.0;
     int64_t footprint = 0;
 };
@@ -96,14 +97,23 @@ struct fh_server
                 return true;
             }
         }
+        // idle matrices (all of other meshes here) are released only when
+        // the request is admitted with it: a refused request frees nothing,
+        // so the bytes counted always match what is allocated
         const int64_t need = p->footprint;
-        while (residentBytes + need > cfg.memoryBudget && !idle.empty())
+        if (residentBytes + need > cfg.memoryBudget)
         {
-            release.push_back(idle.front());
-            residentBytes -= idle.front().bytes;
-            idle.pop_front();
+            int64_t freeable = 0;
+            for (const Resident& r : idle) freeable += r.bytes;
+            if (residentBytes - freeable + need > cfg.memoryBudget && inFlight > 0)
+                return false;              // wait for a running solve to end
+            while (residentBytes + need > cfg.memoryBudget && !idle.empty())
+            {
+                release.push_back(idle.front());
+                residentBytes -= idle.front().bytes;
+                idle.pop_front();
+            }
         }
-        if (residentBytes + need > cfg.memoryBudget && inFlight > 0) return false;
         residentBytes += need;
         res = Resident{nullptr, p->mesh, need};
         job = std::move(queue.front());
@@ -230,6 +240,12 @@ int fh_server_submit(fh_server* s, const fh_request* r)
         if (r->psi0) job->psi.assign(r->psi0, r->psi0 + N);
         else job->psi.assign(N, 0.0);
         job->controls = r->controls;
+        if (r->controls.preconditionerControls)
+        {
+            job->precondControls = *r->controls.preconditionerControls;
+            job->precondControls.preconditionerControls = nullptr;
+            job->controls.preconditionerControls = &job->precondControls;
+        }
         job->footprint = footprint_of(r->mesh);
         {
             std::lock_guard<std::mutex> lk(s->mu);

@@ -1345,6 +1345,13 @@ static void pbicg_solve(fh_matrix* M, const fh_controls* c, double* psi, const d
     fill_perf(M, perf, name);
 }
 
+// the controls a GAMG preconditioner is built from: its own sub-dictionary's
+// when given (lduMatrixPreconditioner.C:72-82), else the solver's
+static const fh_controls* gamg_precond_controls(const fh_controls* c)
+{
+    return c->preconditionerControls ? c->preconditionerControls : c;
+}
+
 // ---------------------------------------------------------------------------
 void solve_device(fh_matrix* M, const fh_controls* c, double* d_psiNat,
                   const double* d_srcNat, fh_perf* perf)
@@ -1373,12 +1380,12 @@ void solve_device(fh_matrix* M, const fh_controls* c, double* d_psiNat,
     }
     else if (c->precision == FH_PRECISION_MIXED && c->solver == FH_SOLVER_PCG)
     {
-        if (c->preconditioner == FH_PRECON_GAMG) gamg_prepare(M, c);
+        if (c->preconditioner == FH_PRECON_GAMG) gamg_prepare(M, gamg_precond_controls(c));
         mixed_pcg(M, c, psi, src, perf);
     }
     else if (c->precision == FH_PRECISION_MIXED && c->solver == FH_SOLVER_PBICGSTAB)
     {
-        if (c->preconditioner == FH_PRECON_GAMG) gamg_prepare(M, c);
+        if (c->preconditioner == FH_PRECON_GAMG) gamg_prepare(M, gamg_precond_controls(c));
         mixed_solve(M, c, psi, src, perf);
     }
     else if (c->precision == FH_PRECISION_MIXED && c->solver == FH_SOLVER_GAMG)
@@ -1390,7 +1397,7 @@ void solve_device(fh_matrix* M, const fh_controls* c, double* d_psiNat,
         if (c->preconditioner == FH_PRECON_GAMG
             && (c->solver == FH_SOLVER_PCG || c->solver == FH_SOLVER_PBICGSTAB
                 || c->solver == FH_SOLVER_PBICG))
-            gamg_prepare(M, c);
+            gamg_prepare(M, gamg_precond_controls(c));
         switch (c->solver)
         {
             case FH_SOLVER_PCG:

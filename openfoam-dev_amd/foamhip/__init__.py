@@ -291,17 +291,35 @@ def controls_from_dict(d):
     (lduMatrixSolver.C:158-164), smoothSolver::readControls and
     GAMGSolver::readControls; the preconditioner may be a word or a
     sub-dictionary with a "preconditioner" entry
-    (lduMatrixPreconditioner.C:40-60).  "precision mixed" selects the FP32
-    inner / FP64 outer path."""
-    c = FhControls()
-    lib().fh_controls_default(ctypes.byref(c))
+    (lduMatrixPreconditioner.C:40-60).  A GAMG preconditioner is a GAMG
+    solver built from that sub-dictionary alone (lduMatrixPreconditioner.C:
+    72-82, GAMGPreconditioner.C:44-63): its smoother, sweeps, agglomeration,
+    nVcycles and coarsest-level tolerances come from there, and the word form
+    leaves it without the required smoother entry, as in the reference.
+    "precision mixed" selects the FP32 inner / FP64 outer path."""
     if "solver" not in d:
         raise FoamHipError(1, "keyword solver is undefined in the solver dictionary")
+    c = _fill_controls(d)
     c.solver = _lookup(lib().fh_solver_lookup, d["solver"])
     pre = d.get("preconditioner", "none")
-    if isinstance(pre, dict):
-        pre = pre.get("preconditioner", "none")
-    c.preconditioner = _lookup(lib().fh_preconditioner_lookup, pre)
+    sub = pre if isinstance(pre, dict) else None
+    name = sub.get("preconditioner", "none") if sub is not None else pre
+    c.preconditioner = _lookup(lib().fh_preconditioner_lookup, name)
+    if c.preconditioner == FH_PRECON["GAMG"]:
+        if sub is None or "smoother" not in sub:
+            raise FoamHipError(1, "GAMG preconditioner: keyword smoother is undefined in "
+                               "the preconditioner sub-dictionary (GAMGPreconditioner reads "
+                               "its controls from it)")
+        gc = _fill_controls(sub)
+        gc.solver = FH_SOLVER["GAMG"]
+        c._precondControls = gc          # kept alive with c
+        c.preconditionerControls = ctypes.addressof(gc)
+    return c
+
+
+def _fill_controls(d):
+    c = FhControls()
+    lib().fh_controls_default(ctypes.byref(c))
     if "smoother" in d:
         c.smoother = _lookup(lib().fh_smoother_lookup, d["smoother"])
     if "innerPreconditioner" in d:

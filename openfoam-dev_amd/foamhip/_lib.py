@@ -41,6 +41,7 @@ class FhControls(ctypes.Structure):
         ("directSolveCoarsest", _i32), ("nCellsInCoarsestLevel", _i32),
         ("mergeLevels", _i32), ("nVcycles", _i32), ("innerRelTol", _dbl),
         ("maxRefinements", _i32), ("innerPreconditioner", _i32), ("agglomerator", _i32),
+        ("preconditionerControls", ctypes.c_void_p),
     ]
 
 

@@ -633,6 +633,7 @@ typedef struct
     double* rD;
     /* GAMG preconditioner state */
     struct gamg_state* gamg;
+    int32_t nVcycles;
 } precon_state;
 
 static int  gamg_setup(const or_matrix* m, const or_controls* c,
@@ -649,7 +650,11 @@ static void precon_setup(const or_matrix* m, const or_controls* c,
     p->gamg = NULL;
     if (p->kind == OR_PRECON_GAMG)
     {
-        gamg_setup(m, c, &p->gamg);
+        /* GAMGPreconditioner is a GAMGSolver built from the preconditioner
+           sub-dictionary (lduMatrixPreconditioner.C:72-82) */
+        const or_controls* gc = c->precondControls ? c->precondControls : c;
+        p->nVcycles = gc->nVcycles;
+        gamg_setup(m, gc, &p->gamg);
         return;
     }
     if (p->kind != OR_PRECON_NONE)
@@ -664,7 +669,7 @@ static void precon_apply(const or_matrix* m, const or_controls* c,
 {
     if (p->kind == OR_PRECON_GAMG)
     {
-        gamg_precondition(p->gamg, w, r, c->nVcycles);
+        gamg_precondition(p->gamg, w, r, p->nVcycles);
         return;
     }
     if (T) or_preconditionT(m, p->kind, p->rD, r, w);

@@ -122,7 +122,7 @@ void or_smooth(const or_matrix* m, int smoother, const double* rD,
 
 /* ---- solver controls / performance (lduMatrixSolver.C:158-164,
         SolverPerformance.H:78-268) ---- */
-typedef struct
+typedef struct or_controls_s
 {
     int32_t solver;
     int32_t preconditioner;
@@ -144,6 +144,11 @@ typedef struct
     /* faceAreaPair agglomeration: the face weights of or_face_area_weights
        (NULL: algebraicPair, weights from the coefficients) */
     const double* faceWeights;
+    /* GAMG preconditioner: the controls of its preconditioner
+       sub-dictionary, which lduMatrix::preconditioner::New hands to
+       GAMGPreconditioner (lduMatrixPreconditioner.C:72-82,
+       GAMGPreconditioner.C:44-63); NULL: these controls */
+    const struct or_controls_s* precondControls;
 } or_controls;
 
 /* faceAreaPairGAMGAgglomeration.C:55-80: w = mag(cmptMultiply(Sf/sqrt(magSf),

@@ -35,7 +35,8 @@ class OrControls(ctypes.Structure):
                 ("postSweepsLevelMultiplier", _i32), ("maxPostSweeps", _i32),
                 ("nFinestSweeps", _i32), ("scaleCorrection", _i32),
                 ("directSolveCoarsest", _i32), ("nCellsInCoarsestLevel", _i32),
-                ("mergeLevels", _i32), ("nVcycles", _i32), ("faceWeights", _vp)]
+                ("mergeLevels", _i32), ("nVcycles", _i32), ("faceWeights", _vp),
+                ("precondControls", _vp)]
 
 
 class OrPerf(ctypes.Structure):
@@ -124,6 +125,41 @@ def _p(a):
 
 def _f(a):
     return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _controls(d, keep=None):
+    """An fvSolution solver dictionary as OrControls (lduMatrixSolver.C:158-164
+    defaults).  A preconditioner sub-dictionary (lduMatrixPreconditioner.C:
+    40-60) becomes the controls of the GAMG preconditioner
+    (GAMGPreconditioner.C:44-63).  Returns (controls, arrays to keep alive)."""
+    keep = [] if keep is None else keep
+    c = OrControls()
+    lib().or_controls_default(ctypes.byref(c))
+    c.solver = SOLVER[d["solver"]] if "solver" in d else SOLVER["GAMG"]
+    pre = d.get("preconditioner", "none")
+    if isinstance(pre, dict):
+        sub, _ = _controls(dict(pre, solver="GAMG", preconditioner="none"), keep)
+        keep.append(sub)
+        c.precondControls = ctypes.cast(ctypes.pointer(sub), _vp)
+        pre = pre["preconditioner"]
+    c.preconditioner = PRECON[pre]
+    if "smoother" in d:
+        c.smoother = SMOOTH[d["smoother"]]
+    for k in ["maxIter", "minIter", "nSweeps", "nPreSweeps", "nPostSweeps", "maxPostSweeps",
+              "nFinestSweeps", "nCellsInCoarsestLevel", "mergeLevels", "nVcycles",
+              "preSweepsLevelMultiplier", "maxPreSweeps", "postSweepsLevelMultiplier"]:
+        if k in d:
+            setattr(c, k, int(d[k]))
+    for k in ["tolerance", "relTol"]:
+        if k in d:
+            setattr(c, k, float(d[k]))
+    for k in ["scaleCorrection", "directSolveCoarsest"]:
+        if k in d:
+            setattr(c, k, int(bool(d[k])))
+    fw = _face_weights(d)
+    keep.append(fw)
+    c.faceWeights = _p(fw)
+    return c, keep
 
 
 class Addr:
@@ -271,26 +307,7 @@ class Matrix:
 
     def solve(self, controls, psi, source, history=False):
         """controls: dict of fvSolution entries.  Returns (psi, perf dict[, hist])."""
-        c = OrControls()
-        lib().or_controls_default(ctypes.byref(c))
-        c.solver = SOLVER[controls["solver"]]
-        pre = controls.get("preconditioner", "none")
-        c.preconditioner = PRECON[pre]
-        if "smoother" in controls:
-            c.smoother = SMOOTH[controls["smoother"]]
-        for k in ["maxIter", "minIter", "nSweeps", "nPreSweeps", "nPostSweeps", "maxPostSweeps",
-                  "nFinestSweeps", "nCellsInCoarsestLevel", "mergeLevels", "nVcycles",
-                  "preSweepsLevelMultiplier", "maxPreSweeps", "postSweepsLevelMultiplier"]:
-            if k in controls:
-                setattr(c, k, int(controls[k]))
-        for k in ["tolerance", "relTol"]:
-            if k in controls:
-                setattr(c, k, float(controls[k]))
-        for k in ["scaleCorrection", "directSolveCoarsest"]:
-            if k in controls:
-                setattr(c, k, int(bool(controls[k])))
-        fw = _face_weights(controls)
-        c.faceWeights = _p(fw)
+        c, keep = _controls(controls)
         psi = _f(psi).copy(); source = _f(source)
         perf = OrPerf()
         hist = np.zeros(c.maxIter + 2) if history else None

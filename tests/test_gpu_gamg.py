@@ -119,14 +119,40 @@ def test_gamg_random_graph():
 
 def test_gamg_preconditioned_krylov():
     c, a, diag, upper, b = laplacian_case(16, sigma=1.0)
-    ctl = {"solver": "PCG", "preconditioner": "GAMG", "smoother": "GaussSeidel",
+    ctl = {"solver": "PCG", "preconditioner": {"preconditioner": "GAMG",
+                                               "smoother": "GaussSeidel"},
            "tolerance": 1e-9, "relTol": 0}
     perf, *_ = run_pair(c.N, c.lower, c.upper, diag, upper, None, b, ctl)
     assert perf.solverName == "GAMGPCG"
     c, a, diag, upper, lower, b = convdiff_case(12, D=1e-2)
-    ctl = {"solver": "PBiCGStab", "preconditioner": "GAMG", "smoother": "GaussSeidel",
+    ctl = {"solver": "PBiCGStab", "preconditioner": {"preconditioner": "GAMG",
+                                                     "smoother": "GaussSeidel"},
            "tolerance": 1e-9, "relTol": 0}
     run_pair(c.N, c.lower, c.upper, diag, upper, lower, b, ctl)
+
+
+def test_gamg_preconditioner_subdict_controls():
+    """The GAMG preconditioner takes its smoother, sweeps, agglomeration,
+    nVcycles and coarsest-level tolerance from the preconditioner
+    sub-dictionary (lduMatrixPreconditioner.C:72-82, GAMGPreconditioner.C:
+    44-63), not from the solver's dictionary, which names others here."""
+    c, a, diag, upper, b = laplacian_case(20, sigma=1.0)
+    sub = {"preconditioner": "GAMG", "smoother": "symGaussSeidel", "nVcycles": 1,
+           "nCellsInCoarsestLevel": 50, "directSolveCoarsest": 0, "tolerance": 1e-2,
+           "relTol": 0.1, "nFinestSweeps": 1}
+    ctl = {"solver": "PCG", "preconditioner": sub, "tolerance": 1e-9, "relTol": 0,
+           "smoother": "GaussSeidel", "nVcycles": 3, "nCellsInCoarsestLevel": 10,
+           "nFinestSweeps": 4}
+    perf, operf, *_ = run_pair(c.N, c.lower, c.upper, diag, upper, None, b, ctl)
+    # the parent's GAMG entries would make another preconditioner
+    parent_like = dict(ctl, preconditioner=dict(sub, smoother="GaussSeidel", nVcycles=3,
+                                                nCellsInCoarsestLevel=10, nFinestSweeps=4))
+    O.reset_direction()
+    _, other = O.Matrix(a, diag, upper).solve(parent_like, np.zeros(c.N), b)
+    assert other["nIterations"] != operf["nIterations"]
+    with pytest.raises(fh.FoamHipError, match="smoother"):
+        fh.controls_from_dict({"solver": "PCG", "preconditioner": "GAMG",
+                               "smoother": "GaussSeidel"})
 
 
 def test_gamg_mixed_precision():

@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 import oracle as O
+import numpy_solvers as NS
 from meshes import asym_coeffs, box, dense, random_graph, spd_coeffs
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
@@ -332,9 +333,10 @@ def _scale(A, field, source):
     return sf * field + (source - sf * Af) / np.diag(A)
 
 
-def _dense_vcycle(A, Ps, b, psi, scale, nPost, maxPost, nFinest):
-    """The V-cycle of GAMGSolverSolve.C:148-443 (no pre-sweeps, exact
-    coarsest solve, Gauss-Seidel) composed from dense Galerkin products."""
+def _dense_vcycle(A, Ps, b, psi, scale, nPost, maxPost, nFinest, coarse_solve=None):
+    """The V-cycle of GAMGSolverSolve.C:148-443 (no pre-sweeps, Gauss-Seidel)
+    composed from dense Galerkin products; the coarsest level is solved
+    exactly unless coarse_solve(Ac, src) is given."""
     As = [A]
     for P in Ps:
         As.append(P.T @ As[-1] @ P)
@@ -344,7 +346,8 @@ def _dense_vcycle(A, Ps, b, psi, scale, nPost, maxPost, nFinest):
         src.append(P.T @ src[-1])
     L = len(Ps)
     corr = [None] * L
-    corr[L - 1] = np.linalg.solve(As[L], src[L - 1])
+    corr[L - 1] = (np.linalg.solve(As[L], src[L - 1]) if coarse_solve is None
+                   else coarse_solve(As[L], src[L - 1]))
     for l in range(L - 2, -1, -1):
         c = Ps[l + 1] @ corr[l + 1]
         if scale and l < L - 2:
@@ -419,6 +422,101 @@ def test_gamg_preconditioner_two_cycles():
            "scaleCorrection": 0, "nFinestSweeps": 1}
     got, _ = m.solve(ctl, np.zeros(n), b)
     np.testing.assert_allclose(got, w2, rtol=1e-9, atol=1e-12 * np.abs(w2).max())
+
+
+def _prolongators(levels):
+    Ps = []
+    for lev in levels:
+        P = np.zeros((lev["nFineCells"], lev["nCoarseCells"]))
+        P[np.arange(lev["nFineCells"]), lev["restrictMap"]] = 1
+        Ps.append(P)
+    return Ps
+
+
+@pytest.mark.parametrize("minCells,tol,relTol", [
+    (40, 1e-3, 0.1),     # relTol stops the coarsest PCG
+    (40, 1e-2, 0),       # tolerance stops it
+    (9, 0, 0.05),        # several levels; relTol only
+    (40, 0, 0),          # neither: PCG runs to its singularity test or maxIter
+])
+def test_gamg_coarsest_iterative_solve(minCells, tol, relTol):
+    """directSolveCoarsest no (GAMGSolverSolve.C:520-552, GAMGSolver.C:270-322):
+    the coarsest correction is PCG+DIC from zero with the GAMG solver's own
+    tolerance and relTol and the default maxIter 1000.  One oracle V-cycle
+    equals the dense composition whose coarsest solve is a textbook PCG with
+    the textbook DIC factor and those controls; an exact solve, a zero
+    initial guess dropped or another tolerance would give a different
+    cycle."""
+    rng = np.random.default_rng(21)
+    n, lo, up = box(6, 5, 4)
+    a, m, A = mk(n, lo, up, rng)
+    b = rng.standard_normal(n)
+    O.reset_direction()
+    levels = O.agglomerate(m, {"nCellsInCoarsestLevel": minCells, "mergeLevels": 1})
+    Ps = _prolongators(levels)
+    nIts = []
+
+    def coarse(Ac, src):
+        x, it, _ = NS.pcg(Ac, src, np.zeros(src.size), NS.dic_dense(Ac), tol, relTol)
+        nIts.append(it)
+        return x
+
+    expect = _dense_vcycle(A, Ps, b, np.zeros(n), 0, 1, 4, 1, coarse_solve=coarse)
+    exact = _dense_vcycle(A, Ps, b, np.zeros(n), 0, 1, 4, 1)
+    O.reset_direction()
+    ctl = {"solver": "GAMG", "smoother": "GaussSeidel", "tolerance": tol, "relTol": relTol,
+           "maxIter": 1, "minIter": 1, "nCellsInCoarsestLevel": minCells,
+           "directSolveCoarsest": 0, "scaleCorrection": 0, "nPostSweeps": 1,
+           "nFinestSweeps": 1}
+    got, perf = m.solve(ctl, np.zeros(n), b)
+    assert perf["nIterations"] == 1
+    np.testing.assert_allclose(got, expect, rtol=1e-9, atol=1e-11 * np.abs(expect).max())
+    if tol > 0 or relTol > 0:
+        # the coarsest PCG stopped early: the cycle is not the exact one
+        assert 0 < nIts[0] < levels[-1]["nCoarseCells"]
+        assert np.abs(got - exact).max() > 1e-6 * np.abs(exact).max()
+
+
+def test_gamg_preconditioner_subdict_controls():
+    """A GAMG preconditioner reads its controls from the preconditioner
+    sub-dictionary (lduMatrixPreconditioner.C:72-82 hands e.dict() to
+    GAMGPreconditioner, GAMGPreconditioner.C:44-63), not from the solver's
+    dictionary.  The sub-dictionary's smoother, sweeps, coarsest settings
+    and nVcycles differ from the parent's here; the oracle's PCG+GAMG
+    residual history equals a textbook PCG whose preconditioner is the
+    dense V-cycle of the sub-dictionary's settings, and differs from the
+    one the parent's settings would give."""
+    rng = np.random.default_rng(22)
+    n, lo, up = box(6, 5, 4)
+    a, m, A = mk(n, lo, up, rng)
+    b = rng.standard_normal(n)
+    sub = {"preconditioner": "GAMG", "smoother": "GaussSeidel", "nCellsInCoarsestLevel": 40,
+           "directSolveCoarsest": 1, "scaleCorrection": 0, "nPostSweeps": 0,
+           "nFinestSweeps": 1, "nVcycles": 1, "tolerance": 1e-3}
+    # 30 iterations: a forward Gauss-Seidel V-cycle is not a symmetric
+    # preconditioner, so PCG stagnates near 1e-4 instead of converging
+    parent = {"solver": "PCG", "preconditioner": sub, "tolerance": 1e-10, "relTol": 0,
+              "maxIter": 30, "smoother": "symGaussSeidel", "nCellsInCoarsestLevel": 10,
+              "nFinestSweeps": 3, "nVcycles": 2, "directSolveCoarsest": 0}
+    O.reset_direction()
+    Ps = _prolongators(O.agglomerate(m, {"nCellsInCoarsestLevel": 40, "mergeLevels": 1}))
+
+    def vcycles(r, nV, nFinest):
+        w = np.zeros(n)
+        for _ in range(nV):
+            w = _dense_vcycle(A, Ps, r, w, 0, 0, 4, nFinest)
+        return w
+
+    x, its, hist = NS.pcg(A, b, np.zeros(n), lambda r: vcycles(r, 1, 1), 1e-10, 0, maxIter=30)
+    _, _, hist_p = NS.pcg(A, b, np.zeros(n), lambda r: vcycles(r, 2, 3), 1e-10, 0, maxIter=30)
+    O.reset_direction()
+    got, perf, ohist = m.solve(parent, np.zeros(n), b, history=True)
+    assert perf["nIterations"] == its == 30
+    np.testing.assert_allclose(ohist[:31], hist, rtol=1e-8)
+    np.testing.assert_allclose(got, x, rtol=1e-8, atol=1e-10 * np.abs(x).max())
+    # the parent's settings give another history
+    k = min(len(hist_p), len(hist))
+    assert np.abs(hist_p[1:k] / hist[1:k] - 1).max() > 1e-1
 
 
 def test_lu_matches_numpy():
@@ -540,6 +638,50 @@ def test_gauss_grad_exact_for_linear_fields():
                                       (sign * g[axis]))
     np.testing.assert_allclose(grad.reshape(-1, 3), np.tile(g, (c["N"], 1)), rtol=1e-10)
     np.testing.assert_allclose(bGrad.reshape(-1, 3), np.tile(g, (len(c["bfc"]), 1)), rtol=1e-10)
+
+
+def test_gauss_grad_boundary_correction():
+    """gradScheme::correctBoundaryConditions (gradScheme.C:123-148): on a
+    non-coupled patch the boundary gradient is the adjacent cell's gradient
+    with its normal component replaced by the patch snGrad,
+    g_b = g_c + n (snGrad - n.g_c).  With a non-linear cell field and
+    snGrad unrelated to n.g_c the correction term is far from zero; the
+    normal component must equal snGrad and the tangential components the
+    cell's, on every patch (both normal signs)."""
+    c = uniform_case(5)
+    a = O.Addr(c["N"], c["lo"], c["up"])
+    n, h = 5, c["h"]
+    rng = np.random.default_rng(23)
+    phi = rng.uniform(-1, 1, c["N"])
+    F = len(c["lo"])
+    Sf = np.zeros((F, 3))
+    Sf[np.arange(F), c["dirs"]] = h * h
+    nB = len(c["bfc"])
+    axis = c["bpatch"] // 2
+    sign = np.where(c["bpatch"] % 2 == 1, 1.0, -1.0)
+    bSf = np.zeros((nB, 3))
+    bSf[np.arange(nB), axis] = sign * h * h
+    bPhi = rng.uniform(-1, 1, nB)
+    snGrad = rng.uniform(-5, 5, nB)
+    grad, bGrad = O.grad_gauss_linear(a, phi, c["w"], Sf.ravel(), np.full(c["N"], h ** 3),
+                                      c["bfc"], bSf.ravel(), c["bMagSf"], bPhi, snGrad)
+    grad, bGrad = grad.reshape(-1, 3), bGrad.reshape(-1, 3)
+    nrm = bSf / c["bMagSf"][:, None]
+    gc = grad[c["bfc"]]
+    # the correction is not small here
+    assert np.abs(snGrad - (nrm * gc).sum(1)).min() > 1e-3
+    np.testing.assert_allclose((nrm * bGrad).sum(1), snGrad, rtol=1e-12, atol=1e-12)
+    tang = bGrad - nrm * (nrm * bGrad).sum(1)[:, None]
+    np.testing.assert_allclose(tang, gc - nrm * (nrm * gc).sum(1)[:, None], rtol=1e-12,
+                               atol=1e-12)
+    # the cell gradient itself: Gauss theorem with the face values
+    # w phi_P + (1 - w) phi_N and the patch values, over the volume
+    gexp = np.zeros((c["N"], 3))
+    phif = c["w"] * phi[c["lo"]] + (1 - c["w"]) * phi[c["up"]]
+    np.add.at(gexp, c["lo"], Sf * phif[:, None])
+    np.subtract.at(gexp, c["up"], Sf * phif[:, None])
+    np.add.at(gexp, c["bfc"], bSf * bPhi[:, None])
+    np.testing.assert_allclose(grad, gexp / h ** 3, rtol=1e-12, atol=1e-9)
 
 
 # ---- lduMatrix H, H1, faceH, sumMagOffDiag -------------------------------
