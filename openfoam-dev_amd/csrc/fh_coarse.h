@@ -21,6 +21,16 @@ void launch_coarse_dense(int32_t nT, int32_t nTI, int32_t nTJ, const double* cti
 // s*Ac, rows of ld floats
 void launch_coarse_inv32(int32_t nT, int32_t ld, const double* dense, const double* ctile,
                          float* inv, hipStream_t s);
+// the FP32 inverse of Ac through its block-tridiagonal structure over the
+// planes of P = nTI*nTJ tiles (P <= 256, at least two planes; returns false
+// and launches nothing otherwise): U holds the K = nT/P Schur complement
+// inverses (K*P*P doubles), Z the dense FP64 inverse (nT*nT), work
+// coarse_block_work_doubles(nT, P) doubles; *bad (zeroed by the caller)
+// becomes 1 if the matrix is not definite
+bool launch_coarse_block_inverse(int32_t nT, int32_t nTI, int32_t nTJ, int32_t ld,
+                                 const double* ctile, double* U, double* Z, double* work,
+                                 float* inv, int32_t* bad, hipStream_t s);
+int64_t coarse_block_work_doubles(int32_t nT, int32_t P);
 // z = inv rc, and sum z*rc plus the nF fine partials of sum w*r, finalized
 // by the last block with F (FIN_PCG_WARA); returns the block count
 int32_t launch_coarse_apply(int32_t nT, int32_t ld, const float* inv, const double* rc,

@@ -130,6 +130,318 @@ void launch_coarse_inv32(int32_t nT, int32_t ld, const double* dense, const doub
                        nT, ld, dense, ctile, inv);
 }
 
+// ---------------------------------------------------------------------------
+// Block-tridiagonal inverse of s*Ac (no vendor library).  In natural tile
+// order Ac is block tridiagonal over the K = nTK planes of P = nTI*nTJ
+// tiles: diagonal blocks A_k (five-point couplings within the plane) and,
+// between planes k-1 and k, the diagonal matrix C_k of the z couplings
+// (ctile[4t + 3] of the tiles of plane k).  With the Schur complements
+//     S_0 = A_0,  S_k = A_k - C_k S_{k-1}^-1 C_k,  U_k = S_k^-1,
+// the inverse Z = (s*Ac)^-1 follows by the backward block recurrence
+//     Z_{K-1,K-1} = U_{K-1},
+//     Z_{j,k} = -Z_{j,k+1} C_{k+1} U_k          (j > k),
+//     Z_{k,k} = U_k - (C_{k+1} U_k)^T Z_{k+1,k},
+// of which only the blocks j >= k are formed.  The K inversions of P x P
+// SPD matrices run one after another, each in one workgroup (Gauss-Jordan
+// in registers, FP64); the recurrence is batches of FP64 block products.
+// ---------------------------------------------------------------------------
+
+#define CO_GJ 512           // Gauss-Jordan workgroup: P <= 256, 128 entries per thread
+#define GJ_R 8              // row groups per thread (rows tr + 32a)
+
+// S_k = s*A_k - C_k U_{k-1} C_k (FP64, to S), and its inverse by
+// Gauss-Jordan without pivoting (S_k is positive definite) in FP32 in the
+// registers of one workgroup, to U0 as doubles; the caller refines it by a
+// Newton-Schulz step in FP64.  A non-positive pivot sets *bad (the caller
+// falls back to the one-level preconditioner).  Thread (tr, tc) =
+// (t / 16, t % 16) owns rows tr + 32a (a < 8) and columns tc + 16b (b < 16).
+__global__ __launch_bounds__(CO_GJ)
+void k_coarse_schur_inv(int32_t k, int32_t P, int32_t nTI, const double* __restrict__ ctile,
+                        const double* __restrict__ U, double* __restrict__ S,
+                        double* __restrict__ U0, int32_t* bad)
+{
+    __shared__ float rowp[256], colp[256];
+    const double sign = coarse_sign(ctile);
+    const int tr = threadIdx.x >> 4, tc = threadIdx.x & 15;
+    const double* ct = ctile + 4*(int64_t)k*P;
+    float a[GJ_R][16];
+#pragma unroll
+    for (int ra = 0; ra < GJ_R; ra++)
+    {
+        const int i = tr + 32*ra;
+#pragma unroll
+        for (int cb = 0; cb < 16; cb++)
+        {
+            const int j = tc + 16*cb;
+            double v = 0.0;
+            if (i < P && j < P)
+            {
+                const int lo = min(i, j), hi = max(i, j), d = hi - lo;
+                if (d == 0) v = ct[4*i];
+                else if (d == 1 && hi % nTI > 0) v = ct[4*hi + 1];
+                else if (d == nTI) v = ct[4*hi + 2];
+                v *= sign;
+                if (k > 0)
+                {
+                    const double* Up = U + (int64_t)(k - 1)*P*P;
+                    v -= (sign*ct[4*i + 3])*Up[(int64_t)i*P + j]*(sign*ct[4*j + 3]);
+                }
+                S[(int64_t)i*P + j] = v;
+            }
+            a[ra][cb] = (float)v;
+        }
+        // one row group's loads at a time (the scheduler would otherwise
+        // issue all 128 FP64 loads ahead and spill)
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    for (int p = 0; p < P; p++)
+    {
+        // the pivot row and column as they stand before step p
+        const int pr = p & 31, pa = p >> 5, pc = p & 15, pb = p >> 4;
+        if (tr == pr)
+        {
+#pragma unroll
+            for (int ra = 0; ra < GJ_R; ra++)
+                if (ra == pa)
+#pragma unroll
+                    for (int cb = 0; cb < 16; cb++) rowp[tc + 16*cb] = a[ra][cb];
+        }
+        if (tc == pc)
+        {
+#pragma unroll
+            for (int cb = 0; cb < 16; cb++)
+                if (cb == pb)
+#pragma unroll
+                    for (int ra = 0; ra < GJ_R; ra++) colp[tr + 32*ra] = a[ra][cb];
+        }
+        __syncthreads();
+        const float piv = rowp[p];
+        if (!(piv > 0.f))
+        {
+            // every thread reads the same pivot: the block leaves together
+            if (threadIdx.x == 0) *bad = 1;
+            return;
+        }
+        const float ip = 1.f/piv;
+        float cp[GJ_R];
+#pragma unroll
+        for (int ra = 0; ra < GJ_R; ra++) cp[ra] = colp[tr + 32*ra];
+#pragma unroll
+        for (int cb = 0; cb < 16; cb++)
+        {
+            const int j = tc + 16*cb;
+            const float rp = rowp[j]*ip;
+#pragma unroll
+            for (int ra = 0; ra < GJ_R; ra++)
+            {
+                const int i = tr + 32*ra;
+                float v;
+                if (i == p) v = (j == p) ? ip : rp;
+                else if (j == p) v = -cp[ra]*ip;
+                else v = __builtin_fmaf(-cp[ra], rp, a[ra][cb]);
+                a[ra][cb] = v;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int ra = 0; ra < GJ_R; ra++)
+    {
+        const int i = tr + 32*ra;
+#pragma unroll
+        for (int cb = 0; cb < 16; cb++)
+        {
+            const int j = tc + 16*cb;
+            if (i < P && j < P) U0[(int64_t)i*P + j] = (double)a[ra][cb];
+        }
+    }
+}
+
+// Batched FP64 block products of the recurrence, 64 x 64 outputs per
+// workgroup (256 threads, 4 x 4 each), the inner dimension in LDS chunks
+// of 16:  out = beta*D + alpha*op(A) diag(c) B  for blockIdx.z = 0 .. nb-1
+// P x P blocks (row-major, leading dimensions ldA, ldB, ldD, ldO; block z
+// of each operand at z*stride).  op(A) = A^T when transA.  out must not
+// overlap A, B or D.
+#define GM_T 64
+#define GM_K 16
+struct BlockGemm
+{
+    int32_t P;
+    bool transA;
+    double alpha, beta;
+    const double* A; int64_t ldA, strideA;
+    const double* B; int64_t ldB, strideB;
+    const double* c;
+    const double* D; int64_t ldD, strideD;
+    double* out; int64_t ldO, strideO;
+};
+
+__global__ __launch_bounds__(256)
+void k_coarse_block_gemm(BlockGemm g)
+{
+    __shared__ double As[GM_K][GM_T + 1], Bs[GM_K][GM_T + 1];
+    const int bi = blockIdx.x, bj = blockIdx.y, z = blockIdx.z;
+    const int32_t P = g.P;
+    const double* Ab = g.A + z*g.strideA;
+    const double* Bb = g.B + z*g.strideB;
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    double acc[4][4] = {};
+    for (int m0 = 0; m0 < P; m0 += GM_K)
+    {
+        for (int e = threadIdx.x; e < GM_T*GM_K; e += 256)
+        {
+            // A tile: row r of the output block, inner index mm
+            const int r = e / GM_K, mm = e % GM_K;
+            const int gi = bi*GM_T + r, gm = m0 + mm;
+            double av = 0.0;
+            if (gi < P && gm < P)
+                av = g.transA ? Ab[(int64_t)gm*g.ldA + gi] : Ab[(int64_t)gi*g.ldA + gm];
+            As[mm][r] = av;
+            // B tile: inner index mb, column cc, scaled by c of the inner index
+            const int mb = e / GM_T, cc = e % GM_T;
+            const int gm2 = m0 + mb, gj = bj*GM_T + cc;
+            double bv = 0.0;
+            if (gm2 < P && gj < P) bv = g.c[gm2]*Bb[(int64_t)gm2*g.ldB + gj];
+            Bs[mb][cc] = bv;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int mm = 0; mm < GM_K; mm++)
+        {
+            double av[4], bv[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) { av[u] = As[mm][ty + 16*u]; bv[u] = Bs[mm][tx + 16*u]; }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) acc[u][v] = __builtin_fma(av[u], bv[v], acc[u][v]);
+        }
+        __syncthreads();
+    }
+    double* Ob = g.out + z*g.strideO;
+    const double* Db = g.D ? g.D + z*g.strideD : nullptr;
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+        {
+            const int gi = bi*GM_T + ty + 16*u, gj = bj*GM_T + tx + 16*v;
+            if (gi < P && gj < P)
+            {
+                double r = g.alpha*acc[u][v];
+                if (Db) r += g.beta*Db[(int64_t)gi*g.ldD + gj];
+                Ob[(int64_t)gi*g.ldO + gj] = r;
+            }
+        }
+}
+
+// cvec[t] = s * (coupling of tile t to its lower-z neighbour)
+__global__ void k_coarse_cvec(int32_t nT, const double* __restrict__ ctile,
+                              double* __restrict__ cvec)
+{
+    const int32_t t = blockIdx.x*blockDim.x + threadIdx.x;
+    if (t < nT) cvec[t] = coarse_sign(ctile)*ctile[4*(int64_t)t + 3];
+}
+
+// Z_{K-1,K-1} = U_{K-1} into the dense row-major Z (n = K*P)
+__global__ void k_coarse_zlast(int32_t P, int32_t K, const double* __restrict__ U,
+                               double* __restrict__ Z)
+{
+    const int64_t e = blockIdx.x*(int64_t)blockDim.x + threadIdx.x;
+    if (e >= (int64_t)P*P) return;
+    const int64_t n = (int64_t)K*P;
+    const int32_t a = (int32_t)(e / P), b = (int32_t)(e % P);
+    Z[((int64_t)(K - 1)*P + a)*n + (int64_t)(K - 1)*P + b] = U[(int64_t)(K - 1)*P*P + e];
+}
+
+// the full FP32 inverse of Ac (sign * Z), rows of ld floats, from the
+// blocks j >= k of Z
+__global__ void k_coarse_inv32_blocks(int32_t nT, int32_t P, int32_t ld,
+                                      const double* __restrict__ Z,
+                                      const double* __restrict__ ctile, float* __restrict__ inv)
+{
+    const int64_t idx = blockIdx.x*(int64_t)blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)nT*ld) return;
+    const double sign = coarse_sign(ctile);
+    const int32_t j = (int32_t)(idx % ld), i = (int32_t)(idx/ld);
+    float v = 0.f;
+    if (j < nT)
+        v = (float)(sign*(i/P >= j/P ? Z[(int64_t)i*nT + j] : Z[(int64_t)j*nT + i]));
+    inv[idx] = v;
+}
+
+__global__ void k_coarse_fill(int64_t n, double v, double* __restrict__ x)
+{
+    const int64_t i = blockIdx.x*(int64_t)blockDim.x + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+
+bool launch_coarse_block_inverse(int32_t nT, int32_t nTI, int32_t nTJ, int32_t ld,
+                                 const double* ctile, double* U, double* Z, double* work,
+                                 float* inv, int32_t* bad, hipStream_t s)
+{
+    const int32_t P = nTI*nTJ, K = nT/P;
+    if (P > 256 || K < 2 || K*P != nT) return false;
+    const int64_t PP = (int64_t)P*P;
+    double* S = work;               // S_k (FP64)
+    double* U0 = work + PP;         // the FP32 inverse, as doubles
+    double* E = work + 2*PP;        // S_k U0
+    double* ones = work + 3*PP;     // P ones
+    double* cvec = work + 3*PP + P; // nT: s * z couplings
+    hipLaunchKernelGGL(k_coarse_fill, dim3((P + CO_B - 1)/CO_B), dim3(CO_B), 0, s, (int64_t)P,
+                       1.0, ones);
+    const unsigned gt = (unsigned)((P + GM_T - 1)/GM_T);
+    for (int32_t k = 0; k < K; k++)
+    {
+        hipLaunchKernelGGL(k_coarse_schur_inv, dim3(1), dim3(CO_GJ), 0, s, k, P, nTI, ctile, U,
+                           S, U0, bad);
+        // Newton-Schulz in FP64: U_k = 2 U0 - U0 (S_k U0)
+        BlockGemm e{};
+        e.P = P; e.alpha = 1.0;
+        e.A = S; e.ldA = P; e.B = U0; e.ldB = P; e.c = ones; e.out = E; e.ldO = P;
+        hipLaunchKernelGGL(k_coarse_block_gemm, dim3(gt, gt, 1u), dim3(256), 0, s, e);
+        BlockGemm r{};
+        r.P = P; r.alpha = -1.0; r.beta = 2.0;
+        r.A = U0; r.ldA = P; r.B = E; r.ldB = P; r.c = ones; r.D = U0; r.ldD = P;
+        r.out = U + (int64_t)k*PP; r.ldO = P;
+        hipLaunchKernelGGL(k_coarse_block_gemm, dim3(gt, gt, 1u), dim3(256), 0, s, r);
+    }
+    hipLaunchKernelGGL(k_coarse_cvec, dim3((nT + CO_B - 1)/CO_B), dim3(CO_B), 0, s, nT, ctile,
+                       cvec);
+    const int64_t n = (int64_t)K*P;
+    hipLaunchKernelGGL(k_coarse_zlast, dim3((P*P + CO_B - 1)/CO_B), dim3(CO_B), 0, s, P, K, U, Z);
+    for (int32_t k = K - 2; k >= 0; k--)
+    {
+        const double* Uk = U + (int64_t)k*PP;
+        const double* c1 = cvec + (int64_t)(k + 1)*P;
+        // Z_{j,k} = -Z_{j,k+1} C_{k+1} U_k for j = k+1 .. K-1
+        BlockGemm a{};
+        a.P = P; a.transA = false; a.alpha = -1.0; a.beta = 0.0;
+        a.A = Z + (int64_t)(k + 1)*P*n + (int64_t)(k + 1)*P; a.ldA = n; a.strideA = (int64_t)P*n;
+        a.B = Uk; a.ldB = P; a.strideB = 0;
+        a.c = c1;
+        a.out = Z + (int64_t)(k + 1)*P*n + (int64_t)k*P; a.ldO = n; a.strideO = (int64_t)P*n;
+        hipLaunchKernelGGL(k_coarse_block_gemm, dim3(gt, gt, (unsigned)(K - 1 - k)), dim3(256),
+                           0, s, a);
+        // Z_{k,k} = U_k - U_k^T C_{k+1} Z_{k+1,k}
+        BlockGemm b{};
+        b.P = P; b.transA = true; b.alpha = -1.0; b.beta = 1.0;
+        b.A = Uk; b.ldA = P;
+        b.B = Z + (int64_t)(k + 1)*P*n + (int64_t)k*P; b.ldB = n;
+        b.c = c1;
+        b.D = Uk; b.ldD = P;
+        b.out = Z + (int64_t)k*P*n + (int64_t)k*P; b.ldO = n;
+        hipLaunchKernelGGL(k_coarse_block_gemm, dim3(gt, gt, 1u), dim3(256), 0, s, b);
+    }
+    const int64_t m = (int64_t)nT*ld;
+    hipLaunchKernelGGL(k_coarse_inv32_blocks, dim3((unsigned)((m + CO_B - 1)/CO_B)), dim3(CO_B),
+                       0, s, nT, P, ld, Z, ctile, inv);
+    return true;
+}
+
+int64_t coarse_block_work_doubles(int32_t nT, int32_t P) { return 3*(int64_t)P*P + P + nT; }
+
 // z = Ainv rc (one wavefront per row, four rows per block) and sum z*rc;
 // block 0 also adds the fine partials of the tile-local apply (sum w*r, nF
 // of them), so that the last block finalizes the whole w.r of the

@@ -42,6 +42,11 @@ struct fh_matrix
     // preconditioner)
     fh::DBuf<__half> d_dir16[3];
     bool dir16Valid = false;
+    // FP16 storage scale 2^e of those coefficients: state 0 not decided, 1
+    // FP16 at halfScale, -1 FP32 (the magnitudes span more than the FP16
+    // normal range holds)
+    int halfState = 0;
+    float halfScale = 1.f;
     // FP64 direction-packed coefficients of hex boxes (FP64 box product and
     // sweeps): lower K, J, I, and for asymmetric matrices upper I, J, K
     fh::DBuf<double> d_dir64[6];
@@ -55,6 +60,7 @@ struct fh_matrix
     int32_t cN = 0, cLd = 0;
     fh::DBuf<float> d_cinv, d_cz;
     fh::DBuf<double> d_cdense, d_ctile, d_crc;
+    fh::DBuf<double> d_cU, d_cwork;        // block-tridiagonal inverse: Schur inverses, work
     fh::DBuf<int32_t> d_cinfo;
     void* cblas = nullptr;                 // rocblas handle of the factorization
 

@@ -47,6 +47,11 @@ void matrix_init(fh_matrix* M, fh_mesh* mesh, hipStream_t shared)
                                                           mesh->boxNz) + 64);
         // one partial per tile of the fused sweep sums
         nPart = std::max(nPart, 2*(size_t)mesh->maxTiles + 64);
+        // the tile-local apply (one partial per 16^3 tile) followed by the
+        // coarse correction's partials
+        const size_t nT16 = (size_t)((mesh->boxNx + 15)/16)*((mesh->boxNy + 15)/16)
+                            *((mesh->boxNz + 15)/16);
+        nPart = std::max(nPart, 4*nT16 + 64);
     }
     // (nPart counts a and b partials; room for the group sums of the
     // self-finalizing kernels)
@@ -163,6 +168,7 @@ void matrix_coeffs_updated(fh_matrix* M)
     M->hasCoeffs = true;
     M->valTValid = M->sumAValid = M->f32Valid = M->dir32Valid = M->dir64Valid = false;
     M->dir16Valid = false;
+    M->halfState = 0;
     M->coarseState = 0;
     for (int i = 0; i < 5; i++) M->rDValid[i] = M->rD32Valid[i] = false;
     gamg_invalidate(M);
@@ -293,10 +299,52 @@ const double* ensure_rD(fh_matrix* M, int which)
 // The FP32 tile-local DIC of a symmetric box whose rows are whole 16-wide
 // vectors stores its off-diagonal coefficients in FP16: the factor and both
 // passes use the coefficients rounded to FP16 precision (the incomplete
-// Cholesky factor of that matrix; diagonal, factor and arithmetic FP32)
-static bool tile_dic_half(const fh_matrix* M)
+// Cholesky factor of that matrix; diagonal, factor and arithmetic FP32).
+// They are stored scaled by 2^e, e chosen from the largest magnitude so
+// that it lands just below 2^14; if the smallest non-zero magnitude would
+// then fall below the FP16 normal range (2^-14, a span of 2^28), the
+// coefficients stay FP32.  Decided once per coefficient update.
+static constexpr int HALF_TOP = 14;      // largest scaled magnitude < 2^14
+static constexpr int HALF_BOTTOM = -14;  // smallest FP16 normal 2^-14
+
+static bool tile_dic_half(fh_matrix* M)
 {
-    return M->symmetric && M->mesh->tiled && (M->mesh->boxNx & 15) == 0;
+    if (!(M->symmetric && M->mesh->tiled && (M->mesh->boxNx & 15) == 0)) return false;
+    if (M->halfState == 0)
+    {
+        ensure_f32(M);
+        DBuf<unsigned int> range;
+        range.alloc(2);
+        const unsigned int init[2] = {0u, 0xffffffffu};
+        FH_HIP(hipMemcpyAsync(range.p, init, sizeof(init), hipMemcpyHostToDevice, M->stream));
+        launch_abs_range(std::max<int64_t>(M->mesh->nnzPad, 1), M->d_val32.p, range.p,
+                         M->stream);
+        unsigned int got[2];
+        FH_HIP(hipMemcpyAsync(got, range.p, sizeof(got), hipMemcpyDeviceToHost, M->stream));
+        FH_HIP(hipStreamSynchronize(M->stream));
+        float mx, mn;
+        std::memcpy(&mx, &got[0], 4);
+        std::memcpy(&mn, &got[1], 4);
+        M->halfState = -1;
+        if (got[0] == 0u)
+        {
+            // no off-diagonal coefficient: any scale
+            M->halfState = 1;
+            M->halfScale = 1.f;
+        }
+        else if (std::isfinite(mx))
+        {
+            int ex;
+            std::frexp(mx, &ex);              // mx in [2^(ex-1), 2^ex)
+            const int e = HALF_TOP - ex;      // mx*2^e < 2^14
+            if (std::ldexp((double)mn, e) >= std::ldexp(1.0, HALF_BOTTOM) && std::abs(e) < 100)
+            {
+                M->halfState = 1;
+                M->halfScale = (float)std::ldexp(1.0, e);
+            }
+        }
+    }
+    return M->halfState == 1;
 }
 
 const float* ensure_rD32(fh_matrix* M, int which)
@@ -321,7 +369,7 @@ const float* ensure_rD32(fh_matrix* M, int which)
         {
             const int64_t nnz = std::max<int64_t>(M->mesh->nnzPad, 1);
             rounded.alloc(nnz);
-            launch_round_half(nnz, M->d_val32.p, rounded.p, M->stream);
+            launch_round_half(nnz, M->d_val32.p, rounded.p, M->halfScale, M->stream);
             d.val = rounded.p;
         }
         sweep_f(M, which == 4 ? SWEEP_RD_DILU : SWEEP_RD_DIC, d, nullptr, true);
@@ -549,7 +597,7 @@ int32_t apply_precond(fh_matrix* M, int precon, const T* r, T* w, double* partia
                 {
                     __half* dst[3];
                     for (int q = 0; q < 3; q++) { M->d_dir16[q].alloc(N); dst[q] = M->d_dir16[q].p; }
-                    launch_pack_half(N, dir, dst, M->stream);
+                    launch_pack_half(N, dir, dst, M->halfScale, M->stream);
                     M->dir16Valid = true;
                 }
                 for (int q = 0; q < 3; q++) h3[q] = M->d_dir16[q].p;
@@ -558,7 +606,7 @@ int32_t apply_precond(fh_matrix* M, int precon, const T* r, T* w, double* partia
                                                                st, M->stream, partials,
                                                                half ? h3 : nullptr,
                                                                fuse ? F : FusedFinalize(),
-                                                               tileSum);
+                                                               tileSum, M->halfScale);
                 if (nb >= 0)
                 {
                     if (fused) *fused = fuse;
@@ -1030,22 +1078,13 @@ static bool ensure_coarse(fh_matrix* M)
     const int64_t nT = (int64_t)nTI*nTJ*nTK;
     if (nT < 2 || nT > 4096) return false;
     const int32_t n = (int32_t)nT, ld = (n + 3) & ~3;
-    // room for the fine partials and the coarse kernel's partials
-    const int64_t nF = (nT + 3)/4, nC = (nT + 3)/4;
+    // room for the fine partials (one per tile) and the coarse kernel's
+    // partials
+    const int64_t nF = nT, nC = (nT + 3)/4;
     if (2*nF + (int64_t)fin_partials(nC) > (int64_t)M->d_partials.n
         || (int64_t)fin_counters(nC) > (int64_t)M->d_finCount.n)
         return false;
     hipStream_t s = M->stream;
-    if (!M->cblas)
-    {
-        rocblas_handle h = nullptr;
-        if (rocblas_create_handle(&h) != rocblas_status_success)
-            fail(FH_ERR_HIP, "rocblas_create_handle failed");
-        M->cblas = h;
-    }
-    rocblas_handle h = static_cast<rocblas_handle>(M->cblas);
-    if (rocblas_set_stream(h, s) != rocblas_status_success)
-        fail(FH_ERR_HIP, "rocblas_set_stream failed");
     ensure_dir64(M);
     const double* dir[3] = {M->d_dir64[0].p, M->d_dir64[1].p, M->d_dir64[2].p};
     M->d_ctile.alloc(4*(size_t)n);
@@ -1056,8 +1095,40 @@ static bool ensure_coarse(fh_matrix* M)
     M->d_cinfo.alloc(2);
     launch_coarse_tiles(mesh->boxNx, mesh->boxNy, mesh->boxNz, nTI, nTJ, n, M->d_diag.p, dir,
                         M->d_ctile.p, s);
-    // a negative definite matrix (the Laplacian) is factored as -Ac: the
-    // kernels take the sign of the first tile's diagonal
+    // planes of at most 256 tiles: the block-tridiagonal inverse (Schur
+    // complements over the tile planes, hand-written kernels); otherwise the
+    // dense Cholesky of rocSOLVER.  A negative definite matrix (the
+    // Laplacian) is inverted as -Ac: the kernels take the sign of the first
+    // tile's diagonal.
+    const int32_t P = nTI*nTJ;
+    if (P <= 256 && nTK >= 2)
+    {
+        M->d_cU.alloc((size_t)nTK*P*P);
+        M->d_cwork.alloc((size_t)coarse_block_work_doubles(n, P));
+        FH_HIP(hipMemsetAsync(M->d_cinfo.p, 0, M->d_cinfo.bytes(), s));
+        if (!launch_coarse_block_inverse(n, nTI, nTJ, ld, M->d_ctile.p, M->d_cU.p,
+                                         M->d_cdense.p, M->d_cwork.p, M->d_cinv.p,
+                                         M->d_cinfo.p, s))
+            fail(FH_ERR_INVALID, "internal: coarse plane shape");
+        int32_t bad = 0;
+        FH_HIP(hipMemcpyAsync(&bad, M->d_cinfo.p, sizeof(bad), hipMemcpyDeviceToHost, s));
+        FH_HIP(hipStreamSynchronize(s));
+        if (bad) return false;                   // not definite: one-level
+        M->cN = n;
+        M->cLd = ld;
+        M->coarseState = 1;
+        return true;
+    }
+    if (!M->cblas)
+    {
+        rocblas_handle h = nullptr;
+        if (rocblas_create_handle(&h) != rocblas_status_success)
+            fail(FH_ERR_HIP, "rocblas_create_handle failed");
+        M->cblas = h;
+    }
+    rocblas_handle h = static_cast<rocblas_handle>(M->cblas);
+    if (rocblas_set_stream(h, s) != rocblas_status_success)
+        fail(FH_ERR_HIP, "rocblas_set_stream failed");
     launch_coarse_dense(n, nTI, nTJ, M->d_ctile.p, M->d_cdense.p, s);
     if (rocsolver_dpotrf(h, rocblas_fill_lower, n, M->d_cdense.p, n, M->d_cinfo.p)
             != rocblas_status_success

@@ -9,6 +9,11 @@
 
 #include <hip/hip_fp16.h>
 
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <type_traits>
+
 
 namespace fh
 {
@@ -288,10 +293,16 @@ __device__ __forceinline__ bool tl_row(const TlTile& T, int l16, int step, int& 
 
 // coefficient arrays of a pass: K, J, I; with SHIFT (symmetric backward)
 // read at the rows (j, k+1), (j+1, k) and one cell further along i
+// FP16 coefficients are stored as a*2^e (e chosen per matrix so that the
+// magnitudes lie in the FP16 normal range); inv = 2^-e turns them back.
+// Power-of-two scaling commutes with rounding, so (rD*inv)*h and
+// rD*(h*inv) are the same FP32 value: the apply is exactly the one with the
+// unscaled rounded coefficients.
 template<class CT>
 struct TlCoef
 {
     const CT *K, *J, *I;
+    float inv = 1.f;
 };
 
 template<bool BWD, bool SHIFT, bool DOT, class CT>
@@ -324,8 +335,9 @@ __device__ __forceinline__ void tl_load(const TlTile& T, int l16, int step, cons
 template<bool BWD, int BZ, bool SHIFT, bool DOT, class CT>
 __device__ __forceinline__ void tl_step(const TlTile& T, int l16, int step, const TlRow<CT>& c,
                                         float (*bufg)[BZ + 1][16], float* w, double& dotWR,
-                                        double& rsum)
+                                        double& rsum, float cinv)
 {
+    constexpr bool scaled = std::is_same<CT, __half>::value;
     int j, k;
     const bool act = tl_row<BWD>(T, l16, step, j, k);
     // every lane computes (an idle lane on its clamped row, results
@@ -355,18 +367,22 @@ __device__ __forceinline__ void tl_step(const TlTile& T, int l16, int step, cons
     {
         const int i = BWD ? 15 - q : q;
         const float Di = q4(c.D, i);
+        // the coefficient products' factor: rD, times 2^-e for FP16 storage
+        const float Dc = scaled ? Di*cinv : Di;
         float v = BWD ? q4(c.A, i) : Di*q4(c.A, i);
-        const float tK = Di*c.K.at(i)*q4(NK, i);
-        const float tJ = Di*c.J.at(i)*q4(NJ, i);
+        const float tK = Dc*c.K.at(i)*q4(NK, i);
+        const float tJ = Dc*c.J.at(i)*q4(NJ, i);
         v -= hasK ? tK : 0.f;
         v -= hasJ ? tJ : 0.f;
         // SHIFT: the upper i coefficient of cell i is the lower one of
         // cell i+1 (cell 15 has no i+1 inside the tile)
         const bool hasI = BWD ? i < 15 : i > 0;
-        if (hasI) v -= Di*c.I.at(SHIFT ? min(i + 1, 15) : i)*prev;
+        if (hasI) v -= Dc*c.I.at(SHIFT ? min(i + 1, 15) : i)*prev;
         if constexpr (!BWD && DOT)
         {
-            float t = q4(c.A, i);
+            // t scaled by 2^e with the stored coefficients; the row's sum is
+            // scaled back below (exact)
+            float t = scaled ? q4(c.A, i)/cinv : q4(c.A, i);
             t -= hasK ? c.K.at(i)*q4(NK, i) : 0.f;
             t -= hasJ ? c.J.at(i)*q4(NJ, i) : 0.f;
             if (hasI) t -= c.I.at(i)*prev;
@@ -377,7 +393,7 @@ __device__ __forceinline__ void tl_step(const TlTile& T, int l16, int step, cons
     }
     if constexpr (!BWD && DOT)
     {
-        const double t = (st[0] + st[1]) + (st[2] + st[3]);
+        const double t = ((st[0] + st[1]) + (st[2] + st[3]))*(scaled ? (double)cinv : 1.0);
         dotWR += act ? t : 0.0;
         // the row's sum of r (the tile sums of the coarse correction)
         double sr[4] = {0.0, 0.0, 0.0, 0.0};
@@ -429,12 +445,12 @@ __device__ __forceinline__ void tl_walk(const TlTile& T, int l16, int nD, const 
     {
         tl_load<BWD, SHIFT, DOT>(T, l16, step + 1, a, rD, C, r1, r);
         __builtin_amdgcn_sched_barrier(0);
-        tl_step<BWD, BZ, SHIFT, DOT>(T, l16, step, r0, bufg, w, dotWR, rsum);
+        tl_step<BWD, BZ, SHIFT, DOT>(T, l16, step, r0, bufg, w, dotWR, rsum, C.inv);
         tl_load<BWD, SHIFT, DOT>(T, l16, step + 2, a, rD, C, r0, r);
         __builtin_amdgcn_sched_barrier(0);
-        tl_step<BWD, BZ, SHIFT, DOT>(T, l16, step + 1, r1, bufg, w, dotWR, rsum);
+        tl_step<BWD, BZ, SHIFT, DOT>(T, l16, step + 1, r1, bufg, w, dotWR, rsum, C.inv);
     }
-    if (step < nD) tl_step<BWD, BZ, SHIFT, DOT>(T, l16, step, r0, bufg, w, dotWR, rsum);
+    if (step < nD) tl_step<BWD, BZ, SHIFT, DOT>(T, l16, step, r0, bufg, w, dotWR, rsum, C.inv);
 }
 
 // PASSES: 1 forward, 2 backward, 3 forward then backward; TPW tiles per
@@ -505,6 +521,244 @@ void k_tile_local_vec(TileView G, const float* x, const float* __restrict__ rD,
     }
 }
 
+// Quarter-row pipeline of the symmetric tile-local DIC (both passes, one
+// 16 x 16 x 16 tile per wavefront, 16 lanes per quarter of an i-row).
+//
+// The tile's rows are walked by diagonals j + k = d as in k_tile_local_vec,
+// but each row is split into four quarters of four cells: lane (q, m),
+// lane = q + 4m, processes quarter q of row m of diagonal d = t - q at step
+// t.  Its inputs are all from step t - 1: the i - 1 neighbour is the last
+// cell of quarter q - 1 of the same row (lane - 1, DPP row shift), the
+// (j - 1, k) and (j, k - 1) neighbours are quarter q of rows of diagonal
+// d - 1 (LDS, one buffer per diagonal parity).  A step is four cells of
+// dependent arithmetic instead of a whole row, and a wavefront owns one tile
+// (4096 wavefronts at 256^3: four per SIMD) instead of four, so the walk's
+// latency is hidden by the other wavefronts.  The backward pass walks the
+// diagonals and the quarters in reverse (lane q processes d = 33 - t - q),
+// with the i + 1 neighbour from lane + 1.
+//
+// Forward (symmetric, with the sum w.r taken as sum w'.t, see
+// k_tile_local_vec): t = r - cK w'(k-1) - cJ w'(j-1) - cI w'(i-1),
+// w' = rD t; backward: w = w' - rD (cK(k+1) w(k+1) + cJ(j+1) w(j+1) +
+// cI(i+1) w(i+1)), the upper coefficients being the lower ones of the
+// neighbour cells.  FP16 coefficients are stored scaled by 2^e (scale):
+// t is carried scaled by 2^e, rD by 2^-e, and the sum w'.t scaled back.
+template<class CT> struct Quarter;
+template<> struct Quarter<float>
+{
+    float4 v;
+    __device__ void load(const float* p) { v = *reinterpret_cast<const float4*>(p); }
+    __device__ float at(int c) const { return c == 0 ? v.x : (c == 1 ? v.y : (c == 2 ? v.z : v.w)); }
+};
+template<> struct Quarter<__half>
+{
+    uint2 v;
+    __device__ void load(const __half* p) { v = *reinterpret_cast<const uint2*>(p); }
+    __device__ float at(int c) const
+    {
+        const unsigned int wd = c < 2 ? v.x : v.y;
+        return __half2float(__ushort_as_half((unsigned short)(wd >> (16*(c & 1)))));
+    }
+};
+__device__ __forceinline__ float q4at(const float4& v, int c)
+{
+    return c == 0 ? v.x : (c == 1 ? v.y : (c == 2 ? v.z : v.w));
+}
+__device__ __forceinline__ float cvt_coef(float c) { return c; }
+__device__ __forceinline__ float cvt_coef(__half c) { return __half2float(c); }
+
+struct QRow
+{
+    int j, k;
+    bool act;
+    int64_t row;            // first cell of the lane's quarter (a valid cell when !act)
+};
+
+// row of diagonal d for the lane's m (k = max(0, d - 15) + m, j = d - k)
+__device__ __forceinline__ QRow q_row(int d, int m, int ey, int ez, int64_t base, int64_t sy,
+                                      int64_t sz)
+{
+    QRow R;
+    R.k = max(0, d - 15) + m;
+    R.j = d - R.k;
+    R.act = d >= 0 && d <= 30 && R.k < ez && R.j >= 0 && R.j < ey;
+    R.row = R.act ? base + sy*R.j + sz*R.k : base;
+    return R;
+}
+
+template<class CT>
+struct QIn
+{
+    float4 X, D;
+    Quarter<CT> K, J, I;
+    float I4;               // backward: the lower i coefficient of the next quarter's first cell
+};
+
+template<bool BWD, class CT>
+__device__ __forceinline__ void q_load(const QRow& R, int q, int ey, int ez, int64_t sy,
+                                       int64_t sz, const float* x, const float* rD,
+                                       const TlCoef<CT>& C, QIn<CT>& o)
+{
+    o.X = *reinterpret_cast<const float4*>(x + R.row);
+    o.D = *reinterpret_cast<const float4*>(rD + R.row);
+    if constexpr (!BWD)
+    {
+        o.K.load(C.K + R.row);
+        o.J.load(C.J + R.row);
+        o.I.load(C.I + R.row);
+    }
+    else
+    {
+        // upper coefficients: the lower ones of (j, k+1), (j+1, k) and of
+        // the cells one further along i (read where the neighbour exists)
+        o.K.load(C.K + (R.act && R.k + 1 < ez ? R.row + sz : R.row));
+        o.J.load(C.J + (R.act && R.j + 1 < ey ? R.row + sy : R.row));
+        o.I.load(C.I + R.row);
+        o.I4 = q < 3 ? cvt_coef(C.I[R.row + 4]) : 0.f;
+    }
+}
+
+template<class CT, bool DOT>
+__global__ __launch_bounds__(64)
+void k_tile_dic_q(TileView G, const float* x, const float* __restrict__ rD, TlCoef<CT> C,
+                  float scale, float* w, const DevState* st, double* partials, FusedFinalize F,
+                  double* tileSum)
+{
+    if (st && st->done) return;
+    constexpr bool scaled = std::is_same<CT, __half>::value;
+    __shared__ __align__(16) float buf[2][17][16];
+    const int lane = threadIdx.x;
+    const int q = lane & 3, m = lane >> 2;
+    const int tile = blockIdx.x;
+    const int TI = tile % G.nTI, TJ = (tile/G.nTI) % G.nTJ, TK = tile/(G.nTI*G.nTJ);
+    const int ey = min(16, G.ny - TJ*16), ez = min(16, G.nz - TK*16);
+    const int64_t sy = G.nx, sz = (int64_t)G.nx*G.ny;
+    const int64_t base = TI*16 + sy*(TJ*16) + sz*(TK*16) + 4*q;
+    const float inv = C.inv;
+    double dotWR = 0.0, rsum = 0.0;
+
+    // ---- forward ----
+    {
+        QIn<CT> in0, in1;
+        QRow R0 = q_row(-q, m, ey, ez, base, sy, sz), R1;
+        q_load<false>(R0, q, ey, ez, sy, sz, x, rD, C, in0);
+        float last = 0.f;       // this lane's last cell of the previous step
+        auto step = [&](int t, const QRow& R, const QIn<CT>& in) {
+            const int d = t - q;
+            const bool hasK = R.act && R.k > 0;
+            const bool hasJ = R.act && R.j > 0;
+            const float4 NK = *reinterpret_cast<const float4*>(&buf[(d + 1) & 1][hasK ? R.k - 1 : 16][4*q]);
+            const float4 NJ = *reinterpret_cast<const float4*>(&buf[(d + 1) & 1][hasJ ? R.k : 16][4*q]);
+            float prev = __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(last),
+                                        __float_as_int(last), 0x111, 0xf, 0xf, false));
+            prev = q > 0 ? prev : 0.f;
+            float out[4];
+            float dp = 0.f, rs = 0.f;
+#pragma unroll
+            for (int c = 0; c < 4; c++)
+            {
+                const float xc = q4at(in.X, c);
+                float ts = scaled ? xc*scale : xc;
+                ts = __builtin_fmaf(-in.K.at(c), hasK ? q4at(NK, c) : 0.f, ts);
+                ts = __builtin_fmaf(-in.J.at(c), hasJ ? q4at(NJ, c) : 0.f, ts);
+                ts = __builtin_fmaf(-in.I.at(c), prev, ts);
+                const float Dc = scaled ? q4at(in.D, c)*inv : q4at(in.D, c);
+                const float v = Dc*ts;
+                if constexpr (DOT) dp = __builtin_fmaf(v, ts, dp);
+                rs += xc;
+                prev = v;
+                out[c] = v;
+            }
+            last = out[3];
+            const float4 o4 = make_float4(out[0], out[1], out[2], out[3]);
+            *reinterpret_cast<float4*>(&buf[d & 1][R.act ? R.k : 16][4*q]) = o4;
+            if (R.act)
+            {
+                *reinterpret_cast<float4*>(w + R.row) = o4;
+                if constexpr (DOT)
+                {
+                    dotWR += (double)dp;
+                    rsum += (double)rs;
+                }
+            }
+            __syncthreads();
+        };
+        // 34 steps: two per loop pass, the next step's inputs loaded ahead
+        for (int t = 0; t < 34; t += 2)
+        {
+            R1 = q_row(t + 1 - q, m, ey, ez, base, sy, sz);
+            q_load<false>(R1, q, ey, ez, sy, sz, x, rD, C, in1);
+            __builtin_amdgcn_sched_barrier(0);
+            step(t, R0, in0);
+            R0 = q_row(t + 2 - q, m, ey, ez, base, sy, sz);
+            if (t + 2 < 34) q_load<false>(R0, q, ey, ez, sy, sz, x, rD, C, in0);
+            __builtin_amdgcn_sched_barrier(0);
+            step(t + 1, R1, in1);
+        }
+    }
+    // ---- backward (w in place: each lane re-reads what it wrote) ----
+    {
+        QIn<CT> in0, in1;
+        QRow R0 = q_row(33 - q, m, ey, ez, base, sy, sz), R1;
+        q_load<true>(R0, q, ey, ez, sy, sz, w, rD, C, in0);
+        float last = 0.f;
+        auto step = [&](int t, const QRow& R, const QIn<CT>& in) {
+            const int d = 33 - t - q;
+            const bool hasK = R.act && R.k + 1 < ez;
+            const bool hasJ = R.act && R.j + 1 < ey;
+            const float4 NK = *reinterpret_cast<const float4*>(&buf[(d + 1) & 1][hasK ? R.k + 1 : 16][4*q]);
+            const float4 NJ = *reinterpret_cast<const float4*>(&buf[(d + 1) & 1][hasJ ? R.k : 16][4*q]);
+            float next = __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(last),
+                                        __float_as_int(last), 0x101, 0xf, 0xf, false));
+            float out[4];
+#pragma unroll
+            for (int cc = 0; cc < 4; cc++)
+            {
+                const int c = 3 - cc;
+                const float cI = c < 3 ? in.I.at(c + 1) : in.I4;
+                // cell 15 of the row has no i + 1 neighbour (cI = 0 there
+                // only when q = 3, where I4 is 0)
+                float acc = in.K.at(c)*(hasK ? q4at(NK, c) : 0.f);
+                acc = __builtin_fmaf(in.J.at(c), hasJ ? q4at(NJ, c) : 0.f, acc);
+                acc = __builtin_fmaf(cI, (c < 3 || q < 3) ? next : 0.f, acc);
+                const float Dc = scaled ? q4at(in.D, c)*inv : q4at(in.D, c);
+                const float v = __builtin_fmaf(-Dc, acc, q4at(in.X, c));
+                next = v;
+                out[c] = v;
+            }
+            last = out[0];
+            const float4 o4 = make_float4(out[0], out[1], out[2], out[3]);
+            *reinterpret_cast<float4*>(&buf[d & 1][R.act ? R.k : 16][4*q]) = o4;
+            if (R.act) *reinterpret_cast<float4*>(w + R.row) = o4;
+            __syncthreads();
+        };
+        for (int t = 0; t < 34; t += 2)
+        {
+            R1 = q_row(33 - (t + 1) - q, m, ey, ez, base, sy, sz);
+            q_load<true>(R1, q, ey, ez, sy, sz, w, rD, C, in1);
+            __builtin_amdgcn_sched_barrier(0);
+            step(t, R0, in0);
+            R0 = q_row(33 - (t + 2) - q, m, ey, ez, base, sy, sz);
+            if (t + 2 < 34) q_load<true>(R0, q, ey, ez, sy, sz, w, rD, C, in0);
+            __builtin_amdgcn_sched_barrier(0);
+            step(t + 1, R1, in1);
+        }
+    }
+    if constexpr (DOT)
+    {
+        if (tileSum)
+        {
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) rsum += __shfl_xor(rsum, off, 64);
+            if (lane == 0) tileSum[tile] = rsum;
+        }
+        dotWR *= scaled ? (double)inv : 1.0;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) dotWR += __shfl_xor(dotWR, off, 64);
+        partials_arrive(dotWR, 0.0, partials, gridDim.x, blockIdx.x, F);
+    }
+}
+
 // launch with the w*r sum compiled in when partials are given
 template<int PASSES, int BZ, bool SYM, class CT>
 static void tlv_launch(dim3 grid, hipStream_t s, const TileView& G, const float* x,
@@ -524,16 +778,46 @@ int32_t launch_tile_local_apply_f32(const TileView& G, const float* x, const flo
                                     const float* const dir[6], bool symmetric, float* w,
                                     const DevState* st, hipStream_t s, double* partials,
                                     const __half* const half3[3], const FusedFinalize& F,
-                                    double* tileSum)
+                                    double* tileSum, float halfScale)
 {
     if (G.bx != 16 || G.by != 16 || G.bz != TileBZ<float>::pre || (G.nx & 15))
         return -1;
     const int nTiles = G.nTI*G.nTJ*G.nTK;
     const dim3 grid((nTiles + 3)/4);
     constexpr int BZ = TileBZ<float>::pre;
+    static const bool rowWalk = [] {
+        const char* e = std::getenv("FOAMHIP_TILE_APPLY");
+        return e && std::strcmp(e, "rows") == 0;
+    }();
+    if (symmetric && BZ == 16 && !rowWalk)
+    {
+        // quarter-row pipeline: one tile per wavefront
+        const dim3 gq((unsigned)nTiles);
+        if (half3)
+        {
+            const TlCoef<__half> h{half3[0], half3[1], half3[2], 1.f/halfScale};
+            if (partials)
+                hipLaunchKernelGGL((k_tile_dic_q<__half, true>), gq, dim3(64), 0, s, G, x, rD, h,
+                                   halfScale, w, st, partials, F, tileSum);
+            else
+                hipLaunchKernelGGL((k_tile_dic_q<__half, false>), gq, dim3(64), 0, s, G, x, rD,
+                                   h, halfScale, w, st, partials, F, tileSum);
+        }
+        else
+        {
+            const TlCoef<float> c{dir[0], dir[1], dir[2]};
+            if (partials)
+                hipLaunchKernelGGL((k_tile_dic_q<float, true>), gq, dim3(64), 0, s, G, x, rD, c,
+                                   1.f, w, st, partials, F, tileSum);
+            else
+                hipLaunchKernelGGL((k_tile_dic_q<float, false>), gq, dim3(64), 0, s, G, x, rD, c,
+                                   1.f, w, st, partials, F, tileSum);
+        }
+        return (int32_t)gq.x;
+    }
     if (symmetric && half3)
     {
-        const TlCoef<__half> h{half3[0], half3[1], half3[2]};
+        const TlCoef<__half> h{half3[0], half3[1], half3[2], 1.f/halfScale};
         tlv_launch<3, BZ, true>(grid, s, G, x, rD, h, h, w, st, x, partials, F, tileSum);
         return (int32_t)grid.x;
     }
@@ -547,35 +831,74 @@ int32_t launch_tile_local_apply_f32(const TileView& G, const float* x, const flo
     return (int32_t)grid.x;
 }
 
-// FP16 copies of the three lower coefficient arrays of a box
+// FP16 copies of the three lower coefficient arrays of a box, scaled by
+// 2^e (scale)
 __global__ void k_pack_half(int64_t n, const float* __restrict__ a, const float* __restrict__ b,
                             const float* __restrict__ c, __half* __restrict__ ha,
-                            __half* __restrict__ hb, __half* __restrict__ hc)
+                            __half* __restrict__ hb, __half* __restrict__ hc, float scale)
 {
     const int64_t i = (int64_t)blockIdx.x*TL_B + threadIdx.x;
     if (i >= n) return;
-    ha[i] = __float2half_rn(a[i]);
-    hb[i] = __float2half_rn(b[i]);
-    hc[i] = __float2half_rn(c[i]);
+    ha[i] = __float2half_rn(a[i]*scale);
+    hb[i] = __float2half_rn(b[i]*scale);
+    hc[i] = __float2half_rn(c[i]*scale);
 }
 
-// values rounded to FP16 precision, kept in FP32
-__global__ void k_round_half(int64_t n, const float* __restrict__ a, float* __restrict__ b)
+// values rounded to FP16 precision at the storage scale 2^e, kept in FP32
+// and unscaled (exact: power-of-two factors)
+__global__ void k_round_half(int64_t n, const float* __restrict__ a, float* __restrict__ b,
+                             float scale)
 {
     const int64_t i = (int64_t)blockIdx.x*TL_B + threadIdx.x;
-    if (i < n) b[i] = __half2float(__float2half_rn(a[i]));
+    if (i < n) b[i] = __half2float(__float2half_rn(a[i]*scale))/scale;
 }
 
-void launch_round_half(int64_t n, const float* a, float* b, hipStream_t s)
+// largest and smallest non-zero |a| as FP32 bit patterns (non-negative
+// floats order as unsigned integers): range[0] max (start 0), range[1] min
+// (start 0xffffffff)
+__global__ void k_abs_range(int64_t n, const float* __restrict__ a, unsigned int* range)
+{
+    unsigned int mx = 0u, mn = 0xffffffffu;
+    for (int64_t i = (int64_t)blockIdx.x*TL_B + threadIdx.x; i < n; i += (int64_t)gridDim.x*TL_B)
+    {
+        const unsigned int u = __float_as_uint(fabsf(a[i]));
+        if (u != 0u)
+        {
+            mx = max(mx, u);
+            mn = min(mn, u);
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+    {
+        mx = max(mx, (unsigned int)__shfl_xor((int)mx, off, 64));
+        mn = min(mn, (unsigned int)__shfl_xor((int)mn, off, 64));
+    }
+    if ((threadIdx.x & 63) == 0)
+    {
+        atomicMax(&range[0], mx);
+        atomicMin(&range[1], mn);
+    }
+}
+
+void launch_round_half(int64_t n, const float* a, float* b, float scale, hipStream_t s)
 {
     hipLaunchKernelGGL(k_round_half, dim3((unsigned)((n + TL_B - 1)/TL_B)), dim3(TL_B), 0, s, n,
-                       a, b);
+                       a, b, scale);
 }
 
-void launch_pack_half(int64_t n, const float* const src[3], __half* const dst[3], hipStream_t s)
+void launch_pack_half(int64_t n, const float* const src[3], __half* const dst[3], float scale,
+                      hipStream_t s)
 {
     hipLaunchKernelGGL(k_pack_half, dim3((unsigned)((n + TL_B - 1)/TL_B)), dim3(TL_B), 0, s, n,
-                       src[0], src[1], src[2], dst[0], dst[1], dst[2]);
+                       src[0], src[1], src[2], dst[0], dst[1], dst[2], scale);
+}
+
+void launch_abs_range(int64_t n, const float* a, unsigned int* range, hipStream_t s)
+{
+    const int64_t blocks = std::min<int64_t>((n + TL_B - 1)/TL_B, 2048);
+    hipLaunchKernelGGL(k_abs_range, dim3((unsigned)std::max<int64_t>(blocks, 1)), dim3(TL_B), 0,
+                       s, n, a, range);
 }
 
 int32_t launch_tile_local_f32(bool backward, const TileView& G, const float* a, const float* rD,

@@ -71,11 +71,17 @@ int32_t launch_tile_local_apply_f32(const TileView& G, const float* x, const flo
                                     const DevState* st, hipStream_t s, double* partials,
                                     const __half* const half3[3] = nullptr,
                                     const FusedFinalize& F = FusedFinalize(),
-                                    double* tileSum = nullptr);
-// FP16 copies of three FP32 arrays of n values
-void launch_pack_half(int64_t n, const float* const src[3], __half* const dst[3], hipStream_t s);
-// b = a rounded to FP16 precision (kept in FP32)
-void launch_round_half(int64_t n, const float* a, float* b, hipStream_t s);
+                                    double* tileSum = nullptr, float halfScale = 1.f);
+// FP16 copies of three FP32 arrays of n values, scaled by scale (a power of
+// two)
+void launch_pack_half(int64_t n, const float* const src[3], __half* const dst[3], float scale,
+                      hipStream_t s);
+// b = a rounded to FP16 precision at the storage scale (kept in FP32,
+// unscaled)
+void launch_round_half(int64_t n, const float* a, float* b, float scale, hipStream_t s);
+// range[0] = bits of max |a|, range[1] = bits of min non-zero |a| (the
+// caller initialises them to 0 and 0xffffffff)
+void launch_abs_range(int64_t n, const float* a, unsigned int* range, hipStream_t s);
 int32_t launch_tile_local_f32(bool backward, const TileView& G, const float* a, const float* rD,
                               const float* cK, const float* cJ, const float* cI, float* w,
                               const DevState* st, hipStream_t s, const float* r = nullptr,

@@ -442,3 +442,65 @@ def test_fp32_preconditioners(kind, shape):
     w = m.precondition_f32("tile" + kind, r).astype(np.float64)
     ref = om_t.precondition(kind, r)
     assert rel_l2(w, ref) <= 1e-5, rel_l2(w, ref)
+
+
+def _half_stored(up32):
+    """The FP16 storage of the tile-local DIC: scaled by 2^e so that the
+    largest magnitude lies in [2^13, 2^14), rounded to FP16, scaled back;
+    FP32 when the smallest non-zero magnitude would fall below 2^-14."""
+    a = np.abs(up32[up32 != 0]).astype(np.float64)
+    e = 14 - int(np.frexp(a.max())[1])
+    if a.min() * 2.0 ** e < 2.0 ** -14:
+        return up32.astype(np.float64), None
+    h = (up32 * np.float32(2.0 ** e)).astype(np.float16).astype(np.float64) / 2.0 ** e
+    return h, e
+
+
+@pytest.mark.parametrize("scale", [1e6, 1e-7, "span"])
+def test_tile_dic_half_scaled(scale):
+    """The FP16 coefficient storage of the tile-local DIC holds for any
+    coefficient magnitude: scaled by 1e6 (plain FP16 would overflow to inf)
+    and by 1e-7 (plain FP16 would flush to subnormals or zero) the apply is
+    the block-Jacobi DIC of the coefficients rounded at the storage scale;
+    magnitudes spanning more than 2^28 keep FP32 coefficients.  Mixed PCG on
+    the scaled systems converges in the unscaled system's iterations."""
+    nx = ny = nz = 32
+    n, lower, upper = box(nx, ny, nz)
+    rng = np.random.default_rng(17)
+    diag, up = spd_coeffs(n, lower, upper, rng)
+    if scale == "span":
+        # one z-plane of faces 1e-9 weaker: the span exceeds 2^28
+        weak = (lower // (nx * ny)) == 3
+        up = np.where(weak, up * 1e-9, up)
+        diag = np.zeros(n)
+        np.subtract.at(diag, lower, up)
+        np.subtract.at(diag, upper, up)
+        diag += rng.uniform(0.01, 0.5, n)
+        s = 1.0
+    else:
+        s = scale
+    diag, up = diag * s, up * s
+    mesh = fh.lduAddressing(n, lower, upper)
+    m = fh.lduMatrix(mesh, diag, up)
+    r = rng.standard_normal(n)
+    up_t, e = _half_stored(up.astype(np.float32))
+    assert (e is None) == (scale == "span")
+    cut = _tile_blocks(nx, ny, nz, lower, upper, 16)
+    om_t = O.Matrix(O.Addr(n, lower, upper), diag, np.where(cut, 0.0, up_t))
+    w = m.precondition_f32("tileDIC", r).astype(np.float64)
+    assert np.all(np.isfinite(w))
+    ref = om_t.precondition("DIC", r)
+    assert rel_l2(w, ref) <= 1e-5, rel_l2(w, ref)
+    if scale == "span":
+        return
+    b = rng.uniform(-1, 1, n) * s
+    ctl = {"solver": "PCG", "preconditioner": "DIC", "tolerance": 1e-8, "relTol": 0,
+           "maxIter": 3000, "precision": "mixed"}
+    psi = np.zeros(n)
+    perf = fh.solver_New("p", m, ctl).solve(psi, b)
+    m1 = fh.lduMatrix(mesh, diag / s, up / s)
+    perf1 = fh.solver_New("p", m1, ctl).solve(np.zeros(n), b / s)
+    om = O.Matrix(O.Addr(n, lower, upper), diag, up)
+    assert perf.converged
+    assert np.abs(b - om.Amul(psi)).sum() / perf.normFactor < 1e-8
+    assert abs(perf.nIterations - perf1.nIterations) <= 2
