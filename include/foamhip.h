@@ -200,6 +200,21 @@ int fh_mesh_info_get(const fh_mesh* mesh, fh_mesh_info* info);
    (faceAreaPairGAMGAgglomeration.C:55-80).  A change drops the mesh's
    cached agglomeration. */
 int fh_mesh_set_face_areas(fh_mesh* mesh, const double* Sf, const double* magSf);
+/* A mesh with coupled (cyclic) interfaces, the lduInterfaceFieldPtrsList
+   of the matrix (lduMatrixUpdateMatrixInterfaces.C:30-268, one GPU: the
+   patches' cells are all on this device).  Interface i owns faces
+   interfaceStart[i] .. interfaceStart[i+1]-1 of faceCells / nbrFaceCells
+   (concatenated in interface order): face e couples faceCells[e] to
+   nbrFaceCells[e], as cyclicFvPatchField::updateInterfaceMatrix does with
+   the neighbour patch's faceCells (cyclicFvPatchField.C:150-174; scalar
+   fields, no transformation).  Amul, Tmul, residual, sumA, normFactor and
+   the Gauss-Seidel smoothers add the interfaces after the face loop in this
+   order; DIC/DILU ignore them, as the reference's do.  GAMG on a coupled
+   mesh is FH_ERR_UNSUPPORTED.  nInterfaces 0 is fh_mesh_create. */
+int fh_mesh_create_coupled(int32_t nCells, int32_t nFaces, const int32_t* lowerAddr,
+                           const int32_t* upperAddr, int32_t nInterfaces,
+                           const int32_t* interfaceStart, const int32_t* faceCells,
+                           const int32_t* nbrFaceCells, fh_mesh** mesh);
 int fh_mesh_destroy(fh_mesh* mesh);
 
 /* ---- matrix: lduMatrix coefficients (lduMatrix.H:79-87) ----------------
@@ -210,6 +225,13 @@ int fh_matrix_set(fh_matrix* matrix, const double* diag, const double* upper,
                   const double* lower);
 int fh_matrix_set_device(fh_matrix* matrix, const double* d_diag,
                          const double* d_upper, const double* d_lower);
+/* interfaceBouCoeffs and interfaceIntCoeffs of a coupled mesh's matrix
+   (host arrays, one value per interface face in the mesh's interface-face
+   order; intCoeffs NULL: the bouCoeffs), as fvMatrix::solveSegregated hands
+   them to the solver (fvMatrixSolve.C:103-216); required before a solve on
+   a coupled mesh, and kept across fh_matrix_set calls. */
+int fh_matrix_set_interfaces(fh_matrix* matrix, const double* interfaceBouCoeffs,
+                             const double* interfaceIntCoeffs);
 /* lduMatrix::diag(), upper(), lower() (lduMatrix.C:168-312): copies of the
    coefficients, natural order; any output may be NULL.  lower of a
    symmetric matrix is its upper (lower() returns upper when lowerPtr_ is
@@ -385,6 +407,34 @@ int fh_grad_gauss_linear(fh_mesh* mesh, const fh_boundary* boundary, const doubl
                          const double* d_weights, const double* d_Sf, const double* d_V,
                          const double* d_bSf, const double* d_bMagSf, const double* d_bPhi,
                          const double* d_bSnGrad, double* d_grad, double* d_bGrad);
+
+/* ---- mesh geometry (device arrays; vectors packed x, y, z) -------------
+   Faces are polygons: face f has the points facePoints[faceStart[f] ..
+   faceStart[f+1]-1], internal faces first (in the mesh's face order), then
+   the boundary faces (in the boundary's bFaceCells order). */
+/* face::areaAndCentre per face (faceTemplates.C:79-149) and magSf =
+   max(mag(Sf), rootVSmall) (primitiveMeshFaceCentresAndAreas.C):
+   Cf, Sf [3*nFaces], magSf [nFaces] */
+int fh_face_geometry(int32_t nFaces, const int32_t* d_faceStart, const int32_t* d_facePoints,
+                     const double* d_points, double* d_Cf, double* d_Sf, double* d_magSf);
+/* primitiveMesh::makeCellCentresAndVols (primitiveMeshCellCentresAndVols.C:72-154):
+   C [3*nCells], V [nCells] from the internal (Cf, Sf) and boundary (bCf,
+   bSf) face centres and areas */
+int fh_cell_geometry(fh_mesh* mesh, const fh_boundary* boundary, const double* d_Cf,
+                     const double* d_Sf, const double* d_bCf, const double* d_bSf,
+                     double* d_C, double* d_V);
+/* surfaceInterpolation::makeWeights, makeDeltaCoeffs, makeNonOrthDeltaCoeffs
+   (surfaceInterpolation.C:135-330) of the internal faces, and of the
+   non-coupled patch faces deltaCoeffs = 1/mag(delta) and
+   nonOrthDeltaCoeffs = 1/max(nf & delta, 0.05 mag(delta)) with delta =
+   nf (nf & (Cf - Cn)) (fvPatch.C:138-164); d_bNonOrthDeltaCoeffs may be
+   NULL */
+int fh_surface_interpolation(fh_mesh* mesh, const fh_boundary* boundary, const double* d_C,
+                             const double* d_Cf, const double* d_Sf, const double* d_magSf,
+                             const double* d_bCf, const double* d_bSf, const double* d_bMagSf,
+                             double* d_weights, double* d_deltaCoeffs,
+                             double* d_nonOrthDeltaCoeffs, double* d_bDeltaCoeffs,
+                             double* d_bNonOrthDeltaCoeffs);
 
 /* ------------------------------------------------------------------------
    Serving: a request loop for many independent solves.  No counterpart in

@@ -213,6 +213,33 @@ int fh_mesh_create(int32_t nCells, int32_t nFaces, const int32_t* lowerAddr,
     });
 }
 
+int fh_mesh_create_coupled(int32_t nCells, int32_t nFaces, const int32_t* lowerAddr,
+                           const int32_t* upperAddr, int32_t nInterfaces,
+                           const int32_t* interfaceStart, const int32_t* faceCells,
+                           const int32_t* nbrFaceCells, fh_mesh** out)
+{
+    return guarded([&] {
+        if (!out) fail(FH_ERR_INVALID, "NULL output handle");
+        *out = nullptr;
+        fh_mesh* m = new fh_mesh();
+        try
+        {
+            m->device = g_device;
+            FH_HIP(hipSetDevice(m->device));
+            FH_HIP(hipStreamCreateWithFlags(&m->setupStream, hipStreamNonBlocking));
+            build_mesh(m, nCells, nFaces, lowerAddr, upperAddr, nInterfaces == 0);
+            build_interfaces(m, nInterfaces, interfaceStart, faceCells, nbrFaceCells);
+        }
+        catch (...)
+        {
+            if (m->setupStream) (void)hipStreamDestroy(m->setupStream);
+            delete m;
+            throw;
+        }
+        *out = m;
+    });
+}
+
 int fh_mesh_set_face_areas(fh_mesh* mesh, const double* Sf, const double* magSf)
 {
     return guarded([&] {
@@ -285,6 +312,24 @@ int fh_matrix_set_device(fh_matrix* M, const double* d_diag, const double* d_upp
     return guarded([&] {
         if (!M) fail(FH_ERR_INVALID, "NULL matrix handle");
         matrix_set(M, d_diag, d_upper, d_lower, true);
+    });
+}
+
+int fh_matrix_set_interfaces(fh_matrix* M, const double* interfaceBouCoeffs,
+                             const double* interfaceIntCoeffs)
+{
+    return guarded([&] {
+        if (!M) fail(FH_ERR_INVALID, "NULL matrix handle");
+        const int32_t nE = M->mesh->nIfFaces;
+        if (nE == 0) fail(FH_ERR_INVALID, "the mesh has no coupled interfaces");
+        if (!interfaceBouCoeffs) fail(FH_ERR_INVALID, "NULL interfaceBouCoeffs");
+        FH_HIP(hipSetDevice(M->mesh->device));
+        M->d_ifBou.upload(interfaceBouCoeffs, nE, M->stream);
+        M->d_ifInt.upload(interfaceIntCoeffs ? interfaceIntCoeffs : interfaceBouCoeffs, nE,
+                          M->stream);
+        M->ifSet = true;
+        if (M->hasCoeffs) matrix_coeffs_updated(M);
+        else FH_HIP(hipStreamSynchronize(M->stream));
     });
 }
 

@@ -216,4 +216,62 @@ int fh_grad_gauss_linear(fh_mesh* mesh, const fh_boundary* b, const double* d_ph
     });
 }
 
+int fh_face_geometry(int32_t nFaces, const int32_t* d_faceStart, const int32_t* d_facePoints,
+                     const double* d_points, double* d_Cf, double* d_Sf, double* d_magSf)
+{
+    return run([&] {
+        if (nFaces < 0) fail(FH_ERR_INVALID, "nFaces must be >= 0");
+        if (nFaces > 0 && (!d_faceStart || !d_facePoints || !d_points || !d_Cf || !d_Sf || !d_magSf))
+            fail(FH_ERR_INVALID, "NULL input/output");
+        launch_face_geometry(nFaces, d_faceStart, d_facePoints, d_points, d_Cf, d_Sf, d_magSf,
+                             nullptr);
+        FH_HIP(hipStreamSynchronize(nullptr));
+    });
+}
+
+int fh_cell_geometry(fh_mesh* mesh, const fh_boundary* b, const double* d_Cf, const double* d_Sf,
+                     const double* d_bCf, const double* d_bSf, double* d_C, double* d_V)
+{
+    return run([&] {
+        if (!mesh) fail(FH_ERR_INVALID, "NULL mesh handle");
+        check_boundary(b, mesh);
+        if (!d_C || !d_V || (mesh->nFaces && (!d_Cf || !d_Sf)))
+            fail(FH_ERR_INVALID, "NULL input/output");
+        if (b->nB && (!d_bCf || !d_bSf)) fail(FH_ERR_INVALID, "NULL boundary input");
+        RowsView R;
+        R.n = mesh->nCells;
+        R.rowInfo = mesh->d_rowInfo.p;
+        R.sliceStart = mesh->d_sliceStart.p;
+        R.col = mesh->d_col.p;
+        R.face = mesh->d_face.p;
+        launch_cell_geometry(R, mesh->d_perm.p, b->view(), d_Cf, d_Sf, d_bCf, d_bSf, d_C, d_V,
+                             mesh->setupStream);
+        FH_HIP(hipStreamSynchronize(mesh->setupStream));
+    });
+}
+
+int fh_surface_interpolation(fh_mesh* mesh, const fh_boundary* b, const double* d_C,
+                             const double* d_Cf, const double* d_Sf, const double* d_magSf,
+                             const double* d_bCf, const double* d_bSf, const double* d_bMagSf,
+                             double* d_weights, double* d_deltaCoeffs,
+                             double* d_nonOrthDeltaCoeffs, double* d_bDeltaCoeffs,
+                             double* d_bNonOrthDeltaCoeffs)
+{
+    return run([&] {
+        if (!mesh) fail(FH_ERR_INVALID, "NULL mesh handle");
+        check_boundary(b, mesh);
+        if (!d_C || (mesh->nFaces && (!d_Cf || !d_Sf || !d_magSf || !d_weights || !d_deltaCoeffs
+                                      || !d_nonOrthDeltaCoeffs)))
+            fail(FH_ERR_INVALID, "NULL input/output");
+        if (b->nB && (!d_bCf || !d_bSf || !d_bMagSf || !d_bDeltaCoeffs))
+            fail(FH_ERR_INVALID, "NULL boundary input/output");
+        launch_surface_interpolation(mesh->nFaces, mesh->d_lower.p, mesh->d_upper.p, d_C, d_Cf,
+                                     d_Sf, d_magSf, d_weights, d_deltaCoeffs,
+                                     d_nonOrthDeltaCoeffs, b->nB, b->d_faceCells.p, d_bCf, d_bSf,
+                                     d_bMagSf, d_bDeltaCoeffs, d_bNonOrthDeltaCoeffs,
+                                     mesh->setupStream);
+        FH_HIP(hipStreamSynchronize(mesh->setupStream));
+    });
+}
+
 } // extern "C"

@@ -9,6 +9,7 @@
 // 2 y-min, 3 y-max, 4 z-min, 5 z-max, faces in cell order within a patch.
 //
 // Random numbers: counter-based SplitMix64, u(seed, i) in [0, 1).
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -280,6 +281,104 @@ void fhc_fourier_flux(int32_t nx, int32_t ny, int32_t nz, uint64_t seed, const d
         double u[3];
         U(bCf + 3*b, u);
         bPhi[b] = scale*(u[0]*bSf[3*b] + u[1]*bSf[3*b + 1] + u[2]*bSf[3*b + 2]);
+    }
+}
+
+} // extern "C"
+
+extern "C" {
+
+// Points of a box [0,Lx]x[0,Ly]x[0,Lz] with nx x ny x nz hex cells, graded
+// along z with expansion ratio zRatio (last cell over first, blockMesh's
+// simpleGrading (1 1 zRatio): sizes in geometric progression with ratio
+// zRatio^(1/(nz-1))), then jittered: every coordinate of a point that is not
+// on a boundary plane normal to it moves by jitter * (local cell size along
+// that axis, the smaller of the two cells it separates) * U[-1, 1) (seeded).
+// Point P(i, j, k) = i + (nx+1)*(j + (ny+1)*k); points[3p + axis].
+void fhc_graded_box_points(int32_t nx, int32_t ny, int32_t nz, double Lx, double Ly, double Lz,
+                           double zRatio, double jitter, uint64_t seed, double* points)
+{
+    std::vector<double> zs(nz + 1);
+    if (nz > 1 && zRatio != 1.0)
+    {
+        const double r = std::pow(zRatio, 1.0/(nz - 1));
+        for (int32_t k = 0; k <= nz; k++) zs[k] = Lz*(1.0 - std::pow(r, k))/(1.0 - std::pow(r, nz));
+    }
+    else
+        for (int32_t k = 0; k <= nz; k++) zs[k] = Lz*k/nz;
+    zs[nz] = Lz;
+    const double hx = Lx/nx, hy = Ly/ny;
+    for (int32_t k = 0; k <= nz; k++)
+        for (int32_t j = 0; j <= ny; j++)
+            for (int32_t i = 0; i <= nx; i++)
+            {
+                const int64_t p = i + (int64_t)(nx + 1)*(j + (int64_t)(ny + 1)*k);
+                double x = hx*i, y = hy*j, z = zs[k];
+                if (i > 0 && i < nx) x += jitter*hx*(2.0*u01(seed, 3*p) - 1.0);
+                if (j > 0 && j < ny) y += jitter*hy*(2.0*u01(seed, 3*p + 1) - 1.0);
+                if (k > 0 && k < nz)
+                {
+                    const double hz = std::min(zs[k] - zs[k - 1], zs[k + 1] - zs[k]);
+                    z += jitter*hz*(2.0*u01(seed, 3*p + 2) - 1.0);
+                }
+                points[3*p] = x;
+                points[3*p + 1] = y;
+                points[3*p + 2] = z;
+            }
+}
+
+// The faces of that box as quadrilaterals, in the order of fhc_box_faces
+// (internal) followed by fhc_box_bfaces (boundary): facePoints[4f .. 4f+3],
+// ordered so that the face normal points from owner to neighbour (internal)
+// or out of the box (boundary), as polyMesh orients them.
+void fhc_box_face_points(int32_t nx, int32_t ny, int32_t nz, int32_t* facePoints)
+{
+    auto P = [&](int32_t i, int32_t j, int32_t k) {
+        return (int32_t)(i + (int64_t)(nx + 1)*(j + (int64_t)(ny + 1)*k));
+    };
+    // a face on the plane normal to axis at index (i, j, k) of its low
+    // corner, normal +axis (reversed for -axis)
+    auto quad = [&](int axis, int32_t i, int32_t j, int32_t k, bool neg, int32_t* out) {
+        int32_t q[4];
+        if (axis == 0)
+        {
+            q[0] = P(i, j, k); q[1] = P(i, j + 1, k); q[2] = P(i, j + 1, k + 1); q[3] = P(i, j, k + 1);
+        }
+        else if (axis == 1)
+        {
+            q[0] = P(i, j, k); q[1] = P(i, j, k + 1); q[2] = P(i + 1, j, k + 1); q[3] = P(i + 1, j, k);
+        }
+        else
+        {
+            q[0] = P(i, j, k); q[1] = P(i + 1, j, k); q[2] = P(i + 1, j + 1, k); q[3] = P(i, j + 1, k);
+        }
+        if (neg) { out[0] = q[0]; out[1] = q[3]; out[2] = q[2]; out[3] = q[1]; }
+        else for (int m = 0; m < 4; m++) out[m] = q[m];
+    };
+    int64_t f = 0;
+    for (int32_t k = 0; k < nz; k++)
+        for (int32_t j = 0; j < ny; j++)
+            for (int32_t i = 0; i < nx; i++)
+            {
+                if (i < nx - 1) quad(0, i + 1, j, k, false, facePoints + 4*f++);
+                if (j < ny - 1) quad(1, i, j + 1, k, false, facePoints + 4*f++);
+                if (k < nz - 1) quad(2, i, j, k + 1, false, facePoints + 4*f++);
+            }
+    for (int p = 0; p < 6; p++)
+    {
+        const int axis = p/2;
+        const bool hi = p & 1;
+        for (int32_t k = 0; k < nz; k++)
+            for (int32_t j = 0; j < ny; j++)
+                for (int32_t i = 0; i < nx; i++)
+                {
+                    const int32_t idx[3] = {i, j, k};
+                    const int32_t nmax[3] = {nx, ny, nz};
+                    if (idx[axis] != (hi ? nmax[axis] - 1 : 0)) continue;
+                    int32_t c[3] = {i, j, k};
+                    if (hi) c[axis] += 1;
+                    quad(axis, c[0], c[1], c[2], !hi, facePoints + 4*f++);
+                }
     }
 }
 

@@ -146,33 +146,108 @@ void launch_coarse_inv32(int32_t nT, int32_t ld, const double* dense, const doub
 // in registers, FP64); the recurrence is batches of FP64 block products.
 // ---------------------------------------------------------------------------
 
-#define CO_GJ 512           // Gauss-Jordan workgroup: P <= 256, 128 entries per thread
-#define GJ_R 8              // row groups per thread (rows tr + 32a)
+#define CO_GJ 1024          // Gauss-Jordan workgroup: P <= 256, 64 entries per thread
 
 // S_k = s*A_k - C_k U_{k-1} C_k (FP64, to S), and its inverse by
 // Gauss-Jordan without pivoting (S_k is positive definite) in FP32 in the
 // registers of one workgroup, to U0 as doubles; the caller refines it by a
 // Newton-Schulz step in FP64.  A non-positive pivot sets *bad (the caller
-// falls back to the one-level preconditioner).  Thread (tr, tc) =
-// (t / 16, t % 16) owns rows tr + 32a (a < 8) and columns tc + 16b (b < 16).
+// falls back to the one-level preconditioner).
+//
+// Thread (tr, tc) = (t / 16, t % 16) owns rows 4tr .. 4tr+3 and columns
+// 16tc .. 16tc+15, so the pivot row and column it reads (and, as their
+// owner, writes) are contiguous in LDS (16-byte accesses).  The pivot loop
+// is unrolled over p mod 16, so the owners of the pivot row (tr = p/4, row
+// p mod 4) and column (tc = p/16, column p mod 16) index their registers
+// statically: a step is the rank-one update of every entry (one FMA each),
+// then the owners overwrite the pivot row and column with the scaled values.
+template<int C16>
+__device__ __forceinline__ bool gj_step(float (&a)[4][16], int tr, int tc, int p, float* rowp,
+                                        float* colp, int32_t* bad)
+{
+    constexpr int RA = C16 & 3;                // the pivot row's register row
+    const int prow = p >> 2, pcol = p >> 4;    // owners: tr == prow, tc == pcol
+    if (tr == prow)
+    {
+#pragma unroll
+        for (int c = 0; c < 16; c += 4)
+            *reinterpret_cast<float4*>(&rowp[16*tc + c]) =
+                make_float4(a[RA][c], a[RA][c + 1], a[RA][c + 2], a[RA][c + 3]);
+    }
+    if (tc == pcol)
+        *reinterpret_cast<float4*>(&colp[4*tr]) =
+            make_float4(a[0][C16], a[1][C16], a[2][C16], a[3][C16]);
+    __syncthreads();
+    const float piv = rowp[p];
+    if (!(piv > 0.f))
+    {
+        // every thread reads the same pivot: the block leaves together
+        if (threadIdx.x == 0) *bad = 1;
+        return false;
+    }
+    const float ip = 1.f/piv;
+    const float4 cp4 = *reinterpret_cast<const float4*>(&colp[4*tr]);
+    const float cp[4] = {cp4.x, cp4.y, cp4.z, cp4.w};
+    float rp[16];
+#pragma unroll
+    for (int c = 0; c < 16; c += 4)
+    {
+        const float4 r4 = *reinterpret_cast<const float4*>(&rowp[16*tc + c]);
+        rp[c] = r4.x*ip; rp[c + 1] = r4.y*ip; rp[c + 2] = r4.z*ip; rp[c + 3] = r4.w*ip;
+    }
+#pragma unroll
+    for (int ra = 0; ra < 4; ra++)
+#pragma unroll
+        for (int c = 0; c < 16; c++) a[ra][c] = __builtin_fmaf(-cp[ra], rp[c], a[ra][c]);
+    // the pivot row: the old row over the pivot; the pivot column: -old
+    // column over the pivot; the pivot entry: its reciprocal
+    if (tr == prow)
+    {
+#pragma unroll
+        for (int c = 0; c < 16; c++) a[RA][c] = rp[c];
+    }
+    if (tc == pcol)
+    {
+#pragma unroll
+        for (int ra = 0; ra < 4; ra++) a[ra][C16] = -cp[ra]*ip;
+        if (tr == prow) a[RA][C16] = ip;
+    }
+    __syncthreads();
+    return true;
+}
+
+template<int C16>
+__device__ __forceinline__ bool gj_unrolled(float (&a)[4][16], int tr, int tc, int base, int P,
+                                            float* rowp, float* colp, int32_t* bad)
+{
+    if constexpr (C16 < 16)
+    {
+        const int p = base + C16;
+        if (p >= P) return true;
+        if (!gj_step<C16>(a, tr, tc, p, rowp, colp, bad)) return false;
+        return gj_unrolled<C16 + 1>(a, tr, tc, base, P, rowp, colp, bad);
+    }
+    return true;
+}
+
 __global__ __launch_bounds__(CO_GJ)
 void k_coarse_schur_inv(int32_t k, int32_t P, int32_t nTI, const double* __restrict__ ctile,
                         const double* __restrict__ U, double* __restrict__ S,
                         double* __restrict__ U0, int32_t* bad)
 {
-    __shared__ float rowp[256], colp[256];
+    __shared__ __align__(16) float rowp[256], colp[256];
     const double sign = coarse_sign(ctile);
     const int tr = threadIdx.x >> 4, tc = threadIdx.x & 15;
     const double* ct = ctile + 4*(int64_t)k*P;
-    float a[GJ_R][16];
+    float a[4][16];
 #pragma unroll
-    for (int ra = 0; ra < GJ_R; ra++)
+    for (int ra = 0; ra < 4; ra++)
     {
-        const int i = tr + 32*ra;
+        const int i = 4*tr + ra;
 #pragma unroll
-        for (int cb = 0; cb < 16; cb++)
+        for (int c = 0; c < 16; c++)
         {
-            const int j = tc + 16*cb;
+            const int j = 16*tc + c;
             double v = 0.0;
             if (i < P && j < P)
             {
@@ -188,83 +263,37 @@ void k_coarse_schur_inv(int32_t k, int32_t P, int32_t nTI, const double* __restr
                 }
                 S[(int64_t)i*P + j] = v;
             }
-            a[ra][cb] = (float)v;
+            a[ra][c] = (float)v;
         }
-        // one row group's loads at a time (the scheduler would otherwise
-        // issue all 128 FP64 loads ahead and spill)
+        // one row's loads at a time (the scheduler would otherwise issue all
+        // of them ahead and spill)
         __builtin_amdgcn_sched_barrier(0);
     }
-    for (int p = 0; p < P; p++)
+#pragma unroll 1
+    for (int base = 0; base < P; base += 16)
+        if (!gj_unrolled<0>(a, tr, tc, base, P, rowp, colp, bad)) return;
+#pragma unroll
+    for (int ra = 0; ra < 4; ra++)
     {
-        // the pivot row and column as they stand before step p
-        const int pr = p & 31, pa = p >> 5, pc = p & 15, pb = p >> 4;
-        if (tr == pr)
+        const int i = 4*tr + ra;
+#pragma unroll
+        for (int c = 0; c < 16; c++)
         {
-#pragma unroll
-            for (int ra = 0; ra < GJ_R; ra++)
-                if (ra == pa)
-#pragma unroll
-                    for (int cb = 0; cb < 16; cb++) rowp[tc + 16*cb] = a[ra][cb];
-        }
-        if (tc == pc)
-        {
-#pragma unroll
-            for (int cb = 0; cb < 16; cb++)
-                if (cb == pb)
-#pragma unroll
-                    for (int ra = 0; ra < GJ_R; ra++) colp[tr + 32*ra] = a[ra][cb];
-        }
-        __syncthreads();
-        const float piv = rowp[p];
-        if (!(piv > 0.f))
-        {
-            // every thread reads the same pivot: the block leaves together
-            if (threadIdx.x == 0) *bad = 1;
-            return;
-        }
-        const float ip = 1.f/piv;
-        float cp[GJ_R];
-#pragma unroll
-        for (int ra = 0; ra < GJ_R; ra++) cp[ra] = colp[tr + 32*ra];
-#pragma unroll
-        for (int cb = 0; cb < 16; cb++)
-        {
-            const int j = tc + 16*cb;
-            const float rp = rowp[j]*ip;
-#pragma unroll
-            for (int ra = 0; ra < GJ_R; ra++)
-            {
-                const int i = tr + 32*ra;
-                float v;
-                if (i == p) v = (j == p) ? ip : rp;
-                else if (j == p) v = -cp[ra]*ip;
-                else v = __builtin_fmaf(-cp[ra], rp, a[ra][cb]);
-                a[ra][cb] = v;
-            }
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int ra = 0; ra < GJ_R; ra++)
-    {
-        const int i = tr + 32*ra;
-#pragma unroll
-        for (int cb = 0; cb < 16; cb++)
-        {
-            const int j = tc + 16*cb;
-            if (i < P && j < P) U0[(int64_t)i*P + j] = (double)a[ra][cb];
+            const int j = 16*tc + c;
+            if (i < P && j < P) U0[(int64_t)i*P + j] = (double)a[ra][c];
         }
     }
 }
 
 // Batched FP64 block products of the recurrence, 64 x 64 outputs per
 // workgroup (256 threads, 4 x 4 each), the inner dimension in LDS chunks
-// of 16:  out = beta*D + alpha*op(A) diag(c) B  for blockIdx.z = 0 .. nb-1
+// of 32, the next chunk's loads issued before the current chunk's
+// products:  out = beta*D + alpha*op(A) diag(c) B  for blockIdx.z = 0 .. nb-1
 // P x P blocks (row-major, leading dimensions ldA, ldB, ldD, ldO; block z
 // of each operand at z*stride).  op(A) = A^T when transA.  out must not
 // overlap A, B or D.
 #define GM_T 64
-#define GM_K 16
+#define GM_K 32
 struct BlockGemm
 {
     int32_t P;
@@ -277,6 +306,26 @@ struct BlockGemm
     double* out; int64_t ldO, strideO;
 };
 
+// the chunk's operands, eight values per thread each (A: 64 rows x 32 inner,
+// B: 32 inner x 64 columns)
+__device__ __forceinline__ void gemm_fetch(const BlockGemm& g, const double* Ab, const double* Bb,
+                                          int bi, int bj, int m0, double (&av)[8], double (&bv)[8])
+{
+    const int32_t P = g.P;
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+    {
+        const int e = threadIdx.x + 256*u;
+        const int r = e / GM_K, mm = e % GM_K;
+        const int gi = bi*GM_T + r, gm = m0 + mm;
+        av[u] = (gi < P && gm < P)
+              ? (g.transA ? Ab[(int64_t)gm*g.ldA + gi] : Ab[(int64_t)gi*g.ldA + gm]) : 0.0;
+        const int mb = e / GM_T, cc = e % GM_T;
+        const int gm2 = m0 + mb, gj = bj*GM_T + cc;
+        bv[u] = (gm2 < P && gj < P) ? g.c[gm2]*Bb[(int64_t)gm2*g.ldB + gj] : 0.0;
+    }
+}
+
 __global__ __launch_bounds__(256)
 void k_coarse_block_gemm(BlockGemm g)
 {
@@ -287,35 +336,30 @@ void k_coarse_block_gemm(BlockGemm g)
     const double* Bb = g.B + z*g.strideB;
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
     double acc[4][4] = {};
+    double av[8], bv[8];
+    gemm_fetch(g, Ab, Bb, bi, bj, 0, av, bv);
     for (int m0 = 0; m0 < P; m0 += GM_K)
     {
-        for (int e = threadIdx.x; e < GM_T*GM_K; e += 256)
+#pragma unroll
+        for (int u = 0; u < 8; u++)
         {
-            // A tile: row r of the output block, inner index mm
-            const int r = e / GM_K, mm = e % GM_K;
-            const int gi = bi*GM_T + r, gm = m0 + mm;
-            double av = 0.0;
-            if (gi < P && gm < P)
-                av = g.transA ? Ab[(int64_t)gm*g.ldA + gi] : Ab[(int64_t)gi*g.ldA + gm];
-            As[mm][r] = av;
-            // B tile: inner index mb, column cc, scaled by c of the inner index
-            const int mb = e / GM_T, cc = e % GM_T;
-            const int gm2 = m0 + mb, gj = bj*GM_T + cc;
-            double bv = 0.0;
-            if (gm2 < P && gj < P) bv = g.c[gm2]*Bb[(int64_t)gm2*g.ldB + gj];
-            Bs[mb][cc] = bv;
+            const int e = threadIdx.x + 256*u;
+            As[e % GM_K][e / GM_K] = av[u];
+            Bs[e / GM_T][e % GM_T] = bv[u];
         }
         __syncthreads();
+        // the next chunk's loads in flight during this chunk's products
+        if (m0 + GM_K < P) gemm_fetch(g, Ab, Bb, bi, bj, m0 + GM_K, av, bv);
 #pragma unroll
         for (int mm = 0; mm < GM_K; mm++)
         {
-            double av[4], bv[4];
+            double a4[4], b4[4];
 #pragma unroll
-            for (int u = 0; u < 4; u++) { av[u] = As[mm][ty + 16*u]; bv[u] = Bs[mm][tx + 16*u]; }
+            for (int u = 0; u < 4; u++) { a4[u] = As[mm][ty + 16*u]; b4[u] = Bs[mm][tx + 16*u]; }
 #pragma unroll
             for (int u = 0; u < 4; u++)
 #pragma unroll
-                for (int v = 0; v < 4; v++) acc[u][v] = __builtin_fma(av[u], bv[v], acc[u][v]);
+                for (int v = 0; v < 4; v++) acc[u][v] = __builtin_fma(a4[u], b4[v], acc[u][v]);
         }
         __syncthreads();
     }
@@ -356,19 +400,49 @@ __global__ void k_coarse_zlast(int32_t P, int32_t K, const double* __restrict__ 
 }
 
 // the full FP32 inverse of Ac (sign * Z), rows of ld floats, from the
-// blocks j >= k of Z
-__global__ void k_coarse_inv32_blocks(int32_t nT, int32_t P, int32_t ld,
-                                      const double* __restrict__ Z,
-                                      const double* __restrict__ ctile, float* __restrict__ inv)
+// blocks j >= k of Z: 32 x 32 output tiles; a tile above the block diagonal
+// is the transpose of its mirror tile, read whole rows at a time into LDS
+// and written transposed (P is a multiple of 32 or the tiles are masked)
+#define CI_T 32
+__global__ __launch_bounds__(256)
+void k_coarse_inv32_blocks(int32_t nT, int32_t P, int32_t ld, const double* __restrict__ Z,
+                           const double* __restrict__ ctile, float* __restrict__ inv)
 {
-    const int64_t idx = blockIdx.x*(int64_t)blockDim.x + threadIdx.x;
-    if (idx >= (int64_t)nT*ld) return;
+    __shared__ double tileT[CI_T][CI_T + 1];
     const double sign = coarse_sign(ctile);
-    const int32_t j = (int32_t)(idx % ld), i = (int32_t)(idx/ld);
-    float v = 0.f;
-    if (j < nT)
-        v = (float)(sign*(i/P >= j/P ? Z[(int64_t)i*nT + j] : Z[(int64_t)j*nT + i]));
-    inv[idx] = v;
+    const int32_t i0 = blockIdx.y*CI_T, j0 = blockIdx.x*CI_T;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;      // 32 x 8
+    if (i0 >= nT) return;
+    // the plane order of the tile's rows and columns (one plane per tile
+    // row range when 32 divides P; else decided per element below)
+    const bool direct = (i0/P) >= (j0/P) && ((i0 + CI_T - 1)/P) >= ((j0 + CI_T - 1)/P)
+                        && (P % CI_T) == 0;
+    if (direct || (P % CI_T) != 0)
+    {
+        for (int r = ty; r < CI_T; r += 8)
+        {
+            const int32_t i = i0 + r, j = j0 + tx;
+            if (i >= nT || j >= ld) continue;
+            float v = 0.f;
+            if (j < nT)
+                v = (float)(sign*((i/P) >= (j/P) ? Z[(int64_t)i*nT + j] : Z[(int64_t)j*nT + i]));
+            inv[(int64_t)i*ld + j] = v;
+        }
+        return;
+    }
+    // transposed tile: read rows j0.. of Z at columns i0.. (coalesced)
+    for (int r = ty; r < CI_T; r += 8)
+    {
+        const int32_t j = j0 + r, i = i0 + tx;
+        tileT[r][tx] = (j < nT && i < nT) ? Z[(int64_t)j*nT + i] : 0.0;
+    }
+    __syncthreads();
+    for (int r = ty; r < CI_T; r += 8)
+    {
+        const int32_t i = i0 + r, j = j0 + tx;
+        if (i >= nT || j >= ld) continue;
+        inv[(int64_t)i*ld + j] = j < nT ? (float)(sign*tileT[tx][r]) : 0.f;
+    }
 }
 
 __global__ void k_coarse_fill(int64_t n, double v, double* __restrict__ x)
@@ -434,9 +508,9 @@ bool launch_coarse_block_inverse(int32_t nT, int32_t nTI, int32_t nTJ, int32_t l
         b.out = Z + (int64_t)k*P*n + (int64_t)k*P; b.ldO = n;
         hipLaunchKernelGGL(k_coarse_block_gemm, dim3(gt, gt, 1u), dim3(256), 0, s, b);
     }
-    const int64_t m = (int64_t)nT*ld;
-    hipLaunchKernelGGL(k_coarse_inv32_blocks, dim3((unsigned)((m + CO_B - 1)/CO_B)), dim3(CO_B),
-                       0, s, nT, P, ld, Z, ctile, inv);
+    hipLaunchKernelGGL(k_coarse_inv32_blocks,
+                       dim3((unsigned)((ld + CI_T - 1)/CI_T), (unsigned)((nT + CI_T - 1)/CI_T)),
+                       dim3(256), 0, s, nT, P, ld, Z, ctile, inv);
     return true;
 }
 

@@ -159,6 +159,14 @@ struct fh_mesh
     // until the face areas are given)
     std::vector<double> faceAreaWeights;
 
+    // coupled (cyclic) interfaces: the caller's interface faces (concatenated
+    // in interface order) regrouped per library row in that order; entry e
+    // of row r couples it to ifNbr[e] with the caller's face ifFace[e]
+    int32_t nInterfaces = 0;
+    int32_t nIfFaces = 0;
+    std::vector<int32_t> ifStart, ifNbr, ifFace;   // [nCells+1], [nIfFaces], [nIfFaces]
+    fh::DBuf<int32_t> d_ifStart, d_ifNbr, d_ifFace;
+
     // multi-colour row order (greedy over the caller's cell order; red-black
     // on a box), built on first use by the FP32 Gauss-Seidel of the mixed
     // GAMG V-cycles
@@ -173,6 +181,9 @@ namespace fh
 {
 // mesh construction (fh_mesh.cpp)
 void build_mesh(fh_mesh* m, int32_t nCells, int32_t nFaces,
-                const int32_t* lower, const int32_t* upper);
+                const int32_t* lower, const int32_t* upper, bool allowBox = true);
+// the per-row interface entries of a coupled mesh (after build_mesh)
+void build_interfaces(fh_mesh* m, int32_t nInterfaces, const int32_t* start,
+                      const int32_t* faceCells, const int32_t* nbrCells);
 size_t mesh_device_bytes(const fh_mesh* m);
 } // namespace fh

@@ -47,7 +47,25 @@ struct RowsView
     const int32_t* sliceStart;  // SELL-64 slice offsets
     const int32_t* col;
     const int32_t* face;        // (f << 1 | isUpper), -1 pad
+    // coupled interfaces (null: none): row r's interface entries are
+    // ifStart[r] .. ifStart[r+1]-1, column ifNbr[e], value val[ifBase + e]
+    // (the negated interface coefficient), after the row's face entries --
+    // the reference adds the interfaces after the face loop
+    const int32_t* ifStart = nullptr;
+    const int32_t* ifNbr = nullptr;
+    int64_t ifBase = 0;
 };
+
+// acc += sum of the row's interface entries val*x(nbr), in entry order
+template<class T, class X>
+__device__ __forceinline__ T add_interfaces(const RowsView& R, int32_t r, const T* val,
+                                            const X* x, T acc)
+{
+    if (R.ifStart)
+        for (int32_t e = R.ifStart[r]; e < R.ifStart[r + 1]; e++)
+            acc += val[R.ifBase + e]*(T)x[R.ifNbr[e]];
+    return acc;
+}
 
 // Partitioned level sweep schedule (one workgroup per part)
 struct SchedView
@@ -159,6 +177,13 @@ void launch_residual(const RowsView& R, const T* diag, const T* val, const T* x,
                      const T* b, T* r, const DevState* st, hipStream_t s);
 void launch_sum_a(const RowsView& R, const double* diag, const double* val,
                   double* sumA, hipStream_t s);
+// bPrime = src - (interface entries)*psi for the Gauss-Seidel smoothers of
+// a coupled mesh
+void launch_interface_bprime(const RowsView& R, const double* val, const double* psi,
+                             const double* src, double* bPrime, hipStream_t s);
+// tail[e] = -coeffs[face[e]]: the interface entries' values
+void launch_interface_values(int32_t nE, const int32_t* face, const double* coeffs,
+                             double* tail, hipStream_t s);
 
 // sweeps
 // streaming probes: 0 read buf[0]; 1 copy buf[0] -> buf[1]; 2 buf[0] + buf[2]

@@ -145,7 +145,9 @@ __global__ void k_amul(RowsView R, const T* __restrict__ diag, const T* __restri
         const int32_t e = base + (k << 6);
         acc += val[e]*x[R.col[e]];
     }
-    y[r] = acc;
+    // coupled interfaces (lduMatrixATmul.C:81-88, updateMatrixInterfaces):
+    // result -= coeff*psi(nbr) with the stored value -coeff
+    y[r] = add_interfaces(R, r, val, x, acc);
 }
 
 template<class T>
@@ -182,6 +184,12 @@ __global__ void k_residual(RowsView R, const T* __restrict__ diag, const T* __re
         const int32_t e = base + (k << 6);
         acc -= val[e]*x[R.col[e]];
     }
+    // coupled interfaces with the negated coefficients (mBouCoeffs,
+    // lduMatrixATmul.C:230-244, 266-273): rA -= mBou*psi(nbr), mBou the
+    // stored value
+    if (R.ifStart)
+        for (int32_t e = R.ifStart[r]; e < R.ifStart[r + 1]; e++)
+            acc -= val[R.ifBase + e]*x[R.ifNbr[e]];
     res[r] = acc;
 }
 
@@ -203,7 +211,48 @@ __global__ void k_sum_a(RowsView R, const double* __restrict__ diag,
     const int32_t base = sell_base(R.sliceStart, r);
     double acc = diag[r];
     for (int32_t k = 0; k < nEnt; k++) acc += val[base + (k << 6)];
+    // sumA[faceCells] -= interfaceBouCoeffs (lduMatrixATmul.C:185-199)
+    if (R.ifStart)
+        for (int32_t e = R.ifStart[r]; e < R.ifStart[r + 1]; e++) acc += val[R.ifBase + e];
     sumA[r] = acc;
+}
+
+// Gauss-Seidel with coupled interfaces: bPrime = source, then the
+// interfaces as a Jacobi term of psi before the sweep with the negated
+// coefficients (GaussSeidelSmoother.C:117-143): bPrime -= mBou*psi(nbr)
+__global__ void k_interface_bprime(RowsView R, const double* __restrict__ val,
+                                   const double* __restrict__ psi,
+                                   const double* __restrict__ src, double* __restrict__ bPrime)
+{
+    const int32_t r = blockIdx.x*FH_BLOCK + threadIdx.x;
+    if (r >= R.n) return;
+    double acc = src[r];
+    for (int32_t e = R.ifStart[r]; e < R.ifStart[r + 1]; e++)
+        acc -= val[R.ifBase + e]*psi[R.ifNbr[e]];
+    bPrime[r] = acc;
+}
+
+void launch_interface_bprime(const RowsView& R, const double* val, const double* psi,
+                             const double* src, double* bPrime, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_interface_bprime, dim3(reduction_blocks(R.n)), dim3(FH_BLOCK), 0, s,
+                       R, val, psi, src, bPrime);
+}
+
+// the interface entries' values: val[ifBase + e] = -coeffs[face[e]]
+__global__ void k_interface_values(int32_t nE, const int32_t* __restrict__ face,
+                                   const double* __restrict__ coeffs, double* __restrict__ tail)
+{
+    const int32_t e = blockIdx.x*FH_BLOCK + threadIdx.x;
+    if (e < nE) tail[e] = -coeffs[face[e]];
+}
+
+void launch_interface_values(int32_t nE, const int32_t* face, const double* coeffs,
+                             double* tail, hipStream_t s)
+{
+    if (nE <= 0) return;
+    hipLaunchKernelGGL(k_interface_values, dim3((nE + FH_BLOCK - 1)/FH_BLOCK), dim3(FH_BLOCK), 0,
+                       s, nE, face, coeffs, tail);
 }
 
 void launch_sum_a(const RowsView& R, const double* diag, const double* val,

@@ -30,6 +30,10 @@ struct fh_matrix
     bool valTValid = false;
     fh::DBuf<double> d_sumA;
     bool sumAValid = false;
+    // coupled interfaces: interfaceBouCoeffs and interfaceIntCoeffs in the
+    // caller's interface-face order (set before a solve on a coupled mesh)
+    fh::DBuf<double> d_ifBou, d_ifInt;
+    bool ifSet = false;
 
     // FP32 copies for the mixed-precision inner solves
     fh::DBuf<float> d_diag32, d_val32;
@@ -120,8 +124,17 @@ struct fh_matrix
         R.sliceStart = mesh->d_sliceStart.p;
         R.col = mesh->d_col.p;
         R.face = mesh->d_face.p;
+        if (mesh->nIfFaces > 0)
+        {
+            R.ifStart = mesh->d_ifStart.p;
+            R.ifNbr = mesh->d_ifNbr.p;
+            R.ifBase = mesh->nnzPad;
+        }
         return R;
     }
+    // length of the value arrays: the SELL entries, then the interface
+    // entries of a coupled mesh
+    int64_t nVal() const { return std::max<int64_t>(mesh->nnzPad + mesh->nIfFaces, 1); }
 
     fh::SchedView sched() const
     {

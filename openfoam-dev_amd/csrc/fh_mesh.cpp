@@ -87,8 +87,49 @@ static void counting_sort(const std::vector<int32_t>& key, int32_t nKeys,
     idx.swap(out);
 }
 
+void build_interfaces(fh_mesh* m, int32_t nInterfaces, const int32_t* start,
+                      const int32_t* faceCells, const int32_t* nbrCells)
+{
+    if (nInterfaces < 0) fail(FH_ERR_INVALID, "nInterfaces must be >= 0");
+    if (nInterfaces == 0) return;
+    if (!start || !faceCells || !nbrCells) fail(FH_ERR_INVALID, "NULL interface addressing");
+    if (start[0] != 0) fail(FH_ERR_INVALID, "interfaceStart[0] must be 0");
+    for (int32_t i = 0; i < nInterfaces; i++)
+        if (start[i + 1] < start[i])
+            fail(FH_ERR_INVALID, "interfaceStart decreases at interface %d", i);
+    const int32_t nE = start[nInterfaces];
+    const int32_t N = m->nCells;
+    for (int32_t e = 0; e < nE; e++)
+        if (faceCells[e] < 0 || faceCells[e] >= N || nbrCells[e] < 0 || nbrCells[e] >= N)
+            fail(FH_ERR_INVALID, "interface face %d: cells %d / %d out of range", e,
+                 faceCells[e], nbrCells[e]);
+    m->nInterfaces = nInterfaces;
+    m->nIfFaces = nE;
+    // per library row, the interface faces in the caller's order (interface
+    // by interface, face by face): the order updateMatrixInterfaces adds them
+    std::vector<int32_t> cnt(N + 1, 0);
+    for (int32_t e = 0; e < nE; e++) cnt[m->iperm[faceCells[e]] + 1]++;
+    for (int32_t r = 0; r < N; r++) cnt[r + 1] += cnt[r];
+    m->ifStart = cnt;
+    m->ifNbr.assign(nE, 0);
+    m->ifFace.assign(nE, 0);
+    std::vector<int32_t> pos(cnt.begin(), cnt.end() - 1);
+    for (int32_t e = 0; e < nE; e++)
+    {
+        const int32_t r = m->iperm[faceCells[e]];
+        const int32_t k = pos[r]++;
+        m->ifNbr[k] = m->iperm[nbrCells[e]];
+        m->ifFace[k] = e;
+    }
+    hipStream_t s = m->setupStream;
+    m->d_ifStart.upload(m->ifStart, s);
+    m->d_ifNbr.upload(m->ifNbr, s);
+    m->d_ifFace.upload(m->ifFace, s);
+    FH_HIP(hipStreamSynchronize(s));
+}
+
 void build_mesh(fh_mesh* m, int32_t nCells, int32_t nFaces,
-                const int32_t* lower, const int32_t* upper)
+                const int32_t* lower, const int32_t* upper, bool allowBox)
 {
     if (nCells <= 0) fail(FH_ERR_INVALID, "nCells must be positive (got %d)", nCells);
     if (nFaces < 0) fail(FH_ERR_INVALID, "nFaces must be >= 0 (got %d)", nFaces);
@@ -130,7 +171,9 @@ void build_mesh(fh_mesh* m, int32_t nCells, int32_t nFaces,
     std::vector<int32_t> order(1, 0);
     const char* sweepEnv = std::getenv("FOAMHIP_SWEEP");
     const bool levelSweeps = sweepEnv && std::strcmp(sweepEnv, "levels") == 0;
-    const bool box = detect_box(m, nx, ny, nz);
+    // a coupled mesh takes the general rows (the box kernels have no
+    // interface terms)
+    const bool box = allowBox && detect_box(m, nx, ny, nz);
     if (box && !levelSweeps)
     {
         // natural numbering; the sweeps walk 16 x 16 x bz tiles
@@ -420,7 +463,8 @@ size_t mesh_device_bytes(const fh_mesh* m)
          + S.d_segStart.bytes() + S.d_upOff.bytes() + S.d_upList.bytes()
          + S.d_downOff.bytes() + S.d_downList.bytes() + S.d_order.bytes()
          + m->d_chunkStart.bytes() + m->d_fwdOff.bytes() + m->d_fwdDeps.bytes()
-         + m->d_bwdOff.bytes() + m->d_bwdDeps.bytes();
+         + m->d_bwdOff.bytes() + m->d_bwdDeps.bytes() + m->d_ifStart.bytes()
+         + m->d_ifNbr.bytes() + m->d_ifFace.bytes();
 }
 
 } // namespace fh

@@ -158,12 +158,22 @@ void matrix_coeffs_updated(fh_matrix* M)
 {
     fh_mesh* mesh = M->mesh;
     hipStream_t s = M->stream;
-    M->d_val.alloc(std::max<int64_t>(mesh->nnzPad, 1));
+    M->d_val.alloc(M->nVal());
     const double* lo = M->symmetric ? M->d_upperF.p : M->d_lowerF.p;
     if (M->diagonalOnly || mesh->nFaces == 0)
         FH_HIP(hipMemsetAsync(M->d_val.p, 0, M->d_val.bytes(), s));
     else
         launch_gather_values<double>(M->rows(), mesh->nnzPad, M->d_upperF.p, lo, M->d_val.p, s);
+    // the interface entries: -interfaceBouCoeffs (zero until they are set)
+    if (mesh->nIfFaces > 0)
+    {
+        if (M->ifSet)
+            launch_interface_values(mesh->nIfFaces, mesh->d_ifFace.p, M->d_ifBou.p,
+                                    M->d_val.p + mesh->nnzPad, s);
+        else
+            FH_HIP(hipMemsetAsync(M->d_val.p + mesh->nnzPad, 0,
+                                  sizeof(double)*mesh->nIfFaces, s));
+    }
     FH_HIP(hipStreamSynchronize(s));
     M->hasCoeffs = true;
     M->valTValid = M->sumAValid = M->f32Valid = M->dir32Valid = M->dir64Valid = false;
@@ -186,7 +196,7 @@ void ensure_valT(fh_matrix* M)
 {
     if (M->valTValid) return;
     fh_mesh* mesh = M->mesh;
-    M->d_valT.alloc(std::max<int64_t>(mesh->nnzPad, 1));
+    M->d_valT.alloc(M->nVal());
     if (M->diagonalOnly || mesh->nFaces == 0)
         FH_HIP(hipMemsetAsync(M->d_valT.p, 0, M->d_valT.bytes(), M->stream));
     else
@@ -196,6 +206,16 @@ void ensure_valT(fh_matrix* M)
         launch_gather_values<double>(M->rows(), mesh->nnzPad, lo, M->d_upperF.p,
                                      M->d_valT.p, M->stream);
     }
+    // Tmul's interfaces take the interfaceIntCoeffs (lduMatrixATmul.C:95-151)
+    if (mesh->nIfFaces > 0)
+    {
+        if (M->ifSet)
+            launch_interface_values(mesh->nIfFaces, mesh->d_ifFace.p, M->d_ifInt.p,
+                                    M->d_valT.p + mesh->nnzPad, M->stream);
+        else
+            FH_HIP(hipMemsetAsync(M->d_valT.p + mesh->nnzPad, 0,
+                                  sizeof(double)*mesh->nIfFaces, M->stream));
+    }
     M->valTValid = true;
 }
 
@@ -204,10 +224,9 @@ void ensure_f32(fh_matrix* M)
     if (M->f32Valid) return;
     fh_mesh* mesh = M->mesh;
     M->d_diag32.alloc(mesh->nCells);
-    M->d_val32.alloc(std::max<int64_t>(mesh->nnzPad, 1));
+    M->d_val32.alloc(M->nVal());
     launch_convert<double, float>(mesh->nCells, M->d_diag.p, M->d_diag32.p, M->stream);
-    launch_convert<double, float>(std::max<int64_t>(mesh->nnzPad, 1), M->d_val.p,
-                                  M->d_val32.p, M->stream);
+    launch_convert<double, float>(M->nVal(), M->d_val.p, M->d_val32.p, M->stream);
     M->f32Valid = true;
 }
 
@@ -981,6 +1000,26 @@ void smooth_t(fh_matrix* M, int smoother, T* psi, const T* src, int32_t nSweeps,
             }
             return;
         }
+        if constexpr (std::is_same<T, double>::value)
+        {
+            if (M->mesh->nIfFaces > 0)
+            {
+                // coupled interfaces: each sweep starts from bPrime = source
+                // less the interfaces' Jacobi term of psi before the sweep
+                // (GaussSeidelSmoother.C:117-143; the symmetric smoother's
+                // backward pass uses the same bPrime)
+                DBuf<T>* wk = smooth_work<T>(M);
+                wk[1].alloc(N);
+                d.x = wk[1].p;
+                for (int32_t s = 0; s < nSweeps; s++)
+                {
+                    launch_interface_bprime(M->rows(), val, psi, src, wk[1].p, M->stream);
+                    sweep_t<T>(M, SWEEP_GS_FWD, d, st);
+                    if (sym) sweep_t<T>(M, SWEEP_GS_BWD, d, st);
+                }
+                return;
+            }
+        }
         for (int32_t s = 0; s < nSweeps; s++)
         {
             sweep_t<T>(M, SWEEP_GS_FWD, d, st);
@@ -1429,6 +1468,17 @@ void solve_device(fh_matrix* M, const fh_controls* c, double* d_psiNat,
 {
     if (!M->hasCoeffs) fail(FH_ERR_INVALID, "matrix coefficients not set");
     validate_selection(M, c);
+    if (M->mesh->nIfFaces > 0)
+    {
+        if (!M->ifSet)
+            fail(FH_ERR_INVALID, "the mesh has coupled interfaces: set the matrix's interface "
+                                 "coefficients (fh_matrix_set_interfaces) before solving");
+        // GAMG would need the coupled interfaces agglomerated onto every
+        // level (GAMGInterface); the mixed precision path has no interface
+        // terms in its box kernels either way
+        if (c->solver == FH_SOLVER_GAMG || c->preconditioner == FH_PRECON_GAMG)
+            fail(FH_ERR_UNSUPPORTED, "GAMG with coupled interfaces is not supported");
+    }
     const int32_t N = M->mesh->nCells;
     const int32_t* perm = M->mesh->d_perm.p;
     hipStream_t s = M->stream;

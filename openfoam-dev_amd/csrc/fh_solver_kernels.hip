@@ -6,6 +6,7 @@
 #include "fh_kernels.h"
 #include "fh_solver_kernels.h"
 #include "fh_finalize_dev.h"
+#include "fh_reduce_dev.h"
 
 #include <algorithm>
 #include <cstdlib>
@@ -20,49 +21,6 @@ namespace fh
 #define FH_BLOCK 256
 // grid of the strided reduction kernels (few partials to sum)
 #define MIX_BLOCKS 1024
-
-__device__ __forceinline__ int32_t sell_base2(const int32_t* sliceStart, int32_t r)
-{
-    return sliceStart[r >> 6] + (r & 63);
-}
-
-// deterministic two-value block sum; results valid in thread 0
-__device__ __forceinline__ void block_sum2(double& a, double& b, double* smem)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
-    {
-        a += __shfl_xor(a, off, 64);
-        b += __shfl_xor(b, off, 64);
-    }
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0)
-    {
-        smem[wid] = a;
-        smem[FH_BLOCK/64 + wid] = b;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0)
-    {
-        double sa = 0.0, sb = 0.0;
-#pragma unroll
-        for (int i = 0; i < FH_BLOCK/64; i++) { sa += smem[i]; sb += smem[FH_BLOCK/64 + i]; }
-        a = sa;
-        b = sb;
-    }
-}
-
-__device__ __forceinline__ void write_partials(double a, double b, double* partials,
-                                               int32_t nBlocks)
-{
-    __shared__ double smem[2*FH_BLOCK/64];
-    block_sum2(a, b, smem);
-    if (threadIdx.x == 0)
-    {
-        partials[blockIdx.x] = a;
-        partials[nBlocks + blockIdx.x] = b;
-    }
-}
 
 // block sum of (a, b) over an FH_BLOCK block, then partials_arrive
 __device__ __forceinline__ void write_partials_fin(double a, double b, double* partials,
@@ -172,6 +130,7 @@ __global__ void k_spmv_dot(RowsView R, const T* __restrict__ diag, const T* __re
             const int32_t e = base + (k << 6);
             acc += val[e]*x[R.col[e]];
         }
+        acc = add_interfaces(R, r, val, x, acc);
         y[r] = acc;
         s0 = (double)acc*(double)(z ? z[r] : acc);
         if (v) s1 = (double)acc*(double)v[r];
@@ -1083,6 +1042,12 @@ __global__ void k_mixed_replace(RowsView R, const double* __restrict__ diag,
             const int32_t c = R.col[e];
             acc -= val[e]*(x[c] + (double)xc[c]);
         }
+        if (R.ifStart)
+            for (int32_t e = R.ifStart[r]; e < R.ifStart[r + 1]; e++)
+            {
+                const int32_t c = R.ifNbr[e];
+                acc -= val[R.ifBase + e]*(x[c] + (double)xc[c]);
+            }
         r32[r] = (float)acc;
         s1 += fabs(acc);
     }

@@ -785,11 +785,14 @@ int32_t launch_tile_local_apply_f32(const TileView& G, const float* x, const flo
     const int nTiles = G.nTI*G.nTJ*G.nTK;
     const dim3 grid((nTiles + 3)/4);
     constexpr int BZ = TileBZ<float>::pre;
-    static const bool rowWalk = [] {
+    // the quarter-row pipeline is selectable (FOAMHIP_TILE_APPLY=quarter);
+    // its per-lane quarter-row accesses touch four times the cache lines of
+    // the row walk, which is the faster one on this device
+    static const bool quarter = [] {
         const char* e = std::getenv("FOAMHIP_TILE_APPLY");
-        return e && std::strcmp(e, "rows") == 0;
+        return e && std::strcmp(e, "quarter") == 0;
     }();
-    if (symmetric && BZ == 16 && !rowWalk)
+    if (symmetric && BZ == 16 && quarter)
     {
         // quarter-row pipeline: one tile per wavefront
         const dim3 gq((unsigned)nTiles);

@@ -62,14 +62,26 @@ def _i32(a):
 class lduAddressing:
     """lduAddressing (lduAddressing.H:112-207) resident on the device."""
 
-    def __init__(self, nCells, lowerAddr, upperAddr):
+    def __init__(self, nCells, lowerAddr, upperAddr, interfaces=None):
+        """interfaces: None, or the coupled (cyclic) interfaces as a dict of
+        start [nInterfaces+1], faceCells and nbrCells (faces concatenated in
+        interface order; fh_mesh_create_coupled)."""
         self.lower = _i32(lowerAddr)
         self.upper = _i32(upperAddr)
         self.nCells = int(nCells)
         self.nFaces = int(self.lower.size)
         h = ctypes.c_void_p()
-        _check(lib().fh_mesh_create(self.nCells, self.nFaces, _ptr(self.lower),
-                                    _ptr(self.upper), ctypes.byref(h)))
+        self.nIfFaces = 0
+        if interfaces is None:
+            _check(lib().fh_mesh_create(self.nCells, self.nFaces, _ptr(self.lower),
+                                        _ptr(self.upper), ctypes.byref(h)))
+        else:
+            self._ifs = [_i32(interfaces[k]) for k in ("start", "faceCells", "nbrCells")]
+            self.nIfFaces = int(self._ifs[0][-1])
+            _check(lib().fh_mesh_create_coupled(self.nCells, self.nFaces, _ptr(self.lower),
+                                                _ptr(self.upper), len(self._ifs[0]) - 1,
+                                                *[_ptr(a) for a in self._ifs],
+                                                ctypes.byref(h)))
         self.handle = h
 
     def set_face_areas(self, Sf, magSf):
@@ -135,6 +147,14 @@ class lduMatrix:
         self.symmetric = lower is None
         _check(lib().fh_matrix_set(self.handle, _ptr(self._diag), _ptr(self._upper),
                                    _ptr(self._lower)))
+
+    def set_interfaces(self, bouCoeffs, intCoeffs=None):
+        """interfaceBouCoeffs (and interfaceIntCoeffs, default the same) of
+        a coupled mesh, one value per interface face."""
+        self._ifBou = _f64(bouCoeffs)
+        self._ifInt = _f64(intCoeffs) if intCoeffs is not None else None
+        _check(lib().fh_matrix_set_interfaces(self.handle, _ptr(self._ifBou),
+                                              _ptr(self._ifInt)))
 
     def set_device(self, d_diag, d_upper=None, d_lower=None):
         """Device pointers (ints), e.g. torch tensor data_ptr()."""

@@ -93,6 +93,12 @@ FH_SYMBOLS = {
     "fh_gamg_direction": (_i32, []),
     "fh_gamg_reset_direction": (None, []),
     "fh_mesh_create": (_i32, [_i32, _i32, _vp, _vp, ctypes.POINTER(_vp)]),
+    "fh_mesh_create_coupled": (_i32, [_i32, _i32, _vp, _vp, _i32, _vp, _vp, _vp,
+                                      ctypes.POINTER(_vp)]),
+    "fh_matrix_set_interfaces": (_i32, [_vp, _vp, _vp]),
+    "fh_face_geometry": (_i32, [_i32, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "fh_cell_geometry": (_i32, [_vp] * 8),
+    "fh_surface_interpolation": (_i32, [_vp] * 14),
     "fh_mesh_info_get": (_i32, [_vp, ctypes.POINTER(FhMeshInfo)]),
     "fh_mesh_destroy": (_i32, [_vp]),
     "fh_matrix_create": (_i32, [_vp, ctypes.POINTER(_vp)]),
@@ -151,6 +157,8 @@ CASE_SYMBOLS = {
     "fhc_rhs": (None, [_u64, _i64, _vp]),
     "fhc_lognormal": (None, [_i32, _i32, _i32, _u64, _dbl, _dbl, _vp]),
     "fhc_fourier_flux": (None, [_i32, _i32, _i32, _u64] + [_vp] * 7),
+    "fhc_graded_box_points": (None, [_i32, _i32, _i32, _dbl, _dbl, _dbl, _dbl, _dbl, _u64, _vp]),
+    "fhc_box_face_points": (None, [_i32, _i32, _i32, _vp]),
 }
 
 
@@ -160,6 +168,16 @@ def _load(name, symbols):
         raise RuntimeError(
             f"{path} is missing: build it with `make -C {PKG_ROOT}` "
             "(or __graft_entry__.build()); there is no fallback implementation")
+    if name == "libfoamhip.so":
+        # One HIP runtime per process: PyTorch (used for device buffers by
+        # callers and tests) ships its own libamdhip64 under the same soname.
+        # Loaded first, it is the one libfoamhip binds to as well; loaded
+        # after libfoamhip, torch would run on the other runtime and find no
+        # device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
     lib = ctypes.CDLL(path)
     for sym, (res, args) in symbols.items():
         fn = getattr(lib, sym)

@@ -94,3 +94,31 @@ class BoxCase:
         diag = diag + np.bincount(self.bFaceCells, ic, self.N)
         source = rhs + np.bincount(self.bFaceCells, bc, self.N)
         return diag, upper, source
+
+
+class GradedBoxCase(BoxCase):
+    """A box of nx x ny x nz hexes graded along z (expansion ratio zRatio,
+    last cell over first) with its interior points jittered by `jitter` of
+    the local cell size (seeded): the addressing and patches of BoxCase, the
+    points and the quadrilateral faces (internal faces, then boundary faces
+    in bFaceCells order, oriented owner -> neighbour / outwards).  The
+    geometry (face centres and areas, cell centres and volumes, weights and
+    delta coefficients) is computed from them, not assumed."""
+
+    def __init__(self, nx, ny=None, nz=None, L=(1.0, 1.0, 1.0), zRatio=100.0, jitter=0.15,
+                 seed=1, fixed_patches=(0,)):
+        super().__init__(nx, ny, nz, L, fixed_patches)
+        lib = _lib.case()
+        nx, ny, nz = self.nx, self.ny, self.nz
+        self.nPoints = (nx + 1) * (ny + 1) * (nz + 1)
+        self.points = np.empty(3 * self.nPoints)
+        lib.fhc_graded_box_points(nx, ny, nz, L[0], L[1], L[2], zRatio, jitter, seed,
+                                  _p(self.points))
+        nAll = self.F + self.nB
+        self.facePoints = np.empty(4 * nAll, np.int32)
+        lib.fhc_box_face_points(nx, ny, nz, _p(self.facePoints))
+        self.faceStart = (4 * np.arange(nAll + 1)).astype(np.int32)
+        # the uniform-box geometry of BoxCase does not describe this mesh
+        for k in ("weights", "magSf", "Sf", "deltaCoeffs", "V", "C", "Cf", "bMagSf", "bSf",
+                  "bDeltaCoeffs", "bCf", "nonOrthDeltaCoeffs"):
+            setattr(self, k, None)

@@ -139,6 +139,26 @@ double or_average(int32_t n, const double* f)
 /* matrix kernels                                                      */
 /* ------------------------------------------------------------------ */
 
+/* lduMatrix::updateMatrixInterfaces, blocking communication
+   (lduMatrixUpdateMatrixInterfaces.C:97-116): every interface in order
+   calls its updateInterfaceMatrix, for a cyclic patch
+   (cyclicFvPatchField.C:150-174) result[faceCells] -= coeffs*psi[nbr];
+   negate: the coefficients enter negated (mBouCoeffs of residual and of
+   the Gauss-Seidel smoothers, lduMatrixATmul.C:230-244,
+   GaussSeidelSmoother.C:118-125) */
+static void update_interfaces(const or_matrix* m, const double* coeffs, int negate,
+                              const double* psi, double* result)
+{
+    const or_interfaces* I = m->interfaces;
+    if (!I) return;
+    for (int32_t i = 0; i < I->nInterfaces; i++)
+        for (int32_t e = I->start[i]; e < I->start[i + 1]; e++)
+        {
+            const double c = negate ? -coeffs[e] : coeffs[e];
+            result[I->faceCells[e]] -= c*psi[I->nbrCells[e]];
+        }
+}
+
 /* lduMatrixATmul.C:34-92 */
 void or_amul(const or_matrix* m, const double* psi, double* Apsi)
 {
@@ -149,6 +169,7 @@ void or_amul(const or_matrix* m, const double* psi, double* Apsi)
         Apsi[a->upper[f]] += m->lower[f]*psi[a->lower[f]];
         Apsi[a->lower[f]] += m->upper[f]*psi[a->upper[f]];
     }
+    if (m->interfaces) update_interfaces(m, m->interfaces->bouCoeffs, 0, psi, Apsi);
 }
 
 /* lduMatrixTemplates.C:33-64: H starts at zero; each face takes the
@@ -282,9 +303,11 @@ void or_tmul(const or_matrix* m, const double* psi, double* Tpsi)
         Tpsi[a->upper[f]] += m->upper[f]*psi[a->lower[f]];
         Tpsi[a->lower[f]] += m->lower[f]*psi[a->upper[f]];
     }
+    /* lduMatrixATmul.C:132-141: the interfaceIntCoeffs */
+    if (m->interfaces) update_interfaces(m, m->interfaces->intCoeffs, 0, psi, Tpsi);
 }
 
-/* lduMatrixATmul.C:154-200 (no interfaces) */
+/* lduMatrixATmul.C:154-200 */
 void or_sumA(const or_matrix* m, double* sumA)
 {
     const or_addr* a = m->addr;
@@ -294,9 +317,16 @@ void or_sumA(const or_matrix* m, double* sumA)
         sumA[a->upper[f]] += m->lower[f];
         sumA[a->lower[f]] += m->upper[f];
     }
+    /* lduMatrixATmul.C:185-199: the interface boundary coefficients
+       subtracted from the sum, interface by interface in face order */
+    const or_interfaces* I = m->interfaces;
+    if (I)
+        for (int32_t i = 0; i < I->nInterfaces; i++)
+            for (int32_t e = I->start[i]; e < I->start[i + 1]; e++)
+                sumA[I->faceCells[e]] -= I->bouCoeffs[e];
 }
 
-/* lduMatrixATmul.C:203-280 (no interfaces) */
+/* lduMatrixATmul.C:203-280 */
 void or_residual(const or_matrix* m, const double* psi, const double* source,
                  double* rA)
 {
@@ -308,6 +338,9 @@ void or_residual(const or_matrix* m, const double* psi, const double* source,
         rA[a->upper[f]] -= m->lower[f]*psi[a->lower[f]];
         rA[a->lower[f]] -= m->upper[f]*psi[a->upper[f]];
     }
+    /* the interfaces with the negated boundary coefficients
+       (lduMatrixATmul.C:230-244, 266-273) */
+    if (m->interfaces) update_interfaces(m, m->interfaces->bouCoeffs, 1, psi, rA);
 }
 
 /* lduMatrixSolver.C:174-197 */
@@ -468,6 +501,10 @@ static void gauss_seidel(const or_matrix* m, double* psi, const double* source,
     for (int32_t sweep = 0; sweep < nSweeps; sweep++)
     {
         memcpy(bPrime, source, sizeof(double)*nCells);
+        /* the interfaces as an effective Jacobi term with the negated
+           boundary coefficients and psi before the sweep
+           (GaussSeidelSmoother.C:117-143) */
+        if (m->interfaces) update_interfaces(m, m->interfaces->bouCoeffs, 1, psi, bPrime);
         int32_t fStart, fEnd = own[0];
         for (int32_t c = 0; c < nCells; c++)
         {
@@ -1747,7 +1784,11 @@ int or_solve(const or_matrix* m, const or_controls* c, double* psi,
         case OR_SOLVER_PBICGSTAB: return pbicgstab(m, c, psi, source, perf, history);
         case OR_SOLVER_PBICG:     return pbicg(m, c, psi, source, perf, history);
         case OR_SOLVER_SMOOTH:    return smooth_solver(m, c, psi, source, perf, history);
-        case OR_SOLVER_GAMG:      return gamg_solver(m, c, psi, source, perf, history);
+        case OR_SOLVER_GAMG:
+            /* coupled interfaces need GAMGInterface agglomeration: not
+               restated */
+            if (m->interfaces) return -2;
+            return gamg_solver(m, c, psi, source, perf, history);
         case OR_SOLVER_DIAGONAL:  return diagonal_solver(m, psi, source, perf);
         default: return -1;
     }
@@ -1876,5 +1917,187 @@ void or_grad_gauss_linear(const or_addr* a, const double* phi,
             for (int d = 0; d < 3; d++)
                 bGrad[3*b + d] = grad[3*c + d] + n[d]*(bSnGrad[b] - ng);
         }
+    }
+}
+
+
+/* ------------------------------------------------------------------ */
+/* mesh geometry                                                       */
+/* ------------------------------------------------------------------ */
+
+typedef struct { double x, y, z; } or_vec;
+
+static or_vec v_ld(const double* a, int64_t i) { or_vec v = {a[3*i], a[3*i + 1], a[3*i + 2]}; return v; }
+static void v_st(double* a, int64_t i, or_vec v) { a[3*i] = v.x; a[3*i + 1] = v.y; a[3*i + 2] = v.z; }
+static or_vec v_add(or_vec a, or_vec b) { or_vec v = {a.x + b.x, a.y + b.y, a.z + b.z}; return v; }
+static or_vec v_sub(or_vec a, or_vec b) { or_vec v = {a.x - b.x, a.y - b.y, a.z - b.z}; return v; }
+static or_vec v_mul(double s, or_vec a) { or_vec v = {s*a.x, s*a.y, s*a.z}; return v; }
+static or_vec v_div(or_vec a, double s) { or_vec v = {a.x/s, a.y/s, a.z/s}; return v; }
+/* Vector::operator& (VectorI.H:145-148) */
+static double v_dot(or_vec a, or_vec b) { return a.x*b.x + a.y*b.y + a.z*b.z; }
+/* Vector::operator^ (VectorI.H:152-160) */
+static or_vec v_cross(or_vec a, or_vec b)
+{
+    or_vec v = {a.y*b.z - a.z*b.y, a.z*b.x - a.x*b.z, a.x*b.y - a.y*b.x};
+    return v;
+}
+/* mag = sqrt(magSqr), magSqr summed x, y, z (VectorSpaceI.H:392-415) */
+static double v_mag(or_vec a) { return sqrt(a.x*a.x + a.y*a.y + a.z*a.z); }
+
+#define OR_VSMALL DBL_MIN           /* doubleScalarVSmall */
+
+/* faceTemplates.C:79-149 */
+static void face_area_centre(int32_t n, const int32_t* pts, const double* points,
+                             or_vec* area, or_vec* centre)
+{
+    if (n == 3)
+    {
+        const or_vec p0 = v_ld(points, pts[0]), p1 = v_ld(points, pts[1]);
+        const or_vec p2 = v_ld(points, pts[2]);
+        *area = v_mul(1.0/2.0, v_cross(v_sub(p1, p0), v_sub(p2, p0)));
+        *centre = v_mul(1.0/3.0, v_add(v_add(p0, p1), p2));
+        return;
+    }
+    or_vec pAvg = {0.0, 0.0, 0.0};
+    for (int32_t i = 0; i < n; i++) pAvg = v_add(pAvg, v_ld(points, pts[i]));
+    pAvg = v_div(pAvg, (double)n);
+    or_vec sumA = {0.0, 0.0, 0.0};
+    for (int32_t i = 0; i < n; i++)
+    {
+        const or_vec p = v_ld(points, pts[i]), pNext = v_ld(points, pts[(i + 1) % n]);
+        sumA = v_add(sumA, v_cross(v_sub(pNext, p), v_sub(pAvg, p)));
+    }
+    /* normalised (VectorSpaceI.H:419-426) */
+    const double magA = v_mag(sumA);
+    const or_vec zero = {0.0, 0.0, 0.0};
+    const or_vec sumAHat = magA > 0 ? v_div(sumA, magA) : zero;
+    double sumAn = 0.0;
+    or_vec sumAnc = {0.0, 0.0, 0.0};
+    for (int32_t i = 0; i < n; i++)
+    {
+        const or_vec p = v_ld(points, pts[i]), pNext = v_ld(points, pts[(i + 1) % n]);
+        const or_vec a = v_cross(v_sub(pNext, p), v_sub(pAvg, p));
+        const or_vec c = v_add(v_add(p, pNext), pAvg);
+        const double an = v_dot(a, sumAHat);
+        sumAn += an;
+        sumAnc = v_add(sumAnc, v_mul(an, c));
+    }
+    *area = v_mul(1.0/2.0, sumA);
+    *centre = sumAn > OR_VSMALL ? v_div(v_mul(1.0/3.0, sumAnc), sumAn) : pAvg;
+}
+
+void or_face_geometry(int32_t nFaces, const int32_t* faceStart, const int32_t* facePoints,
+                      const double* points, double* Cf, double* Sf, double* magSf)
+{
+    const double rootVSmall = sqrt(OR_VSMALL);
+    for (int32_t f = 0; f < nFaces; f++)
+    {
+        or_vec a, c;
+        face_area_centre(faceStart[f + 1] - faceStart[f], facePoints + faceStart[f], points,
+                         &a, &c);
+        v_st(Cf, f, c);
+        v_st(Sf, f, a);
+        const double m = v_mag(a);
+        magSf[f] = m > rootVSmall ? m : rootVSmall;
+    }
+}
+
+void or_cell_geometry(int32_t nCells, int32_t nInternal, const int32_t* lower,
+                      const int32_t* upper, int32_t nB, const int32_t* bFaceCells,
+                      const double* Cf, const double* Sf, const double* bCf, const double* bSf,
+                      double* C, double* V)
+{
+    or_vec* cEst = (or_vec*)calloc((size_t)nCells, sizeof(or_vec));
+    or_vec* cc = (or_vec*)calloc((size_t)nCells, sizeof(or_vec));
+    int32_t* nCellFaces = (int32_t*)calloc((size_t)nCells, sizeof(int32_t));
+    for (int32_t c = 0; c < nCells; c++) V[c] = 0.0;
+    /* owner faces (internal, then boundary), then neighbour faces */
+    for (int32_t f = 0; f < nInternal; f++)
+    {
+        cEst[lower[f]] = v_add(cEst[lower[f]], v_ld(Cf, f));
+        nCellFaces[lower[f]] += 1;
+    }
+    for (int32_t b = 0; b < nB; b++)
+    {
+        cEst[bFaceCells[b]] = v_add(cEst[bFaceCells[b]], v_ld(bCf, b));
+        nCellFaces[bFaceCells[b]] += 1;
+    }
+    for (int32_t f = 0; f < nInternal; f++)
+    {
+        cEst[upper[f]] = v_add(cEst[upper[f]], v_ld(Cf, f));
+        nCellFaces[upper[f]] += 1;
+    }
+    for (int32_t c = 0; c < nCells; c++) cEst[c] = v_div(cEst[c], (double)nCellFaces[c]);
+    for (int32_t f = 0; f < nInternal + nB; f++)
+    {
+        const int internal = f < nInternal;
+        const int32_t o = internal ? lower[f] : bFaceCells[f - nInternal];
+        const or_vec fc = internal ? v_ld(Cf, f) : v_ld(bCf, f - nInternal);
+        const or_vec fa = internal ? v_ld(Sf, f) : v_ld(bSf, f - nInternal);
+        const double pyr3Vol = v_dot(fa, v_sub(fc, cEst[o]));
+        const or_vec pc = v_add(v_mul(3.0/4.0, fc), v_mul(1.0/4.0, cEst[o]));
+        cc[o] = v_add(cc[o], v_mul(pyr3Vol, pc));
+        V[o] += pyr3Vol;
+    }
+    for (int32_t f = 0; f < nInternal; f++)
+    {
+        const int32_t nb = upper[f];
+        const or_vec fc = v_ld(Cf, f), fa = v_ld(Sf, f);
+        const double pyr3Vol = v_dot(fa, v_sub(cEst[nb], fc));
+        const or_vec pc = v_add(v_mul(3.0/4.0, fc), v_mul(1.0/4.0, cEst[nb]));
+        cc[nb] = v_add(cc[nb], v_mul(pyr3Vol, pc));
+        V[nb] += pyr3Vol;
+    }
+    for (int32_t c = 0; c < nCells; c++)
+    {
+        const or_vec r = fabs(V[c]) > OR_VSMALL ? v_div(cc[c], V[c]) : cEst[c];
+        v_st(C, c, r);
+        V[c] *= (1.0/3.0);
+    }
+    free(cEst);
+    free(cc);
+    free(nCellFaces);
+}
+
+void or_surface_interpolation(int32_t nInternal, const int32_t* lower, const int32_t* upper,
+                              const double* C, const double* Cf, const double* Sf,
+                              const double* magSf, double* weights, double* deltaCoeffs,
+                              double* nonOrthDeltaCoeffs, int32_t nB,
+                              const int32_t* bFaceCells, const double* bCf, const double* bSf,
+                              const double* bMagSf, double* bDeltaCoeffs,
+                              double* bNonOrthDeltaCoeffs)
+{
+    const double vGreat = DBL_MAX/10;
+    for (int32_t f = 0; f < nInternal; f++)
+    {
+        const or_vec sf = v_ld(Sf, f), cf = v_ld(Cf, f);
+        const or_vec cO = v_ld(C, lower[f]), cN = v_ld(C, upper[f]);
+        /* surfaceInterpolation.C:174-195 */
+        const double SfdOwn = fabs(v_dot(sf, v_sub(cf, cO)));
+        const double SfdNei = fabs(v_dot(sf, v_sub(cN, cf)));
+        const double SfdOwnNei = SfdOwn + SfdNei;
+        if (SfdNei/vGreat < SfdOwnNei) weights[f] = SfdNei/SfdOwnNei;
+        else
+        {
+            const double dOwn = v_mag(v_sub(cf, cO)), dNei = v_mag(v_sub(cN, cf));
+            weights[f] = dNei/(dOwn + dNei);
+        }
+        /* surfaceInterpolation.C:240-243 */
+        const or_vec delta = v_sub(cN, cO);
+        deltaCoeffs[f] = 1.0/v_mag(delta);
+        /* surfaceInterpolation.C:300-315 */
+        const or_vec unitArea = v_div(sf, magSf[f]);
+        const double a = v_dot(unitArea, delta), b = 0.05*v_mag(delta);
+        nonOrthDeltaCoeffs[f] = 1.0/(a > b ? a : b);
+    }
+    for (int32_t k = 0; k < nB; k++)
+    {
+        /* fvPatch::nf, fvPatch::delta (fvPatch.C:138-164) */
+        const or_vec nf = v_div(v_ld(bSf, k), bMagSf[k]);
+        const or_vec d = v_sub(v_ld(bCf, k), v_ld(C, bFaceCells[k]));
+        const or_vec delta = v_mul(v_dot(nf, d), nf);
+        bDeltaCoeffs[k] = 1.0/v_mag(delta);
+        const double a = v_dot(nf, delta), b = 0.05*v_mag(delta);
+        bNonOrthDeltaCoeffs[k] = 1.0/(a > b ? a : b);
     }
 }

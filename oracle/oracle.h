@@ -40,15 +40,32 @@ int  or_addr_init(or_addr* a, int32_t nCells, int32_t nFaces,
                   const int32_t* lower, const int32_t* upper);
 void or_addr_free(or_addr* a);
 
+/* Coupled interfaces of a matrix (lduInterfaceFieldPtrsList with their
+   interfaceBouCoeffs / interfaceIntCoeffs): interface i owns faces
+   start[i] .. start[i+1]-1 of the concatenated arrays; face e couples
+   faceCells[e] to nbrCells[e] (cyclicFvPatchField::updateInterfaceMatrix,
+   cyclicFvPatchField.C:150-174: result[faceCells] -= coeffs*psi[nbr
+   faceCells], no transformation for scalars). */
+typedef struct
+{
+    int32_t nInterfaces;
+    const int32_t* start;       /* [nInterfaces+1] */
+    const int32_t* faceCells;   /* [start[nInterfaces]] */
+    const int32_t* nbrCells;
+    const double* bouCoeffs;    /* interfaceBouCoeffs */
+    const double* intCoeffs;    /* interfaceIntCoeffs (Tmul) */
+} or_interfaces;
+
 /* Matrix view: lower == upper (same pointer) for a symmetric matrix,
    as lduMatrix::lower() const returns upper when lowerPtr_ is unset
-   (lduMatrix.C:168-312). */
+   (lduMatrix.C:168-312); interfaces NULL: no coupled patches. */
 typedef struct
 {
     const or_addr* addr;
     const double* diag;
     const double* upper;
     const double* lower;
+    const or_interfaces* interfaces;
 } or_matrix;
 
 /* ---- reductions (FieldReductionFunctions.C) ---- */
@@ -257,6 +274,34 @@ void or_grad_gauss_linear(const or_addr* a, const double* phi,
                           const double* bSf, const double* bMagSf,
                           const double* bPhi, const double* bSnGrad,
                           double* grad, double* bGrad);
+
+/* ---- mesh geometry (primitiveMesh, surfaceInterpolation, fvPatch) ----
+   Faces are polygons: face f has points facePoints[faceStart[f] ..
+   faceStart[f+1]-1]; internal faces 0..nInternal-1 (owner lower[f],
+   neighbour upper[f]) then boundary faces, owned by bFaceCells[b].
+   Vectors are packed x, y, z. */
+/* face::areaAndCentre (faceTemplates.C:79-149) per face, magSf =
+   max(mag(Sf), rootVSmall) (primitiveMeshFaceCentresAndAreas.C) */
+void or_face_geometry(int32_t nFaces, const int32_t* faceStart, const int32_t* facePoints,
+                      const double* points, double* Cf, double* Sf, double* magSf);
+/* primitiveMesh::makeCellCentresAndVols (primitiveMeshCellCentresAndVols.C:72-154);
+   Cf/Sf of the internal faces then of the boundary faces */
+void or_cell_geometry(int32_t nCells, int32_t nInternal, const int32_t* lower,
+                      const int32_t* upper, int32_t nB, const int32_t* bFaceCells,
+                      const double* Cf, const double* Sf, const double* bCf, const double* bSf,
+                      double* C, double* V);
+/* surfaceInterpolation::makeWeights / makeDeltaCoeffs /
+   makeNonOrthDeltaCoeffs (surfaceInterpolation.C:135-330) for the internal
+   faces, and the non-coupled patch values (fvPatch.C:159-164: delta =
+   nf (nf & (Cf - Cn)); weights 1, deltaCoeffs 1/mag(delta),
+   nonOrthDeltaCoeffs 1/max(nf & delta, 0.05 mag(delta))) */
+void or_surface_interpolation(int32_t nInternal, const int32_t* lower, const int32_t* upper,
+                              const double* C, const double* Cf, const double* Sf,
+                              const double* magSf, double* weights, double* deltaCoeffs,
+                              double* nonOrthDeltaCoeffs, int32_t nB,
+                              const int32_t* bFaceCells, const double* bCf, const double* bSf,
+                              const double* bMagSf, double* bDeltaCoeffs,
+                              double* bNonOrthDeltaCoeffs);
 
 #ifdef __cplusplus
 }

@@ -23,8 +23,14 @@ class OrAddr(ctypes.Structure):
                 ("losortStart", ctypes.POINTER(_i32))]
 
 
+class OrInterfaces(ctypes.Structure):
+    _fields_ = [("nInterfaces", _i32), ("start", _vp), ("faceCells", _vp), ("nbrCells", _vp),
+                ("bouCoeffs", _vp), ("intCoeffs", _vp)]
+
+
 class OrMatrix(ctypes.Structure):
-    _fields_ = [("addr", ctypes.POINTER(OrAddr)), ("diag", _vp), ("upper", _vp), ("lower", _vp)]
+    _fields_ = [("addr", ctypes.POINTER(OrAddr)), ("diag", _vp), ("upper", _vp), ("lower", _vp),
+                ("interfaces", ctypes.POINTER(OrInterfaces))]
 
 
 class OrControls(ctypes.Structure):
@@ -72,6 +78,11 @@ def lib():
             getattr(L, name).restype = _dbl
         L.or_sumProd.argtypes = [_i32, _vp, _vp]
         L.or_sumProd.restype = _dbl
+        L.or_face_geometry.argtypes = [_i32, _vp, _vp, _vp, _vp, _vp, _vp]
+        L.or_cell_geometry.argtypes = [_i32, _i32, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp,
+                                       _vp, _vp]
+        L.or_surface_interpolation.argtypes = [_i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                               _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp]
         M = ctypes.POINTER(OrMatrix)
         L.or_amul.argtypes = [M, _vp, _vp]
         L.or_tmul.argtypes = [M, _vp, _vp]
@@ -192,13 +203,27 @@ class Matrix:
     """Oracle lduMatrix; lower None -> symmetric (lower aliases upper),
     upper None -> diagonal."""
 
-    def __init__(self, addr, diag, upper=None, lower=None):
+    def __init__(self, addr, diag, upper=None, lower=None, interfaces=None):
+        """interfaces: None, or the coupled interfaces as a dict of arrays
+        start [nInterfaces+1], faceCells, nbrCells, bouCoeffs and (for Tmul)
+        intCoeffs (defaults to bouCoeffs), faces concatenated in interface
+        order."""
         self.addr = addr
         self.diag = _f(diag)
         self.upper = _f(upper) if upper is not None else None
         self.lower = _f(lower) if lower is not None else self.upper
         self.symmetric = lower is None
         self.s = OrMatrix(ctypes.pointer(addr.s), _p(self.diag), _p(self.upper), _p(self.lower))
+        self._if = None
+        if interfaces is not None:
+            st = np.ascontiguousarray(interfaces["start"], np.int32)
+            fc = np.ascontiguousarray(interfaces["faceCells"], np.int32)
+            nb = np.ascontiguousarray(interfaces["nbrCells"], np.int32)
+            bou = _f(interfaces["bouCoeffs"])
+            itc = _f(interfaces.get("intCoeffs", bou))
+            self._if = (st, fc, nb, bou, itc)
+            self._ifs = OrInterfaces(len(st) - 1, _p(st), _p(fc), _p(nb), _p(bou), _p(itc))
+            self.s.interfaces = ctypes.pointer(self._ifs)
         self.ref = ctypes.byref(self.s)
 
     def _out(self):
@@ -461,3 +486,36 @@ def convection_diffusion(addr, case, phi, bPhi, D):
         case.bFaceCells, case.bMagSf, case.bDeltaCoeffs, case.bType, case.bValue)
     # gamma_f of a uniform field: w*(D - D) + D = D, as the kernel computes
     return dg_c - dg_l, up_c - up_l, lo_c - up_l, ic_c - ic_l, bc_c - bc_l
+
+
+def mesh_geometry(nCells, lower, upper, bFaceCells, faceStart, facePoints, points):
+    """primitiveMesh face and cell geometry and the surfaceInterpolation
+    coefficients of a polyhedral mesh (faces: internal then boundary).
+    Returns a dict of Cf, Sf, magSf, bCf, bSf, bMagSf, C, V, weights,
+    deltaCoeffs, nonOrthDeltaCoeffs, bDeltaCoeffs, bNonOrthDeltaCoeffs."""
+    lower = np.ascontiguousarray(lower, np.int32)
+    upper = np.ascontiguousarray(upper, np.int32)
+    bfc = np.ascontiguousarray(bFaceCells, np.int32)
+    fs = np.ascontiguousarray(faceStart, np.int32)
+    fp = np.ascontiguousarray(facePoints, np.int32)
+    pts = _f(points)
+    F, B = len(lower), len(bfc)
+    nAll = F + B
+    Cf, Sf, magSf = np.empty(3 * nAll), np.empty(3 * nAll), np.empty(nAll)
+    lib().or_face_geometry(nAll, _p(fs), _p(fp), _p(pts), _p(Cf), _p(Sf), _p(magSf))
+    g = {"Cf": Cf[:3 * F].copy(), "Sf": Sf[:3 * F].copy(), "magSf": magSf[:F].copy(),
+         "bCf": Cf[3 * F:].copy(), "bSf": Sf[3 * F:].copy(), "bMagSf": magSf[F:].copy()}
+    C, V = np.empty(3 * nCells), np.empty(nCells)
+    lib().or_cell_geometry(nCells, F, _p(lower), _p(upper), B, _p(bfc), _p(g["Cf"]),
+                           _p(g["Sf"]), _p(g["bCf"]), _p(g["bSf"]), _p(C), _p(V))
+    g["C"], g["V"] = C, V
+    for k in ("weights", "deltaCoeffs", "nonOrthDeltaCoeffs"):
+        g[k] = np.empty(F)
+    for k in ("bDeltaCoeffs", "bNonOrthDeltaCoeffs"):
+        g[k] = np.empty(B)
+    lib().or_surface_interpolation(F, _p(lower), _p(upper), _p(C), _p(g["Cf"]), _p(g["Sf"]),
+                                   _p(g["magSf"]), _p(g["weights"]), _p(g["deltaCoeffs"]),
+                                   _p(g["nonOrthDeltaCoeffs"]), B, _p(bfc), _p(g["bCf"]),
+                                   _p(g["bSf"]), _p(g["bMagSf"]), _p(g["bDeltaCoeffs"]),
+                                   _p(g["bNonOrthDeltaCoeffs"]))
+    return g

@@ -10,10 +10,11 @@ the oracle's minutes per solve at these sizes:
                               (the reference's own spread)
   c3_pcg_dic_256.json         PCG+DIC, 256^3 log-normal Poisson, tolerance
                               1e-8 and 1e-12 (the tight solution the error
-                              statements are measured against)
+                              statements are measured against), and the 1e-8
+                              solve with b perturbed by 1e-15 relative
 
 Run from the repository root after building the oracle:
-    python tests/golden/make_config_fixtures.py [c2] [c3]
+    python tests/golden/make_config_fixtures.py [c2] [c3] [c3-perturbed]
 """
 import json
 import os
@@ -49,25 +50,36 @@ def c2():
     om = O.Matrix(a, diag, upper, lower)
     out = {"case": "c2_convection_diffusion(n=128, seed=2, D=1e-3)",
            "solve": solve(om, CC.C2_CONTROLS, c.N, b)}
-    # the reference's own sensitivity: the same solve with b perturbed by a
-    # relative 1e-15 (seeded), i.e. at the level of a different summation order
-    rng = np.random.default_rng(5)
-    bp = b * (1.0 + 1e-15 * rng.standard_normal(c.N))
-    psi_p, perf_p = om.solve(CC.C2_CONTROLS, np.zeros(c.N), bp)
-    out["perturbed"] = {"perf": perf_dict(perf_p),
-                        "rel_l2_vs_solve": CC.rel_diff_estimate(psi_p, out["solve"]["psi"])}
-    print("  perturbed:", out["perturbed"], flush=True)
+    out["perturbed"] = perturbed(om, CC.C2_CONTROLS, c.N, b, out["solve"]["psi"])
     with open(os.path.join(HERE, "c2_pbicgstab_dilu_128.json"), "w") as f:
         json.dump(out, f)
 
 
-def c3():
+def perturbed(om, ctl, N, b, ref_summary):
+    """The reference's own sensitivity: the same solve with b perturbed by a
+    relative 1e-15 (seeded), i.e. at the level of a different summation
+    order."""
+    rng = np.random.default_rng(5)
+    bp = b * (1.0 + 1e-15 * rng.standard_normal(N))
+    psi_p, perf_p = om.solve(ctl, np.zeros(N), bp)
+    out = {"perf": perf_dict(perf_p), "rel_l2_vs_solve": CC.rel_diff_estimate(psi_p, ref_summary)}
+    print("  perturbed:", out, flush=True)
+    return out
+
+
+def c3(only_perturbed=False):
     c, a, diag, upper, b = CC.c3_poisson_lognormal()
     om = O.Matrix(a, diag, upper)
-    out = {"case": "c3_poisson_lognormal(n=256)",
-           "solve": solve(om, CC.C3_CONTROLS, c.N, b),
-           "tight": solve(om, dict(CC.C3_CONTROLS, tolerance=CC.C3_TIGHT), c.N, b)}
-    with open(os.path.join(HERE, "c3_pcg_dic_256.json"), "w") as f:
+    path = os.path.join(HERE, "c3_pcg_dic_256.json")
+    if only_perturbed:
+        with open(path) as f:
+            out = json.load(f)
+    else:
+        out = {"case": "c3_poisson_lognormal(n=256)",
+               "solve": solve(om, CC.C3_CONTROLS, c.N, b),
+               "tight": solve(om, dict(CC.C3_CONTROLS, tolerance=CC.C3_TIGHT), c.N, b)}
+    out["perturbed"] = perturbed(om, CC.C3_CONTROLS, c.N, b, out["solve"]["psi"])
+    with open(path, "w") as f:
         json.dump(out, f)
 
 
@@ -75,4 +87,4 @@ if __name__ == "__main__":
     which = sys.argv[1:] or ["c2", "c3"]
     for w in which:
         print(w, flush=True)
-        {"c2": c2, "c3": c3}[w]()
+        {"c2": c2, "c3": c3, "c3-perturbed": lambda: c3(only_perturbed=True)}[w]()

@@ -82,3 +82,62 @@ def dense(n, lower, upper, diag, up, lo=None):
     A[lower, upper] = up
     A[upper, lower] = lo
     return A
+
+
+def cyclic_box(nx, ny, nz):
+    """A hex box whose x-min and x-max patches form one cyclic pair: the
+    LDU addressing of the box, and the two coupled interfaces (x-min
+    patch first) with their faces in (k, j) order, face e of one patch
+    matching face e of the other."""
+    n, lower, upper = box(nx, ny, nz)
+    lo = np.array([nx * (j + ny * k) for k in range(nz) for j in range(ny)], np.int32)
+    hi = lo + (nx - 1)
+    nf = len(lo)
+    return n, lower, upper, {"start": np.array([0, nf, 2 * nf], np.int32),
+                             "faceCells": np.concatenate([lo, hi]),
+                             "nbrCells": np.concatenate([hi, lo])}
+
+
+def dense_interfaces(n, ifc, coeffs):
+    """The interfaces' part of the dense matrix: result[faceCells] -=
+    coeffs*psi[nbrCells] is A[faceCells, nbrCells] -= coeffs."""
+    C = np.zeros((n, n))
+    np.add.at(C, (ifc["faceCells"], ifc["nbrCells"]), -np.asarray(coeffs))
+    return C
+
+
+def cyclic_system(kind, seed=3, shape=(6, 5, 4), with_dense=True):
+    """Random coefficients on a box with an x cyclic pair: symmetric
+    ('sym', the same coefficient on both faces of a pair) or asymmetric
+    (interfaceIntCoeffs of a face = bouCoeffs of its partner face, so that
+    Tmul is the transpose); the couplings -bou have the sign of the internal
+    off-diagonals and their magnitude is added to the diagonal.  Returns the
+    addressing, coefficients, interfaces and the dense matrix and its
+    Tmul counterpart."""
+    rng = np.random.default_rng(seed)
+    n, lower, upper, ifc = cyclic_box(*shape)
+    nf = ifc["start"][1]
+    if kind == "sym":
+        diag, up = spd_coeffs(n, lower, upper, rng)
+        lo = None
+        # the same coefficient on both sides of a cyclic face pair
+        b0 = rng.uniform(0.1, 1.0, nf)
+        bou = np.concatenate([b0, b0])
+        itc = bou
+    else:
+        diag, up, lo = asym_coeffs(n, lower, upper, rng)
+        bou = rng.uniform(0.1, 1.0, 2 * nf)
+        # the transpose of the coupling: face e of one patch carries as
+        # interfaceIntCoeffs the boundary coefficient of face e of the other
+        itc = np.concatenate([bou[nf:], bou[:nf]])
+    # the coupled faces' couplings -bou have the sign of the internal
+    # off-diagonal coefficients (negative here); their magnitude goes on the
+    # diagonal, as addBoundaryDiag does with the coupled patch's
+    # internalCoeffs, keeping the matrix diagonally dominant
+    np.add.at(diag, ifc["faceCells"], bou + 0.05)
+    ifc = dict(ifc, bouCoeffs=bou, intCoeffs=itc)
+    if not with_dense:
+        return n, lower, upper, diag, up, lo, ifc, None, None
+    A = dense(n, lower, upper, diag, up, lo) + dense_interfaces(n, ifc, bou)
+    AT = dense(n, lower, upper, diag, up, lo).T + dense_interfaces(n, ifc, itc)
+    return n, lower, upper, diag, up, lo, ifc, A, AT

@@ -12,7 +12,8 @@ the C-ABI:
       128^3 bitwise against a live oracle (gaussGrad.C:30-75).
   C3  256^3 log-normal Poisson: the first iterations against a live oracle
       (same trajectory), the FP64 solve to 1e-8 against the oracle's
-      solution summary (tests/golden/c3_pcg_dic_256.json), the mixed solve's
+      solution summary (tests/golden/c3_pcg_dic_256.json) within the
+      reference's own spread under a 1e-15 perturbation of b, the mixed solve's
       FP64 residual recomputed and its error against the oracle's tight
       solution.
   C5  a serving stream whose meshes span 16^3..128^3: each result is the
@@ -50,17 +51,27 @@ def rel_l2(x, y):
     return np.linalg.norm(x - y) / np.linalg.norm(y)
 
 
-def check_against_summary(psi, perf, ref, tol=SOLUTION_TOL):
+def check_against_summary(psi, perf, ref, spread=None):
+    """The solve against the oracle's summary.  Bar: +-ITER_TOL iterations
+    and SOLUTION_TOL relative L2, or, where the reference itself moves by
+    more when b is perturbed by a relative 1e-15 (the level of a different
+    summation order; `spread`, stored with the fixture), twice that
+    movement."""
     op = ref["perf"]
+    tol, its = SOLUTION_TOL, ITER_TOL
+    if spread is not None:
+        tol = max(tol, 2 * spread["rel_l2_vs_solve"])
+        its = max(its, 2 * abs(spread["perf"]["nIterations"] - op["nIterations"]))
     assert perf.converged == op["converged"]
-    assert abs(perf.nIterations - op["nIterations"]) <= ITER_TOL, (perf, op)
+    assert abs(perf.nIterations - op["nIterations"]) <= its, (perf, op)
     assert perf.normFactor == pytest.approx(op["normFactor"], rel=1e-12)
     assert perf.initialResidual == pytest.approx(op["initialResidual"], rel=1e-12)
     est = CC.rel_diff_estimate(psi, ref["psi"])
     sample = np.asarray(ref["psi"]["sample"])
     worst = np.max(np.abs(psi[CC.sample_index(psi.size)] - sample)) / np.max(np.abs(sample))
     print(f"iterations {perf.nIterations} (oracle {op['nIterations']}), estimated rel L2 "
-          f"{est:.2e}, sampled max rel {worst:.2e}")
+          f"{est:.2e}, sampled max rel {worst:.2e} (bar {tol:.2e}; reference spread "
+          f"{spread})")
     assert est <= tol
     assert worst <= 10 * tol
     assert np.linalg.norm(psi) == pytest.approx(ref["psi"]["norm"], rel=tol)
@@ -116,6 +127,11 @@ def test_c2_pbicgstab_dilu_128():
     m = fh.lduMatrix(fh.lduAddressing(c.N, c.lower, c.upper), diag, upper, lower)
     psi = np.zeros(c.N)
     perf = fh.solver_New("T", m, CC.C2_CONTROLS).solve(psi, b)
+    # deterministic: the same bits on a second solve (fixed-order sums)
+    psi2 = np.zeros(c.N)
+    perf2 = fh.solver_New("T", m, CC.C2_CONTROLS).solve(psi2, b)
+    np.testing.assert_array_equal(psi2, psi)
+    assert perf2.nIterations == perf.nIterations
     op = ref["perf"]
     its_spread = abs(spread["perf"]["nIterations"] - op["nIterations"])
     est = CC.rel_diff_estimate(psi, ref["psi"])
@@ -188,7 +204,10 @@ def test_c3_fp64_256(c3):
     ref = fixture("c3_pcg_dic_256.json")
     psi = np.zeros(c.N)
     perf = fh.solver_New("p", m, CC.C3_CONTROLS).solve(psi, b)
-    check_against_summary(psi, perf, ref["solve"])
+    # PCG over 760 iterations at 256^3 (condition number ~1e7) carries a
+    # rounding-level difference of the dot products into the solution at
+    # ~1e-8: the reference does the same under a 1e-15 perturbation of b
+    check_against_summary(psi, perf, ref["solve"], ref["perturbed"])
     r = b - O.Matrix(a, diag, upper).Amul(psi)
     assert np.abs(r).sum() / perf.normFactor == pytest.approx(perf.finalResidual, rel=1e-6)
 

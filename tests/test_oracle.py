@@ -3,6 +3,7 @@ dense linear algebra on the same matrix, closed forms, invariants and
 hand-worked known answers.  The reference ships no golden vectors for this
 path, so these checks are what makes the oracle trustworthy; each is chosen
 so that a dropped term, a swapped upper/lower or a wrong index fails."""
+import ctypes
 import json
 import os
 
@@ -877,3 +878,118 @@ def test_flux_hand_worked():
     fl, bf = m.flux([0, 2], ic3, bc3, P3)
     assert bf.tolist() == [[1.0, -2.0, 6.0], [50.0, -25.0, -200.0]]
     np.testing.assert_array_equal(fl[:, 1], -m.faceH(psi))
+
+
+# ---------------------------------------------------------------------------
+# coupled (cyclic) interfaces: lduMatrixATmul.C:34-280 with
+# interfaceBouCoeffs / interfaceIntCoeffs, lduMatrixUpdateMatrixInterfaces.C,
+# cyclicFvPatchField.C:150-174, GaussSeidelSmoother.C:117-143
+# ---------------------------------------------------------------------------
+
+from meshes import cyclic_box, cyclic_system, dense_interfaces  # noqa: E402
+
+
+def _cyclic_system(kind, seed=3, shape=(6, 5, 4)):
+    n, lower, upper, diag, up, lo, ifc, A, AT = cyclic_system(kind, seed, shape)
+    return n, O.Addr(n, lower, upper), diag, up, lo, ifc, A, AT
+
+
+@pytest.mark.parametrize("kind", ["sym", "asym"])
+def test_interfaces_vs_dense(kind):
+    """Amul, Tmul, residual and sumA with coupled interfaces = the dense
+    matrix with A[faceCells, nbrCells] -= interface coefficients (Tmul:
+    the transposed internal part and the interfaceIntCoeffs)."""
+    n, a, diag, up, lo, ifc, A, AT = _cyclic_system(kind)
+    m = O.Matrix(a, diag, up, lo, interfaces=ifc)
+    rng = np.random.default_rng(9)
+    psi, b = rng.standard_normal(n), rng.standard_normal(n)
+    np.testing.assert_allclose(m.Amul(psi), A @ psi, rtol=1e-13, atol=1e-13)
+    np.testing.assert_allclose(m.Tmul(psi), AT @ psi, rtol=1e-13, atol=1e-13)
+    np.testing.assert_allclose(m.residual(psi, b), b - A @ psi, rtol=1e-13, atol=1e-13)
+    np.testing.assert_allclose(m.sumA(), A.sum(axis=1), rtol=1e-13, atol=1e-13)
+    # a plain matrix is unchanged by the extension
+    m0 = O.Matrix(a, diag, up, lo)
+    np.testing.assert_allclose(m0.Amul(psi), (A - dense_interfaces(n, ifc, ifc["bouCoeffs"])) @ psi,
+                               rtol=1e-13, atol=1e-13)
+
+
+def test_cyclic_laplacian_is_periodic():
+    """fvm::laplacian(1, p) on a uniform box with an x cyclic pair (coupled
+    patch: internalCoeffs = boundaryCoeffs = -|Sf|*deltaCoeffs, the other
+    patches zeroGradient): A sin(2 pi i/nx) = 2 (cos(2 pi/nx) - 1) h^-2 V
+    sin(2 pi i/nx), the periodic second difference, exactly up to
+    rounding."""
+    nx, ny, nz = 8, 3, 2
+    n, lower, upper, ifc = cyclic_box(nx, ny, nz)
+    h = 1.0 / nx
+    # unit cube cells of size h in every direction: |Sf| deltaCoeffs = h
+    up = np.full(len(lower), h)
+    diag = np.zeros(n)
+    a = O.Addr(n, lower, upper)
+    O.lib().or_negSumDiag(ctypes.byref(a.s), O._p(up), O._p(up), O._p(diag))
+    bou = np.full(len(ifc["faceCells"]), -h)
+    np.add.at(diag, ifc["faceCells"], bou)
+    m = O.Matrix(a, diag, up, interfaces=dict(ifc, bouCoeffs=bou))
+    i = np.arange(n) % nx
+    f = np.sin(2 * np.pi * i / nx)
+    lam = 2 * (np.cos(2 * np.pi / nx) - 1) * h
+    np.testing.assert_allclose(m.Amul(f), lam * f, atol=1e-14)
+    # a constant field is in the null space (all patches non-fixed)
+    np.testing.assert_allclose(m.Amul(np.ones(n)), 0.0, atol=1e-14)
+
+
+@pytest.mark.parametrize("ctl,kind", [
+    ({"solver": "PCG", "preconditioner": "DIC"}, "sym"),
+    ({"solver": "PCG", "preconditioner": "none"}, "sym"),
+    ({"solver": "PBiCGStab", "preconditioner": "DILU"}, "asym"),
+    ({"solver": "PBiCG", "preconditioner": "DILU"}, "asym"),
+    ({"solver": "smoothSolver", "smoother": "symGaussSeidel", "nSweeps": 2}, "sym"),
+    ({"solver": "smoothSolver", "smoother": "GaussSeidel", "nSweeps": 1}, "asym"),
+    ({"solver": "smoothSolver", "smoother": "DIC", "nSweeps": 1}, "sym"),
+])
+def test_solvers_with_interfaces_converge(ctl, kind):
+    """The Krylov solvers and the smoothSolver with coupled interfaces
+    converge to the solution of the dense coupled matrix; the reported
+    residual is the reference's, recomputed with the interfaces."""
+    n, a, diag, up, lo, ifc, A, AT = _cyclic_system(kind, seed=5)
+    m = O.Matrix(a, diag, up, lo, interfaces=ifc)
+    b = np.random.default_rng(2).standard_normal(n)
+    ctl = dict(ctl, tolerance=1e-10, relTol=0, maxIter=5000)
+    psi, perf = m.solve(ctl, np.zeros(n), b)
+    assert perf["converged"]
+    x = np.linalg.solve(A, b)
+    assert np.linalg.norm(psi - x) / np.linalg.norm(x) < 1e-7
+    # (PBiCGStab and PBiCG report their recurrence residual)
+    r = b - A @ psi
+    assert np.abs(r).sum() / perf["normFactor"] == pytest.approx(perf["finalResidual"],
+                                                                rel=1e-2)
+
+
+@pytest.mark.parametrize("sym", [False, True])
+def test_gauss_seidel_with_interfaces_closed_form(sym):
+    """One Gauss-Seidel sweep with coupled interfaces treats them as a
+    Jacobi term of the field before the sweep: psi1 = (D + L)^-1 (b - U psi1
+    ... ) = (D + L)^-1 (b - C psi0 - U psi0) for the forward sweep (C the
+    interfaces' part); the symmetric sweep then runs backward with the same
+    interface term: psi2 = (D + U)^-1 (b - C psi0 - L psi1)."""
+    n, a, diag, up, lo, ifc, A, AT = _cyclic_system("asym", seed=8)
+    m = O.Matrix(a, diag, up, lo, interfaces=ifc)
+    rng = np.random.default_rng(4)
+    psi0, b = rng.standard_normal(n), rng.standard_normal(n)
+    C = dense_interfaces(n, ifc, ifc["bouCoeffs"])
+    Ai = A - C
+    D = np.diag(np.diag(Ai))
+    L = np.tril(Ai, -1)
+    U = np.triu(Ai, 1)
+    psi1 = np.linalg.solve(D + L, b - C @ psi0 - U @ psi0)
+    ref = psi1 if not sym else np.linalg.solve(D + U, b - C @ psi0 - L @ psi1)
+    got = m.smooth("symGaussSeidel" if sym else "GaussSeidel", psi0, b, 1)
+    np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-12)
+
+
+def test_gamg_with_interfaces_is_refused():
+    n, a, diag, up, lo, ifc, A, AT = _cyclic_system("sym")
+    m = O.Matrix(a, diag, up, lo, interfaces=ifc)
+    with pytest.raises(RuntimeError):
+        m.solve({"solver": "GAMG", "smoother": "GaussSeidel", "tolerance": 1e-8},
+                np.zeros(n), np.ones(n))
