@@ -280,6 +280,18 @@ int fh_face_h_device(fh_matrix* m, int32_t nCmpt, const double* d_psi, double* d
 /* lduMatrixSolver.C:174-197 */
 int fh_norm_factor(fh_matrix* m, const double* psi, const double* source,
                    double* normFactor);
+/* No counterpart in the reference: fh_norm_factor for callers that cannot
+   allocate device memory (code under HIP graph capture, tight serving
+   loops, harnesses that forbid hipMalloc).  psi and source are device
+   arrays in the caller's cell order, normFactor one device double; the
+   result is written on the matrix's stream (synchronize before reading it)
+   and equals fh_norm_factor's bits.  It calls no hipMalloc, hipFree or
+   allocating routine: its reduction state lives in a static device array
+   of the library (about 1 MiB per device, held until the process ends),
+   so meshes are limited to 2^24 cells (FH_ERR_INVALID above) and calls that
+   overlap on one device race. */
+int fh_norm_factor_device_noalloc(fh_matrix* m, const double* d_psi, const double* d_source,
+                                  double* d_normFactor);
 /* DICPreconditioner.C:57-84 / DILUPreconditioner.C:57-85 /
    diagonalPreconditioner.C:50-63: the reciprocal factor rD */
 int fh_reciprocal_d(fh_matrix* m, int32_t preconditioner, double* rD);

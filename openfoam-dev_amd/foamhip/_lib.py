@@ -99,6 +99,7 @@ FH_SYMBOLS = {
     "fh_face_geometry": (_i32, [_i32, _vp, _vp, _vp, _vp, _vp, _vp]),
     "fh_cell_geometry": (_i32, [_vp] * 8),
     "fh_surface_interpolation": (_i32, [_vp] * 14),
+    "fh_norm_factor_device_noalloc": (_i32, [_vp, _vp, _vp, _vp]),
     "fh_mesh_info_get": (_i32, [_vp, ctypes.POINTER(FhMeshInfo)]),
     "fh_mesh_destroy": (_i32, [_vp]),
     "fh_matrix_create": (_i32, [_vp, ctypes.POINTER(_vp)]),
