@@ -45,8 +45,8 @@ def parse():
     p.add_argument("--corr", type=float, default=8.0)
     p.add_argument("--tol", type=float, default=1e-8)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    p.add_argument("--serve-requests", type=int, default=256,
-                   help="requests of the serving sample (0 skips it)")
+    p.add_argument("--serve-requests", type=int, default=160,
+                   help="requests per arrival rate of the serving sample (0 skips it)")
     p.add_argument("--profile-kernels", action="store_true",
                    help="only launch the profiled kernels (and a plain copy of known size "
                         "for counter calibration), for rocprofv3 counter runs")
@@ -56,6 +56,8 @@ def parse():
                    help="box size of the GAMG FP64 / mixed comparison (0: skip)")
     p.add_argument("--no-configs", dest="configs", action="store_false",
                    help="skip the 64^3 PCG+DIC and 128^3 PBiCGStab+DILU legs")
+    p.add_argument("--c4-n", type=int, default=200,
+                   help="box size of the graded, jittered GAMG leg (0: skip)")
     return p.parse_args()
 
 
@@ -264,17 +266,21 @@ def main():
     serve = None
     if a.serve_requests > 0:
         # continuous-batching sample of the serving configuration: Poisson
-        # arrivals at 0.8x the saturated rate (the full study is
-        # python -m foamhip.serve_bench)
+        # arrivals at 0.5x, 0.8x and 1.0x the saturated rate (the full study,
+        # 4096 requests per rate, is python -m foamhip.serve_bench)
         from foamhip import serve_bench
-        sb = serve_bench.benchmark(a.serve_requests, (0.8,), pool_size=24, log=log)
-        r = sb["rates"]["0.8"]
-        serve = {"requests": r["requests"], "arrival_rate_per_s": r["arrival_rate_per_s"],
-                 "solves_per_s": r["solves_per_s"], "p50_ms": r["p50_ms"], "p99_ms": r["p99_ms"],
+        sb = serve_bench.benchmark(a.serve_requests, (0.5, 0.8, 1.0), pool_size=24, log=log)
+        serve = {"requests_per_rate": a.serve_requests,
                  "saturated_solves_per_s": sb["saturated"]["solves_per_s"],
+                 "saturated_p50_ms": sb["saturated"]["p50_ms"],
+                 "saturated_p99_ms": sb["saturated"]["p99_ms"],
+                 "rates": {f: {k: sb["rates"][f][k] for k in
+                               ("arrival_rate_per_s", "solves_per_s", "p50_ms", "p99_ms")}
+                           for f in sb["rates"]},
                  "mean_cells": sb["mean_cells"], "max_concurrent": sb["max_concurrent"],
                  "workload": "n^3 boxes, n log-uniform in [16, 128], log-normal gamma sigma "
-                             "U[0.5, 1.5], mixed PCG to 1e-8, pool of 24 seeded systems"}
+                             "U[0.5, 1.5], mixed PCG to 1e-8; matrices from a pool of 24 "
+                             "seeded systems, a seeded right-hand side per request"}
 
     cpu = None
     if not a.no_cpu:
@@ -284,6 +290,8 @@ def main():
     if a.configs:
         legs["c1_pcg_dic_64"] = c1_leg(fh, torch, a)
         legs["c2_pbicgstab_dilu_128"] = c2_leg(fh, torch, a)
+    if a.c4_n > 0:
+        legs[f"c4_gamg_graded_{a.c4_n}"] = c4_leg(fh, torch, a)
 
     out = {
         "metric": "s per Poisson solve to 1e-8 at 256^3 (mixed vs FP64), HBM GB/s; serve p99",
@@ -670,6 +678,84 @@ def c2_leg(fh, torch, a):
                       "grad_ms": 1e3 * float(np.median(tg))}
         out["gpu_vs_cpu"] = out["cpu"]["value"] / t
     log(f"c2 128^3 PBiCGStab+DILU: {out}")
+    return out
+
+
+def c4_leg(fh, torch, a):
+    """GAMG on an n^3 box graded along z (expansion ratio 100) with 15% node
+    jitter: the geometry computed on the device from the points
+    (fh_face_geometry, fh_cell_geometry, fh_surface_interpolation),
+    fvm::laplacian with a log-normal diffusivity assembled on the device,
+    then GAMG (algebraicPair, mergeLevels 2, direct coarsest solve of ~1000
+    rows, symGaussSeidel) to 1e-8: the mixed solve timed whole (after a
+    first solve that builds the hierarchy); the FP64 solve, whose coarse
+    Gauss-Seidel sweeps are reference-ordered dependency chains, timed over
+    its first cycles and reported per cycle."""
+    from foamhip.cases import GradedBoxCase
+    n = a.c4_n
+    t0 = time.time()
+    c = GradedBoxCase(n, zRatio=100.0, jitter=0.15, seed=3)
+    log(f"c4 mesh {n}^3 generated in {time.time() - t0:.1f}s")
+    dev = lambda x, dt=None: torch.from_numpy(
+        np.ascontiguousarray(x if dt is None else x.astype(dt))).to("cuda")
+    P = lambda t: ctypes.c_void_p(t.data_ptr())
+    mesh = fh.lduAddressing(c.N, c.lower, c.upper)
+    bnd = fh.Boundary(mesh, c.bFaceCells)
+    F, B = c.F, c.nB
+    z = lambda m: torch.empty(m, dtype=torch.float64, device="cuda")
+    fs, fp, pts = dev(c.faceStart, np.int32), dev(c.facePoints, np.int32), dev(c.points)
+    Cf, Sf, magSf = z(3 * (F + B)), z(3 * (F + B)), z(F + B)
+    C, V = z(3 * c.N), z(c.N)
+    w, dc, ndc, bdc = z(F), z(F), z(F), z(B)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fh._check(fh.lib().fh_face_geometry(F + B, P(fs), P(fp), P(pts), P(Cf), P(Sf), P(magSf)))
+    iCf, iSf, iMag = Cf[:3 * F], Sf[:3 * F], magSf[:F]
+    bCf, bSf, bMag = Cf[3 * F:], Sf[3 * F:], magSf[F:]
+    fh._check(fh.lib().fh_cell_geometry(mesh.handle, bnd.handle, P(iCf), P(iSf), P(bCf),
+                                        P(bSf), P(C), P(V)))
+    fh._check(fh.lib().fh_surface_interpolation(mesh.handle, bnd.handle, P(C), P(iCf), P(iSf),
+                                                P(iMag), P(bCf), P(bSf), P(bMag), P(w), P(dc),
+                                                P(ndc), P(bdc), None))
+    torch.cuda.synchronize()
+    geometry_ms = 1e3 * (time.perf_counter() - t0)
+    mat = fh.lduMatrix(mesh)
+    gamma = dev(c.lognormal(9, 1.0, 8.0))
+    src = dev(c.rhs(2))
+    ic, bc = z(B), z(B)
+    bType, bValue = dev(c.bType, np.int32), dev(c.bValue)
+    fh._check(fh.lib().fh_assemble_laplacian(mat.handle, bnd.handle, P(gamma), P(w), P(iMag),
+                                             P(ndc), P(bMag), P(bdc), P(bType), P(bValue), P(ic),
+                                             P(bc), P(src), 1))
+    torch.cuda.synchronize()
+    ctl = {"solver": "GAMG", "smoother": "symGaussSeidel", "agglomerator": "algebraicPair",
+           "mergeLevels": 2, "nCellsInCoarsestLevel": 1000, "directSolveCoarsest": True,
+           "tolerance": 1e-8, "relTol": 0, "maxIter": 1000}
+    psi = torch.zeros(c.N, dtype=torch.float64, device="cuda")
+    out = {"workload": f"{n}^3 box, z-grading 100, 15% jitter, log-normal gamma; GAMG "
+                       "algebraicPair mergeLevels 2, direct coarsest (~1000 rows), "
+                       "symGaussSeidel, to 1e-8",
+           "cells": c.N, "geometry_ms": geometry_ms}
+    mixed = fh.solver_New("p", mat, dict(ctl, precision="mixed"))
+    for rep in range(2):
+        psi.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pf = mixed.solve_device(psi.data_ptr(), src.data_ptr())
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+    out["mixed"] = {"s_per_solve": t, "cycles": pf.nIterations, "final_residual": pf.finalResidual}
+    log(f"c4 {n}^3 mixed GAMG: {pf.line()}, {t:.3f} s/solve")
+    fp64 = fh.solver_New("p", mat, dict(ctl, precision="double", maxIter=3, tolerance=0))
+    fp64.solve_device(psi.data_ptr(), src.data_ptr())     # hierarchy and factors
+    psi.zero_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pf = fp64.solve_device(psi.data_ptr(), src.data_ptr())
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    out["fp64"] = {"s_per_cycle": t / max(pf.nIterations, 1), "cycles_timed": pf.nIterations}
+    log(f"c4 {n}^3 FP64 GAMG: {pf.nIterations} cycles in {t:.3f} s")
     return out
 
 

@@ -2,6 +2,7 @@
 #include "addToRunTimeSelectionTable.H"
 #include "fvMesh.H"
 #include "surfaceFields.H"
+#include "cyclicLduInterface.H"
 
 #include <map>
 
@@ -103,6 +104,18 @@ namespace
 
 fh_mesh* Foam::hipLduSolver::deviceMesh(const lduAddressing& addr)
 {
+    return deviceMesh(addr, labelList(), labelList(), labelList());
+}
+
+
+fh_mesh* Foam::hipLduSolver::deviceMesh
+(
+    const lduAddressing& addr,
+    const labelList& interfaceStart,
+    const labelList& faceCells,
+    const labelList& nbrFaceCells
+)
+{
     // keyed by the addressing object; a hit is checked against the cell
     // and face counts and the contents of lowerAddr/upperAddr, and a stale
     // entry (topology change, or another addressing allocated at the same
@@ -110,12 +123,14 @@ fh_mesh* Foam::hipLduSolver::deviceMesh(const lduAddressing& addr)
     static std::map<const lduAddressing*, cachedMesh> meshes;
     const labelUList& l = addr.lowerAddr();
     const labelUList& u = addr.upperAddr();
-    const uint64_t h = contentHash
+    uint64_t h = contentHash
     (
         u.begin(),
         u.size()*sizeof(label),
         contentHash(l.begin(), l.size()*sizeof(label))
     );
+    h = contentHash(faceCells.cdata(), faceCells.size()*sizeof(label), h);
+    h = contentHash(nbrFaceCells.cdata(), nbrFaceCells.size()*sizeof(label), h);
     auto it = meshes.find(&addr);
     if (it != meshes.end())
     {
@@ -134,7 +149,22 @@ fh_mesh* Foam::hipLduSolver::deviceMesh(const lduAddressing& addr)
         meshes.erase(it);
     }
     fh_mesh* m = nullptr;
-    check(fh_mesh_create(addr.size(), l.size(), l.begin(), u.begin(), &m), "fh_mesh_create");
+    check
+    (
+        fh_mesh_create_coupled
+        (
+            addr.size(),
+            l.size(),
+            l.begin(),
+            u.begin(),
+            interfaceStart.size() ? interfaceStart.size() - 1 : 0,
+            interfaceStart.cdata(),
+            faceCells.cdata(),
+            nbrFaceCells.cdata(),
+            &m
+        ),
+        "fh_mesh_create_coupled"
+    );
     meshes[&addr] = cachedMesh{addr.size(), l.size(), h, m};
     return m;
 }
@@ -268,24 +298,47 @@ Foam::solverPerformance Foam::hipLduSolver::solve
     const direction
 ) const
 {
-    // coupled interfaces (processor/cyclic patches) are outside this
-    // single-GPU path
+    // coupled interfaces: cyclic patches without a transformation of the
+    // field (cyclicFvPatchField::updateInterfaceMatrix with pnf from the
+    // neighbour patch's faceCells, cyclicFvPatchField.C:150-174) go to the
+    // device; processor patches (another rank's cells) and transforming
+    // cyclics are outside this single-GPU path
+    labelList ifStart(1, 0), ifCells, ifNbr;
+    DynamicList<scalar> ifBou, ifInt;
     forAll(interfaces_, patchi)
     {
-        if (interfaces_.set(patchi))
+        if (!interfaces_.set(patchi)) continue;
+        const lduInterface& intf = interfaces_[patchi].interface();
+        if
+        (
+            !isA<cyclicLduInterface>(intf)
+         || refCast<const cyclicLduInterface>(intf).transform().transforms()
+        )
         {
             FatalErrorInFunction
-                << type() << " does not handle coupled interfaces (patch "
-                << patchi << "); use the host solver for this field"
+                << type() << " handles non-transforming cyclic interfaces only (patch "
+                << patchi << " is " << intf.type() << "); use the host solver for this field"
                 << exit(FatalError);
         }
+        const label nbr = refCast<const cyclicLduInterface>(intf).nbrPatchIndex();
+        const labelUList& fc = intf.faceCells();
+        const labelUList& nfc = interfaces_[nbr].interface().faceCells();
+        forAll(fc, i)
+        {
+            ifCells.append(fc[i]);
+            ifNbr.append(nfc[i]);
+            ifBou.append(interfaceBouCoeffs_[patchi][i]);
+            ifInt.append(interfaceIntCoeffs_[patchi][i]);
+        }
+        ifStart.append(ifCells.size());
     }
+    if (ifStart.size() == 1) ifStart.clear();
 
     fh_controls c, gamgPrecond;
     readControls(controlDict_, c, gamgPrecond);
     c.solver = solverId_;
 
-    fh_mesh* mesh = deviceMesh(matrix_.lduAddr());
+    fh_mesh* mesh = deviceMesh(matrix_.lduAddr(), ifStart, ifCells, ifNbr);
     fh_matrix* m = deviceMatrix(mesh);
 
     // faceAreaPair weights from fvMesh::Sf() and magSf()
@@ -329,6 +382,14 @@ Foam::solverPerformance Foam::hipLduSolver::solve
     const scalar* lower =
         matrix_.asymmetric() ? matrix_.lower().begin() : nullptr;
     check(fh_matrix_set(m, matrix_.diag().begin(), upper, lower), "fh_matrix_set");
+    if (ifBou.size())
+    {
+        check
+        (
+            fh_matrix_set_interfaces(m, ifBou.cdata(), ifInt.cdata()),
+            "fh_matrix_set_interfaces"
+        );
+    }
 
     fh_perf perf;
     check(fh_solve(m, &c, psi.begin(), source.begin(), &perf), "fh_solve");

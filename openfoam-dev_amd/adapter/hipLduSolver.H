@@ -63,6 +63,16 @@ public:
 
     //- Device mesh of an lduAddressing, built on first use and cached
     static fh_mesh* deviceMesh(const lduAddressing& addr);
+    //- The device mesh with the matrix's coupled (cyclic) interfaces:
+    //  interfaceStart, faceCells and nbrFaceCells concatenated in interface
+    //  order (empty: no interfaces)
+    static fh_mesh* deviceMesh
+    (
+        const lduAddressing& addr,
+        const labelList& interfaceStart,
+        const labelList& faceCells,
+        const labelList& nbrFaceCells
+    );
 
     //- Device matrix of a mesh (coefficients replaced on every solve)
     static fh_matrix* deviceMatrix(fh_mesh* mesh);

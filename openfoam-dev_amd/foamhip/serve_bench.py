@@ -4,11 +4,12 @@ Requests: hex boxes n^3 with n = round(exp(U[ln 16, ln 128])), each with a
 seeded log-normal diffusivity (sigma ~ U[0.5, 1.5], correlation length 8
 cells) and a seeded right-hand side, fvm::laplacian with the x-min wall
 fixed; mixed precision PCG (FP32 inner, FP64 residual) to 1e-8.  The
-request stream draws its systems from a pool of distinct seeded systems
-(``pool`` of them, reused round robin) so that generation stays off the
-clock.  First the saturated throughput is measured (all requests queued at
-once), then Poisson arrivals at fractions of it; latency is completion
-minus submission on the server clock.
+matrices come from a pool of distinct seeded systems (``pool`` of them,
+reused round robin); every request carries its own seeded right-hand side
+(generated before the clock starts, so generation stays off it).  First
+the saturated throughput is measured (all requests queued at once), then
+Poisson arrivals at fractions of it; latency is completion minus
+submission on the server clock.
 
     python -m foamhip.serve_bench --requests 4096 --rates 0.5,0.8,1.0
 """
@@ -43,11 +44,22 @@ def build_pool(n_systems, seed=2024, nmin=16, nmax=128):
         gamma = c.lognormal(1000 + i, sigma, 8.0)
         diag, upper, source = c.laplacian_system(gamma, c.rhs(5000 + i))
         pool.append({"n": n, "sigma": sigma, "mesh": mesh, "diag": diag, "upper": upper,
-                     "source": source})
+                     "source": source, "case": c})
     return pool
 
 
-def run_stream(srv, pool, n_requests, rate, rng):
+def request_sources(pool, n_requests, seed):
+    """Per request k: the right-hand side U[-1,1] minus its mean, seeded by
+    (seed, k), on the mesh of pool system k mod len(pool) (the x-min wall's
+    value is 0, so the boundary adds nothing to the source)."""
+    out = []
+    for k in range(n_requests):
+        s = pool[k % len(pool)]
+        out.append(s["case"].rhs(seed * 1000003 + k))
+    return out
+
+
+def run_stream(srv, pool, n_requests, rate, rng, sources=None):
     """Submit n_requests (Poisson arrivals at `rate`/s, or all at once when
     rate is None) and collect every result; returns the result records."""
     results = []
@@ -82,7 +94,8 @@ def run_stream(srv, pool, n_requests, rate, rng):
                     break
                 time.sleep(min(dt, 0.01))
         s = pool[k % len(pool)]
-        srv.submit(k, s["mesh"], s["diag"], s["upper"], s["source"], CONTROLS)
+        src = s["source"] if sources is None else sources[k]
+        srv.submit(k, s["mesh"], s["diag"], s["upper"], src, CONTROLS)
     th.join()
     return results, t0
 
@@ -106,13 +119,15 @@ def benchmark(n_requests=4096, rates=(0.5, 0.8, 1.0), pool_size=48, max_concurre
     rng = np.random.default_rng(seed)
     # warm up: every pooled system once (resident matrices and kernels)
     res, t0 = run_stream(srv, pool, len(pool), None, rng)
-    sat_res, t0 = run_stream(srv, pool, n_requests, None, rng)
+    sat_res, t0 = run_stream(srv, pool, n_requests, None, rng,
+                             request_sources(pool, n_requests, 1))
     sat = summarize(sat_res, t0)
     log(f"saturated: {sat}")
     out = {"saturated": sat, "rates": {}}
-    for f in rates:
+    for i, f in enumerate(rates):
         lam = f * sat["solves_per_s"]
-        r, t0 = run_stream(srv, pool, n_requests, lam, rng)
+        r, t0 = run_stream(srv, pool, n_requests, lam, rng,
+                           request_sources(pool, n_requests, 2 + i))
         s = summarize(r, t0)
         s["arrival_rate_per_s"] = lam
         out["rates"][str(f)] = s

@@ -101,33 +101,38 @@ void or_addr_free(or_addr* a)
 /* 334-357 (serial left-to-right loops; reduce() is a no-op serially)  */
 /* ------------------------------------------------------------------ */
 
-double or_sum(int32_t n, const double* f)
-{
-    double s = 0.0;
-    for (int32_t i = 0; i < n; i++) s += f[i];
-    return s;
-}
+/* Summation order of the reductions: 0 (the reference's serial loop), or 1
+   (sums of consecutive blocks of 256 values, then the block sums in order:
+   the same algorithm with another order of additions, as a measure of how
+   far the reference's results move with the order of the global sums --
+   test infrastructure for setting parity bars, never the reference). */
+static int g_sum_order = 0;
 
-double or_sumMag(int32_t n, const double* f)
-{
-    double s = 0.0;
-    for (int32_t i = 0; i < n; i++) s += fabs(f[i]);
-    return s;
-}
+void or_set_sum_order(int order) { g_sum_order = order; }
 
-double or_sumSqr(int32_t n, const double* f)
-{
-    double s = 0.0;
-    for (int32_t i = 0; i < n; i++) s += f[i]*f[i];
+#define OR_SUM_LOOP(EXPR)                                               \
+    double s = 0.0;                                                     \
+    if (g_sum_order == 0)                                               \
+    {                                                                   \
+        for (int32_t i = 0; i < n; i++) s += (EXPR);                    \
+        return s;                                                       \
+    }                                                                   \
+    for (int32_t i0 = 0; i0 < n; i0 += 256)                             \
+    {                                                                   \
+        double b = 0.0;                                                 \
+        const int32_t i1 = i0 + 256 < n ? i0 + 256 : n;                 \
+        for (int32_t i = i0; i < i1; i++) b += (EXPR);                  \
+        s += b;                                                         \
+    }                                                                   \
     return s;
-}
 
-double or_sumProd(int32_t n, const double* a, const double* b)
-{
-    double s = 0.0;
-    for (int32_t i = 0; i < n; i++) s += a[i]*b[i];
-    return s;
-}
+double or_sum(int32_t n, const double* f) { OR_SUM_LOOP(f[i]) }
+
+double or_sumMag(int32_t n, const double* f) { OR_SUM_LOOP(fabs(f[i])) }
+
+double or_sumSqr(int32_t n, const double* f) { OR_SUM_LOOP(f[i]*f[i]) }
+
+double or_sumProd(int32_t n, const double* a, const double* b_) { OR_SUM_LOOP(a[i]*b_[i]) }
 
 double or_average(int32_t n, const double* f)
 {

@@ -74,6 +74,9 @@ double or_sumMag(int32_t n, const double* f);
 double or_sumSqr(int32_t n, const double* f);
 double or_sumProd(int32_t n, const double* a, const double* b);
 double or_average(int32_t n, const double* f);
+/* 0: serial (the reference), 1: blocks of 256 then the block sums (a
+   different order of the same additions, for measuring sensitivity) */
+void or_set_sum_order(int order);
 
 /* ---- matrix kernels (lduMatrixATmul.C, lduMatrixOperations.C) ---- */
 void or_amul(const or_matrix* m, const double* psi, double* Apsi);

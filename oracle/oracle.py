@@ -83,6 +83,7 @@ def lib():
                                        _vp, _vp]
         L.or_surface_interpolation.argtypes = [_i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                                _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp]
+        L.or_set_sum_order.argtypes = [ctypes.c_int]
         M = ctypes.POINTER(OrMatrix)
         L.or_amul.argtypes = [M, _vp, _vp]
         L.or_tmul.argtypes = [M, _vp, _vp]
@@ -519,3 +520,9 @@ def mesh_geometry(nCells, lower, upper, bFaceCells, faceStart, facePoints, point
                                    _p(g["bSf"]), _p(g["bMagSf"]), _p(g["bDeltaCoeffs"]),
                                    _p(g["bNonOrthDeltaCoeffs"]))
     return g
+
+
+def set_sum_order(order):
+    """0: the reference's serial reductions; 1: blocks of 256 summed, then
+    the block sums (a different order of the same additions)."""
+    lib().or_set_sum_order(int(order))

@@ -7,14 +7,17 @@ the oracle's minutes per solve at these sizes:
   c2_pbicgstab_dilu_128.json  PBiCGStab+DILU, 128^3 upwind convection-
                               diffusion (D 1e-3), tolerance 1e-8, and the
                               same solve with b perturbed by 1e-15 relative
-                              (the reference's own spread)
+                              and with its global sums in another order (the
+                              reference's own spread)
   c3_pcg_dic_256.json         PCG+DIC, 256^3 log-normal Poisson, tolerance
                               1e-8 and 1e-12 (the tight solution the error
-                              statements are measured against), and the 1e-8
-                              solve with b perturbed by 1e-15 relative
+                              statements are measured against), the 1e-8
+                              solve with b perturbed by 1e-15 relative, and
+                              with its global sums in another order
 
 Run from the repository root after building the oracle:
-    python tests/golden/make_config_fixtures.py [c2] [c3] [c3-perturbed]
+    python tests/golden/make_config_fixtures.py [c2] [c3] [c3-perturbed] [c2-reordered]
+        [c3-reordered]
 """
 import json
 import os
@@ -67,6 +70,41 @@ def perturbed(om, ctl, N, b, ref_summary):
     return out
 
 
+def reordered(om, ctl, N, b, ref_summary):
+    """The reference's sensitivity to the order of its global sums: the same
+    solve with the reductions summed in blocks of 256 (oracle.set_sum_order),
+    the only arithmetic in which the device solve differs from the oracle."""
+    O.set_sum_order(1)
+    try:
+        psi_r, perf_r = om.solve(ctl, np.zeros(N), b)
+    finally:
+        O.set_sum_order(0)
+    out = {"perf": perf_dict(perf_r), "rel_l2_vs_solve": CC.rel_diff_estimate(psi_r, ref_summary)}
+    print("  reordered sums:", out, flush=True)
+    return out
+
+
+def add_reordered(name, build):
+    """Adds the reordered-sum spread to an existing fixture."""
+    path = os.path.join(HERE, name)
+    with open(path) as f:
+        out = json.load(f)
+    om, ctl, N, b = build()
+    out["reordered"] = reordered(om, ctl, N, b, out["solve"]["psi"])
+    with open(path, "w") as f:
+        json.dump(out, f)
+
+
+def c2_system():
+    c, a, diag, upper, lower, b = CC.c2_convection_diffusion()
+    return O.Matrix(a, diag, upper, lower), CC.C2_CONTROLS, c.N, b
+
+
+def c3_system():
+    c, a, diag, upper, b = CC.c3_poisson_lognormal()
+    return O.Matrix(a, diag, upper), CC.C3_CONTROLS, c.N, b
+
+
 def c3(only_perturbed=False):
     c, a, diag, upper, b = CC.c3_poisson_lognormal()
     om = O.Matrix(a, diag, upper)
@@ -87,4 +125,6 @@ if __name__ == "__main__":
     which = sys.argv[1:] or ["c2", "c3"]
     for w in which:
         print(w, flush=True)
-        {"c2": c2, "c3": c3, "c3-perturbed": lambda: c3(only_perturbed=True)}[w]()
+        {"c2": c2, "c3": c3, "c3-perturbed": lambda: c3(only_perturbed=True),
+         "c2-reordered": lambda: add_reordered("c2_pbicgstab_dilu_128.json", c2_system),
+         "c3-reordered": lambda: add_reordered("c3_pcg_dic_256.json", c3_system)}[w]()

@@ -141,3 +141,27 @@ def cyclic_system(kind, seed=3, shape=(6, 5, 4), with_dense=True):
     A = dense(n, lower, upper, diag, up, lo) + dense_interfaces(n, ifc, bou)
     AT = dense(n, lower, upper, diag, up, lo).T + dense_interfaces(n, ifc, itc)
     return n, lower, upper, diag, up, lo, ifc, A, AT
+
+
+def renumber_cells(N, lower, upper, faceStart, facePoints, q):
+    """The same mesh with cell c renumbered q[c]: internal faces re-oriented
+    so that the lower new index owns them (a flipped face has its points
+    reversed, so its normal still points from owner to neighbour) and
+    re-sorted into upper-triangular order.  Returns the new lower, upper,
+    faceStart, facePoints and, per new internal face, its old index."""
+    q = np.asarray(q)
+    l, u = q[lower], q[upper]
+    flip = l > u
+    nl, nu = np.where(flip, u, l), np.where(flip, l, u)
+    order = np.lexsort((nu, nl))
+    F = len(lower)
+    pts = [facePoints[faceStart[f]:faceStart[f + 1]] for f in range(len(faceStart) - 1)]
+    new_pts = []
+    for f in order:
+        p = pts[f]
+        new_pts.append(p[::-1] if flip[f] else p)
+    new_pts += pts[F:]
+    fs = np.zeros(len(new_pts) + 1, np.int32)
+    fs[1:] = np.cumsum([len(p) for p in new_pts])
+    return (nl[order].astype(np.int32), nu[order].astype(np.int32), fs,
+            np.concatenate(new_pts).astype(np.int32), order)

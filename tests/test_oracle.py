@@ -993,3 +993,59 @@ def test_gamg_with_interfaces_is_refused():
     with pytest.raises(RuntimeError):
         m.solve({"solver": "GAMG", "smoother": "GaussSeidel", "tolerance": 1e-8},
                 np.zeros(n), np.ones(n))
+
+
+# ---------------------------------------------------------------------------
+# mesh geometry: face::areaAndCentre, primitiveMesh::makeCellCentresAndVols,
+# surfaceInterpolation::makeWeights / makeDeltaCoeffs / makeNonOrthDeltaCoeffs
+# ---------------------------------------------------------------------------
+
+def _geometry(c):
+    return O.mesh_geometry(c.N, c.lower, c.upper, c.bFaceCells, c.faceStart, c.facePoints,
+                           c.points)
+
+
+def test_geometry_uniform_box_closed_form():
+    """Without grading or jitter the geometry is the uniform box's closed
+    form (BoxCase): face areas and centres, outward boundary normals, cell
+    centres and volumes, weights 1/2, delta coefficients 1/h (2/h at the
+    boundary)."""
+    from foamhip.cases import BoxCase, GradedBoxCase
+    c = GradedBoxCase(5, 4, 3, L=(1.0, 2.0, 0.5), zRatio=1.0, jitter=0.0)
+    g = _geometry(c)
+    b = BoxCase(5, 4, 3, L=(1.0, 2.0, 0.5))
+    for k in ("weights", "magSf", "Sf", "deltaCoeffs", "V", "C", "Cf", "bMagSf", "bSf",
+              "bDeltaCoeffs", "bCf"):
+        np.testing.assert_allclose(g[k], getattr(b, k), rtol=1e-13, atol=1e-14, err_msg=k)
+    np.testing.assert_allclose(g["nonOrthDeltaCoeffs"], g["deltaCoeffs"], rtol=1e-13)
+
+
+def test_geometry_invariants_graded_jittered():
+    """On a z-graded (ratio 100), 15%-jittered box: the cell volumes add to
+    the box volume; each cell's outward face-area vectors add to zero (the
+    vector area of a polygon is half the sum of its edge cross products, so
+    shared faces cancel); each volume is a third of the sum of its outward
+    Sf . Cf; the weights lie in (0, 1); the grading of the unjittered mesh
+    is the expansion ratio."""
+    from foamhip.cases import GradedBoxCase
+    c = GradedBoxCase(6, 5, 8, L=(1.0, 1.0, 2.0), zRatio=100.0, jitter=0.15, seed=4)
+    g = _geometry(c)
+    assert g["V"].sum() == pytest.approx(2.0, rel=1e-13)
+    assert np.all(g["V"] > 0)
+    Sf, Cf = g["Sf"].reshape(-1, 3), g["Cf"].reshape(-1, 3)
+    bSf, bCf = g["bSf"].reshape(-1, 3), g["bCf"].reshape(-1, 3)
+    tot = np.zeros((c.N, 3))
+    np.add.at(tot, c.lower, Sf)
+    np.add.at(tot, c.upper, -Sf)
+    np.add.at(tot, c.bFaceCells, bSf)
+    assert np.abs(tot).max() < 1e-15
+    sc = np.zeros(c.N)
+    np.add.at(sc, c.lower, (Sf * Cf).sum(1))
+    np.add.at(sc, c.upper, -(Sf * Cf).sum(1))
+    np.add.at(sc, c.bFaceCells, (bSf * bCf).sum(1))
+    np.testing.assert_allclose(g["V"], sc / 3.0, rtol=1e-10)
+    assert np.all((g["weights"] > 0) & (g["weights"] < 1))
+    u = GradedBoxCase(2, 2, 8, zRatio=100.0, jitter=0.0)
+    z = u.points.reshape(-1, 3)[:, 2][::9]       # the corner line i = j = 0
+    dz = np.diff(z)
+    assert dz[-1] / dz[0] == pytest.approx(100.0, rel=1e-12)
