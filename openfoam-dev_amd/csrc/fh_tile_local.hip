@@ -759,6 +759,266 @@ void k_tile_dic_q(TileView G, const float* x, const float* __restrict__ rD, TlCo
     }
 }
 
+// The symmetric tile-local DIC with the whole 16 x 16 x 16 tile staged in
+// LDS (r, rD and the FP16 coefficients, 59 KB): one wavefront per tile
+// stages it with whole-row vector loads, runs both triangular passes out of
+// LDS and writes w once, so every input is read from HBM once and w written
+// once.
+//
+// Lane (j, q) = (lane & 15, lane >> 4) owns quarter q (cells 4q .. 4q+3) of
+// the 16 rows (j, k), k = 0 .. 15.  Forward step t takes k = t - j - q: the
+// (j, k-1) neighbour is the lane's own result of step t-1, the (j-1, k) one
+// that of lane j-1 at step t-1 (DPP row shift, same 16-lane row) and the i-1
+// neighbour, the last cell of quarter q-1, that of lane j + 16(q-1) (a lane
+// shuffle).  The backward pass mirrors it (k = 33 - t - j - q).  A lane's own
+// r, rD, coefficients and forward results are read from and written to LDS
+// at its own addresses; only the read-only coefficients of the upper
+// neighbours are another lane's, so the passes need no barrier.  Planes are
+// 268 cells apart in LDS: the 16 lanes of a step then fall on distinct banks.
+constexpr int TL_PLANE = 268;
+constexpr int TL_LDS_CELLS = 15*TL_PLANE + 256 + 20;     // a multiple of 8
+// FP16 coefficients: pairs of planes (one direct-to-LDS load each) 520 cells
+// apart, the layout of the fewest bank conflicts among those the loads allow
+constexpr int TL_HPAIR = 520;
+constexpr int TL_HCELLS = 8*TL_HPAIR;
+static_assert(TL_LDS_CELLS % 8 == 0 && TL_PLANE % 4 == 0 && TL_HPAIR % 8 == 0,
+              "LDS tile layout alignment");
+
+__device__ inline int tl_lds(int l)   // tile cell l = i + 16 (j + 16 k) -> LDS cell (FP32)
+{
+    return (l & 255) + TL_PLANE*(l >> 8);
+}
+
+__device__ inline int tl_hlds(int j, int k, int q)   // quarter q of row (j, k) -> FP16 LDS cell
+{
+    return 4*q + 16*j + 256*(k & 1) + TL_HPAIR*(k >> 1);
+}
+
+// 16 bytes per lane from global memory straight into LDS: lane L writes
+// bytes [16 L, 16 L + 16) after the wave-uniform LDS address `l`
+__device__ inline void tl_dma16(const void* g, void* l)
+{
+    __builtin_amdgcn_global_load_lds(
+        (const __attribute__((address_space(1))) void*)g,
+        (__attribute__((address_space(3))) void*)l, 16, 0, 0);
+}
+
+__device__ inline float4 tl_half4(const __half* a, int c)   // cells c .. c+3, c % 4 == 0
+{
+    const uint2 v = *reinterpret_cast<const uint2*>(a + c);
+    const __half2 lo = *reinterpret_cast<const __half2*>(&v.x);
+    const __half2 hi = *reinterpret_cast<const __half2*>(&v.y);
+    return make_float4(__low2float(lo), __high2float(lo), __low2float(hi), __high2float(hi));
+}
+
+__device__ inline float tl_dpp_from_lower(float v)    // lane j-1 of the 16-lane row; 0 at j = 0
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xf, 0xf,
+                                                      false));
+}
+
+__device__ inline float tl_dpp_from_upper(float v)    // lane j+1 of the 16-lane row; 0 at j = 15
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x101, 0xf, 0xf,
+                                                      false));
+}
+
+// the value of the previous (dn) and of the next (up) 16-lane row's lane in
+// the same position, 0 in row 0 (dn) and row 3 (up): VALU lane swaps, no
+// LDS round trip (permlane16_swap exchanges rows 0<->1 and 2<->3 between
+// its operands, permlane32_swap the two 32-lane halves)
+__device__ inline float2 tl_rows_dn_up(float v)
+{
+    const auto a = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v),
+                                                    false, false);
+    const auto b = __builtin_amdgcn_permlane32_swap(a[1], a[1], false, false);
+    const auto c = __builtin_amdgcn_permlane32_swap(a[0], a[0], false, false);
+    const int row = threadIdx.x >> 4;
+    const float dn = row == 0 ? 0.f : __uint_as_float(row == 2 ? b[0] : a[0]);
+    const float up = row == 3 ? 0.f : __uint_as_float(row == 1 ? c[1] : a[1]);
+    return make_float2(dn, up);
+}
+
+template<bool DOT>
+__global__ __launch_bounds__(64)
+void k_tile_dic_lds(TileView G, const float* x, const float* __restrict__ rD,
+                    TlCoef<__half> C, float scale, float* w, const DevState* st,
+                    double* partials, FusedFinalize F, double* tileSum)
+{
+    if (st && st->done) return;
+    extern __shared__ __align__(16) unsigned char lds[];
+    float* X = reinterpret_cast<float*>(lds);
+    float* Dv = X + TL_LDS_CELLS;
+    __half* cK = reinterpret_cast<__half*>(Dv + TL_LDS_CELLS);
+    __half* cJ = cK + TL_HCELLS;
+    __half* cI = cJ + TL_HCELLS;
+    const int lane = threadIdx.x;
+    // XCD-aware order: workgroups go round robin over the 8 XCDs, so XCD x
+    // is given the x-th eighth of the tiles in order -- tiles adjacent along
+    // i, whose rows share cache lines, then run together on one L2
+    const int nT = G.nTI*G.nTJ*G.nTK;
+    const int tile = (nT & 7) == 0 ? (int)(blockIdx.x & 7)*(nT >> 3) + (int)(blockIdx.x >> 3)
+                                   : (int)blockIdx.x;
+    const int TI = tile % G.nTI, TJ = (tile/G.nTI) % G.nTJ, TK = tile/(G.nTI*G.nTJ);
+    const int ey = min(16, G.ny - TJ*16), ez = min(16, G.nz - TK*16);
+    const int64_t sy = G.nx, sz = (int64_t)G.nx*G.ny;
+    const int64_t base = TI*16 + sy*(TJ*16) + sz*(TK*16);
+    // ---- stage with direct-to-LDS loads: one instruction per plane of r
+    // and rD (lane L: row L/4, quarter L%4), one per pair of planes of each
+    // FP16 coefficient array (lane L: plane L/32, row (L%32)/2, half-row
+    // L%2).  Rows and planes outside the mesh load the nearest ones inside
+    // it, so every slot holds finite values: the slots outside the mesh are
+    // only ever multiplied by a zero neighbour value.
+    {
+        const int jr = min(lane >> 2, ey - 1);
+        const int64_t gr = base + 4*(lane & 3) + sy*jr;
+#pragma unroll
+        for (int k = 0; k < 16; k++)
+        {
+            const int64_t g = gr + sz*min(k, ez - 1);
+            tl_dma16(x + g, X + TL_PLANE*k);
+            tl_dma16(rD + g, Dv + TL_PLANE*k);
+        }
+        const int jh = min((lane & 31) >> 1, ey - 1);
+        const int64_t gh = base + 8*(lane & 1) + sy*jh;
+#pragma unroll
+        for (int p = 0; p < 8; p++)
+        {
+            const int64_t g = gh + sz*min(2*p + (lane >> 5), ez - 1);
+            tl_dma16(C.K + g, cK + TL_HPAIR*p);
+            tl_dma16(C.J + g, cJ + TL_HPAIR*p);
+            tl_dma16(C.I + g, cI + TL_HPAIR*p);
+        }
+        // vmcnt(0): the direct-to-LDS loads have landed
+        __builtin_amdgcn_s_waitcnt(0x0F70);
+    }
+    __syncthreads();
+    const int j = lane & 15, q = lane >> 4;
+    const bool rowIn = j < ey;
+    const float inv = C.inv;
+    const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    auto h4 = [](const __half* a, int h) { return tl_half4(a, h); };
+    double dotWR = 0.0, rsum = 0.0;
+    // ---- forward ----
+    // po, the lane's result of its previous step, is zero until its first
+    // row (k = 0): the (j, k-1) value needs no mask, the (j-1, k) one comes
+    // from lane j-1 (0 at j = 0) and the i-1 one from row q-1 (0 at q = 0)
+    {
+        float4 po = zero4;
+        for (int t = 0; t < 34; t++)
+        {
+            const int k = t - j - q;
+            const bool act = rowIn && k >= 0 && k < ez;
+            float4 nj;
+            nj.x = tl_dpp_from_lower(po.x);
+            nj.y = tl_dpp_from_lower(po.y);
+            nj.z = tl_dpp_from_lower(po.z);
+            nj.w = tl_dpp_from_lower(po.w);
+            float prev = tl_rows_dn_up(po.w).x;
+            if (act)
+            {
+                const int c0 = 4*q + 16*j + TL_PLANE*k, h0 = tl_hlds(j, k, q);
+                const float4 xr = *reinterpret_cast<const float4*>(X + c0);
+                const float4 dr = *reinterpret_cast<const float4*>(Dv + c0);
+                const float4 ck = h4(cK, h0), cj = h4(cJ, h0), ci = h4(cI, h0);
+                const float4 nk = po;
+                float dp = 0.f;
+                auto fwd = [&](float xv, float dv, float kv, float jv, float a, float b, float c) {
+                    float ts = xv*scale;
+                    ts = __builtin_fmaf(-a, kv, ts);
+                    ts = __builtin_fmaf(-b, jv, ts);
+                    ts = __builtin_fmaf(-c, prev, ts);
+                    const float v = (dv*inv)*ts;
+                    if constexpr (DOT) dp = __builtin_fmaf(v, ts, dp);
+                    prev = v;
+                    return v;
+                };
+                float4 o;
+                o.x = fwd(xr.x, dr.x, nk.x, nj.x, ck.x, cj.x, ci.x);
+                o.y = fwd(xr.y, dr.y, nk.y, nj.y, ck.y, cj.y, ci.y);
+                o.z = fwd(xr.z, dr.z, nk.z, nj.z, ck.z, cj.z, ci.z);
+                o.w = fwd(xr.w, dr.w, nk.w, nj.w, ck.w, cj.w, ci.w);
+                *reinterpret_cast<float4*>(X + c0) = o;
+                po = o;
+                if constexpr (DOT)
+                {
+                    dotWR += (double)dp;
+                    rsum += (double)((xr.x + xr.y) + (xr.z + xr.w));
+                }
+            }
+        }
+    }
+    // ---- backward (in place) ----
+    // po is zero until the lane's first row (k = ez-1); lane j+1 of a row
+    // outside the mesh (or j = 15) gives 0, as does row q+1 at q = 3
+    {
+        float4 po = zero4;
+        for (int t = 0; t < 34; t++)
+        {
+            const int k = 33 - t - j - q;
+            const bool act = rowIn && k >= 0 && k < ez;
+            float4 nj;
+            nj.x = tl_dpp_from_upper(po.x);
+            nj.y = tl_dpp_from_upper(po.y);
+            nj.z = tl_dpp_from_upper(po.z);
+            nj.w = tl_dpp_from_upper(po.w);
+            float next = tl_rows_dn_up(po.x).y;
+            if (act)
+            {
+                const int c0 = 4*q + 16*j + TL_PLANE*k, h0 = tl_hlds(j, k, q);
+                const float4 xr = *reinterpret_cast<const float4*>(X + c0);
+                const float4 dr = *reinterpret_cast<const float4*>(Dv + c0);
+                // upper coefficients: the lower ones of (j, k+1), (j+1, k) and
+                // of the next cell along i (slots past the tile's end hold
+                // finite values whose neighbour value is zero)
+                const float4 ck = h4(cK, tl_hlds(j, min(k + 1, 15), q));
+                const float4 cj = h4(cJ, h0 + (j < 15 ? 16 : 0));
+                const float4 ci = h4(cI, h0);
+                const float ci4 = __half2float(cI[h0 + (q < 3 ? 4 : 0)]);
+                const float4 nk = po;
+                auto bwd = [&](float xv, float dv, float kv, float jv, float a, float b, float c) {
+                    float acc = a*kv;
+                    acc = __builtin_fmaf(b, jv, acc);
+                    acc = __builtin_fmaf(c, next, acc);
+                    const float v = __builtin_fmaf(-(dv*inv), acc, xv);
+                    next = v;
+                    return v;
+                };
+                float4 o;
+                o.w = bwd(xr.w, dr.w, nk.w, nj.w, ck.w, cj.w, ci4);
+                o.z = bwd(xr.z, dr.z, nk.z, nj.z, ck.z, cj.z, ci.w);
+                o.y = bwd(xr.y, dr.y, nk.y, nj.y, ck.y, cj.y, ci.z);
+                o.x = bwd(xr.x, dr.x, nk.x, nj.x, ck.x, cj.x, ci.y);
+                *reinterpret_cast<float4*>(X + c0) = o;
+                po = o;
+            }
+        }
+    }
+    __syncthreads();
+    // ---- write w (whole rows) ----
+#pragma unroll 8
+    for (int u = lane; u < 1024; u += 64)
+    {
+        const int l = 4*u, jj = (l >> 4) & 15, kk = l >> 8;
+        if (jj < ey && kk < ez)
+            *reinterpret_cast<float4*>(w + base + (l & 15) + sy*jj + sz*kk) =
+                *reinterpret_cast<const float4*>(X + tl_lds(l));
+    }
+    if constexpr (DOT)
+    {
+        if (tileSum)
+        {
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) rsum += __shfl_xor(rsum, off, 64);
+            if (lane == 0) tileSum[tile] = rsum;
+        }
+        dotWR *= (double)inv;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) dotWR += __shfl_xor(dotWR, off, 64);
+        partials_arrive(dotWR, 0.0, partials, gridDim.x, blockIdx.x, F);
+    }
+}
+
 // launch with the w*r sum compiled in when partials are given
 template<int PASSES, int BZ, bool SYM, class CT>
 static void tlv_launch(dim3 grid, hipStream_t s, const TileView& G, const float* x,
@@ -785,14 +1045,41 @@ int32_t launch_tile_local_apply_f32(const TileView& G, const float* x, const flo
     const int nTiles = G.nTI*G.nTJ*G.nTK;
     const dim3 grid((nTiles + 3)/4);
     constexpr int BZ = TileBZ<float>::pre;
-    // the quarter-row pipeline is selectable (FOAMHIP_TILE_APPLY=quarter);
-    // its per-lane quarter-row accesses touch four times the cache lines of
-    // the row walk, which is the faster one on this device
-    static const bool quarter = [] {
+    // symmetric: the row walk (default; it reads the tile's inputs again in
+    // the backward pass), the tile staged in LDS (FOAMHIP_TILE_APPLY=lds:
+    // inputs read once, but two tiles per CU leave its latency-bound steps
+    // on one wavefront per SIMD) or the quarter-row pipeline from HBM
+    // (FOAMHIP_TILE_APPLY=quarter; its per-lane quarter rows touch four
+    // times the cache lines)
+    static const int mode = [] {
         const char* e = std::getenv("FOAMHIP_TILE_APPLY");
-        return e && std::strcmp(e, "quarter") == 0;
+        if (e && std::strcmp(e, "lds") == 0) return 0;
+        if (e && std::strcmp(e, "quarter") == 0) return 1;
+        return 2;
     }();
-    if (symmetric && BZ == 16 && quarter)
+    // the staged tile (FP16 coefficients): 59 KB of LDS per workgroup, taken
+    // when the device allows that much
+    const size_t ldsBytes = (size_t)TL_LDS_CELLS*(4 + 4) + (size_t)TL_HCELLS*3*sizeof(__half);
+    static const size_t ldsMax = [] {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+            return (size_t)0;
+        return (size_t)v;
+    }();
+    if (symmetric && half3 && BZ == 16 && mode == 0 && ldsBytes <= ldsMax)
+    {
+        const dim3 gq((unsigned)nTiles);
+        const TlCoef<__half> h{half3[0], half3[1], half3[2], 1.f/halfScale};
+        if (partials)
+            hipLaunchKernelGGL((k_tile_dic_lds<true>), gq, dim3(64), ldsBytes, s, G, x, rD, h,
+                               halfScale, w, st, partials, F, tileSum);
+        else
+            hipLaunchKernelGGL((k_tile_dic_lds<false>), gq, dim3(64), ldsBytes, s, G, x, rD, h,
+                               halfScale, w, st, partials, F, tileSum);
+        return (int32_t)gq.x;
+    }
+    if (symmetric && BZ == 16 && mode == 1)
     {
         // quarter-row pipeline: one tile per wavefront
         const dim3 gq((unsigned)nTiles);

@@ -205,13 +205,19 @@ def test_mixed_precision_vs_fp64_oracle(solver, precon):
     r = b - om.Amul(psi)
     assert np.abs(r).sum() / perf.normFactor < 1e-8
     e_mixed, e_fp64 = rel_l2(psi, exact), rel_l2(opsi, exact)
-    # the error at a given residual depends on the preconditioner (tileDIC
-    # inside); it is reported, and the accuracy the path can reach is
-    # checked at the tight tolerance below
     print(f"mixed {solver}: refinements {perf.nRefinements}, its {perf.nIterations}, "
           f"fp64 its {operf['nIterations']}, error {e_mixed:.3e} (fp64 oracle {e_fp64:.3e}), "
           f"vs fp64 oracle {rel_l2(psi, opsi):.3e}")
-    assert e_mixed <= 100 * e_fp64
+    # as accurate as the FP64 oracle at the same tolerance (the error at a
+    # given residual also depends on the preconditioner: a factor 3 allowed)
+    assert e_mixed <= 3 * e_fp64
+    # the accuracy bar: error <= 1e-8 against the FP64 oracle's converged
+    # solution, reached by the mixed solve at tolerance 1e-10
+    psi10 = np.zeros(c.N)
+    assert fh.solver_New("p", m, dict(ctl, tolerance=1e-10, precision="mixed")).solve(
+        psi10, b).converged
+    print(f"mixed {solver} at 1e-10: error {rel_l2(psi10, exact):.3e}")
+    assert rel_l2(psi10, exact) <= 1e-8
     psi2 = np.zeros(c.N)
     tight = dict(ctl, tolerance=1e-12)
     perf2 = fh.solver_New("p", m, dict(tight, precision="mixed")).solve(psi2, b)
