@@ -142,63 +142,86 @@ void launch_coarse_inv32(int32_t nT, int32_t ld, const double* dense, const doub
 //     Z_{j,k} = -Z_{j,k+1} C_{k+1} U_k          (j > k),
 //     Z_{k,k} = U_k - (C_{k+1} U_k)^T Z_{k+1,k},
 // of which only the blocks j >= k are formed.  The K inversions of P x P
-// SPD matrices run one after another, each in one workgroup (Gauss-Jordan
-// in registers, FP64); the recurrence is batches of FP64 block products.
+// SPD matrices run one after another; each is split 2 x 2 at 128 rows
+// (S = [A B; B^T D]: A^-1 and T^-1 for T = D - B^T A^-1 B by Gauss-Jordan
+// in the registers of one workgroup each, joined by block products), all in
+// FP64; the recurrence is batches of FP64 block products.
 // ---------------------------------------------------------------------------
 
-#define CO_GJ 1024          // Gauss-Jordan workgroup: P <= 256, 64 entries per thread
+// S_k = s*A_k - C_k U_{k-1} C_k (FP64), one entry per thread
+__global__ void k_coarse_schur_form(int32_t k, int32_t P, int32_t nTI,
+                                    const double* __restrict__ ctile,
+                                    const double* __restrict__ U, double* __restrict__ S)
+{
+    const int64_t e = blockIdx.x*(int64_t)blockDim.x + threadIdx.x;
+    if (e >= (int64_t)P*P) return;
+    const int i = (int)(e / P), j = (int)(e % P);
+    const double sign = coarse_sign(ctile);
+    const double* ct = ctile + 4*(int64_t)k*P;
+    const int lo = min(i, j), hi = max(i, j), d = hi - lo;
+    double v = 0.0;
+    if (d == 0) v = ct[4*i];
+    else if (d == 1 && hi % nTI > 0) v = ct[4*hi + 1];
+    else if (d == nTI) v = ct[4*hi + 2];
+    v *= sign;
+    if (k > 0)
+    {
+        const double* Up = U + (int64_t)(k - 1)*P*P;
+        v -= (sign*ct[4*i + 3])*Up[e]*(sign*ct[4*j + 3]);
+    }
+    S[e] = v;
+}
 
-// S_k = s*A_k - C_k U_{k-1} C_k (FP64, to S), and its inverse by
-// Gauss-Jordan without pivoting (S_k is positive definite) in FP32 in the
-// registers of one workgroup, to U0 as doubles; the caller refines it by a
-// Newton-Schulz step in FP64.  A non-positive pivot sets *bad (the caller
-// falls back to the one-level preconditioner).
+// Gauss-Jordan inverse of an n x n SPD block (n <= H, rows of ld doubles)
+// without pivoting, in FP64 in the registers of one workgroup of
+// (H/4)*(H/16) threads.  A non-positive pivot sets *bad (the caller falls
+// back to the one-level preconditioner).
 //
-// Thread (tr, tc) = (t / 16, t % 16) owns rows 4tr .. 4tr+3 and columns
-// 16tc .. 16tc+15, so the pivot row and column it reads (and, as their
-// owner, writes) are contiguous in LDS (16-byte accesses).  The pivot loop
-// is unrolled over p mod 16, so the owners of the pivot row (tr = p/4, row
-// p mod 4) and column (tc = p/16, column p mod 16) index their registers
-// statically: a step is the rank-one update of every entry (one FMA each),
-// then the owners overwrite the pivot row and column with the scaled values.
-template<int C16>
-__device__ __forceinline__ bool gj_step(float (&a)[4][16], int tr, int tc, int p, float* rowp,
-                                        float* colp, int32_t* bad)
+// Thread (tr, tc) = (t / (H/16), t % (H/16)) owns rows 4tr .. 4tr+3 and
+// columns 16tc .. 16tc+15, so the pivot row and column it reads (and, as
+// their owner, writes) are contiguous in LDS.  The pivot loop is unrolled
+// over p mod 16, so the owners of the pivot row (tr = p/4, row p mod 4) and
+// column (tc = p/16, column p mod 16) index their registers statically: a
+// step is the rank-one update of every entry (one FMA each), then the
+// owners overwrite the pivot row and column with the scaled values.
+template<class T, int C16>
+__device__ __forceinline__ bool gj_step(T (&a)[4][16], int tr, int tc, int p, T (*rowp)[256],
+                                        T (*colp)[256], int32_t* bad)
 {
     constexpr int RA = C16 & 3;                // the pivot row's register row
+    constexpr int B = C16 & 1;                 // pivot buffer: the step's parity
     const int prow = p >> 2, pcol = p >> 4;    // owners: tr == prow, tc == pcol
     if (tr == prow)
     {
 #pragma unroll
-        for (int c = 0; c < 16; c += 4)
-            *reinterpret_cast<float4*>(&rowp[16*tc + c]) =
-                make_float4(a[RA][c], a[RA][c + 1], a[RA][c + 2], a[RA][c + 3]);
+        for (int c = 0; c < 16; c++) rowp[B][16*tc + c] = a[RA][c];
     }
     if (tc == pcol)
-        *reinterpret_cast<float4*>(&colp[4*tr]) =
-            make_float4(a[0][C16], a[1][C16], a[2][C16], a[3][C16]);
+    {
+#pragma unroll
+        for (int ra = 0; ra < 4; ra++) colp[B][4*tr + ra] = a[ra][C16];
+    }
+    // one barrier per step: the next step writes the other buffer, and the
+    // step after it this one only once every thread is past the next
+    // step's barrier, i.e. done reading this step's values
     __syncthreads();
-    const float piv = rowp[p];
-    if (!(piv > 0.f))
+    const T piv = rowp[B][p];
+    if (!(piv > T(0)))
     {
         // every thread reads the same pivot: the block leaves together
         if (threadIdx.x == 0) *bad = 1;
         return false;
     }
-    const float ip = 1.f/piv;
-    const float4 cp4 = *reinterpret_cast<const float4*>(&colp[4*tr]);
-    const float cp[4] = {cp4.x, cp4.y, cp4.z, cp4.w};
-    float rp[16];
+    const T ip = T(1)/piv;
+    T cp[4], rp[16];
 #pragma unroll
-    for (int c = 0; c < 16; c += 4)
-    {
-        const float4 r4 = *reinterpret_cast<const float4*>(&rowp[16*tc + c]);
-        rp[c] = r4.x*ip; rp[c + 1] = r4.y*ip; rp[c + 2] = r4.z*ip; rp[c + 3] = r4.w*ip;
-    }
+    for (int ra = 0; ra < 4; ra++) cp[ra] = colp[B][4*tr + ra];
+#pragma unroll
+    for (int c = 0; c < 16; c++) rp[c] = rowp[B][16*tc + c]*ip;
 #pragma unroll
     for (int ra = 0; ra < 4; ra++)
 #pragma unroll
-        for (int c = 0; c < 16; c++) a[ra][c] = __builtin_fmaf(-cp[ra], rp[c], a[ra][c]);
+        for (int c = 0; c < 16; c++) a[ra][c] = fma(-cp[ra], rp[c], a[ra][c]);
     // the pivot row: the old row over the pivot; the pivot column: -old
     // column over the pivot; the pivot entry: its reciprocal
     if (tr == prow)
@@ -212,34 +235,33 @@ __device__ __forceinline__ bool gj_step(float (&a)[4][16], int tr, int tc, int p
         for (int ra = 0; ra < 4; ra++) a[ra][C16] = -cp[ra]*ip;
         if (tr == prow) a[RA][C16] = ip;
     }
-    __syncthreads();
     return true;
 }
 
-template<int C16>
-__device__ __forceinline__ bool gj_unrolled(float (&a)[4][16], int tr, int tc, int base, int P,
-                                            float* rowp, float* colp, int32_t* bad)
+template<class T, int C16>
+__device__ __forceinline__ bool gj_unrolled(T (&a)[4][16], int tr, int tc, int base, int P,
+                                            T (*rowp)[256], T (*colp)[256], int32_t* bad)
 {
     if constexpr (C16 < 16)
     {
         const int p = base + C16;
         if (p >= P) return true;
-        if (!gj_step<C16>(a, tr, tc, p, rowp, colp, bad)) return false;
-        return gj_unrolled<C16 + 1>(a, tr, tc, base, P, rowp, colp, bad);
+        if (!gj_step<T, C16>(a, tr, tc, p, rowp, colp, bad)) return false;
+        return gj_unrolled<T, C16 + 1>(a, tr, tc, base, P, rowp, colp, bad);
     }
     return true;
 }
 
-__global__ __launch_bounds__(CO_GJ)
-void k_coarse_schur_inv(int32_t k, int32_t P, int32_t nTI, const double* __restrict__ ctile,
-                        const double* __restrict__ U, double* __restrict__ S,
-                        double* __restrict__ U0, int32_t* bad)
+template<int H>
+__global__ __launch_bounds__((H/4)*(H/16))
+void k_coarse_gj(int32_t n, const double* __restrict__ src, int64_t lds, double* __restrict__ dst,
+                 int64_t ldd, int32_t* bad)
 {
-    __shared__ __align__(16) float rowp[256], colp[256];
-    const double sign = coarse_sign(ctile);
-    const int tr = threadIdx.x >> 4, tc = threadIdx.x & 15;
-    const double* ct = ctile + 4*(int64_t)k*P;
-    float a[4][16];
+    static_assert(H <= 256 && H % 16 == 0, "Gauss-Jordan block size");
+    __shared__ __align__(16) double rowp[2][256], colp[2][256];
+    constexpr int NCG = H/16;
+    const int tr = threadIdx.x / NCG, tc = threadIdx.x % NCG;
+    double a[4][16];
 #pragma unroll
     for (int ra = 0; ra < 4; ra++)
     {
@@ -248,30 +270,12 @@ void k_coarse_schur_inv(int32_t k, int32_t P, int32_t nTI, const double* __restr
         for (int c = 0; c < 16; c++)
         {
             const int j = 16*tc + c;
-            double v = 0.0;
-            if (i < P && j < P)
-            {
-                const int lo = min(i, j), hi = max(i, j), d = hi - lo;
-                if (d == 0) v = ct[4*i];
-                else if (d == 1 && hi % nTI > 0) v = ct[4*hi + 1];
-                else if (d == nTI) v = ct[4*hi + 2];
-                v *= sign;
-                if (k > 0)
-                {
-                    const double* Up = U + (int64_t)(k - 1)*P*P;
-                    v -= (sign*ct[4*i + 3])*Up[(int64_t)i*P + j]*(sign*ct[4*j + 3]);
-                }
-                S[(int64_t)i*P + j] = v;
-            }
-            a[ra][c] = (float)v;
+            a[ra][c] = (i < n && j < n) ? src[(int64_t)i*lds + j] : 0.0;
         }
-        // one row's loads at a time (the scheduler would otherwise issue all
-        // of them ahead and spill)
-        __builtin_amdgcn_sched_barrier(0);
     }
 #pragma unroll 1
-    for (int base = 0; base < P; base += 16)
-        if (!gj_unrolled<0>(a, tr, tc, base, P, rowp, colp, bad)) return;
+    for (int base = 0; base < n; base += 16)
+        if (!gj_unrolled<double, 0>(a, tr, tc, base, n, rowp, colp, bad)) return;
 #pragma unroll
     for (int ra = 0; ra < 4; ra++)
     {
@@ -280,24 +284,41 @@ void k_coarse_schur_inv(int32_t k, int32_t P, int32_t nTI, const double* __restr
         for (int c = 0; c < 16; c++)
         {
             const int j = 16*tc + c;
-            if (i < P && j < P) U0[(int64_t)i*P + j] = (double)a[ra][c];
+            if (i < n && j < n) dst[(int64_t)i*ldd + j] = a[ra][c];
         }
     }
+}
+
+// the inverse of S = [A B; B^T D] (A: h x h) from A^-1, W = A^-1 B T^-1,
+// T^-1 and the top-left block A^-1 + W (A^-1 B)^T:  [TL -W; -W^T T^-1]
+__global__ void k_coarse_assemble(int32_t P, int32_t h, const double* __restrict__ TL,
+                                  const double* __restrict__ W, const double* __restrict__ Ti,
+                                  double* __restrict__ U0)
+{
+    const int64_t e = blockIdx.x*(int64_t)blockDim.x + threadIdx.x;
+    if (e >= (int64_t)P*P) return;
+    const int i = (int)(e / P), j = (int)(e % P), h2 = P - h;
+    double v;
+    if (i < h && j < h) v = TL[(int64_t)i*h + j];
+    else if (i < h) v = -W[(int64_t)i*h2 + (j - h)];
+    else if (j < h) v = -W[(int64_t)j*h2 + (i - h)];
+    else v = Ti[(int64_t)(i - h)*h2 + (j - h)];
+    U0[e] = v;
 }
 
 // Batched FP64 block products of the recurrence, 64 x 64 outputs per
 // workgroup (256 threads, 4 x 4 each), the inner dimension in LDS chunks
 // of 32, the next chunk's loads issued before the current chunk's
-// products:  out = beta*D + alpha*op(A) diag(c) B  for blockIdx.z = 0 .. nb-1
-// P x P blocks (row-major, leading dimensions ldA, ldB, ldD, ldO; block z
-// of each operand at z*stride).  op(A) = A^T when transA.  out must not
-// overlap A, B or D.
+// products:  out = beta*D + alpha*op(A) diag(c) op(B)  for blockIdx.z =
+// 0 .. nb-1 blocks of M x N outputs over an inner dimension Kd (row-major,
+// leading dimensions ldA, ldB, ldD, ldO; block z of each operand at
+// z*stride).  op(X) = X^T when transX.  out must not overlap A, B or D.
 #define GM_T 64
 #define GM_K 32
 struct BlockGemm
 {
-    int32_t P;
-    bool transA;
+    int32_t M, N, Kd;
+    bool transA, transB;
     double alpha, beta;
     const double* A; int64_t ldA, strideA;
     const double* B; int64_t ldB, strideB;
@@ -311,18 +332,19 @@ struct BlockGemm
 __device__ __forceinline__ void gemm_fetch(const BlockGemm& g, const double* Ab, const double* Bb,
                                           int bi, int bj, int m0, double (&av)[8], double (&bv)[8])
 {
-    const int32_t P = g.P;
 #pragma unroll
     for (int u = 0; u < 8; u++)
     {
         const int e = threadIdx.x + 256*u;
         const int r = e / GM_K, mm = e % GM_K;
         const int gi = bi*GM_T + r, gm = m0 + mm;
-        av[u] = (gi < P && gm < P)
+        av[u] = (gi < g.M && gm < g.Kd)
               ? (g.transA ? Ab[(int64_t)gm*g.ldA + gi] : Ab[(int64_t)gi*g.ldA + gm]) : 0.0;
         const int mb = e / GM_T, cc = e % GM_T;
         const int gm2 = m0 + mb, gj = bj*GM_T + cc;
-        bv[u] = (gm2 < P && gj < P) ? g.c[gm2]*Bb[(int64_t)gm2*g.ldB + gj] : 0.0;
+        bv[u] = (gm2 < g.Kd && gj < g.N)
+              ? g.c[gm2]*(g.transB ? Bb[(int64_t)gj*g.ldB + gm2] : Bb[(int64_t)gm2*g.ldB + gj])
+              : 0.0;
     }
 }
 
@@ -331,14 +353,13 @@ void k_coarse_block_gemm(BlockGemm g)
 {
     __shared__ double As[GM_K][GM_T + 1], Bs[GM_K][GM_T + 1];
     const int bi = blockIdx.x, bj = blockIdx.y, z = blockIdx.z;
-    const int32_t P = g.P;
     const double* Ab = g.A + z*g.strideA;
     const double* Bb = g.B + z*g.strideB;
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
     double acc[4][4] = {};
     double av[8], bv[8];
     gemm_fetch(g, Ab, Bb, bi, bj, 0, av, bv);
-    for (int m0 = 0; m0 < P; m0 += GM_K)
+    for (int m0 = 0; m0 < g.Kd; m0 += GM_K)
     {
 #pragma unroll
         for (int u = 0; u < 8; u++)
@@ -349,7 +370,7 @@ void k_coarse_block_gemm(BlockGemm g)
         }
         __syncthreads();
         // the next chunk's loads in flight during this chunk's products
-        if (m0 + GM_K < P) gemm_fetch(g, Ab, Bb, bi, bj, m0 + GM_K, av, bv);
+        if (m0 + GM_K < g.Kd) gemm_fetch(g, Ab, Bb, bi, bj, m0 + GM_K, av, bv);
 #pragma unroll
         for (int mm = 0; mm < GM_K; mm++)
         {
@@ -371,7 +392,7 @@ void k_coarse_block_gemm(BlockGemm g)
         for (int v = 0; v < 4; v++)
         {
             const int gi = bi*GM_T + ty + 16*u, gj = bj*GM_T + tx + 16*v;
-            if (gi < P && gj < P)
+            if (gi < g.M && gj < g.N)
             {
                 double r = g.alpha*acc[u][v];
                 if (Db) r += g.beta*Db[(int64_t)gi*g.ldD + gj];
@@ -451,6 +472,62 @@ __global__ void k_coarse_fill(int64_t n, double v, double* __restrict__ x)
     if (i < n) x[i] = v;
 }
 
+static void gemm_launch(const BlockGemm& g, unsigned nb, hipStream_t s)
+{
+    const dim3 grid((unsigned)((g.M + GM_T - 1)/GM_T), (unsigned)((g.N + GM_T - 1)/GM_T), nb);
+    hipLaunchKernelGGL(k_coarse_block_gemm, grid, dim3(256), 0, s, g);
+}
+
+static BlockGemm gemm_sq(int32_t P)
+{
+    BlockGemm g{};
+    g.M = g.N = g.Kd = P;
+    return g;
+}
+
+// U0 = S^-1 for the P x P SPD matrix S (FP64): one Gauss-Jordan workgroup
+// for P <= 128, else the 2 x 2 split at h = 128
+static void coarse_spd_inverse(int32_t P, const double* S, double* U0, double* t,
+                               const double* ones, int32_t* bad, hipStream_t s)
+{
+    if (P <= 128)
+    {
+        hipLaunchKernelGGL(k_coarse_gj<128>, dim3(1), dim3(256), 0, s, P, S, (int64_t)P, U0,
+                           (int64_t)P, bad);
+        return;
+    }
+    const int32_t h = 128, h2 = P - h;
+    double* Ai = t;                         // h x h
+    double* X = Ai + h*h;                   // h x h2: A^-1 B
+    double* T = X + h*h2;                   // h2 x h2: D - B^T A^-1 B, then T^-1
+    double* Ti = T + h2*h2;
+    double* W = Ti + h2*h2;                 // h x h2: X T^-1
+    double* TL = W + h*h2;                  // h x h
+    hipLaunchKernelGGL(k_coarse_gj<128>, dim3(1), dim3(256), 0, s, h, S, (int64_t)P, Ai,
+                       (int64_t)h, bad);
+    BlockGemm x{};
+    x.M = h; x.N = h2; x.Kd = h; x.alpha = 1.0; x.c = ones;
+    x.A = Ai; x.ldA = h; x.B = S + h; x.ldB = P; x.out = X; x.ldO = h2;
+    gemm_launch(x, 1, s);
+    BlockGemm tt{};
+    tt.M = h2; tt.N = h2; tt.Kd = h; tt.transA = true; tt.alpha = -1.0; tt.beta = 1.0; tt.c = ones;
+    tt.A = S + h; tt.ldA = P; tt.B = X; tt.ldB = h2; tt.D = S + (int64_t)h*P + h; tt.ldD = P;
+    tt.out = T; tt.ldO = h2;
+    gemm_launch(tt, 1, s);
+    hipLaunchKernelGGL(k_coarse_gj<128>, dim3(1), dim3(256), 0, s, h2, T, (int64_t)h2, Ti,
+                       (int64_t)h2, bad);
+    BlockGemm w{};
+    w.M = h; w.N = h2; w.Kd = h2; w.alpha = 1.0; w.c = ones;
+    w.A = X; w.ldA = h2; w.B = Ti; w.ldB = h2; w.out = W; w.ldO = h2;
+    gemm_launch(w, 1, s);
+    BlockGemm tl{};
+    tl.M = h; tl.N = h; tl.Kd = h2; tl.transB = true; tl.alpha = 1.0; tl.beta = 1.0; tl.c = ones;
+    tl.A = W; tl.ldA = h2; tl.B = X; tl.ldB = h2; tl.D = Ai; tl.ldD = h; tl.out = TL; tl.ldO = h;
+    gemm_launch(tl, 1, s);
+    hipLaunchKernelGGL(k_coarse_assemble, dim3((unsigned)((P*P + CO_B - 1)/CO_B)), dim3(CO_B), 0,
+                       s, P, h, TL, W, Ti, U0);
+}
+
 bool launch_coarse_block_inverse(int32_t nT, int32_t nTI, int32_t nTJ, int32_t ld,
                                  const double* ctile, double* U, double* Z, double* work,
                                  float* inv, int32_t* bad, hipStream_t s)
@@ -459,27 +536,16 @@ bool launch_coarse_block_inverse(int32_t nT, int32_t nTI, int32_t nTJ, int32_t l
     if (P > 256 || K < 2 || K*P != nT) return false;
     const int64_t PP = (int64_t)P*P;
     double* S = work;               // S_k (FP64)
-    double* U0 = work + PP;         // the FP32 inverse, as doubles
-    double* E = work + 2*PP;        // S_k U0
-    double* ones = work + 3*PP;     // P ones
-    double* cvec = work + 3*PP + P; // nT: s * z couplings
+    double* ones = S + PP;          // P ones
+    double* cvec = ones + P;        // nT: s * z couplings
+    double* split = cvec + nT;      // the 2 x 2 split's blocks
     hipLaunchKernelGGL(k_coarse_fill, dim3((P + CO_B - 1)/CO_B), dim3(CO_B), 0, s, (int64_t)P,
                        1.0, ones);
-    const unsigned gt = (unsigned)((P + GM_T - 1)/GM_T);
     for (int32_t k = 0; k < K; k++)
     {
-        hipLaunchKernelGGL(k_coarse_schur_inv, dim3(1), dim3(CO_GJ), 0, s, k, P, nTI, ctile, U,
-                           S, U0, bad);
-        // Newton-Schulz in FP64: U_k = 2 U0 - U0 (S_k U0)
-        BlockGemm e{};
-        e.P = P; e.alpha = 1.0;
-        e.A = S; e.ldA = P; e.B = U0; e.ldB = P; e.c = ones; e.out = E; e.ldO = P;
-        hipLaunchKernelGGL(k_coarse_block_gemm, dim3(gt, gt, 1u), dim3(256), 0, s, e);
-        BlockGemm r{};
-        r.P = P; r.alpha = -1.0; r.beta = 2.0;
-        r.A = U0; r.ldA = P; r.B = E; r.ldB = P; r.c = ones; r.D = U0; r.ldD = P;
-        r.out = U + (int64_t)k*PP; r.ldO = P;
-        hipLaunchKernelGGL(k_coarse_block_gemm, dim3(gt, gt, 1u), dim3(256), 0, s, r);
+        hipLaunchKernelGGL(k_coarse_schur_form, dim3((unsigned)((PP + CO_B - 1)/CO_B)), dim3(CO_B),
+                           0, s, k, P, nTI, ctile, U, S);
+        coarse_spd_inverse(P, S, U + (int64_t)k*PP, split, ones, bad, s);
     }
     hipLaunchKernelGGL(k_coarse_cvec, dim3((nT + CO_B - 1)/CO_B), dim3(CO_B), 0, s, nT, ctile,
                        cvec);
@@ -490,23 +556,22 @@ bool launch_coarse_block_inverse(int32_t nT, int32_t nTI, int32_t nTJ, int32_t l
         const double* Uk = U + (int64_t)k*PP;
         const double* c1 = cvec + (int64_t)(k + 1)*P;
         // Z_{j,k} = -Z_{j,k+1} C_{k+1} U_k for j = k+1 .. K-1
-        BlockGemm a{};
-        a.P = P; a.transA = false; a.alpha = -1.0; a.beta = 0.0;
+        BlockGemm a = gemm_sq(P);
+        a.alpha = -1.0; a.beta = 0.0;
         a.A = Z + (int64_t)(k + 1)*P*n + (int64_t)(k + 1)*P; a.ldA = n; a.strideA = (int64_t)P*n;
         a.B = Uk; a.ldB = P; a.strideB = 0;
         a.c = c1;
         a.out = Z + (int64_t)(k + 1)*P*n + (int64_t)k*P; a.ldO = n; a.strideO = (int64_t)P*n;
-        hipLaunchKernelGGL(k_coarse_block_gemm, dim3(gt, gt, (unsigned)(K - 1 - k)), dim3(256),
-                           0, s, a);
+        gemm_launch(a, (unsigned)(K - 1 - k), s);
         // Z_{k,k} = U_k - U_k^T C_{k+1} Z_{k+1,k}
-        BlockGemm b{};
-        b.P = P; b.transA = true; b.alpha = -1.0; b.beta = 1.0;
+        BlockGemm b = gemm_sq(P);
+        b.transA = true; b.alpha = -1.0; b.beta = 1.0;
         b.A = Uk; b.ldA = P;
         b.B = Z + (int64_t)(k + 1)*P*n + (int64_t)k*P; b.ldB = n;
         b.c = c1;
         b.D = Uk; b.ldD = P;
         b.out = Z + (int64_t)k*P*n + (int64_t)k*P; b.ldO = n;
-        hipLaunchKernelGGL(k_coarse_block_gemm, dim3(gt, gt, 1u), dim3(256), 0, s, b);
+        gemm_launch(b, 1, s);
     }
     hipLaunchKernelGGL(k_coarse_inv32_blocks,
                        dim3((unsigned)((ld + CI_T - 1)/CI_T), (unsigned)((nT + CI_T - 1)/CI_T)),
@@ -514,7 +579,12 @@ bool launch_coarse_block_inverse(int32_t nT, int32_t nTI, int32_t nTJ, int32_t l
     return true;
 }
 
-int64_t coarse_block_work_doubles(int32_t nT, int32_t P) { return 3*(int64_t)P*P + P + nT; }
+// S, ones, cvec and the six blocks of the 2 x 2 split (each at most
+// 128 x 128)
+int64_t coarse_block_work_doubles(int32_t nT, int32_t P)
+{
+    return (int64_t)P*P + P + nT + 6*128*128;
+}
 
 // z = Ainv rc (one wavefront per row, four rows per block) and sum z*rc;
 // block 0 also adds the fine partials of the tile-local apply (sum w*r, nF

@@ -378,12 +378,13 @@ def test_mixed_pcg_tile_dic_half():
     assert abs(p8.nIterations - exact) <= max(3, 0.05 * exact)
 
 
-@pytest.mark.parametrize("shape", [(64, 48, 32), (32, 32, 80), (16, 16, 16)])
+@pytest.mark.parametrize("shape", [(64, 48, 32), (32, 32, 80), (16, 16, 16), (208, 160, 32)])
 def test_mixed_pcg_two_level_boxes(shape):
     """Mixed PCG with the two-level inner preconditioner on non-cubic boxes
-    (and a single tile, where the coarse part is off): the FP64 residual
-    test holds for the true residual, and at a tight tolerance the solution
-    matches the FP64 oracle to 1e-8."""
+    (and a single tile, where the coarse part is off; 208 x 160 x 32 has 130
+    tiles per plane, so each plane's inverse is split 2 x 2 at 128): the FP64
+    residual test holds for the true residual, and at a tight tolerance the
+    solution matches the FP64 oracle to 1e-8."""
     from foamhip.cases import BoxCase
     nx, ny, nz = shape
     c = BoxCase(nx, ny, nz)
