@@ -132,7 +132,8 @@ def main():
         """One solve right after the coefficients were set again (as every
         fvMatrix::solve call sets them): the solver's whole setup -- FP32
         copies, factors, packed coefficients, the coarse matrix and its
-        Cholesky inverse -- is inside the timed call; the assembly is not."""
+        block-tridiagonal inverse -- is inside the timed call; the assembly
+        is not."""
         assemble()
         t = time.perf_counter()
         pf = solve(s)
@@ -315,8 +316,8 @@ def main():
                         "vs the FP64 drop-in (reference DIC)",
             "cells": case.N, "faces": case.F,
             "timing": "every timed solve follows a fresh coefficient set, so the solver "
-                      "setup (factors, packed coefficients, coarse matrix and its Cholesky "
-                      "inverse) is inside it",
+                      "setup (factors, packed coefficients, coarse matrix and its "
+                      "block-tridiagonal inverse) is inside it",
             "mixed_s_per_solve_setup_cached": t_mixed_warm,
             "fp64_s_per_solve": t_fp64,
             "speedup_mixed_vs_fp64": t_fp64 / t_mixed,
