@@ -12,7 +12,6 @@
 #include "fh_gamg.h"
 #include "fh_gamg_kernels.h"
 
-#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <cstdio>
@@ -474,8 +473,9 @@ struct GamgHierarchy
     DBuf<double> csDiagN, csRD, csSumA, csWork;
     DBuf<double> luLU, luWork;
     DBuf<int32_t> luPivot;
-    DBuf<double> dA, dX, dT, dPart;
-    rocblas_handle blas = nullptr;
+    DBuf<double> dA, dX, dX1, dT, dPart, dGemv;
+    DBuf<NsState> dNs;
+    DBuf<int32_t> dFlag;
     GamgWork<double> w64;
     GamgWork<float> w32;
 
@@ -486,7 +486,6 @@ struct GamgHierarchy
             matrix_free(A);
             delete A;
         }
-        if (blas) (void)rocblas_destroy_handle(blas);
     }
 };
 
@@ -547,53 +546,6 @@ void gamg_check_sweep_errors(fh_matrix* M)
 bool gamg_forward() { return g_forward; }
 void gamg_set_forward(bool f) { g_forward = f; }
 
-// host Crout decomposition with implicit partial pivoting
-// (scalarMatrices.C:77-180), row-major, natural order
-static void crout_lu(int32_t n, std::vector<double>& A, std::vector<int32_t>& piv)
-{
-    std::vector<double> scale(n);
-    piv.assign(n, 0);
-    for (int32_t i = 0; i < n; i++)
-    {
-        double big = 0.0;
-        for (int32_t j = 0; j < n; j++) big = std::max(big, std::fabs(A[(size_t)i*n + j]));
-        if (big == 0.0) fail(FH_ERR_NUMERIC, "singular coarsest-level matrix (row %d)", i);
-        scale[i] = 1.0/big;
-    }
-    for (int32_t j = 0; j < n; j++)
-    {
-        for (int32_t i = 0; i < j; i++)
-        {
-            double sum = A[(size_t)i*n + j];
-            for (int32_t k = 0; k < i; k++) sum -= A[(size_t)i*n + k]*A[(size_t)k*n + j];
-            A[(size_t)i*n + j] = sum;
-        }
-        int32_t iMax = 0;
-        double best = 0.0;
-        for (int32_t i = j; i < n; i++)
-        {
-            double sum = A[(size_t)i*n + j];
-            for (int32_t k = 0; k < j; k++) sum -= A[(size_t)i*n + k]*A[(size_t)k*n + j];
-            A[(size_t)i*n + j] = sum;
-            const double t = scale[i]*std::fabs(sum);
-            if (t >= best) { best = t; iMax = i; }
-        }
-        piv[j] = iMax;
-        if (j != iMax)
-        {
-            for (int32_t k = 0; k < n; k++)
-                std::swap(A[(size_t)j*n + k], A[(size_t)iMax*n + k]);
-            scale[iMax] = scale[j];
-        }
-        if (A[(size_t)j*n + j] == 0.0) A[(size_t)j*n + j] = 2.220446049250313e-16;
-        if (j != n - 1)
-        {
-            const double r = 1.0/A[(size_t)j*n + j];
-            for (int32_t i = j + 1; i < n; i++) A[(size_t)i*n + j] *= r;
-        }
-    }
-}
-
 static void setup_coarsest(fh_matrix* M, GamgHierarchy* H, int mode)
 {
     fh_matrix* C = H->mats.back();
@@ -626,102 +578,54 @@ static void setup_coarsest(fh_matrix* M, GamgHierarchy* H, int mode)
     }
     else
     {
-        // dense natural copy (LUscalarMatrix.C:66-200 convert)
-        std::vector<double> diagLib(n), upper(nF), lower;
-        FH_HIP(hipMemcpyAsync(diagLib.data(), C->d_diag.p, sizeof(double)*n,
-                              hipMemcpyDeviceToHost, s));
-        if (nF > 0)
-            FH_HIP(hipMemcpyAsync(upper.data(), C->d_upperF.p, sizeof(double)*nF,
-                                  hipMemcpyDeviceToHost, s));
-        if (!C->symmetric && nF > 0)
-        {
-            lower.resize(nF);
-            FH_HIP(hipMemcpyAsync(lower.data(), C->d_lowerF.p, sizeof(double)*nF,
-                                  hipMemcpyDeviceToHost, s));
-        }
-        FH_HIP(hipStreamSynchronize(s));
-        const std::vector<double>& lo = C->symmetric ? upper : lower;
-        std::vector<double> A((size_t)n*n, 0.0);
-        for (int32_t r = 0; r < n; r++) A[(size_t)cm->perm[r]*n + cm->perm[r]] = diagLib[r];
-        for (int32_t f = 0; f < nF; f++)
-        {
-            const int32_t l = cm->lower[f], u = cm->upper[f];
-            A[(size_t)l*n + u] = upper[f];
-            A[(size_t)u*n + l] = lo[f];
-        }
+        // the dense coarsest matrix on the device (LUscalarMatrix.C:66-200
+        // convert), library order, column-major: no host copy
+        const size_t nn = (size_t)n*n;
+        H->dA.alloc(nn);
+        FH_HIP(hipMemsetAsync(H->dA.p, 0, sizeof(double)*nn, s));
+        launch_dense_fill(n, C->rows(), C->d_diag.p, C->d_val.p, H->dA.p, s);
         if (mode == COARSEST_LU)
         {
-            std::vector<int32_t> piv;
-            crout_lu(n, A, piv);
-            H->luLU.upload(A, s);
-            H->luPivot.upload(piv, s);
+            // Crout with implicit pivoting in the reference's order
+            // (scalarMatrices.C:77-180), one workgroup
+            H->luLU.alloc(nn);
+            H->luPivot.alloc(n);
             H->luWork.alloc(n);
+            H->dFlag.alloc(1);
+            FH_HIP(hipMemsetAsync(H->dFlag.p, 0, sizeof(int32_t), s));
+            launch_crout_lu(n, H->dA.p, cm->d_perm.p, H->luLU.p, H->luPivot.p, H->dFlag.p, s);
+            int32_t bad = 0;
+            FH_HIP(hipMemcpyAsync(&bad, H->dFlag.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
             FH_HIP(hipStreamSynchronize(s));
+            if (bad) fail(FH_ERR_NUMERIC, "singular coarsest-level matrix (a zero row)");
         }
         else
         {
             // explicit inverse by Newton-Schulz iteration X <- X (2I - A X)
-            // from X0 = A^T/(|A|_1 |A|_inf); the products are rocBLAS DGEMMs
-            // (FP64 MFMA).  Library row order, column-major.
-            double n1 = 0.0, ninf = 0.0;
-            for (int32_t j = 0; j < n; j++)
-            {
-                double cs = 0.0, rs = 0.0;
-                for (int32_t i = 0; i < n; i++)
-                {
-                    cs += std::fabs(A[(size_t)i*n + j]);
-                    rs += std::fabs(A[(size_t)j*n + i]);
-                }
-                n1 = std::max(n1, cs);
-                ninf = std::max(ninf, rs);
-            }
-            if (!(n1 > 0.0) || !(ninf > 0.0))
-                fail(FH_ERR_NUMERIC, "zero coarsest-level matrix");
-            const size_t nn = (size_t)n*n;
-            H->dA.alloc(nn);
+            // from X0 = A^T/(|A|_1 |A|_inf): FP64 MFMA products, the
+            // convergence test in the device state; the host polls it between
+            // batches of iterations
             H->dX.alloc(nn);
             H->dT.alloc(nn);
-            DBuf<double> Xn;
-            Xn.alloc(nn);
+            H->dX1.alloc(nn);
             H->dPart.alloc((nn + 255)/256 + 1);
-            launch_dense_fill(n, C->rows(), C->d_diag.p, C->d_val.p, H->dA.p, s);
-            launch_scaled_transpose(n, H->dA.p, 1.0/(n1*ninf), H->dX.p, s);
-            if (!H->blas)
+            H->dNs.alloc(1);
+            H->dGemv.alloc((size_t)std::max<int64_t>(gemv_work_doubles(n), 2*(int64_t)n));
+            launch_ns_init(n, H->dA.p, H->dX.p, H->dGemv.p, H->dNs.p, s);
+            NsState ns{};
+            for (int it = 0; it < 200; it += 8)
             {
-                if (rocblas_create_handle(&H->blas) != rocblas_status_success)
-                    fail(FH_ERR_HIP, "rocblas_create_handle failed");
-            }
-            if (rocblas_set_stream(H->blas, s) != rocblas_status_success)
-                fail(FH_ERR_HIP, "rocblas_set_stream failed");
-            const double one = 1.0, zero = 0.0;
-            std::vector<double> pm;
-            double prev = 1e300, err = 1e300;
-            for (int it = 0; it < 200; it++)
-            {
-                if (rocblas_dgemm(H->blas, rocblas_operation_none, rocblas_operation_none, n, n,
-                                  n, &one, H->dA.p, n, H->dX.p, n, &zero, H->dT.p, n)
-                    != rocblas_status_success)
-                    fail(FH_ERR_HIP, "rocblas_dgemm failed");
-                const int64_t nb = launch_ident_err(n, H->dT.p, H->dPart.p, s);
-                pm.resize(nb);
-                FH_HIP(hipMemcpyAsync(pm.data(), H->dPart.p, sizeof(double)*nb,
-                                      hipMemcpyDeviceToHost, s));
+                launch_ns_iterations(n, H->dA.p, H->dX.p, H->dX1.p, H->dT.p, H->dPart.p,
+                                     H->dNs.p, 8, s);
+                FH_HIP(hipMemcpyAsync(&ns, H->dNs.p, sizeof(NsState), hipMemcpyDeviceToHost, s));
                 FH_HIP(hipStreamSynchronize(s));
-                err = 0.0;
-                for (double v : pm) err = std::max(err, v);
-                if (!std::isfinite(err)) fail(FH_ERR_NUMERIC, "coarsest inverse diverged");
-                if (err < 1e-13 || (err < 1e-8 && err > 0.5*prev)) break;
-                prev = err;
-                launch_two_minus(n, H->dT.p, s);
-                if (rocblas_dgemm(H->blas, rocblas_operation_none, rocblas_operation_none, n, n,
-                                  n, &one, H->dX.p, n, H->dT.p, n, &zero, Xn.p, n)
-                    != rocblas_status_success)
-                    fail(FH_ERR_HIP, "rocblas_dgemm failed");
-                std::swap(H->dX.p, Xn.p);
+                if (ns.done) break;
             }
-            FH_HIP(hipStreamSynchronize(s));
-            if (!(err < 1e-6))
-                fail(FH_ERR_NUMERIC, "coarsest inverse did not converge (|I - AX| = %g)", err);
+            if (ns.bad && !(ns.scale > 0.0)) fail(FH_ERR_NUMERIC, "zero coarsest-level matrix");
+            if (ns.bad) fail(FH_ERR_NUMERIC, "coarsest inverse diverged");
+            if (!(ns.err < 1e-6))
+                fail(FH_ERR_NUMERIC, "coarsest inverse did not converge (|I - AX| = %g)", ns.err);
+            launch_ns_final(n, H->dX1.p, H->dX.p, H->dNs.p, s);
         }
     }
     H->coarseValid = true;
@@ -838,7 +742,7 @@ static void solve_coarsest(fh_matrix* M, GamgHierarchy* H, T* corr, const T* src
                                  st, s);
             break;
         case COARSEST_INVERSE:
-            launch_gemv<T>(n, H->dX.p, src, corr, st, s);
+            launch_gemv<T>(n, H->dX.p, src, corr, H->dGemv.p, st, s);
             break;
         default:
         {

@@ -63,16 +63,34 @@ template<class T>
 void launch_lu_backsub(int32_t n, const double* LU, const int32_t* pivot, const int32_t* perm,
                        double* work, const T* srcLib, T* corrLib, const DevState* st,
                        hipStream_t s);
-// coarsest level: dense explicit inverse (library order, column-major)
+// coarsest level: the dense matrix (library order, column-major)
 void launch_dense_fill(int32_t n, const RowsView& R, const double* diag, const double* val,
                        double* A, hipStream_t s);
-void launch_two_minus(int32_t n, double* T_, hipStream_t s);
-int64_t launch_ident_err(int32_t n, const double* AX, double* partMax, hipStream_t s);
-void launch_scaled_transpose(int32_t n, const double* A, double scale, double* X,
-                             hipStream_t s);
+// coarsest level, n <= 96: Crout LU with implicit pivoting in the reference's
+// order (the dense column-major library-order matrix in, natural order out)
+void launch_crout_lu(int32_t n, const double* Acm, const int32_t* perm, double* LU,
+                     int32_t* piv, int32_t* bad, hipStream_t s);
+// coarsest level, larger: explicit inverse by Newton-Schulz X <- X (2I - A X)
+// from X0 = A^T/(|A|_1 |A|_inf), every step on the device (FP64 MFMA
+// products, the convergence test in the state); the host polls `done`
+struct NsState
+{
+    double scale, prev, err;
+    int32_t done, bad, cur, its;
+};
+// sums: 2n doubles
+void launch_ns_init(int32_t n, const double* A, double* X0, double* sums, NsState* ns,
+                    hipStream_t s);
+void launch_ns_iterations(int32_t n, const double* A, double* X0, double* X1, double* T_,
+                          double* partMax, NsState* ns, int iterations, hipStream_t s);
+void launch_ns_final(int32_t n, const double* X1, double* X0, const NsState* ns, hipStream_t s);
+// C = A B, column-major n x n FP64 (the Newton-Schulz product kernel)
+void launch_dense_gemm(int32_t n, const double* A, const double* B, double* C, hipStream_t s);
+// part: gemv_work_doubles(n) doubles
 template<class T>
-void launch_gemv(int32_t n, const double* X, const T* b, T* x, const DevState* st,
-                 hipStream_t s);
+void launch_gemv(int32_t n, const double* X, const T* b, T* x, double* part,
+                 const DevState* st, hipStream_t s);
+int64_t gemv_work_doubles(int32_t n);
 
 // small gated vector helpers
 template<class T>

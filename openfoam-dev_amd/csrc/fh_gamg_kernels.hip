@@ -194,16 +194,19 @@ __global__ void k_dense_fill(int32_t n, RowsView R, const double* __restrict__ d
 }
 
 // Newton-Schulz helpers: T = 2I - T; and ||I - A X||_max partial maxima
-__global__ void k_two_minus(int32_t n, double* __restrict__ Tm)
+__global__ void k_two_minus(int32_t n, double* __restrict__ Tm, const NsState* ns)
 {
+    if (ns && ns->done) return;
     const int64_t i = (int64_t)blockIdx.x*GB + threadIdx.x;
     if (i >= (int64_t)n*n) return;
     const int64_t r = i % n, c = i/n;
     Tm[i] = (r == c ? 2.0 : 0.0) - Tm[i];
 }
 
-__global__ void k_ident_err(int32_t n, const double* __restrict__ AX, double* partMax)
+__global__ void k_ident_err(int32_t n, const double* __restrict__ AX, double* partMax,
+                            const NsState* ns)
 {
+    if (ns && ns->done) return;
     __shared__ double sm[GB];
     const int64_t i = (int64_t)blockIdx.x*GB + threadIdx.x;
     double e = 0.0;
@@ -222,26 +225,337 @@ __global__ void k_ident_err(int32_t n, const double* __restrict__ AX, double* pa
     if (threadIdx.x == 0) partMax[blockIdx.x] = sm[0];
 }
 
-__global__ void k_scaled_transpose(int32_t n, const double* __restrict__ A, double scale,
-                                   double* __restrict__ X)
+// X0 = A^T / (|A|_1 |A|_inf), the scale from the device state
+__global__ void k_scaled_transpose(int32_t n, const double* __restrict__ A,
+                                   const NsState* __restrict__ ns, double* __restrict__ X)
 {
     const int64_t i = (int64_t)blockIdx.x*GB + threadIdx.x;
     if (i >= (int64_t)n*n) return;
     const int64_t r = i % n, c = i/n;
-    X[i] = scale*A[r*n + c];
+    X[i] = ns->scale*A[r*n + c];
 }
 
-// coarsest correction x = X b: one thread per row, columns in ascending
-// order (fixed summation order, coalesced over the rows of a column)
+// column sums of |a| (one workgroup per column) and row sums (one thread
+// per row) of the column-major n x n matrix, into sums[0..n) and [n..2n)
+__global__ __launch_bounds__(256)
+void k_ns_colsum(int32_t n, const double* __restrict__ A, double* __restrict__ sums)
+{
+    __shared__ double sm[256];
+    const int32_t c = blockIdx.x;
+    double v = 0.0;
+    for (int32_t i = threadIdx.x; i < n; i += 256) v += fabs(A[(int64_t)c*n + i]);
+    sm[threadIdx.x] = v;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1)
+    {
+        if ((int)threadIdx.x < h) sm[threadIdx.x] += sm[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) sums[c] = sm[0];
+}
+
+__global__ void k_ns_rowsum(int32_t n, const double* __restrict__ A, double* __restrict__ sums)
+{
+    const int32_t r = blockIdx.x*GB + threadIdx.x;
+    if (r >= n) return;
+    double v = 0.0;
+    for (int32_t c = 0; c < n; c++) v += fabs(A[(int64_t)c*n + r]);
+    sums[n + r] = v;
+}
+
+// |A|_1 (largest column sum) and |A|_inf (largest row sum): the scale of
+// X0; initialises the Newton-Schulz state
+__global__ __launch_bounds__(1024)
+void k_ns_init(int32_t n, const double* __restrict__ sums, NsState* ns)
+{
+    __shared__ double sc[1024], sr[1024];
+    double mc = 0.0, mr = 0.0;
+    for (int32_t j = threadIdx.x; j < n; j += 1024)
+    {
+        mc = fmax(mc, sums[j]);
+        mr = fmax(mr, sums[n + j]);
+    }
+    sc[threadIdx.x] = mc;
+    sr[threadIdx.x] = mr;
+    __syncthreads();
+    for (int h = 512; h > 0; h >>= 1)
+    {
+        if ((int)threadIdx.x < h)
+        {
+            sc[threadIdx.x] = fmax(sc[threadIdx.x], sc[threadIdx.x + h]);
+            sr[threadIdx.x] = fmax(sr[threadIdx.x], sr[threadIdx.x + h]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+    {
+        const bool ok = sc[0] > 0.0 && sr[0] > 0.0;
+        ns->scale = ok ? 1.0/(sc[0]*sr[0]) : 0.0;
+        ns->prev = 1e300;
+        ns->err = 1e300;
+        ns->done = ok ? 0 : 1;
+        ns->bad = ok ? 0 : 1;
+        ns->cur = 0;
+        ns->its = 0;
+    }
+}
+
+// the convergence test of an iteration from the partial maxima of |I - AX|
+// (one workgroup; a maximum does not depend on the order): stop below
+// 1e-13, or below 1e-8 once the error no longer halves; a non-finite error
+// stops with bad set
+__global__ __launch_bounds__(1024)
+void k_ns_check(int64_t nb, const double* __restrict__ partMax, NsState* ns)
+{
+    if (ns->done) return;
+    __shared__ double sm[1024];
+    __shared__ int nonFinite;
+    if (threadIdx.x == 0) nonFinite = 0;
+    __syncthreads();
+    double m = 0.0;
+    for (int64_t b = threadIdx.x; b < nb; b += 1024)
+    {
+        const double v = partMax[b];
+        if (!isfinite(v)) nonFinite = 1;
+        m = fmax(m, v);
+    }
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int h = 512; h > 0; h >>= 1)
+    {
+        if ((int)threadIdx.x < h) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + h]);
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    const double err = nonFinite ? NAN : sm[0];
+    ns->err = err;
+    ns->its++;
+    if (!isfinite(err)) { ns->bad = 1; ns->done = 1; return; }
+    if (err < 1e-13 || (err < 1e-8 && err > 0.5*ns->prev)) { ns->done = 1; return; }
+    ns->prev = err;
+}
+
+// after X_{k+1} = X_k T went to the other buffer: it becomes the current one
+__global__ void k_ns_flip(NsState* ns)
+{
+    if (!ns->done && threadIdx.x == 0) ns->cur = 1 - ns->cur;
+}
+
+// the result into buffer 0
+__global__ void k_ns_final(int32_t n, const double* __restrict__ X1, double* __restrict__ X0,
+                           const NsState* __restrict__ ns)
+{
+    if (ns->cur == 0) return;
+    const int64_t i = (int64_t)blockIdx.x*GB + threadIdx.x;
+    if (i < (int64_t)n*n) X0[i] = X1[i];
+}
+
+// C = A B for column-major n x n FP64 matrices on the matrix cores
+// (v_mfma_f64_16x16x4f64).  A workgroup of four wavefronts owns a 64 x 64
+// block of C, a wavefront 32 x 32 (four 16 x 16 MFMA blocks); the inner
+// dimension is staged through LDS in chunks of 16, the next chunk's loads
+// issued before the current chunk's products.  The MFMA (lane l holds its
+// A operand's (l%16, l/16) entry and its B operand's (l/16, l%16) entry;
+// element v of its result is entry (l/16 + 4v, l%16)) computes the
+// transposed product, its A operand from B and its B operand from A: lane
+// l, element v then holds C(i0 + l%16, j0 + l/16 + 4v), so a store writes
+// 16 consecutive rows of a column.  Skipped when the Newton-Schulz state is
+// done; a null A or B is the state's current X, and outOther writes the
+// other X buffer instead of C.
+typedef double mfma_d4 __attribute__((ext_vector_type(4)));
+#define DG_T 64
+#define DG_K 16
+__global__ __launch_bounds__(256)
+void k_dense_gemm(int32_t n, const double* __restrict__ A, const double* __restrict__ B,
+                  const double* __restrict__ X0, const double* __restrict__ X1,
+                  double* __restrict__ C, int32_t outOther, const NsState* __restrict__ ns)
+{
+    if (ns && ns->done) return;
+    const double* Xc = ns && ns->cur ? X1 : X0;
+    if (!A) A = Xc;
+    if (!B) B = Xc;
+    if (outOther) C = const_cast<double*>(ns->cur ? X0 : X1);
+    __shared__ double As[DG_K][DG_T + 1];        // A(i0 + i, k0 + k) at [k][i]
+    __shared__ double Bs[DG_T][DG_K + 1];        // B(k0 + k, j0 + j) at [j][k]
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int64_t i0 = (int64_t)blockIdx.x*DG_T, j0 = (int64_t)blockIdx.y*DG_T;
+    const int wi = 32*(w & 1), wj = 32*(w >> 1);
+    mfma_d4 acc[2][2];
+#pragma unroll
+    for (int r = 0; r < 2; r++)
+#pragma unroll
+        for (int c = 0; c < 2; c++) acc[r][c] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+    double av[4], bv[4];
+    auto fetch = [&](int64_t k0) {
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+        {
+            const int e = t + 256*u;
+            const int ia = e & 63, ka = e >> 6;              // A tile: 64 rows x 16
+            const int64_t gi = i0 + ia, gk = k0 + ka;
+            av[u] = (gi < n && gk < n) ? A[gk*n + gi] : 0.0;
+            const int kb = e & 15, jb = e >> 4;              // B tile: 16 x 64 columns
+            const int64_t gk2 = k0 + kb, gj = j0 + jb;
+            bv[u] = (gk2 < n && gj < n) ? B[gj*n + gk2] : 0.0;
+        }
+    };
+    fetch(0);
+    for (int64_t k0 = 0; k0 < n; k0 += DG_K)
+    {
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+        {
+            const int e = t + 256*u;
+            As[e >> 6][e & 63] = av[u];
+            Bs[e >> 4][e & 15] = bv[u];
+        }
+        __syncthreads();
+        if (k0 + DG_K < n) fetch(k0 + DG_K);
+#pragma unroll
+        for (int kk = 0; kk < DG_K; kk += 4)
+        {
+            const int k = kk + (lane >> 4);
+            double bop[2], aop[2];
+#pragma unroll
+            for (int r = 0; r < 2; r++) bop[r] = As[k][wi + 16*r + (lane & 15)];
+#pragma unroll
+            for (int c = 0; c < 2; c++) aop[c] = Bs[wj + 16*c + (lane & 15)][k];
+#pragma unroll
+            for (int r = 0; r < 2; r++)
+#pragma unroll
+                for (int c = 0; c < 2; c++)
+                    acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(aop[c], bop[r], acc[r][c],
+                                                                    0, 0, 0);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < 2; r++)
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+            {
+                const int64_t gi = i0 + wi + 16*r + (lane & 15);
+                const int64_t gj = j0 + wj + 16*c + (lane >> 4) + 4*v;
+                if (gi < n && gj < n) C[gj*n + gi] = acc[r][c][v];
+            }
+}
+
+// Crout decomposition with implicit partial pivoting in one workgroup, in
+// the loop order of scalarMatrices.C:77-180 (every sum in the same order,
+// the pivot the last row of the largest scaled magnitude): the same bits as
+// that routine.  The dense natural-order matrix (row-major, in LDS) is
+// gathered from the column-major library-order one; LU row-major and the
+// pivots out.  A zero row sets *bad.  INLDS: the matrix in LDS (when the
+// device allows n*n doubles per workgroup), else in LU itself.
+template<bool INLDS>
+__global__ __launch_bounds__(128)
+void k_crout_lu(int32_t n, const double* __restrict__ Acm, const int32_t* __restrict__ perm,
+                double* __restrict__ LU, int32_t* __restrict__ piv, int32_t* bad)
+{
+    extern __shared__ double Mlds[];             // n*n
+    double* M = INLDS ? Mlds : LU;
+    __shared__ double sc[128];
+    __shared__ int32_t sMax;
+    const int t = threadIdx.x;
+    for (int64_t e = t; e < (int64_t)n*n; e += blockDim.x)
+    {
+        const int32_t c = (int32_t)(e / n), r = (int32_t)(e % n);   // library (r, c)
+        M[(int64_t)perm[r]*n + perm[c]] = Acm[e];
+    }
+    __syncthreads();
+    if (t < n)
+    {
+        double big = 0.0;
+        for (int32_t j = 0; j < n; j++) big = fmax(big, fabs(M[(int64_t)t*n + j]));
+        if (big == 0.0) *bad = 1;
+        sc[t] = 1.0/big;
+    }
+    __syncthreads();
+    for (int32_t j = 0; j < n; j++)
+    {
+        if (t == 0)
+            for (int32_t i = 0; i < j; i++)
+            {
+                double sum = M[(int64_t)i*n + j];
+                for (int32_t k = 0; k < i; k++) sum -= M[(int64_t)i*n + k]*M[(int64_t)k*n + j];
+                M[(int64_t)i*n + j] = sum;
+            }
+        __syncthreads();
+        if (t >= j && t < n)
+        {
+            double sum = M[(int64_t)t*n + j];
+            for (int32_t k = 0; k < j; k++) sum -= M[(int64_t)t*n + k]*M[(int64_t)k*n + j];
+            M[(int64_t)t*n + j] = sum;
+        }
+        __syncthreads();
+        if (t == 0)
+        {
+            int32_t iMax = 0;
+            double best = 0.0;
+            for (int32_t i = j; i < n; i++)
+            {
+                const double s2 = sc[i]*fabs(M[(int64_t)i*n + j]);
+                if (s2 >= best) { best = s2; iMax = i; }
+            }
+            piv[j] = iMax;
+            sMax = iMax;
+        }
+        __syncthreads();
+        const int32_t iMax = sMax;
+        if (iMax != j)
+        {
+            for (int32_t k = t; k < n; k += blockDim.x)
+            {
+                const double a = M[(int64_t)j*n + k];
+                M[(int64_t)j*n + k] = M[(int64_t)iMax*n + k];
+                M[(int64_t)iMax*n + k] = a;
+            }
+            if (t == 0) sc[iMax] = sc[j];
+        }
+        __syncthreads();
+        if (t == 0 && M[(int64_t)j*n + j] == 0.0) M[(int64_t)j*n + j] = 2.220446049250313e-16;
+        __syncthreads();
+        if (j != n - 1)
+        {
+            const double r = 1.0/M[(int64_t)j*n + j];
+            if (t > j && t < n) M[(int64_t)t*n + j] *= r;
+        }
+        __syncthreads();
+    }
+    if constexpr (INLDS)
+        for (int64_t e = t; e < (int64_t)n*n; e += blockDim.x) LU[e] = M[e];
+}
+
+// coarsest correction x = X b: the columns in GEMV_SPLIT fixed ranges, one
+// thread per row and range summing its range in ascending order
+// (coalesced over the rows of a column), then the ranges summed in order
+#define GEMV_SPLIT 16
 template<class T>
-__global__ void k_gemv(int32_t n, const double* __restrict__ X, const T* __restrict__ b,
-                       T* __restrict__ x, const DevState* st)
+__global__ __launch_bounds__(64)
+void k_gemv_part(int32_t n, const double* __restrict__ X, const T* __restrict__ b,
+                 double* __restrict__ part, const DevState* st)
+{
+    if (st && st->done) return;
+    const int32_t r = blockIdx.x*64 + threadIdx.x;
+    if (r >= n) return;
+    const int32_t cs = (n + GEMV_SPLIT - 1)/GEMV_SPLIT;
+    const int32_t c0 = blockIdx.y*cs, c1 = min(n, c0 + cs);
+    double s = 0.0;
+    for (int32_t c = c0; c < c1; c++) s += X[(int64_t)c*n + r]*(double)b[c];
+    part[(int64_t)blockIdx.y*n + r] = s;
+}
+
+template<class T>
+__global__ void k_gemv_sum(int32_t n, const double* __restrict__ part, T* __restrict__ x,
+                           const DevState* st)
 {
     if (st && st->done) return;
     const int32_t r = blockIdx.x*GB + threadIdx.x;
     if (r >= n) return;
     double s = 0.0;
-    for (int32_t c = 0; c < n; c++) s += X[(int64_t)c*n + r]*(double)b[c];
+    for (int k = 0; k < GEMV_SPLIT; k++) s += part[(int64_t)k*n + r];
     x[r] = (T)s;
 }
 
@@ -633,30 +947,76 @@ void launch_dense_fill(int32_t n, const RowsView& R, const double* diag, const d
     hipLaunchKernelGGL(k_dense_fill, gg(n), dim3(GB), 0, s, n, R, diag, val, A);
 }
 
-void launch_two_minus(int32_t n, double* T_, hipStream_t s)
+void launch_ns_init(int32_t n, const double* A, double* X0, double* sums, NsState* ns,
+                    hipStream_t s)
 {
-    hipLaunchKernelGGL(k_two_minus, gg((int64_t)n*n), dim3(GB), 0, s, n, T_);
+    hipLaunchKernelGGL(k_ns_colsum, dim3((unsigned)n), dim3(256), 0, s, n, A, sums);
+    hipLaunchKernelGGL(k_ns_rowsum, gg(n), dim3(GB), 0, s, n, A, sums);
+    hipLaunchKernelGGL(k_ns_init, dim3(1), dim3(1024), 0, s, n, sums, ns);
+    hipLaunchKernelGGL(k_scaled_transpose, gg((int64_t)n*n), dim3(GB), 0, s, n, A, ns, X0);
 }
 
-int64_t launch_ident_err(int32_t n, const double* AX, double* partMax, hipStream_t s)
+void launch_ns_iterations(int32_t n, const double* A, double* X0, double* X1, double* T_,
+                          double* partMax, NsState* ns, int iterations, hipStream_t s)
 {
     const int64_t nb = ((int64_t)n*n + GB - 1)/GB;
-    hipLaunchKernelGGL(k_ident_err, dim3((unsigned)nb), dim3(GB), 0, s, n, AX, partMax);
-    return nb;
+    const dim3 grid((unsigned)((n + DG_T - 1)/DG_T), (unsigned)((n + DG_T - 1)/DG_T));
+    for (int it = 0; it < iterations; it++)
+    {
+        // T = A X; the test on |I - T|; T = 2I - T; X' = X T
+        hipLaunchKernelGGL(k_dense_gemm, grid, dim3(256), 0, s, n, A, nullptr, X0, X1, T_, 0,
+                           ns);
+        hipLaunchKernelGGL(k_ident_err, dim3((unsigned)nb), dim3(GB), 0, s, n, T_, partMax, ns);
+        hipLaunchKernelGGL(k_ns_check, dim3(1), dim3(1024), 0, s, nb, partMax, ns);
+        hipLaunchKernelGGL(k_two_minus, gg((int64_t)n*n), dim3(GB), 0, s, n, T_, ns);
+        hipLaunchKernelGGL(k_dense_gemm, grid, dim3(256), 0, s, n, nullptr, T_, X0, X1,
+                           nullptr, 1, ns);
+        hipLaunchKernelGGL(k_ns_flip, dim3(1), dim3(64), 0, s, ns);
+    }
 }
 
-void launch_scaled_transpose(int32_t n, const double* A, double scale, double* X,
-                             hipStream_t s)
+void launch_ns_final(int32_t n, const double* X1, double* X0, const NsState* ns, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_scaled_transpose, gg((int64_t)n*n), dim3(GB), 0, s, n, A, scale, X);
+    hipLaunchKernelGGL(k_ns_final, gg((int64_t)n*n), dim3(GB), 0, s, n, X1, X0, ns);
+}
+
+void launch_dense_gemm(int32_t n, const double* A, const double* B, double* C, hipStream_t s)
+{
+    const dim3 grid((unsigned)((n + DG_T - 1)/DG_T), (unsigned)((n + DG_T - 1)/DG_T));
+    hipLaunchKernelGGL(k_dense_gemm, grid, dim3(256), 0, s, n, A, B, nullptr, nullptr, C, 0,
+                       nullptr);
+}
+
+void launch_crout_lu(int32_t n, const double* Acm, const int32_t* perm, double* LU,
+                     int32_t* piv, int32_t* bad, hipStream_t s)
+{
+    if (n > 128) fail(FH_ERR_INVALID, "internal: the one-workgroup Crout LU takes n <= 128");
+    static const size_t ldsMax = [] {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+            return (size_t)0;
+        return (size_t)v;
+    }();
+    const size_t lds = sizeof(double)*(size_t)n*n;
+    if (lds + 2048 <= ldsMax)
+        hipLaunchKernelGGL(k_crout_lu<true>, dim3(1), dim3(128), lds, s, n, Acm, perm, LU, piv,
+                           bad);
+    else
+        hipLaunchKernelGGL(k_crout_lu<false>, dim3(1), dim3(128), 0, s, n, Acm, perm, LU, piv,
+                           bad);
 }
 
 template<class T>
-void launch_gemv(int32_t n, const double* X, const T* b, T* x, const DevState* st,
-                 hipStream_t s)
+void launch_gemv(int32_t n, const double* X, const T* b, T* x, double* part,
+                 const DevState* st, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_gemv<T>, gg(n), dim3(GB), 0, s, n, X, b, x, st);
+    hipLaunchKernelGGL(k_gemv_part<T>, dim3((unsigned)((n + 63)/64), GEMV_SPLIT), dim3(64), 0,
+                       s, n, X, b, part, st);
+    hipLaunchKernelGGL(k_gemv_sum<T>, gg(n), dim3(GB), 0, s, n, part, x, st);
 }
+
+int64_t gemv_work_doubles(int32_t n) { return (int64_t)GEMV_SPLIT*n; }
 
 template<class T>
 void launch_axpy_gamg(int32_t n, T* y, const T* x, const DevState* st, hipStream_t s)
@@ -733,8 +1093,8 @@ void launch_lu_backsub(int32_t n, const double* LU, const int32_t* pivot, const 
                                     hipStream_t);                                        \
     template void launch_gamg_scale<T>(const RowsView&, const T*, const T*, T*, T*, const T*, \
                                        double*, DevState*, hipStream_t);                 \
-    template void launch_gemv<T>(int32_t, const double*, const T*, T*, const DevState*,   \
-                                 hipStream_t);                                           \
+    template void launch_gemv<T>(int32_t, const double*, const T*, T*, double*,           \
+                                 const DevState*, hipStream_t);                          \
     template void launch_axpy_gamg<T>(int32_t, T*, const T*, const DevState*, hipStream_t); \
     template void launch_sub_gamg<T>(int32_t, T*, const T*, const DevState*, hipStream_t); \
     template void launch_diff_gamg<T>(int32_t, const T*, const T*, T*, const DevState*,   \
