@@ -184,22 +184,22 @@ __global__ void k_coarse_schur_form(int32_t k, int32_t P, int32_t nTI,
 // column (tc = p/16, column p mod 16) index their registers statically: a
 // step is the rank-one update of every entry (one FMA each), then the
 // owners overwrite the pivot row and column with the scaled values.
-template<class T, int C16>
-__device__ __forceinline__ bool gj_step(T (&a)[4][16], int tr, int tc, int p, T (*rowp)[256],
+template<class T, int RPT, int CPT, int CU>
+__device__ __forceinline__ bool gj_step(T (&a)[RPT][CPT], int tr, int tc, int p, T (*rowp)[256],
                                         T (*colp)[256], int32_t* bad)
 {
-    constexpr int RA = C16 & 3;                // the pivot row's register row
-    constexpr int B = C16 & 1;                 // pivot buffer: the step's parity
-    const int prow = p >> 2, pcol = p >> 4;    // owners: tr == prow, tc == pcol
+    constexpr int RA = CU % RPT;               // the pivot row's register row
+    constexpr int B = CU & 1;                  // pivot buffer: the step's parity
+    const int prow = p / RPT, pcol = p / CPT;  // owners: tr == prow, tc == pcol
     if (tr == prow)
     {
 #pragma unroll
-        for (int c = 0; c < 16; c++) rowp[B][16*tc + c] = a[RA][c];
+        for (int c = 0; c < CPT; c++) rowp[B][CPT*tc + c] = a[RA][c];
     }
     if (tc == pcol)
     {
 #pragma unroll
-        for (int ra = 0; ra < 4; ra++) colp[B][4*tr + ra] = a[ra][C16];
+        for (int ra = 0; ra < RPT; ra++) colp[B][RPT*tr + ra] = a[ra][CU];
     }
     // one barrier per step: the next step writes the other buffer, and the
     // step after it this one only once every thread is past the next
@@ -213,80 +213,94 @@ __device__ __forceinline__ bool gj_step(T (&a)[4][16], int tr, int tc, int p, T 
         return false;
     }
     const T ip = T(1)/piv;
-    T cp[4], rp[16];
+    T cp[RPT], rp[CPT];
 #pragma unroll
-    for (int ra = 0; ra < 4; ra++) cp[ra] = colp[B][4*tr + ra];
+    for (int ra = 0; ra < RPT; ra++) cp[ra] = colp[B][RPT*tr + ra];
 #pragma unroll
-    for (int c = 0; c < 16; c++) rp[c] = rowp[B][16*tc + c]*ip;
+    for (int c = 0; c < CPT; c++) rp[c] = rowp[B][CPT*tc + c]*ip;
 #pragma unroll
-    for (int ra = 0; ra < 4; ra++)
+    for (int ra = 0; ra < RPT; ra++)
 #pragma unroll
-        for (int c = 0; c < 16; c++) a[ra][c] = fma(-cp[ra], rp[c], a[ra][c]);
+        for (int c = 0; c < CPT; c++) a[ra][c] = fma(-cp[ra], rp[c], a[ra][c]);
     // the pivot row: the old row over the pivot; the pivot column: -old
     // column over the pivot; the pivot entry: its reciprocal
     if (tr == prow)
     {
 #pragma unroll
-        for (int c = 0; c < 16; c++) a[RA][c] = rp[c];
+        for (int c = 0; c < CPT; c++) a[RA][c] = rp[c];
     }
     if (tc == pcol)
     {
 #pragma unroll
-        for (int ra = 0; ra < 4; ra++) a[ra][C16] = -cp[ra]*ip;
-        if (tr == prow) a[RA][C16] = ip;
+        for (int ra = 0; ra < RPT; ra++) a[ra][CU] = -cp[ra]*ip;
+        if (tr == prow) a[RA][CU] = ip;
     }
     return true;
 }
 
-template<class T, int C16>
-__device__ __forceinline__ bool gj_unrolled(T (&a)[4][16], int tr, int tc, int base, int P,
+// the pivots base .. base+CPT-1 (p mod CPT static: the owners index their
+// registers statically; CPT is a multiple of RPT)
+template<class T, int RPT, int CPT, int CU>
+__device__ __forceinline__ bool gj_unrolled(T (&a)[RPT][CPT], int tr, int tc, int base, int P,
                                             T (*rowp)[256], T (*colp)[256], int32_t* bad)
 {
-    if constexpr (C16 < 16)
+    if constexpr (CU < CPT)
     {
-        const int p = base + C16;
+        const int p = base + CU;
         if (p >= P) return true;
-        if (!gj_step<T, C16>(a, tr, tc, p, rowp, colp, bad)) return false;
-        return gj_unrolled<T, C16 + 1>(a, tr, tc, base, P, rowp, colp, bad);
+        if (!gj_step<T, RPT, CPT, CU>(a, tr, tc, p, rowp, colp, bad)) return false;
+        return gj_unrolled<T, RPT, CPT, CU + 1>(a, tr, tc, base, P, rowp, colp, bad);
     }
     return true;
 }
 
-template<int H>
-__global__ __launch_bounds__((H/4)*(H/16))
+// one workgroup of (H/RPT)*(H/CPT) threads, thread (tr, tc) owning rows
+// RPT tr .. RPT tr + RPT-1 and columns CPT tc .. CPT tc + CPT-1
+template<int H, int RPT, int CPT>
+__global__ __launch_bounds__((H/RPT)*(H/CPT))
 void k_coarse_gj(int32_t n, const double* __restrict__ src, int64_t lds, double* __restrict__ dst,
                  int64_t ldd, int32_t* bad)
 {
-    static_assert(H <= 256 && H % 16 == 0, "Gauss-Jordan block size");
+    static_assert(H <= 256 && H % CPT == 0 && CPT % RPT == 0, "Gauss-Jordan layout");
     __shared__ __align__(16) double rowp[2][256], colp[2][256];
-    constexpr int NCG = H/16;
+    constexpr int NCG = H/CPT;
     const int tr = threadIdx.x / NCG, tc = threadIdx.x % NCG;
-    double a[4][16];
+    double a[RPT][CPT];
 #pragma unroll
-    for (int ra = 0; ra < 4; ra++)
+    for (int ra = 0; ra < RPT; ra++)
     {
-        const int i = 4*tr + ra;
+        const int i = RPT*tr + ra;
 #pragma unroll
-        for (int c = 0; c < 16; c++)
+        for (int c = 0; c < CPT; c++)
         {
-            const int j = 16*tc + c;
+            const int j = CPT*tc + c;
             a[ra][c] = (i < n && j < n) ? src[(int64_t)i*lds + j] : 0.0;
         }
     }
 #pragma unroll 1
-    for (int base = 0; base < n; base += 16)
-        if (!gj_unrolled<double, 0>(a, tr, tc, base, n, rowp, colp, bad)) return;
+    for (int base = 0; base < n; base += CPT)
+        if (!gj_unrolled<double, RPT, CPT, 0>(a, tr, tc, base, n, rowp, colp, bad)) return;
 #pragma unroll
-    for (int ra = 0; ra < 4; ra++)
+    for (int ra = 0; ra < RPT; ra++)
     {
-        const int i = 4*tr + ra;
+        const int i = RPT*tr + ra;
 #pragma unroll
-        for (int c = 0; c < 16; c++)
+        for (int c = 0; c < CPT; c++)
         {
-            const int j = 16*tc + c;
+            const int j = CPT*tc + c;
             if (i < n && j < n) dst[(int64_t)i*ldd + j] = a[ra][c];
         }
     }
+}
+
+// the Gauss-Jordan workgroup of one 128-block: 256 threads of 4 x 16
+// entries (1024 threads of 2 x 8 measured 7% slower: the steps are bound by
+// the pivot broadcast and the barrier, not by the rank-one update)
+static void launch_gj128(int32_t n, const double* src, int64_t lds, double* dst, int64_t ldd,
+                         int32_t* bad, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_coarse_gj<128, 4, 16>), dim3(1), dim3(256), 0, s, n, src, lds, dst,
+                       ldd, bad);
 }
 
 // the inverse of S = [A B; B^T D] (A: h x h) from A^-1, W = A^-1 B T^-1,
@@ -306,15 +320,20 @@ __global__ void k_coarse_assemble(int32_t P, int32_t h, const double* __restrict
     U0[e] = v;
 }
 
-// Batched FP64 block products of the recurrence, 64 x 64 outputs per
-// workgroup (256 threads, 4 x 4 each), the inner dimension in LDS chunks
-// of 32, the next chunk's loads issued before the current chunk's
-// products:  out = beta*D + alpha*op(A) diag(c) op(B)  for blockIdx.z =
-// 0 .. nb-1 blocks of M x N outputs over an inner dimension Kd (row-major,
-// leading dimensions ldA, ldB, ldD, ldO; block z of each operand at
-// z*stride).  op(X) = X^T when transX.  out must not overlap A, B or D.
+// Batched FP64 block products of the recurrence on the matrix cores
+// (v_mfma_f64_16x16x4f64):  out = beta*D + alpha*op(A) diag(c) op(B)  for
+// blockIdx.z = 0 .. nb-1 blocks of M x N outputs over an inner dimension Kd
+// (row-major, leading dimensions ldA, ldB, ldD, ldO; block z of each
+// operand at z*stride).  op(X) = X^T when transX.  out must not overlap A,
+// B or D.  A workgroup of four wavefronts owns 64 x 64 outputs, a
+// wavefront 32 x 32 (four 16 x 16 MFMA blocks); the inner dimension is
+// staged through LDS in chunks of 16 (c applied to the B chunk as it is
+// staged), the next chunk's loads issued before the current chunk's
+// products.  MFMA lane layout: lane l holds its A operand's (l%16, l/16)
+// entry and its B operand's (l/16, l%16) entry; element v of the result is
+// entry (l/16 + 4v, l%16), so a store writes 16 consecutive outputs of a row.
 #define GM_T 64
-#define GM_K 32
+#define GM_K 16
 struct BlockGemm
 {
     int32_t M, N, Kd;
@@ -327,78 +346,97 @@ struct BlockGemm
     double* out; int64_t ldO, strideO;
 };
 
-// the chunk's operands, eight values per thread each (A: 64 rows x 32 inner,
-// B: 32 inner x 64 columns)
-__device__ __forceinline__ void gemm_fetch(const BlockGemm& g, const double* Ab, const double* Bb,
-                                          int bi, int bj, int m0, double (&av)[8], double (&bv)[8])
-{
-#pragma unroll
-    for (int u = 0; u < 8; u++)
-    {
-        const int e = threadIdx.x + 256*u;
-        const int r = e / GM_K, mm = e % GM_K;
-        const int gi = bi*GM_T + r, gm = m0 + mm;
-        av[u] = (gi < g.M && gm < g.Kd)
-              ? (g.transA ? Ab[(int64_t)gm*g.ldA + gi] : Ab[(int64_t)gi*g.ldA + gm]) : 0.0;
-        const int mb = e / GM_T, cc = e % GM_T;
-        const int gm2 = m0 + mb, gj = bj*GM_T + cc;
-        bv[u] = (gm2 < g.Kd && gj < g.N)
-              ? g.c[gm2]*(g.transB ? Bb[(int64_t)gj*g.ldB + gm2] : Bb[(int64_t)gm2*g.ldB + gj])
-              : 0.0;
-    }
-}
+typedef double co_d4 __attribute__((ext_vector_type(4)));
 
 __global__ __launch_bounds__(256)
 void k_coarse_block_gemm(BlockGemm g)
 {
-    __shared__ double As[GM_K][GM_T + 1], Bs[GM_K][GM_T + 1];
-    const int bi = blockIdx.x, bj = blockIdx.y, z = blockIdx.z;
+    __shared__ double As[GM_K][GM_T + 1];      // op(A)(i0 + i, k0 + k) at [k][i]
+    __shared__ double Bs[GM_K][GM_T + 1];      // c(k) op(B)(k0 + k, j0 + j) at [k][j]
+    const int z = blockIdx.z;
     const double* Ab = g.A + z*g.strideA;
     const double* Bb = g.B + z*g.strideB;
-    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    double acc[4][4] = {};
-    double av[8], bv[8];
-    gemm_fetch(g, Ab, Bb, bi, bj, 0, av, bv);
-    for (int m0 = 0; m0 < g.Kd; m0 += GM_K)
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int i0 = blockIdx.x*GM_T, j0 = blockIdx.y*GM_T;
+    const int wi = 32*(w & 1), wj = 32*(w >> 1);
+    co_d4 acc[2][2];
+#pragma unroll
+    for (int r = 0; r < 2; r++)
+#pragma unroll
+        for (int c = 0; c < 2; c++) acc[r][c] = co_d4{0.0, 0.0, 0.0, 0.0};
+    double av[4], bv[4];
+    // the chunk's operands, four values per thread each: the index that is
+    // contiguous in memory runs fastest over the threads
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+        {
+            const int e = t + 256*u;
+            int ia, ka;
+            if (g.transA) { ia = e & 63; ka = e >> 6; }     // A(k, i): i contiguous
+            else { ka = e & 15; ia = e >> 4; }              // A(i, k): k contiguous
+            const int gi = i0 + ia, gk = k0 + ka;
+            av[u] = (gi < g.M && gk < g.Kd)
+                  ? (g.transA ? Ab[(int64_t)gk*g.ldA + gi] : Ab[(int64_t)gi*g.ldA + gk]) : 0.0;
+            int jb, kb;
+            if (g.transB) { kb = e & 15; jb = e >> 4; }     // B(j, k): k contiguous
+            else { jb = e & 63; kb = e >> 6; }              // B(k, j): j contiguous
+            const int gj = j0 + jb, gk2 = k0 + kb;
+            bv[u] = (gj < g.N && gk2 < g.Kd)
+                  ? g.c[gk2]*(g.transB ? Bb[(int64_t)gj*g.ldB + gk2] : Bb[(int64_t)gk2*g.ldB + gj])
+                  : 0.0;
+        }
+    };
+    fetch(0);
+    for (int k0 = 0; k0 < g.Kd; k0 += GM_K)
     {
 #pragma unroll
-        for (int u = 0; u < 8; u++)
+        for (int u = 0; u < 4; u++)
         {
-            const int e = threadIdx.x + 256*u;
-            As[e % GM_K][e / GM_K] = av[u];
-            Bs[e / GM_T][e % GM_T] = bv[u];
+            const int e = t + 256*u;
+            if (g.transA) As[e >> 6][e & 63] = av[u];
+            else As[e & 15][e >> 4] = av[u];
+            if (g.transB) Bs[e & 15][e >> 4] = bv[u];
+            else Bs[e >> 6][e & 63] = bv[u];
         }
         __syncthreads();
-        // the next chunk's loads in flight during this chunk's products
-        if (m0 + GM_K < g.Kd) gemm_fetch(g, Ab, Bb, bi, bj, m0 + GM_K, av, bv);
+        if (k0 + GM_K < g.Kd) fetch(k0 + GM_K);
 #pragma unroll
-        for (int mm = 0; mm < GM_K; mm++)
+        for (int kk = 0; kk < GM_K; kk += 4)
         {
-            double a4[4], b4[4];
+            const int k = kk + (lane >> 4);
+            double aop[2], bop[2];
 #pragma unroll
-            for (int u = 0; u < 4; u++) { a4[u] = As[mm][ty + 16*u]; b4[u] = Bs[mm][tx + 16*u]; }
+            for (int r = 0; r < 2; r++) aop[r] = As[k][wi + 16*r + (lane & 15)];
 #pragma unroll
-            for (int u = 0; u < 4; u++)
+            for (int c = 0; c < 2; c++) bop[c] = Bs[k][wj + 16*c + (lane & 15)];
 #pragma unroll
-                for (int v = 0; v < 4; v++) acc[u][v] = __builtin_fma(a4[u], b4[v], acc[u][v]);
+            for (int r = 0; r < 2; r++)
+#pragma unroll
+                for (int c = 0; c < 2; c++)
+                    acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(aop[r], bop[c], acc[r][c],
+                                                                    0, 0, 0);
         }
         __syncthreads();
     }
     double* Ob = g.out + z*g.strideO;
     const double* Db = g.D ? g.D + z*g.strideD : nullptr;
 #pragma unroll
-    for (int u = 0; u < 4; u++)
+    for (int r = 0; r < 2; r++)
 #pragma unroll
-        for (int v = 0; v < 4; v++)
-        {
-            const int gi = bi*GM_T + ty + 16*u, gj = bj*GM_T + tx + 16*v;
-            if (gi < g.M && gj < g.N)
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int v = 0; v < 4; v++)
             {
-                double r = g.alpha*acc[u][v];
-                if (Db) r += g.beta*Db[(int64_t)gi*g.ldD + gj];
-                Ob[(int64_t)gi*g.ldO + gj] = r;
+                const int gi = i0 + wi + 16*r + (lane >> 4) + 4*v;
+                const int gj = j0 + wj + 16*c + (lane & 15);
+                if (gi < g.M && gj < g.N)
+                {
+                    double rr = g.alpha*acc[r][c][v];
+                    if (Db) rr += g.beta*Db[(int64_t)gi*g.ldD + gj];
+                    Ob[(int64_t)gi*g.ldO + gj] = rr;
+                }
             }
-        }
 }
 
 // cvec[t] = s * (coupling of tile t to its lower-z neighbour)
@@ -492,8 +530,7 @@ static void coarse_spd_inverse(int32_t P, const double* S, double* U0, double* t
 {
     if (P <= 128)
     {
-        hipLaunchKernelGGL(k_coarse_gj<128>, dim3(1), dim3(256), 0, s, P, S, (int64_t)P, U0,
-                           (int64_t)P, bad);
+        launch_gj128(P, S, (int64_t)P, U0, (int64_t)P, bad, s);
         return;
     }
     const int32_t h = 128, h2 = P - h;
@@ -503,8 +540,7 @@ static void coarse_spd_inverse(int32_t P, const double* S, double* U0, double* t
     double* Ti = T + h2*h2;
     double* W = Ti + h2*h2;                 // h x h2: X T^-1
     double* TL = W + h*h2;                  // h x h
-    hipLaunchKernelGGL(k_coarse_gj<128>, dim3(1), dim3(256), 0, s, h, S, (int64_t)P, Ai,
-                       (int64_t)h, bad);
+    launch_gj128(h, S, (int64_t)P, Ai, (int64_t)h, bad, s);
     BlockGemm x{};
     x.M = h; x.N = h2; x.Kd = h; x.alpha = 1.0; x.c = ones;
     x.A = Ai; x.ldA = h; x.B = S + h; x.ldB = P; x.out = X; x.ldO = h2;
@@ -514,8 +550,7 @@ static void coarse_spd_inverse(int32_t P, const double* S, double* U0, double* t
     tt.A = S + h; tt.ldA = P; tt.B = X; tt.ldB = h2; tt.D = S + (int64_t)h*P + h; tt.ldD = P;
     tt.out = T; tt.ldO = h2;
     gemm_launch(tt, 1, s);
-    hipLaunchKernelGGL(k_coarse_gj<128>, dim3(1), dim3(256), 0, s, h2, T, (int64_t)h2, Ti,
-                       (int64_t)h2, bad);
+    launch_gj128(h2, T, (int64_t)h2, Ti, (int64_t)h2, bad, s);
     BlockGemm w{};
     w.M = h; w.N = h2; w.Kd = h2; w.alpha = 1.0; w.c = ones;
     w.A = X; w.ldA = h2; w.B = Ti; w.ldB = h2; w.out = W; w.ldO = h2;
