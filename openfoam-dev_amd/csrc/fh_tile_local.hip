@@ -629,7 +629,11 @@ void k_tile_dic_q(TileView G, const float* x, const float* __restrict__ rD, TlCo
     __shared__ __align__(16) float buf[2][17][16];
     const int lane = threadIdx.x;
     const int q = lane & 3, m = lane >> 2;
-    const int tile = blockIdx.x;
+    // XCD-aware order (see k_tile_dic_lds): tiles adjacent along i, whose
+    // rows share cache lines, run together on one XCD's L2
+    const int nT = G.nTI*G.nTJ*G.nTK;
+    const int tile = (nT & 7) == 0 ? (int)(blockIdx.x & 7)*(nT >> 3) + (int)(blockIdx.x >> 3)
+                                   : (int)blockIdx.x;
     const int TI = tile % G.nTI, TJ = (tile/G.nTI) % G.nTJ, TK = tile/(G.nTI*G.nTJ);
     const int ey = min(16, G.ny - TJ*16), ez = min(16, G.nz - TK*16);
     const int64_t sy = G.nx, sz = (int64_t)G.nx*G.ny;
