@@ -84,8 +84,6 @@ void launch_ns_init(int32_t n, const double* A, double* X0, double* sums, NsStat
 void launch_ns_iterations(int32_t n, const double* A, double* X0, double* X1, double* T_,
                           double* partMax, NsState* ns, int iterations, hipStream_t s);
 void launch_ns_final(int32_t n, const double* X1, double* X0, const NsState* ns, hipStream_t s);
-// C = A B, column-major n x n FP64 (the Newton-Schulz product kernel)
-void launch_dense_gemm(int32_t n, const double* A, const double* B, double* C, hipStream_t s);
 // part: gemv_work_doubles(n) doubles
 template<class T>
 void launch_gemv(int32_t n, const double* X, const T* b, T* x, double* part,

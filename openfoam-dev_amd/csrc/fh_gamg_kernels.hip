@@ -980,13 +980,6 @@ void launch_ns_final(int32_t n, const double* X1, double* X0, const NsState* ns,
     hipLaunchKernelGGL(k_ns_final, gg((int64_t)n*n), dim3(GB), 0, s, n, X1, X0, ns);
 }
 
-void launch_dense_gemm(int32_t n, const double* A, const double* B, double* C, hipStream_t s)
-{
-    const dim3 grid((unsigned)((n + DG_T - 1)/DG_T), (unsigned)((n + DG_T - 1)/DG_T));
-    hipLaunchKernelGGL(k_dense_gemm, grid, dim3(256), 0, s, n, A, B, nullptr, nullptr, C, 0,
-                       nullptr);
-}
-
 void launch_crout_lu(int32_t n, const double* Acm, const int32_t* perm, double* LU,
                      int32_t* piv, int32_t* bad, hipStream_t s)
 {
