@@ -68,6 +68,10 @@ def main():
               "domain decomposition is a design, not part of this build.", file=sys.stderr)
         sys.exit(2)
 
+    # hardware queues per process: the serving sample's concurrent solves
+    # (one stream each) share HIP's default of 4; 16 measured 45% more
+    # saturated throughput (set before the runtime starts)
+    os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
     import torch
     import foamhip as fh
     from foamhip.cases import BoxCase
@@ -270,7 +274,8 @@ def main():
         # arrivals at 0.5x, 0.8x and 1.0x the saturated rate (the full study,
         # 4096 requests per rate, is python -m foamhip.serve_bench)
         from foamhip import serve_bench
-        sb = serve_bench.benchmark(a.serve_requests, (0.5, 0.8, 1.0), pool_size=24, log=log)
+        sb = serve_bench.benchmark(a.serve_requests, (0.5, 0.8, 1.0), pool_size=24,
+                                   max_concurrent=16, log=log)
         serve = {"requests_per_rate": a.serve_requests,
                  "saturated_solves_per_s": sb["saturated"]["solves_per_s"],
                  "saturated_p50_ms": sb["saturated"]["p50_ms"],

@@ -11,11 +11,12 @@ the saturated throughput is measured (all requests queued at once), then
 Poisson arrivals at fractions of it; latency is completion minus
 submission on the server clock.
 
-    python -m foamhip.serve_bench --requests 4096 --rates 0.5,0.8,1.0
+    python -m foamhip.serve_bench --requests 4096 --rates 0.5,0.8,1.0 --concurrent 16
 """
 import argparse
 import json
 import math
+import os
 import sys
 import threading
 import time
@@ -140,11 +141,14 @@ def benchmark(n_requests=4096, rates=(0.5, 0.8, 1.0), pool_size=48, max_concurre
 
 
 def main():
+    # hardware queues for the concurrent solves' streams (HIP's default is
+    # 4); set before the runtime starts
+    os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
     p = argparse.ArgumentParser()
     p.add_argument("--requests", type=int, default=4096)
     p.add_argument("--rates", default="0.5,0.8,1.0")
     p.add_argument("--pool", type=int, default=48)
-    p.add_argument("--concurrent", type=int, default=8)
+    p.add_argument("--concurrent", type=int, default=16)
     a = p.parse_args()
     out = benchmark(a.requests, tuple(float(x) for x in a.rates.split(",")), a.pool,
                     a.concurrent, log=lambda m: print(m, file=sys.stderr, flush=True))
