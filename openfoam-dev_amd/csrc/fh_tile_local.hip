@@ -1055,12 +1055,10 @@ int32_t launch_tile_local_apply_f32(const TileView& G, const float* x, const flo
     // on one wavefront per SIMD) or the quarter-row pipeline from HBM
     // (FOAMHIP_TILE_APPLY=quarter; its per-lane quarter rows touch four
     // times the cache lines)
-    static const int mode = [] {
-        const char* e = std::getenv("FOAMHIP_TILE_APPLY");
-        if (e && std::strcmp(e, "lds") == 0) return 0;
-        if (e && std::strcmp(e, "quarter") == 0) return 1;
-        return 2;
-    }();
+    // (read at every launch, so a process can switch between them)
+    const char* env = std::getenv("FOAMHIP_TILE_APPLY");
+    const int mode = env && std::strcmp(env, "lds") == 0 ? 0
+                   : env && std::strcmp(env, "quarter") == 0 ? 1 : 2;
     // the staged tile (FP16 coefficients): 59 KB of LDS per workgroup, taken
     // when the device allows that much
     const size_t ldsBytes = (size_t)TL_LDS_CELLS*(4 + 4) + (size_t)TL_HCELLS*3*sizeof(__half);

@@ -505,3 +505,38 @@ def test_tile_dic_half_scaled(scale):
     assert perf.converged
     assert np.abs(b - om.Amul(psi)).sum() / perf.normFactor < 1e-8
     assert abs(perf.nIterations - perf1.nIterations) <= 2
+
+
+@pytest.mark.parametrize("apply", ["lds", "quarter"])
+def test_tile_dic_apply_variants(apply, monkeypatch):
+    """The other kernels of the symmetric tile-local DIC with FP16
+    coefficients (FOAMHIP_TILE_APPLY: the tile staged in LDS, the
+    quarter-row pipeline) give the row walk's result: full and partial
+    tiles (nx a multiple of 16, ny and nz not), against the oracle's
+    block-Jacobi DIC of the FP16-stored coefficients, and in a mixed PCG
+    solve (FP64 residual test)."""
+    nx, ny, nz = 48, 40, 37
+    n, lower, upper = box(nx, ny, nz)
+    rng = np.random.default_rng(23)
+    diag, up = spd_coeffs(n, lower, upper, rng)
+    mesh = fh.lduAddressing(n, lower, upper)
+    m = fh.lduMatrix(mesh, diag, up)
+    r = rng.standard_normal(n)
+    w_rows = m.precondition_f32("tileDIC", r).astype(np.float64)
+    monkeypatch.setenv("FOAMHIP_TILE_APPLY", apply)
+    w = m.precondition_f32("tileDIC", r).astype(np.float64)
+    up_t, e = _half_stored(up.astype(np.float32))
+    assert e is not None
+    cut = _tile_blocks(nx, ny, nz, lower, upper, 16)
+    om_t = O.Matrix(O.Addr(n, lower, upper), diag, np.where(cut, 0.0, up_t))
+    ref = om_t.precondition("DIC", r)
+    assert rel_l2(w, ref) <= 1e-5, rel_l2(w, ref)
+    assert rel_l2(w, w_rows) <= 1e-6, rel_l2(w, w_rows)
+    b = rng.uniform(-1, 1, n)
+    ctl = {"solver": "PCG", "preconditioner": "DIC", "tolerance": 1e-8, "relTol": 0,
+           "maxIter": 3000, "precision": "mixed"}
+    psi = np.zeros(n)
+    perf = fh.solver_New("p", m, ctl).solve(psi, b)
+    assert perf.converged
+    om = O.Matrix(O.Addr(n, lower, upper), diag, up)
+    assert np.abs(b - om.Amul(psi)).sum() / perf.normFactor < 1e-8
