@@ -14,6 +14,7 @@
 
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cmath>
 #include <cstring>
@@ -358,8 +359,17 @@ static std::shared_ptr<GamgAgglomeration> build_agglomeration(fh_matrix* M,
             const double u = std::fabs(up[f]);
             w[f] = lo.empty() ? u : std::max(u, std::fabs(lo[f]));
         }
+    // FOAMHIP_GAMG_VERBOSE: the time of each step of the hierarchy build
+    static const bool verbose = std::getenv("FOAMHIP_GAMG_VERBOSE") != nullptr;
+    using clk = std::chrono::steady_clock;
+    auto ms = [](clk::time_point a) {
+        return std::chrono::duration<double, std::milli>(clk::now() - a).count(); };
+    auto t0 = clk::now();
     std::vector<HostLevel> host = agglomerate(mesh->nCells, mesh->lower, mesh->upper, w,
                                               c->nCellsInCoarsestLevel, c->mergeLevels);
+    if (verbose)
+        std::fprintf(stderr, "foamhip GAMG agglomeration: %zu levels in %.1f ms\n",
+                     host.size(), ms(t0));
     auto A = std::make_shared<GamgAgglomeration>();
     A->minCells = c->nCellsInCoarsestLevel;
     A->mergeLevels = c->mergeLevels;
@@ -368,7 +378,10 @@ static std::shared_ptr<GamgAgglomeration> build_agglomeration(fh_matrix* M,
     for (const HostLevel& L : host)
     {
         auto D = std::make_unique<AggLevel>();
+        t0 = clk::now();
         D->mesh = make_mesh(mesh->device, L.nCoarseCells, L.cLower, L.cUpper);
+        const double meshMs = ms(t0);
+        t0 = clk::now();
         const fh_mesh* cm = D->mesh;
         D->nFine = L.nFineCells;
         D->nCoarse = L.nCoarseCells;
@@ -407,6 +420,9 @@ static std::shared_ptr<GamgAgglomeration> build_agglomeration(fh_matrix* M,
         D->fFaces.upload(list, us);
         D->fFlip.upload(flip, us);
         FH_HIP(hipStreamSynchronize(us));
+        if (verbose)
+            std::fprintf(stderr, "foamhip GAMG level %zu: mesh %.1f ms, transfer lists %.1f ms\n",
+                         A->levels.size(), meshMs, ms(t0));
         fm = cm;
         A->levels.push_back(std::move(D));
     }

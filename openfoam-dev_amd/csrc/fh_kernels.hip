@@ -577,6 +577,132 @@ void k_sweep_parts(RowsView R, SchedView S, SweepSync sync, SweepData<T> d,
     if (expired && st) st->scheduleError = 1;
 }
 
+// A row of the chunk sweep in two halves, for the Gauss-Seidel and the
+// DIC/DILU application kinds.  Before the chunk's wait: the row's shape and
+// scalars (source and diagonal, or rD and x / the forward result), the
+// columns and coefficients of its first ROW_PREFETCH entries, and (Gauss-
+// Seidel) the values of those entries that are not dependencies of this
+// sweep (upper neighbours forward, lower ones backward: no chunk of this
+// launch writes them before this row is done, so they hold the previous
+// values).  After the wait: the dependency values, all loads in flight
+// together, then the row's arithmetic in the order of sweep_row
+// (GaussSeidelSmoother.C:150-176, DICPreconditioner.C:108-122).
+#define ROW_PREFETCH 16
+
+template<int KIND>
+struct RowPreKind
+{
+    static constexpr bool gs = KIND == SWEEP_GS_FWD || KIND == SWEEP_GS_BWD;
+    static constexpr bool any = gs || KIND == SWEEP_PRECOND_FWD || KIND == SWEEP_PRECOND_BWD;
+};
+
+template<class T>
+struct RowPre
+{
+    int32_t nLo = 0, nEnt = 0, base = 0;
+    T a = T(0), b = T(1);       // GS: source, diagonal; DIC/DILU: x or w(r), rD
+    int32_t col[ROW_PREFETCH];
+    T val[ROW_PREFETCH], w[ROW_PREFETCH];
+};
+
+template<class T, int KIND, bool BACKWARD>
+__device__ __forceinline__ void row_prefetch(const RowsView& R, const SweepData<T>& d, int32_t r,
+                                             bool have, RowPre<T>& p)
+{
+    if (have)
+    {
+        const int32_t info = R.rowInfo[r];
+        p.nLo = info & 0xffff;
+        p.nEnt = info >> 16;
+        p.base = sell_base(R.sliceStart, r);
+        if constexpr (RowPreKind<KIND>::gs)
+        {
+            p.a = d.x[r];
+            p.b = d.diag[r];
+        }
+        else
+        {
+            // backward: this row's forward result, written by an earlier launch
+            p.a = KIND == SWEEP_PRECOND_FWD ? d.x[r] : d.w[r];
+            p.b = d.rD[r];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < ROW_PREFETCH; k++)
+    {
+        const bool in = k < p.nEnt;
+        const int32_t e = p.base + (k << 6);
+        p.col[k] = in ? R.col[e] : 0;
+        p.val[k] = in ? d.val[e] : T(0);
+    }
+    if constexpr (RowPreKind<KIND>::gs)
+    {
+#pragma unroll
+        for (int k = 0; k < ROW_PREFETCH; k++)
+        {
+            const bool dep = BACKWARD ? k >= p.nLo : k < p.nLo;
+            p.w[k] = k < p.nEnt && !dep ? d.w[p.col[k]] : T(0);
+        }
+    }
+}
+
+template<class T, int KIND, bool BACKWARD>
+__device__ __forceinline__ void row_finish(const RowsView& R, const SweepData<T>& d, int32_t r,
+                                           RowPre<T>& p)
+{
+#pragma unroll
+    for (int k = 0; k < ROW_PREFETCH; k++)
+    {
+        const bool dep = BACKWARD ? k >= p.nLo : k < p.nLo;
+        if (k < p.nEnt && dep) p.w[k] = sw_ld<true>(&d.w[p.col[k]]);
+    }
+    if constexpr (RowPreKind<KIND>::gs)
+    {
+        T t = p.a;
+#pragma unroll
+        for (int k = 0; k < ROW_PREFETCH; k++)
+            if (k < p.nEnt) t -= p.val[k]*p.w[k];
+        for (int32_t k = ROW_PREFETCH; k < p.nEnt; k++)
+        {
+            const int32_t e = p.base + (k << 6);
+            t -= d.val[e]*sw_ld<true>(&d.w[R.col[e]]);
+        }
+        sw_st<true>(&d.w[r], t/p.b);
+    }
+    else if constexpr (KIND == SWEEP_PRECOND_FWD)
+    {
+        // lower entries ascending
+        const T rDr = p.b;
+        T w = rDr*p.a;
+#pragma unroll
+        for (int k = 0; k < ROW_PREFETCH; k++)
+            if (k < p.nLo) w -= rDr*p.val[k]*p.w[k];
+        for (int32_t k = ROW_PREFETCH; k < p.nLo; k++)
+        {
+            const int32_t e = p.base + (k << 6);
+            w -= rDr*d.val[e]*sw_ld<true>(&d.w[R.col[e]]);
+        }
+        sw_st<true>(&d.w[r], w);
+    }
+    else
+    {
+        // upper entries descending
+        const T rDr = p.b;
+        T w = p.a;
+        // entries past the prefetched ones, down to the first upper entry
+        const int32_t kEnd = p.nLo > ROW_PREFETCH ? p.nLo : ROW_PREFETCH;
+        for (int32_t k = p.nEnt - 1; k >= kEnd; k--)
+        {
+            const int32_t e = p.base + (k << 6);
+            w -= rDr*d.val[e]*sw_ld<true>(&d.w[R.col[e]]);
+        }
+#pragma unroll
+        for (int k = ROW_PREFETCH - 1; k >= 0; k--)
+            if (k >= p.nLo && k < p.nEnt) w -= rDr*p.val[k]*p.w[k];
+        sw_st<true>(&d.w[r], w);
+    }
+}
+
 // --------------------------------------------------------------------------
 // Sync-free chunk sweep (meshes without a tile or part structure, e.g. GAMG
 // coarse levels).  The library numbering of such a mesh is sorted by
@@ -612,6 +738,10 @@ void k_sweep_chunks(RowsView R, SweepData<T> d, ChunkView C, RowSync sync,
         if (stop) continue;
         const int32_t c = BACKWARD ? C.nChunks - 1 - (int32_t)t : (int32_t)t;
         const int32_t q0 = depOff[c], q1 = depOff[c + 1];
+        const int32_t r = C.start[c] + lane;
+        const bool have = r < C.start[c + 1];
+        RowPre<T> pre;
+        if constexpr (RowPreKind<KIND>::any) row_prefetch<T, KIND, BACKWARD>(R, d, r, have, pre);
         bool expired = false;
         if (q1 > q0)
         {
@@ -635,8 +765,13 @@ void k_sweep_chunks(RowsView R, SweepData<T> d, ChunkView C, RowSync sync,
                 __builtin_amdgcn_s_sleep(1);
             }
         }
-        const int32_t r = C.start[c] + lane;
-        if (!expired && r < C.start[c + 1]) sweep_row<T, KIND, true>(R, d, r);
+        if (!expired && have)
+        {
+            if constexpr (RowPreKind<KIND>::any)
+                row_finish<T, KIND, BACKWARD>(R, d, r, pre);
+            else
+                sweep_row<T, KIND, true>(R, d, r);
+        }
         // the chunk's row stores complete before its flag is published
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         if (lane == 0)

@@ -250,14 +250,20 @@ static void sweep_any(fh_matrix* M, int kind, const SweepData<T>& d, DevState* s
         M->tileEpoch = ts.epoch;
         return;
     }
-    // one-part meshes (GAMG coarse levels): one workgroup walks the levels
-    // with the next level's rows loaded ahead, unless FOAMHIP_SWEEP asks
-    // for the chunk or part sweeps
+    // one-part meshes (GAMG coarse levels, unstructured meshes): the
+    // multi-workgroup chunk sweep where the mesh has chunks (levels of 16
+    // rows or more on average), with each row's operands loaded before its
+    // chunk's wait, so a dependency level costs one flag hand-off and one
+    // round of value loads (GAMG levels of 7k-2M rows: 1.0-3.2 ms per
+    // Gauss-Seidel sweep against 1.5-43 ms for one workgroup);
+    // narrower dependency graphs (chains) are walked by one workgroup.
+    // FOAMHIP_SWEEP=chunks or parts forces the multi-workgroup sweeps.
     static const bool multiWg = [] {
         const char* e = std::getenv("FOAMHIP_SWEEP");
         return e && (std::strcmp(e, "chunks") == 0 || std::strcmp(e, "parts") == 0);
     }();
-    if (!multiWg && M->mesh->sched.nParts == 1 && M->mesh->nFaces > 0
+    const fh_mesh* mesh = M->mesh;
+    if (!multiWg && !mesh->chunkSweep && mesh->sched.nParts == 1 && mesh->nFaces > 0
         && launch_sweep_level_wg<T>(kind, M->rows(), M->sched(), d, st, M->stream))
         return;
     if (M->rowSweep)

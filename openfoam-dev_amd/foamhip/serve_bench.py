@@ -141,15 +141,18 @@ def benchmark(n_requests=4096, rates=(0.5, 0.8, 1.0), pool_size=48, max_concurre
 
 
 def main():
-    # hardware queues for the concurrent solves' streams (HIP's default is
-    # 4); set before the runtime starts
-    os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
     p = argparse.ArgumentParser()
+    p.add_argument("--hw-queues", type=int, default=16,
+                   help="GPU_MAX_HW_QUEUES for this process (0: keep the environment's)")
     p.add_argument("--requests", type=int, default=4096)
     p.add_argument("--rates", default="0.5,0.8,1.0")
     p.add_argument("--pool", type=int, default=48)
     p.add_argument("--concurrent", type=int, default=16)
     a = p.parse_args()
+    # hardware queues for the concurrent solves' streams (HIP's default is
+    # 4); set before the runtime starts, over an exported value
+    if a.hw_queues > 0:
+        os.environ["GPU_MAX_HW_QUEUES"] = str(a.hw_queues)
     out = benchmark(a.requests, tuple(float(x) for x in a.rates.split(",")), a.pool,
                     a.concurrent, log=lambda m: print(m, file=sys.stderr, flush=True))
     print(json.dumps(out))

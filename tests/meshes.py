@@ -53,6 +53,26 @@ def random_graph(n, avg_deg, rng, shuffle_within_owner=False, connect_chain=True
     return n, lower, upper
 
 
+def hub_graph(n, avg_deg, n_hubs, hub_deg, rng):
+    """A random graph (no chain) plus hub cells, each joined to hub_deg
+    random cells below it and hub_deg above it: rows with more lower and
+    more upper neighbours than a sweep keeps in registers."""
+    pairs = set()
+    m = int(n * avg_deg / 2)
+    while len(pairs) < m:
+        a, b = rng.integers(0, n, 2)
+        if a != b:
+            pairs.add((min(a, b), max(a, b)))
+    for h in rng.choice(np.arange(n // 4, 3 * n // 4), n_hubs, replace=False):
+        for c in rng.choice(h, hub_deg, replace=False):
+            pairs.add((int(c), int(h)))
+        for c in h + 1 + rng.choice(n - h - 1, hub_deg, replace=False):
+            pairs.add((int(h), int(c)))
+    pairs = sorted(pairs)
+    return (n, np.array([p[0] for p in pairs], np.int32),
+            np.array([p[1] for p in pairs], np.int32))
+
+
 def spd_coeffs(n, lower, upper, rng, negative=False):
     """Diagonally dominant symmetric coefficients (M-matrix pattern)."""
     up = -rng.uniform(0.1, 1.0, len(lower))

@@ -8,7 +8,7 @@ import pytest
 
 import foamhip as fh
 import oracle as O
-from meshes import asym_coeffs, box, random_graph, spd_coeffs
+from meshes import asym_coeffs, box, hub_graph, random_graph, spd_coeffs
 
 pytestmark = pytest.mark.gpu
 
@@ -62,6 +62,7 @@ CASES = [
     ("rand_parts", lambda r: random_graph(500, 6, r), "parts"),
     ("rand_many_chunks", lambda r: random_graph(140000, 6, r, shuffle_within_owner=True,
                                                 connect_chain=False), None),
+    ("rand_hubs", lambda r: hub_graph(20000, 5, 12, 40, r), None),
     ("gamg_coarse", lambda r: coarse_box_level(40, 1), None),
     ("chain", lambda r: box(70, 1, 1), None),
 ]
