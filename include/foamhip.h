@@ -186,6 +186,22 @@ int         fh_smoother_lookup(const char* name, int32_t* id);
    state.  These read and reset it (reset = true). */
 int         fh_gamg_direction(void);
 void        fh_gamg_reset_direction(void);
+/* The pair-agglomeration hierarchy a GAMG solve of this matrix with these
+   controls uses (built on the device on first use and cached with the mesh,
+   as GAMGAgglomeration is cached on the mesh database): the number of
+   levels (GAMGAgglomeration::size, GAMGAgglomeration.H:325), and per
+   level sizes[4] = {fine cells, fine faces, coarse cells, coarse faces}
+   and, for each non-NULL host array, restrictAddressing [fine cells],
+   faceRestrictAddressing [fine faces] (coarse face, or -1 - coarse cell
+   for a face inside a coarse cell), faceFlipMap [fine faces] and the
+   coarse level's lduAddressing lower/upper [coarse faces]
+   (GAMGAgglomeration.H:331-367), in the caller's cell and face order. */
+int fh_gamg_agglomeration_levels(fh_matrix* matrix, const fh_controls* controls,
+                                 int32_t* nLevels);
+int fh_gamg_agglomeration_level(fh_matrix* matrix, const fh_controls* controls,
+                                int32_t level, int32_t* sizes, int32_t* restrictMap,
+                                int32_t* faceRestrict, uint8_t* faceFlip, int32_t* lower,
+                                int32_t* upper);
 
 /* ---- mesh: lduAddressing (lduAddressing.H:112-207) --------------------
    lowerAddr = owner, upperAddr = neighbour of each internal face, faces in

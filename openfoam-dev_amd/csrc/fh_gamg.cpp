@@ -2,13 +2,13 @@
 // (GAMGSolver.C:48-332, GAMGSolverSolve.C:31-552, GAMGSolverScale.C:31-76,
 // GAMGSolverAgglomerateMatrix.C:33-193, GAMGPreconditioner.C:87-148).
 //
-// The pairwise agglomeration (pairGAMGAgglomerate.C:31-301) is a greedy
-// sequential walk whose result depends on the cell order and on a
-// process-global direction toggle; it is computed on the host once per mesh
-// (the reference caches it on the mesh database, GAMGAgglomeration.C:357-399)
-// and everything that runs per solve -- coarse coefficients, restriction,
-// prolongation, scaling, smoothing and the coarsest solve -- runs on the
-// device.
+// The pairwise agglomeration (pairGAMGAgglomerate.C:31-301, fh_agglom.hip)
+// runs on the device once per mesh (the reference caches it on the mesh
+// database, GAMGAgglomeration.C:357-399), as does everything that runs per
+// solve -- coarse coefficients, restriction, prolongation, scaling,
+// smoothing and the coarsest solve.  The coarse levels' row structures and
+// sweep schedules are built on the host from the downloaded addressing.
+#include "fh_agglom.h"
 #include "fh_gamg.h"
 #include "fh_gamg_kernels.h"
 
@@ -27,136 +27,8 @@ namespace fh
 // pairGAMGAgglomeration::forward_ (pairGAMGAgglomeration.C:33)
 bool g_forward = true;
 
-// ---------------------------------------------------------------------------
-// host agglomeration
-// ---------------------------------------------------------------------------
-
-// pairGAMGAgglomerate.C:123-301: greedy pairing over the cells (forward or
-// backward), pairing with the free neighbour of largest face weight, else
-// joining the neighbouring cluster across the heaviest face, else a
-// singleton; the direction flips after every call.
-static int32_t pair_agglomerate(int32_t nCells, const std::vector<int32_t>& lower,
-                                const std::vector<int32_t>& upper,
-                                const std::vector<double>& weights,
-                                std::vector<int32_t>& cmap)
-{
-    const int32_t nF = (int32_t)lower.size();
-    std::vector<int32_t> off(nCells + 1, 0), cellFaces(2*(size_t)nF);
-    for (int32_t f = 0; f < nF; f++) { off[upper[f] + 1]++; off[lower[f] + 1]++; }
-    for (int32_t c = 0; c < nCells; c++) off[c + 1] += off[c];
-    {
-        std::vector<int32_t> fill(off.begin(), off.end() - 1);
-        // a cell's faces: those where it is the upper address first, then
-        // those where it is the lower address, each in face order
-        for (int32_t f = 0; f < nF; f++) cellFaces[fill[upper[f]]++] = f;
-        for (int32_t f = 0; f < nF; f++) cellFaces[fill[lower[f]]++] = f;
-    }
-    cmap.assign(nCells, -1);
-    int32_t nCoarse = 0;
-    const double great = 1.0/2.220446049250313e-16;
-    for (int32_t cf = 0; cf < nCells; cf++)
-    {
-        const int32_t c = g_forward ? cf : nCells - cf - 1;
-        if (cmap[c] >= 0) continue;
-        int32_t match = -1;
-        double best = -great;
-        for (int32_t q = off[c]; q < off[c + 1]; q++)
-        {
-            const int32_t f = cellFaces[q];
-            if (cmap[upper[f]] < 0 && cmap[lower[f]] < 0 && weights[f] > best)
-            {
-                match = f;
-                best = weights[f];
-            }
-        }
-        if (match >= 0)
-        {
-            cmap[upper[match]] = nCoarse;
-            cmap[lower[match]] = nCoarse;
-            nCoarse++;
-        }
-        else
-        {
-            int32_t cm = -1;
-            double cbest = -great;
-            for (int32_t q = off[c]; q < off[c + 1]; q++)
-            {
-                const int32_t f = cellFaces[q];
-                if (weights[f] > cbest) { cm = f; cbest = weights[f]; }
-            }
-            if (cm >= 0) cmap[c] = std::max(cmap[upper[cm]], cmap[lower[cm]]);
-        }
-    }
-    for (int32_t cf = 0; cf < nCells; cf++)
-    {
-        const int32_t c = g_forward ? cf : nCells - cf - 1;
-        if (cmap[c] < 0) cmap[c] = nCoarse++;
-    }
-    if (!g_forward)
-    {
-        const int32_t last = nCoarse - 1;
-        for (auto& v : cmap) v = last - v;
-    }
-    g_forward = !g_forward;
-    return nCoarse;
-}
-
-// GAMGAgglomerateLduAddressing.C:32-239: coarse faces numbered by coarse
-// owner, within an owner in order of first appearance in the fine faces;
-// fine faces inside a coarse cell map to -1-cell.
-static void coarse_addressing(const std::vector<int32_t>& lower,
-                              const std::vector<int32_t>& upper,
-                              const std::vector<int32_t>& rmap, int32_t nCoarse,
-                              std::vector<int32_t>& cLower, std::vector<int32_t>& cUpper,
-                              std::vector<int32_t>& faceRestrict, std::vector<uint8_t>& flip)
-{
-    const int32_t nF = (int32_t)lower.size();
-    faceRestrict.assign(nF, 0);
-    flip.assign(nF, 0);
-    // per coarse owner: list of (neighbour, provisional id) in discovery order
-    std::vector<std::vector<std::pair<int32_t, int32_t>>> own(nCoarse);
-    std::vector<int32_t> provOwner, provNei;
-    for (int32_t f = 0; f < nF; f++)
-    {
-        const int32_t ru = rmap[upper[f]], rl = rmap[lower[f]];
-        if (ru == rl)
-        {
-            faceRestrict[f] = -(ru + 1);
-            continue;
-        }
-        const int32_t cOwn = std::min(ru, rl), cNei = std::max(ru, rl);
-        int32_t id = -1;
-        for (const auto& pr : own[cOwn])
-            if (pr.first == cNei) { id = pr.second; break; }
-        if (id < 0)
-        {
-            id = (int32_t)provOwner.size();
-            own[cOwn].push_back({cNei, id});
-            provOwner.push_back(cOwn);
-            provNei.push_back(cNei);
-        }
-        faceRestrict[f] = id;
-    }
-    std::vector<int32_t> renum(provOwner.size());
-    cLower.clear();
-    cUpper.clear();
-    for (int32_t c = 0; c < nCoarse; c++)
-        for (const auto& pr : own[c])
-        {
-            renum[pr.second] = (int32_t)cLower.size();
-            cLower.push_back(c);
-            cUpper.push_back(pr.first);
-        }
-    for (int32_t f = 0; f < nF; f++)
-    {
-        if (faceRestrict[f] < 0) continue;
-        const int32_t cf = renum[faceRestrict[f]];
-        faceRestrict[f] = cf;
-        // flipped when the fine upper cell lands on the coarse owner
-        flip[f] = (cLower[cf] == rmap[upper[f]] && cUpper[cf] == rmap[lower[f]]) ? 1 : 0;
-    }
-}
-
+// The maps of one hierarchy level, downloaded for the coarse mesh build
+// and the per-level transfer lists
 struct HostLevel
 {
     int32_t nFineCells, nCoarseCells;
@@ -166,70 +38,11 @@ struct HostLevel
     std::vector<int32_t> cLower, cUpper;   // coarse addressing
 };
 
-// pairGAMGAgglomerate.C:31-118 with combineLevels
-// (GAMGAgglomerateLduAddressing.C:493-611) for mergeLevels > 1 and
-// continueAgglomerating (GAMGAgglomeration.C:205-230)
-static std::vector<HostLevel> agglomerate(int32_t nCells, const std::vector<int32_t>& lower,
-                                          const std::vector<int32_t>& upper,
-                                          std::vector<double> weights, int32_t minCells,
-                                          int32_t mergeLevels)
+template<class T>
+static void download(const DBuf<T>& d, size_t n, std::vector<T>& h, hipStream_t s)
 {
-    const int maxLevels = 50;
-    if (mergeLevels < 1) mergeLevels = 1;
-    std::vector<HostLevel> levels;
-    std::vector<int32_t> curLower = lower, curUpper = upper;
-    int32_t curN = nCells;
-    int nPairLevels = 0;
-    while ((int)levels.size() < maxLevels - 1)
-    {
-        std::vector<int32_t> rmap;
-        const int32_t nCoarse = pair_agglomerate(curN, curLower, curUpper, weights, rmap);
-        if (!(nCoarse >= minCells && nCoarse < curN)) break;
-        HostLevel L;
-        L.nFineCells = curN;
-        L.nCoarseCells = nCoarse;
-        L.restrictMap = rmap;
-        coarse_addressing(curLower, curUpper, rmap, nCoarse, L.cLower, L.cUpper,
-                          L.faceRestrict, L.faceFlip);
-        // restrictFaceField of the weights (GAMGAgglomerationTemplates.C:137-167)
-        std::vector<double> cw(L.cLower.size(), 0.0);
-        for (size_t f = 0; f < L.faceRestrict.size(); f++)
-            if (L.faceRestrict[f] >= 0) cw[L.faceRestrict[f]] += weights[f];
-        weights.swap(cw);
-        curLower = L.cLower;
-        curUpper = L.cUpper;
-        curN = nCoarse;
-        if (nPairLevels % mergeLevels)
-        {
-            // fold the new level into the previous one
-            HostLevel& P = levels.back();
-            for (size_t i = 0; i < P.faceRestrict.size(); i++)
-            {
-                if (P.faceRestrict[i] >= 0)
-                {
-                    const int32_t ff = P.faceRestrict[i];
-                    P.faceRestrict[i] = L.faceRestrict[ff];
-                    P.faceFlip[i] = L.faceFlip[ff];
-                }
-                else
-                {
-                    const int32_t fc = -P.faceRestrict[i] - 1;
-                    P.faceRestrict[i] = -L.restrictMap[fc] - 1;
-                    P.faceFlip[i] = L.faceFlip[fc];
-                }
-            }
-            for (auto& v : P.restrictMap) v = L.restrictMap[v];
-            P.nCoarseCells = L.nCoarseCells;
-            P.cLower = L.cLower;
-            P.cUpper = L.cUpper;
-        }
-        else
-        {
-            levels.push_back(std::move(L));
-        }
-        nPairLevels++;
-    }
-    return levels;
+    h.resize(n);
+    if (n) FH_HIP(hipMemcpyAsync(h.data(), d.p, n*sizeof(T), hipMemcpyDeviceToHost, s));
 }
 
 // ---------------------------------------------------------------------------
@@ -264,6 +77,7 @@ static fh_mesh* make_mesh(int device, int32_t n, const std::vector<int32_t>& lo,
 
 struct AggLevel
 {
+    std::unique_ptr<AggMaps> maps;      // the level's maps in natural numbering
     fh_mesh* mesh = nullptr;            // the coarse mesh of this level
     int32_t nFine = 0, nCoarse = 0, nCoarseFaces = 0;
     DBuf<int32_t> rStart, rCells;       // coarse library row -> fine library rows
@@ -339,34 +153,36 @@ static std::shared_ptr<GamgAgglomeration> build_agglomeration(fh_matrix* M,
     if (byArea && (int32_t)mesh->faceAreaWeights.size() != F)
         fail(FH_ERR_INVALID, "agglomerator faceAreaPair needs the mesh's face areas "
                              "(fh_mesh_set_face_areas)");
-    std::vector<double> up(byArea ? 0 : F), lo;
-    if (!byArea && F > 0)
-        FH_HIP(hipMemcpyAsync(up.data(), M->d_upperF.p, sizeof(double)*F,
-                              hipMemcpyDeviceToHost, s));
-    if (!byArea && !M->symmetric && F > 0)
-    {
-        lo.resize(F);
-        FH_HIP(hipMemcpyAsync(lo.data(), M->d_lowerF.p, sizeof(double)*F,
-                              hipMemcpyDeviceToHost, s));
-    }
-    FH_HIP(hipStreamSynchronize(s));
-    std::vector<double> w(F);
+    DBuf<double> w;
+    w.alloc(std::max(F, 1));
     if (byArea)
-        w = mesh->faceAreaWeights;
+        w.upload(mesh->faceAreaWeights, s);
     else
-        for (int32_t f = 0; f < F; f++)
-        {
-            const double u = std::fabs(up[f]);
-            w[f] = lo.empty() ? u : std::max(u, std::fabs(lo[f]));
-        }
+        launch_agg_weights(F, M->d_upperF.p, M->symmetric ? nullptr : M->d_lowerF.p, w.p, s);
     // FOAMHIP_GAMG_VERBOSE: the time of each step of the hierarchy build
     static const bool verbose = std::getenv("FOAMHIP_GAMG_VERBOSE") != nullptr;
     using clk = std::chrono::steady_clock;
     auto ms = [](clk::time_point a) {
         return std::chrono::duration<double, std::milli>(clk::now() - a).count(); };
     auto t0 = clk::now();
-    std::vector<HostLevel> host = agglomerate(mesh->nCells, mesh->lower, mesh->upper, w,
-                                              c->nCellsInCoarsestLevel, c->mergeLevels);
+    std::vector<HostLevel> host;
+    auto dev = device_agglomerate(mesh->nCells, F, mesh->d_lower.p, mesh->d_upper.p, w.p,
+                                  c->nCellsInCoarsestLevel, c->mergeLevels, g_forward, s);
+    {
+        for (const auto& D : dev)
+        {
+            HostLevel L;
+            L.nFineCells = D->nFine;
+            L.nCoarseCells = D->nCoarse;
+            download(D->restrictMap, D->nFine, L.restrictMap, s);
+            download(D->faceRestrict, D->nFineFaces, L.faceRestrict, s);
+            download(D->faceFlip, D->nFineFaces, L.faceFlip, s);
+            download(D->lower, D->nCoarseFaces, L.cLower, s);
+            download(D->upper, D->nCoarseFaces, L.cUpper, s);
+            host.push_back(std::move(L));
+        }
+        FH_HIP(hipStreamSynchronize(s));
+    }
     if (verbose)
         std::fprintf(stderr, "foamhip GAMG agglomeration: %zu levels in %.1f ms\n",
                      host.size(), ms(t0));
@@ -375,9 +191,11 @@ static std::shared_ptr<GamgAgglomeration> build_agglomeration(fh_matrix* M,
     A->mergeLevels = c->mergeLevels;
     A->kind = c->agglomerator;
     const fh_mesh* fm = mesh;
-    for (const HostLevel& L : host)
+    for (size_t li = 0; li < host.size(); li++)
     {
+        const HostLevel& L = host[li];
         auto D = std::make_unique<AggLevel>();
+        D->maps = std::move(dev[li]);
         t0 = clk::now();
         D->mesh = make_mesh(mesh->device, L.nCoarseCells, L.cLower, L.cUpper);
         const double meshMs = ms(t0);
@@ -658,20 +476,24 @@ void gamg_set_face_weights(fh_mesh* mesh, std::vector<double>&& w)
     mesh->gamgAgg.reset();
 }
 
-void gamg_prepare(fh_matrix* M, const fh_controls* c)
+// the mesh's cached agglomeration for these controls, built on first use
+static std::shared_ptr<GamgAgglomeration> agglomeration_of(fh_matrix* M, const fh_controls* c)
 {
     fh_mesh* mesh = M->mesh;
-    std::shared_ptr<GamgAgglomeration> agg;
+    std::lock_guard<std::mutex> lk(g_aggMutex);
+    auto agg = std::static_pointer_cast<GamgAgglomeration>(mesh->gamgAgg);
+    if (!agg || agg->minCells != c->nCellsInCoarsestLevel
+        || agg->mergeLevels != c->mergeLevels || agg->kind != c->agglomerator)
     {
-        std::lock_guard<std::mutex> lk(g_aggMutex);
-        agg = std::static_pointer_cast<GamgAgglomeration>(mesh->gamgAgg);
-        if (!agg || agg->minCells != c->nCellsInCoarsestLevel
-            || agg->mergeLevels != c->mergeLevels || agg->kind != c->agglomerator)
-        {
-            agg = build_agglomeration(M, c);
-            mesh->gamgAgg = agg;
-        }
+        agg = build_agglomeration(M, c);
+        mesh->gamgAgg = agg;
     }
+    return agg;
+}
+
+void gamg_prepare(fh_matrix* M, const fh_controls* c)
+{
+    std::shared_ptr<GamgAgglomeration> agg = agglomeration_of(M, c);
     GamgHierarchy* H = static_cast<GamgHierarchy*>(M->gamg);
     if (H && H->agg != agg)
     {
@@ -920,6 +742,68 @@ void gamg_solve(fh_matrix* M, const fh_controls* c, double* psi, const double* s
 }
 
 } // namespace fh
+
+// GAMGAgglomeration's level maps (GAMGAgglomeration.H:325-367) as seen by
+// the C-ABI
+namespace
+{
+struct AggQuery
+{
+    fh_matrix* A;
+    const fh_controls* c;
+    int32_t level;
+    int32_t* out;
+    int32_t *restrictMap, *faceRestrict, *lower, *upper;
+    uint8_t* faceFlip;
+};
+} // namespace
+
+extern "C" int fh_gamg_agglomeration_levels(fh_matrix* A, const fh_controls* c,
+                                            int32_t* nLevels)
+{
+    AggQuery q{A, c, 0, nLevels, nullptr, nullptr, nullptr, nullptr, nullptr};
+    return fh::guarded_call([](void* p) {
+        AggQuery& q = *static_cast<AggQuery*>(p);
+        if (!q.A || !q.c || !q.out) fh::fail(FH_ERR_INVALID, "NULL argument");
+        if (q.A->mesh->nInterfaces > 0)
+            fh::fail(FH_ERR_UNSUPPORTED, "GAMG agglomeration of coupled interfaces");
+        *q.out = (int32_t)fh::agglomeration_of(q.A, q.c)->levels.size();
+    }, &q);
+}
+
+extern "C" int fh_gamg_agglomeration_level(fh_matrix* A, const fh_controls* c, int32_t level,
+                                           int32_t* sizes, int32_t* restrictMap,
+                                           int32_t* faceRestrict, uint8_t* faceFlip,
+                                           int32_t* lower, int32_t* upper)
+{
+    AggQuery q{A, c, level, sizes, restrictMap, faceRestrict, lower, upper, faceFlip};
+    return fh::guarded_call([](void* p) {
+        AggQuery& q = *static_cast<AggQuery*>(p);
+        if (!q.A || !q.c || !q.out) fh::fail(FH_ERR_INVALID, "NULL argument");
+        if (q.A->mesh->nInterfaces > 0)
+            fh::fail(FH_ERR_UNSUPPORTED, "GAMG agglomeration of coupled interfaces");
+        auto agg = fh::agglomeration_of(q.A, q.c);
+        if (q.level < 0 || q.level >= (int32_t)agg->levels.size())
+            fh::fail(FH_ERR_INVALID, "agglomeration level %d of %zu", q.level,
+                     agg->levels.size());
+        const fh::AggMaps& m = *agg->levels[q.level]->maps;
+        q.out[0] = m.nFine;
+        q.out[1] = m.nFineFaces;
+        q.out[2] = m.nCoarse;
+        q.out[3] = m.nCoarseFaces;
+        hipStream_t s = q.A->stream;
+        auto get = [&](auto* dst, const auto& src, size_t n) {
+            if (dst && n)
+                FH_HIP(hipMemcpyAsync(dst, src.p, n*sizeof(*dst), hipMemcpyDeviceToHost, s));
+        };
+        get(q.restrictMap, m.restrictMap, (size_t)m.nFine);
+        get(q.faceRestrict, m.faceRestrict, (size_t)m.nFineFaces);
+        get(q.faceFlip, m.faceFlip, (size_t)m.nFineFaces);
+        get(q.lower, m.lower, (size_t)m.nCoarseFaces);
+        get(q.upper, m.upper, (size_t)m.nCoarseFaces);
+        FH_HIP(hipStreamSynchronize(s));
+    }, &q);
+}
 
 // pairGAMGAgglomeration::forward_ as seen by the C-ABI
 extern "C" int fh_gamg_direction(void)

@@ -254,6 +254,32 @@ class lduMatrix:
                                          _ptr(out)))
         return out
 
+    def agglomeration(self, controls):
+        """The pair-agglomeration hierarchy a GAMG solve with these controls
+        ("agglomerator", "nCellsInCoarsestLevel", "mergeLevels") uses, one
+        dict per level with GAMGAgglomeration's maps (GAMGAgglomeration.H:
+        325-367): restrictMap, faceRestrict, faceFlip and the coarse
+        lower/upper addressing."""
+        c = controls_from_dict(dict({"smoother": "GaussSeidel"}, **controls, solver="GAMG"))
+        n = ctypes.c_int32()
+        _check(lib().fh_gamg_agglomeration_levels(self.handle, ctypes.byref(c), ctypes.byref(n)))
+        out = []
+        for lev in range(n.value):
+            sz = np.zeros(4, np.int32)
+            _check(lib().fh_gamg_agglomeration_level(self.handle, ctypes.byref(c), lev,
+                                                     _ptr(sz), None, None, None, None, None))
+            nF, nFF, nC, nCF = (int(v) for v in sz)
+            rm, fr = np.empty(nF, np.int32), np.empty(nFF, np.int32)
+            ff = np.empty(nFF, np.uint8)
+            lo, up = np.empty(nCF, np.int32), np.empty(nCF, np.int32)
+            _check(lib().fh_gamg_agglomeration_level(self.handle, ctypes.byref(c), lev,
+                                                     _ptr(sz), _ptr(rm), _ptr(fr), _ptr(ff),
+                                                     _ptr(lo), _ptr(up)))
+            out.append({"nFineCells": nF, "nFineFaces": nFF, "nCoarseCells": nC,
+                        "nCoarseFaces": nCF, "restrictMap": rm, "faceRestrict": fr,
+                        "faceFlip": ff, "lower": lo, "upper": up})
+        return out
+
     def smooth(self, smoother, psi, source, nSweeps):
         psi = _f64(psi).copy()
         source = _f64(source)

@@ -92,6 +92,10 @@ FH_SYMBOLS = {
     "fh_smoother_lookup": (_i32, [_cp, ctypes.POINTER(_i32)]),
     "fh_gamg_direction": (_i32, []),
     "fh_gamg_reset_direction": (None, []),
+    "fh_gamg_agglomeration_levels": (_i32, [_vp, ctypes.POINTER(FhControls),
+                                            ctypes.POINTER(_i32)]),
+    "fh_gamg_agglomeration_level": (_i32, [_vp, ctypes.POINTER(FhControls), _i32, _vp, _vp,
+                                           _vp, _vp, _vp, _vp]),
     "fh_mesh_create": (_i32, [_i32, _i32, _vp, _vp, ctypes.POINTER(_vp)]),
     "fh_mesh_create_coupled": (_i32, [_i32, _i32, _vp, _vp, _i32, _vp, _vp, _vp,
                                       ctypes.POINTER(_vp)]),

@@ -540,3 +540,78 @@ def test_tile_dic_apply_variants(apply, monkeypatch):
     assert perf.converged
     om = O.Matrix(O.Addr(n, lower, upper), diag, up)
     assert np.abs(b - om.Amul(psi)).sum() / perf.normFactor < 1e-8
+
+
+AGG_CASES = [
+    ("box", lambda r: box(19, 14, 11), "spd", 1),
+    ("box_merge2", lambda r: box(24, 20, 9), "spd", 2),
+    ("box_merge3", lambda r: box(17, 16, 15), "spd", 3),
+    ("rand", lambda r: random_graph(3000, 6, r), "spd", 1),
+    ("rand_shuffled_merge2", lambda r: random_graph(2500, 5, r, shuffle_within_owner=True), "spd", 2),
+    ("asym", lambda r: box(15, 13, 12), "asym", 1),
+]
+
+
+@pytest.mark.parametrize("name,gen,kind,merge", AGG_CASES)
+@pytest.mark.parametrize("start_backward", [False, True])
+def test_agglomeration_maps_equal_oracle(name, gen, kind, merge, start_backward):
+    """pairGAMGAgglomerate.C:123-301 and GAMGAgglomerateLduAddressing.C:
+    32-239, 514-611 on the device: every level's restrictAddressing,
+    faceRestrictAddressing, faceFlipMap and coarse addressing equal the
+    oracle's sequential walk (the direction toggle starting either way)."""
+    rng = np.random.default_rng(31)
+    n, lo, up = gen(rng)
+    if kind == "spd":
+        d, u = spd_coeffs(n, lo, up, rng)
+        l = None
+    else:
+        d, u, l = asym_coeffs(n, lo, up, rng)
+    ctl = {"nCellsInCoarsestLevel": 8, "mergeLevels": merge}
+    O.reset_direction()
+    fh.lib().fh_gamg_reset_direction()
+    if start_backward:
+        # one pair level of a throwaway two-cell mesh (one coarse cell, below
+        # nCellsInCoarsestLevel 2: no level kept) flips both toggles once
+        tiny = {"nCellsInCoarsestLevel": 2, "mergeLevels": 1}
+        assert O.agglomerate(O.Matrix(O.Addr(*box(2, 1, 1)), np.full(2, 4.0), np.ones(1)),
+                             tiny) == []
+        tm = fh.lduMatrix(fh.lduAddressing(*box(2, 1, 1)), np.full(2, 4.0), np.ones(1))
+        assert tm.agglomeration(tiny) == []
+        assert fh.lib().fh_gamg_direction() == 0
+    ref = O.agglomerate(O.Matrix(O.Addr(n, lo, up), d, u, l), ctl)
+    got = fh.lduMatrix(fh.lduAddressing(n, lo, up), d, u, l).agglomeration(ctl)
+    assert len(got) == len(ref) >= 2
+    for g, r in zip(got, ref):
+        for key in ("nFineCells", "nCoarseCells", "nCoarseFaces"):
+            assert g[key] == r[key], key
+        for key in ("restrictMap", "faceRestrict", "lower", "upper"):
+            np.testing.assert_array_equal(g[key], r[key], err_msg=key)
+        # the flip of a face inside a coarse cell is not used (the face goes
+        # to the diagonal)
+        ext = r["faceRestrict"] >= 0
+        np.testing.assert_array_equal(g["faceFlip"][ext], r["faceFlip"][ext].astype(np.uint8))
+    assert fh.lib().fh_gamg_direction() == O.get_direction()
+
+
+def test_agglomeration_face_area_ties():
+    """faceAreaPair on a uniform box: every weight ties, so the choice is the
+    first face of each cell's list (the cellFaces order)."""
+    n, lo, up = box(12, 10, 8)
+    rng = np.random.default_rng(32)
+    d, u = spd_coeffs(n, lo, up, rng)
+    Sf = np.zeros((len(lo), 3))
+    Sf[:, 0] = 1.0
+    magSf = np.ones(len(lo))
+    ctl = {"nCellsInCoarsestLevel": 8, "mergeLevels": 1, "agglomerator": "faceAreaPair"}
+    O.reset_direction()
+    ref = O.agglomerate(O.Matrix(O.Addr(n, lo, up), d, u), dict(ctl, Sf=Sf, magSf=magSf))
+    mesh = fh.lduAddressing(n, lo, up)
+    mesh.set_face_areas(Sf, magSf)
+    fh.lib().fh_gamg_reset_direction()
+    got = fh.lduMatrix(mesh, d, u).agglomeration(ctl)
+    assert len(got) == len(ref)
+    for g, r in zip(got, ref):
+        np.testing.assert_array_equal(g["restrictMap"], r["restrictMap"])
+        np.testing.assert_array_equal(g["faceRestrict"], r["faceRestrict"])
+        np.testing.assert_array_equal(g["lower"], r["lower"])
+        np.testing.assert_array_equal(g["upper"], r["upper"])
