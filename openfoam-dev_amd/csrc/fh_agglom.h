@@ -34,6 +34,21 @@ std::vector<std::unique_ptr<AggMaps>> device_agglomerate(int32_t nCells, int32_t
                                                          int32_t minCells, int32_t mergeLevels,
                                                          bool& forward, hipStream_t s);
 
+// The transfer lists of one level in the library row numbering of the fine
+// and coarse meshes (perm[row] = natural cell): rStart/rCells the fine rows
+// of each coarse row (ascending fine natural order), prolong the coarse row
+// of each fine row, iStart/iFaces the fine faces inside each coarse row's
+// cell, fStart/fFaces/fFlip the fine faces of each coarse face (coarse face
+// order); every list ascending, the order the reference adds in
+// (GAMGAgglomerationTemplates.C:76-219, GAMGSolverAgglomerateMatrix.C:33-193).
+struct TransferLists
+{
+    DBuf<int32_t> rStart, rCells, prolong, iStart, iFaces, fStart, fFaces;
+    DBuf<uint8_t> fFlip;
+};
+void device_transfer_lists(const AggMaps& maps, const int32_t* finePerm,
+                           const int32_t* coarsePerm, TransferLists& T, hipStream_t s);
+
 // The algebraicPair face weights of a matrix: |upper|, or max(|upper|,
 // |lower|) when lowerF is given (algebraicPairGAMGAgglomeration.C:57-64).
 void launch_agg_weights(int32_t nF, const double* upperF, const double* lowerF, double* w,

@@ -622,6 +622,159 @@ void launch_agg_weights(int32_t nF, const double* upperF, const double* lowerF, 
 }
 
 // ---------------------------------------------------------------------------
+// grouped lists (the per-level transfer lists of the V-cycle)
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(AG_BLOCK)
+void k_bucket_count(int32_t nItems, const int32_t* __restrict__ key, int32_t* __restrict__ cnt)
+{
+    const int32_t i = (int32_t)(blockIdx.x*AG_BLOCK + threadIdx.x);
+    if (i < nItems && key[i] >= 0) atomicAdd(&cnt[key[i]], 1);
+}
+
+__global__ __launch_bounds__(AG_BLOCK)
+void k_bucket_rows(int32_t nRows, const int32_t* __restrict__ rowKey,
+                   const int32_t* __restrict__ cnt, int32_t* __restrict__ cntRow,
+                   int32_t* __restrict__ keyRow)
+{
+    const int32_t r = (int32_t)(blockIdx.x*AG_BLOCK + threadIdx.x);
+    if (r >= nRows) return;
+    const int32_t k = rowKey ? rowKey[r] : r;
+    cntRow[r] = cnt[k];
+    keyRow[k] = r;
+}
+
+__global__ __launch_bounds__(AG_BLOCK)
+void k_bucket_fill(int32_t nItems, const int32_t* __restrict__ key,
+                   const int32_t* __restrict__ keyRow, const int32_t* __restrict__ start,
+                   int32_t* __restrict__ fillCnt, int32_t* __restrict__ list)
+{
+    const int32_t i = (int32_t)(blockIdx.x*AG_BLOCK + threadIdx.x);
+    if (i >= nItems || key[i] < 0) return;
+    const int32_t r = keyRow[key[i]];
+    list[start[r] + atomicAdd(&fillCnt[r], 1)] = i;
+}
+
+__global__ __launch_bounds__(AG_BLOCK)
+void k_bucket_sort(int32_t nRows, const int32_t* __restrict__ start, int32_t* __restrict__ list)
+{
+    const int32_t r = (int32_t)(blockIdx.x*AG_BLOCK + threadIdx.x);
+    if (r < nRows) ag_isort(list + start[r], start[r + 1] - start[r]);
+}
+
+void device_bucket(int32_t nItems, const int32_t* key, int32_t nRows, const int32_t* rowKey,
+                   DBuf<int32_t>& start, DBuf<int32_t>& list, hipStream_t s)
+{
+    DBuf<int32_t> cnt, cntRow, keyRow, fillCnt;
+    cnt.alloc(std::max(nRows, 1));
+    cntRow.alloc(std::max(nRows, 1));
+    keyRow.alloc(std::max(nRows, 1));
+    fillCnt.alloc(std::max(nRows, 1));
+    fill(cnt.p, nRows, 0, s);
+    fill(fillCnt.p, nRows, 0, s);
+    if (nItems > 0)
+        hipLaunchKernelGGL(k_bucket_count, dim3(ag_grid(nItems)), dim3(AG_BLOCK), 0, s, nItems,
+                           key, cnt.p);
+    if (nRows > 0)
+        hipLaunchKernelGGL(k_bucket_rows, dim3(ag_grid(nRows)), dim3(AG_BLOCK), 0, s, nRows,
+                           rowKey, cnt.p, cntRow.p, keyRow.p);
+    start.alloc((size_t)nRows + 1);
+    device_exclusive_scan(cntRow.p, start.p, nRows, s);
+    const int32_t total = read_int(start.p + nRows, s);
+    list.alloc(std::max(total, 1));
+    if (nItems > 0)
+        hipLaunchKernelGGL(k_bucket_fill, dim3(ag_grid(nItems)), dim3(AG_BLOCK), 0, s, nItems,
+                           key, keyRow.p, start.p, fillCnt.p, list.p);
+    if (nRows > 0)
+        hipLaunchKernelGGL(k_bucket_sort, dim3(ag_grid(nRows)), dim3(AG_BLOCK), 0, s, nRows,
+                           start.p, list.p);
+    FH_HIP(hipStreamSynchronize(s));
+}
+
+__global__ __launch_bounds__(AG_BLOCK)
+void k_inverse_perm(int32_t n, const int32_t* __restrict__ perm, int32_t* __restrict__ iperm)
+{
+    const int32_t r = (int32_t)(blockIdx.x*AG_BLOCK + threadIdx.x);
+    if (r < n) iperm[perm[r]] = r;
+}
+
+// out[i] = map[in[i]]
+__global__ __launch_bounds__(AG_BLOCK)
+void k_gather_map(int32_t n, const int32_t* __restrict__ in, const int32_t* __restrict__ map,
+                  int32_t* __restrict__ out)
+{
+    const int32_t i = (int32_t)(blockIdx.x*AG_BLOCK + threadIdx.x);
+    if (i < n) out[i] = map[in[i]];
+}
+
+__global__ __launch_bounds__(AG_BLOCK)
+void k_gather_u8(int32_t n, const int32_t* __restrict__ in, const uint8_t* __restrict__ src,
+                 uint8_t* __restrict__ out)
+{
+    const int32_t i = (int32_t)(blockIdx.x*AG_BLOCK + threadIdx.x);
+    if (i < n) out[i] = src[in[i]];
+}
+
+// keys of the fine faces: the coarse cell of a face inside one (inner) or
+// the coarse face (outer), -1 otherwise
+__global__ __launch_bounds__(AG_BLOCK)
+void k_face_keys(int32_t nF, const int32_t* __restrict__ faceRestrict, int32_t inner,
+                 int32_t* __restrict__ key)
+{
+    const int32_t f = (int32_t)(blockIdx.x*AG_BLOCK + threadIdx.x);
+    if (f >= nF) return;
+    const int32_t r = faceRestrict[f];
+    key[f] = inner ? (r < 0 ? -r - 1 : -1) : (r >= 0 ? r : -1);
+}
+
+__global__ __launch_bounds__(AG_BLOCK)
+void k_prolong_rows(int32_t nFine, const int32_t* __restrict__ finePerm,
+                    const int32_t* __restrict__ restrictMap,
+                    const int32_t* __restrict__ coarseIperm, int32_t* __restrict__ prolong)
+{
+    const int32_t r = (int32_t)(blockIdx.x*AG_BLOCK + threadIdx.x);
+    if (r < nFine) prolong[r] = coarseIperm[restrictMap[finePerm[r]]];
+}
+
+void device_transfer_lists(const AggMaps& M, const int32_t* finePerm, const int32_t* coarsePerm,
+                           TransferLists& T, hipStream_t s)
+{
+    const int32_t nF = M.nFine, nC = M.nCoarse, nFF = M.nFineFaces, nCF = M.nCoarseFaces;
+    DBuf<int32_t> fineIperm, coarseIperm, key;
+    fineIperm.alloc(std::max(nF, 1));
+    coarseIperm.alloc(std::max(nC, 1));
+    hipLaunchKernelGGL(k_inverse_perm, dim3(ag_grid(nF)), dim3(AG_BLOCK), 0, s, nF, finePerm,
+                       fineIperm.p);
+    hipLaunchKernelGGL(k_inverse_perm, dim3(ag_grid(nC)), dim3(AG_BLOCK), 0, s, nC, coarsePerm,
+                       coarseIperm.p);
+    // fine cells of each coarse row, ascending fine natural order, as fine rows
+    device_bucket(nF, M.restrictMap.p, nC, coarsePerm, T.rStart, T.rCells, s);
+    if (nF > 0)
+        hipLaunchKernelGGL(k_gather_map, dim3(ag_grid(nF)), dim3(AG_BLOCK), 0, s, nF, T.rCells.p,
+                           fineIperm.p, T.rCells.p);
+    T.prolong.alloc(std::max(nF, 1));
+    hipLaunchKernelGGL(k_prolong_rows, dim3(ag_grid(nF)), dim3(AG_BLOCK), 0, s, nF, finePerm,
+                       M.restrictMap.p, coarseIperm.p, T.prolong.p);
+    // fine faces inside each coarse row's cell, ascending
+    key.alloc(std::max(nFF, 1));
+    if (nFF > 0)
+        hipLaunchKernelGGL(k_face_keys, dim3(ag_grid(nFF)), dim3(AG_BLOCK), 0, s, nFF,
+                           M.faceRestrict.p, 1, key.p);
+    device_bucket(nFF, key.p, nC, coarsePerm, T.iStart, T.iFaces, s);
+    // fine faces of each coarse face (coarse face order), ascending, with flips
+    if (nFF > 0)
+        hipLaunchKernelGGL(k_face_keys, dim3(ag_grid(nFF)), dim3(AG_BLOCK), 0, s, nFF,
+                           M.faceRestrict.p, 0, key.p);
+    device_bucket(nFF, key.p, nCF, nullptr, T.fStart, T.fFaces, s);
+    const int32_t nOuter = read_int(T.fStart.p + nCF, s);
+    T.fFlip.alloc(std::max(nOuter, 1));
+    if (nOuter > 0)
+        hipLaunchKernelGGL(k_gather_u8, dim3(ag_grid(nOuter)), dim3(AG_BLOCK), 0, s, nOuter,
+                           T.fFaces.p, M.faceFlip.p, T.fFlip.p);
+    FH_HIP(hipStreamSynchronize(s));
+}
+
+// ---------------------------------------------------------------------------
 // one pair level
 // ---------------------------------------------------------------------------
 

@@ -27,15 +27,12 @@ namespace fh
 // pairGAMGAgglomeration::forward_ (pairGAMGAgglomeration.C:33)
 bool g_forward = true;
 
-// The maps of one hierarchy level, downloaded for the coarse mesh build
-// and the per-level transfer lists
+// The coarse addressing of one hierarchy level, downloaded for the coarse
+// mesh build
 struct HostLevel
 {
     int32_t nFineCells, nCoarseCells;
-    std::vector<int32_t> restrictMap;      // fine cell -> coarse cell
-    std::vector<int32_t> faceRestrict;     // fine face -> coarse face / -1-cell
-    std::vector<uint8_t> faceFlip;
-    std::vector<int32_t> cLower, cUpper;   // coarse addressing
+    std::vector<int32_t> cLower, cUpper;
 };
 
 template<class T>
@@ -75,16 +72,13 @@ static fh_mesh* make_mesh(int device, int32_t n, const std::vector<int32_t>& lo,
     return m;
 }
 
-struct AggLevel
+// a level: its maps, its coarse mesh and the transfer lists (rStart/rCells,
+// prolong, iStart/iFaces, fStart/fFaces/fFlip) in library row numbering
+struct AggLevel : TransferLists
 {
     std::unique_ptr<AggMaps> maps;      // the level's maps in natural numbering
     fh_mesh* mesh = nullptr;            // the coarse mesh of this level
     int32_t nFine = 0, nCoarse = 0, nCoarseFaces = 0;
-    DBuf<int32_t> rStart, rCells;       // coarse library row -> fine library rows
-    DBuf<int32_t> prolong;              // fine library row -> coarse library row
-    DBuf<int32_t> iStart, iFaces;       // coarse library row -> intra-cluster fine faces
-    DBuf<int32_t> fStart, fFaces;       // coarse face -> fine faces
-    DBuf<uint8_t> fFlip;
 
     ~AggLevel()
     {
@@ -103,42 +97,6 @@ struct GamgAgglomeration
     // natural rows of the coarsest mesh for the single-workgroup solve
     DBuf<int32_t> csRowStart, csEnt, csCol;
 };
-
-// CSR lists: bucket `key[i]` (natural coarse index, -1 skips) of the items
-// i (ascending), re-ordered by coarse library row
-static void bucket_by_lib(int32_t nKeys, const std::vector<int32_t>& key,
-                          const std::vector<int32_t>& value, const std::vector<int32_t>& perm,
-                          std::vector<int32_t>& start, std::vector<int32_t>& list,
-                          std::vector<int32_t>* itemOrder = nullptr)
-{
-    std::vector<int32_t> cnt(nKeys + 1, 0);
-    for (int32_t k : key) if (k >= 0) cnt[k + 1]++;
-    for (int32_t c = 0; c < nKeys; c++) cnt[c + 1] += cnt[c];
-    std::vector<int32_t> natList(cnt[nKeys]), natItem(cnt[nKeys]);
-    {
-        std::vector<int32_t> fill(cnt.begin(), cnt.end() - 1);
-        for (size_t i = 0; i < key.size(); i++)
-            if (key[i] >= 0)
-            {
-                natItem[fill[key[i]]] = (int32_t)i;
-                natList[fill[key[i]]++] = value[i];
-            }
-    }
-    start.assign(nKeys + 1, 0);
-    list.clear();
-    list.reserve(natList.size());
-    if (itemOrder) { itemOrder->clear(); itemOrder->reserve(natList.size()); }
-    for (int32_t r = 0; r < nKeys; r++)
-    {
-        const int32_t c = perm.empty() ? r : perm[r];
-        for (int32_t q = cnt[c]; q < cnt[c + 1]; q++)
-        {
-            list.push_back(natList[q]);
-            if (itemOrder) itemOrder->push_back(natItem[q]);
-        }
-        start[r + 1] = (int32_t)list.size();
-    }
-}
 
 static std::shared_ptr<GamgAgglomeration> build_agglomeration(fh_matrix* M,
                                                               const fh_controls* c)
@@ -174,9 +132,6 @@ static std::shared_ptr<GamgAgglomeration> build_agglomeration(fh_matrix* M,
             HostLevel L;
             L.nFineCells = D->nFine;
             L.nCoarseCells = D->nCoarse;
-            download(D->restrictMap, D->nFine, L.restrictMap, s);
-            download(D->faceRestrict, D->nFineFaces, L.faceRestrict, s);
-            download(D->faceFlip, D->nFineFaces, L.faceFlip, s);
             download(D->lower, D->nCoarseFaces, L.cLower, s);
             download(D->upper, D->nCoarseFaces, L.cUpper, s);
             host.push_back(std::move(L));
@@ -205,38 +160,9 @@ static std::shared_ptr<GamgAgglomeration> build_agglomeration(fh_matrix* M,
         D->nCoarse = L.nCoarseCells;
         D->nCoarseFaces = (int32_t)L.cLower.size();
         hipStream_t us = cm->setupStream;
-        std::vector<int32_t> start, list;
-        // restriction / Galerkin diagonal: fine cells of each coarse cell in
-        // ascending fine natural order, as fine library rows
-        std::vector<int32_t> fineLib(L.nFineCells);
-        for (int32_t i = 0; i < L.nFineCells; i++) fineLib[i] = fm->iperm[i];
-        bucket_by_lib(L.nCoarseCells, L.restrictMap, fineLib, cm->perm, start, list);
-        D->rStart.upload(start, us);
-        D->rCells.upload(list, us);
-        std::vector<int32_t> pro(L.nFineCells);
-        for (int32_t r = 0; r < L.nFineCells; r++)
-            pro[r] = cm->iperm[L.restrictMap[fm->perm[r]]];
-        D->prolong.upload(pro, us);
-        // intra-cluster fine faces per coarse library row
-        const int32_t nFF = (int32_t)L.faceRestrict.size();
-        std::vector<int32_t> key(nFF), ids(nFF);
-        for (int32_t f = 0; f < nFF; f++)
-        {
-            key[f] = L.faceRestrict[f] < 0 ? -L.faceRestrict[f] - 1 : -1;
-            ids[f] = f;
-        }
-        bucket_by_lib(L.nCoarseCells, key, ids, cm->perm, start, list);
-        D->iStart.upload(start, us);
-        D->iFaces.upload(list, us);
-        // fine faces per coarse face (coarse face order), with flips
-        for (int32_t f = 0; f < nFF; f++) key[f] = L.faceRestrict[f];
-        std::vector<int32_t> noPerm;
-        bucket_by_lib(D->nCoarseFaces, key, ids, noPerm, start, list);
-        std::vector<uint8_t> flip(list.size());
-        for (size_t q = 0; q < list.size(); q++) flip[q] = L.faceFlip[list[q]];
-        D->fStart.upload(start, us);
-        D->fFaces.upload(list, us);
-        D->fFlip.upload(flip, us);
+        // restriction / Galerkin diagonal lists, prolongation, intra-cluster
+        // and coarse-face fine faces (fh_agglom.hip)
+        device_transfer_lists(*D->maps, fm->d_perm.p, cm->d_perm.p, *D, us);
         FH_HIP(hipStreamSynchronize(us));
         if (verbose)
             std::fprintf(stderr, "foamhip GAMG level %zu: mesh %.1f ms, transfer lists %.1f ms\n",
