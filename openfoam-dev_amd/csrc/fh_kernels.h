@@ -209,9 +209,12 @@ bool launch_sweep_level_wg(int kind, const RowsView& R, const SchedView& S,
 template<class T>
 void launch_sweep(int kind, const RowsView& R, const SchedView& S, SweepSync& sync,
                   const SweepData<T>& d, DevState* st, hipStream_t s);
+// tmp (nRows values, or null): the Gauss-Seidel and DIC/DILU passes poll
+// dependency values in a sentinel-filled output (tmp for in-place passes)
+// instead of waiting on chunk flags
 template<class T>
 void launch_sweep_rows(int kind, const RowsView& R, const SweepData<T>& d, const ChunkView& C,
-                       RowSync& sync, DevState* st, hipStream_t s);
+                       RowSync& sync, DevState* st, hipStream_t s, int32_t nRows, T* tmp);
 template<class T>
 void launch_reciprocal(int32_t n, T* a, hipStream_t s);
 

@@ -580,14 +580,17 @@ void k_sweep_parts(RowsView R, SchedView S, SweepSync sync, SweepData<T> d,
 // A row of the chunk sweep in two halves, for the Gauss-Seidel and the
 // DIC/DILU application kinds.  Before the chunk's wait: the row's shape and
 // scalars (source and diagonal, or rD and x / the forward result), the
-// columns and coefficients of its first ROW_PREFETCH entries, and (Gauss-
+// columns and coefficients of its first ROW_PREFETCH entries (32: with 16,
+// the coarse GAMG levels' longer rows loaded their last entries one by one
+// after the wait and set the pace, 0.200 against 0.163 s for three
+// configuration-4 V-cycles), and (Gauss-
 // Seidel) the values of those entries that are not dependencies of this
 // sweep (upper neighbours forward, lower ones backward: no chunk of this
 // launch writes them before this row is done, so they hold the previous
 // values).  After the wait: the dependency values, all loads in flight
 // together, then the row's arithmetic in the order of sweep_row
 // (GaussSeidelSmoother.C:150-176, DICPreconditioner.C:108-122).
-#define ROW_PREFETCH 16
+#define ROW_PREFETCH 32
 
 template<int KIND>
 struct RowPreKind
@@ -606,8 +609,8 @@ struct RowPre
 };
 
 template<class T, int KIND, bool BACKWARD>
-__device__ __forceinline__ void row_prefetch(const RowsView& R, const SweepData<T>& d, int32_t r,
-                                             bool have, RowPre<T>& p)
+__device__ __forceinline__ void row_prefetch(const RowsView& R, const SweepData<T>& d,
+                                             const T* wIn, int32_t r, bool have, RowPre<T>& p)
 {
     if (have)
     {
@@ -623,7 +626,7 @@ __device__ __forceinline__ void row_prefetch(const RowsView& R, const SweepData<
         else
         {
             // backward: this row's forward result, written by an earlier launch
-            p.a = KIND == SWEEP_PRECOND_FWD ? d.x[r] : d.w[r];
+            p.a = KIND == SWEEP_PRECOND_FWD ? d.x[r] : wIn[r];
             p.b = d.rD[r];
         }
     }
@@ -641,9 +644,105 @@ __device__ __forceinline__ void row_prefetch(const RowsView& R, const SweepData<
         for (int k = 0; k < ROW_PREFETCH; k++)
         {
             const bool dep = BACKWARD ? k >= p.nLo : k < p.nLo;
-            p.w[k] = k < p.nEnt && !dep ? d.w[p.col[k]] : T(0);
+            p.w[k] = k < p.nEnt && !dep ? wIn[p.col[k]] : T(0);
         }
     }
+}
+
+// the signalling-NaN sentinel of the value-polling passes (see
+// k_sweep_chunks_vf)
+template<class T> struct SweepSentinel;
+template<> struct SweepSentinel<double>
+{
+    static constexpr long long bits = 0x7FF0DEAD5EED0001ll;
+    __device__ static double value() { return __longlong_as_double(bits); }
+    __device__ static bool is(double v) { return __double_as_longlong(v) == bits; }
+};
+template<> struct SweepSentinel<float>
+{
+    static constexpr int bits = 0x7FA05EED;
+    __device__ static float value() { return __int_as_float(bits); }
+    __device__ static bool is(float v) { return __float_as_int(v) == bits; }
+};
+
+// A row's entries past the prefetched ones, [kLo, kHi), 16 at a time with
+// every load of a block in flight (columns and coefficients, then the
+// values): ascending for Gauss-Seidel and the forward pass, descending for
+// the backward pass.  Dependency values come from wDep (with VF: polled
+// until no longer the sentinel), the others from wIn.  acc -= c*w
+// (Gauss-Seidel) or acc -= rD*c*w.
+template<class T, int KIND, bool BACKWARD, bool VF>
+__device__ __forceinline__ T row_tail(const RowsView& R, const SweepData<T>& d, const T* wIn,
+                                      const T* wDep, const RowPre<T>& p, int32_t kLo,
+                                      int32_t kHi, T acc, T rDr, unsigned long long t0,
+                                      unsigned long long waitTicks, bool& expired)
+{
+    constexpr int B = 16;
+    constexpr bool desc = KIND == SWEEP_PRECOND_BWD;
+    for (int32_t b = 0; b < kHi - kLo; b += B)
+    {
+        int32_t col[B];
+        T val[B], w[B];
+#pragma unroll
+        for (int j = 0; j < B; j++)
+        {
+            const int32_t k = desc ? kHi - 1 - b - j : kLo + b + j;
+            const bool in = desc ? k >= kLo : k < kHi;
+            const int32_t e = p.base + (k << 6);
+            col[j] = in ? R.col[e] : 0;
+            val[j] = in ? d.val[e] : T(0);
+        }
+#pragma unroll
+        for (int j = 0; j < B; j++)
+        {
+            const int32_t k = desc ? kHi - 1 - b - j : kLo + b + j;
+            const bool in = desc ? k >= kLo : k < kHi;
+            const bool dep = BACKWARD ? k >= p.nLo : k < p.nLo;
+            w[j] = !in ? T(0) : dep ? sw_ld<true>(&wDep[col[j]]) : wIn[col[j]];
+        }
+        if constexpr (VF)
+        {
+            for (;;)
+            {
+                bool pending = false;
+#pragma unroll
+                for (int j = 0; j < B; j++)
+                {
+                    const int32_t k = desc ? kHi - 1 - b - j : kLo + b + j;
+                    const bool in = desc ? k >= kLo : k < kHi;
+                    const bool dep = BACKWARD ? k >= p.nLo : k < p.nLo;
+                    pending |= in && dep && SweepSentinel<T>::is(w[j]);
+                }
+                if (!pending || expired) break;
+                if ((unsigned long long)(wall_clock64() - t0) > waitTicks)
+                {
+                    expired = true;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+                for (int j = 0; j < B; j++)
+                {
+                    const int32_t k = desc ? kHi - 1 - b - j : kLo + b + j;
+                    const bool in = desc ? k >= kLo : k < kHi;
+                    const bool dep = BACKWARD ? k >= p.nLo : k < p.nLo;
+                    if (in && dep && SweepSentinel<T>::is(w[j])) w[j] = sw_ld<true>(&wDep[col[j]]);
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < B; j++)
+        {
+            const int32_t k = desc ? kHi - 1 - b - j : kLo + b + j;
+            const bool in = desc ? k >= kLo : k < kHi;
+            if (in)
+            {
+                if constexpr (RowPreKind<KIND>::gs) acc -= val[j]*w[j];
+                else acc -= rDr*val[j]*w[j];
+            }
+        }
+    }
+    return acc;
 }
 
 template<class T, int KIND, bool BACKWARD>
@@ -662,11 +761,9 @@ __device__ __forceinline__ void row_finish(const RowsView& R, const SweepData<T>
 #pragma unroll
         for (int k = 0; k < ROW_PREFETCH; k++)
             if (k < p.nEnt) t -= p.val[k]*p.w[k];
-        for (int32_t k = ROW_PREFETCH; k < p.nEnt; k++)
-        {
-            const int32_t e = p.base + (k << 6);
-            t -= d.val[e]*sw_ld<true>(&d.w[R.col[e]]);
-        }
+        bool unused = false;
+        t = row_tail<T, KIND, BACKWARD, false>(R, d, d.w, d.w, p, ROW_PREFETCH, p.nEnt, t, T(0),
+                                               0, 0, unused);
         sw_st<true>(&d.w[r], t/p.b);
     }
     else if constexpr (KIND == SWEEP_PRECOND_FWD)
@@ -677,11 +774,9 @@ __device__ __forceinline__ void row_finish(const RowsView& R, const SweepData<T>
 #pragma unroll
         for (int k = 0; k < ROW_PREFETCH; k++)
             if (k < p.nLo) w -= rDr*p.val[k]*p.w[k];
-        for (int32_t k = ROW_PREFETCH; k < p.nLo; k++)
-        {
-            const int32_t e = p.base + (k << 6);
-            w -= rDr*d.val[e]*sw_ld<true>(&d.w[R.col[e]]);
-        }
+        bool unused = false;
+        w = row_tail<T, KIND, BACKWARD, false>(R, d, d.w, d.w, p, ROW_PREFETCH, p.nLo, w, rDr,
+                                               0, 0, unused);
         sw_st<true>(&d.w[r], w);
     }
     else
@@ -691,11 +786,9 @@ __device__ __forceinline__ void row_finish(const RowsView& R, const SweepData<T>
         T w = p.a;
         // entries past the prefetched ones, down to the first upper entry
         const int32_t kEnd = p.nLo > ROW_PREFETCH ? p.nLo : ROW_PREFETCH;
-        for (int32_t k = p.nEnt - 1; k >= kEnd; k--)
-        {
-            const int32_t e = p.base + (k << 6);
-            w -= rDr*d.val[e]*sw_ld<true>(&d.w[R.col[e]]);
-        }
+        bool unused = false;
+        w = row_tail<T, KIND, BACKWARD, false>(R, d, d.w, d.w, p, kEnd, p.nEnt, w, rDr, 0, 0,
+                                               unused);
 #pragma unroll
         for (int k = ROW_PREFETCH - 1; k >= 0; k--)
             if (k >= p.nLo && k < p.nEnt) w -= rDr*p.val[k]*p.w[k];
@@ -741,7 +834,7 @@ void k_sweep_chunks(RowsView R, SweepData<T> d, ChunkView C, RowSync sync,
         const int32_t r = C.start[c] + lane;
         const bool have = r < C.start[c + 1];
         RowPre<T> pre;
-        if constexpr (RowPreKind<KIND>::any) row_prefetch<T, KIND, BACKWARD>(R, d, r, have, pre);
+        if constexpr (RowPreKind<KIND>::any) row_prefetch<T, KIND, BACKWARD>(R, d, d.w, r, have, pre);
         bool expired = false;
         if (q1 > q0)
         {
@@ -787,6 +880,206 @@ void k_sweep_chunks(RowsView R, SweepData<T> d, ChunkView C, RowSync sync,
     }
 }
 
+// --------------------------------------------------------------------------
+// Value-polling chunk sweep (the Gauss-Seidel and DIC/DILU kinds).  The
+// pass writes into an output that holds a sentinel before the launch (a
+// signalling NaN: no arithmetic produces one, a NaN result is quiet).  One
+// lane of the wavefront polls the chunk's dependency written last (rows
+// are numbered by level: the highest row forward, the lowest backward) in
+// the output until it is no longer the sentinel; then every lane loads its
+// dependency values and re-polls any still unwritten.  The hand-off
+// between levels is the value itself: no completion flag, no wait for the
+// stores.  Non-dependency values (and, backward, the row's own forward
+// result) come from the input.  Chunks are still ticketed in sweep order,
+// so a row only waits for rows of chunks whose holders are running; every
+// wait is bounded in time, an expired wait is reported and the row still
+// written (never the sentinel), so that the launch drains.
+// --------------------------------------------------------------------------
+// the fill and the copy back do nothing once the solve is done, like the
+// pass itself (which then computes no row)
+template<class T>
+__global__ __launch_bounds__(FH_BLOCK)
+void k_fill_sentinel(int32_t n, T* __restrict__ p, const DevState* st)
+{
+    const int32_t i = (int32_t)(blockIdx.x*FH_BLOCK + threadIdx.x);
+    if (i < n && !(st && st->done)) p[i] = SweepSentinel<T>::value();
+}
+
+template<class T>
+__global__ __launch_bounds__(FH_BLOCK)
+void k_copy_rows(int32_t n, const T* __restrict__ a, T* __restrict__ b, const DevState* st)
+{
+    const int32_t i = (int32_t)(blockIdx.x*FH_BLOCK + threadIdx.x);
+    if (i < n && !(st && st->done)) b[i] = a[i];
+}
+
+// one dependency value: poll until written (or the wait expires)
+template<class T>
+__device__ __forceinline__ T vf_poll(const T* a, unsigned long long t0,
+                                     unsigned long long waitTicks, bool& expired)
+{
+    for (;;)
+    {
+        const T v = sw_ld<true>(a);
+        if (!SweepSentinel<T>::is(v) || expired) return v;
+        if ((unsigned long long)(wall_clock64() - t0) > waitTicks)
+        {
+            expired = true;
+            return v;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+template<class T, int KIND, bool BACKWARD>
+__global__ __launch_bounds__(FH_BLOCK)
+void k_sweep_chunks_vf(RowsView R, SweepData<T> d, const T* wIn, T* wOut, ChunkView C,
+                       RowSync sync, DevState* st)
+{
+    const bool stop = st && st->done;
+    const int lane = (int)(threadIdx.x & 63);
+    for (;;)
+    {
+        unsigned long long t = 0;
+        if (lane == 0) t = atomicAdd(sync.ticket, 1ull) - sync.ticketBase;
+        t = __shfl(t, 0, 64);
+        if (t >= (unsigned long long)C.nChunks) return;
+        if (stop) continue;
+        const int32_t c = BACKWARD ? C.nChunks - 1 - (int32_t)t : (int32_t)t;
+        const int32_t r = C.start[c] + lane;
+        const bool have = r < C.start[c + 1];
+        RowPre<T> p;
+        row_prefetch<T, KIND, BACKWARD>(R, d, wIn, r, have, p);
+        bool expired = false;
+        const unsigned long long t0 = wall_clock64();
+        // the chunk's dependency written last, polled by one lane
+        {
+            int32_t key = INT32_MIN;
+#pragma unroll
+            for (int k = 0; k < ROW_PREFETCH; k++)
+            {
+                const bool dep = BACKWARD ? k >= p.nLo : k < p.nLo;
+                if (k < p.nEnt && dep)
+                    key = max(key, BACKWARD ? -p.col[k] : p.col[k]);
+            }
+            for (int off = 32; off > 0; off >>= 1) key = max(key, __shfl_xor(key, off, 64));
+            if (lane == 0 && key != INT32_MIN)
+                (void)vf_poll(&wOut[BACKWARD ? -key : key], t0, sync.waitTicks, expired);
+        }
+        if (have)
+        {
+#pragma unroll
+            for (int k = 0; k < ROW_PREFETCH; k++)
+            {
+                const bool dep = BACKWARD ? k >= p.nLo : k < p.nLo;
+                if (k < p.nEnt && dep) p.w[k] = SweepSentinel<T>::value();
+            }
+            // the prefetched entries' dependencies: all loads in flight, then
+            // again for those not yet written
+            for (;;)
+            {
+#pragma unroll
+                for (int k = 0; k < ROW_PREFETCH; k++)
+                {
+                    const bool dep = BACKWARD ? k >= p.nLo : k < p.nLo;
+                    if (k < p.nEnt && dep && SweepSentinel<T>::is(p.w[k]))
+                        p.w[k] = sw_ld<true>(&wOut[p.col[k]]);
+                }
+                bool pending = false;
+#pragma unroll
+                for (int k = 0; k < ROW_PREFETCH; k++)
+                {
+                    const bool dep = BACKWARD ? k >= p.nLo : k < p.nLo;
+                    pending |= k < p.nEnt && dep && SweepSentinel<T>::is(p.w[k]);
+                }
+                if (!pending || expired) break;
+                if ((unsigned long long)(wall_clock64() - t0) > sync.waitTicks)
+                {
+                    expired = true;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            T v;
+            if constexpr (RowPreKind<KIND>::gs)
+            {
+                // GaussSeidelSmoother.C:150-176: entries in order
+                T tt = p.a;
+#pragma unroll
+                for (int k = 0; k < ROW_PREFETCH; k++)
+                    if (k < p.nEnt) tt -= p.val[k]*p.w[k];
+                tt = row_tail<T, KIND, BACKWARD, true>(R, d, wIn, wOut, p, ROW_PREFETCH, p.nEnt,
+                                                       tt, T(0), t0, sync.waitTicks, expired);
+                v = tt/p.b;
+            }
+            else if constexpr (KIND == SWEEP_PRECOND_FWD)
+            {
+                // DICPreconditioner.C:108-117: lower entries ascending
+                const T rDr = p.b;
+                T w = rDr*p.a;
+#pragma unroll
+                for (int k = 0; k < ROW_PREFETCH; k++)
+                    if (k < p.nLo) w -= rDr*p.val[k]*p.w[k];
+                w = row_tail<T, KIND, BACKWARD, true>(R, d, wIn, wOut, p, ROW_PREFETCH, p.nLo, w,
+                                                      rDr, t0, sync.waitTicks, expired);
+                v = w;
+            }
+            else
+            {
+                // DICPreconditioner.C:119-122: upper entries descending
+                const T rDr = p.b;
+                T w = p.a;
+                const int32_t kEnd = p.nLo > ROW_PREFETCH ? p.nLo : ROW_PREFETCH;
+                w = row_tail<T, KIND, BACKWARD, true>(R, d, wIn, wOut, p, kEnd, p.nEnt, w, rDr,
+                                                      t0, sync.waitTicks, expired);
+#pragma unroll
+                for (int k = ROW_PREFETCH - 1; k >= 0; k--)
+                    if (k >= p.nLo && k < p.nEnt) w -= rDr*p.val[k]*p.w[k];
+                v = w;
+            }
+            if (expired)
+            {
+                // never the sentinel: later rows drain
+                if (SweepSentinel<T>::is(v)) v = T(0);
+                __hip_atomic_store(sync.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (st) st->scheduleError = 1;
+            }
+            sw_st<true>(&wOut[r], v);
+        }
+    }
+}
+
+template<class T, int KIND, bool BACKWARD>
+static void launch_rows_vf(const RowsView& R, const SweepData<T>& d, const T* wIn, T* wOut,
+                           int32_t nRows, const ChunkView& C, RowSync& sync, DevState* st,
+                           hipStream_t s)
+{
+    const unsigned int g = (unsigned int)((nRows + FH_BLOCK - 1)/FH_BLOCK);
+    if (nRows > 0)
+        hipLaunchKernelGGL(k_fill_sentinel<T>, dim3(g), dim3(FH_BLOCK), 0, s, nRows, wOut,
+                           (const DevState*)st);
+    const int32_t wantBlocks = (C.nChunks + 3)/4;
+    const int32_t grid = wantBlocks < kRowSweepBlocks ? wantBlocks : kRowSweepBlocks;
+    hipLaunchKernelGGL((k_sweep_chunks_vf<T, KIND, BACKWARD>), dim3(grid), dim3(FH_BLOCK), 0, s,
+                       R, d, wIn, wOut, C, sync, st);
+    sync.ticketBase += (unsigned long long)C.nChunks + 4ull*(unsigned long long)grid;
+    sync.epoch++;
+}
+
+// in-place passes through the scratch rows: sentinel-filled output, then
+// copied back
+template<class T, int KIND, bool BACKWARD>
+static void launch_rows_vf_inplace(const RowsView& R, const SweepData<T>& d, int32_t nRows,
+                                   T* tmp, const ChunkView& C, RowSync& sync, DevState* st,
+                                   hipStream_t s)
+{
+    launch_rows_vf<T, KIND, BACKWARD>(R, d, d.w, tmp, nRows, C, sync, st, s);
+    const unsigned int g = (unsigned int)((nRows + FH_BLOCK - 1)/FH_BLOCK);
+    if (nRows > 0)
+        hipLaunchKernelGGL(k_copy_rows<T>, dim3(g), dim3(FH_BLOCK), 0, s, nRows,
+                           (const T*)tmp, d.w, (const DevState*)st);
+}
+
 template<class T, int KIND, bool BACKWARD>
 static void launch_rows_kind(const RowsView& R, const SweepData<T>& d, const ChunkView& C,
                              RowSync& sync, DevState* st, hipStream_t s)
@@ -803,8 +1096,35 @@ static void launch_rows_kind(const RowsView& R, const SweepData<T>& d, const Chu
 
 template<class T>
 void launch_sweep_rows(int kind, const RowsView& R, const SweepData<T>& d, const ChunkView& C,
-                       RowSync& sync, DevState* st, hipStream_t s)
+                       RowSync& sync, DevState* st, hipStream_t s, int32_t nRows, T* tmp)
 {
+    if (tmp)
+    {
+        switch (kind)
+        {
+            case SWEEP_GS_FWD:
+                launch_rows_vf_inplace<T, SWEEP_GS_FWD, false>(R, d, nRows, tmp, C, sync, st, s);
+                return;
+            case SWEEP_GS_BWD:
+                launch_rows_vf_inplace<T, SWEEP_GS_BWD, true>(R, d, nRows, tmp, C, sync, st, s);
+                return;
+            case SWEEP_PRECOND_FWD:
+                // w is output only, unless it is also the input
+                if (d.w != d.x)
+                    launch_rows_vf<T, SWEEP_PRECOND_FWD, false>(R, d, d.w, d.w, nRows, C, sync,
+                                                                st, s);
+                else
+                    launch_rows_vf_inplace<T, SWEEP_PRECOND_FWD, false>(R, d, nRows, tmp, C,
+                                                                        sync, st, s);
+                return;
+            case SWEEP_PRECOND_BWD:
+                launch_rows_vf_inplace<T, SWEEP_PRECOND_BWD, true>(R, d, nRows, tmp, C, sync,
+                                                                   st, s);
+                return;
+            default:
+                break;
+        }
+    }
     switch (kind)
     {
         case SWEEP_PRECOND_FWD:   launch_rows_kind<T, SWEEP_PRECOND_FWD, false>(R, d, C, sync, st, s); break;
@@ -818,8 +1138,8 @@ void launch_sweep_rows(int kind, const RowsView& R, const SweepData<T>& d, const
     }
 }
 
-template void launch_sweep_rows<double>(int, const RowsView&, const SweepData<double>&, const ChunkView&, RowSync&, DevState*, hipStream_t);
-template void launch_sweep_rows<float>(int, const RowsView&, const SweepData<float>&, const ChunkView&, RowSync&, DevState*, hipStream_t);
+template void launch_sweep_rows<double>(int, const RowsView&, const SweepData<double>&, const ChunkView&, RowSync&, DevState*, hipStream_t, int32_t, double*);
+template void launch_sweep_rows<float>(int, const RowsView&, const SweepData<float>&, const ChunkView&, RowSync&, DevState*, hipStream_t, int32_t, float*);
 
 template<class T, int KIND, bool BACKWARD>
 static void launch_sweep_kind(const RowsView& R, const SchedView& S, SweepSync& sync,

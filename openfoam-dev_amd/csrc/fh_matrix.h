@@ -84,6 +84,9 @@ struct fh_matrix
     fh::DBuf<unsigned int> d_rowErr;
     unsigned long long rowTicketBase = 0, rowEpoch = 0, rowWaitTicks = 0;
     bool rowSweep = false;
+    // output rows of the value-polling chunk passes that run in place
+    fh::DBuf<double> d_chunkTmp;
+    fh::DBuf<float> d_chunkTmp32;
     // tiled sweeps (hex boxes)
     fh::DBuf<unsigned int> d_tileFlags;
     fh::DBuf<unsigned long long> d_tileTicket;
