@@ -46,6 +46,12 @@ struct fh_matrix
     // preconditioner)
     fh::DBuf<__half> d_dir16[3];
     bool dir16Valid = false;
+    // the same on rows padded to a multiple of 16 cells (boxes whose nx is
+    // not one), with the factor, for the fused tile-local kernels: pad
+    // cells hold zero coefficients and factor; x and w staged padded
+    fh::DBuf<float> d_padDir32[6], d_padRD, d_padX, d_padW;
+    fh::DBuf<__half> d_padDir16[3];
+    bool padValid = false;
     // FP16 storage scale 2^e of those coefficients: state 0 not decided, 1
     // FP16 at halfScale, -1 FP32 (the magnitudes span more than the FP16
     // normal range holds)

@@ -178,6 +178,7 @@ void matrix_coeffs_updated(fh_matrix* M)
     M->hasCoeffs = true;
     M->valTValid = M->sumAValid = M->f32Valid = M->dir32Valid = M->dir64Valid = false;
     M->dir16Valid = false;
+    M->padValid = false;
     M->halfState = 0;
     M->coarseState = 0;
     for (int i = 0; i < 5; i++) M->rDValid[i] = M->rD32Valid[i] = false;
@@ -340,8 +341,9 @@ const double* ensure_rD(fh_matrix* M, int which)
     return b.p;
 }
 
-// The FP32 tile-local DIC of a symmetric box whose rows are whole 16-wide
-// vectors stores its off-diagonal coefficients in FP16: the factor and both
+// The FP32 tile-local DIC of a symmetric box stores its off-diagonal
+// coefficients in FP16 (rows that are not whole 16-wide vectors run on
+// zero-padded rows): the factor and both
 // passes use the coefficients rounded to FP16 precision (the incomplete
 // Cholesky factor of that matrix; diagonal, factor and arithmetic FP32).
 // They are stored scaled by 2^e, e chosen from the largest magnitude so
@@ -353,7 +355,7 @@ static constexpr int HALF_BOTTOM = -14;  // smallest FP16 normal 2^-14
 
 static bool tile_dic_half(fh_matrix* M)
 {
-    if (!(M->symmetric && M->mesh->tiled && (M->mesh->boxNx & 15) == 0)) return false;
+    if (!(M->symmetric && M->mesh->tiled)) return false;
     if (M->halfState == 0)
     {
         ensure_f32(M);
@@ -637,7 +639,7 @@ int32_t apply_precond(fh_matrix* M, int precon, const T* r, T* w, double* partia
                 for (int q = 0; q < 6; q++) dir[q] = M->d_dir32[q].p;
                 const __half* h3[3] = {};
                 const bool half = which == 3 && tile_dic_half(M);
-                if (half && !M->dir16Valid)
+                if (half && !M->dir16Valid && (G.nx & 15) == 0)
                 {
                     __half* dst[3];
                     for (int q = 0; q < 3; q++) { M->d_dir16[q].alloc(N); dst[q] = M->d_dir16[q].p; }
@@ -657,11 +659,59 @@ int32_t apply_precond(fh_matrix* M, int precon, const T* r, T* w, double* partia
                     return nb;
                 }
             }
-            launch_tile_local_f32(false, G, r, d.rD, M->d_dir32[0].p, M->d_dir32[1].p,
-                                  M->d_dir32[2].p, w, st, M->stream);
-            // the backward pass also sums w*r (one partial per workgroup)
-            return launch_tile_local_f32(true, G, w, d.rD, M->d_dir32[5].p, M->d_dir32[4].p,
-                                         M->d_dir32[3].p, w, st, M->stream, r, partials);
+            {
+                // rows not a multiple of 16: the fused kernels on rows padded
+                // with zero cells (zero coefficients and factor), x and w
+                // staged through padded copies; the same tiles, partials and
+                // tile sums, the pad cells contributing zero
+                const int32_t nx = G.nx, nxp = (nx + 15) & ~15;
+                const int64_t rows = (int64_t)G.ny*G.nz, np = (int64_t)nxp*rows;
+                const bool half = which == 3 && tile_dic_half(M);
+                if (!M->padValid)
+                {
+                    for (int q = 0; q < 6; q++)
+                    {
+                        M->d_padDir32[q].alloc(np);
+                        launch_pad_rows(nx, nxp, rows, M->d_dir32[q].p, M->d_padDir32[q].p,
+                                        M->stream);
+                    }
+                    M->d_padRD.alloc(np);
+                    launch_pad_rows(nx, nxp, rows, d.rD, M->d_padRD.p, M->stream);
+                    if (half)
+                    {
+                        const float* src[3];
+                        __half* dst[3];
+                        for (int q = 0; q < 3; q++)
+                        {
+                            M->d_padDir16[q].alloc(np);
+                            src[q] = M->d_padDir32[q].p;
+                            dst[q] = M->d_padDir16[q].p;
+                        }
+                        launch_pack_half(np, src, dst, M->halfScale, M->stream);
+                    }
+                    M->d_padX.alloc(np);
+                    M->d_padW.alloc(np);
+                    M->padValid = true;
+                }
+                TileView Gp = G;
+                Gp.nx = nxp;
+                const float* dir[6];
+                for (int q = 0; q < 6; q++) dir[q] = M->d_padDir32[q].p;
+                const __half* h3[3];
+                for (int q = 0; q < 3; q++) h3[q] = M->d_padDir16[q].p;
+                const bool fuse = partials && F.counter;
+                launch_pad_rows(nx, nxp, rows, r, M->d_padX.p, M->stream);
+                const int32_t nb = launch_tile_local_apply_f32(Gp, M->d_padX.p, M->d_padRD.p,
+                                                               dir, M->symmetric, M->d_padW.p,
+                                                               st, M->stream, partials,
+                                                               half ? h3 : nullptr,
+                                                               fuse ? F : FusedFinalize(),
+                                                               tileSum, M->halfScale);
+                if (nb < 0) fail(FH_ERR_INVALID, "internal: padded tile-local shape");
+                launch_unpad_rows(nx, nxp, rows, M->d_padW.p, w, M->stream);
+                if (fused) *fused = fuse;
+                return nb;
+            }
         }
     }
     if (M->mesh->tiled && !local && !transpose)
@@ -1136,7 +1186,9 @@ static bool ensure_coarse(fh_matrix* M)
     if (M->coarseState != 0) return M->coarseState > 0;
     M->coarseState = -1;
     const fh_mesh* mesh = M->mesh;
-    if (off || !mesh->tiled || !M->symmetric || (mesh->boxNx & 15)) return false;
+    // (the correction is carried into p by the fused product, which needs
+    // rows of a multiple of 4 cells)
+    if (off || !mesh->tiled || !M->symmetric || (mesh->boxNx & 3)) return false;
     const int32_t nTI = (mesh->boxNx + 15)/16, nTJ = (mesh->boxNy + 15)/16;
     const int32_t nTK = (mesh->boxNz + 15)/16;
     const int64_t nT = (int64_t)nTI*nTJ*nTK;

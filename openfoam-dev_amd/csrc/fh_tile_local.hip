@@ -1193,6 +1193,46 @@ void launch_abs_range(int64_t n, const float* a, unsigned int* range, hipStream_
                        s, n, a, range);
 }
 
+// Box fields with rows padded to a multiple of 16 cells (the layout of the
+// fused tile-local kernels): dst[i + nxp (j + ny k)] = src[i + nx (j + ny k)]
+// for i < nx, 0 in the padding (pad), and back (unpad)
+__global__ void k_pad_rows(int32_t nx, int32_t nxp, int64_t np, const float* __restrict__ src,
+                           float* __restrict__ dst)
+{
+    const int64_t q = (int64_t)blockIdx.x*TL_B + threadIdx.x;
+    if (q >= np) return;
+    const int32_t i = (int32_t)(q % nxp);
+    const int64_t row = q/nxp;
+    dst[q] = i < nx ? src[i + row*nx] : 0.f;
+}
+
+__global__ void k_unpad_rows(int32_t nx, int32_t nxp, int64_t n, const float* __restrict__ src,
+                             float* __restrict__ dst)
+{
+    const int64_t c = (int64_t)blockIdx.x*TL_B + threadIdx.x;
+    if (c >= n) return;
+    const int32_t i = (int32_t)(c % nx);
+    dst[c] = src[i + (c/nx)*nxp];
+}
+
+void launch_pad_rows(int32_t nx, int32_t nxp, int64_t rows, const float* src, float* dst,
+                     hipStream_t s)
+{
+    const int64_t np = (int64_t)nxp*rows;
+    if (np > 0)
+        hipLaunchKernelGGL(k_pad_rows, dim3((unsigned)((np + TL_B - 1)/TL_B)), dim3(TL_B), 0, s,
+                           nx, nxp, np, src, dst);
+}
+
+void launch_unpad_rows(int32_t nx, int32_t nxp, int64_t rows, const float* src, float* dst,
+                       hipStream_t s)
+{
+    const int64_t n = (int64_t)nx*rows;
+    if (n > 0)
+        hipLaunchKernelGGL(k_unpad_rows, dim3((unsigned)((n + TL_B - 1)/TL_B)), dim3(TL_B), 0, s,
+                           nx, nxp, n, src, dst);
+}
+
 int32_t launch_tile_local_f32(bool backward, const TileView& G, const float* a, const float* rD,
                               const float* cK, const float* cJ, const float* cI, float* w,
                               const DevState* st, hipStream_t s, const float* r,

@@ -86,5 +86,11 @@ int32_t launch_tile_local_f32(bool backward, const TileView& G, const float* a, 
                               const float* cK, const float* cJ, const float* cI, float* w,
                               const DevState* st, hipStream_t s, const float* r = nullptr,
                               double* partials = nullptr);
+// a box field (rows of nx cells) copied to rows of nxp cells (padding 0),
+// and back
+void launch_pad_rows(int32_t nx, int32_t nxp, int64_t rows, const float* src, float* dst,
+                     hipStream_t s);
+void launch_unpad_rows(int32_t nx, int32_t nxp, int64_t rows, const float* src, float* dst,
+                       hipStream_t s);
 
 } // namespace fh

@@ -306,15 +306,24 @@ def test_mixed_pcg_tile_dic_inner():
     """Mixed PCG with the tile-local inner preconditioner meets the FP64
     residual test, matches the FP64 oracle at a tight tolerance, and at the
     working tolerance takes about the iterations of PCG with the same
-    (16 x 16 x 16 block-Jacobi DIC) preconditioner in exact arithmetic."""
+    preconditioner in exact arithmetic: on this 40^3 box (rows of 40 cells,
+    run on rows zero-padded to 48) the FP16-rounded 16 x 16 x 16
+    block-Jacobi DIC plus the coarse correction over the tiles."""
     c, a, diag, upper, b = laplacian_case(40, sigma=1.0)
     om = O.Matrix(a, diag, upper)
     mesh = fh.lduAddressing(c.N, c.lower, c.upper)
     m = fh.lduMatrix(mesh, diag, upper)
     cut = _tile_blocks(40, 40, 40, c.lower, c.upper, 16)
-    om_t = O.Matrix(O.Addr(c.N, c.lower, c.upper), diag, np.where(cut, 0.0, upper))
+    up16 = upper.astype(np.float32).astype(np.float16).astype(np.float64)
+    om_t = O.Matrix(O.Addr(c.N, c.lower, c.upper), diag, np.where(cut, 0.0, up16))
+    tile, nT, Aci = _tile_coarse(om, c.N, 40, 40, 40)
+
+    def two_level(r):
+        rc = np.bincount(tile, r, nT)
+        return om_t.precondition("DIC", r) + (Aci @ rc)[tile]
+
     exact = {"DIC": _numpy_pcg(om, b, lambda r: om.precondition("DIC", r), 1e-8),
-             "tileDIC": _numpy_pcg(om, b, lambda r: om_t.precondition("DIC", r), 1e-8)}
+             "tileDIC": _numpy_pcg(om, b, two_level, 1e-8)}
     for inner in ["tileDIC", "DIC"]:
         ctl = {"solver": "PCG", "preconditioner": "DIC", "tolerance": 1e-12, "relTol": 0,
                "maxIter": 3000, "precision": "mixed", "innerPreconditioner": inner}
@@ -378,11 +387,14 @@ def test_mixed_pcg_tile_dic_half():
     assert abs(p8.nIterations - exact) <= max(3, 0.05 * exact)
 
 
-@pytest.mark.parametrize("shape", [(64, 48, 32), (32, 32, 80), (16, 16, 16), (208, 160, 32)])
+@pytest.mark.parametrize("shape", [(64, 48, 32), (32, 32, 80), (16, 16, 16), (208, 160, 32),
+                                   (36, 28, 20), (37, 21, 19)])
 def test_mixed_pcg_two_level_boxes(shape):
     """Mixed PCG with the two-level inner preconditioner on non-cubic boxes
     (and a single tile, where the coarse part is off; 208 x 160 x 32 has 130
-    tiles per plane, so each plane's inverse is split 2 x 2 at 128): the FP64
+    tiles per plane, so each plane's inverse is split 2 x 2 at 128; rows of
+    36 and 37 cells run the tile-local apply on zero-padded rows, 37 without
+    the coarse part, whose fused product needs rows of a multiple of 4): the FP64
     residual test holds for the true residual, and at a tight tolerance the
     solution matches the FP64 oracle to 1e-8."""
     from foamhip.cases import BoxCase
@@ -416,9 +428,9 @@ def test_fp32_preconditioners(kind, shape):
     """The FP32 inner preconditioners against the FP64 oracle: the exact
     factor, and the tile-local one = the oracle factor of the matrix with the
     faces between the FP32 tiles (16 x 16 x 16) removed -- for a symmetric
-    box with nx a multiple of 16 of that matrix with its off-diagonal
-    coefficients rounded to FP16 (the storage of that path).  FP32 bar:
-    relative L2 difference 1e-5."""
+    box of that matrix with its off-diagonal coefficients rounded to FP16
+    (the storage of that path; rows that are not a multiple of 16 cells run
+    on zero-padded rows).  FP32 bar: relative L2 difference 1e-5."""
     nx, ny, nz = shape
     n, lower, upper = box(nx, ny, nz)
     rng = np.random.default_rng(9)
@@ -436,7 +448,7 @@ def test_fp32_preconditioners(kind, shape):
     assert rel_l2(w, ref) <= 1e-5, rel_l2(w, ref)
     cut = _tile_blocks(nx, ny, nz, lower, upper, 16)
     up_t = up
-    if kind == "DIC" and nx % 16 == 0:
+    if kind == "DIC":
         up_t = up.astype(np.float32).astype(np.float16).astype(np.float64)
     om_t = O.Matrix(O.Addr(n, lower, upper), diag, np.where(cut, 0.0, up_t),
                     None if lo is None else np.where(cut, 0.0, lo))
