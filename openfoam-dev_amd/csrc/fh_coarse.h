@@ -36,5 +36,9 @@ int64_t coarse_block_work_doubles(int32_t nT, int32_t P);
 int32_t launch_coarse_apply(int32_t nT, int32_t ld, const float* inv, const double* rc,
                             float* z, const double* fine, int32_t nF, double* partials,
                             const FusedFinalize& F, const DevState* st, hipStream_t s);
+// w += z of the cell's 16^3 tile (the coarse correction added to the
+// tile-local apply where the product does not add it)
+void launch_add_tile_correction(int32_t nx, int32_t ny, int32_t nz, const float* z, float* w,
+                                const DevState* st, hipStream_t s);
 
 } // namespace fh

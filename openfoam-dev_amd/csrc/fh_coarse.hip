@@ -677,6 +677,28 @@ void k_coarse_apply(int32_t nT, int32_t ld, const float* __restrict__ inv,
     partials_arrive(t, 0.0, partials, gridDim.x, blockIdx.x, F);
 }
 
+__global__ __launch_bounds__(CO_B)
+void k_add_tile_correction(int32_t nx, int32_t ny, int64_t n, int32_t nTI, int32_t nTJ,
+                           const float* __restrict__ z, float* __restrict__ w,
+                           const DevState* st)
+{
+    const int64_t c = (int64_t)blockIdx.x*CO_B + threadIdx.x;
+    if (c >= n || (st && st->done)) return;
+    const int32_t i = (int32_t)(c % nx);
+    const int64_t jk = c/nx;
+    const int32_t j = (int32_t)(jk % ny), k = (int32_t)(jk/ny);
+    w[c] += z[(i >> 4) + nTI*((j >> 4) + nTJ*(k >> 4))];
+}
+
+void launch_add_tile_correction(int32_t nx, int32_t ny, int32_t nz, const float* z, float* w,
+                                const DevState* st, hipStream_t s)
+{
+    const int64_t n = (int64_t)nx*ny*nz;
+    if (n > 0)
+        hipLaunchKernelGGL(k_add_tile_correction, dim3((unsigned)((n + CO_B - 1)/CO_B)),
+                           dim3(CO_B), 0, s, nx, ny, n, (nx + 15)/16, (ny + 15)/16, z, w, st);
+}
+
 int32_t launch_coarse_apply(int32_t nT, int32_t ld, const float* inv, const double* rc,
                             float* z, const double* fine, int32_t nF, double* partials,
                             const FusedFinalize& F, const DevState* st, hipStream_t s)

@@ -1186,9 +1186,7 @@ static bool ensure_coarse(fh_matrix* M)
     if (M->coarseState != 0) return M->coarseState > 0;
     M->coarseState = -1;
     const fh_mesh* mesh = M->mesh;
-    // (the correction is carried into p by the fused product, which needs
-    // rows of a multiple of 4 cells)
-    if (off || !mesh->tiled || !M->symmetric || (mesh->boxNx & 3)) return false;
+    if (off || !mesh->tiled || !M->symmetric) return false;
     const int32_t nTI = (mesh->boxNx + 15)/16, nTJ = (mesh->boxNy + 15)/16;
     const int32_t nTK = (mesh->boxNz + 15)/16;
     const int64_t nT = (int64_t)nTI*nTJ*nTK;
@@ -1317,8 +1315,9 @@ static void mixed_pcg(fh_matrix* M, const fh_controls* c, double* psi, const dou
     // buffer when the product reads w to form p
     float* ap = fuseP ? M->f[4].p : w32;
     // two-level inner preconditioner on boxes (tile-local DIC plus the tile
-    // coarse correction, carried into p by the product)
-    const bool coarse = precon == FH_PRECON_TILE_DIC && fuseP && ensure_coarse(M);
+    // coarse correction, carried into p by the fused product, or added to w
+    // before p is formed)
+    const bool coarse = precon == FH_PRECON_TILE_DIC && box && ensure_coarse(M);
     run_batches(M, c->maxIter, [&](int32_t seq)
     {
         kt_mark(M, FH_KERNEL_PRECONDITION);
@@ -1360,6 +1359,10 @@ static void mixed_pcg(fh_matrix* M, const fh_controls* c, double* psi, const dou
         }
         else if (box)
         {
+            // p = (w + z) + beta p, as the fused product forms it
+            if (coarse)
+                launch_add_tile_correction(mesh->boxNx, mesh->boxNy, mesh->boxNz, M->d_cz.p,
+                                           w32, st, M->stream);
             launch_pcg_p<float>(N, w32, p32, st, M->stream);
             const int32_t nb = launch_box_spmv_dot_f32(mesh->boxNx, mesh->boxNy, mesh->boxNz,
                                                        diag, dir, M->symmetric, p32, w32, p32,

@@ -393,8 +393,9 @@ def test_mixed_pcg_two_level_boxes(shape):
     """Mixed PCG with the two-level inner preconditioner on non-cubic boxes
     (and a single tile, where the coarse part is off; 208 x 160 x 32 has 130
     tiles per plane, so each plane's inverse is split 2 x 2 at 128; rows of
-    36 and 37 cells run the tile-local apply on zero-padded rows, 37 without
-    the coarse part, whose fused product needs rows of a multiple of 4): the FP64
+    36 and 37 cells run the tile-local apply on zero-padded rows, and with 37
+    the coarse correction is added to w before p is formed instead of inside
+    the fused product, which needs rows of a multiple of 4): the FP64
     residual test holds for the true residual, and at a tight tolerance the
     solution matches the FP64 oracle to 1e-8."""
     from foamhip.cases import BoxCase
