@@ -1055,10 +1055,16 @@ int32_t launch_tile_local_apply_f32(const TileView& G, const float* x, const flo
     // on one wavefront per SIMD) or the quarter-row pipeline from HBM
     // (FOAMHIP_TILE_APPLY=quarter; its per-lane quarter rows touch four
     // times the cache lines)
-    // (read at every launch, so a process can switch between them)
+    // (read at every launch, so a process can switch between them).  By
+    // default the staged tile up to 512 tiles (all tiles in one round at two
+    // per CU: its shorter per-tile latency wins -- a serving sample of 16^3
+    // to 128^3 boxes: 196 against 180 solves/s, p50 15 against 31 ms), the
+    // row walk above (256^3: 149 against 167 us)
     const char* env = std::getenv("FOAMHIP_TILE_APPLY");
     const int mode = env && std::strcmp(env, "lds") == 0 ? 0
-                   : env && std::strcmp(env, "quarter") == 0 ? 1 : 2;
+                   : env && std::strcmp(env, "quarter") == 0 ? 1
+                   : env && std::strcmp(env, "rows") == 0 ? 2
+                   : nTiles <= 512 ? 0 : 2;
     // the staged tile (FP16 coefficients): 59 KB of LDS per workgroup, taken
     // when the device allows that much
     const size_t ldsBytes = (size_t)TL_LDS_CELLS*(4 + 4) + (size_t)TL_HCELLS*3*sizeof(__half);

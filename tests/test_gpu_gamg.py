@@ -520,21 +520,23 @@ def test_tile_dic_half_scaled(scale):
     assert abs(perf.nIterations - perf1.nIterations) <= 2
 
 
+@pytest.mark.parametrize("shape", [(48, 40, 37), (37, 40, 21)])
 @pytest.mark.parametrize("apply", ["lds", "quarter"])
-def test_tile_dic_apply_variants(apply, monkeypatch):
+def test_tile_dic_apply_variants(apply, shape, monkeypatch):
     """The other kernels of the symmetric tile-local DIC with FP16
     coefficients (FOAMHIP_TILE_APPLY: the tile staged in LDS, the
     quarter-row pipeline) give the row walk's result: full and partial
-    tiles (nx a multiple of 16, ny and nz not), against the oracle's
-    block-Jacobi DIC of the FP16-stored coefficients, and in a mixed PCG
-    solve (FP64 residual test)."""
-    nx, ny, nz = 48, 40, 37
+    tiles (ny and nz not multiples of 16; nx 37 run on zero-padded rows),
+    against the oracle's block-Jacobi DIC of the FP16-stored coefficients,
+    and in a mixed PCG solve (FP64 residual test)."""
+    nx, ny, nz = shape
     n, lower, upper = box(nx, ny, nz)
     rng = np.random.default_rng(23)
     diag, up = spd_coeffs(n, lower, upper, rng)
     mesh = fh.lduAddressing(n, lower, upper)
     m = fh.lduMatrix(mesh, diag, up)
     r = rng.standard_normal(n)
+    monkeypatch.setenv("FOAMHIP_TILE_APPLY", "rows")
     w_rows = m.precondition_f32("tileDIC", r).astype(np.float64)
     monkeypatch.setenv("FOAMHIP_TILE_APPLY", apply)
     w = m.precondition_f32("tileDIC", r).astype(np.float64)
