@@ -126,7 +126,10 @@ def main():
     psi = torch.zeros(case.N, dtype=torch.float64, device="cuda")
 
     def solve(s):
+        # the solve runs on the matrix's stream: the zeroing on torch's
+        # stream completes first
         psi.zero_()
+        torch.cuda.synchronize()
         return s.solve_device(psi.data_ptr(), src.data_ptr())
 
     def assemble():
