@@ -40,5 +40,8 @@ int32_t launch_coarse_apply(int32_t nT, int32_t ld, const float* inv, const doub
 // tile-local apply where the product does not add it)
 void launch_add_tile_correction(int32_t nx, int32_t ny, int32_t nz, const float* z, float* w,
                                 const DevState* st, hipStream_t s);
+// p = (w + z) + beta p with w on rows padded to nxp cells (z may be null)
+void launch_pcg_p_padded(int32_t nx, int32_t ny, int32_t nz, int32_t nxp, const float* wp,
+                         const float* z, float* p, const DevState* st, hipStream_t s);
 
 } // namespace fh

@@ -690,6 +690,43 @@ void k_add_tile_correction(int32_t nx, int32_t ny, int64_t n, int32_t nTI, int32
     w[c] += z[(i >> 4) + nTI*((j >> 4) + nTJ*(k >> 4))];
 }
 
+// p = (w + z) + beta p (PCG.C:137-146 on the two-level apply), w read from
+// rows padded to nxp cells (the tile-local apply's padded output), z the
+// coarse correction of the cell's 16^3 tile or null: the arithmetic of the
+// unpadding copy, k_add_tile_correction and k_pcg_p in sequence
+__global__ __launch_bounds__(CO_B)
+void k_pcg_p_padded(int32_t nx, int32_t ny, int64_t n, int32_t nxp, int32_t nTI, int32_t nTJ,
+                    const float* __restrict__ wp, const float* __restrict__ z,
+                    float* __restrict__ p, const DevState* st)
+{
+    if (st->done) return;
+    const int64_t c = (int64_t)blockIdx.x*CO_B + threadIdx.x;
+    if (c >= n) return;
+    const int32_t i = (int32_t)(c % nx);
+    const int64_t row = c/nx;
+    float w = wp[i + row*nxp];
+    if (z)
+    {
+        const int32_t j = (int32_t)(row % ny), k = (int32_t)(row/ny);
+        w += z[(i >> 4) + nTI*((j >> 4) + nTJ*(k >> 4))];
+    }
+    if (st->nIterations == 0) p[c] = w;
+    else
+    {
+        const float beta = (float)st->beta;
+        p[c] = w + beta*p[c];
+    }
+}
+
+void launch_pcg_p_padded(int32_t nx, int32_t ny, int32_t nz, int32_t nxp, const float* wp,
+                         const float* z, float* p, const DevState* st, hipStream_t s)
+{
+    const int64_t n = (int64_t)nx*ny*nz;
+    if (n > 0)
+        hipLaunchKernelGGL(k_pcg_p_padded, dim3((unsigned)((n + CO_B - 1)/CO_B)), dim3(CO_B), 0,
+                           s, nx, ny, n, nxp, (nx + 15)/16, (ny + 15)/16, wp, z, p, st);
+}
+
 void launch_add_tile_correction(int32_t nx, int32_t ny, int32_t nz, const float* z, float* w,
                                 const DevState* st, hipStream_t s)
 {

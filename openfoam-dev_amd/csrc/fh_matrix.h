@@ -52,6 +52,9 @@ struct fh_matrix
     fh::DBuf<float> d_padDir32[6], d_padRD, d_padX, d_padW;
     fh::DBuf<__half> d_padDir16[3];
     bool padValid = false;
+    // set by a caller that reads w padded: the apply then leaves w in
+    // d_padW (padLeft) instead of copying it back
+    bool padKeep = false, padLeft = false;
     // FP16 storage scale 2^e of those coefficients: state 0 not decided, 1
     // FP16 at halfScale, -1 FP32 (the magnitudes span more than the FP16
     // normal range holds)

@@ -708,7 +708,8 @@ int32_t apply_precond(fh_matrix* M, int precon, const T* r, T* w, double* partia
                                                                fuse ? F : FusedFinalize(),
                                                                tileSum, M->halfScale);
                 if (nb < 0) fail(FH_ERR_INVALID, "internal: padded tile-local shape");
-                launch_unpad_rows(nx, nxp, rows, M->d_padW.p, w, M->stream);
+                M->padLeft = M->padKeep;
+                if (!M->padKeep) launch_unpad_rows(nx, nxp, rows, M->d_padW.p, w, M->stream);
                 if (fused) *fused = fuse;
                 return nb;
             }
@@ -1327,8 +1328,12 @@ static void mixed_pcg(fh_matrix* M, const fh_controls* c, double* psi, const dou
             // tile sums of r; the coarse solve adds P Ac^-1 P^T r (through
             // p in the product) and finalizes the whole w.r (the live timing
             // covers the tile-local apply only)
+            // the non-fused product path reads w padded, where the apply pads
+            M->padKeep = !fuseP;
+            M->padLeft = false;
             const int32_t nbw = apply_precond<float>(M, precon, r32, w32, part, st, false,
                                                      FusedFinalize(), nullptr, M->d_crc.p);
+            M->padKeep = false;
             kt_mark(M, FH_KERNEL_PRECONDITION);
             const FusedFinalize fc = fused_fin(M, FIN_PCG_WARA);
             double* partC = part + 2*(size_t)nbw;
@@ -1360,10 +1365,18 @@ static void mixed_pcg(fh_matrix* M, const fh_controls* c, double* psi, const dou
         else if (box)
         {
             // p = (w + z) + beta p, as the fused product forms it
-            if (coarse)
-                launch_add_tile_correction(mesh->boxNx, mesh->boxNy, mesh->boxNz, M->d_cz.p,
-                                           w32, st, M->stream);
-            launch_pcg_p<float>(N, w32, p32, st, M->stream);
+            if (M->padLeft)
+                launch_pcg_p_padded(mesh->boxNx, mesh->boxNy, mesh->boxNz,
+                                    (mesh->boxNx + 15) & ~15, M->d_padW.p,
+                                    coarse ? M->d_cz.p : nullptr, p32, st, M->stream);
+            else
+            {
+                if (coarse)
+                    launch_add_tile_correction(mesh->boxNx, mesh->boxNy, mesh->boxNz, M->d_cz.p,
+                                               w32, st, M->stream);
+                launch_pcg_p<float>(N, w32, p32, st, M->stream);
+            }
+            M->padLeft = false;
             const int32_t nb = launch_box_spmv_dot_f32(mesh->boxNx, mesh->boxNy, mesh->boxNz,
                                                        diag, dir, M->symmetric, p32, w32, p32,
                                                        part, st, M->stream);
