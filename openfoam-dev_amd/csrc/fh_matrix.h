@@ -55,6 +55,8 @@ struct fh_matrix
     // set by a caller that reads w padded: the apply then leaves w in
     // d_padW (padLeft) instead of copying it back
     bool padKeep = false, padLeft = false;
+    // set by a caller whose kernels wrote x padded already (d_padX)
+    bool padXFresh = false;
     // FP16 storage scale 2^e of those coefficients: state 0 not decided, 1
     // FP16 at halfScale, -1 FP32 (the magnitudes span more than the FP16
     // normal range holds)

@@ -700,7 +700,7 @@ int32_t apply_precond(fh_matrix* M, int precon, const T* r, T* w, double* partia
                 const __half* h3[3];
                 for (int q = 0; q < 3; q++) h3[q] = M->d_padDir16[q].p;
                 const bool fuse = partials && F.counter;
-                launch_pad_rows(nx, nxp, rows, r, M->d_padX.p, M->stream);
+                if (!M->padXFresh) launch_pad_rows(nx, nxp, rows, r, M->d_padX.p, M->stream);
                 const int32_t nb = launch_tile_local_apply_f32(Gp, M->d_padX.p, M->d_padRD.p,
                                                                dir, M->symmetric, M->d_padW.p,
                                                                st, M->stream, partials,
@@ -1319,8 +1319,15 @@ static void mixed_pcg(fh_matrix* M, const fh_controls* c, double* psi, const dou
     // coarse correction, carried into p by the fused product, or added to w
     // before p is formed)
     const bool coarse = precon == FH_PRECON_TILE_DIC && box && ensure_coarse(M);
+    // rows not a multiple of 16: the tile-local apply reads r on padded
+    // rows; the update and the replacement write that copy as well, so from
+    // the second iteration on the apply need not pad r (the first pads it)
+    const bool padR = box && (precon == FH_PRECON_TILE_DIC || precon == FH_PRECON_TILE_DILU)
+                   && (mesh->boxNx & 15);
+    const int32_t nxp = (mesh->boxNx + 15) & ~15;
     run_batches(M, c->maxIter, [&](int32_t seq)
     {
+        M->padXFresh = padR && seq > 0;
         kt_mark(M, FH_KERNEL_PRECONDITION);
         if (coarse)
         {
@@ -1389,17 +1396,19 @@ static void mixed_pcg(fh_matrix* M, const fh_controls* c, double* psi, const dou
                                    M->stream);
             launch_finalize(N, part, FIN_PCG_WAPA, st, 0, M->stream);
         }
+        M->padXFresh = false;
+        float* rPad = padR ? M->d_padX.p : nullptr;
         const FusedFinalize fu = fused_fin(M, FIN_MIXED_RES);
         kt_mark(M, FH_KERNEL_AXPY_NORM);
         const int32_t nbu = launch_mixed_update(N, psi, xc, r32, p32, ap, part, st, M->stream,
-                                                fu);
+                                                fu, mesh->boxNx, nxp, rPad);
         kt_mark(M, FH_KERNEL_AXPY_NORM);
         if (!fu.counter) launch_finalize_blocks(nbu, N, part, FIN_MIXED_RES, st, 0, M->stream);
         const FusedFinalize ft = fused_fin(M, FIN_MIXED_TRUE);
         const int32_t nbr =
             box ? launch_mixed_replace_box(mesh->boxNx, mesh->boxNy, mesh->boxNz, M->d_diag.p,
                                            dir64, M->symmetric, psi, xc, src, r32, part, st,
-                                           M->stream, ft)
+                                           M->stream, ft, nxp, rPad)
                 : launch_mixed_replace(M->rows(), M->d_diag.p, M->d_val.p, psi, xc, src, r32,
                                        part, st, M->stream, ft);
         if (!ft.counter) launch_finalize_blocks(nbr, N, part, FIN_MIXED_TRUE, st, 0, M->stream);

@@ -104,9 +104,11 @@ void launch_pcg_update_nostate(int32_t n, T* psi, T* r, const T* p, const T* w,
                                double* partials, hipStream_t s);
 // mixed PCG: x64 += alpha*p32, r32 -= alpha*w32, sum |r32|; returns the
 // partial block count
+// rPad (or null): r also written on rows of nx cells padded to nxp
 int32_t launch_mixed_update(int32_t n, double* x, float* xc, float* r, const float* p,
                             const float* w, double* partials, const DevState* st,
-                            hipStream_t s, const FusedFinalize& F = FusedFinalize());
+                            hipStream_t s, const FusedFinalize& F = FusedFinalize(),
+                            int32_t nx = 1, int32_t nxp = 1, float* rPad = nullptr);
 // mixed PCG residual replacement (only when st->replace): r = b - A (x + xc)
 // in FP64, r32 = r, sum |r|; returns the partial count
 // the same on a box from the packed FP64 coefficients (bitwise equal)
@@ -114,7 +116,8 @@ int32_t launch_mixed_replace_box(int32_t nx, int32_t ny, int32_t nz, const doubl
                                  const double* const dir[6], bool symmetric, const double* x,
                                  const float* xc, const double* b, float* r32,
                                  double* partials, const DevState* st, hipStream_t s,
-                                 const FusedFinalize& F);
+                                 const FusedFinalize& F, int32_t nxp = 1,
+                                 float* rPad = nullptr);
 int32_t launch_mixed_replace(const RowsView& R, const double* diag, const double* val,
                              const double* x, const float* xc, const double* b, float* r32,
                              double* partials, const DevState* st, hipStream_t s,

@@ -963,10 +963,20 @@ void k_finalize(int32_t nBlocks, const double* partials, int32_t op,
 // replacements and is folded into the FP64 solution x on the iteration after
 // each replacement (x += xc + alpha*p in FP64, xc = 0): the FP64 solution is
 // read and written a few times per solve instead of every iteration.
+// rPad (or null): r also written on rows padded to nxp cells (the input
+// layout of the tile-local apply on boxes whose rows are not a multiple of
+// 16 cells)
+__device__ __forceinline__ void mix_pad_store(float* rPad, int32_t nx, int32_t nxp, int32_t i,
+                                              float v)
+{
+    if (rPad) rPad[(i % nx) + (int64_t)(i/nx)*nxp] = v;
+}
+
 __global__ void k_mixed_update(int32_t n, double* __restrict__ x, float* __restrict__ xc,
                                float* __restrict__ r, const float* __restrict__ p,
                                const float* __restrict__ w, double* partials,
-                               const DevState* st, FusedFinalize F)
+                               const DevState* st, FusedFinalize F, int32_t nx, int32_t nxp,
+                               float* __restrict__ rPad)
 {
     if (st->done) return;
     const double alpha = st->alpha;
@@ -983,6 +993,7 @@ __global__ void k_mixed_update(int32_t n, double* __restrict__ x, float* __restr
             xc[i] = 0.f;
             const float ri = r[i] - alphaF*w[i];
             r[i] = ri;
+            mix_pad_store(rPad, nx, nxp, i, ri);
             s0 += fabs((double)ri);
         }
     }
@@ -1006,6 +1017,7 @@ __global__ void k_mixed_update(int32_t n, double* __restrict__ x, float* __restr
                 xc[i + u*stride] = c[u] + alphaF*pp[u];
                 const float ri = rr[u] - alphaF*ww[u];
                 r[i + u*stride] = ri;
+                mix_pad_store(rPad, nx, nxp, i + u*stride, ri);
                 s0 += fabs((double)ri);
             }
         }
@@ -1014,6 +1026,7 @@ __global__ void k_mixed_update(int32_t n, double* __restrict__ x, float* __restr
             xc[i] += alphaF*p[i];
             const float ri = r[i] - alphaF*w[i];
             r[i] = ri;
+            mix_pad_store(rPad, nx, nxp, i, ri);
             s0 += fabs((double)ri);
         }
     }
@@ -1067,7 +1080,8 @@ __global__ void k_mixed_replace_box(int32_t nx, int32_t ny, int32_t nz,
                                     const double* __restrict__ uJ, const double* __restrict__ uK,
                                     const double* __restrict__ x, const float* __restrict__ xc,
                                     const double* __restrict__ b, float* __restrict__ r32,
-                                    double* partials, const DevState* st, FusedFinalize F)
+                                    double* partials, const DevState* st, FusedFinalize F,
+                                    int32_t nxp, float* __restrict__ rPad)
 {
     if (st->done || !st->replace) return;
     const int32_t sy = nx, sz = nx*ny, n = sz*nz;
@@ -1083,6 +1097,7 @@ __global__ void k_mixed_replace_box(int32_t nx, int32_t ny, int32_t nz,
         if (gj < ny - 1) acc -= (uJ ? uJ[c] : lJ[c + sy])*(x[c + sy] + (double)xc[c + sy]);
         if (gk < nz - 1) acc -= (uK ? uK[c] : lK[c + sz])*(x[c + sz] + (double)xc[c + sz]);
         r32[c] = (float)acc;
+        mix_pad_store(rPad, nx, nxp, c, (float)acc);
         s1 += fabs(acc);
     }
     write_partials_fin(0.0, s1, partials, F);
@@ -1092,13 +1107,13 @@ int32_t launch_mixed_replace_box(int32_t nx, int32_t ny, int32_t nz, const doubl
                                  const double* const dir[6], bool symmetric, const double* x,
                                  const float* xc, const double* b, float* r32,
                                  double* partials, const DevState* st, hipStream_t s,
-                                 const FusedFinalize& F)
+                                 const FusedFinalize& F, int32_t nxp, float* rPad)
 {
     const int32_t nb = std::min(reduction_blocks(nx*ny*nz), MIX_BLOCKS);
     hipLaunchKernelGGL(k_mixed_replace_box, dim3(nb), dim3(FH_BLOCK), 0, s, nx, ny, nz, diag,
                        dir[0], dir[1], dir[2], symmetric ? nullptr : dir[3],
                        symmetric ? nullptr : dir[4], symmetric ? nullptr : dir[5], x, xc, b,
-                       r32, partials, st, F);
+                       r32, partials, st, F, nxp, rPad);
     return nb;
 }
 
@@ -1118,11 +1133,12 @@ void launch_mixed_setup(DevState* st, double replaceRatio, hipStream_t s)
 
 int32_t launch_mixed_update(int32_t n, double* x, float* xc, float* r, const float* p,
                             const float* w, double* partials, const DevState* st,
-                            hipStream_t s, const FusedFinalize& F)
+                            hipStream_t s, const FusedFinalize& F, int32_t nx, int32_t nxp,
+                            float* rPad)
 {
     const int32_t nb = std::min(reduction_blocks(n), MIX_BLOCKS);
     hipLaunchKernelGGL(k_mixed_update, dim3(nb), dim3(FH_BLOCK), 0, s,
-                       n, x, xc, r, p, w, partials, st, F);
+                       n, x, xc, r, p, w, partials, st, F, nx, nxp, rPad);
     return nb;
 }
 
