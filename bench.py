@@ -776,6 +776,32 @@ def c4_leg(fh, torch, a):
                    "final_residual": pf.finalResidual, "s_per_cycle": t / max(pf.nIterations, 1)}
     out["speedup_mixed_vs_fp64"] = t / out["mixed"]["s_per_solve"]
     log(f"c4 {n}^3 FP64 GAMG: {pf.line()}, {t:.3f} s/solve")
+    if not a.no_cpu:
+        # the oracle's FP64 GAMG on the same matrix and source, one pinned
+        # core: solves of 1 and 3 V-cycles (each agglomerates and factors the
+        # coarsest level again), the cost per cycle from their difference,
+        # extrapolated to the device FP64 solve's cycle count
+        O = oracle_module()
+        d, u, _ = mat.coeffs()
+        b = src.cpu().numpy()
+        om = O.Matrix(O.Addr(c.N, c.lower, c.upper), d, u)
+        tk = {}
+        with one_core() as core:
+            for k in (1, 3):
+                O.reset_direction()
+                t0 = time.perf_counter()
+                om.solve(dict(ctl, tolerance=0, maxIter=k), np.zeros(c.N), b)
+                tk[k] = time.perf_counter() - t0
+        per_cycle = (tk[3] - tk[1]) / 2
+        setup = max(tk[1] - per_cycle, 0.0)
+        out["cpu"] = {"value": setup + per_cycle * pf.nIterations, "unit": "s/solve",
+                      "cores": 1, "kind": "port", "core": core,
+                      "sample": f"oracle FP64 GAMG solves of 1 and 3 V-cycles ({tk[1]:.1f} s "
+                                f"and {tk[3]:.1f} s: {per_cycle:.2f} s per cycle, {setup:.1f} s "
+                                "of agglomeration and coarsest factorization), extrapolated to "
+                                f"the device FP64 solve's {pf.nIterations} cycles"}
+        out["gpu_fp64_vs_cpu"] = out["cpu"]["value"] / t
+        log(f"c4 {n}^3 oracle GAMG: {out['cpu']['sample']}")
     return out
 
 
