@@ -564,6 +564,10 @@ AGG_CASES = [
     ("rand", lambda r: random_graph(3000, 6, r), "spd", 1),
     ("rand_shuffled_merge2", lambda r: random_graph(2500, 5, r, shuffle_within_owner=True), "spd", 2),
     ("asym", lambda r: box(15, 13, 12), "asym", 1),
+    # sparse, no chain: cells without faces (numbered after the pairs) and
+    # unpaired cells joining clusters through other unpaired cells
+    ("sparse_isolated", lambda r: random_graph(4000, 1.2, r, connect_chain=False), "spd", 1),
+    ("sparse_merge2", lambda r: random_graph(3000, 2, r, connect_chain=False), "spd", 2),
 ]
 
 
