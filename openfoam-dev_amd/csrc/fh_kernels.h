@@ -201,6 +201,12 @@ template<class T>
 void launch_sweep_colours_wg(const RowsView& R, const SweepData<T>& d, const int32_t* rows,
                              const int32_t* cstart, int32_t nCol, int32_t nSweeps, bool sym,
                              const DevState* st, hipStream_t s);
+// colours [k0, k1) (each of at most a few thousand rows) in one workgroup,
+// ascending or descending
+template<class T>
+void launch_sweep_colour_run(const RowsView& R, const SweepData<T>& d, const int32_t* rows,
+                             const int32_t* cstart, int32_t k0, int32_t k1, bool desc,
+                             const DevState* st, hipStream_t s);
 // One-workgroup pipelined level sweep of a one-part mesh (Gauss-Seidel and
 // the DIC/DILU passes; fh_level_sweep.hip); false: kind or mesh not covered
 template<class T>

@@ -466,6 +466,43 @@ void launch_sweep_colours_wg(const RowsView& R, const SweepData<T>& d, const int
                        nCol, nSweeps, (int32_t)sym, st);
 }
 
+// A run of consecutive small colours [k0, k1) in one workgroup, ascending
+// or (desc) descending, a barrier between colours: one launch where each
+// colour would have been its own
+template<class T>
+__global__ __launch_bounds__(1024)
+void k_sweep_colour_run(RowsView R, SweepData<T> d, const int32_t* __restrict__ rows,
+                        const int32_t* __restrict__ cstart, int32_t k0, int32_t k1,
+                        int32_t desc, const DevState* st)
+{
+    if (st && st->done) return;
+    for (int32_t q = 0; q < k1 - k0; q++)
+    {
+        const int32_t k = desc ? k1 - 1 - q : k0 + q;
+        const int32_t r0 = cstart[k], r1 = cstart[k + 1];
+        for (int32_t i = r0 + (int32_t)threadIdx.x; i < r1; i += (int32_t)blockDim.x)
+            sweep_row<T, SWEEP_GS_FWD>(R, d, rows[i]);
+        __syncthreads();
+    }
+}
+
+template<class T>
+void launch_sweep_colour_run(const RowsView& R, const SweepData<T>& d, const int32_t* rows,
+                             const int32_t* cstart, int32_t k0, int32_t k1, bool desc,
+                             const DevState* st, hipStream_t s)
+{
+    if (k1 <= k0) return;
+    hipLaunchKernelGGL(k_sweep_colour_run<T>, dim3(1), dim3(1024), 0, s, R, d, rows, cstart, k0,
+                       k1, (int32_t)desc, st);
+}
+
+template void launch_sweep_colour_run<double>(const RowsView&, const SweepData<double>&,
+                                              const int32_t*, const int32_t*, int32_t, int32_t,
+                                              bool, const DevState*, hipStream_t);
+template void launch_sweep_colour_run<float>(const RowsView&, const SweepData<float>&,
+                                             const int32_t*, const int32_t*, int32_t, int32_t,
+                                             bool, const DevState*, hipStream_t);
+
 template void launch_sweep_colours_wg<double>(const RowsView&, const SweepData<double>&,
                                               const int32_t*, const int32_t*, int32_t, int32_t,
                                               bool, const DevState*, hipStream_t);

@@ -1032,6 +1032,10 @@ static void ensure_colours(fh_mesh* m)
 // GaussSeidelSmoother.C:66-187, symGaussSeidelSmoother.C:66-216,
 // DICSmoother.C:67-116, DILUSmoother.C:67-119, DICGaussSeidelSmoother.C:79-89,
 // DILUGaussSeidelSmoother.C:79-89 (library numbering)
+// colours of at most this many rows (one per thread of a 1024-thread
+// workgroup) are swept by one workgroup, consecutive ones in one launch
+static constexpr int32_t kColourRunRows = 1024;
+
 template<class T>
 void smooth_t(fh_matrix* M, int smoother, T* psi, const T* src, int32_t nSweeps,
               DevState* st)
@@ -1056,23 +1060,45 @@ void smooth_t(fh_matrix* M, int smoother, T* psi, const T* src, int32_t nSweeps,
             const std::vector<int32_t>& cs = M->mesh->colourStart;
             const int32_t nCol = (int32_t)cs.size() - 1;
             const int32_t* rows = M->mesh->d_colourRows.p;
-            // small meshes (at most four rows per thread in every colour):
-            // all sweeps in one workgroup launch
-            if (M->mesh->maxColourRows <= 4096)
+            // small meshes (at most one row per thread in every colour): all
+            // sweeps in one workgroup launch
+            if (M->mesh->maxColourRows <= kColourRunRows)
             {
                 launch_sweep_colours_wg<T>(M->rows(), d, rows, M->mesh->d_colourStart.p, nCol,
                                            nSweeps, sym, st, M->stream);
                 return;
             }
+            // large colours one launch each; runs of consecutive colours of
+            // at most kColourRunRows rows (the tail of the greedy colouring)
+            // in one workgroup launch
+            const int32_t* dcs = M->mesh->d_colourStart.p;
+            auto small = [&](int32_t k) { return cs[k + 1] - cs[k] <= kColourRunRows; };
+            auto pass = [&](bool desc)
+            {
+                int32_t q = 0;
+                while (q < nCol)
+                {
+                    const int32_t k = desc ? nCol - 1 - q : q;
+                    if (!small(k))
+                    {
+                        launch_sweep_colour<T>(M->rows(), d, rows + cs[k], cs[k + 1] - cs[k], st,
+                                               M->stream);
+                        q++;
+                        continue;
+                    }
+                    int32_t e = q;
+                    while (e < nCol && small(desc ? nCol - 1 - e : e)) e++;
+                    // colours [q, e) in pass order
+                    const int32_t k0 = desc ? nCol - e : q, k1 = desc ? nCol - q : e;
+                    launch_sweep_colour_run<T>(M->rows(), d, rows, dcs, k0, k1, desc, st,
+                                               M->stream);
+                    q = e;
+                }
+            };
             for (int32_t s = 0; s < nSweeps; s++)
             {
-                for (int32_t k = 0; k < nCol; k++)
-                    launch_sweep_colour<T>(M->rows(), d, rows + cs[k], cs[k + 1] - cs[k], st,
-                                           M->stream);
-                if (sym)
-                    for (int32_t k = nCol - 1; k >= 0; k--)
-                        launch_sweep_colour<T>(M->rows(), d, rows + cs[k], cs[k + 1] - cs[k],
-                                               st, M->stream);
+                pass(false);
+                if (sym) pass(true);
             }
             return;
         }
