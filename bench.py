@@ -67,7 +67,7 @@ def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {a.gpus}); the multi-GPU "
-              "domain decomposition is a design, not part of this build.", file=sys.stderr)
+              "domain decomposition is a design and is not implemented.", file=sys.stderr)
         sys.exit(2)
 
     # hardware queues per process: the serving sample's concurrent solves

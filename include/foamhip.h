@@ -34,7 +34,7 @@ extern "C" {
 #define FH_ERR_INVALID      1   /* bad argument / unknown name            */
 #define FH_ERR_HIP          2   /* HIP runtime failure                     */
 #define FH_ERR_NOMEM        3   /* device allocation failed               */
-#define FH_ERR_UNSUPPORTED  4   /* valid request this build does not cover */
+#define FH_ERR_UNSUPPORTED  4   /* valid request the library does not cover */
 #define FH_ERR_NUMERIC      5   /* NaN/Inf in a residual or a setup factor */
 #define FH_ERR_SCHEDULE     6   /* a device-side bounded wait expired      */
 

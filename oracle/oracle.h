@@ -13,7 +13,7 @@
  *
  * Scope: serial (single-rank) solves without coupled interfaces, i.e. the
  * interfaceBouCoeffs / interfaces lists are empty.  That is the case for
- * every synthetic configuration this build measures.
+ * every synthetic configuration that is measured.
  */
 #ifndef FOAMHIP_ORACLE_H
 #define FOAMHIP_ORACLE_H

@@ -1,5 +1,5 @@
 """GPU solves against the FP64 oracle on the same matrix and right-hand
-side.  Bar (north star): solution relative L2 difference <= 1e-10,
+side.  Bar: solution relative L2 difference <= 1e-10,
 iteration count within +-2, the same residual definition (normFactor) --
 the only difference between the two is the order of the global sums.
 Mixed precision: error vs the FP64 oracle solution <= MIXED_TOL with the
