@@ -11,9 +11,8 @@
  * are bitwise reproducible in plain FP64 with no FMA contraction
  * (compile with -O3 -ffp-contract=off, no -ffast-math).
  *
- * Scope: serial (single-rank) solves without coupled interfaces, i.e. the
- * interfaceBouCoeffs / interfaces lists are empty.  That is the case for
- * every synthetic configuration that is measured.
+ * Scope: serial (single-rank) solves; coupled interfaces (non-transforming
+ * cyclic patches, or_interfaces) where a matrix is given them.
  */
 #ifndef FOAMHIP_ORACLE_H
 #define FOAMHIP_ORACLE_H
