@@ -327,13 +327,16 @@ __global__ void k_coarse_assemble(int32_t P, int32_t h, const double* __restrict
 // operand at z*stride).  op(X) = X^T when transX.  out must not overlap A,
 // B or D.  A workgroup of four wavefronts owns 64 x 64 outputs, a
 // wavefront 32 x 32 (four 16 x 16 MFMA blocks); the inner dimension is
-// staged through LDS in chunks of 16 (c applied to the B chunk as it is
+// staged through LDS in chunks of 64 (c applied to the B chunk as it is
 // staged), the next chunk's loads issued before the current chunk's
-// products.  MFMA lane layout: lane l holds its A operand's (l%16, l/16)
+// products.  The products here are small (128 or 256 square), so a launch
+// is bound by the chain of chunk loads: chunks of 64 make it two or four
+// round trips (chunks of 16 took 33.6 us per launch).  MFMA lane layout: lane l holds its A operand's (l%16, l/16)
 // entry and its B operand's (l/16, l%16) entry; element v of the result is
 // entry (l/16 + 4v, l%16), so a store writes 16 consecutive outputs of a row.
 #define GM_T 64
-#define GM_K 16
+#define GM_K 64
+#define GM_U (GM_T*GM_K/256)   // chunk values per thread and operand
 struct BlockGemm
 {
     int32_t M, N, Kd;
@@ -364,23 +367,23 @@ void k_coarse_block_gemm(BlockGemm g)
     for (int r = 0; r < 2; r++)
 #pragma unroll
         for (int c = 0; c < 2; c++) acc[r][c] = co_d4{0.0, 0.0, 0.0, 0.0};
-    double av[4], bv[4];
-    // the chunk's operands, four values per thread each: the index that is
+    double av[GM_U], bv[GM_U];
+    // the chunk's operands, GM_U values per thread each: the index that is
     // contiguous in memory runs fastest over the threads
     auto fetch = [&](int k0) {
 #pragma unroll
-        for (int u = 0; u < 4; u++)
+        for (int u = 0; u < GM_U; u++)
         {
             const int e = t + 256*u;
             int ia, ka;
-            if (g.transA) { ia = e & 63; ka = e >> 6; }     // A(k, i): i contiguous
-            else { ka = e & 15; ia = e >> 4; }              // A(i, k): k contiguous
+            if (g.transA) { ia = e % GM_T; ka = e / GM_T; }     // A(k, i): i contiguous
+            else { ka = e % GM_K; ia = e / GM_K; }              // A(i, k): k contiguous
             const int gi = i0 + ia, gk = k0 + ka;
             av[u] = (gi < g.M && gk < g.Kd)
                   ? (g.transA ? Ab[(int64_t)gk*g.ldA + gi] : Ab[(int64_t)gi*g.ldA + gk]) : 0.0;
             int jb, kb;
-            if (g.transB) { kb = e & 15; jb = e >> 4; }     // B(j, k): k contiguous
-            else { jb = e & 63; kb = e >> 6; }              // B(k, j): j contiguous
+            if (g.transB) { kb = e % GM_K; jb = e / GM_K; }     // B(j, k): k contiguous
+            else { jb = e % GM_T; kb = e / GM_T; }              // B(k, j): j contiguous
             const int gj = j0 + jb, gk2 = k0 + kb;
             bv[u] = (gj < g.N && gk2 < g.Kd)
                   ? g.c[gk2]*(g.transB ? Bb[(int64_t)gj*g.ldB + gk2] : Bb[(int64_t)gk2*g.ldB + gj])
@@ -391,13 +394,13 @@ void k_coarse_block_gemm(BlockGemm g)
     for (int k0 = 0; k0 < g.Kd; k0 += GM_K)
     {
 #pragma unroll
-        for (int u = 0; u < 4; u++)
+        for (int u = 0; u < GM_U; u++)
         {
             const int e = t + 256*u;
-            if (g.transA) As[e >> 6][e & 63] = av[u];
-            else As[e & 15][e >> 4] = av[u];
-            if (g.transB) Bs[e & 15][e >> 4] = bv[u];
-            else Bs[e >> 6][e & 63] = bv[u];
+            if (g.transA) As[e / GM_T][e % GM_T] = av[u];
+            else As[e % GM_K][e / GM_K] = av[u];
+            if (g.transB) Bs[e % GM_K][e / GM_K] = bv[u];
+            else Bs[e / GM_T][e % GM_T] = bv[u];
         }
         __syncthreads();
         if (k0 + GM_K < g.Kd) fetch(k0 + GM_K);
