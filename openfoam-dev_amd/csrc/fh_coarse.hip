@@ -628,6 +628,11 @@ int64_t coarse_block_work_doubles(int32_t nT, int32_t P)
 // block 0 also adds the fine partials of the tile-local apply (sum w*r, nF
 // of them), so that the last block finalizes the whole w.r of the
 // two-level apply (op FIN_PCG_WARA)
+// NIT > 0 (ld <= 256*NIT): the wavefront's row is loaded into registers,
+// all NIT 16-byte loads per lane in flight, before rc is staged, so the
+// row's latency overlaps the staging; NIT = 0: one load per step of a loop
+// over the row (any ld).  Both sum in the same order.
+template<int NIT>
 __global__ __launch_bounds__(CO_B)
 void k_coarse_apply(int32_t nT, int32_t ld, const float* __restrict__ inv,
                     const double* __restrict__ rc, float* __restrict__ z,
@@ -636,19 +641,47 @@ void k_coarse_apply(int32_t nT, int32_t ld, const float* __restrict__ inv,
 {
     if (st && st->done) return;
     extern __shared__ float rcs[];
-    for (int32_t j = threadIdx.x; j < ld; j += CO_B) rcs[j] = j < nT ? (float)rc[j] : 0.f;
-    __syncthreads();
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int32_t row = blockIdx.x*4 + wid;
+    const float* a = inv + (int64_t)row*ld;
+    float4 v[NIT > 0 ? NIT : 1];
+    if constexpr (NIT > 0)
+    {
+        if (row < nT)
+        {
+#pragma unroll
+            for (int it = 0; it < NIT; it++)
+            {
+                const int32_t j = lane*4 + 256*it;
+                v[it] = j < ld ? *reinterpret_cast<const float4*>(a + j)
+                               : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+        }
+    }
+    for (int32_t j = threadIdx.x; j < ld; j += CO_B) rcs[j] = j < nT ? (float)rc[j] : 0.f;
+    __syncthreads();
     double part = 0.0;
     if (row < nT)
     {
-        const float* a = inv + (int64_t)row*ld;
         float acc = 0.f;
-        for (int32_t j = lane*4; j < ld; j += 256)
+        if constexpr (NIT > 0)
         {
-            const float4 v = *reinterpret_cast<const float4*>(a + j);
-            acc += v.x*rcs[j] + v.y*rcs[j + 1] + v.z*rcs[j + 2] + v.w*rcs[j + 3];
+#pragma unroll
+            for (int it = 0; it < NIT; it++)
+            {
+                const int32_t j = lane*4 + 256*it;
+                if (j < ld)
+                    acc += v[it].x*rcs[j] + v[it].y*rcs[j + 1] + v[it].z*rcs[j + 2]
+                         + v[it].w*rcs[j + 3];
+            }
+        }
+        else
+        {
+            for (int32_t j = lane*4; j < ld; j += 256)
+            {
+                v[0] = *reinterpret_cast<const float4*>(a + j);
+                acc += v[0].x*rcs[j] + v[0].y*rcs[j + 1] + v[0].z*rcs[j + 2] + v[0].w*rcs[j + 3];
+            }
         }
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
@@ -744,8 +777,19 @@ int32_t launch_coarse_apply(int32_t nT, int32_t ld, const float* inv, const doub
                             const FusedFinalize& F, const DevState* st, hipStream_t s)
 {
     const int32_t nb = (nT + 3)/4;
-    hipLaunchKernelGGL(k_coarse_apply, dim3(nb), dim3(CO_B), (size_t)ld*sizeof(float), s, nT,
-                       ld, inv, rc, z, fine, nF, partials, F, st);
+    const size_t sh = (size_t)ld*sizeof(float);
+    if (ld <= 1024)
+        hipLaunchKernelGGL(k_coarse_apply<4>, dim3(nb), dim3(CO_B), sh, s, nT, ld, inv, rc, z,
+                           fine, nF, partials, F, st);
+    else if (ld <= 2048)
+        hipLaunchKernelGGL(k_coarse_apply<8>, dim3(nb), dim3(CO_B), sh, s, nT, ld, inv, rc, z,
+                           fine, nF, partials, F, st);
+    else if (ld <= 4096)
+        hipLaunchKernelGGL(k_coarse_apply<16>, dim3(nb), dim3(CO_B), sh, s, nT, ld, inv, rc, z,
+                           fine, nF, partials, F, st);
+    else
+        hipLaunchKernelGGL(k_coarse_apply<0>, dim3(nb), dim3(CO_B), sh, s, nT, ld, inv, rc, z,
+                           fine, nF, partials, F, st);
     return nb;
 }
 

@@ -206,8 +206,8 @@ __device__ __forceinline__ unsigned int fin_arrive(unsigned int* c)
 // group sums the group's partials (one lane each, fixed butterfly order)
 // into a group pair stored after the 2 nb partials and counts the group in;
 // the last group sums the group pairs the same way, applies F.op to F.st and
-// resets the counters (F.counter[0]: groups, F.counter[1 + g]: group g) for
-// the next launch.  Every thread of every block of the launch calls it with
+// resets the counters (F.counter[0]: groups, F.counter[S*(1 + g)]: group g,
+// S = FIN_COUNTER_STRIDE) for the next launch.  Every thread of every block of the launch calls it with
 // the same nb (a block that leaves early would leave a count short); the
 // sums depend on nb only, not on the arrival order.  Without F.counter the
 // block only writes its partials and a separate finalize follows.
@@ -233,7 +233,7 @@ __device__ inline void partials_arrive(double a, double b, double* partials, int
         // the partials reach the device coherence point before the count
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         const unsigned int inG = (unsigned int)min(64, nb - (g << 6));
-        lastInGroup = fin_arrive(&F.counter[1 + g]) == inG - 1;
+        lastInGroup = fin_arrive(&F.counter[FIN_COUNTER_STRIDE*(1 + g)]) == inG - 1;
     }
     __syncthreads();
     if (!lastInGroup || threadIdx.x >= 64) return;
@@ -254,7 +254,7 @@ __device__ inline void partials_arrive(double a, double b, double* partials, int
     {
         fin_store(&grp[g], sa);
         fin_store(&grp[nG + g], sb);
-        __hip_atomic_store(&F.counter[1 + g], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&F.counter[FIN_COUNTER_STRIDE*(1 + g)], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         last = fin_arrive(&F.counter[0]) == (unsigned int)(nG - 1);
     }

@@ -31,7 +31,10 @@ enum FinalizeOp
 // counters that the kernel leaves zeroed, and the partials buffer holds
 // fin_partials(nb) values; null counter: the kernel only writes its
 // partials and a separate finalize follows.
-inline size_t fin_counters(size_t nb) { return 1 + (nb + 63)/64; }
+// each counter on a 128-byte line of its own: counters sharing a line
+// serialize every block's arrival at that line (12 us for 1024 blocks)
+constexpr int FIN_COUNTER_STRIDE = 32;
+inline size_t fin_counters(size_t nb) { return FIN_COUNTER_STRIDE*(1 + (nb + 63)/64); }
 inline size_t fin_partials(size_t nb) { return 2*nb + 2*((nb + 63)/64); }
 struct FusedFinalize
 {
